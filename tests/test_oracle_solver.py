@@ -1,0 +1,190 @@
+"""Pins the oracle's solver setup, tridiagonal elimination and Poisson solve.
+
+- eigenvalues / tridmatrix / initgrid against their closed forms
+  (src/initsolver.f90:85-187, src/initgrid.f90:14-118);
+- gaussel (src/solver.f90:114-307) against dense numpy solves, including the
+  pinned singular (zero-wavenumber) mode and the periodic Sherman-Morrison closure;
+- the whole solver (src/solver.f90:17-112) through the residual of the 7-point
+  Laplacian it inverts (the chk_poisson formula, src/debug.f90:114-125) and
+  through manufactured eigenmodes.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+
+def test_eigenvalues_closed_forms():
+    n = 12
+    l = np.arange(1, n + 1)
+    pp = -2 * (1 - np.cos(2 * np.pi * (l - 1) / n))
+    np.testing.assert_allclose(O.eigenvalues(n, "PP", "c"), pp, atol=1e-15)
+    np.testing.assert_allclose(O.eigenvalues(n, "NN", "c"), -2 * (1 - np.cos(np.pi * (l - 1) / n)), atol=1e-15)
+    np.testing.assert_allclose(O.eigenvalues(n, "DD", "c"), -2 * (1 - np.cos(np.pi * l / n)), atol=1e-15)
+    np.testing.assert_allclose(O.eigenvalues(n, "ND", "c"), -2 * (1 - np.cos((2 * l - 1) * np.pi / (2 * n))), atol=1e-15)
+    lf = O.eigenvalues(n, "DD", "f")
+    np.testing.assert_allclose(lf[:-1], -2 * (1 - np.cos(np.pi * l[:-1] / n)), atol=1e-15)
+    assert lf[-1] == 0.0
+    # PP eigenvalues really diagonalise the periodic second difference in R2HC order
+    x = np.random.default_rng(0).uniform(-1, 1, n)
+    d2 = np.roll(x, 1) - 2 * x + np.roll(x, -1)
+    np.testing.assert_allclose(O.r2r(O.R2HC, d2), pp * O.r2r(O.R2HC, x), atol=1e-13)
+    # NN-c eigenvalues diagonalise the Neumann (ghost = interior) second difference under REDFT10
+    xe = np.concatenate([[x[0]], x, [x[-1]]])
+    d2n = xe[:-2] - 2 * xe[1:-1] + xe[2:]
+    np.testing.assert_allclose(O.r2r(O.REDFT10, d2n), O.eigenvalues(n, "NN", "c") * O.r2r(O.REDFT10, x), atol=1e-13)
+    # DD-c with antisymmetric ghosts under RODFT10
+    xd = np.concatenate([[-x[0]], x, [-x[-1]]])
+    d2d = xd[:-2] - 2 * xd[1:-1] + xd[2:]
+    np.testing.assert_allclose(O.r2r(O.RODFT10, d2d), O.eigenvalues(n, "DD", "c") * O.r2r(O.RODFT10, x), atol=1e-13)
+
+
+def test_initgrid_two_end_closed_form():
+    n, gr, lz = 32, 1.8, 1.0
+    dzc, dzf, zc, zf = O.initgrid(1, n, gr, lz, False)
+    k = np.arange(1, n + 1)
+    z0 = (k.astype(np.float32) / np.float32(n)).astype(np.float64)  # default-real division
+    zfref = 0.5 * (1 + np.tanh((z0 - 0.5) * gr) / np.tanh(gr / 2)) * lz
+    np.testing.assert_allclose(np.cumsum(dzf[1:n + 1]), zfref, rtol=0, atol=1e-15)
+    assert abs(zf[n] - lz) < 1e-14 and dzf[0] == dzf[1] and dzf[n + 1] == dzf[n]
+    np.testing.assert_allclose(dzc[:n + 1], 0.5 * (dzf[:-1] + dzf[1:]), atol=0)
+    # uniform grid
+    dzc, dzf, zc, zf = O.initgrid(1, 16, 0.0, 2.0, True)
+    assert np.all(dzf == 2.0 / 16) and np.all(dzc == 2.0 / 16)
+    np.testing.assert_allclose(zc[1:17], (np.arange(16) + 0.5) / 8, atol=1e-15)
+
+
+def _grid(n, gr, periodic):
+    dzc, dzf, zc, zf = O.initgrid(1, n, gr, 1.0, periodic)
+    return 1 / dzc, 1 / dzf
+
+
+def _dense(a, b, c, lam, periodic):
+    n = len(b)
+    A = np.diag(b + lam) + np.diag(a[1:], -1) + np.diag(c[:-1], 1)
+    if periodic:
+        A[0, -1] = a[0]
+        A[-1, 0] = c[-1]
+    return A
+
+
+def test_tridmatrix_closed_form():
+    dzci, dzfi = _grid(16, 1.5, False)
+    a, b, c = O.tridmatrix("NN", 16, dzci, dzfi, "c")
+    k = np.arange(1, 17)
+    np.testing.assert_array_equal(a, dzfi[k] * dzci[k - 1])
+    np.testing.assert_array_equal(c, dzfi[k] * dzci[k])
+    assert b[0] == -(a[0] + c[0]) + a[0] and b[-1] == -(a[-1] + c[-1]) + c[-1]
+    a, b, c = O.tridmatrix("DD", 16, dzci, dzfi, "c")
+    assert b[0] == -(a[0] + c[0]) - a[0]
+    a, b, c = O.tridmatrix("DN", 16, dzci, dzfi, "f")
+    np.testing.assert_array_equal(a, dzfi[k] * dzci[k])
+    np.testing.assert_array_equal(c, dzfi[k + 1] * dzci[k])
+    assert b[0] == -(a[0] + c[0]) and b[-1] == -(a[-1] + c[-1]) + c[-1]
+
+
+@pytest.mark.parametrize("nz,gr", [(8, 0.0), (33, 1.8), (64, 2.0)])
+def test_gaussel_nonsingular_vs_dense(nz, gr):
+    dzci, dzfi = _grid(nz, gr, False)
+    a, b, c = O.tridmatrix("NN", nz, dzci, dzfi, "c")
+    rng = np.random.default_rng(nz)
+    nx, ny = 5, 3
+    lam = -rng.uniform(0.5, 50.0, (nx, ny))
+    r = rng.uniform(-1, 1, (nx, ny, nz))
+    p = np.asfortranarray(r.copy())
+    norm = 0.37
+    O.gaussel(p, a, b, c, False, norm, lam)
+    for i in range(nx):
+        for j in range(ny):
+            A = _dense(a, b, c, lam[i, j], False)
+            x = np.linalg.solve(A, norm * r[i, j])
+            np.testing.assert_allclose(p[i, j], x, rtol=1e-12, atol=1e-14 * np.abs(x).max())
+
+
+def test_gaussel_pinned_zero_mode():
+    """lambda = 0 with Neumann-Neumann z is singular: the reference pins the last pivot (x_n = 0)."""
+    nz = 40
+    dzci, dzfi = _grid(nz, 1.8, False)
+    a, b, c = O.tridmatrix("NN", nz, dzci, dzfi, "c")
+    rng = np.random.default_rng(5)
+    r = rng.uniform(-1, 1, nz)
+    r -= (r / dzfi[1:nz + 1]).sum() / (1 / dzfi[1:nz + 1]).sum()  # consistent (volume-weighted mean zero)
+    p = np.asfortranarray(r.reshape(1, 1, nz).copy())
+    O.gaussel(p, a, b, c, False, 1.0, np.zeros((1, 1)))
+    A = _dense(a, b, c, 0.0, False)
+    x = np.zeros(nz)
+    x[:-1] = np.linalg.solve(A[:-1, :-1], r[:-1])
+    assert p[0, 0, -1] == 0.0
+    np.testing.assert_allclose(p[0, 0], x, rtol=0, atol=1e-11 * np.abs(x).max())
+    # and the full system is satisfied, the last row by consistency
+    np.testing.assert_allclose(A @ p[0, 0], r, rtol=0, atol=1e-10)
+
+
+@pytest.mark.parametrize("lam", [-3.7, 0.0])
+def test_gaussel_periodic_vs_dense(lam):
+    nz = 24
+    dzci, dzfi = _grid(nz, 0.0, True)
+    a, b, c = O.tridmatrix("PP", nz, dzci, dzfi, "c")
+    rng = np.random.default_rng(11)
+    r = rng.uniform(-1, 1, nz)
+    if lam == 0.0:
+        r -= r.mean()
+    p = np.asfortranarray(r.reshape(1, 1, nz).copy())
+    O.gaussel(p, a, b, c, True, 1.0, np.full((1, 1), lam))
+    A = _dense(a, b, c, lam, True)
+    if lam != 0.0:
+        x = np.linalg.solve(A, r)
+        np.testing.assert_allclose(p[0, 0], x, rtol=1e-12, atol=1e-13)
+    else:
+        assert p[0, 0, -1] == 0.0  # closure pin
+        np.testing.assert_allclose(A @ p[0, 0], r, atol=1e-10)
+
+
+def _laplacian(case, p):
+    n1, n2, n3 = case.n
+    dxi, dyi = case.dli[0], case.dli[1]
+    dzci, dzfi = case.dzci, case.dzfi
+    k = np.arange(1, n3 + 1)
+    c = p[1:-1, 1:-1, 1:-1]
+    return ((p[2:, 1:-1, 1:-1] - 2 * c + p[:-2, 1:-1, 1:-1]) * dxi ** 2 +
+            (p[1:-1, 2:, 1:-1] - 2 * c + p[1:-1, :-2, 1:-1]) * dyi ** 2 +
+            ((p[1:-1, 1:-1, 2:] - c) * dzci[k] - (c - p[1:-1, 1:-1, :-2]) * dzci[k - 1]) * dzfi[k])
+
+
+@pytest.mark.parametrize("kind", ["triperiodic", "channel", "cavity_nn", "duct_dd"])
+def test_poisson_residual(kind):
+    rng = np.random.default_rng(21)
+    if kind == "triperiodic":
+        case = O.OracleCase(ng=(16, 12, 10), l=(2 * np.pi,) * 3, cbcpre="PPPPPP")
+    elif kind == "channel":
+        case = O.OracleCase(ng=(16, 8, 12), l=(4 * np.pi, 2 * np.pi, 1.0), gr=1.8, cbcpre="PPPPNN")
+    elif kind == "cavity_nn":
+        case = O.OracleCase(ng=(10, 12, 14), l=(1.0, 1.5, 1.0), gr=1.0, cbcpre="NNNNNN")
+    else:
+        case = O.OracleCase(ng=(8, 10, 12), l=(1.0, 1.0, 2.0), gr=0.5, cbcpre="PPDDDD")
+    rhs = rng.uniform(-1, 1, tuple(case.n))
+    if kind != "duct_dd":
+        w = np.broadcast_to(1 / case.dzfi[1:-1], rhs.shape)
+        rhs -= (rhs * w).sum() / w.sum()
+    pp = O.field(case.n)
+    pp[1:-1, 1:-1, 1:-1] = rhs
+    case.solver(pp)
+    case.boundp(pp)
+    res = _laplacian(case, pp) - rhs
+    assert np.abs(res).max() < 1e-11 * np.abs(rhs).max() * max(case.dli) ** 0, np.abs(res).max()
+
+
+def test_poisson_manufactured_mode():
+    """A single discrete eigenmode is reproduced exactly (up to round-off)."""
+    case = O.OracleCase(ng=(16, 8, 8), l=(2 * np.pi,) * 3, cbcpre="PPPPPP")
+    n1, n2, n3 = case.n
+    i = np.arange(1, n1 + 1)[:, None, None]
+    j = np.arange(1, n2 + 1)[None, :, None]
+    k = np.arange(1, n3 + 1)[None, None, :]
+    pex = np.cos(2 * np.pi * 3 * i / n1) * np.sin(2 * np.pi * 2 * j / n2) * np.cos(2 * np.pi * 1 * k / n3)
+    lam = sum(-2 * (1 - np.cos(2 * np.pi * m / nn)) * dli ** 2
+              for m, nn, dli in ((3, n1, case.dli[0]), (2, n2, case.dli[1]), (1, n3, case.dzci[1])))
+    pp = O.field(case.n)
+    pp[1:-1, 1:-1, 1:-1] = lam * pex
+    case.solver(pp)
+    np.testing.assert_allclose(pp[1:-1, 1:-1, 1:-1], pex, atol=1e-12)
