@@ -1,0 +1,111 @@
+"""ctypes binding of libcans_hip.so (declared in include/cans_hip.h).
+
+The shared library is built in-tree (``cans_amd/libcans_hip.so``) by
+``cans_amd.build.build()``.  Importing this module never compiles anything; a
+missing library raises immediately so that no caller can silently run without
+the HIP path.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libcans_hip.so")
+
+CANS_OK, CANS_EINVAL, CANS_EHIP, CANS_ENOMEM, CANS_EUNSUPPORTED = 0, -1, -2, -3, -4
+CANS_FP64, CANS_MIXED = 0, 1
+HALO_ABSENT, HALO_LOWER, HALO_NONE, HALO_UPPER = -99, -1, 0, 1
+
+_vp = C.c_void_p
+_ip = C.POINTER(C.c_int)
+_dp = C.POINTER(C.c_double)
+_cp = C.c_char_p
+
+
+class StepParams(C.Structure):
+    """cans_step_params (include/cans_hip.h)."""
+    _fields_ = [("cbcvel", C.c_char * 18), ("bcvel", C.c_double * 18), ("cbcpre", C.c_char * 6),
+                ("bcpre", C.c_double * 6), ("nb", C.c_int * 6), ("is_bound", C.c_int * 6),
+                ("dl", C.c_double * 3), ("dli", C.c_double * 3), ("visc", C.c_double),
+                ("is_forced", C.c_int * 3), ("velf", C.c_double * 3), ("bforce", C.c_double * 3),
+                ("dzc_h", _vp), ("dzf_h", _vp), ("dzci_d", _vp), ("dzfi_d", _vp),
+                ("grid_vol_ratio_c_d", _vp), ("grid_vol_ratio_f_d", _vp), ("plan", _vp),
+                ("normfft", C.c_double), ("lambdaxy_d", _vp), ("a_d", _vp), ("b_d", _vp), ("c_d", _vp),
+                ("rhsbx_d", _vp), ("rhsby_d", _vp), ("rhsbz_d", _vp)]
+
+
+class State(C.Structure):
+    """cans_state (include/cans_hip.h)."""
+    _fields_ = [("u", _vp), ("v", _vp), ("w", _vp), ("p", _vp), ("pp", _vp),
+                ("dudtrko", _vp), ("dvdtrko", _vp), ("dwdtrko", _vp)]
+
+
+# name -> (restype, argtypes); the exported C-ABI
+SIGNATURES = {
+    "cans_ctx_create": (C.c_int, [_ip, C.c_int, C.c_int, _vp, C.POINTER(_vp)]),
+    "cans_ctx_destroy": (C.c_int, [_vp]),
+    "cans_ctx_layout": (C.c_int, [_vp, C.POINTER(C.c_long), C.POINTER(C.c_long), C.POINTER(C.c_long)]),
+    "cans_ctx_sync": (C.c_int, [_vp]),
+    "cans_last_error": (C.c_char_p, []),
+    "cans_field_alloc": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "cans_field_free": (C.c_int, [_vp, _vp]),
+    "cans_field_upload": (C.c_int, [_vp, _vp, _dp]),
+    "cans_field_download": (C.c_int, [_vp, _vp, _dp]),
+    "cans_initgrid": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp, _dp, _dp]),
+    "cans_eigenvalues": (C.c_int, [C.c_int, _cp, C.c_char, _dp]),
+    "cans_tridmatrix": (C.c_int, [_cp, C.c_int, _dp, _dp, C.c_char, _dp, _dp, _dp]),
+    "cans_fftini": (C.c_int, [_vp, _ip, _cp, _cp, C.POINTER(_vp), _dp]),
+    "cans_fftend": (C.c_int, [_vp]),
+    "cans_solver": (C.c_int, [_vp, _ip, _ip, _vp, C.c_double, _vp, _vp, _vp, _vp, _cp, _cp, _vp]),
+    "cans_solve_helmholtz": (C.c_int, [_vp, _ip, _ip, _vp, C.c_double, C.c_double, _vp, _vp, _vp, _vp,
+                                       _vp, _vp, _vp, _ip, _cp, _cp, _vp]),
+    "cans_rk": (C.c_int, [_vp, _dp, _ip, _dp, _vp, _vp, _vp, _vp, C.c_double, C.c_double, _vp, _ip, _dp,
+                          _dp, _vp, _vp, _vp, _vp, _vp, _vp, _dp]),
+    "cans_fillps": (C.c_int, [_vp, _ip, _dp, _vp, C.c_double, _vp, _vp, _vp, _vp]),
+    "cans_correc": (C.c_int, [_vp, _ip, _dp, _vp, C.c_double, _vp, _vp, _vp, _vp]),
+    "cans_updatep": (C.c_int, [_vp, _ip, _dp, _vp, _vp, C.c_double, _vp, _vp, C.c_int]),
+    "cans_bounduvw": (C.c_int, [_vp, _cp, _ip, _dp, _ip, _ip, _dp, _dp, _dp, _vp, _vp, _vp, C.c_int,
+                                C.c_int, C.c_int]),
+    "cans_boundp": (C.c_int, [_vp, _cp, _ip, _dp, _ip, _ip, _dp, _dp, _vp, C.c_int]),
+    "cans_updt_rhs_b": (C.c_int, [_vp, _cp, _cp, _ip, _ip, _vp, _vp, _vp, _vp, _dp]),
+    "cans_chkdiv": (C.c_int, [_vp, _ip, _ip, _dp, _dp, _vp, _vp, _vp, _vp, _dp, _dp]),
+    "cans_chkdt": (C.c_int, [_vp, _ip, _dp, _vp, _vp, C.c_double, C.c_double, _vp, _vp, _vp, C.c_int,
+                             C.c_int, _dp]),
+    "cans_bulk_mean": (C.c_int, [_vp, _ip, _vp, _vp, _dp]),
+    "cans_rk3_substep": (C.c_int, [_vp, C.POINTER(StepParams), C.c_int, C.c_double, C.POINTER(State)]),
+    "cans_ctx_forcing": (C.c_int, [_vp, C.POINTER(_vp)]),
+}
+
+_lib = None
+
+
+class CansError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise CansError("libcans_hip.so is not built: run cans_amd.build.build() (or __graft_entry__.build())")
+        L = C.CDLL(LIBPATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc, what=""):
+    if rc != CANS_OK:
+        msg = lib().cans_last_error().decode(errors="replace")
+        raise CansError("%s failed (%d): %s" % (what, rc, msg))
+
+
+def ivec(seq):
+    return (C.c_int * len(seq))(*[int(x) for x in seq])
+
+
+def dvec(seq):
+    return (C.c_double * len(seq))(*[float(x) for x in seq])

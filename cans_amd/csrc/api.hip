@@ -1,0 +1,411 @@
+// C-ABI entry points: argument checking, boundary-plane planning (host) and
+// launch sequencing.  Kernels live in stencil.hip and solver.hip.
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+using namespace cans;
+
+namespace {
+
+bool same_n(const cans_ctx* c, const int* n) { return n && n[0] == c->L.n1 && n[1] == c->L.n2 && n[2] == c->L.n3; }
+
+#define CHECK_CTX(name)                                                   \
+  if (!c) return fail(CANS_EINVAL, std::string(name) + ": null context"); \
+  if (c->precision != CANS_FP64)                                          \
+    return fail(CANS_EUNSUPPORTED, std::string(name) + ": this entry point takes FP64 fields");
+#define CHECK_N(name, n) \
+  if (!same_n(c, n)) return fail(CANS_EINVAL, std::string(name) + ": n differs from the context grid");
+
+// set_bc (bound.f90:182-463) for one face of one field -> plane op
+PlaneOp make_setbc(double* f, char ctype, int ib, int idir, bool centered, double rvalue, double dr, int nd) {
+  PlaneOp op;
+  op.f = f;
+  op.dir = idir;
+  op.side = ib;
+  op.nd = nd;
+  op.factor = rvalue;
+  op.sgn = 0.;
+  if (ctype == 'P') {
+    op.type = BT_PERIODIC;
+  } else if (ctype == 'D' && centered) {
+    op.type = BT_CENTERED;
+    op.factor = 2. * rvalue;
+    op.sgn = -1.;
+  } else if (ctype == 'N') {
+    op.type = centered ? BT_CENTERED : BT_STAG_N;
+    op.factor = ib == 0 ? -dr * rvalue : dr * rvalue;
+    op.sgn = 1.;
+  } else {
+    op.type = BT_STAG_D;
+  }
+  return op;
+}
+
+PlaneOp make_copy(double* f, int idir, int side, int nd) {
+  PlaneOp op;
+  op.f = f;
+  op.dir = idir;
+  op.side = side;
+  op.nd = nd;
+  op.type = BT_PERIODIC;
+  op.factor = op.sgn = 0.;
+  return op;
+}
+
+void push(PlaneOps& ops, const PlaneOp& op) { ops.op[ops.nops++] = op; }
+
+// halo exchange of one field along a decomposed direction (bound.f90:600-713):
+// on one GPU it is a periodic copy when the neighbour exists (periodic), else nothing
+void push_exchange(PlaneOps& ops, double* f, int idir, int nd, int nb0, int nb1, int halo_side) {
+  if (idir == 0) return;  // x is the pencil axis: no exchange
+  bool up_lower = true, up_upper = true;
+  if (halo_side == CANS_HALO_LOWER) up_upper = false;
+  else if (halo_side == CANS_HALO_NONE) up_lower = up_upper = false;
+  else if (halo_side == CANS_HALO_UPPER) up_lower = false;
+  if (up_upper && nb1 >= 0) push(ops, make_copy(f, idir, 1, nd));
+  if (up_lower && nb0 >= 0) push(ops, make_copy(f, idir, 0, nd));
+}
+
+int plan_bounduvw(cans_ctx* c, const char* cbc, const double* bc, const int* nb, const int* is_bound,
+                  const double* dl, const double* dzc, const double* dzf, double* u, double* v, double* w,
+                  int keep_norm_values, int halos_norm, int halos_tang) {
+  const Layout& L = c->L;
+  const int n[3] = {L.n1, L.n2, L.n3};
+#define CBC(ib, d, cc) cbc[(ib) + 2 * (d) + 6 * (cc)]
+#define BCV(ib, d, cc) bc[(ib) + 2 * (d) + 6 * (cc)]
+  const bool keep_norm = keep_norm_values > 0;
+  bool updt_norm[2] = {true, true};
+  if (halos_norm == CANS_HALO_LOWER) updt_norm[1] = false;
+  else if (halos_norm == CANS_HALO_NONE) updt_norm[0] = updt_norm[1] = false;
+  else if (halos_norm == CANS_HALO_UPPER) updt_norm[0] = false;
+  double* f[3] = {u, v, w};
+  // The reference exchanges y then z halos first and applies set_bc x, y, z next.
+  // Periodic copies commute with the other directions' plane updates (each
+  // direction's update reads only interior-or-final values along that direction),
+  // so the same final state is produced direction by direction: x, y, z.
+  for (int d = 0; d < 3; d++) {
+    PlaneOps ops;
+    ops.nops = 0;
+    for (int cc = 0; cc < 3; cc++)
+      push_exchange(ops, f[cc], d, n[d], nb[2 * d], nb[2 * d + 1], cc == d ? halos_norm : halos_tang);
+    const bool pp = CBC(0, d, d) == 'P' && CBC(1, d, d) == 'P';
+    for (int ib = 0; ib < 2; ib++) {
+      if (!is_bound[ib + 2 * d]) continue;
+      const bool apply_norm = !keep_norm || (pp && updt_norm[ib]);
+      for (int cc = 0; cc < 3; cc++) {
+        const bool normal = cc == d;
+        if (normal && !apply_norm) continue;
+        double dr;
+        if (d < 2) dr = dl[d];
+        else dr = normal ? (ib == 0 ? dzf[0] : dzf[L.n3]) : (ib == 0 ? dzc[0] : dzc[L.n3]);
+        push(ops, make_setbc(f[cc], CBC(ib, d, cc), ib, d, !normal, BCV(ib, d, cc), dr, n[d]));
+      }
+    }
+    int rc = launch_bound_dir(c, ops);
+    if (rc) return rc;
+  }
+#undef CBC
+#undef BCV
+  return CANS_OK;
+}
+
+int plan_boundp(cans_ctx* c, const char* cbc, const double* bc, const int* nb, const int* is_bound, const double* dl,
+                const double* dzc, double* p, int halo_side) {
+  const Layout& L = c->L;
+  const int n[3] = {L.n1, L.n2, L.n3};
+  bool update_halo[2] = {true, true};
+  if (halo_side == CANS_HALO_LOWER) update_halo[1] = false;
+  else if (halo_side == CANS_HALO_NONE) update_halo[0] = update_halo[1] = false;
+  else if (halo_side == CANS_HALO_UPPER) update_halo[0] = false;
+  for (int d = 0; d < 3; d++) {
+    PlaneOps ops;
+    ops.nops = 0;
+    push_exchange(ops, p, d, n[d], nb[2 * d], nb[2 * d + 1], halo_side);
+    for (int ib = 0; ib < 2; ib++) {
+      if (!(is_bound[ib + 2 * d] && update_halo[ib])) continue;
+      const double dr = d < 2 ? dl[d] : (ib == 0 ? dzc[0] : dzc[L.n3]);
+      push(ops, make_setbc(p, cbc[ib + 2 * d], ib, d, true, bc[ib + 2 * d], dr, n[d]));
+    }
+    int rc = launch_bound_dir(c, ops);
+    if (rc) return rc;
+  }
+  return CANS_OK;
+}
+
+int do_updt_rhs_b(cans_ctx* c, const char* c_or_f, const char* cbc, const int* is_bound, const double* rx,
+                  const double* ry, const double* rz, double* p, double norm) {
+  const Layout& L = c->L;
+  const int n[3] = {L.n1, L.n2, L.n3};
+  const double* r[3] = {rx, ry, rz};
+  for (int d = 0; d < 3; d++) {
+    if (!r[d]) continue;
+    const int q = (c_or_f[d] == 'f' && cbc[1 + 2 * d] == 'D') ? 1 : 0;
+    const long plane = (long)(d == 0 ? L.n2 : L.n1) * (d == 2 ? L.n2 : L.n3);
+    int rc;
+    if (is_bound[2 * d] && (rc = launch_rhs_b(c, p, r[d], d, 1, norm))) return rc;
+    if (is_bound[2 * d + 1] && (rc = launch_rhs_b(c, p, r[d] + plane, d, n[d] - q, norm))) return rc;
+  }
+  return CANS_OK;
+}
+
+int ensure_tmp_fields(cans_ctx* c, double** t) {
+  void* base = nullptr;
+  const size_t fe = (size_t)c->L.sz * (c->L.n3 + 2);
+  int rc = ensure_work(c, 0, 3 * fe * sizeof(double) + 256, &base);
+  if (rc) return rc;
+  for (int k = 0; k < 3; k++) t[k] = (double*)((char*)base + 256 - sizeof(double)) + k * fe;
+  return CANS_OK;
+}
+
+const double RKCOEFF[3][2] = {{32. / 60., 0.}, {25. / 60., -17. / 60.}, {45. / 60., -25. / 60.}};
+
+}  // namespace
+
+struct cans_rk_state {
+  bool first = true;
+};
+static cans_rk_state* rk_state(cans_ctx* c) {
+  // per-context "first call" flag of rk.f90:72-100 (zeroes RHS_old once)
+  static std::mutex mu;
+  static std::map<cans_ctx*, cans_rk_state> m;
+  std::lock_guard<std::mutex> lk(mu);
+  return &m[c];
+}
+
+static int rk_core(cans_ctx* c, const double* rkpar, const double* dli, const double* dzci, const double* dzfi,
+                   const double* gvr_c, const double* gvr_f, double dt, double visc, const double* p,
+                   const int* is_forced, const double* velf, const double* bforce, double* ru, double* rv,
+                   double* rw, const double* u, const double* v, const double* w, double* uo, double* vo,
+                   double* wo) {
+  cans_rk_state* st = rk_state(c);
+  if (st->first) {
+    st->first = false;
+    const size_t bytes = (size_t)c->L.cells() * sizeof(double);
+    CANS_HIP_TRY(hipMemsetAsync(ru, 0, bytes, c->stream));
+    CANS_HIP_TRY(hipMemsetAsync(rv, 0, bytes, c->stream));
+    CANS_HIP_TRY(hipMemsetAsync(rw, 0, bytes, c->stream));
+  }
+  MomLaunch m;
+  m.u = u; m.v = v; m.w = w; m.p = p;
+  m.uo = uo; m.vo = vo; m.wo = wo;
+  m.ru = ru; m.rv = rv; m.rw = rw;
+  m.dzci = dzci; m.dzfi = dzfi; m.gvr_c = gvr_c; m.gvr_f = gvr_f;
+  m.dxi = dli[0]; m.dyi = dli[1]; m.visc = visc; m.dt = dt;
+  m.rkpar[0] = rkpar[0]; m.rkpar[1] = rkpar[1];
+  for (int d = 0; d < 3; d++) {
+    m.bforce[d] = bforce[d];
+    m.velf[d] = velf[d];
+    m.is_forced[d] = is_forced[d];
+  }
+  return launch_mom_rk(c, m);
+}
+
+extern "C" int cans_rk(cans_ctx* c, const double* rkpar, const int* n, const double* dli, const double* dzci,
+                       const double* dzfi, const double* gvr_c, const double* gvr_f, double dt, double visc,
+                       const void* p, const int* is_forced, const double* velf, const double* bforce, void* ru,
+                       void* rv, void* rw, void* u, void* v, void* w, double* f_h) {
+  CHECK_CTX("cans_rk");
+  CHECK_N("cans_rk", n);
+  if (!rkpar || !dli || !dzci || !dzfi || !gvr_c || !gvr_f || !p || !is_forced || !velf || !bforce || !ru || !rv ||
+      !rw || !u || !v || !w)
+    return fail(CANS_EINVAL, "cans_rk: null argument");
+  double* t[3];
+  int rc = ensure_tmp_fields(c, t);
+  if (rc) return rc;
+  rc = rk_core(c, rkpar, dli, dzci, dzfi, gvr_c, gvr_f, dt, visc, (const double*)p, is_forced, velf, bforce,
+               (double*)ru, (double*)rv, (double*)rw, (const double*)u, (const double*)v, (const double*)w, t[0],
+               t[1], t[2]);
+  if (rc) return rc;
+  double* dst[3] = {(double*)u, (double*)v, (double*)w};
+  for (int k = 0; k < 3; k++)
+    if ((rc = launch_add_forcing(c, t[k], dst[k], k, is_forced[k] ? 1 : 0))) return rc;
+  if (f_h) {
+    CANS_HIP_TRY(hipMemcpyAsync(f_h, c->dscal, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CANS_HIP_TRY(hipStreamSynchronize(c->stream));
+  }
+  return CANS_OK;
+}
+
+extern "C" int cans_fillps(cans_ctx* c, const int* n, const double* dli, const double* dzfi, double dti,
+                           const void* u, const void* v, const void* w, void* p) {
+  CHECK_CTX("cans_fillps");
+  CHECK_N("cans_fillps", n);
+  if (!dli || !dzfi || !u || !v || !w || !p) return fail(CANS_EINVAL, "cans_fillps: null argument");
+  return launch_fillps(c, (const double*)u, (const double*)v, (const double*)w, (double*)p, dzfi, dti, dli);
+}
+
+extern "C" int cans_correc(cans_ctx* c, const int* n, const double* dli, const double* dzci, double dt,
+                           const void* p, void* u, void* v, void* w) {
+  CHECK_CTX("cans_correc");
+  CHECK_N("cans_correc", n);
+  if (!dli || !dzci || !p || !u || !v || !w) return fail(CANS_EINVAL, "cans_correc: null argument");
+  return launch_correc(c, (const double*)p, (double*)u, (double*)v, (double*)w, (double*)u, (double*)v, (double*)w,
+                       dzci, dt, dli);
+}
+
+extern "C" int cans_updatep(cans_ctx* c, const int* n, const double* dli, const double* dzci, const double* dzfi,
+                            double alpha, const void* pp, void* p, int is_impdiff) {
+  CHECK_CTX("cans_updatep");
+  CHECK_N("cans_updatep", n);
+  if (!dli || !dzci || !dzfi || !pp || !p) return fail(CANS_EINVAL, "cans_updatep: null argument");
+  return launch_updatep(c, (const double*)pp, (double*)p, dzci, dzfi, alpha, dli, is_impdiff);
+}
+
+extern "C" int cans_bounduvw(cans_ctx* c, const char* cbc, const int* n, const double* bc, const int* nb,
+                             const int* is_bound, const double* dl, const double* dzc, const double* dzf, void* u,
+                             void* v, void* w, int keep_norm, int halos_norm, int halos_tang) {
+  CHECK_CTX("cans_bounduvw");
+  CHECK_N("cans_bounduvw", n);
+  if (!cbc || !bc || !nb || !is_bound || !dl || !dzc || !dzf || !u || !v || !w)
+    return fail(CANS_EINVAL, "cans_bounduvw: null argument");
+  return plan_bounduvw(c, cbc, bc, nb, is_bound, dl, dzc, dzf, (double*)u, (double*)v, (double*)w, keep_norm,
+                       halos_norm, halos_tang);
+}
+
+extern "C" int cans_boundp(cans_ctx* c, const char* cbc, const int* n, const double* bc, const int* nb,
+                           const int* is_bound, const double* dl, const double* dzc, void* p, int halo_side) {
+  CHECK_CTX("cans_boundp");
+  CHECK_N("cans_boundp", n);
+  if (!cbc || !bc || !nb || !is_bound || !dl || !dzc || !p) return fail(CANS_EINVAL, "cans_boundp: null argument");
+  return plan_boundp(c, cbc, bc, nb, is_bound, dl, dzc, (double*)p, halo_side);
+}
+
+extern "C" int cans_updt_rhs_b(cans_ctx* c, const char* c_or_f, const char* cbc, const int* n, const int* is_bound,
+                               const double* rx, const double* ry, const double* rz, void* p, const double* alpha) {
+  CHECK_CTX("cans_updt_rhs_b");
+  CHECK_N("cans_updt_rhs_b", n);
+  if (!c_or_f || !cbc || !is_bound || !p) return fail(CANS_EINVAL, "cans_updt_rhs_b: null argument");
+  return do_updt_rhs_b(c, c_or_f, cbc, is_bound, rx, ry, rz, (double*)p, alpha ? *alpha : 1.);
+}
+
+extern "C" int cans_solver(cans_ctx* c, const int* n, const int* ng, const cans_plan* plan, double normfft,
+                           const double* lam, const double* a, const double* b, const double* cc, const char* bc,
+                           const char* c_or_f, void* p) {
+  CHECK_CTX("cans_solver");
+  CHECK_N("cans_solver", n);
+  CHECK_N("cans_solver", ng);
+  if (!plan || !lam || !a || !b || !cc || !bc || !c_or_f || !p) return fail(CANS_EINVAL, "cans_solver: null argument");
+  return run_solver(c, plan, normfft, lam, a, b, cc, bc, c_or_f, (double*)p);
+}
+
+extern "C" int cans_solve_helmholtz(cans_ctx* c, const int* n, const int* ng, const cans_plan* plan, double normfft,
+                                    double alpha, const double* lam, const double* a, const double* b,
+                                    const double* cc, const double* rx, const double* ry, const double* rz,
+                                    const int* is_bound, const char* cbc, const char* c_or_f, void* p) {
+  CHECK_CTX("cans_solve_helmholtz");
+  CHECK_N("cans_solve_helmholtz", n);
+  if (!plan || !lam || !a || !b || !cc || !is_bound || !cbc || !c_or_f || !p)
+    return fail(CANS_EINVAL, "cans_solve_helmholtz: null argument");
+  // solve_helmholtz.f90:58-73: rhs boundary terms scaled by alpha, bb = b + 1/alpha, norm/alpha
+  int rc = do_updt_rhs_b(c, c_or_f, cbc, is_bound, rx, ry, rz, (double*)p, alpha);
+  if (rc) return rc;
+  const double alphai = 1. / alpha;
+  void* bbv = nullptr;
+  rc = ensure_work(c, 3, 2 * (c->L.n3 + 2) * sizeof(double) + (size_t)c->L.n3 * sizeof(double), &bbv);
+  if (rc) return rc;
+  double* bb = (double*)bbv + 2 * (c->L.n3 + 2);
+  std::vector<double> hb(c->L.n3);
+  CANS_HIP_TRY(hipMemcpyAsync(hb.data(), b, hb.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  CANS_HIP_TRY(hipStreamSynchronize(c->stream));
+  for (auto& x : hb) x = x + alphai;
+  CANS_HIP_TRY(hipMemcpyAsync(bb, hb.data(), hb.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  rc = run_solver(c, plan, normfft * alphai, lam, a, bb, cc, cbc, c_or_f, (double*)p);
+  if (rc) return rc;
+  CANS_HIP_TRY(hipStreamSynchronize(c->stream));
+  return CANS_OK;
+}
+
+extern "C" int cans_chkdiv(cans_ctx* c, const int* lo, const int* hi, const double* l, const double* dli,
+                           const double* dzfi, const void* u, const void* v, const void* w, double* divtot,
+                           double* divmax) {
+  CHECK_CTX("cans_chkdiv");
+  if (!lo || !hi || !l || !dli || !dzfi || !u || !v || !w || !divtot || !divmax)
+    return fail(CANS_EINVAL, "cans_chkdiv: null argument");
+  const int n[3] = {hi[0] - lo[0] + 1, hi[1] - lo[1] + 1, hi[2] - lo[2] + 1};
+  CHECK_N("cans_chkdiv", n);
+  double tot, mx;
+  int rc = run_chkdiv(c, (const double*)u, (const double*)v, (const double*)w, dzfi, dli, &tot, &mx);
+  if (rc) return rc;
+  *divtot = tot / (l[0] * l[1] * l[2]);
+  *divmax = mx;
+  return CANS_OK;
+}
+
+extern "C" int cans_chkdt(cans_ctx* c, const int* n, const double* dl, const double* dzci, const double* dzfi,
+                          double visc, double alpha, const void* u, const void* v, const void* w, int is_impdiff,
+                          int is_impdiff_1d, double* dtmax) {
+  CHECK_CTX("cans_chkdt");
+  CHECK_N("cans_chkdt", n);
+  if (!dl || !dzci || !dzfi || !u || !v || !w || !dtmax) return fail(CANS_EINVAL, "cans_chkdt: null argument");
+  double dti;
+  int rc = run_chkdt(c, (const double*)u, (const double*)v, (const double*)w, dzci, dzfi, dl, &dti);
+  if (rc) return rc;
+  // dlmin over dl(1:2) and 1/dzfi(0:n3+1) (chkdt.f90:37-44)
+  std::vector<double> hz(c->L.n3 + 2);
+  CANS_HIP_TRY(hipMemcpy(hz.data(), dzfi, hz.size() * sizeof(double), hipMemcpyDeviceToHost));
+  double dlmin = fmin(dl[0], dl[1]);
+  if (!is_impdiff_1d)
+    for (double x : hz) dlmin = fmin(dlmin, 1. / x);
+  if (dti < 2.220446049250313e-16) dti = 1.;
+  const double sqrt3 = (double)sqrtf(3.f);    // sqrt(3.) is a default-real expression
+  const double c165 = (double)(1.65f / 12.f);  // so is 1.65/12.
+  if (is_impdiff && !is_impdiff_1d) *dtmax = sqrt3 / dti;
+  else *dtmax = fmin(c165 / fmax(visc, alpha) * (dlmin * dlmin), sqrt3 / dti);
+  return CANS_OK;
+}
+
+extern "C" int cans_bulk_mean(cans_ctx* c, const int* n, const double* gvr, const void* p, double* mean) {
+  CHECK_CTX("cans_bulk_mean");
+  CHECK_N("cans_bulk_mean", n);
+  if (!gvr || !p || !mean) return fail(CANS_EINVAL, "cans_bulk_mean: null argument");
+  return run_bulk_mean(c, (const double*)p, gvr, mean);
+}
+
+extern "C" int cans_rk3_substep(cans_ctx* c, const cans_step_params* s, int irk, double dt, cans_state* st) {
+  CHECK_CTX("cans_rk3_substep");
+  if (!s || !st || irk < 1 || irk > 3) return fail(CANS_EINVAL, "cans_rk3_substep: bad argument");
+  if (!st->u || !st->v || !st->w || !st->p || !st->pp || !st->dudtrko || !st->dvdtrko || !st->dwdtrko)
+    return fail(CANS_EINVAL, "cans_rk3_substep: null field");
+  if (!s->plan || !s->lambdaxy_d || !s->a_d || !s->b_d || !s->c_d || !s->dzci_d || !s->dzfi_d || !s->dzc_h ||
+      !s->dzf_h || !s->grid_vol_ratio_c_d || !s->grid_vol_ratio_f_d)
+    return fail(CANS_EINVAL, "cans_rk3_substep: null parameter array");
+  const double* rkpar = RKCOEFF[irk - 1];
+  const double dtrk = (rkpar[0] + rkpar[1]) * dt;  // main.f90:438-439
+  const double dtrki = 1. / dtrk;
+  double* u = (double*)st->u;
+  double* v = (double*)st->v;
+  double* w = (double*)st->w;
+  double* p = (double*)st->p;
+  double* pp = (double*)st->pp;
+  double* t[3];
+  int rc = ensure_tmp_fields(c, t);
+  if (rc) return rc;
+  // rk: updated velocity into the scratch fields t (u,v,w keep the old level for the stencil)
+  rc = rk_core(c, rkpar, s->dli, s->dzci_d, s->dzfi_d, s->grid_vol_ratio_c_d, s->grid_vol_ratio_f_d, dt, s->visc, p,
+               s->is_forced, s->velf, s->bforce, (double*)st->dudtrko, (double*)st->dvdtrko, (double*)st->dwdtrko, u,
+               v, w, t[0], t[1], t[2]);
+  if (rc) return rc;
+  for (int k = 0; k < 3; k++)
+    if (s->is_forced[k] && (rc = launch_add_forcing(c, t[k], t[k], k, 1))) return rc;
+  if ((rc = plan_bounduvw(c, s->cbcvel, s->bcvel, s->nb, s->is_bound, s->dl, s->dzc_h, s->dzf_h, t[0], t[1], t[2], 0,
+                          CANS_HALO_LOWER, CANS_HALO_ABSENT)))
+    return rc;
+  if ((rc = launch_fillps(c, t[0], t[1], t[2], pp, s->dzfi_d, dtrki, s->dli))) return rc;
+  if (s->rhsbx_d || s->rhsby_d || s->rhsbz_d)
+    if ((rc = do_updt_rhs_b(c, "ccc", s->cbcpre, s->is_bound, s->rhsbx_d, s->rhsby_d, s->rhsbz_d, pp, 1.))) return rc;
+  if ((rc = run_solver(c, s->plan, s->normfft, s->lambdaxy_d, s->a_d, s->b_d, s->c_d, s->cbcpre, "ccc", pp))) return rc;
+  if ((rc = plan_boundp(c, s->cbcpre, s->bcpre, s->nb, s->is_bound, s->dl, s->dzc_h, pp, CANS_HALO_ABSENT))) return rc;
+  // correc reads the scratch velocity and writes the caller's fields (whole halo'd box)
+  if ((rc = launch_correc(c, pp, t[0], t[1], t[2], u, v, w, s->dzci_d, dtrk, s->dli))) return rc;
+  if ((rc = plan_bounduvw(c, s->cbcvel, s->bcvel, s->nb, s->is_bound, s->dl, s->dzc_h, s->dzf_h, u, v, w, 1,
+                          CANS_HALO_UPPER, CANS_HALO_NONE)))
+    return rc;
+  if ((rc = launch_updatep(c, pp, p, s->dzci_d, s->dzfi_d, 0., s->dli, 0))) return rc;
+  return plan_boundp(c, s->cbcpre, s->bcpre, s->nb, s->is_bound, s->dl, s->dzc_h, p, CANS_HALO_UPPER);
+}

@@ -1,0 +1,72 @@
+// Shared definitions for the CDNA4 (gfx950) kernels of the pressure-correction step.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "cans_hip.h"
+
+namespace cans {
+
+// Halo'd field layout: element (i,j,k), i in [0,n1+1], j in [0,n2+1], k in [0,n3+1],
+// lives at origin[i + sy*j + sz*k].  sy >= n1+2, sz >= sy*(n2+2).
+struct Layout {
+  int n1, n2, n3;
+  long sy, sz;
+  __host__ __device__ long at(int i, int j, int k) const { return i + sy * j + sz * k; }
+  __host__ __device__ long cells() const { return (long)n1 * n2 * n3; }
+};
+
+template <typename T>
+struct cplx {
+  T re, im;
+};
+
+template <typename T>
+__device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cmulc(cplx<T> a, cplx<T> b) {  // a * conj(b)
+  return {a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im};
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cadd(cplx<T> a, cplx<T> b) { return {a.re + b.re, a.im + b.im}; }
+template <typename T>
+__device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T> b) { return {a.re - b.re, a.im - b.im}; }
+
+// error reporting (thread-local message, returned by cans_last_error)
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+#define CANS_HIP_TRY(expr)                                                               \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return ::cans::fail(CANS_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define CANS_LAUNCH_CHECK(name)                                                                  \
+  do {                                                                                           \
+    hipError_t e_ = hipGetLastError();                                                           \
+    if (e_ != hipSuccess) return ::cans::fail(CANS_EHIP, std::string(name) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+inline unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
+
+}  // namespace cans
+
+// opaque handles
+struct cans_ctx {
+  cans::Layout L;
+  int precision;          // CANS_FP64 or CANS_MIXED
+  hipStream_t stream;
+  int device;
+  // scratch owned by the context (grown on demand, freed at destroy)
+  void* work[4] = {nullptr, nullptr, nullptr, nullptr};
+  size_t work_bytes[4] = {0, 0, 0, 0};
+  double* dscal = nullptr;     // device scalars: [0..2] forcing f, [3..] reductions
+  double* partials = nullptr;  // per-block reduction partials
+  size_t partials_n = 0;
+};

@@ -1,0 +1,137 @@
+// LDS-resident batched complex FFT building block (radix 8/4/2 Stockham).
+//
+// One "team" of T threads transforms one line of N = 2^m complex values that
+// sits in LDS with element stride S.  T = max(1, N/8): every stage leaves each
+// thread exactly 8 (or N, when N < 8) values in registers.  All teams of a
+// work-group run the same N and call this together (it contains barriers).
+//
+// Formulation: the Stockham autosort iteration of Govindaraju et al., "High
+// performance discrete Fourier transforms on graphics processors" (SC'08):
+// stage with radix R and span Ns maps butterfly j to inputs j + r*N/R and
+// outputs (j/Ns)*Ns*R + j%Ns + r*Ns, after twiddling input r by W_{Ns*R}^{r*(j%Ns)}.
+#pragma once
+#include "common.hpp"
+
+namespace cans {
+
+template <typename T>
+struct Rsqrt2;
+template <>
+struct Rsqrt2<double> {
+  static constexpr double v = 0.70710678118654752440;
+};
+template <>
+struct Rsqrt2<float> {
+  static constexpr float v = 0.70710678f;
+};
+
+// multiply by s*i (s = -1 forward, +1 inverse)
+template <typename T, int s>
+__device__ __forceinline__ cplx<T> mul_si(cplx<T> x) {
+  return {-(T)s * x.im, (T)s * x.re};
+}
+
+template <typename T, int s>
+__device__ __forceinline__ void bfly2(cplx<T>* v) {
+  cplx<T> a = v[0], b = v[1];
+  v[0] = cadd(a, b);
+  v[1] = csub(a, b);
+}
+
+template <typename T, int s>
+__device__ __forceinline__ void bfly4(cplx<T>* v) {
+  cplx<T> a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+  cplx<T> b0 = cadd(v[1], v[3]), b1 = mul_si<T, s>(csub(v[1], v[3]));
+  v[0] = cadd(a0, b0);
+  v[2] = csub(a0, b0);
+  v[1] = cadd(a1, b1);
+  v[3] = csub(a1, b1);
+}
+
+template <typename T, int s>
+__device__ __forceinline__ void bfly8(cplx<T>* v) {
+  cplx<T> e[4] = {v[0], v[2], v[4], v[6]};
+  cplx<T> o[4] = {v[1], v[3], v[5], v[7]};
+  bfly4<T, s>(e);
+  bfly4<T, s>(o);
+  const T r = Rsqrt2<T>::v;
+  // W8^1 = (1 + s i)/sqrt2, W8^2 = s i, W8^3 = (-1 + s i)/sqrt2
+  cplx<T> o1 = {(o[1].re - (T)s * o[1].im) * r, (o[1].im + (T)s * o[1].re) * r};
+  cplx<T> o2 = mul_si<T, s>(o[2]);
+  cplx<T> o3 = {(-o[3].re - (T)s * o[3].im) * r, (-o[3].im + (T)s * o[3].re) * r};
+  v[0] = cadd(e[0], o[0]);
+  v[4] = csub(e[0], o[0]);
+  v[1] = cadd(e[1], o1);
+  v[5] = csub(e[1], o1);
+  v[2] = cadd(e[2], o2);
+  v[6] = csub(e[2], o2);
+  v[3] = cadd(e[3], o3);
+  v[7] = csub(e[3], o3);
+}
+
+template <typename T, int s, int R>
+__device__ __forceinline__ void bfly(cplx<T>* v) {
+  if constexpr (R == 2) bfly2<T, s>(v);
+  else if constexpr (R == 4) bfly4<T, s>(v);
+  else bfly8<T, s>(v);
+}
+
+// one Stockham stage: R radix, NBF butterflies per thread
+template <typename T, int s, int R, int NBF>
+__device__ __forceinline__ void fft_stage(cplx<T>* buf, int S, int N, int lNs, int lt, int Tn,
+                                          const cplx<T>* __restrict__ tw) {
+  constexpr int LR = R == 2 ? 1 : R == 4 ? 2 : 3;
+  const int NR = N >> LR;
+  const int Ns = 1 << lNs;
+  cplx<T> v[NBF][R];
+#pragma unroll
+  for (int bb = 0; bb < NBF; bb++) {
+    const int j = lt + bb * Tn;
+#pragma unroll
+    for (int r = 0; r < R; r++) v[bb][r] = buf[(j + r * NR) * S];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int bb = 0; bb < NBF; bb++) {
+    const int j = lt + bb * Tn;
+    const int k = j & (Ns - 1);
+    if (k) {
+      const int step = k * (N >> (lNs + LR));  // k * N/(Ns*R)
+#pragma unroll
+      for (int r = 1; r < R; r++) {
+        cplx<T> w = tw[r * step];
+        v[bb][r] = (s < 0) ? cmul(v[bb][r], w) : cmulc(v[bb][r], w);
+      }
+    }
+    bfly<T, s, R>(v[bb]);
+    const int idxD = ((j >> lNs) << (lNs + LR)) + k;
+#pragma unroll
+    for (int r = 0; r < R; r++) buf[(idxD + r * Ns) * S] = v[bb][r];
+  }
+  __syncthreads();
+}
+
+// full transform of one line (unnormalised), s = -1 forward, +1 inverse
+template <typename T, int s>
+__device__ void fft_line(cplx<T>* buf, int S, int N, int logN, int lt, int Tn, const cplx<T>* __restrict__ tw) {
+  if (logN == 0) return;
+  if (logN == 1) {
+    fft_stage<T, s, 2, 1>(buf, S, N, 0, lt, Tn, tw);
+    return;
+  }
+  if (logN == 2) {
+    fft_stage<T, s, 4, 1>(buf, S, N, 0, lt, Tn, tw);
+    return;
+  }
+  int lNs = 0;
+  int rem = logN;
+  while (rem >= 3) {
+    fft_stage<T, s, 8, 1>(buf, S, N, lNs, lt, Tn, tw);
+    lNs += 3;
+    rem -= 3;
+  }
+  if (rem == 2) fft_stage<T, s, 4, 2>(buf, S, N, lNs, lt, Tn, tw);
+  else if (rem == 1) fft_stage<T, s, 2, 4>(buf, S, N, lNs, lt, Tn, tw);
+}
+
+}  // namespace cans

@@ -1,0 +1,487 @@
+// Explicit-step kernels: momentum right-hand side + RK3 update, bulk forcing,
+// divergence, projection, pressure update, boundary planes and reductions.
+//
+// Compiled with -ffp-contract=off: every expression keeps the reference's
+// association and rounding (Fortran evaluates left to right, no fused
+// multiply-add in the reference's -O3 CPU build), so these kernels reproduce the
+// CPU arithmetic bit for bit on identical inputs.
+#include <cmath>
+#include <vector>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace cans {
+
+// ---------------------------------------------------------------------------
+// momentum RHS (mom.f90:771-1019, mom_xyz_ad) fused with the RK update
+// (rk.f90:156-186) and the RHS_old copy (rk.f90:266-276).  Reads u,v,w at
+// the old time level and writes the updated velocity to (uo,vo,wo), which must
+// not alias (u,v,w).  Per-block partial sums of the updated velocity weighted
+// by the grid-volume ratio feed the bulk forcing (utils.f90:15-42).
+// ---------------------------------------------------------------------------
+struct MomArgs {
+  Layout L;
+  const double *u, *v, *w, *p;
+  double *uo, *vo, *wo;
+  double *ru, *rv, *rw;          // dudtrko (1:n1,1:n2,1:n3), in: old RHS, out: new RHS
+  const double *dzci, *dzfi;     // (0:n3+1)
+  const double *rdzm, *rdzp;     // (0:n3+1): dzci(k)/dzfi(k), dzci(k)/dzfi(k+1)
+  const double *gvr_c, *gvr_f;   // grid volume ratios (0:n3+1)
+  double dxi, dyi, visc;
+  double factor1, factor2, factor12;
+  double bforce[3];
+  int sum_mask;                  // bit c: accumulate the bulk mean of component c
+  double* partials;              // [3][nblocks]
+};
+
+constexpr int MOM_BX = 64, MOM_BY = 4;
+
+// rdzm(k) = dzci(k)/dzfi(k), rdzp(k) = dzci(k)/dzfi(k+1) for k = 1..n3 (mom.f90:801-809)
+__global__ void k_rdz(int n3, const double* dzci, const double* dzfi, double* tab) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > n3 + 1) return;
+  const bool in = k >= 1 && k <= n3;
+  tab[k] = in ? dzci[k] / dzfi[k] : 0.;
+  tab[n3 + 2 + k] = in ? dzci[k] / dzfi[k + 1] : 0.;
+}
+
+__global__ void __launch_bounds__(MOM_BX* MOM_BY) k_mom_rk(MomArgs a) {
+  const Layout& L = a.L;
+  const int i = blockIdx.x * MOM_BX + threadIdx.x + 1;
+  const int j = blockIdx.y * MOM_BY + threadIdx.y + 1;
+  const int k = blockIdx.z + 1;
+  const bool in = i <= L.n1 && j <= L.n2;
+  double s0 = 0., s1 = 0., s2 = 0.;
+  if (in) {
+    const long c = L.at(i, j, k);
+    const long sy = L.sy, sz = L.sz;
+#define AT(f, di, dj, dk) f[c + (di) + sy * (dj) + sz * (dk)]
+    const double u_ccm = AT(a.u, 0, 0, -1), u_cmc = AT(a.u, 0, -1, 0), u_mcc = AT(a.u, -1, 0, 0),
+                 u_ccc = AT(a.u, 0, 0, 0), u_pcc = AT(a.u, 1, 0, 0), u_mpc = AT(a.u, -1, 1, 0),
+                 u_cpc = AT(a.u, 0, 1, 0), u_mcp = AT(a.u, -1, 0, 1), u_ccp = AT(a.u, 0, 0, 1);
+    const double v_ccm = AT(a.v, 0, 0, -1), v_cmc = AT(a.v, 0, -1, 0), v_pmc = AT(a.v, 1, -1, 0),
+                 v_mcc = AT(a.v, -1, 0, 0), v_ccc = AT(a.v, 0, 0, 0), v_pcc = AT(a.v, 1, 0, 0),
+                 v_cpc = AT(a.v, 0, 1, 0), v_cmp = AT(a.v, 0, -1, 1), v_ccp = AT(a.v, 0, 0, 1);
+    const double w_ccm = AT(a.w, 0, 0, -1), w_pcm = AT(a.w, 1, 0, -1), w_cpm = AT(a.w, 0, 1, -1),
+                 w_cmc = AT(a.w, 0, -1, 0), w_mcc = AT(a.w, -1, 0, 0), w_ccc = AT(a.w, 0, 0, 0),
+                 w_pcc = AT(a.w, 1, 0, 0), w_cpc = AT(a.w, 0, 1, 0), w_ccp = AT(a.w, 0, 0, 1);
+    const double dxi = a.dxi, dyi = a.dyi, visc = a.visc;
+    const double dzci_k = a.dzci[k], dzci_km = a.dzci[k - 1], dzfi_k = a.dzfi[k], dzfi_kp = a.dzfi[k + 1];
+    const double rdzm = a.rdzm[k], rdzp = a.rdzp[k];
+    // x
+    const double dudxp = (u_pcc - u_ccc) * dxi, dudxm = (u_ccc - u_mcc) * dxi;
+    const double dudyp = (u_cpc - u_ccc) * dyi, dudym = (u_ccc - u_cmc) * dyi;
+    const double dudzp = (u_ccp - u_ccc) * dzci_k, dudzm = (u_ccc - u_ccm) * dzci_km;
+    const double uuip = 0.25 * (u_ccc + u_pcc) * u_pcc, uuim = 0.25 * (u_ccc + u_mcc) * u_mcc;
+    const double vujp = 0.25 * (v_ccc + v_pcc) * u_cpc, vujm = 0.25 * (v_cmc + v_pmc) * u_cmc;
+    const double wukp = 0.25 * (w_ccc + w_pcc) * u_ccp, wukm = 0.25 * (w_ccm + w_pcm) * u_ccm;
+    double dudt = -(uuip - uuim) * dxi - (vujp - vujm) * dyi - (wukp - wukm) * dzfi_k;
+    dudt = dudt + (visc * (dudxp - dudxm) * dxi + visc * (dudyp - dudym) * dyi) +
+           visc * (dudzp - dudzm) * dzfi_k;
+    // y
+    const double dvdxp = (v_pcc - v_ccc) * dxi, dvdxm = (v_ccc - v_mcc) * dxi;
+    const double dvdyp = (v_cpc - v_ccc) * dyi, dvdym = (v_ccc - v_cmc) * dyi;
+    const double dvdzp = (v_ccp - v_ccc) * dzci_k, dvdzm = (v_ccc - v_ccm) * dzci_km;
+    const double uvip = 0.25 * (u_ccc + u_cpc) * v_pcc, uvim = 0.25 * (u_mcc + u_mpc) * v_mcc;
+    const double vvjp = 0.25 * (v_ccc + v_cpc) * v_cpc, vvjm = 0.25 * (v_ccc + v_cmc) * v_cmc;
+    const double wvkp = 0.25 * (w_ccc + w_cpc) * v_ccp, wvkm = 0.25 * (w_ccm + w_cpm) * v_ccm;
+    double dvdt = -(uvip - uvim) * dxi - (vvjp - vvjm) * dyi - (wvkp - wvkm) * dzfi_k;
+    dvdt = dvdt + (visc * (dvdxp - dvdxm) * dxi + visc * (dvdyp - dvdym) * dyi) +
+           visc * (dvdzp - dvdzm) * dzfi_k;
+    // z
+    const double dwdxp = (w_pcc - w_ccc) * dxi, dwdxm = (w_ccc - w_mcc) * dxi;
+    const double dwdyp = (w_cpc - w_ccc) * dyi, dwdym = (w_ccc - w_cmc) * dyi;
+    const double dwdzp = (w_ccp - w_ccc) * dzfi_kp, dwdzm = (w_ccc - w_ccm) * dzfi_k;
+    const double uwip = 0.25 * (rdzm * u_ccc + rdzp * u_ccp) * w_pcc;
+    const double uwim = 0.25 * (rdzm * u_mcc + rdzp * u_mcp) * w_mcc;
+    const double vwjp = 0.25 * (rdzm * v_ccc + rdzp * v_ccp) * w_cpc;
+    const double vwjm = 0.25 * (rdzm * v_cmc + rdzp * v_cmp) * w_cmc;
+    const double wwkp = 0.25 * (w_ccc + w_ccp) * w_ccp, wwkm = 0.25 * (w_ccc + w_ccm) * w_ccm;
+    double dwdt = -(uwip - uwim) * dxi - (vwjp - vwjm) * dyi - (wwkp - wwkm) * dzci_k;
+    dwdt = dwdt + (visc * (dwdxp - dwdxm) * dxi + visc * (dwdyp - dwdym) * dyi) +
+           visc * (dwdzp - dwdzm) * dzci_k;
+    // RK update (rk.f90:156-186) and RHS_old <- RHS (rk.f90:266-276)
+    const long o = (long)(i - 1) + (long)L.n1 * ((j - 1) + (long)L.n2 * (k - 1));
+    const double p_c = a.p[c];
+    const double un = u_ccc + a.factor1 * dudt + a.factor2 * a.ru[o] +
+                      a.factor12 * (a.bforce[0] - dxi * (a.p[c + 1] - p_c));
+    const double vn = v_ccc + a.factor1 * dvdt + a.factor2 * a.rv[o] +
+                      a.factor12 * (a.bforce[1] - dyi * (a.p[c + sy] - p_c));
+    const double wn = w_ccc + a.factor1 * dwdt + a.factor2 * a.rw[o] +
+                      a.factor12 * (a.bforce[2] - dzci_k * (a.p[c + sz] - p_c));
+#undef AT
+    a.uo[c] = un;
+    a.vo[c] = vn;
+    a.wo[c] = wn;
+    a.ru[o] = dudt;
+    a.rv[o] = dvdt;
+    a.rw[o] = dwdt;
+    if (a.sum_mask) {
+      s0 = un * a.gvr_f[k];
+      s1 = vn * a.gvr_f[k];
+      s2 = wn * a.gvr_c[k];
+    }
+  }
+  if (a.sum_mask) {
+    const int nb = gridDim.x * gridDim.y * gridDim.z;
+    const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    block_sum3(s0, s1, s2, a.partials + bid, a.partials + nb + bid, a.partials + 2 * nb + bid);
+  }
+}
+
+// bulk forcing f = velf - mean (rk.f90:493-518); fixed-order sum of the partials
+__global__ void k_forcing_finalize(const double* partials, int nb, int mask, double vx, double vy, double vz,
+                                   double* f) {
+  __shared__ double red[256];
+  for (int c = 0; c < 3; c++) {
+    double s = 0.;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) s += partials[c * nb + b];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+      if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      const double velf = c == 0 ? vx : c == 1 ? vy : vz;
+      f[c] = (mask >> c & 1) ? velf - red[0] : 0.;
+    }
+    __syncthreads();
+  }
+}
+
+// u(1:n) += f(c) (mom.f90:725-769); src may differ from dst (copy + forcing)
+__global__ void k_add_forcing(Layout L, const double* src, double* dst, const double* f, int comp, int apply) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y + 1, k = blockIdx.z + 1;
+  if (i > L.n1) return;
+  const long c = L.at(i, j, k);
+  dst[c] = apply ? src[c] + f[comp] : src[c];
+}
+
+// fillps.f90:13-51
+__global__ void k_fillps(Layout L, const double* u, const double* v, const double* w, double* p,
+                         const double* dzfi, double dti, double dtidxi, double dtidyi) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y + 1, k = blockIdx.z + 1;
+  if (i > L.n1) return;
+  const long c = L.at(i, j, k);
+  p[c] = ((w[c] - w[c - L.sz]) * dti * dzfi[k] + (v[c] - v[c - L.sy]) * dtidyi + (u[c] - u[c - 1]) * dtidxi);
+}
+
+// correc.f90:13-61 over the whole halo'd box; (su,sv,sw) -> (du,dv,dw), which may alias.
+__global__ void k_correc(Layout L, const double* p, const double* su, const double* sv, const double* sw,
+                         double* du, double* dv, double* dw, const double* dzci, double dt, double factori,
+                         double factorj) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  if (i > L.n1 + 1) return;
+  const long c = L.at(i, j, k);
+  const double pc = p[c];
+  du[c] = (i <= L.n1) ? su[c] - factori * (p[c + 1] - pc) : su[c];
+  dv[c] = (j <= L.n2) ? sv[c] - factorj * (p[c + L.sy] - pc) : sv[c];
+  dw[c] = (k <= L.n3) ? sw[c] - dt * dzci[k] * (p[c + L.sz] - pc) : sw[c];
+}
+
+// updatep.f90:14-88
+__global__ void k_updatep(Layout L, const double* pp, double* p, const double* dzci, const double* dzfi,
+                          double alpha, double dxi2, double dyi2, int impdiff) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y + 1, k = blockIdx.z + 1;
+  if (i > L.n1) return;
+  const long c = L.at(i, j, k);
+  if (!impdiff) {
+    p[c] = p[c] + pp[c];
+    return;
+  }
+  const double q = pp[c];
+  double lap = 0.;
+  lap = lap + (pp[c + 1] - 2. * q + pp[c - 1]) * dxi2 + (pp[c + L.sy] - 2. * q + pp[c - L.sy]) * dyi2;
+  lap = lap + ((pp[c + L.sz] - q) * dzci[k] - (q - pp[c - L.sz]) * dzci[k - 1]) * dzfi[k];
+  p[c] = p[c] + q + alpha * lap;
+}
+
+// one boundary plane per op (bound.f90:182-463 and the single-rank halo copy of 600-713)
+__global__ void k_bound(Layout L, PlaneOps ops) {
+  const PlaneOp op = ops.op[blockIdx.y];
+  const int e1 = op.dir == 0 ? L.n2 + 2 : L.n1 + 2;
+  const int e2 = op.dir == 2 ? L.n2 + 2 : L.n3 + 2;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)e1 * e2) return;
+  const int a = (int)(idx % e1), b = (int)(idx / e1);
+  long base, st;  // base: element at normal index 0; st: stride along the normal
+  if (op.dir == 0) { base = L.at(0, a, b); st = 1; }
+  else if (op.dir == 1) { base = L.at(a, 0, b); st = L.sy; }
+  else { base = L.at(a, b, 0); st = L.sz; }
+  double* f = op.f;
+  const int nd = op.nd;
+  switch (op.type) {
+    case BT_PERIODIC:
+      if (op.side == 0) f[base] = f[base + st * nd];
+      else f[base + st * (nd + 1)] = f[base + st];
+      break;
+    case BT_CENTERED:
+      if (op.side == 0) f[base] = op.factor + op.sgn * f[base + st];
+      else f[base + st * (nd + 1)] = op.factor + op.sgn * f[base + st * nd];
+      break;
+    case BT_STAG_D:
+      if (op.side == 0) f[base] = op.factor;
+      else {
+        f[base + st * (nd + 1)] = f[base + st * (nd - 1)];
+        f[base + st * nd] = op.factor;
+      }
+      break;
+    case BT_STAG_N:
+      if (op.side == 0) f[base] = 1. * op.factor + f[base + st];
+      else {
+        f[base + st * (nd + 1)] = f[base + st * nd];
+        f[base + st * nd] = 1. * op.factor + f[base + st * (nd - 1)];
+      }
+      break;
+  }
+}
+
+// updt_rhs_b (bound.f90:514-598): p(plane) += rhsb(:,:,side)*norm
+__global__ void k_rhs_b(Layout L, double* p, const double* r, int dir, int plane, double norm) {
+  const int e1 = dir == 0 ? L.n2 : L.n1;
+  const int e2 = dir == 2 ? L.n2 : L.n3;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)e1 * e2) return;
+  const int a = (int)(idx % e1) + 1, b = (int)(idx / e1) + 1;
+  long c;
+  if (dir == 0) c = L.at(plane, a, b);
+  else if (dir == 1) c = L.at(a, plane, b);
+  else c = L.at(a, b, plane);
+  p[c] = p[c] + r[idx] * norm;
+}
+
+// divergence norms (chkdiv.f90:15-54): partial sum of |div|*vol and max |div|
+__global__ void k_chkdiv(Layout L, const double* u, const double* v, const double* w, const double* dzfi,
+                         double dxi, double dyi, double* partials, int nb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y + 1, k = blockIdx.z + 1;
+  double tot = 0., mx = 0.;
+  if (i <= L.n1) {
+    const long c = L.at(i, j, k);
+    const double div = (w[c] - w[c - L.sz]) * dzfi[k] + (v[c] - v[c - L.sy]) * dyi + (u[c] - u[c - 1]) * dxi;
+    mx = fabs(div);
+    tot = fabs(div) / (dxi * dyi * dzfi[k]);
+  }
+  const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  block_sum_max(tot, mx, partials + bid, partials + nb + bid);
+}
+
+// CFL estimate (chkdt.f90:16-78): partial max of dti
+__global__ void k_chkdt(Layout L, const double* u, const double* v, const double* w, const double* dzci,
+                        const double* dzfi, double dxi, double dyi, double* partials) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y + 1, k = blockIdx.z + 1;
+  double dti = 0.;
+  if (i <= L.n1) {
+    const long c = L.at(i, j, k);
+    const long sy = L.sy, sz = L.sz;
+    const double ux = fabs(u[c]);
+    const double vx = 0.25 * fabs(v[c] + v[c - sy] + v[c + 1] + v[c + 1 - sy]);
+    const double wx = 0.25 * fabs(w[c] + w[c - sz] + w[c + 1] + w[c + 1 - sz]);
+    const double dtix = ux * dxi + vx * dyi + wx * dzfi[k];
+    const double uy = 0.25 * fabs(u[c] + u[c + sy] + u[c - 1 + sy] + u[c - 1]);
+    const double vy = fabs(v[c]);
+    const double wy = 0.25 * fabs(w[c] + w[c + sy] + w[c + sy - sz] + w[c - sz]);
+    const double dtiy = uy * dxi + vy * dyi + wy * dzfi[k];
+    const double uz = 0.25 * fabs(u[c] + u[c - 1] + u[c - 1 + sz] + u[c + sz]);
+    const double vz = 0.25 * fabs(v[c] + v[c - sy] + v[c - sy + sz] + v[c + sz]);
+    const double wz = fabs(w[c]);
+    const double dtiz = uz * dxi + vz * dyi + wz * dzci[k];
+    dti = fmax(dti, fmax(dtix, fmax(dtiy, dtiz)));
+  }
+  const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  block_max(dti, partials + bid);
+}
+
+// bulk mean of one field (utils.f90:15-42)
+__global__ void k_bulk_mean(Layout L, const double* q, const double* gvr, double* partials) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y + 1, k = blockIdx.z + 1;
+  double s = 0.;
+  if (i <= L.n1) s = q[L.at(i, j, k)] * gvr[k];
+  const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  double d1, d2;
+  block_sum3(s, 0., 0., partials + bid, &d1, &d2, true);
+}
+
+// fixed-order final reduction of nb partials: op 0 sum, 1 max
+__global__ void k_finalize(const double* partials, int nb, int op, double* out) {
+  __shared__ double red[256];
+  double s = 0.;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) s = op ? fmax(s, partials[b]) : s + partials[b];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] = op ? fmax(red[threadIdx.x], red[threadIdx.x + st])
+                                                     : red[threadIdx.x] + red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static dim3 row_grid(const Layout& L, int bx) { return dim3(cdiv(L.n1, bx), L.n2, L.n3); }
+
+int launch_mom_rk(cans_ctx* c, const MomLaunch& m) {
+  const Layout& L = c->L;
+  MomArgs a;
+  a.L = L;
+  a.u = m.u; a.v = m.v; a.w = m.w; a.p = m.p;
+  a.uo = m.uo; a.vo = m.vo; a.wo = m.wo;
+  a.ru = m.ru; a.rv = m.rv; a.rw = m.rw;
+  a.dzci = m.dzci; a.dzfi = m.dzfi;
+  a.gvr_c = m.gvr_c; a.gvr_f = m.gvr_f;
+  a.dxi = m.dxi; a.dyi = m.dyi; a.visc = m.visc;
+  a.factor1 = m.rkpar[0] * m.dt;
+  a.factor2 = m.rkpar[1] * m.dt;
+  a.factor12 = a.factor1 + a.factor2;
+  for (int d = 0; d < 3; d++) a.bforce[d] = m.bforce[d];
+  a.sum_mask = (m.is_forced[0] ? 1 : 0) | (m.is_forced[1] ? 2 : 0) | (m.is_forced[2] ? 4 : 0);
+  // rdzm/rdzp tables (mom.f90:801-809), kept in context scratch slot 3
+  void* tab = nullptr;
+  int rc = ensure_work(c, 3, 2 * (L.n3 + 2) * sizeof(double), &tab);
+  if (rc) return rc;
+  a.rdzm = (const double*)tab;
+  a.rdzp = (const double*)tab + (L.n3 + 2);
+  k_rdz<<<cdiv(L.n3 + 2, 256), 256, 0, c->stream>>>(L.n3, m.dzci, m.dzfi, (double*)tab);
+  CANS_LAUNCH_CHECK("k_rdz");
+  dim3 grid(cdiv(L.n1, MOM_BX), cdiv(L.n2, MOM_BY), L.n3);
+  const long nb = (long)grid.x * grid.y * grid.z;
+  if (a.sum_mask && (size_t)(3 * nb) > c->partials_n) {
+    CANS_HIP_TRY(hipStreamSynchronize(c->stream));
+    hipFree(c->partials);
+    c->partials_n = 3 * nb;
+    CANS_HIP_TRY(hipMalloc((void**)&c->partials, c->partials_n * sizeof(double)));
+  }
+  a.partials = c->partials;
+  k_mom_rk<<<grid, dim3(MOM_BX, MOM_BY), 0, c->stream>>>(a);
+  CANS_LAUNCH_CHECK("k_mom_rk");
+  k_forcing_finalize<<<1, 256, 0, c->stream>>>(c->partials, (int)nb, a.sum_mask, m.velf[0], m.velf[1],
+                                                m.velf[2], c->dscal);
+  CANS_LAUNCH_CHECK("k_forcing_finalize");
+  return CANS_OK;
+}
+
+int launch_add_forcing(cans_ctx* c, const double* src, double* dst, int comp, int apply) {
+  const Layout& L = c->L;
+  k_add_forcing<<<row_grid(L, 256), 256, 0, c->stream>>>(L, src, dst, c->dscal, comp, apply);
+  CANS_LAUNCH_CHECK("k_add_forcing");
+  return CANS_OK;
+}
+
+int launch_fillps(cans_ctx* c, const double* u, const double* v, const double* w, double* p, const double* dzfi,
+                  double dti, const double* dli) {
+  const Layout& L = c->L;
+  k_fillps<<<row_grid(L, 256), 256, 0, c->stream>>>(L, u, v, w, p, dzfi, dti, dti * dli[0], dti * dli[1]);
+  CANS_LAUNCH_CHECK("k_fillps");
+  return CANS_OK;
+}
+
+int launch_correc(cans_ctx* c, const double* p, const double* su, const double* sv, const double* sw, double* du,
+                  double* dv, double* dw, const double* dzci, double dt, const double* dli) {
+  const Layout& L = c->L;
+  dim3 grid(cdiv(L.n1 + 2, 256), L.n2 + 2, L.n3 + 2);
+  k_correc<<<grid, 256, 0, c->stream>>>(L, p, su, sv, sw, du, dv, dw, dzci, dt, dt * dli[0], dt * dli[1]);
+  CANS_LAUNCH_CHECK("k_correc");
+  return CANS_OK;
+}
+
+int launch_updatep(cans_ctx* c, const double* pp, double* p, const double* dzci, const double* dzfi, double alpha,
+                   const double* dli, int impdiff) {
+  const Layout& L = c->L;
+  k_updatep<<<row_grid(L, 256), 256, 0, c->stream>>>(L, pp, p, dzci, dzfi, alpha, dli[0] * dli[0],
+                                                       dli[1] * dli[1], impdiff);
+  CANS_LAUNCH_CHECK("k_updatep");
+  return CANS_OK;
+}
+
+int launch_bound_dir(cans_ctx* c, const PlaneOps& ops) {
+  if (ops.nops == 0) return CANS_OK;
+  const Layout& L = c->L;
+  const int dir = ops.op[0].dir;
+  const long e1 = dir == 0 ? L.n2 + 2 : L.n1 + 2;
+  const long e2 = dir == 2 ? L.n2 + 2 : L.n3 + 2;
+  dim3 grid(cdiv(e1 * e2, 256), ops.nops);
+  k_bound<<<grid, 256, 0, c->stream>>>(L, ops);
+  CANS_LAUNCH_CHECK("k_bound");
+  return CANS_OK;
+}
+
+int launch_rhs_b(cans_ctx* c, double* p, const double* r, int dir, int plane, double norm) {
+  const Layout& L = c->L;
+  const long e1 = dir == 0 ? L.n2 : L.n1;
+  const long e2 = dir == 2 ? L.n2 : L.n3;
+  k_rhs_b<<<cdiv(e1 * e2, 256), 256, 0, c->stream>>>(L, p, r, dir, plane, norm);
+  CANS_LAUNCH_CHECK("k_rhs_b");
+  return CANS_OK;
+}
+
+static int ensure_partials(cans_ctx* c, size_t n) {
+  if (n <= c->partials_n) return CANS_OK;
+  CANS_HIP_TRY(hipStreamSynchronize(c->stream));
+  hipFree(c->partials);
+  c->partials_n = n;
+  CANS_HIP_TRY(hipMalloc((void**)&c->partials, n * sizeof(double)));
+  return CANS_OK;
+}
+
+int run_chkdiv(cans_ctx* c, const double* u, const double* v, const double* w, const double* dzfi, const double* dli,
+               double* tot, double* mx) {
+  const Layout& L = c->L;
+  dim3 grid = row_grid(L, 256);
+  const long nb = (long)grid.x * grid.y * grid.z;
+  int rc = ensure_partials(c, 2 * nb);
+  if (rc) return rc;
+  k_chkdiv<<<grid, 256, 0, c->stream>>>(L, u, v, w, dzfi, dli[0], dli[1], c->partials, (int)nb);
+  CANS_LAUNCH_CHECK("k_chkdiv");
+  k_finalize<<<1, 256, 0, c->stream>>>(c->partials, (int)nb, 0, c->dscal + 8);
+  k_finalize<<<1, 256, 0, c->stream>>>(c->partials + nb, (int)nb, 1, c->dscal + 9);
+  CANS_LAUNCH_CHECK("k_finalize");
+  double h[2];
+  CANS_HIP_TRY(hipMemcpyAsync(h, c->dscal + 8, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  CANS_HIP_TRY(hipStreamSynchronize(c->stream));
+  *tot = h[0];
+  *mx = h[1];
+  return CANS_OK;
+}
+
+int run_chkdt(cans_ctx* c, const double* u, const double* v, const double* w, const double* dzci, const double* dzfi,
+              const double* dl, double* dti_out) {
+  const Layout& L = c->L;
+  dim3 grid = row_grid(L, 256);
+  const long nb = (long)grid.x * grid.y * grid.z;
+  int rc = ensure_partials(c, nb);
+  if (rc) return rc;
+  k_chkdt<<<grid, 256, 0, c->stream>>>(L, u, v, w, dzci, dzfi, 1. / dl[0], 1. / dl[1], c->partials);
+  CANS_LAUNCH_CHECK("k_chkdt");
+  k_finalize<<<1, 256, 0, c->stream>>>(c->partials, (int)nb, 1, c->dscal + 10);
+  CANS_LAUNCH_CHECK("k_finalize");
+  CANS_HIP_TRY(hipMemcpyAsync(dti_out, c->dscal + 10, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  CANS_HIP_TRY(hipStreamSynchronize(c->stream));
+  return CANS_OK;
+}
+
+int run_bulk_mean(cans_ctx* c, const double* q, const double* gvr, double* mean) {
+  const Layout& L = c->L;
+  dim3 grid = row_grid(L, 256);
+  const long nb = (long)grid.x * grid.y * grid.z;
+  int rc = ensure_partials(c, nb);
+  if (rc) return rc;
+  k_bulk_mean<<<grid, 256, 0, c->stream>>>(L, q, gvr, c->partials);
+  CANS_LAUNCH_CHECK("k_bulk_mean");
+  k_finalize<<<1, 256, 0, c->stream>>>(c->partials, (int)nb, 0, c->dscal + 11);
+  CANS_LAUNCH_CHECK("k_finalize");
+  CANS_HIP_TRY(hipMemcpyAsync(mean, c->dscal + 11, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  CANS_HIP_TRY(hipStreamSynchronize(c->stream));
+  return CANS_OK;
+}
+
+}  // namespace cans

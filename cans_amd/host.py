@@ -1,0 +1,364 @@
+"""Host-side mirror of the reference's set-up and time loop over the HIP C-ABI.
+
+The reference's Fortran host (src/main.f90, src/initsolver.f90, src/param.f90)
+computes the grid and solver coefficients on the CPU and drives the per-routine
+device calls.  This module is the same host in Python: ``Case`` holds the
+namelist parameters of src/param.f90, ``Simulation`` performs the
+initialisation of src/main.f90:241-395 and the RK3 substeps of
+src/main.f90:437-473 through ``libcans_hip.so``.  Device memory comes from
+PyTorch tensors laid out as the context's halo'd fields.
+"""
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as LB
+
+RKCOEFF = ((32. / 60., 0.), (25. / 60., -17. / 60.), (45. / 60., -25. / 60.))  # src/param.f90:20-22
+
+
+def _torch():
+    import torch
+    return torch
+
+
+@dataclass
+class Case:
+    """Parameters of the &dns namelist (src/param.f90:27-105) that the step uses."""
+    ng: tuple
+    l: tuple
+    gtype: int = 1
+    gr: float = 0.0
+    cbcvel: str = "PP" * 9          # cbcvel(0:1,3,3) flattened, component slowest
+    cbcpre: str = "PP" * 3          # cbcpre(0:1,3)
+    bcvel: tuple = (0.0,) * 18
+    bcpre: tuple = (0.0,) * 6
+    visci: float = 1.0
+    bforce: tuple = (0.0, 0.0, 0.0)
+    is_forced: tuple = (False, False, False)
+    velf: tuple = (0.0, 0.0, 0.0)
+
+    @property
+    def n(self):
+        return tuple(int(x) for x in self.ng)
+
+
+class Grid:
+    """dl, dli, z-grid and volume ratios (src/param.f90:196-198, src/main.f90:241-285)."""
+
+    def __init__(self, case: Case):
+        L = LB.lib()
+        ng = case.n
+        self.n = ng
+        self.l = tuple(float(x) for x in case.l)
+        # (1.*ng) is a default-real (single precision) expression in param.f90:196
+        self.dl = tuple(self.l[d] / float(np.float32(ng[d])) for d in range(3))
+        self.dli = tuple(self.dl[d] ** (-1) for d in range(3))
+        self.visc = float(case.visci) ** (-1)
+        n3 = ng[2]
+        self.dzc, self.dzf, self.zc, self.zf = (np.zeros(n3 + 2) for _ in range(4))
+        periodic_z = case.cbcpre[4:6] == "PP"
+        LB.check(L.cans_initgrid(case.gtype, n3, case.gr, self.l[2], int(periodic_z), _dp(self.dzc),
+                                 _dp(self.dzf), _dp(self.zc), _dp(self.zf)), "cans_initgrid")
+        self.dzci = self.dzc ** (-1)
+        self.dzfi = self.dzf ** (-1)
+        vol = self.l[0] * self.l[1] * self.l[2]
+        self.grid_vol_ratio_c = self.dl[0] * self.dl[1] * self.dzc / vol
+        self.grid_vol_ratio_f = self.dl[0] * self.dl[1] * self.dzf / vol
+        self.is_periodic = tuple(case.cbcpre[2 * d] == "P" and case.cbcpre[2 * d + 1] == "P" for d in range(3))
+        # one GPU holds the whole domain as an x-pencil (src/initmpi.f90:226-267):
+        # x is always a bounded direction, y/z are bounded unless periodic;
+        # periodic y/z have this rank as neighbour (nb >= 0), others MPI_PROC_NULL (-1)
+        self.is_bound = (1, 1) + tuple(0 if self.is_periodic[d] else 1 for d in (1, 1, 2, 2))
+        self.nb = (-1, -1) + tuple(0 if self.is_periodic[d] else -1 for d in (1, 1, 2, 2))
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def eigenvalues(n, bc, c_or_f):
+    lam = np.zeros(n)
+    LB.check(LB.lib().cans_eigenvalues(n, bc.encode(), c_or_f.encode(), _dp(lam)), "cans_eigenvalues")
+    return lam
+
+
+def tridmatrix(bc, n, dzci, dzfi, c_or_f):
+    a, b, c = np.zeros(n), np.zeros(n), np.zeros(n)
+    LB.check(LB.lib().cans_tridmatrix(bc.encode(), n, _dp(np.ascontiguousarray(dzci)),
+                                      _dp(np.ascontiguousarray(dzfi)), c_or_f.encode(), _dp(a), _dp(b), _dp(c)),
+             "cans_tridmatrix")
+    return a, b, c
+
+
+class Context:
+    """cans_ctx: one grid, one HIP stream, one precision mode."""
+
+    def __init__(self, n, ld=None, precision=LB.CANS_FP64, stream=None, device="cuda"):
+        torch = _torch()
+        self.torch = torch
+        self.device = torch.device(device)
+        torch.cuda.init()
+        self.n = tuple(int(x) for x in n)
+        if ld is None:  # interior rows start 256-byte aligned
+            ld = ((self.n[0] + 2 + 31) // 32) * 32
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device).cuda_stream
+        self.stream = stream
+        h = C.c_void_p()
+        LB.check(LB.lib().cans_ctx_create(LB.ivec(self.n), int(ld), int(precision), C.c_void_p(stream),
+                                          C.byref(h)), "cans_ctx_create")
+        self.h = h
+        sy, sz, el = C.c_long(), C.c_long(), C.c_long()
+        LB.check(LB.lib().cans_ctx_layout(h, C.byref(sy), C.byref(sz), C.byref(el)), "cans_ctx_layout")
+        self.sy, self.sz, self.elems = sy.value, sz.value, el.value
+
+    def close(self):
+        if self.h:
+            LB.lib().cans_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        LB.check(LB.lib().cans_ctx_sync(self.h), "cans_ctx_sync")
+
+    # halo'd field backed by a torch tensor; .ptr is the address of element (0,0,0)
+    def field(self, fill=0.0):
+        return DeviceField(self, fill)
+
+    def darray(self, a, dtype=None):
+        torch = self.torch
+        return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype or torch.float64, device=self.device)
+
+
+class DeviceField:
+    def __init__(self, ctx: Context, fill=0.0):
+        torch = ctx.torch
+        self.ctx = ctx
+        es = 8
+        off = 256 // es - 1
+        self.buf = torch.full((ctx.elems + 2 * 256 // es,), float(fill), dtype=torch.float64, device=ctx.device)
+        base = self.buf.data_ptr()
+        # element 1 of every row on a 256-byte boundary
+        shift = ((-(base + es)) % 256) // es
+        self.off = shift
+        self.ptr = base + es * shift
+        n1, n2, n3 = ctx.n
+        self.view = self.buf[shift:shift + ctx.elems].view(n3 + 2, n2 + 2, ctx.sy)[:, :, :n1 + 2]
+        del off
+
+    def numpy(self):
+        """Fortran-ordered (n1+2, n2+2, n3+2) float64 copy on the host."""
+        return np.asfortranarray(self.view.cpu().numpy().transpose(2, 1, 0))
+
+    def set(self, a):
+        a = np.asarray(a, dtype=np.float64)
+        n1, n2, n3 = self.ctx.n
+        assert a.shape == (n1 + 2, n2 + 2, n3 + 2)
+        self.view.copy_(self.ctx.torch.from_numpy(np.ascontiguousarray(a.transpose(2, 1, 0))))
+
+    def interior(self):
+        return self.numpy()[1:-1, 1:-1, 1:-1]
+
+
+class Plan:
+    """Transforms + solver arrays of one variable (src/initsolver.f90:15-83)."""
+
+    def __init__(self, ctx: Context, grid: Grid, cbc: str, bc, c_or_f="ccc"):
+        self.ctx = ctx
+        ng = grid.n
+        lx = eigenvalues(ng[0], cbc[0:2], c_or_f[0]) * grid.dli[0] ** 2
+        ly = eigenvalues(ng[1], cbc[2:4], c_or_f[1]) * grid.dli[1] ** 2
+        self.lam_h = np.asfortranarray(lx[:, None] + ly[None, :])  # lambdaxy(i,j), device ordering
+        self.a_h, self.b_h, self.c_h = tridmatrix(cbc[4:6], ng[2], grid.dzci, grid.dzfi, c_or_f[2])
+        h = C.c_void_p()
+        nf = C.c_double()
+        LB.check(LB.lib().cans_fftini(ctx.h, LB.ivec(ng), cbc[0:4].encode(), c_or_f[0:2].encode(), C.byref(h),
+                                      C.byref(nf)), "cans_fftini")
+        self.h = h
+        self.normfft = nf.value
+        self.cbc = cbc
+        self.c_or_f = c_or_f
+        self.lam = ctx.darray(self.lam_h.ravel(order="F"))
+        self.a = ctx.darray(self.a_h)
+        self.b = ctx.darray(self.b_h)
+        self.c = ctx.darray(self.c_h)
+
+    def close(self):
+        if self.h:
+            LB.lib().cans_fftend(self.h)
+            self.h = None
+
+    def solve(self, f: DeviceField):
+        n = LB.ivec(self.ctx.n)
+        LB.check(LB.lib().cans_solver(self.ctx.h, n, n, self.h, self.normfft, C.c_void_p(self.lam.data_ptr()),
+                                      C.c_void_p(self.a.data_ptr()), C.c_void_p(self.b.data_ptr()),
+                                      C.c_void_p(self.c.data_ptr()), self.cbc.encode(), self.c_or_f.encode(),
+                                      C.c_void_p(f.ptr)), "cans_solver")
+
+
+class Simulation:
+    """One case on one GPU: set-up of src/main.f90:241-395 and its RK3 loop."""
+
+    def __init__(self, case: Case, ctx: Context = None):
+        self.case = case
+        self.grid = Grid(case)
+        self.ctx = ctx or Context(case.n)
+        ctx = self.ctx
+        g = self.grid
+        self.plan = Plan(ctx, g, case.cbcpre, case.bcpre, "ccc")
+        self.u, self.v, self.w, self.p, self.pp = (ctx.field() for _ in range(5))
+        torch = ctx.torch
+        n1, n2, n3 = case.n
+        self.rhs_o = [torch.zeros(n1 * n2 * n3, dtype=torch.float64, device=ctx.device) for _ in range(3)]
+        self.dzci_d = ctx.darray(g.dzci)
+        self.dzfi_d = ctx.darray(g.dzfi)
+        self.gvr_c_d = ctx.darray(g.grid_vol_ratio_c)
+        self.gvr_f_d = ctx.darray(g.grid_vol_ratio_f)
+        self._dzc = np.ascontiguousarray(g.dzc)
+        self._dzf = np.ascontiguousarray(g.dzf)
+        prm = LB.StepParams()
+        prm.cbcvel = case.cbcvel.encode()
+        prm.bcvel = LB.dvec(case.bcvel)
+        prm.cbcpre = case.cbcpre.encode()
+        prm.bcpre = LB.dvec(case.bcpre)
+        prm.nb = LB.ivec(g.nb)
+        prm.is_bound = LB.ivec(g.is_bound)
+        prm.dl = LB.dvec(g.dl)
+        prm.dli = LB.dvec(g.dli)
+        prm.visc = g.visc
+        prm.is_forced = LB.ivec([int(bool(x)) for x in case.is_forced])
+        prm.velf = LB.dvec(case.velf)
+        prm.bforce = LB.dvec(case.bforce)
+        prm.dzc_h = self._dzc.ctypes.data
+        prm.dzf_h = self._dzf.ctypes.data
+        prm.dzci_d = self.dzci_d.data_ptr()
+        prm.dzfi_d = self.dzfi_d.data_ptr()
+        prm.grid_vol_ratio_c_d = self.gvr_c_d.data_ptr()
+        prm.grid_vol_ratio_f_d = self.gvr_f_d.data_ptr()
+        prm.plan = self.plan.h.value
+        prm.normfft = self.plan.normfft
+        prm.lambdaxy_d = self.plan.lam.data_ptr()
+        prm.a_d = self.plan.a.data_ptr()
+        prm.b_d = self.plan.b.data_ptr()
+        prm.c_d = self.plan.c.data_ptr()
+        prm.rhsbx_d = prm.rhsby_d = prm.rhsbz_d = None  # homogeneous pressure BCs in the solved cases
+        if any(abs(x) > 0 for x in case.bcpre):
+            raise NotImplementedError("non-homogeneous pressure boundary values")
+        self.prm = prm
+        st = LB.State()
+        st.u, st.v, st.w, st.p, st.pp = (f.ptr for f in (self.u, self.v, self.w, self.p, self.pp))
+        st.dudtrko, st.dvdtrko, st.dwdtrko = (t.data_ptr() for t in self.rhs_o)
+        self.st = st
+
+    # ---- state ----
+    def set_state(self, u=None, v=None, w=None, p=None):
+        for f, a in ((self.u, u), (self.v, v), (self.w, w), (self.p, p)):
+            if a is not None:
+                f.set(a)
+
+    def init_bounds(self, restart=False):
+        """main.f90:389-390."""
+        self.bounduvw(self.u, self.v, self.w, keep_norm=1 if restart else 0)
+        self.boundp(self.p, LB.HALO_UPPER)
+
+    # ---- operators (reference argument lists; single GPU) ----
+    def bounduvw(self, u, v, w, keep_norm=-1, halos_norm=LB.HALO_ABSENT, halos_tang=LB.HALO_ABSENT):
+        c, g = self.case, self.grid
+        LB.check(LB.lib().cans_bounduvw(self.ctx.h, c.cbcvel.encode(), LB.ivec(c.n), LB.dvec(c.bcvel),
+                                        LB.ivec(g.nb), LB.ivec(g.is_bound), LB.dvec(g.dl), _dp(self._dzc),
+                                        _dp(self._dzf), C.c_void_p(u.ptr), C.c_void_p(v.ptr), C.c_void_p(w.ptr),
+                                        keep_norm, halos_norm, halos_tang), "cans_bounduvw")
+
+    def boundp(self, p, halo_side=LB.HALO_ABSENT):
+        c, g = self.case, self.grid
+        LB.check(LB.lib().cans_boundp(self.ctx.h, c.cbcpre.encode(), LB.ivec(c.n), LB.dvec(c.bcpre),
+                                      LB.ivec(g.nb), LB.ivec(g.is_bound), LB.dvec(g.dl), _dp(self._dzc),
+                                      C.c_void_p(p.ptr), halo_side), "cans_boundp")
+
+    def rk(self, irk, dt, with_f=True):
+        c, g = self.case, self.grid
+        f = (C.c_double * 3)()
+        LB.check(LB.lib().cans_rk(self.ctx.h, LB.dvec(RKCOEFF[irk - 1]), LB.ivec(c.n), LB.dvec(g.dli),
+                                  C.c_void_p(self.dzci_d.data_ptr()), C.c_void_p(self.dzfi_d.data_ptr()),
+                                  C.c_void_p(self.gvr_c_d.data_ptr()), C.c_void_p(self.gvr_f_d.data_ptr()),
+                                  dt, g.visc, C.c_void_p(self.p.ptr),
+                                  LB.ivec([int(bool(x)) for x in c.is_forced]), LB.dvec(c.velf),
+                                  LB.dvec(c.bforce), *[C.c_void_p(t.data_ptr()) for t in self.rhs_o],
+                                  C.c_void_p(self.u.ptr), C.c_void_p(self.v.ptr), C.c_void_p(self.w.ptr),
+                                  f if with_f else None), "cans_rk")
+        return np.array(f[:]) if with_f else None
+
+    def fillps(self, dti):
+        g = self.grid
+        LB.check(LB.lib().cans_fillps(self.ctx.h, LB.ivec(g.n), LB.dvec(g.dli), C.c_void_p(self.dzfi_d.data_ptr()),
+                                      dti, C.c_void_p(self.u.ptr), C.c_void_p(self.v.ptr), C.c_void_p(self.w.ptr),
+                                      C.c_void_p(self.pp.ptr)), "cans_fillps")
+
+    def correc(self, dt):
+        g = self.grid
+        LB.check(LB.lib().cans_correc(self.ctx.h, LB.ivec(g.n), LB.dvec(g.dli), C.c_void_p(self.dzci_d.data_ptr()),
+                                      dt, C.c_void_p(self.pp.ptr), C.c_void_p(self.u.ptr), C.c_void_p(self.v.ptr),
+                                      C.c_void_p(self.w.ptr)), "cans_correc")
+
+    def updatep(self):
+        g = self.grid
+        LB.check(LB.lib().cans_updatep(self.ctx.h, LB.ivec(g.n), LB.dvec(g.dli),
+                                       C.c_void_p(self.dzci_d.data_ptr()), C.c_void_p(self.dzfi_d.data_ptr()), 0.0,
+                                       C.c_void_p(self.pp.ptr), C.c_void_p(self.p.ptr), 0), "cans_updatep")
+
+    def solver(self, f=None):
+        self.plan.solve(f or self.pp)
+
+    def chkdt(self):
+        g = self.grid
+        out = C.c_double()
+        LB.check(LB.lib().cans_chkdt(self.ctx.h, LB.ivec(g.n), LB.dvec(g.dl), C.c_void_p(self.dzci_d.data_ptr()),
+                                     C.c_void_p(self.dzfi_d.data_ptr()), g.visc, 0.0, C.c_void_p(self.u.ptr),
+                                     C.c_void_p(self.v.ptr), C.c_void_p(self.w.ptr), 0, 0, C.byref(out)),
+                 "cans_chkdt")
+        return out.value
+
+    def chkdiv(self):
+        g = self.grid
+        tot, mx = C.c_double(), C.c_double()
+        LB.check(LB.lib().cans_chkdiv(self.ctx.h, LB.ivec((1, 1, 1)), LB.ivec(g.n), LB.dvec(g.l), LB.dvec(g.dli),
+                                      C.c_void_p(self.dzfi_d.data_ptr()), C.c_void_p(self.u.ptr),
+                                      C.c_void_p(self.v.ptr), C.c_void_p(self.w.ptr), C.byref(tot), C.byref(mx)),
+                 "cans_chkdiv")
+        return tot.value, mx.value
+
+    # ---- main.f90:437-473 ----
+    def rk3_substep(self, irk, dt):
+        LB.check(LB.lib().cans_rk3_substep(self.ctx.h, C.byref(self.prm), irk, dt, C.byref(self.st)),
+                 "cans_rk3_substep")
+
+    def rk3_step(self, dt):
+        for irk in (1, 2, 3):
+            self.rk3_substep(irk, dt)
+
+    def rk3_step_unfused(self, dt):
+        """The same substeps through the per-routine entry points (main.f90:452-472)."""
+        for irk in (1, 2, 3):
+            rkpar = RKCOEFF[irk - 1]
+            dtrk = (rkpar[0] + rkpar[1]) * dt
+            dtrki = dtrk ** (-1)
+            self.rk(irk, dt, with_f=False)
+            self.bounduvw(self.u, self.v, self.w, 0, LB.HALO_LOWER)
+            self.fillps(dtrki)
+            self.solver()
+            self.boundp(self.pp)
+            self.correc(dtrk)
+            self.bounduvw(self.u, self.v, self.w, 1, LB.HALO_UPPER, LB.HALO_NONE)
+            self.updatep()
+            self.boundp(self.p, LB.HALO_UPPER)
+
+    def close(self):
+        self.plan.close()
+        self.ctx.sync()

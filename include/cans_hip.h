@@ -1,0 +1,173 @@
+/*
+ * cans_hip.h -- C ABI of the MI355X (gfx950) pressure-correction time step.
+ *
+ * Drop-in boundary for CaNS's per-routine Fortran interfaces on one GPU.  Every
+ * entry point takes plain pointers and sizes, is ordered on the context's HIP
+ * stream, returns 0 on success or a negative CANS_E* code (message in
+ * cans_last_error()), and synchronises only where the reference reads a result
+ * back on the host (reductions that return a value, downloads).
+ *
+ * Arrays follow the reference's Fortran conventions:
+ *   - fields are halo'd (0:n1+1, 0:n2+1, 0:n3+1) with i fastest; the context
+ *     fixes the leading dimension ld >= n1+2 and plane stride ld*(n2+2); a field
+ *     pointer is the address of element (0,0,0);
+ *   - z arrays are (0:n3+1); a,b,c are (1:n3); lambdaxy is (1:n1,1:n2);
+ *   - character arrays are the flattened Fortran arrays (cbcvel(0:1,3,3) -> 18
+ *     chars, cbcpre(0:1,3) -> 6, bcxy(0:1,2) -> 4), logicals are int 0/1;
+ *   - nb(0:1,3): on one GPU a value >= 0 marks a periodic neighbour (this rank),
+ *     a negative value (MPI_PROC_NULL) marks none.
+ * Device pointers are marked _d, host pointers _h; unmarked scalars are by value.
+ * Field storage is double (CANS_FP64); the mixed mode is documented per call.
+ */
+#ifndef CANS_HIP_H
+#define CANS_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cans_ctx cans_ctx;
+typedef struct cans_plan cans_plan;
+
+enum { CANS_OK = 0, CANS_EINVAL = -1, CANS_EHIP = -2, CANS_ENOMEM = -3, CANS_EUNSUPPORTED = -4 };
+enum { CANS_FP64 = 0, CANS_MIXED = 1 };
+enum { CANS_HALO_ABSENT = -99, CANS_HALO_LOWER = -1, CANS_HALO_NONE = 0, CANS_HALO_UPPER = 1 };
+
+/* ---------------------------------------------------------------------------
+ * Context, memory (replaces the device setup of src/initmpi.f90:40-279 and the
+ * work buffers of src/workspaces.f90:16-147; the host keeps ownership of its
+ * Fortran arrays and moves them with upload/download, as `!$acc enter data` /
+ * `!$acc update self` do in src/main.f90:388,423).
+ * ------------------------------------------------------------------------- */
+/* n: interior sizes; ld: leading dimension (0 selects n1+2); stream: hipStream_t (NULL = default) */
+int cans_ctx_create(const int *n, int ld, int precision, void *stream, cans_ctx **ctx);
+int cans_ctx_destroy(cans_ctx *ctx);
+/* strides of the halo'd layout, in elements */
+int cans_ctx_layout(const cans_ctx *ctx, long *sy, long *sz, long *field_elems);
+int cans_ctx_sync(cans_ctx *ctx);
+const char *cans_last_error(void);
+
+/* field = device pointer to element (0,0,0) of a halo'd field owned by the context */
+int cans_field_alloc(cans_ctx *ctx, void **field_d);
+int cans_field_free(cans_ctx *ctx, void *field_d);
+/* host arrays are the reference's contiguous (0:n1+1,0:n2+1,0:n3+1) Fortran arrays */
+int cans_field_upload(cans_ctx *ctx, void *field_d, const double *host_h);
+int cans_field_download(cans_ctx *ctx, const void *field_d, double *host_h);
+
+/* ---------------------------------------------------------------------------
+ * Host-side set-up helpers, for hosts that do not keep the Fortran versions.
+ * ------------------------------------------------------------------------- */
+/* src/initgrid.f90:14-118 (gtype 1-4); arrays (0:n+1) */
+int cans_initgrid(int gtype, int n, double gr, double lz, int is_periodic,
+                  double *dzc_h, double *dzf_h, double *zc_h, double *zf_h);
+/* src/initsolver.f90:85-144 in the device ordering of this library (the
+ * reference's GPU ordering, initsolver.f90:98-117, for periodic directions) */
+int cans_eigenvalues(int n, const char *bc, char c_or_f, double *lambda_h);
+/* src/initsolver.f90:146-187; a,b,c (1:n) */
+int cans_tridmatrix(const char *bc, int n, const double *dzci_h, const double *dzfi_h,
+                    char c_or_f, double *a_h, double *b_h, double *c_h);
+
+/* ---------------------------------------------------------------------------
+ * Transforms: twins of fftini/fftend (src/fft.f90:25-245).  arrplan holds the
+ * four batched transforms ((fwd,bwd) x (x,y)); normfft is returned as the
+ * reference computes it (fft.f90:75,140,202,208).
+ * ------------------------------------------------------------------------- */
+int cans_fftini(cans_ctx *ctx, const int *ng, const char *bcxy, const char *c_or_f,
+                cans_plan **arrplan, double *normfft);
+int cans_fftend(cans_plan *arrplan);
+
+/* ---------------------------------------------------------------------------
+ * Operators.  Each replaces the reference routine cited; argument meaning and
+ * order follow the Fortran interface, with the context first.
+ * ------------------------------------------------------------------------- */
+/* src/solver.f90:17 / src/solver_gpu.f90:34 -- p(0:,0:,0:) overwritten with the solution */
+int cans_solver(cans_ctx *ctx, const int *n, const int *ng, const cans_plan *arrplan,
+                double normfft, const double *lambdaxy_d, const double *a_d, const double *b_d,
+                const double *c_d, const char *bc, const char *c_or_f, void *p_d);
+/* src/solve_helmholtz.f90:28 (3-D branch) -- rhsb*_d may be NULL */
+int cans_solve_helmholtz(cans_ctx *ctx, const int *n, const int *ng, const cans_plan *arrplan,
+                         double normfft, double alpha, const double *lambdaxy_d,
+                         const double *a_d, const double *b_d, const double *c_d,
+                         const double *rhsbx_d, const double *rhsby_d, const double *rhsbz_d,
+                         const int *is_bound, const char *cbc, const char *c_or_f, void *p_d);
+/* src/rk.f90:24 (no scalars, explicit diffusion) -- f_h (3) receives the bulk
+ * forcing like the reference's intent(out) f; pass NULL to keep it on the
+ * device (no host synchronisation) */
+int cans_rk(cans_ctx *ctx, const double *rkpar, const int *n, const double *dli,
+            const double *dzci_d, const double *dzfi_d, const double *grid_vol_ratio_c_d,
+            const double *grid_vol_ratio_f_d, double dt, double visc, const void *p_d,
+            const int *is_forced, const double *velf, const double *bforce,
+            void *dudtrko_d, void *dvdtrko_d, void *dwdtrko_d,
+            void *u_d, void *v_d, void *w_d, double *f_h);
+/* src/fillps.f90:13 */
+int cans_fillps(cans_ctx *ctx, const int *n, const double *dli, const double *dzfi_d, double dti,
+                const void *u_d, const void *v_d, const void *w_d, void *p_d);
+/* src/correc.f90:13 */
+int cans_correc(cans_ctx *ctx, const int *n, const double *dli, const double *dzci_d, double dt,
+                const void *p_d, void *u_d, void *v_d, void *w_d);
+/* src/updatep.f90:14 (is_impdiff adds alpha*lap(pp)) */
+int cans_updatep(cans_ctx *ctx, const int *n, const double *dli, const double *dzci_d,
+                 const double *dzfi_d, double alpha, const void *pp_d, void *p_d, int is_impdiff);
+/* src/bound.f90:17 -- keep_norm < 0 and halos_* == CANS_HALO_ABSENT mean "argument absent" */
+int cans_bounduvw(cans_ctx *ctx, const char *cbc, const int *n, const double *bc, const int *nb,
+                  const int *is_bound, const double *dl, const double *dzc_h, const double *dzf_h,
+                  void *u_d, void *v_d, void *w_d, int keep_norm, int halos_norm, int halos_tang);
+/* src/bound.f90:124 */
+int cans_boundp(cans_ctx *ctx, const char *cbc, const int *n, const double *bc, const int *nb,
+                const int *is_bound, const double *dl, const double *dzc_h, void *p_d, int halo_side);
+/* src/bound.f90:514 -- rhsb*_d are (n2,n3,0:1), (n1,n3,0:1), (n1,n2,0:1) or NULL; alpha_h NULL = absent */
+int cans_updt_rhs_b(cans_ctx *ctx, const char *c_or_f, const char *cbc, const int *n,
+                    const int *is_bound, const double *rhsbx_d, const double *rhsby_d,
+                    const double *rhsbz_d, void *p_d, const double *alpha_h);
+/* src/chkdiv.f90:15 (synchronises) */
+int cans_chkdiv(cans_ctx *ctx, const int *lo, const int *hi, const double *l, const double *dli,
+                const double *dzfi_d, const void *u_d, const void *v_d, const void *w_d,
+                double *divtot_h, double *divmax_h);
+/* src/chkdt.f90:16 (synchronises) */
+int cans_chkdt(cans_ctx *ctx, const int *n, const double *dl, const double *dzci_d,
+               const double *dzfi_d, double visc, double alpha, const void *u_d, const void *v_d,
+               const void *w_d, int is_impdiff, int is_impdiff_1d, double *dtmax_h);
+/* src/utils.f90:15 (synchronises) */
+int cans_bulk_mean(cans_ctx *ctx, const int *n, const double *grid_vol_ratio_d, const void *p_d,
+                   double *mean_h);
+
+/* ---------------------------------------------------------------------------
+ * Fused RK3 substep: the whole call sequence of src/main.f90:452-472 (no
+ * scalars, explicit diffusion) in one stream-ordered call with no host
+ * synchronisation.  Same results as calling the operators above in order.
+ * ------------------------------------------------------------------------- */
+typedef struct cans_step_params {
+  char cbcvel[18];
+  double bcvel[18];
+  char cbcpre[6];
+  double bcpre[6];
+  int nb[6];
+  int is_bound[6];
+  double dl[3], dli[3];
+  double visc;
+  int is_forced[3];
+  double velf[3];
+  double bforce[3];
+  const double *dzc_h, *dzf_h;                 /* (0:n3+1) host copies */
+  const double *dzci_d, *dzfi_d;               /* (0:n3+1) device copies */
+  const double *grid_vol_ratio_c_d, *grid_vol_ratio_f_d;
+  const cans_plan *plan;                       /* pressure plan from cans_fftini */
+  double normfft;
+  const double *lambdaxy_d, *a_d, *b_d, *c_d;  /* pressure solver arrays */
+  const double *rhsbx_d, *rhsby_d, *rhsbz_d;   /* pressure BC rhs, NULL if homogeneous */
+} cans_step_params;
+
+typedef struct cans_state {
+  void *u, *v, *w, *p, *pp;                    /* halo'd fields */
+  void *dudtrko, *dvdtrko, *dwdtrko;           /* (1:n1,1:n2,1:n3) */
+} cans_state;
+
+/* irk in 1..3; rkcoeff as src/param.f90:20-22 */
+int cans_rk3_substep(cans_ctx *ctx, const cans_step_params *prm, int irk, double dt, cans_state *st);
+/* forcing of the last substep, device pointer to 3 doubles (f(1:3) of rk.f90) */
+int cans_ctx_forcing(cans_ctx *ctx, const double **f_d);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

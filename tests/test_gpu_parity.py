@@ -1,0 +1,176 @@
+"""Parity of the HIP path (through the C ABI) against the oracle on identical inputs.
+
+Stencil, boundary and reduction kernels keep the reference's association and
+rounding, so they are compared bit for bit.  The Poisson solve differs from the
+oracle only through the transforms' rounding; it and whole RK3 steps are held to
+the FP64 bar of 1e-12 relative L-inf (max |gpu - oracle| / max |oracle|, taken
+over the full halo'd arrays).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from cans_amd import initflow as IF
+from tests import cases
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-12
+
+
+def relerr(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def pair(name, ng, seed=7):
+    from cans_amd.host import Simulation
+    print("\n[case] %s ng=%s seed=%d" % (name, ng, seed), flush=True)
+    p = cases.params(name, ng=ng)
+    sim = Simulation(cases.host_case(p))
+    orc = cases.oracle_case(p)
+    u, v, w, pr = cases.initial_state(name, p, sim.grid, seed=seed)
+    st = orc.new_state()
+    for k, q in zip("uvwp", (u, v, w, pr)):
+        st[k][1:-1, 1:-1, 1:-1] = q
+    sim.set_state(*(IF.halo(p["ng"], q) for q in (u, v, w, pr)))
+    return p, sim, orc, st
+
+
+SOLVER_CASES = [("tgv", (16, 16, 16)), ("tgv", (32, 8, 12)), ("channel", (32, 16, 24)),
+                ("ldc", (2, 64, 64)), ("duct", (16, 32, 20)), ("cavity_dd", (16, 32, 24)),
+                ("channel", (128, 64, 48))]
+
+
+@pytest.mark.parametrize("name,ng", SOLVER_CASES, ids=["%s-%dx%dx%d" % ((c,) + g) for c, g in SOLVER_CASES])
+def test_solver_matches_oracle(torch, name, ng):
+    p, sim, orc, st = pair(name, ng)
+    rng = np.random.default_rng(11)
+    rhs = rng.uniform(-1, 1, tuple(ng))
+    if p["cbcpre"][4:6] != "DD":
+        wgt = np.broadcast_to(1 / sim.grid.dzfi[1:-1], rhs.shape)
+        rhs -= (rhs * wgt).sum() / wgt.sum()
+    f = IF.halo(ng, rhs)
+    sim.pp.set(f)
+    sim.solver()
+    sim.ctx.sync()
+    st["pp"][...] = f
+    orc.solver(st["pp"])
+    got = sim.pp.numpy()
+    err = relerr(got[1:-1, 1:-1, 1:-1], st["pp"][1:-1, 1:-1, 1:-1])
+    print("solver rel err %.3e" % err, flush=True)
+    assert err < RTOL
+
+
+BOUND_CASES = [("tgv", (8, 6, 5)), ("channel", (8, 6, 5)), ("ldc", (2, 8, 6)), ("cavity_dd", (6, 8, 5))]
+FLAGS = [(-1, -99, -99), (0, -1, -99), (1, 1, 0), (0, -99, -99), (1, -99, -99)]
+
+
+@pytest.mark.parametrize("name,ng", BOUND_CASES, ids=[c for c, g in BOUND_CASES])
+def test_bounds_bitwise(torch, name, ng):
+    p, sim, orc, st = pair(name, ng)
+    rng = np.random.default_rng(3)
+    bcvel = rng.uniform(-1, 1, 18)  # non-zero wall values exercise the BC factors
+    sim.case.bcvel = tuple(bcvel)
+    orc.bcvel = bcvel.copy()
+    for flags in FLAGS:
+        fields = [np.asfortranarray(rng.uniform(-1, 1, tuple(x + 2 for x in ng))) for _ in range(4)]
+        for f, a in zip((sim.u, sim.v, sim.w, sim.p), fields):
+            f.set(a)
+        ou, ov, ow, op = (a.copy(order="F") for a in fields)
+        sim.bounduvw(sim.u, sim.v, sim.w, *flags)
+        orc.bounduvw(ou, ov, ow, *flags)
+        for hs in (-99, 1, -1):
+            sim.boundp(sim.p, hs)
+            orc.boundp(op, hs)
+        sim.ctx.sync()
+        for f, ref in zip((sim.u, sim.v, sim.w, sim.p), (ou, ov, ow, op)):
+            np.testing.assert_array_equal(f.numpy(), ref, err_msg="flags=%s" % (flags,))
+
+
+@pytest.mark.parametrize("name,ng", [("tgv", (16, 12, 10)), ("channel", (24, 16, 20))])
+def test_operators_bitwise(torch, name, ng):
+    """rk (mom_xyz_ad + update + forcing), fillps, correc, updatep, chkdiv, chkdt."""
+    p, sim, orc, st = pair(name, ng)
+    rng = np.random.default_rng(5)
+    for k in "uvwp":
+        a = np.asfortranarray(rng.uniform(-1, 1, tuple(x + 2 for x in ng)))
+        st[k][...] = a
+        getattr(sim, k).set(a)
+    ro = [np.asfortranarray(rng.uniform(-1, 1, tuple(ng))) for _ in range(3)]
+    for t, a in zip(sim.rhs_o, ro):
+        t.copy_(torch.from_numpy(a.ravel(order="F")))
+    st["dudtrko"][...], st["dvdtrko"][...], st["dwdtrko"][...] = ro
+    # the first rk call zeroes RHS_old in both (rk.f90:72-80)
+    for name_ in ("dudtrko", "dvdtrko", "dwdtrko"):
+        st[name_][...] = 0.0
+    dt = 1e-3
+    for irk in (1, 2):
+        f_gpu = sim.rk(irk, dt)
+        f_orc = orc.rk(cases_rk(irk), dt, st["p"], st["dudtrko"], st["dvdtrko"], st["dwdtrko"],
+                       st["u"], st["v"], st["w"])
+        sim.ctx.sync()
+        for k in "uvw":
+            np.testing.assert_array_equal(getattr(sim, k).numpy(), st[k], err_msg="rk irk=%d %s" % (irk, k))
+        for t, name_ in zip(sim.rhs_o, ("dudtrko", "dvdtrko", "dwdtrko")):
+            np.testing.assert_array_equal(t.cpu().numpy(), st[name_].ravel(order="F"))
+        np.testing.assert_allclose(f_gpu, f_orc, rtol=1e-13, atol=1e-15)
+    dti = 1.0 / (dt * 8 / 60)
+    sim.fillps(dti)
+    orc.fillps(st["u"], st["v"], st["w"], st["pp"], dti)
+    sim.ctx.sync()
+    np.testing.assert_array_equal(sim.pp.numpy()[1:-1, 1:-1, 1:-1], st["pp"][1:-1, 1:-1, 1:-1])
+    pp = np.asfortranarray(rng.uniform(-1, 1, tuple(x + 2 for x in ng)))
+    sim.pp.set(pp)
+    st["pp"][...] = pp
+    sim.correc(dt)
+    orc.correc(st["pp"], st["u"], st["v"], st["w"], dt)
+    sim.updatep()
+    orc.updatep(st["pp"], st["p"])
+    sim.ctx.sync()
+    for k in "uvwp":
+        np.testing.assert_array_equal(getattr(sim, k).numpy(), st[k], err_msg="correc/updatep " + k)
+    tot, mx = sim.chkdiv()
+    otot, omx = orc.chkdiv(st["u"], st["v"], st["w"])
+    assert mx == omx
+    assert abs(tot - otot) <= 1e-13 * abs(otot)
+    dtm = sim.chkdt()
+    odtm = orc.chkdt(st["u"], st["v"], st["w"])
+    assert dtm == odtm
+
+
+def cases_rk(irk):
+    from oracle.oracle import RKCOEFF
+    return RKCOEFF[irk - 1]
+
+
+STEP_CASES = [("tgv", (32, 32, 32), 1e-3, 3), ("channel", (64, 32, 48), 2e-4, 3), ("duct", (32, 16, 24), 1e-3, 2),
+              ("cavity_dd", (16, 16, 12), 1e-3, 2)]
+
+
+@pytest.mark.parametrize("name,ng,dt,nsteps", STEP_CASES, ids=[c[0] for c in STEP_CASES])
+@pytest.mark.parametrize("fused", [True, False], ids=["fused", "per-routine"])
+def test_rk3_steps_match_oracle(torch, name, ng, dt, nsteps, fused):
+    p, sim, orc, st = pair(name, ng)
+    sim.init_bounds()
+    orc.init_bounds(st)
+    for it in range(nsteps):
+        (sim.rk3_step if fused else sim.rk3_step_unfused)(dt)
+        orc.rk3_step(st, dt)
+    sim.ctx.sync()
+    for k in "uvwp":
+        err = relerr(getattr(sim, k).numpy(), st[k])
+        print("%s rel err %.3e" % (k, err), flush=True)
+        assert err < RTOL, (k, err)
+    tot, mx = sim.chkdiv()
+    otot, omx = orc.chkdiv(st["u"], st["v"], st["w"])
+    print("divmax gpu %.3e oracle %.3e" % (mx, omx), flush=True)
+    assert mx < 2.2e-9 and mx <= 10 * max(omx, 1e-15)
