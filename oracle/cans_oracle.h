@@ -47,6 +47,9 @@ int orc_fftini(const int *ng, const char *bcxy, const char *c_or_f,
 
 /* batched in-place r2r transform with FFTW's unnormalised definitions:
  * howmany lines of length n; element e of line l at data[l*dist + e*stride] */
+/* 0 (default) or 1: an alternative factorisation / real-FFT route with the same
+ * definitions and different rounding, used to measure the algorithm's own spread */
+void orc_set_fft_variant(int v);
 int orc_r2r(int kind, int n, long howmany, long stride, long dist, double *data);
 
 /* src/solver.f90:114-307 on a compact (nx,ny,>=n) array */
@@ -112,6 +115,7 @@ double orc_chkdt(const int *n, const double *dl, const double *dzci, const doubl
                  const double *w, int is_impdiff, int is_impdiff_1d);
 
 int orc_num_threads(void);
+void orc_set_threads(int n);
 
 #ifdef __cplusplus
 }

@@ -77,6 +77,16 @@ def _declare(L):
     L.orc_chkdt.argtypes = [_ip, _dp, _dp, _dp, C.c_double, C.c_double, _dp, _dp, _dp, C.c_int, C.c_int]
     L.orc_chkdt.restype = C.c_double
     L.orc_num_threads.restype = C.c_int
+    L.orc_set_fft_variant.argtypes = [C.c_int]
+    L.orc_set_threads.argtypes = [C.c_int]
+
+
+def fft_variant(v):
+    lib().orc_set_fft_variant(int(v))
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
 
 
 def P(a):

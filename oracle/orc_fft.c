@@ -48,6 +48,13 @@ static cx twiddle(long t, long n) {
   return r;
 }
 
+/* Rounding variant: 0 = radix-4-first factorisation and half-length real FFTs;
+ * 1 = radix-2/odd factorisation and full-length complex FFTs for R2HC/HC2R.  Both
+ * compute FFTW's definitions; comparing them measures how far two valid FP64
+ * implementations of the same transforms drift apart through the solver. */
+static int g_variant = 0;
+void orc_set_fft_variant(int v) { g_variant = v; }
+
 typedef struct {
   int n, nfac, fac[40];
   cx *tw;     /* e^{-2 pi i t/n}, t = 0..n-1 */
@@ -56,7 +63,7 @@ typedef struct {
 static void cplan_init(cplan *p, int n) {
   p->n = n; p->nfac = 0;
   int m = n;
-  while (m % 4 == 0) { p->fac[p->nfac++] = 4; m /= 4; }
+  while (!g_variant && m % 4 == 0) { p->fac[p->nfac++] = 4; m /= 4; }
   while (m % 2 == 0) { p->fac[p->nfac++] = 2; m /= 2; }
   for (int f = 3; m > 1; f += 2)
     while (m % f == 0) { p->fac[p->nfac++] = f; m /= f; }
@@ -145,7 +152,7 @@ static void rplan_init(rplan *rp, int kind, int n) {
   rp->kind = kind; rp->n = n;
   switch (kind) {
   case ORC_R2HC: case ORC_HC2R:
-    if (n % 2 == 0) {
+    if (n % 2 == 0 && !g_variant) {
       cplan_init(&rp->cp, n / 2);
       rp->post = (cx *)malloc(sizeof(cx) * (n / 2 + 1));
       for (int k = 0; k <= n / 2; k++) rp->post[k] = twiddle(k, n);
@@ -181,7 +188,7 @@ static void r2r_line(const rplan *rp, double *x, cx *w1, cx *w2) {
   switch (rp->kind) {
   case ORC_R2HC: {
     if (n == 1) return;
-    if (n % 2 == 0) {
+    if (n % 2 == 0 && rp->post) {
       int h = n / 2;
       for (int m = 0; m < h; m++) { w1[m].re = x[2 * m]; w1[m].im = x[2 * m + 1]; }
       cfft(&rp->cp, w1, w2, -1);
@@ -202,12 +209,13 @@ static void r2r_line(const rplan *rp, double *x, cx *w1, cx *w2) {
       cfft(&rp->cp, w1, w2, -1);
       x[0] = w1[0].re;
       for (int k = 1; k <= (n - 1) / 2; k++) { x[k] = w1[k].re; x[n - k] = w1[k].im; }
+      if (n % 2 == 0) x[n / 2] = w1[n / 2].re;
     }
     return;
   }
   case ORC_HC2R: {
     if (n == 1) return;
-    if (n % 2 == 0) {
+    if (n % 2 == 0 && rp->post) {
       int h = n / 2;
       /* unpack halfcomplex */
       w2[0].re = x[0]; w2[0].im = 0;
@@ -228,6 +236,7 @@ static void r2r_line(const rplan *rp, double *x, cx *w1, cx *w2) {
         w1[k].re = x[k]; w1[k].im = x[n - k];
         w1[n - k].re = x[k]; w1[n - k].im = -x[n - k];
       }
+      if (n % 2 == 0) { w1[n / 2].re = x[n / 2]; w1[n / 2].im = 0; }
       cfft(&rp->cp, w1, w2, +1);
       for (int j = 0; j < n; j++) x[j] = w1[j].re;
     }

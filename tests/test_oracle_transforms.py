@@ -85,8 +85,10 @@ KINDS = [O.R2HC, O.HC2R, O.REDFT00, O.REDFT10, O.REDFT01, O.REDFT11,
          O.RODFT00, O.RODFT10, O.RODFT01, O.RODFT11]
 
 
+@pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("kind", KINDS, ids=[O.KIND_NAMES[k] for k in KINDS])
-def test_r2r_against_scipy_and_brute_force(kind):
+def test_r2r_against_scipy_and_brute_force(kind, variant):
+    O.fft_variant(variant)
     rng = np.random.default_rng(1234 + kind)
     for n in SIZES:
         if kind == O.REDFT00 and n == 1:
@@ -99,6 +101,7 @@ def test_r2r_against_scipy_and_brute_force(kind):
                                        err_msg="%s n=%d (brute)" % (O.KIND_NAMES[kind], n))
         np.testing.assert_allclose(got, scipy_ref(kind, x), rtol=0, atol=2e-14 * scale,
                                    err_msg="%s n=%d (scipy)" % (O.KIND_NAMES[kind], n))
+    O.fft_variant(0)
 
 
 PAIRS = [(O.R2HC, O.HC2R, lambda n: n), (O.REDFT10, O.REDFT01, lambda n: 2 * n),
