@@ -109,6 +109,21 @@ def main():
     t_p = s0.elapsed_time(s1) / args.solves * 1e-3
     b_p = 80.0 * ncell
     gbs = b_p / t_p / 1e9
+    # device copy of a known size: the box's streaming bandwidth (and the reference
+    # point that scales the FETCH_SIZE / WRITE_SIZE counters of the kernels)
+    nbytes = 1 << 30
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    c0.record()
+    for _ in range(5):
+        dst.copy_(src)
+    c1.record()
+    torch.cuda.synchronize()
+    copy_gbs = 2 * nbytes * 5 / (c0.elapsed_time(c1) * 1e-3) / 1e9
+    del src, dst
     tot, divmax = sim.chkdiv()
     out = {
         "metric": METRIC,
@@ -130,6 +145,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gbs / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "Poisson solve (x-fwd, y-fwd, z-Thomas, y-bwd, x-bwd passes)"},
+        "copy_GBps": copy_gbs,
         "divmax": divmax,
         "wall_s": wall,
     }

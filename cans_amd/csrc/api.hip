@@ -182,7 +182,7 @@ static int rk_core(cans_ctx* c, const double* rkpar, const double* dli, const do
                    const double* gvr_c, const double* gvr_f, double dt, double visc, const double* p,
                    const int* is_forced, const double* velf, const double* bforce, double* ru, double* rv,
                    double* rw, const double* u, const double* v, const double* w, double* uo, double* vo,
-                   double* wo) {
+                   double* wo, int write_ro = 1) {
   cans_rk_state* st = rk_state(c);
   if (st->first) {
     st->first = false;
@@ -198,6 +198,8 @@ static int rk_core(cans_ctx* c, const double* rkpar, const double* dli, const do
   m.dzci = dzci; m.dzfi = dzfi; m.gvr_c = gvr_c; m.gvr_f = gvr_f;
   m.dxi = dli[0]; m.dyi = dli[1]; m.visc = visc; m.dt = dt;
   m.rkpar[0] = rkpar[0]; m.rkpar[1] = rkpar[1];
+  m.read_ro = 1;
+  m.write_ro = write_ro;
   for (int d = 0; d < 3; d++) {
     m.bforce[d] = bforce[d];
     m.velf[d] = velf[d];
@@ -389,23 +391,35 @@ extern "C" int cans_rk3_substep(cans_ctx* c, const cans_step_params* s, int irk,
   // rk: updated velocity into the scratch fields t (u,v,w keep the old level for the stencil)
   rc = rk_core(c, rkpar, s->dli, s->dzci_d, s->dzfi_d, s->grid_vol_ratio_c_d, s->grid_vol_ratio_f_d, dt, s->visc, p,
                s->is_forced, s->velf, s->bforce, (double*)st->dudtrko, (double*)st->dvdtrko, (double*)st->dwdtrko, u,
-               v, w, t[0], t[1], t[2]);
+               v, w, t[0], t[1], t[2], irk != 3);
   if (rc) return rc;
   for (int k = 0; k < 3; k++)
     if (s->is_forced[k] && (rc = launch_add_forcing(c, t[k], t[k], k, 1))) return rc;
   if ((rc = plan_bounduvw(c, s->cbcvel, s->bcvel, s->nb, s->is_bound, s->dl, s->dzc_h, s->dzf_h, t[0], t[1], t[2], 0,
                           CANS_HALO_LOWER, CANS_HALO_ABSENT)))
     return rc;
-  if ((rc = launch_fillps(c, t[0], t[1], t[2], pp, s->dzfi_d, dtrki, s->dli))) return rc;
-  if (s->rhsbx_d || s->rhsby_d || s->rhsbz_d)
-    if ((rc = do_updt_rhs_b(c, "ccc", s->cbcpre, s->is_bound, s->rhsbx_d, s->rhsby_d, s->rhsbz_d, pp, 1.))) return rc;
-  if ((rc = run_solver(c, s->plan, s->normfft, s->lambdaxy_d, s->a_d, s->b_d, s->c_d, s->cbcpre, "ccc", pp))) return rc;
+  // fillps -> updt_rhs_b -> solver.  With homogeneous pressure BCs and a periodic x
+  // transform the divergence is formed inside the first transform pass.
+  const bool fuse_div = !(s->rhsbx_d || s->rhsby_d || s->rhsbz_d) && s->cbcpre[0] == 'P' && s->cbcpre[1] == 'P' &&
+                        c->L.n1 >= 2 && c->L.n1 % 2 == 0;
+  if (fuse_div) {
+    DivSrc ds = {t[0], t[1], t[2], s->dzfi_d, dtrki, dtrki * s->dli[0], dtrki * s->dli[1]};
+    if ((rc = run_solver(c, s->plan, s->normfft, s->lambdaxy_d, s->a_d, s->b_d, s->c_d, s->cbcpre, "ccc", pp, &ds)))
+      return rc;
+  } else {
+    if ((rc = launch_fillps(c, t[0], t[1], t[2], pp, s->dzfi_d, dtrki, s->dli))) return rc;
+    if (s->rhsbx_d || s->rhsby_d || s->rhsbz_d)
+      if ((rc = do_updt_rhs_b(c, "ccc", s->cbcpre, s->is_bound, s->rhsbx_d, s->rhsby_d, s->rhsbz_d, pp, 1.)))
+        return rc;
+    if ((rc = run_solver(c, s->plan, s->normfft, s->lambdaxy_d, s->a_d, s->b_d, s->c_d, s->cbcpre, "ccc", pp)))
+      return rc;
+  }
   if ((rc = plan_boundp(c, s->cbcpre, s->bcpre, s->nb, s->is_bound, s->dl, s->dzc_h, pp, CANS_HALO_ABSENT))) return rc;
-  // correc reads the scratch velocity and writes the caller's fields (whole halo'd box)
-  if ((rc = launch_correc(c, pp, t[0], t[1], t[2], u, v, w, s->dzci_d, dtrk, s->dli))) return rc;
+  // correc reads the scratch velocity and writes the caller's fields (whole halo'd box);
+  // updatep (p += pp) is independent of the velocity halos and rides along
+  if ((rc = launch_correc(c, pp, t[0], t[1], t[2], u, v, w, s->dzci_d, dtrk, s->dli, p))) return rc;
   if ((rc = plan_bounduvw(c, s->cbcvel, s->bcvel, s->nb, s->is_bound, s->dl, s->dzc_h, s->dzf_h, u, v, w, 1,
                           CANS_HALO_UPPER, CANS_HALO_NONE)))
     return rc;
-  if ((rc = launch_updatep(c, pp, p, s->dzci_d, s->dzfi_d, 0., s->dli, 0))) return rc;
   return plan_boundp(c, s->cbcpre, s->bcpre, s->nb, s->is_bound, s->dl, s->dzc_h, p, CANS_HALO_UPPER);
 }

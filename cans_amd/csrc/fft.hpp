@@ -1,7 +1,7 @@
 // LDS-resident batched complex FFT building block (radix 8/4/2 Stockham).
 //
 // One "team" of T threads transforms one line of N = 2^m complex values that
-// sits in LDS with element stride S.  T = max(1, N/8): every stage leaves each
+// sits in LDS, addressed through an element map.  T = max(1, N/8): every stage leaves each
 // thread exactly 8 (or N, when N < 8) values in registers.  All teams of a
 // work-group run the same N and call this together (it contains barriers).
 //
@@ -76,9 +76,21 @@ __device__ __forceinline__ void bfly(cplx<T>* v) {
   else bfly8<T, s>(v);
 }
 
+// LDS element maps: contiguous line with one pad slot every 16 elements (keeps the
+// strided Stockham writes free of bank conflicts), or a line interleaved with S others.
+struct IxPad {
+  int base;
+  __device__ __forceinline__ int operator()(int e) const { return base + e + (e >> 4); }
+};
+struct IxStride {
+  int base, S;
+  __device__ __forceinline__ int operator()(int e) const { return base + e * S; }
+};
+__host__ __device__ constexpr int padded_len(int N) { return N + (N >> 4) + 1; }
+
 // one Stockham stage: R radix, NBF butterflies per thread
-template <typename T, int s, int R, int NBF>
-__device__ __forceinline__ void fft_stage(cplx<T>* buf, int S, int N, int lNs, int lt, int Tn,
+template <typename T, int s, int R, int NBF, class IX>
+__device__ __forceinline__ void fft_stage(cplx<T>* buf, IX ix, int N, int lNs, int lt, int Tn,
                                           const cplx<T>* __restrict__ tw) {
   constexpr int LR = R == 2 ? 1 : R == 4 ? 2 : 3;
   const int NR = N >> LR;
@@ -88,7 +100,7 @@ __device__ __forceinline__ void fft_stage(cplx<T>* buf, int S, int N, int lNs, i
   for (int bb = 0; bb < NBF; bb++) {
     const int j = lt + bb * Tn;
 #pragma unroll
-    for (int r = 0; r < R; r++) v[bb][r] = buf[(j + r * NR) * S];
+    for (int r = 0; r < R; r++) v[bb][r] = buf[ix(j + r * NR)];
   }
   __syncthreads();
 #pragma unroll
@@ -106,32 +118,32 @@ __device__ __forceinline__ void fft_stage(cplx<T>* buf, int S, int N, int lNs, i
     bfly<T, s, R>(v[bb]);
     const int idxD = ((j >> lNs) << (lNs + LR)) + k;
 #pragma unroll
-    for (int r = 0; r < R; r++) buf[(idxD + r * Ns) * S] = v[bb][r];
+    for (int r = 0; r < R; r++) buf[ix(idxD + r * Ns)] = v[bb][r];
   }
   __syncthreads();
 }
 
 // full transform of one line (unnormalised), s = -1 forward, +1 inverse
-template <typename T, int s>
-__device__ void fft_line(cplx<T>* buf, int S, int N, int logN, int lt, int Tn, const cplx<T>* __restrict__ tw) {
+template <typename T, int s, class IX>
+__device__ void fft_line(cplx<T>* buf, IX ix, int N, int logN, int lt, int Tn, const cplx<T>* __restrict__ tw) {
   if (logN == 0) return;
   if (logN == 1) {
-    fft_stage<T, s, 2, 1>(buf, S, N, 0, lt, Tn, tw);
+    fft_stage<T, s, 2, 1>(buf, ix, N, 0, lt, Tn, tw);
     return;
   }
   if (logN == 2) {
-    fft_stage<T, s, 4, 1>(buf, S, N, 0, lt, Tn, tw);
+    fft_stage<T, s, 4, 1>(buf, ix, N, 0, lt, Tn, tw);
     return;
   }
   int lNs = 0;
   int rem = logN;
   while (rem >= 3) {
-    fft_stage<T, s, 8, 1>(buf, S, N, lNs, lt, Tn, tw);
+    fft_stage<T, s, 8, 1>(buf, ix, N, lNs, lt, Tn, tw);
     lNs += 3;
     rem -= 3;
   }
-  if (rem == 2) fft_stage<T, s, 4, 2>(buf, S, N, lNs, lt, Tn, tw);
-  else if (rem == 1) fft_stage<T, s, 2, 4>(buf, S, N, lNs, lt, Tn, tw);
+  if (rem == 2) fft_stage<T, s, 4, 2>(buf, ix, N, lNs, lt, Tn, tw);
+  else if (rem == 1) fft_stage<T, s, 2, 4>(buf, ix, N, lNs, lt, Tn, tw);
 }
 
 }  // namespace cans

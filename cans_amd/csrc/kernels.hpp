@@ -28,18 +28,29 @@ struct MomLaunch {
   double rkpar[2];
   double bforce[3], velf[3];
   int is_forced[3];
+  int read_ro = 1, write_ro = 1;  // RHS_old input/output needed (see launch_mom_rk)
 };
 
-// deterministic block reductions for 256-thread blocks
+// XCD-aware tile order: the hardware hands work-group b to XCD b % 8; remap so that
+// each XCD walks a contiguous range of tiles (neighbouring tiles then share its L2)
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  const int x = b & 7, q = b >> 3;
+  const int base = nb >> 3, rem = nb & 7;
+  const int start = x * base + (x < rem ? x : rem);
+  return start + q;
+}
+
+// deterministic block reductions for NT-thread blocks
+template <int NT = 256>
 __device__ __forceinline__ void block_sum3(double a, double b, double c, double* oa, double* ob, double* oc,
                                            bool only_a = false) {
-  __shared__ double r[3][256];
+  __shared__ double r[3][NT];
   const int t = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
   r[0][t] = a;
   r[1][t] = b;
   r[2][t] = c;
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
+  for (int s = NT / 2; s > 0; s >>= 1) {
     if (t < s) {
       r[0][t] += r[0][t + s];
       if (!only_a) {
@@ -97,7 +108,8 @@ int launch_add_forcing(cans_ctx* c, const double* src, double* dst, int comp, in
 int launch_fillps(cans_ctx* c, const double* u, const double* v, const double* w, double* p, const double* dzfi,
                   double dti, const double* dli);
 int launch_correc(cans_ctx* c, const double* p, const double* su, const double* sv, const double* sw, double* du,
-                  double* dv, double* dw, const double* dzci, double dt, const double* dli);
+                  double* dv, double* dw, const double* dzci, double dt, const double* dli,
+                  double* p_update = nullptr);
 int launch_updatep(cans_ctx* c, const double* pp, double* p, const double* dzci, const double* dzfi, double alpha,
                    const double* dli, int impdiff);
 int launch_bound_dir(cans_ctx* c, const PlaneOps& ops);
@@ -109,7 +121,14 @@ int run_chkdt(cans_ctx* c, const double* u, const double* v, const double* w, co
 int run_bulk_mean(cans_ctx* c, const double* q, const double* gvr, double* mean);
 
 // solver.hip
+// right-hand side of the pressure equation computed from the velocity inside the
+// first transform pass (fillps.f90:38-50); p then receives only the solution
+struct DivSrc {
+  const double *u, *v, *w, *dzfi;
+  double dti, dtidxi, dtidyi;
+};
 int run_solver(cans_ctx* c, const cans_plan* plan, double normfft, const double* lam, const double* a,
-               const double* b, const double* cc, const char* bc, const char* c_or_f, double* p);
+               const double* b, const double* cc, const char* bc, const char* c_or_f, double* p,
+               const DivSrc* div = nullptr);
 
 }  // namespace cans

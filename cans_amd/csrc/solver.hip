@@ -9,6 +9,7 @@
 // stored packed as (r0, r_{n/2}, r1, i1, ..., r_{n/2-1}, i_{n/2-1}) (fft.f90:402-451,
 // initsolver.f90:98-117); DCT/DST directions in natural order.
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <vector>
 
@@ -42,156 +43,49 @@ static int ilog2(int n) {
 __device__ __forceinline__ int mk_pos(int e, int N) { return (e & 1) ? N - 1 - (e >> 1) : (e >> 1); }
 
 // ---------------------------------------------------------------------------
-// forward: real lines A, B -> complex line z in LDS (pre-processing)
+// Pre/post-processing around the complex FFT.  Two real lines A, B travel as
+// one complex line z = A + iB:
+//   R2HC   : A_k = (Z_k + conj Z_{N-k})/2, B_k = -i (Z_k - conj Z_{N-k})/2,
+//            stored packed (r0, r_{N/2}, r1, i1, ...);
+//   REDFT10: Makhoul's reorder v = (x0, x2, ..., x3, x1), then
+//            Y_k = Re(M_k (Z_k + conj Z_{N-k})) with M_k = exp(-i pi k/(2N));
+//   RODFT10: REDFT10 of (-1)^j x_j, output reversed;
+//   HC2R, REDFT01, RODFT01: the inverses (unnormalised, as FFTW).
+// Global accesses move two doubles per lane (16 bytes).
 // ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ void fwd_pre(cplx<T>* buf, int S, int N, int lt, int Tn, int kind, const T* xa,
-                                        const T* xb, long est) {
-  for (int e = lt; e < N; e += Tn) {
-    T a = xa[e * est];
-    T b = xb ? xb[e * est] : (T)0;
-    if (kind == K_PP) {
-      buf[e * S] = {a, b};
-    } else {
-      if (kind == K_DD && (e & 1)) {  // RODFT10 = REDFT10 of (-1)^j x_j, reversed output
-        a = -a;
-        b = -b;
-      }
-      buf[mk_pos(e, N) * S] = {a, b};
-    }
-  }
-}
+typedef HIP_vector_type<double, 2> d2;
 
-// forward post-processing: complex spectrum Z of (A + iB) -> spectra of A and B.
-// Output element index o of line A is written to ya[o*est] (B likewise).
-template <typename T>
-__device__ __forceinline__ void fwd_post(const cplx<T>* buf, int S, int N, int lt, int Tn, int kind,
-                                         const cplx<T>* __restrict__ mk, T* ya, T* yb, long est) {
-  if (kind == K_PP) {
-    // A_k = (Z_k + conj Z_{N-k})/2, B_k = -i (Z_k - conj Z_{N-k})/2; packed output
-    const int H = N >> 1;
-    if (N == 1) {
-      if (lt == 0) {
-        ya[0] = buf[0].re;
-        if (yb) yb[0] = buf[0].im;
-      }
-      return;
-    }
-    for (int k = lt; k < H; k += Tn) {
-      if (k == 0) {
-        const cplx<T> z0 = buf[0], zh = buf[H * S];
-        ya[0] = z0.re;
-        ya[est] = zh.re;
-        if (yb) {
-          yb[0] = z0.im;
-          yb[est] = zh.im;
-        }
-      } else {
-        const cplx<T> zk = buf[k * S], zc = buf[(N - k) * S];
-        const T are = (T)0.5 * (zk.re + zc.re), aim = (T)0.5 * (zk.im - zc.im);
-        const T bre = (T)0.5 * (zk.im + zc.im), bim = -(T)0.5 * (zk.re - zc.re);
-        ya[(2 * k) * est] = are;
-        ya[(2 * k + 1) * est] = aim;
-        if (yb) {
-          yb[(2 * k) * est] = bre;
-          yb[(2 * k + 1) * est] = bim;
-        }
-      }
-    }
-    return;
-  }
-  // REDFT10: Y_k = 2 Re(M_k V_k), V_A = (Z_k + conj Z_{N-k})/2 -> Y_k = Re(M_k (Z_k + conj Z_{N-k}))
-  for (int k = lt; k < N; k += Tn) {
-    const cplx<T> zk = buf[k * S];
-    const cplx<T> zc = buf[((N - k) & (N - 1)) * S];
-    const cplx<T> m = mk[k];
-    // A: Z_k + conj Z_{N-k};  B: -i (Z_k - conj Z_{N-k})
-    const T sre = zk.re + zc.re, sim = zk.im - zc.im;
-    const T dre = zk.im + zc.im, dim = -(zk.re - zc.re);
-    const T ya_ = m.re * sre - m.im * sim;
-    const T yb_ = m.re * dre - m.im * dim;
-    const int o = kind == K_DD ? N - 1 - k : k;
-    ya[o * est] = ya_;
-    if (yb) yb[o * est] = yb_;
-  }
+__device__ __forceinline__ d2 ld2(const double* p) { return *(const d2*)p; }
+__device__ __forceinline__ void st2(double* p, double x, double y) { *(d2*)p = d2(x, y); }
+
+// spectra of A and B at index k from the complex transform of A + iB
+__device__ __forceinline__ void split_pp(cplx<double> zk, cplx<double> zc, double& are, double& aim, double& bre,
+                                         double& bim) {
+  are = 0.5 * (zk.re + zc.re);
+  aim = 0.5 * (zk.im - zc.im);
+  bre = 0.5 * (zk.im + zc.im);
+  bim = -0.5 * (zk.re - zc.re);
+}
+__device__ __forceinline__ void split_dct(cplx<double> zk, cplx<double> zc, cplx<double> m, double& ya, double& yb) {
+  const double sre = zk.re + zc.re, sim = zk.im - zc.im;
+  const double dre = zk.im + zc.im, dim = -(zk.re - zc.re);
+  ya = m.re * sre - m.im * sim;
+  yb = m.re * dre - m.im * dim;
+}
+// conj(M_k) (X_k - i X_{N-k}) for lines A and B, combined as V_A + i V_B
+__device__ __forceinline__ cplx<double> merge_dct(cplx<double> m, double xka, double xnka, double xkb, double xnkb) {
+  const double va_re = m.re * xka - m.im * xnka, va_im = -m.re * xnka - m.im * xka;
+  const double vb_re = m.re * xkb - m.im * xnkb, vb_im = -m.re * xnkb - m.im * xkb;
+  return {va_re - vb_im, va_im + vb_re};
+}
+// Z = A + iB from packed halfcomplex A_k, B_k (A_{N-k} = conj A_k)
+__device__ __forceinline__ cplx<double> merge_pp(double are, double aim, double bre, double bim) {
+  return {are - bim, aim + bre};
 }
 
 // ---------------------------------------------------------------------------
-// backward: spectra of A, B -> complex Z in LDS (pre-processing)
-// ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ void bwd_pre(cplx<T>* buf, int S, int N, int lt, int Tn, int kind,
-                                        const cplx<T>* __restrict__ mk, const T* xa, const T* xb, long est) {
-  if (kind == K_PP) {
-    const int H = N >> 1;
-    for (int k = lt; k < N; k += Tn) {
-      // A_k from the packed halfcomplex line, with A_{N-k} = conj(A_k)
-      T are, aim, bre = 0, bim = 0;
-      if (N == 1 || k == 0) {
-        are = xa[0];
-        aim = 0;
-        if (xb) bre = xb[0];
-      } else if (k == H) {
-        are = xa[est];
-        aim = 0;
-        if (xb) bre = xb[est];
-      } else {
-        const int kk = k < H ? k : N - k;
-        const T sg = k < H ? (T)1 : (T)-1;
-        are = xa[(2 * kk) * est];
-        aim = sg * xa[(2 * kk + 1) * est];
-        if (xb) {
-          bre = xb[(2 * kk) * est];
-          bim = sg * xb[(2 * kk + 1) * est];
-        }
-      }
-      buf[k * S] = {are - bim, aim + bre};  // Z = A + iB
-    }
-    return;
-  }
-  // REDFT01: V_k = conj(M_k) (X_k - i X_{N-k}), X_N = 0; RODFT01 reads the reversed input
-  for (int k = lt; k < N; k += Tn) {
-    T xk_a, xnk_a, xk_b = 0, xnk_b = 0;
-    if (kind == K_DD) {
-      xk_a = xa[(N - 1 - k) * est];
-      xnk_a = k ? xa[(k - 1) * est] : (T)0;
-      if (xb) {
-        xk_b = xb[(N - 1 - k) * est];
-        xnk_b = k ? xb[(k - 1) * est] : (T)0;
-      }
-    } else {
-      xk_a = xa[k * est];
-      xnk_a = k ? xa[(N - k) * est] : (T)0;
-      if (xb) {
-        xk_b = xb[k * est];
-        xnk_b = k ? xb[(N - k) * est] : (T)0;
-      }
-    }
-    const cplx<T> m = mk[k];
-    // conj(m) * (x, -y) = (m.re x - m.im y, -m.re y - m.im x)
-    const T va_re = m.re * xk_a - m.im * xnk_a, va_im = -m.re * xnk_a - m.im * xk_a;
-    const T vb_re = m.re * xk_b - m.im * xnk_b, vb_im = -m.re * xnk_b - m.im * xk_b;
-    buf[k * S] = {va_re - vb_im, va_im + vb_re};
-  }
-}
-
-template <typename T>
-__device__ __forceinline__ void bwd_post(const cplx<T>* buf, int S, int N, int lt, int Tn, int kind, T* ya, T* yb,
-                                         long est) {
-  for (int e = lt; e < N; e += Tn) {
-    cplx<T> z = kind == K_PP ? buf[e * S] : buf[mk_pos(e, N) * S];
-    if (kind == K_DD && (e & 1)) {
-      z.re = -z.re;
-      z.im = -z.im;
-    }
-    ya[e * est] = z.re;
-    if (yb) yb[e * est] = z.im;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// pass X: rows of the halo'd field (or the work buffer) along x.
-// rows r = j + n2*k (0-based); a complex line carries rows 2q and 2q+1.
+// pass X: rows of the halo'd field and of the work buffer are contiguous along x.
+// rows r = j + n2*k (0-based); the complex line of team q carries rows 2q, 2q+1.
 // ---------------------------------------------------------------------------
 struct XArgs {
   Layout L;
@@ -200,37 +94,172 @@ struct XArgs {
   int N, logN, Tn, pairs, kind;
   const cplx<double>* tw;
   const cplx<double>* mk;
+  int div;         // forward only: build the rhs from the velocity instead of reading fld
+  DivSrc ds;
 };
+
+// two consecutive rhs values (elements e, e+1 of row r) of fillps.f90:38-50
+__device__ __forceinline__ d2 div_pair(const XArgs& a, long r, int e) {
+  const Layout& L = a.L;
+  const int j = (int)(r % L.n2) + 1, k = (int)(r / L.n2) + 1;
+  const long c = L.at(1, j, k) + e;
+  const d2 u2 = ld2(a.ds.u + c), w2 = ld2(a.ds.w + c), wm = ld2(a.ds.w + c - L.sz);
+  const d2 v2 = ld2(a.ds.v + c), vm = ld2(a.ds.v + c - L.sy);
+  const double um = a.ds.u[c - 1];
+  const double dti = a.ds.dti, dzfi = a.ds.dzfi[k], dy = a.ds.dtidyi, dx = a.ds.dtidxi;
+  const double p0 = ((w2.x - wm.x) * dti * dzfi + (v2.x - vm.x) * dy + (u2.x - um) * dx);
+  const double p1 = ((w2.y - wm.y) * dti * dzfi + (v2.y - vm.y) * dy + (u2.y - u2.x) * dx);
+  return d2(p0, p1);
+}
 
 template <int DIR>  // +1 forward (fld -> W), -1 backward (W -> fld)
 __global__ void k_xpass(XArgs a) {
   extern __shared__ unsigned char smem_raw[];
   cplx<double>* smem = (cplx<double>*)smem_raw;
-  const int team = threadIdx.x / a.Tn, lt = threadIdx.x % a.Tn;
+  const int N = a.N, Tn = a.Tn, kind = a.kind;
+  const int team = threadIdx.x / Tn, lt = threadIdx.x % Tn;
   const long nrows = (long)a.L.n2 * a.L.n3;
   const long q = (long)blockIdx.x * a.pairs + team;
   const long ra = 2 * q, rb = 2 * q + 1;
   const bool va = ra < nrows, vb = rb < nrows;
-  cplx<double>* buf = smem + (long)team * a.N;
+  const IxPad ix{team * padded_len(N)};
   const Layout& L = a.L;
   auto frow = [&](long r) { return a.fld + L.at(1, (int)(r % L.n2) + 1, (int)(r / L.n2) + 1); };
   auto wrow = [&](long r) { return a.W + r * L.n1; };
+  const double* xa = DIR > 0 ? (va ? frow(ra) : nullptr) : (va ? wrow(ra) : nullptr);
+  const double* xb = DIR > 0 ? (vb ? frow(rb) : nullptr) : (vb ? wrow(rb) : nullptr);
+  double* ya = DIR > 0 ? (va ? wrow(ra) : nullptr) : (va ? frow(ra) : nullptr);
+  double* yb = DIR > 0 ? (vb ? wrow(rb) : nullptr) : (vb ? frow(rb) : nullptr);
   if (DIR > 0) {
-    if (va) fwd_pre<double>(buf, 1, a.N, lt, a.Tn, a.kind, frow(ra), vb ? frow(rb) : nullptr, 1);
+    if (va) {
+      if (N == 1) {
+        if (lt == 0) smem[ix(0)] = {xa[0], xb ? xb[0] : 0.};
+      } else {
+        for (int e = 2 * lt; e < N; e += 2 * Tn) {
+          d2 a2, b2;
+          if (a.div) {
+            a2 = div_pair(a, ra, e);
+            b2 = xb ? div_pair(a, rb, e) : d2(0., 0.);
+          } else {
+            a2 = ld2(xa + e);
+            b2 = xb ? ld2(xb + e) : d2(0., 0.);
+          }
+          if (kind == K_DD) {  // (-1)^j on the odd element
+            a2.y = -a2.y;
+            b2.y = -b2.y;
+          }
+          const int p0 = kind == K_PP ? e : mk_pos(e, N), p1 = kind == K_PP ? e + 1 : mk_pos(e + 1, N);
+          smem[ix(p0)] = {a2.x, b2.x};
+          smem[ix(p1)] = {a2.y, b2.y};
+        }
+      }
+    }
     __syncthreads();
-    fft_line<double, -1>(buf, 1, a.N, a.logN, lt, a.Tn, a.tw);
-    if (va) fwd_post<double>(buf, 1, a.N, lt, a.Tn, a.kind, a.mk, wrow(ra), vb ? wrow(rb) : nullptr, 1);
+    fft_line<double, -1>(smem, ix, N, a.logN, lt, Tn, a.tw);
+    if (!va) return;
+    if (kind == K_PP) {
+      const int H = N >> 1;
+      if (N == 1) {
+        if (lt == 0) {
+          ya[0] = smem[ix(0)].re;
+          if (yb) yb[0] = smem[ix(0)].im;
+        }
+        return;
+      }
+      for (int k = lt; k < H; k += Tn) {
+        double are, aim, bre, bim;
+        if (k == 0) {
+          const cplx<double> z0 = smem[ix(0)], zh = smem[ix(H)];
+          are = z0.re;
+          aim = zh.re;
+          bre = z0.im;
+          bim = zh.im;
+        } else {
+          const cplx<double> zk = smem[ix(k)], zc = smem[ix(N - k)];
+          split_pp(zk, zc, are, aim, bre, bim);
+        }
+        st2(ya + 2 * k, are, aim);
+        if (yb) st2(yb + 2 * k, bre, bim);
+      }
+    } else {
+      for (int k = 2 * lt; k < N; k += 2 * Tn) {
+        double y0a, y0b, y1a, y1b;
+        split_dct(smem[ix(k)], smem[ix((N - k) & (N - 1))], a.mk[k], y0a, y0b);
+        split_dct(smem[ix(k + 1)], smem[ix(N - k - 1)], a.mk[k + 1], y1a, y1b);
+        if (kind == K_DD) {  // output reversed: element N-1-k
+          st2(ya + N - 2 - k, y1a, y0a);
+          if (yb) st2(yb + N - 2 - k, y1b, y0b);
+        } else {
+          st2(ya + k, y0a, y1a);
+          if (yb) st2(yb + k, y0b, y1b);
+        }
+      }
+    }
   } else {
-    if (va) bwd_pre<double>(buf, 1, a.N, lt, a.Tn, a.kind, a.mk, wrow(ra), vb ? wrow(rb) : nullptr, 1);
+    if (va) {
+      if (kind == K_PP) {
+        const int H = N >> 1;
+        for (int k = lt; k < N; k += Tn) {
+          double are, aim, bre = 0., bim = 0.;
+          if (N == 1 || k == 0) {
+            are = xa[0];
+            aim = 0.;
+            if (xb) bre = xb[0];
+          } else if (k == H) {
+            are = xa[1];
+            aim = 0.;
+            if (xb) bre = xb[1];
+          } else {
+            const int kk = k < H ? k : N - k;
+            const double sg = k < H ? 1. : -1.;
+            const d2 a2 = ld2(xa + 2 * kk);
+            are = a2.x;
+            aim = sg * a2.y;
+            if (xb) {
+              const d2 b2 = ld2(xb + 2 * kk);
+              bre = b2.x;
+              bim = sg * b2.y;
+            }
+          }
+          smem[ix(k)] = merge_pp(are, aim, bre, bim);
+        }
+      } else {
+        for (int k = lt; k < N; k += Tn) {
+          const bool dd = kind == K_DD;  // RODFT01 reads the reversed input
+          const int ik = dd ? N - 1 - k : k, ink = dd ? k - 1 : N - k;
+          const double xka = xa[ik], xnka = k ? xa[ink] : 0.;
+          const double xkb = xb ? xb[ik] : 0., xnkb = (xb && k) ? xb[ink] : 0.;
+          smem[ix(k)] = merge_dct(a.mk[k], xka, xnka, xkb, xnkb);
+        }
+      }
+    }
     __syncthreads();
-    fft_line<double, +1>(buf, 1, a.N, a.logN, lt, a.Tn, a.tw);
-    if (va) bwd_post<double>(buf, 1, a.N, lt, a.Tn, a.kind, frow(ra), vb ? frow(rb) : nullptr, 1);
+    fft_line<double, +1>(smem, ix, N, a.logN, lt, Tn, a.tw);
+    if (!va) return;
+    if (N == 1) {
+      if (lt == 0) {
+        ya[0] = smem[ix(0)].re;
+        if (yb) yb[0] = smem[ix(0)].im;
+      }
+      return;
+    }
+    for (int e = 2 * lt; e < N; e += 2 * Tn) {
+      cplx<double> z0 = smem[ix(kind == K_PP ? e : mk_pos(e, N))];
+      cplx<double> z1 = smem[ix(kind == K_PP ? e + 1 : mk_pos(e + 1, N))];
+      if (kind == K_DD) {  // (-1)^k on the odd output
+        z1.re = -z1.re;
+        z1.im = -z1.im;
+      }
+      st2(ya + e, z0.re, z1.re);
+      if (yb) st2(yb + e, z0.im, z1.im);
+    }
   }
 }
 
 // ---------------------------------------------------------------------------
 // pass Y: in place on W; a work-group owns 2*LINES adjacent x-columns of one z-plane.
 // thread t -> line c = t % LINES, team thread lt = t / LINES; LDS element (e, c) at [e*LINES + c].
+// The two columns of a line are adjacent, so every global access is one 16-byte pair.
 // ---------------------------------------------------------------------------
 struct YArgs {
   int n1, n2, n3;
@@ -244,26 +273,108 @@ template <int DIR>
 __global__ void k_ypass(YArgs a) {
   extern __shared__ unsigned char smem_raw[];
   cplx<double>* smem = (cplx<double>*)smem_raw;
-  const int LN = a.lines;
+  const int LN = a.lines, N = a.N, Tn = a.Tn, kind = a.kind;
   const int c = threadIdx.x % LN, lt = threadIdx.x / LN;
   const int ncolblk = (a.n1 + 2 * LN - 1) / (2 * LN);
   const int k = blockIdx.x / ncolblk;
   const int i0 = (blockIdx.x % ncolblk) * 2 * LN;
-  const int ia = i0 + 2 * c, ib = ia + 1;
-  const bool va = ia < a.n1, vb = ib < a.n1;
-  double* plane = a.W + (long)k * a.n1 * a.n2;
+  const int ia = i0 + 2 * c;
+  const bool va = ia < a.n1, vb = ia + 1 < a.n1;
+  double* col = a.W + (long)k * a.n1 * a.n2 + ia;
   const long est = a.n1;  // element stride along y
-  cplx<double>* buf = smem + c;
+  const IxStride ix{c, LN};
+  // (A, B) pair of element e of this line
+  auto ld = [&](int e) -> d2 { return vb ? ld2(col + e * est) : d2(col[e * est], 0.); };
+  auto st = [&](int e, double x, double y) {
+    if (vb) st2(col + e * est, x, y);
+    else col[e * est] = x;
+  };
   if (DIR > 0) {
-    if (va) fwd_pre<double>(buf, LN, a.N, lt, a.Tn, a.kind, plane + ia, vb ? plane + ib : nullptr, est);
+    if (va) {
+      for (int e = lt; e < N; e += Tn) {
+        d2 v = ld(e);
+        if (kind == K_DD && (e & 1)) {
+          v.x = -v.x;
+          v.y = -v.y;
+        }
+        smem[ix(kind == K_PP ? e : mk_pos(e, N))] = {v.x, v.y};
+      }
+    }
     __syncthreads();
-    fft_line<double, -1>(buf, LN, a.N, a.logN, lt, a.Tn, a.tw);
-    if (va) fwd_post<double>(buf, LN, a.N, lt, a.Tn, a.kind, a.mk, plane + ia, vb ? plane + ib : nullptr, est);
+    fft_line<double, -1>(smem, ix, N, a.logN, lt, Tn, a.tw);
+    if (!va) return;
+    if (kind == K_PP) {
+      const int H = N >> 1;
+      if (N == 1) {
+        if (lt == 0) st(0, smem[ix(0)].re, smem[ix(0)].im);
+        return;
+      }
+      for (int kk = lt; kk < H; kk += Tn) {
+        double are, aim, bre, bim;
+        if (kk == 0) {
+          const cplx<double> z0 = smem[ix(0)], zh = smem[ix(H)];
+          are = z0.re;
+          aim = zh.re;
+          bre = z0.im;
+          bim = zh.im;
+        } else {
+          split_pp(smem[ix(kk)], smem[ix(N - kk)], are, aim, bre, bim);
+        }
+        st(2 * kk, are, bre);
+        st(2 * kk + 1, aim, bim);
+      }
+    } else {
+      for (int kk = lt; kk < N; kk += Tn) {
+        double ya, yb;
+        split_dct(smem[ix(kk)], smem[ix((N - kk) & (N - 1))], a.mk[kk], ya, yb);
+        st(kind == K_DD ? N - 1 - kk : kk, ya, yb);
+      }
+    }
   } else {
-    if (va) bwd_pre<double>(buf, LN, a.N, lt, a.Tn, a.kind, a.mk, plane + ia, vb ? plane + ib : nullptr, est);
+    if (va) {
+      if (kind == K_PP) {
+        const int H = N >> 1;
+        for (int kk = lt; kk < N; kk += Tn) {
+          double are, aim = 0., bre, bim = 0.;
+          if (N == 1 || kk == 0) {
+            const d2 v = ld(0);
+            are = v.x;
+            bre = v.y;
+          } else if (kk == H) {
+            const d2 v = ld(1);
+            are = v.x;
+            bre = v.y;
+          } else {
+            const int m = kk < H ? kk : N - kk;
+            const double sg = kk < H ? 1. : -1.;
+            const d2 re = ld(2 * m), im = ld(2 * m + 1);
+            are = re.x;
+            bre = re.y;
+            aim = sg * im.x;
+            bim = sg * im.y;
+          }
+          smem[ix(kk)] = merge_pp(are, aim, bre, bim);
+        }
+      } else {
+        for (int kk = lt; kk < N; kk += Tn) {
+          const bool dd = kind == K_DD;
+          const d2 xk = ld(dd ? N - 1 - kk : kk);
+          const d2 xnk = kk ? ld(dd ? kk - 1 : N - kk) : d2(0., 0.);
+          smem[ix(kk)] = merge_dct(a.mk[kk], xk.x, xnk.x, xk.y, xnk.y);
+        }
+      }
+    }
     __syncthreads();
-    fft_line<double, +1>(buf, LN, a.N, a.logN, lt, a.Tn, a.tw);
-    if (va) bwd_post<double>(buf, LN, a.N, lt, a.Tn, a.kind, plane + ia, vb ? plane + ib : nullptr, est);
+    fft_line<double, +1>(smem, ix, N, a.logN, lt, Tn, a.tw);
+    if (!va) return;
+    for (int e = lt; e < N; e += Tn) {
+      cplx<double> z = smem[ix(kind == K_PP ? e : mk_pos(e, N))];
+      if (kind == K_DD && (e & 1)) {
+        z.re = -z.re;
+        z.im = -z.im;
+      }
+      st(e, z.re, z.im);
+    }
   }
 }
 
@@ -288,125 +399,123 @@ struct ZArgs {
   double* X2;               // periodic: auxiliary solution p2 [nn][nl]
 };
 
+// one forward elimination row (solver.f90:147-170, with the p2 recurrence of 235-245)
+struct ZRow {
+  double d, pp, q2;
+};
+
+__device__ __forceinline__ void z_row(int k, int nn, int periodic, double l, double r, double norm, const double* a,
+                                      const double* b, const double* c, ZRow& s) {
+  const double eps = 2.220446049250313e-16;
+  if (k == 1) {
+    const double zz = 1. / (b[0] + l);
+    s.d = c[0] * zz;
+    s.pp = r * norm * zz;
+    if (periodic) s.q2 = ((nn == 1) ? (-a[0] - c[nn - 1]) : -a[0]) * zz;
+    return;
+  }
+  const double bl = b[k - 1] + l;
+  const double ad = a[k - 1] * s.d;
+  const double den = bl - ad;
+  if (periodic) {
+    const double zq = 1. / den;
+    const double rq = (k == nn) ? -c[nn - 1] : 0.;
+    s.q2 = (rq - a[k - 1] * s.q2) * zq;
+  }
+  const double tol = eps * fmax(fabs(bl), fabs(ad));
+  if (k == nn && fabs(den) <= tol) {
+    s.d = 0.;
+    s.pp = 0.;
+  } else {
+    const double zz = 1. / den;
+    s.d = c[k - 1] * zz;
+    s.pp = (r * norm - a[k - 1] * s.pp) * zz;
+  }
+}
+
 __global__ void __launch_bounds__(256) k_gaussel(ZArgs z) {
   const long line = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (line >= z.nl) return;
-  const double eps = 2.220446049250313e-16;
   const double l = z.lam[line];
   double* P = z.W + line;
   const long ps = z.nl;
-  const int nn = z.periodic ? z.n - 1 : z.n;
+  const int periodic = z.periodic;
+  const int nn = periodic ? z.n - 1 : z.n;
   const int nblk = (nn + CK - 1) / CK;
   const double* a = z.a;
   const double* b = z.b;
   const double* c = z.c;
   const double norm = z.norm;
-  // ---- forward sweep: checkpoints only ----
-  double d, pp, q2 = 0.;
-  {
-    const double zz = 1. / (b[0] + l);
-    d = c[0] * zz;
-    pp = P[0] * norm * zz;
-    if (z.periodic) q2 = ((nn == 1) ? (-a[0] - c[nn - 1]) : -a[0]) * zz;
-  }
-  if (CK == 1 || nn == 1) {
-    z.ckd[line] = d;
-    z.ckp[line] = pp;
-    if (z.periodic) z.ckq[line] = q2;
-  }
-  for (int k = 2; k <= nn; k++) {
-    const double r = P[(long)(k - 1) * ps];
-    const double bl = b[k - 1] + l;
-    const double ad = a[k - 1] * d;
-    const double den = bl - ad;
-    if (z.periodic) {
-      const double zq = 1. / den;
-      const double rq = (k == nn) ? -c[nn - 1] : 0.;
-      q2 = (rq - a[k - 1] * q2) * zq;
+  // ---- forward sweep (checkpoints only), next chunk's loads issued before this chunk's recurrence ----
+  ZRow s = {0., 0., 0.};
+  double rr[CK], rn[CK];
+#pragma unroll
+  for (int t = 0; t < CK; t++) rr[t] = (t + 1 <= nn) ? P[(long)t * ps] : 0.;
+  for (int bb = 0; bb < nblk; bb++) {
+    const int k0 = bb * CK + 1;
+#pragma unroll
+    for (int t = 0; t < CK; t++) {
+      const int k = k0 + CK + t;
+      rn[t] = (k <= nn) ? P[(long)(k - 1) * ps] : 0.;
     }
-    const double tol = eps * fmax(fabs(bl), fabs(ad));
-    if (k == nn && fabs(den) <= tol) {
-      d = 0.;
-      pp = 0.;
-    } else {
-      const double zz = 1. / den;
-      d = c[k - 1] * zz;
-      pp = (r * norm - a[k - 1] * pp) * zz;
-    }
-    if ((k % CK) == 0 || k == nn) {
-      const long o = (long)((k - 1) / CK) * ps + line;
-      z.ckd[o] = d;
-      z.ckp[o] = pp;
-      if (z.periodic) z.ckq[o] = q2;
-    }
+#pragma unroll
+    for (int t = 0; t < CK; t++)
+      if (k0 + t <= nn) z_row(k0 + t, nn, periodic, l, rr[t], norm, a, b, c, s);
+    const long o = (long)bb * ps + line;
+    z.ckd[o] = s.d;
+    z.ckp[o] = s.pp;
+    if (periodic) z.ckq[o] = s.q2;
+#pragma unroll
+    for (int t = 0; t < CK; t++) rr[t] = rn[t];
   }
-  // ---- backward sweep, one block of CK rows at a time ----
+  // ---- backward sweep: previous block's rhs prefetched while this block is recomputed and substituted ----
   double xn = 0., xq = 0.;
   double x1 = 0., xq1 = 0., xnn = 0., xqnn = 0.;
+  {
+    const int k0 = (nblk - 1) * CK + 1;
+#pragma unroll
+    for (int t = 0; t < CK; t++) rr[t] = (k0 + t <= nn) ? P[(long)(k0 + t - 1) * ps] : 0.;
+  }
   for (int bb = nblk - 1; bb >= 0; bb--) {
     const int k0 = bb * CK + 1;
     const int k1 = min(k0 + CK - 1, nn);
-    double rr[CK], dd[CK], pv[CK], qv[CK];
+    if (bb > 0) {
 #pragma unroll
-    for (int s = 0; s < CK; s++) rr[s] = (k0 + s <= k1) ? P[(long)(k0 + s - 1) * ps] : 0.;
-    double dp, pq, qq;
+      for (int t = 0; t < CK; t++) rn[t] = P[(long)(k0 - CK + t - 1) * ps];
+    }
+    ZRow r0 = {0., 0., 0.};
     if (bb > 0) {
       const long o = (long)(bb - 1) * ps + line;
-      dp = z.ckd[o];
-      pq = z.ckp[o];
-      qq = z.periodic ? z.ckq[o] : 0.;
-    } else {
-      dp = pq = qq = 0.;
+      r0.d = z.ckd[o];
+      r0.pp = z.ckp[o];
+      r0.q2 = periodic ? z.ckq[o] : 0.;
     }
+    double dd[CK], pv[CK], qv[CK];
 #pragma unroll
-    for (int s = 0; s < CK; s++) {
-      const int k = k0 + s;
-      if (k <= k1) {
-        if (k == 1) {
-          const double zz = 1. / (b[0] + l);
-          dp = c[0] * zz;
-          pq = rr[s] * norm * zz;
-          if (z.periodic) qq = ((nn == 1) ? (-a[0] - c[nn - 1]) : -a[0]) * zz;
-        } else {
-          const double bl = b[k - 1] + l;
-          const double ad = a[k - 1] * dp;
-          const double den = bl - ad;
-          if (z.periodic) {
-            const double zq = 1. / den;
-            const double rq = (k == nn) ? -c[nn - 1] : 0.;
-            qq = (rq - a[k - 1] * qq) * zq;
-          }
-          const double tol = eps * fmax(fabs(bl), fabs(ad));
-          if (k == nn && fabs(den) <= tol) {
-            dp = 0.;
-            pq = 0.;
-          } else {
-            const double zz = 1. / den;
-            dp = c[k - 1] * zz;
-            pq = (rr[s] * norm - a[k - 1] * pq) * zz;
-          }
-        }
-        dd[s] = dp;
-        pv[s] = pq;
-        qv[s] = qq;
+    for (int t = 0; t < CK; t++) {
+      if (k0 + t <= k1) {
+        z_row(k0 + t, nn, periodic, l, rr[t], norm, a, b, c, r0);
+        dd[t] = r0.d;
+        pv[t] = r0.pp;
+        qv[t] = r0.q2;
       }
     }
 #pragma unroll
-    for (int s = CK - 1; s >= 0; s--) {
-      const int k = k0 + s;
+    for (int t = CK - 1; t >= 0; t--) {
+      const int k = k0 + t;
       if (k <= k1) {
         double x, x2 = 0.;
         if (k == nn) {
-          x = pv[s];
-          x2 = qv[s];
+          x = pv[t];
+          x2 = qv[t];
           xnn = x;
           xqnn = x2;
         } else {
-          x = pv[s] - dd[s] * xn;
-          if (z.periodic) x2 = qv[s] - dd[s] * xq;
+          x = pv[t] - dd[t] * xn;
+          if (periodic) x2 = qv[t] - dd[t] * xq;
         }
         P[(long)(k - 1) * ps] = x;
-        if (z.periodic) z.X2[(long)(k - 1) * ps + line] = x2;
+        if (periodic) z.X2[(long)(k - 1) * ps + line] = x2;
         xn = x;
         xq = x2;
         if (k == 1) {
@@ -415,9 +524,12 @@ __global__ void __launch_bounds__(256) k_gaussel(ZArgs z) {
         }
       }
     }
+#pragma unroll
+    for (int t = 0; t < CK; t++) rr[t] = rn[t];
   }
-  if (!z.periodic) return;
+  if (!periodic) return;
   // ---- periodic closure (solver.f90:268-296) and Sherman-Morrison correction ----
+  const double eps = 2.220446049250313e-16;
   const int m = nn + 1;  // closure row, 1-based
   const double den = b[m - 1] + l + c[m - 1] * xq1 + a[m - 1] * xqnn;
   const double tol = eps * fmax(fabs(b[m - 1] + l), fabs(c[m - 1] * xq1 + a[m - 1] * xqnn));
@@ -457,8 +569,10 @@ static int upload_tables(cans_plan* p, int d) {
   return CANS_OK;
 }
 
-static int xpass(cans_ctx* c, const cans_plan* p, double* fld, double* W, int dir) {
+static int xpass(cans_ctx* c, const cans_plan* p, double* fld, double* W, int dir, const DivSrc* div = nullptr) {
   XArgs a;
+  a.div = div != nullptr && dir > 0;
+  if (a.div) a.ds = *div;
   a.L = c->L;
   a.fld = fld;
   a.W = W;
@@ -472,7 +586,7 @@ static int xpass(cans_ctx* c, const cans_plan* p, double* fld, double* W, int di
   const long nrows = (long)c->L.n2 * c->L.n3;
   const long npairs = (nrows + 1) / 2;
   dim3 grid(cdiv(npairs, a.pairs));
-  const size_t shm = (size_t)a.pairs * a.N * sizeof(cplx<double>);
+  const size_t shm = (size_t)a.pairs * padded_len(a.N) * sizeof(cplx<double>);
   if (dir > 0) k_xpass<1><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
   else k_xpass<-1><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
   CANS_LAUNCH_CHECK("k_xpass");
@@ -506,7 +620,8 @@ static int ypass(cans_ctx* c, const cans_plan* p, double* W, int dir) {
 }
 
 int run_solver(cans_ctx* c, const cans_plan* p, double normfft, const double* lam, const double* a,
-               const double* b, const double* cc, const char* bc, const char* c_or_f, double* fld) {
+               const double* b, const double* cc, const char* bc, const char* c_or_f, double* fld,
+               const DivSrc* div) {
   const Layout& L = c->L;
   const long N = L.cells();
   const long nl = (long)L.n1 * L.n2;
@@ -523,7 +638,13 @@ int run_solver(cans_ctx* c, const cans_plan* p, double normfft, const double* la
   size_t ckbytes = (size_t)3 * nblk * nl * sizeof(double) + (periodic ? (size_t)N * sizeof(double) : 0);
   rc = ensure_work(c, 2, ckbytes, &ck);
   if (rc) return rc;
-  if ((rc = xpass(c, p, fld, (double*)W, +1))) return rc;
+  // the transform passes move element pairs: interior rows must start on 16-byte boundaries
+  if (L.n1 % 2 || L.sy % 2 || ((uintptr_t)(fld + 1) & 15))
+    return fail(CANS_EINVAL, "cans_solver: rows of p must start 16-byte aligned (n1 and ld even, field from "
+                             "cans_field_alloc or with element (1,0,0) on a 16-byte boundary)");
+  if (div && (p->kind[0] != K_PP || p->len[0] < 2))
+    return fail(CANS_EINVAL, "run_solver: the velocity-divergence input needs a periodic x transform");
+  if ((rc = xpass(c, p, fld, (double*)W, +1, div))) return rc;
   if ((rc = ypass(c, p, (double*)W, +1))) return rc;
   ZArgs z;
   z.W = (double*)W;

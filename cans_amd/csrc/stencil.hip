@@ -5,6 +5,7 @@
 // association and rounding (Fortran evaluates left to right, no fused
 // multiply-add in the reference's -O3 CPU build), so these kernels reproduce the
 // CPU arithmetic bit for bit on identical inputs.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -32,10 +33,16 @@ struct MomArgs {
   double factor1, factor2, factor12;
   double bforce[3];
   int sum_mask;                  // bit c: accumulate the bulk mean of component c
+  int read_ro, write_ro;
   double* partials;              // [3][nblocks]
 };
 
-constexpr int MOM_BX = 64, MOM_BY = 4;
+// Tiling of the momentum kernel: a work-group owns a TX x TY column of cells and
+// marches up KC planes in z.  Plane k+1 of u,v,w (with a one-cell ring) is staged
+// in LDS once per tile; three rotating plane buffers leave one barrier per plane;
+// the z-neighbours at k-1 travel in registers.
+constexpr int MOM_TX = 64, MOM_TY = 8, MOM_KC = 32;
+constexpr int MOM_PX = MOM_TX + 2, MOM_PY = MOM_TY + 2;
 
 // rdzm(k) = dzci(k)/dzfi(k), rdzp(k) = dzci(k)/dzfi(k+1) for k = 1..n3 (mom.f90:801-809)
 __global__ void k_rdz(int n3, const double* dzci, const double* dzfi, double* tab) {
@@ -46,109 +53,220 @@ __global__ void k_rdz(int n3, const double* dzci, const double* dzfi, double* ta
   tab[n3 + 2 + k] = in ? dzci[k] / dzfi[k + 1] : 0.;
 }
 
-__global__ void __launch_bounds__(MOM_BX* MOM_BY) k_mom_rk(MomArgs a) {
+__global__ void __launch_bounds__(MOM_TX* MOM_TY) k_mom_rk(MomArgs a) {
+  __shared__ double su[3][MOM_PY][MOM_PX], sv[3][MOM_PY][MOM_PX], sw[3][MOM_PY][MOM_PX];
   const Layout& L = a.L;
-  const int i = blockIdx.x * MOM_BX + threadIdx.x + 1;
-  const int j = blockIdx.y * MOM_BY + threadIdx.y + 1;
-  const int k = blockIdx.z + 1;
+  const int ntx = (L.n1 + MOM_TX - 1) / MOM_TX, nty = (L.n2 + MOM_TY - 1) / MOM_TY;
+  const int nkc = (L.n3 + MOM_KC - 1) / MOM_KC;
+  const int nb = ntx * nty * nkc;
+  const int tile = xcd_tile(blockIdx.x, nb);
+  const int tx = tile % ntx, ty = (tile / ntx) % nty, tk = tile / (ntx * nty);
+  const int i0 = tx * MOM_TX, j0 = ty * MOM_TY;  // cell (i0+1, j0+1) is the first of the tile
+  const int kb = tk * MOM_KC + 1, ke = min(kb + MOM_KC - 1, L.n3);
+  const int li = threadIdx.x, lj = threadIdx.y;
+  const int t = li + MOM_TX * lj;
+  const int i = i0 + li + 1, j = j0 + lj + 1;
   const bool in = i <= L.n1 && j <= L.n2;
-  double s0 = 0., s1 = 0., s2 = 0.;
+  const long sy = L.sy, sz = L.sz;
+  // stage plane kk of the tile (+ ring) into buffer slot
+  auto stage = [&](int kk, int slot) {
+    for (int q = t; q < MOM_PX * MOM_PY; q += MOM_TX * MOM_TY) {
+      const int pi = q % MOM_PX, pj = q / MOM_PX;
+      const int gi = i0 + pi, gj = j0 + pj;  // halo'd indices (i0+pi is cell i0+pi)
+      if (gi <= L.n1 + 1 && gj <= L.n2 + 1) {
+        const long o = L.at(gi, gj, kk);
+        su[slot][pj][pi] = a.u[o];
+        sv[slot][pj][pi] = a.v[o];
+        sw[slot][pj][pi] = a.w[o];
+      }
+    }
+  };
+  const double dxi = a.dxi, dyi = a.dyi, visc = a.visc;
+  // z-neighbours below the first plane
+  double u_ccm = 0., v_ccm = 0., w_ccm = 0., w_pcm = 0., w_cpm = 0., p_c = 0.;
   if (in) {
-    const long c = L.at(i, j, k);
-    const long sy = L.sy, sz = L.sz;
-#define AT(f, di, dj, dk) f[c + (di) + sy * (dj) + sz * (dk)]
-    const double u_ccm = AT(a.u, 0, 0, -1), u_cmc = AT(a.u, 0, -1, 0), u_mcc = AT(a.u, -1, 0, 0),
-                 u_ccc = AT(a.u, 0, 0, 0), u_pcc = AT(a.u, 1, 0, 0), u_mpc = AT(a.u, -1, 1, 0),
-                 u_cpc = AT(a.u, 0, 1, 0), u_mcp = AT(a.u, -1, 0, 1), u_ccp = AT(a.u, 0, 0, 1);
-    const double v_ccm = AT(a.v, 0, 0, -1), v_cmc = AT(a.v, 0, -1, 0), v_pmc = AT(a.v, 1, -1, 0),
-                 v_mcc = AT(a.v, -1, 0, 0), v_ccc = AT(a.v, 0, 0, 0), v_pcc = AT(a.v, 1, 0, 0),
-                 v_cpc = AT(a.v, 0, 1, 0), v_cmp = AT(a.v, 0, -1, 1), v_ccp = AT(a.v, 0, 0, 1);
-    const double w_ccm = AT(a.w, 0, 0, -1), w_pcm = AT(a.w, 1, 0, -1), w_cpm = AT(a.w, 0, 1, -1),
-                 w_cmc = AT(a.w, 0, -1, 0), w_mcc = AT(a.w, -1, 0, 0), w_ccc = AT(a.w, 0, 0, 0),
-                 w_pcc = AT(a.w, 1, 0, 0), w_cpc = AT(a.w, 0, 1, 0), w_ccp = AT(a.w, 0, 0, 1);
-    const double dxi = a.dxi, dyi = a.dyi, visc = a.visc;
-    const double dzci_k = a.dzci[k], dzci_km = a.dzci[k - 1], dzfi_k = a.dzfi[k], dzfi_kp = a.dzfi[k + 1];
-    const double rdzm = a.rdzm[k], rdzp = a.rdzp[k];
-    // x
-    const double dudxp = (u_pcc - u_ccc) * dxi, dudxm = (u_ccc - u_mcc) * dxi;
-    const double dudyp = (u_cpc - u_ccc) * dyi, dudym = (u_ccc - u_cmc) * dyi;
-    const double dudzp = (u_ccp - u_ccc) * dzci_k, dudzm = (u_ccc - u_ccm) * dzci_km;
-    const double uuip = 0.25 * (u_ccc + u_pcc) * u_pcc, uuim = 0.25 * (u_ccc + u_mcc) * u_mcc;
-    const double vujp = 0.25 * (v_ccc + v_pcc) * u_cpc, vujm = 0.25 * (v_cmc + v_pmc) * u_cmc;
-    const double wukp = 0.25 * (w_ccc + w_pcc) * u_ccp, wukm = 0.25 * (w_ccm + w_pcm) * u_ccm;
-    double dudt = -(uuip - uuim) * dxi - (vujp - vujm) * dyi - (wukp - wukm) * dzfi_k;
-    dudt = dudt + (visc * (dudxp - dudxm) * dxi + visc * (dudyp - dudym) * dyi) +
-           visc * (dudzp - dudzm) * dzfi_k;
-    // y
-    const double dvdxp = (v_pcc - v_ccc) * dxi, dvdxm = (v_ccc - v_mcc) * dxi;
-    const double dvdyp = (v_cpc - v_ccc) * dyi, dvdym = (v_ccc - v_cmc) * dyi;
-    const double dvdzp = (v_ccp - v_ccc) * dzci_k, dvdzm = (v_ccc - v_ccm) * dzci_km;
-    const double uvip = 0.25 * (u_ccc + u_cpc) * v_pcc, uvim = 0.25 * (u_mcc + u_mpc) * v_mcc;
-    const double vvjp = 0.25 * (v_ccc + v_cpc) * v_cpc, vvjm = 0.25 * (v_ccc + v_cmc) * v_cmc;
-    const double wvkp = 0.25 * (w_ccc + w_cpc) * v_ccp, wvkm = 0.25 * (w_ccm + w_cpm) * v_ccm;
-    double dvdt = -(uvip - uvim) * dxi - (vvjp - vvjm) * dyi - (wvkp - wvkm) * dzfi_k;
-    dvdt = dvdt + (visc * (dvdxp - dvdxm) * dxi + visc * (dvdyp - dvdym) * dyi) +
-           visc * (dvdzp - dvdzm) * dzfi_k;
-    // z
-    const double dwdxp = (w_pcc - w_ccc) * dxi, dwdxm = (w_ccc - w_mcc) * dxi;
-    const double dwdyp = (w_cpc - w_ccc) * dyi, dwdym = (w_ccc - w_cmc) * dyi;
-    const double dwdzp = (w_ccp - w_ccc) * dzfi_kp, dwdzm = (w_ccc - w_ccm) * dzfi_k;
-    const double uwip = 0.25 * (rdzm * u_ccc + rdzp * u_ccp) * w_pcc;
-    const double uwim = 0.25 * (rdzm * u_mcc + rdzp * u_mcp) * w_mcc;
-    const double vwjp = 0.25 * (rdzm * v_ccc + rdzp * v_ccp) * w_cpc;
-    const double vwjm = 0.25 * (rdzm * v_cmc + rdzp * v_cmp) * w_cmc;
-    const double wwkp = 0.25 * (w_ccc + w_ccp) * w_ccp, wwkm = 0.25 * (w_ccc + w_ccm) * w_ccm;
-    double dwdt = -(uwip - uwim) * dxi - (vwjp - vwjm) * dyi - (wwkp - wwkm) * dzci_k;
-    dwdt = dwdt + (visc * (dwdxp - dwdxm) * dxi + visc * (dwdyp - dwdym) * dyi) +
-           visc * (dwdzp - dwdzm) * dzci_k;
-    // RK update (rk.f90:156-186) and RHS_old <- RHS (rk.f90:266-276)
-    const long o = (long)(i - 1) + (long)L.n1 * ((j - 1) + (long)L.n2 * (k - 1));
-    const double p_c = a.p[c];
-    const double un = u_ccc + a.factor1 * dudt + a.factor2 * a.ru[o] +
-                      a.factor12 * (a.bforce[0] - dxi * (a.p[c + 1] - p_c));
-    const double vn = v_ccc + a.factor1 * dvdt + a.factor2 * a.rv[o] +
-                      a.factor12 * (a.bforce[1] - dyi * (a.p[c + sy] - p_c));
-    const double wn = w_ccc + a.factor1 * dwdt + a.factor2 * a.rw[o] +
-                      a.factor12 * (a.bforce[2] - dzci_k * (a.p[c + sz] - p_c));
-#undef AT
-    a.uo[c] = un;
-    a.vo[c] = vn;
-    a.wo[c] = wn;
-    a.ru[o] = dudt;
-    a.rv[o] = dvdt;
-    a.rw[o] = dwdt;
-    if (a.sum_mask) {
-      s0 = un * a.gvr_f[k];
-      s1 = vn * a.gvr_f[k];
-      s2 = wn * a.gvr_c[k];
+    const long c0 = L.at(i, j, kb);
+    u_ccm = a.u[c0 - sz];
+    v_ccm = a.v[c0 - sz];
+    w_ccm = a.w[c0 - sz];
+    w_pcm = a.w[c0 - sz + 1];
+    w_cpm = a.w[c0 - sz + sy];
+    p_c = a.p[c0];
+  }
+  stage(kb, kb % 3);
+  double s0 = 0., s1 = 0., s2 = 0.;
+  for (int k = kb; k <= ke; k++) {
+    stage(k + 1, (k + 1) % 3);
+    __syncthreads();
+    const int b0 = k % 3, b1 = (k + 1) % 3;
+    const int pi = li + 1, pj = lj + 1;
+    if (in) {
+      const long c = L.at(i, j, k);
+      // plane k
+      const double u_ccc = su[b0][pj][pi], u_pcc = su[b0][pj][pi + 1], u_mcc = su[b0][pj][pi - 1],
+                   u_cpc = su[b0][pj + 1][pi], u_cmc = su[b0][pj - 1][pi], u_mpc = su[b0][pj + 1][pi - 1];
+      const double v_ccc = sv[b0][pj][pi], v_pcc = sv[b0][pj][pi + 1], v_mcc = sv[b0][pj][pi - 1],
+                   v_cpc = sv[b0][pj + 1][pi], v_cmc = sv[b0][pj - 1][pi], v_pmc = sv[b0][pj - 1][pi + 1];
+      const double w_ccc = sw[b0][pj][pi], w_pcc = sw[b0][pj][pi + 1], w_mcc = sw[b0][pj][pi - 1],
+                   w_cpc = sw[b0][pj + 1][pi], w_cmc = sw[b0][pj - 1][pi];
+      // plane k+1
+      const double u_ccp = su[b1][pj][pi], u_mcp = su[b1][pj][pi - 1];
+      const double v_ccp = sv[b1][pj][pi], v_cmp = sv[b1][pj - 1][pi];
+      const double w_ccp = sw[b1][pj][pi];
+      const double dzci_k = a.dzci[k], dzci_km = a.dzci[k - 1], dzfi_k = a.dzfi[k], dzfi_kp = a.dzfi[k + 1];
+      const double rdzm = a.rdzm[k], rdzp = a.rdzp[k];
+      // x
+      const double dudxp = (u_pcc - u_ccc) * dxi, dudxm = (u_ccc - u_mcc) * dxi;
+      const double dudyp = (u_cpc - u_ccc) * dyi, dudym = (u_ccc - u_cmc) * dyi;
+      const double dudzp = (u_ccp - u_ccc) * dzci_k, dudzm = (u_ccc - u_ccm) * dzci_km;
+      const double uuip = 0.25 * (u_ccc + u_pcc) * u_pcc, uuim = 0.25 * (u_ccc + u_mcc) * u_mcc;
+      const double vujp = 0.25 * (v_ccc + v_pcc) * u_cpc, vujm = 0.25 * (v_cmc + v_pmc) * u_cmc;
+      const double wukp = 0.25 * (w_ccc + w_pcc) * u_ccp, wukm = 0.25 * (w_ccm + w_pcm) * u_ccm;
+      double dudt = -(uuip - uuim) * dxi - (vujp - vujm) * dyi - (wukp - wukm) * dzfi_k;
+      dudt = dudt + (visc * (dudxp - dudxm) * dxi + visc * (dudyp - dudym) * dyi) +
+             visc * (dudzp - dudzm) * dzfi_k;
+      // y
+      const double dvdxp = (v_pcc - v_ccc) * dxi, dvdxm = (v_ccc - v_mcc) * dxi;
+      const double dvdyp = (v_cpc - v_ccc) * dyi, dvdym = (v_ccc - v_cmc) * dyi;
+      const double dvdzp = (v_ccp - v_ccc) * dzci_k, dvdzm = (v_ccc - v_ccm) * dzci_km;
+      const double uvip = 0.25 * (u_ccc + u_cpc) * v_pcc, uvim = 0.25 * (u_mcc + u_mpc) * v_mcc;
+      const double vvjp = 0.25 * (v_ccc + v_cpc) * v_cpc, vvjm = 0.25 * (v_ccc + v_cmc) * v_cmc;
+      const double wvkp = 0.25 * (w_ccc + w_cpc) * v_ccp, wvkm = 0.25 * (w_ccm + w_cpm) * v_ccm;
+      double dvdt = -(uvip - uvim) * dxi - (vvjp - vvjm) * dyi - (wvkp - wvkm) * dzfi_k;
+      dvdt = dvdt + (visc * (dvdxp - dvdxm) * dxi + visc * (dvdyp - dvdym) * dyi) +
+             visc * (dvdzp - dvdzm) * dzfi_k;
+      // z
+      const double dwdxp = (w_pcc - w_ccc) * dxi, dwdxm = (w_ccc - w_mcc) * dxi;
+      const double dwdyp = (w_cpc - w_ccc) * dyi, dwdym = (w_ccc - w_cmc) * dyi;
+      const double dwdzp = (w_ccp - w_ccc) * dzfi_kp, dwdzm = (w_ccc - w_ccm) * dzfi_k;
+      const double uwip = 0.25 * (rdzm * u_ccc + rdzp * u_ccp) * w_pcc;
+      const double uwim = 0.25 * (rdzm * u_mcc + rdzp * u_mcp) * w_mcc;
+      const double vwjp = 0.25 * (rdzm * v_ccc + rdzp * v_ccp) * w_cpc;
+      const double vwjm = 0.25 * (rdzm * v_cmc + rdzp * v_cmp) * w_cmc;
+      const double wwkp = 0.25 * (w_ccc + w_ccp) * w_ccp, wwkm = 0.25 * (w_ccc + w_ccm) * w_ccm;
+      double dwdt = -(uwip - uwim) * dxi - (vwjp - vwjm) * dyi - (wwkp - wwkm) * dzci_k;
+      dwdt = dwdt + (visc * (dwdxp - dwdxm) * dxi + visc * (dwdyp - dwdym) * dyi) +
+             visc * (dwdzp - dwdzm) * dzci_k;
+      // RK update (rk.f90:156-186) and RHS_old <- RHS (rk.f90:266-276).  With rkpar(2) = 0
+      // (first substep) the RHS_old term is factor2*RHS_old = 0 and is not read.
+      const long o = (long)(i - 1) + (long)L.n1 * ((j - 1) + (long)L.n2 * (k - 1));
+      const double p_ip = a.p[c + 1], p_jp = a.p[c + sy], p_kp = a.p[c + sz];
+      const double ru = a.read_ro ? a.ru[o] : 0., rv = a.read_ro ? a.rv[o] : 0., rw = a.read_ro ? a.rw[o] : 0.;
+      const double un = a.read_ro ? u_ccc + a.factor1 * dudt + a.factor2 * ru +
+                                        a.factor12 * (a.bforce[0] - dxi * (p_ip - p_c))
+                                  : u_ccc + a.factor1 * dudt + a.factor12 * (a.bforce[0] - dxi * (p_ip - p_c));
+      const double vn = a.read_ro ? v_ccc + a.factor1 * dvdt + a.factor2 * rv +
+                                        a.factor12 * (a.bforce[1] - dyi * (p_jp - p_c))
+                                  : v_ccc + a.factor1 * dvdt + a.factor12 * (a.bforce[1] - dyi * (p_jp - p_c));
+      const double wn = a.read_ro ? w_ccc + a.factor1 * dwdt + a.factor2 * rw +
+                                        a.factor12 * (a.bforce[2] - dzci_k * (p_kp - p_c))
+                                  : w_ccc + a.factor1 * dwdt + a.factor12 * (a.bforce[2] - dzci_k * (p_kp - p_c));
+      a.uo[c] = un;
+      a.vo[c] = vn;
+      a.wo[c] = wn;
+      if (a.write_ro) {
+        a.ru[o] = dudt;
+        a.rv[o] = dvdt;
+        a.rw[o] = dwdt;
+      }
+      if (a.sum_mask) {
+        s0 += un * a.gvr_f[k];
+        s1 += vn * a.gvr_f[k];
+        s2 += wn * a.gvr_c[k];
+      }
+      // march: plane k becomes plane k-1
+      u_ccm = u_ccc;
+      v_ccm = v_ccc;
+      w_ccm = w_ccc;
+      w_pcm = w_pcc;
+      w_cpm = w_cpc;
+      p_c = p_kp;
     }
   }
   if (a.sum_mask) {
-    const int nb = gridDim.x * gridDim.y * gridDim.z;
-    const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-    block_sum3(s0, s1, s2, a.partials + bid, a.partials + nb + bid, a.partials + 2 * nb + bid);
+    __syncthreads();
+    block_sum3<MOM_TX * MOM_TY>(s0, s1, s2, a.partials + blockIdx.x, a.partials + nb + blockIdx.x,
+                                a.partials + 2 * nb + blockIdx.x);
   }
 }
 
-// bulk forcing f = velf - mean (rk.f90:493-518); fixed-order sum of the partials
-__global__ void k_forcing_finalize(const double* partials, int nb, int mask, double vx, double vy, double vz,
-                                   double* f) {
+__global__ void k_fill(double* p, int n, double v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Deterministic two-stage reduction of per-block partials (fixed summation order).
+// stage A: block g reduces partials[c*nb + [g*chunk, (g+1)*chunk)) -> stage[c*G + g]
+template <int OP>  // 0 sum, 1 max
+__global__ void k_reduce_a(const double* partials, int nb, int chunk, double* stage) {
   __shared__ double red[256];
-  for (int c = 0; c < 3; c++) {
-    double s = 0.;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) s += partials[c * nb + b];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-      if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-      const double velf = c == 0 ? vx : c == 1 ? vy : vz;
-      f[c] = (mask >> c & 1) ? velf - red[0] : 0.;
-    }
+  const int c = blockIdx.y, g = blockIdx.x, G = gridDim.x;
+  const int b0 = g * chunk, b1 = min(nb, b0 + chunk);
+  double s = 0.;
+  for (int b = b0 + (int)threadIdx.x; b < b1; b += blockDim.x) {
+    const double x = partials[(long)c * nb + b];
+    s = OP ? fmax(s, x) : s + x;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] = OP ? fmax(red[threadIdx.x], red[threadIdx.x + st])
+                                                     : red[threadIdx.x] + red[threadIdx.x + st];
     __syncthreads();
   }
+  if (threadIdx.x == 0) stage[c * G + g] = red[0];
+}
+
+// stage B: one block per component reduces the G stage values.
+// mode 0: out[c] = result; mode 1 (bulk forcing, rk.f90:493-518): out[c] = bit c of mask ? velf[c] - mean : 0
+template <int OP>
+__global__ void k_reduce_b(const double* stage, int G, double* out, int mode, int mask, double v0, double v1,
+                           double v2) {
+  __shared__ double red[256];
+  const int c = blockIdx.x;
+  double s = 0.;
+  for (int g = threadIdx.x; g < G; g += blockDim.x) s = OP ? fmax(s, stage[c * G + g]) : s + stage[c * G + g];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] = OP ? fmax(red[threadIdx.x], red[threadIdx.x + st])
+                                                     : red[threadIdx.x] + red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (mode == 0) out[c] = red[0];
+    else {
+      const double velf = c == 0 ? v0 : c == 1 ? v1 : v2;
+      out[c] = (mask >> c & 1) ? velf - red[0] : 0.;
+    }
+  }
+}
+
+// partials (ncomp x nb, component-major) -> out[0..ncomp); stage scratch follows the partials
+// `base` holds ncomp x nb partials inside c->partials; `stage` is scratch for ncomp*1024 values
+static int reduce_partials(cans_ctx* c, const double* base, int nb, int ncomp, int op, double* out, double* stage,
+                           int mode = 0, int mask = 0, const double* velf = nullptr) {
+  const int G = (int)std::min<long>(1024, std::max<long>(1, (nb + 1023) / 1024));
+  const int chunk = (nb + G - 1) / G;
+  dim3 ga(G, ncomp);
+  if (op) k_reduce_a<1><<<ga, 256, 0, c->stream>>>(base, nb, chunk, stage);
+  else k_reduce_a<0><<<ga, 256, 0, c->stream>>>(base, nb, chunk, stage);
+  CANS_LAUNCH_CHECK("k_reduce_a");
+  const double v0 = velf ? velf[0] : 0., v1 = velf ? velf[1] : 0., v2 = velf ? velf[2] : 0.;
+  if (op) k_reduce_b<1><<<ncomp, 256, 0, c->stream>>>(stage, G, out, mode, mask, v0, v1, v2);
+  else k_reduce_b<0><<<ncomp, 256, 0, c->stream>>>(stage, G, out, mode, mask, v0, v1, v2);
+  CANS_LAUNCH_CHECK("k_reduce_b");
+  return CANS_OK;
+}
+
+static int ensure_partials(cans_ctx* c, size_t n) {
+  n += 4096;  // room for the reduction stages (3 x 1024)
+  if (n <= c->partials_n) return CANS_OK;
+  CANS_HIP_TRY(hipStreamSynchronize(c->stream));
+  hipFree(c->partials);
+  c->partials_n = n;
+  CANS_HIP_TRY(hipMalloc((void**)&c->partials, n * sizeof(double)));
+  return CANS_OK;
 }
 
 // u(1:n) += f(c) (mom.f90:725-769); src may differ from dst (copy + forcing)
@@ -171,9 +289,11 @@ __global__ void k_fillps(Layout L, const double* u, const double* v, const doubl
 }
 
 // correc.f90:13-61 over the whole halo'd box; (su,sv,sw) -> (du,dv,dw), which may alias.
+// With pu != nullptr the pressure update p += pp of updatep.f90:78-86 rides along
+// (it needs only pp and p, both final at this point of the substep).
 __global__ void k_correc(Layout L, const double* p, const double* su, const double* sv, const double* sw,
                          double* du, double* dv, double* dw, const double* dzci, double dt, double factori,
-                         double factorj) {
+                         double factorj, double* pu) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y, k = blockIdx.z;
   if (i > L.n1 + 1) return;
@@ -182,6 +302,7 @@ __global__ void k_correc(Layout L, const double* p, const double* su, const doub
   du[c] = (i <= L.n1) ? su[c] - factori * (p[c + 1] - pc) : su[c];
   dv[c] = (j <= L.n2) ? sv[c] - factorj * (p[c + L.sy] - pc) : sv[c];
   dw[c] = (k <= L.n3) ? sw[c] - dt * dzci[k] * (p[c + L.sz] - pc) : sw[c];
+  if (pu && i >= 1 && i <= L.n1 && j >= 1 && j <= L.n2 && k >= 1 && k <= L.n3) pu[c] = pu[c] + pc;
 }
 
 // updatep.f90:14-88
@@ -310,21 +431,6 @@ __global__ void k_bulk_mean(Layout L, const double* q, const double* gvr, double
   block_sum3(s, 0., 0., partials + bid, &d1, &d2, true);
 }
 
-// fixed-order final reduction of nb partials: op 0 sum, 1 max
-__global__ void k_finalize(const double* partials, int nb, int op, double* out) {
-  __shared__ double red[256];
-  double s = 0.;
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) s = op ? fmax(s, partials[b]) : s + partials[b];
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st) red[threadIdx.x] = op ? fmax(red[threadIdx.x], red[threadIdx.x + st])
-                                                     : red[threadIdx.x] + red[threadIdx.x + st];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *out = red[0];
-}
-
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -353,20 +459,18 @@ int launch_mom_rk(cans_ctx* c, const MomLaunch& m) {
   a.rdzp = (const double*)tab + (L.n3 + 2);
   k_rdz<<<cdiv(L.n3 + 2, 256), 256, 0, c->stream>>>(L.n3, m.dzci, m.dzfi, (double*)tab);
   CANS_LAUNCH_CHECK("k_rdz");
-  dim3 grid(cdiv(L.n1, MOM_BX), cdiv(L.n2, MOM_BY), L.n3);
-  const long nb = (long)grid.x * grid.y * grid.z;
-  if (a.sum_mask && (size_t)(3 * nb) > c->partials_n) {
-    CANS_HIP_TRY(hipStreamSynchronize(c->stream));
-    hipFree(c->partials);
-    c->partials_n = 3 * nb;
-    CANS_HIP_TRY(hipMalloc((void**)&c->partials, c->partials_n * sizeof(double)));
-  }
+  const long nb = (long)cdiv(L.n1, MOM_TX) * cdiv(L.n2, MOM_TY) * cdiv(L.n3, MOM_KC);
+  dim3 grid((unsigned)nb);
+  if (a.sum_mask && (rc = ensure_partials(c, 3 * nb))) return rc;
   a.partials = c->partials;
-  k_mom_rk<<<grid, dim3(MOM_BX, MOM_BY), 0, c->stream>>>(a);
+  a.read_ro = m.read_ro && m.rkpar[1] != 0.;
+  a.write_ro = m.write_ro;
+  k_mom_rk<<<grid, dim3(MOM_TX, MOM_TY), 0, c->stream>>>(a);
   CANS_LAUNCH_CHECK("k_mom_rk");
-  k_forcing_finalize<<<1, 256, 0, c->stream>>>(c->partials, (int)nb, a.sum_mask, m.velf[0], m.velf[1],
-                                                m.velf[2], c->dscal);
-  CANS_LAUNCH_CHECK("k_forcing_finalize");
+  if (a.sum_mask)
+    return reduce_partials(c, c->partials, (int)nb, 3, 0, c->dscal, c->partials + 3 * nb, 1, a.sum_mask, m.velf);
+  k_fill<<<1, 64, 0, c->stream>>>(c->dscal, 3, 0.);  // no memset node if the caller captures a graph
+  CANS_LAUNCH_CHECK("k_fill");
   return CANS_OK;
 }
 
@@ -386,10 +490,11 @@ int launch_fillps(cans_ctx* c, const double* u, const double* v, const double* w
 }
 
 int launch_correc(cans_ctx* c, const double* p, const double* su, const double* sv, const double* sw, double* du,
-                  double* dv, double* dw, const double* dzci, double dt, const double* dli) {
+                  double* dv, double* dw, const double* dzci, double dt, const double* dli, double* p_update) {
   const Layout& L = c->L;
   dim3 grid(cdiv(L.n1 + 2, 256), L.n2 + 2, L.n3 + 2);
-  k_correc<<<grid, 256, 0, c->stream>>>(L, p, su, sv, sw, du, dv, dw, dzci, dt, dt * dli[0], dt * dli[1]);
+  k_correc<<<grid, 256, 0, c->stream>>>(L, p, su, sv, sw, du, dv, dw, dzci, dt, dt * dli[0], dt * dli[1],
+                                         p_update);
   CANS_LAUNCH_CHECK("k_correc");
   return CANS_OK;
 }
@@ -424,15 +529,6 @@ int launch_rhs_b(cans_ctx* c, double* p, const double* r, int dir, int plane, do
   return CANS_OK;
 }
 
-static int ensure_partials(cans_ctx* c, size_t n) {
-  if (n <= c->partials_n) return CANS_OK;
-  CANS_HIP_TRY(hipStreamSynchronize(c->stream));
-  hipFree(c->partials);
-  c->partials_n = n;
-  CANS_HIP_TRY(hipMalloc((void**)&c->partials, n * sizeof(double)));
-  return CANS_OK;
-}
-
 int run_chkdiv(cans_ctx* c, const double* u, const double* v, const double* w, const double* dzfi, const double* dli,
                double* tot, double* mx) {
   const Layout& L = c->L;
@@ -442,9 +538,9 @@ int run_chkdiv(cans_ctx* c, const double* u, const double* v, const double* w, c
   if (rc) return rc;
   k_chkdiv<<<grid, 256, 0, c->stream>>>(L, u, v, w, dzfi, dli[0], dli[1], c->partials, (int)nb);
   CANS_LAUNCH_CHECK("k_chkdiv");
-  k_finalize<<<1, 256, 0, c->stream>>>(c->partials, (int)nb, 0, c->dscal + 8);
-  k_finalize<<<1, 256, 0, c->stream>>>(c->partials + nb, (int)nb, 1, c->dscal + 9);
-  CANS_LAUNCH_CHECK("k_finalize");
+  if ((rc = reduce_partials(c, c->partials, (int)nb, 1, 0, c->dscal + 8, c->partials + 2 * nb))) return rc;
+  if ((rc = reduce_partials(c, c->partials + nb, (int)nb, 1, 1, c->dscal + 9, c->partials + 2 * nb + 1024)))
+    return rc;
   double h[2];
   CANS_HIP_TRY(hipMemcpyAsync(h, c->dscal + 8, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   CANS_HIP_TRY(hipStreamSynchronize(c->stream));
@@ -462,8 +558,7 @@ int run_chkdt(cans_ctx* c, const double* u, const double* v, const double* w, co
   if (rc) return rc;
   k_chkdt<<<grid, 256, 0, c->stream>>>(L, u, v, w, dzci, dzfi, 1. / dl[0], 1. / dl[1], c->partials);
   CANS_LAUNCH_CHECK("k_chkdt");
-  k_finalize<<<1, 256, 0, c->stream>>>(c->partials, (int)nb, 1, c->dscal + 10);
-  CANS_LAUNCH_CHECK("k_finalize");
+  if ((rc = reduce_partials(c, c->partials, (int)nb, 1, 1, c->dscal + 10, c->partials + nb))) return rc;
   CANS_HIP_TRY(hipMemcpyAsync(dti_out, c->dscal + 10, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   CANS_HIP_TRY(hipStreamSynchronize(c->stream));
   return CANS_OK;
@@ -477,8 +572,7 @@ int run_bulk_mean(cans_ctx* c, const double* q, const double* gvr, double* mean)
   if (rc) return rc;
   k_bulk_mean<<<grid, 256, 0, c->stream>>>(L, q, gvr, c->partials);
   CANS_LAUNCH_CHECK("k_bulk_mean");
-  k_finalize<<<1, 256, 0, c->stream>>>(c->partials, (int)nb, 0, c->dscal + 11);
-  CANS_LAUNCH_CHECK("k_finalize");
+  if ((rc = reduce_partials(c, c->partials, (int)nb, 1, 0, c->dscal + 11, c->partials + nb))) return rc;
   CANS_HIP_TRY(hipMemcpyAsync(mean, c->dscal + 11, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   CANS_HIP_TRY(hipStreamSynchronize(c->stream));
   return CANS_OK;

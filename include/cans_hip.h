@@ -162,7 +162,10 @@ typedef struct cans_state {
   void *dudtrko, *dvdtrko, *dwdtrko;           /* (1:n1,1:n2,1:n3) */
 } cans_state;
 
-/* irk in 1..3; rkcoeff as src/param.f90:20-22 */
+/* irk in 1..3; rkcoeff as src/param.f90:20-22.  The fused path skips work whose
+ * result is never read: substep 1 does not read dudtrko (its weight rkcoeff(2,1) is
+ * 0) and substep 3 does not write it (the next step's substep 1 overwrites it before
+ * any use), so after substep 3 dudtrko still holds substep 2's right-hand side. */
 int cans_rk3_substep(cans_ctx *ctx, const cans_step_params *prm, int irk, double dt, cans_state *st);
 /* forcing of the last substep, device pointer to 3 doubles (f(1:3) of rk.f90) */
 int cans_ctx_forcing(cans_ctx *ctx, const double **f_d);
