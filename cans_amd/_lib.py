@@ -50,6 +50,10 @@ SIGNATURES = {
     "cans_field_free": (C.c_int, [_vp, _vp]),
     "cans_field_upload": (C.c_int, [_vp, _vp, _dp]),
     "cans_field_download": (C.c_int, [_vp, _vp, _dp]),
+    "cans_device_alloc": (C.c_int, [_vp, C.c_long, C.POINTER(_vp)]),
+    "cans_device_free": (C.c_int, [_vp, _vp]),
+    "cans_memcpy_h2d": (C.c_int, [_vp, _vp, _vp, C.c_long]),
+    "cans_memcpy_d2h": (C.c_int, [_vp, _vp, _vp, C.c_long]),
     "cans_initgrid": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp, _dp, _dp]),
     "cans_eigenvalues": (C.c_int, [C.c_int, _cp, C.c_char, _dp]),
     "cans_tridmatrix": (C.c_int, [_cp, C.c_int, _dp, _dp, C.c_char, _dp, _dp, _dp]),

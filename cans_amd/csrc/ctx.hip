@@ -164,6 +164,35 @@ extern "C" int cans_field_download(cans_ctx* c, const void* field, double* host)
   return CANS_OK;
 }
 
+extern "C" int cans_device_alloc(cans_ctx* c, long bytes, void** ptr) {
+  if (!c || !ptr || bytes < 0) return fail(CANS_EINVAL, "cans_device_alloc: bad argument");
+  *ptr = nullptr;
+  hipError_t e = hipMalloc(ptr, bytes > 0 ? (size_t)bytes : 1);
+  if (e != hipSuccess) return fail(CANS_ENOMEM, std::string("cans_device_alloc: ") + hipGetErrorString(e));
+  return CANS_OK;
+}
+
+extern "C" int cans_device_free(cans_ctx* c, void* ptr) {
+  if (!ptr) return CANS_OK;
+  if (c) hipStreamSynchronize(c->stream);
+  CANS_HIP_TRY(hipFree(ptr));
+  return CANS_OK;
+}
+
+extern "C" int cans_memcpy_h2d(cans_ctx* c, void* dst, const void* src, long bytes) {
+  if (!c || !dst || !src || bytes < 0) return fail(CANS_EINVAL, "cans_memcpy_h2d: bad argument");
+  CANS_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  CANS_HIP_TRY(hipStreamSynchronize(c->stream));
+  return CANS_OK;
+}
+
+extern "C" int cans_memcpy_d2h(cans_ctx* c, void* dst, const void* src, long bytes) {
+  if (!c || !dst || !src || bytes < 0) return fail(CANS_EINVAL, "cans_memcpy_d2h: bad argument");
+  CANS_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  CANS_HIP_TRY(hipStreamSynchronize(c->stream));
+  return CANS_OK;
+}
+
 // ---------------------------------------------------------------------------
 // host set-up helpers (initgrid.f90, initsolver.f90).  Fortran default-real
 // expressions of the reference are evaluated in single precision, as there.

@@ -1,0 +1,256 @@
+! Fortran interface to libcans_hip.so (include/cans_hip.h).
+!
+! A CaNS host keeps its own Fortran (src/main.f90, src/initsolver.f90, the
+! namelist input) and replaces the bodies of the device routines by calls to
+! these interfaces; argument order and meaning follow the reference routines,
+! with the context handle first.  Every function returns 0 on success; the
+! helper cans_check stops with the library's message otherwise, mirroring the
+! reference's `error stop` on fatal conditions (src/fft.f90:525).
+module cans_hip
+  use, intrinsic :: iso_c_binding
+  implicit none
+  integer(c_int), parameter :: CANS_FP64 = 0, CANS_MIXED = 1
+  integer(c_int), parameter :: CANS_HALO_ABSENT = -99, CANS_HALO_LOWER = -1, &
+                               CANS_HALO_NONE = 0, CANS_HALO_UPPER = 1
+  !
+  ! cans_step_params (the argument bundle of one fused RK3 substep)
+  !
+  type, bind(C) :: cans_step_params
+    character(kind=c_char) :: cbcvel(18)
+    real(c_double)         :: bcvel(18)
+    character(kind=c_char) :: cbcpre(6)
+    real(c_double)         :: bcpre(6)
+    integer(c_int)         :: nb(6), is_bound(6)
+    real(c_double)         :: dl(3), dli(3)
+    real(c_double)         :: visc
+    integer(c_int)         :: is_forced(3)
+    real(c_double)         :: velf(3), bforce(3)
+    type(c_ptr)            :: dzc_h, dzf_h, dzci_d, dzfi_d, grid_vol_ratio_c_d, grid_vol_ratio_f_d, plan
+    real(c_double)         :: normfft
+    type(c_ptr)            :: lambdaxy_d, a_d, b_d, c_d, rhsbx_d, rhsby_d, rhsbz_d
+  end type cans_step_params
+  type, bind(C) :: cans_state
+    type(c_ptr) :: u, v, w, p, pp, dudtrko, dvdtrko, dwdtrko
+  end type cans_state
+  !
+  interface
+    ! ---- context and memory (replace initmpi/workspaces device setup) ----
+    integer(c_int) function cans_ctx_create(n, ld, precision, stream, ctx) bind(C, name='cans_ctx_create')
+      import :: c_int, c_ptr
+      integer(c_int), intent(in) :: n(3)
+      integer(c_int), value :: ld, precision
+      type(c_ptr), value :: stream
+      type(c_ptr), intent(out) :: ctx
+    end function
+    integer(c_int) function cans_ctx_destroy(ctx) bind(C, name='cans_ctx_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function cans_ctx_sync(ctx) bind(C, name='cans_ctx_sync')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    type(c_ptr) function cans_last_error() bind(C, name='cans_last_error')
+      import :: c_ptr
+    end function
+    integer(c_int) function cans_field_alloc(ctx, field) bind(C, name='cans_field_alloc')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      type(c_ptr), intent(out) :: field
+    end function
+    integer(c_int) function cans_field_free(ctx, field) bind(C, name='cans_field_free')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, field
+    end function
+    integer(c_int) function cans_field_upload(ctx, field, host) bind(C, name='cans_field_upload')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, field
+      real(c_double), intent(in) :: host(*)
+    end function
+    integer(c_int) function cans_field_download(ctx, field, host) bind(C, name='cans_field_download')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, field
+      real(c_double), intent(out) :: host(*)
+    end function
+    integer(c_int) function cans_device_alloc(ctx, bytes, ptr) bind(C, name='cans_device_alloc')
+      import :: c_int, c_ptr, c_long
+      type(c_ptr), value :: ctx
+      integer(c_long), value :: bytes
+      type(c_ptr), intent(out) :: ptr
+    end function
+    integer(c_int) function cans_device_free(ctx, ptr) bind(C, name='cans_device_free')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, ptr
+    end function
+    integer(c_int) function cans_memcpy_h2d(ctx, dst, src, bytes) bind(C, name='cans_memcpy_h2d')
+      import :: c_int, c_ptr, c_long
+      type(c_ptr), value :: ctx, dst, src
+      integer(c_long), value :: bytes
+    end function
+    integer(c_int) function cans_memcpy_d2h(ctx, dst, src, bytes) bind(C, name='cans_memcpy_d2h')
+      import :: c_int, c_ptr, c_long
+      type(c_ptr), value :: ctx, dst, src
+      integer(c_long), value :: bytes
+    end function
+    ! ---- host set-up helpers (initgrid.f90, initsolver.f90) ----
+    integer(c_int) function cans_initgrid(gtype, n, gr, lz, is_periodic, dzc, dzf, zc, zf) &
+                   bind(C, name='cans_initgrid')
+      import :: c_int, c_double
+      integer(c_int), value :: gtype, n, is_periodic
+      real(c_double), value :: gr, lz
+      real(c_double), intent(out) :: dzc(*), dzf(*), zc(*), zf(*)
+    end function
+    integer(c_int) function cans_eigenvalues(n, bc, c_or_f, lambda) bind(C, name='cans_eigenvalues')
+      import :: c_int, c_double, c_char
+      integer(c_int), value :: n
+      character(kind=c_char), intent(in) :: bc(2)
+      character(kind=c_char), value :: c_or_f
+      real(c_double), intent(out) :: lambda(*)
+    end function
+    integer(c_int) function cans_tridmatrix(bc, n, dzci, dzfi, c_or_f, a, b, c) bind(C, name='cans_tridmatrix')
+      import :: c_int, c_double, c_char
+      character(kind=c_char), intent(in) :: bc(2)
+      integer(c_int), value :: n
+      real(c_double), intent(in) :: dzci(*), dzfi(*)
+      character(kind=c_char), value :: c_or_f
+      real(c_double), intent(out) :: a(*), b(*), c(*)
+    end function
+    ! ---- fftini / fftend (fft.f90:25-245) ----
+    integer(c_int) function cans_fftini(ctx, ng, bcxy, c_or_f, arrplan, normfft) bind(C, name='cans_fftini')
+      import :: c_int, c_ptr, c_char, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(in) :: ng(3)
+      character(kind=c_char), intent(in) :: bcxy(4), c_or_f(2)
+      type(c_ptr), intent(out) :: arrplan
+      real(c_double), intent(out) :: normfft
+    end function
+    integer(c_int) function cans_fftend(arrplan) bind(C, name='cans_fftend')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: arrplan
+    end function
+    ! ---- operators (same argument lists as the reference, context first) ----
+    integer(c_int) function cans_solver(ctx, n, ng, arrplan, normfft, lambdaxy, a, b, c, bc, c_or_f, p) &
+                   bind(C, name='cans_solver')
+      import :: c_int, c_ptr, c_double, c_char
+      type(c_ptr), value :: ctx, arrplan, lambdaxy, a, b, c, p
+      integer(c_int), intent(in) :: n(3), ng(3)
+      real(c_double), value :: normfft
+      character(kind=c_char), intent(in) :: bc(6), c_or_f(3)
+    end function
+    integer(c_int) function cans_solve_helmholtz(ctx, n, ng, arrplan, normfft, alpha, lambdaxy, a, b, c, &
+                                                 rhsbx, rhsby, rhsbz, is_bound, cbc, c_or_f, p) &
+                   bind(C, name='cans_solve_helmholtz')
+      import :: c_int, c_ptr, c_double, c_char
+      type(c_ptr), value :: ctx, arrplan, lambdaxy, a, b, c, rhsbx, rhsby, rhsbz, p
+      integer(c_int), intent(in) :: n(3), ng(3), is_bound(6)
+      real(c_double), value :: normfft, alpha
+      character(kind=c_char), intent(in) :: cbc(6), c_or_f(3)
+    end function
+    integer(c_int) function cans_rk(ctx, rkpar, n, dli, dzci, dzfi, grid_vol_ratio_c, grid_vol_ratio_f, dt, visc, &
+                                    p, is_forced, velf, bforce, dudtrko, dvdtrko, dwdtrko, u, v, w, f) &
+                   bind(C, name='cans_rk')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, dzci, dzfi, grid_vol_ratio_c, grid_vol_ratio_f, p, dudtrko, dvdtrko, dwdtrko, u, v, w
+      real(c_double), intent(in) :: rkpar(2), dli(3), velf(3), bforce(3)
+      integer(c_int), intent(in) :: n(3), is_forced(3)
+      real(c_double), value :: dt, visc
+      real(c_double), intent(out) :: f(3)
+    end function
+    integer(c_int) function cans_fillps(ctx, n, dli, dzfi, dti, u, v, w, p) bind(C, name='cans_fillps')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, dzfi, u, v, w, p
+      integer(c_int), intent(in) :: n(3)
+      real(c_double), intent(in) :: dli(3)
+      real(c_double), value :: dti
+    end function
+    integer(c_int) function cans_correc(ctx, n, dli, dzci, dt, p, u, v, w) bind(C, name='cans_correc')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, dzci, p, u, v, w
+      integer(c_int), intent(in) :: n(3)
+      real(c_double), intent(in) :: dli(3)
+      real(c_double), value :: dt
+    end function
+    integer(c_int) function cans_updatep(ctx, n, dli, dzci, dzfi, alpha, pp, p, is_impdiff) &
+                   bind(C, name='cans_updatep')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, dzci, dzfi, pp, p
+      integer(c_int), intent(in) :: n(3)
+      real(c_double), intent(in) :: dli(3)
+      real(c_double), value :: alpha
+      integer(c_int), value :: is_impdiff
+    end function
+    integer(c_int) function cans_bounduvw(ctx, cbc, n, bc, nb, is_bound, dl, dzc, dzf, u, v, w, &
+                                          keep_norm, halos_norm, halos_tang) bind(C, name='cans_bounduvw')
+      import :: c_int, c_ptr, c_double, c_char
+      type(c_ptr), value :: ctx, u, v, w
+      character(kind=c_char), intent(in) :: cbc(18)
+      integer(c_int), intent(in) :: n(3), nb(6), is_bound(6)
+      real(c_double), intent(in) :: bc(18), dl(3), dzc(*), dzf(*)
+      integer(c_int), value :: keep_norm, halos_norm, halos_tang
+    end function
+    integer(c_int) function cans_boundp(ctx, cbc, n, bc, nb, is_bound, dl, dzc, p, halo_side) &
+                   bind(C, name='cans_boundp')
+      import :: c_int, c_ptr, c_double, c_char
+      type(c_ptr), value :: ctx, p
+      character(kind=c_char), intent(in) :: cbc(6)
+      integer(c_int), intent(in) :: n(3), nb(6), is_bound(6)
+      real(c_double), intent(in) :: bc(6), dl(3), dzc(*)
+      integer(c_int), value :: halo_side
+    end function
+    integer(c_int) function cans_updt_rhs_b(ctx, c_or_f, cbc, n, is_bound, rhsbx, rhsby, rhsbz, p, alpha) &
+                   bind(C, name='cans_updt_rhs_b')
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: ctx, rhsbx, rhsby, rhsbz, p, alpha
+      character(kind=c_char), intent(in) :: c_or_f(3), cbc(6)
+      integer(c_int), intent(in) :: n(3), is_bound(6)
+    end function
+    integer(c_int) function cans_chkdiv(ctx, lo, hi, l, dli, dzfi, u, v, w, divtot, divmax) &
+                   bind(C, name='cans_chkdiv')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, dzfi, u, v, w
+      integer(c_int), intent(in) :: lo(3), hi(3)
+      real(c_double), intent(in) :: l(3), dli(3)
+      real(c_double), intent(out) :: divtot, divmax
+    end function
+    integer(c_int) function cans_chkdt(ctx, n, dl, dzci, dzfi, visc, alpha, u, v, w, is_impdiff, is_impdiff_1d, &
+                                       dtmax) bind(C, name='cans_chkdt')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, dzci, dzfi, u, v, w
+      integer(c_int), intent(in) :: n(3)
+      real(c_double), intent(in) :: dl(3)
+      real(c_double), value :: visc, alpha
+      integer(c_int), value :: is_impdiff, is_impdiff_1d
+      real(c_double), intent(out) :: dtmax
+    end function
+    integer(c_int) function cans_bulk_mean(ctx, n, grid_vol_ratio, p, mean) bind(C, name='cans_bulk_mean')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, grid_vol_ratio, p
+      integer(c_int), intent(in) :: n(3)
+      real(c_double), intent(out) :: mean
+    end function
+    integer(c_int) function cans_rk3_substep(ctx, prm, irk, dt, st) bind(C, name='cans_rk3_substep')
+      import :: c_int, c_ptr, c_double, cans_step_params, cans_state
+      type(c_ptr), value :: ctx
+      type(cans_step_params), intent(in) :: prm
+      integer(c_int), value :: irk
+      real(c_double), value :: dt
+      type(cans_state), intent(inout) :: st
+    end function
+  end interface
+contains
+  subroutine cans_check(istat, what)
+    integer(c_int), intent(in) :: istat
+    character(len=*), intent(in) :: what
+    character(kind=c_char), pointer :: msg(:)
+    integer :: i
+    if(istat == 0) return
+    call c_f_pointer(cans_last_error(), msg, [1024])
+    write(*,'(a)', advance='no') 'ERROR: '//what//' failed: '
+    do i=1,1024
+      if(msg(i) == c_null_char) exit
+      write(*,'(a)', advance='no') msg(i)
+    end do
+    write(*,*)
+    error stop
+  end subroutine cans_check
+end module cans_hip

@@ -54,6 +54,13 @@ int cans_field_free(cans_ctx *ctx, void *field_d);
 int cans_field_upload(cans_ctx *ctx, void *field_d, const double *host_h);
 int cans_field_download(cans_ctx *ctx, const void *field_d, double *host_h);
 
+/* plain device buffers for the small arrays (dzci, lambdaxy, a, b, c, ...): what
+ * `!$acc enter data copyin(...)` provides to the reference (src/main.f90:233-337) */
+int cans_device_alloc(cans_ctx *ctx, long bytes, void **ptr_d);
+int cans_device_free(cans_ctx *ctx, void *ptr_d);
+int cans_memcpy_h2d(cans_ctx *ctx, void *dst_d, const void *src_h, long bytes);
+int cans_memcpy_d2h(cans_ctx *ctx, void *dst_h, const void *src_d, long bytes);
+
 /* ---------------------------------------------------------------------------
  * Host-side set-up helpers, for hosts that do not keep the Fortran versions.
  * ------------------------------------------------------------------------- */
