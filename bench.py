@@ -1,15 +1,22 @@
-"""Benchmark: ns/cell per RK3 step and Poisson-solve GB/s on one MI355X.
+"""Benchmark: ns/cell per RK3 step and Poisson-solve GB/s on one MI355X, mixed vs FP64.
 
-Workload: the periodic channel of 512 x 256 x 192 cells (Lx = 4 pi, Ly = 2 pi,
-Lz = 1, tanh-stretched walls with factor 1.8, Re_b = 5600, constant flow rate),
-FP64, dt = 2e-4, synthetic seeded initial state (laminar profile + 10 % noise).
-A step is one full RK3 step (3 substeps) through the fused C-ABI call; inputs
-are resident in HBM.  The Poisson solve is timed separately with HIP events on
-the context stream (B_P = 80 N bytes per FP64 solve: five passes, 8 B in + 8 B
-out per cell each).  The oracle (C/OpenMP restatement) is timed on the host on
-a bounded number of full steps of the same case as the CPU baseline.
+Workload: the periodic channel of 1024 x 512 x 384 cells (Lx = 4 pi, Ly = 2 pi,
+Lz = 1, tanh-stretched walls with factor 1.8, Re_b = 5600, constant flow rate,
+dt = 2e-4), synthetic seeded initial state (laminar profile + 10 % noise), run
+twice from the same state: in the mixed mode (velocity stored as FP64; p, pp,
+RHS_old, the RK increment and the transform passes stored as FP32; every kernel
+computing in FP64) and in plain FP64.  A step is one full RK3 step (3
+substeps) through the fused C-ABI call; inputs are resident in HBM.  `value` is
+the mixed mode's ns/cell/step; the FP64 figures, the speed-up and the relative
+L-inf difference of the final fields (mixed vs FP64) are reported beside it.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--ng N1 N2 N3] [--no-cpu]
+The Poisson solve is timed on its own with HIP events on the context stream:
+B_P = 5 passes x N cells x (s_in + s_out) = 40 N bytes with FP32 storage, 80 N
+with FP64 storage.  The oracle (C/OpenMP restatement) is timed on the host on a
+bounded number of full steps of the same case as the CPU baseline.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--ng N1 N2 N3]
+                       [--precision both|mixed|fp64] [--solves S] [--no-cpu]
 """
 import argparse
 import json
@@ -22,32 +29,27 @@ sys.path.insert(0, ROOT)
 
 METRIC = "ns/cell per RK3 step + Poisson GB/s (% HBM peak), mixed vs FP64, 1 GPU"
 HBM_PEAK_GBS = 8000.0
+DT = 2e-4
 
 
-def cpu_baseline(p, budget_s=20.0):
+def cpu_baseline(p, init, budget_s=20.0):
     """Oracle steps on the host (rank 0 only): ns/cell per RK3 step."""
     import numpy as np
     from tests import cases
     from oracle import oracle as O
     orc = cases.oracle_case(p)
-
-    class G:
-        pass
-
-    g = G()
-    g.dl, g.l, g.zc, g.dzf = orc.dl, orc.l, orc.zc, orc.dzf
-    u, v, w, pr = cases.initial_state("channel", p, g, seed=1)
     st = orc.new_state()
-    for k, q in zip("uvwp", (u, v, w, pr)):
+    for k, q in zip("uvwp", init):
         st[k][1:-1, 1:-1, 1:-1] = q
     orc.init_bounds(st)
-    orc.rk3_step(st, 2e-4)  # warm-up (page-in, FFT tables)
+    orc.rk3_step(st, DT)  # warm-up (page-in, FFT tables)
     times = []
     t_all = time.perf_counter()
     while True:
         t0 = time.perf_counter()
-        orc.rk3_step(st, 2e-4)
+        orc.rk3_step(st, DT)
         times.append(time.perf_counter() - t0)
+        print("[cpu] oracle step %d: %.2f s" % (len(times), times[-1]), file=sys.stderr, flush=True)
         if time.perf_counter() - t_all > budget_s or len(times) >= 10:
             break
     n = int(np.prod(p["ng"]))
@@ -58,45 +60,36 @@ def cpu_baseline(p, budget_s=20.0):
                       % (len(times), *p["ng"])}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--ng", type=int, nargs=3, default=(512, 256, 192))
-    ap.add_argument("--solves", type=int, default=10)
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; multi-GPU runs (--gpus %d) are not part of it" % args.gpus,
-              file=sys.stderr)
-        sys.exit(2)
-
+def run(p, init, precision, args):
+    """Time K steps and S Poisson solves in one precision mode; return figures and final fields."""
     import numpy as np
     import torch
+    from cans_amd import _lib as LB
     from cans_amd.host import Simulation
     from cans_amd import initflow as IF
     from tests import cases
-
-    p = cases.params("channel", ng=tuple(args.ng))
-    sim = Simulation(cases.host_case(p))
-    u, v, w, pr = cases.initial_state("channel", p, sim.grid, seed=1)
-    sim.set_state(*(IF.halo(p["ng"], q) for q in (u, v, w, pr)))
+    label = "mixed" if precision == LB.CANS_MIXED else "fp64"
+    print("[bench] %s: set-up" % label, file=sys.stderr, flush=True)
+    sim = Simulation(cases.host_case(p), precision=precision)
+    sim.set_state(*(IF.halo(p["ng"], q) for q in init))
     sim.init_bounds()
-    dt = 2e-4
     ncell = int(np.prod(p["ng"]))
     for _ in range(args.warmup):
-        sim.rk3_step(dt)
+        sim.rk3_step(DT)
     torch.cuda.synchronize()
+    # the context runs on torch's current stream, so these events bracket its work
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     e0.record()
     for _ in range(args.steps):
-        sim.rk3_step(dt)
+        sim.rk3_step(DT)
     e1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_step = e0.elapsed_time(e1) / args.steps
+    print("[bench] %s: %.3f ms/step" % (label, ms_step), file=sys.stderr, flush=True)
+    final = {k: getattr(sim, k).interior() for k in "uvwp"}
+    tot, divmax = sim.chkdiv()
     # Poisson solve alone (same stream), repeated on the last right-hand side
     s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     sim.solver()
@@ -107,10 +100,21 @@ def main():
     s1.record()
     torch.cuda.synchronize()
     t_p = s0.elapsed_time(s1) / args.solves * 1e-3
-    b_p = 80.0 * ncell
+    b_p = 5 * 2 * sim.ctx.p_es * ncell
     gbs = b_p / t_p / 1e9
-    # device copy of a known size: the box's streaming bandwidth (and the reference
-    # point that scales the FETCH_SIZE / WRITE_SIZE counters of the kernels)
+    sim.close()
+    del sim
+    torch.cuda.empty_cache()
+    res = {"ms_per_step": ms_step, "ns_per_cell_step": ms_step * 1e-3 / ncell * 1e9, "wall_s": wall,
+           "divmax": divmax,
+           "poisson": {"ms": t_p * 1e3, "GBps": gbs, "pct_hbm_peak": 100.0 * gbs / HBM_PEAK_GBS,
+                       "algorithmic_bytes": b_p}}
+    return res, final
+
+
+def copy_bandwidth():
+    """Device copy of a known size: the box's streaming bandwidth."""
+    import torch
     nbytes = 1 << 30
     src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
     dst = torch.empty_like(src)
@@ -122,35 +126,87 @@ def main():
         dst.copy_(src)
     c1.record()
     torch.cuda.synchronize()
-    copy_gbs = 2 * nbytes * 5 / (c0.elapsed_time(c1) * 1e-3) / 1e9
+    gbs = 2 * nbytes * 5 / (c0.elapsed_time(c1) * 1e-3) / 1e9
     del src, dst
-    tot, divmax = sim.chkdiv()
+    torch.cuda.empty_cache()
+    return gbs
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--ng", type=int, nargs=3, default=(1024, 512, 384))
+    ap.add_argument("--precision", choices=("both", "mixed", "fp64"), default="both")
+    ap.add_argument("--solves", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; multi-GPU runs (--gpus %d) are not part of it" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from cans_amd import _lib as LB
+    from cans_amd.host import Grid
+    from tests import cases
+
+    p = cases.params("channel", ng=tuple(args.ng))
+    ncell = int(np.prod(p["ng"]))
+    grid = Grid(cases.host_case(p))
+    # one seeded state, FP32-representable so that both modes start from the same values
+    init = [q.astype(np.float32).astype(np.float64) for q in cases.initial_state("channel", p, grid, seed=1)]
+    modes = {"both": ("mixed", "fp64"), "mixed": ("mixed",), "fp64": ("fp64",)}[args.precision]
+    res, final = {}, {}
+    for m in modes:
+        res[m], final[m] = run(p, init, LB.CANS_MIXED if m == "mixed" else LB.CANS_FP64, args)
+    head = res[modes[0]]
+    head_mode = modes[0]
+    wl = ("periodic channel %dx%dx%d, stretched z (gr=1.8), Re_b=5600, dt=2e-4; %s"
+          % (tuple(p["ng"]) + ("mixed precision (FP32 pressure-side storage, FP64 velocity and arithmetic) vs FP64"
+                               if len(modes) == 2 else head_mode,)))
+    pois = head["poisson"]
     out = {
         "metric": METRIC,
-        "value": ms_step * 1e-3 / ncell * 1e9,
+        "value": head["ns_per_cell_step"],
         "unit": "ns/cell/step",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded laminar profile + 10% noise)",
-        "config": {"workload": "periodic channel %dx%dx%d, stretched z (gr=1.8), Re_b=5600, dt=2e-4, FP64"
-                               % tuple(p["ng"]), "cells": ncell, "precision": "fp64"},
-        "poisson": {"ms": t_p * 1e3, "GBps": gbs, "pct_hbm_peak": 100.0 * gbs / HBM_PEAK_GBS,
-                    "algorithmic_bytes": b_p},
-        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "Poisson solve (x-fwd, y-fwd, z-Thomas, y-bwd, x-bwd passes)"},
-        "copy_GBps": copy_gbs,
-        "divmax": divmax,
-        "wall_s": wall,
+        "config": {"workload": wl, "cells": ncell, "precision": head_mode,
+                   "storage": ("velocity f64; p, pp, RHS_old, RK increment and transform passes f32"
+                               if head_mode == "mixed" else "f64")},
+        "poisson": pois,
+        "roofline": {"bound": "hbm", "achieved": pois["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": pois["GBps"] / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "Poisson solve (x-fwd, y-fwd, z-Thomas, y-bwd, x-bwd passes), %s storage"
+                               % ("f32" if head_mode == "mixed" else "f64")},
     }
+    for m in modes:
+        out[m] = res[m]
+    if len(modes) == 2:
+        out["speedup_mixed_vs_fp64"] = res["fp64"]["ms_per_step"] / res["mixed"]["ms_per_step"]
+        err = {}
+        a, b = final["mixed"], final["fp64"]
+        for k in "uvw":
+            err[k] = float(np.abs(a[k] - b[k]).max() / np.abs(b[k]).max())
+        wgt = np.broadcast_to(1 / grid.dzfi[1:-1], a["p"].shape)
+        da = a["p"] - (a["p"] * wgt).sum() / wgt.sum()
+        db = b["p"] - (b["p"] * wgt).sum() / wgt.sum()
+        err["p_gauge_fixed"] = float(np.abs(da - db).max() / np.abs(db).max())
+        err["steps"] = args.warmup + args.steps
+        out["mixed_vs_fp64_rel_linf"] = err
+    del final
+    out["copy_GBps"] = copy_bandwidth()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(p)
+        out["cpu_baseline"] = cpu_baseline(p, init)
     print(json.dumps(out))
 
 

@@ -13,6 +13,7 @@ LIBPATH = os.path.join(HERE, "libcans_hip.so")
 
 CANS_OK, CANS_EINVAL, CANS_EHIP, CANS_ENOMEM, CANS_EUNSUPPORTED = 0, -1, -2, -3, -4
 CANS_FP64, CANS_MIXED = 0, 1
+CANS_FIELD_VELOCITY, CANS_FIELD_PRESSURE = 0, 1
 HALO_ABSENT, HALO_LOWER, HALO_NONE, HALO_UPPER = -99, -1, 0, 1
 
 _vp = C.c_void_p
@@ -46,7 +47,9 @@ SIGNATURES = {
     "cans_ctx_layout": (C.c_int, [_vp, C.POINTER(C.c_long), C.POINTER(C.c_long), C.POINTER(C.c_long)]),
     "cans_ctx_sync": (C.c_int, [_vp]),
     "cans_last_error": (C.c_char_p, []),
+    "cans_ctx_elem_size": (C.c_int, [_vp, C.c_int, _ip]),
     "cans_field_alloc": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "cans_field_alloc_kind": (C.c_int, [_vp, C.c_int, C.POINTER(_vp)]),
     "cans_field_free": (C.c_int, [_vp, _vp]),
     "cans_field_upload": (C.c_int, [_vp, _vp, _dp]),
     "cans_field_download": (C.c_int, [_vp, _vp, _dp]),

@@ -1,5 +1,8 @@
 // C-ABI entry points: argument checking, boundary-plane planning (host) and
-// launch sequencing.  Kernels live in stencil.hip and solver.hip.
+// launch sequencing.  Kernels live in stencil.hip and solver.hip.  Velocity
+// fields are double; pressure-like arrays (p, pp, dudtrko) have the context's
+// scalar storage type (double for CANS_FP64, float for CANS_MIXED), bound to T
+// by CANS_DISPATCH.
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -15,17 +18,17 @@ namespace {
 
 bool same_n(const cans_ctx* c, const int* n) { return n && n[0] == c->L.n1 && n[1] == c->L.n2 && n[2] == c->L.n3; }
 
-#define CHECK_CTX(name)                                                   \
-  if (!c) return fail(CANS_EINVAL, std::string(name) + ": null context"); \
-  if (c->precision != CANS_FP64)                                          \
-    return fail(CANS_EUNSUPPORTED, std::string(name) + ": this entry point takes FP64 fields");
+#define CHECK_CTX(name) \
+  if (!c) return fail(CANS_EINVAL, std::string(name) + ": null context");
 #define CHECK_N(name, n) \
   if (!same_n(c, n)) return fail(CANS_EINVAL, std::string(name) + ": n differs from the context grid");
 
 // set_bc (bound.f90:182-463) for one face of one field -> plane op
-PlaneOp make_setbc(double* f, char ctype, int ib, int idir, bool centered, double rvalue, double dr, int nd) {
+PlaneOp make_setbc(void* f, char ctype, int ib, int idir, bool centered, double rvalue, double dr, int nd) {
   PlaneOp op;
   op.f = f;
+  op.base = nullptr;
+  op.fadd = nullptr;
   op.dir = idir;
   op.side = ib;
   op.nd = nd;
@@ -47,9 +50,11 @@ PlaneOp make_setbc(double* f, char ctype, int ib, int idir, bool centered, doubl
   return op;
 }
 
-PlaneOp make_copy(double* f, int idir, int side, int nd) {
+PlaneOp make_copy(void* f, int idir, int side, int nd) {
   PlaneOp op;
   op.f = f;
+  op.base = nullptr;
+  op.fadd = nullptr;
   op.dir = idir;
   op.side = side;
   op.nd = nd;
@@ -62,7 +67,7 @@ void push(PlaneOps& ops, const PlaneOp& op) { ops.op[ops.nops++] = op; }
 
 // halo exchange of one field along a decomposed direction (bound.f90:600-713):
 // on one GPU it is a periodic copy when the neighbour exists (periodic), else nothing
-void push_exchange(PlaneOps& ops, double* f, int idir, int nd, int nb0, int nb1, int halo_side) {
+void push_exchange(PlaneOps& ops, void* f, int idir, int nd, int nb0, int nb1, int halo_side) {
   if (idir == 0) return;  // x is the pencil axis: no exchange
   bool up_lower = true, up_upper = true;
   if (halo_side == CANS_HALO_LOWER) up_upper = false;
@@ -72,9 +77,12 @@ void push_exchange(PlaneOps& ops, double* f, int idir, int nd, int nb0, int nb1,
   if (up_lower && nb0 >= 0) push(ops, make_copy(f, idir, 0, nd));
 }
 
+// bounduvw on (u,v,w) (double), or, with base != nullptr, on base + (u,v,w) + fadd[comp]
+// where u,v,w are increments of the scalar storage type (fadd: device vector of 3 or nullptr)
 int plan_bounduvw(cans_ctx* c, const char* cbc, const double* bc, const int* nb, const int* is_bound,
-                  const double* dl, const double* dzc, const double* dzf, double* u, double* v, double* w,
-                  int keep_norm_values, int halos_norm, int halos_tang) {
+                  const double* dl, const double* dzc, const double* dzf, void* u, void* v, void* w,
+                  int keep_norm_values, int halos_norm, int halos_tang, const double* const* base = nullptr,
+                  const double* fadd = nullptr) {
   const Layout& L = c->L;
   const int n[3] = {L.n1, L.n2, L.n3};
 #define CBC(ib, d, cc) cbc[(ib) + 2 * (d) + 6 * (cc)]
@@ -84,7 +92,7 @@ int plan_bounduvw(cans_ctx* c, const char* cbc, const double* bc, const int* nb,
   if (halos_norm == CANS_HALO_LOWER) updt_norm[1] = false;
   else if (halos_norm == CANS_HALO_NONE) updt_norm[0] = updt_norm[1] = false;
   else if (halos_norm == CANS_HALO_UPPER) updt_norm[0] = false;
-  double* f[3] = {u, v, w};
+  void* f[3] = {u, v, w};
   // The reference exchanges y then z halos first and applies set_bc x, y, z next.
   // Periodic copies commute with the other directions' plane updates (each
   // direction's update reads only interior-or-final values along that direction),
@@ -107,7 +115,15 @@ int plan_bounduvw(cans_ctx* c, const char* cbc, const double* bc, const int* nb,
         push(ops, make_setbc(f[cc], CBC(ib, d, cc), ib, d, !normal, BCV(ib, d, cc), dr, n[d]));
       }
     }
-    int rc = launch_bound_dir(c, ops);
+    if (base) {
+      for (int q = 0; q < ops.nops; q++)
+        for (int cc = 0; cc < 3; cc++)
+          if (ops.op[q].f == f[cc]) {
+            ops.op[q].base = base[cc];
+            ops.op[q].fadd = fadd ? fadd + cc : nullptr;
+          }
+    }
+    int rc = base ? CANS_DISPATCH(c, launch_bound_dir<T>(c, ops)) : launch_bound_dir<double>(c, ops);
     if (rc) return rc;
   }
 #undef CBC
@@ -116,7 +132,7 @@ int plan_bounduvw(cans_ctx* c, const char* cbc, const double* bc, const int* nb,
 }
 
 int plan_boundp(cans_ctx* c, const char* cbc, const double* bc, const int* nb, const int* is_bound, const double* dl,
-                const double* dzc, double* p, int halo_side) {
+                const double* dzc, void* p, int halo_side) {
   const Layout& L = c->L;
   const int n[3] = {L.n1, L.n2, L.n3};
   bool update_halo[2] = {true, true};
@@ -132,14 +148,14 @@ int plan_boundp(cans_ctx* c, const char* cbc, const double* bc, const int* nb, c
       const double dr = d < 2 ? dl[d] : (ib == 0 ? dzc[0] : dzc[L.n3]);
       push(ops, make_setbc(p, cbc[ib + 2 * d], ib, d, true, bc[ib + 2 * d], dr, n[d]));
     }
-    int rc = launch_bound_dir(c, ops);
+    int rc = CANS_DISPATCH(c, launch_bound_dir<T>(c, ops));
     if (rc) return rc;
   }
   return CANS_OK;
 }
 
 int do_updt_rhs_b(cans_ctx* c, const char* c_or_f, const char* cbc, const int* is_bound, const double* rx,
-                  const double* ry, const double* rz, double* p, double norm) {
+                  const double* ry, const double* rz, void* p, double norm) {
   const Layout& L = c->L;
   const int n[3] = {L.n1, L.n2, L.n3};
   const double* r[3] = {rx, ry, rz};
@@ -148,18 +164,22 @@ int do_updt_rhs_b(cans_ctx* c, const char* c_or_f, const char* cbc, const int* i
     const int q = (c_or_f[d] == 'f' && cbc[1 + 2 * d] == 'D') ? 1 : 0;
     const long plane = (long)(d == 0 ? L.n2 : L.n1) * (d == 2 ? L.n2 : L.n3);
     int rc;
-    if (is_bound[2 * d] && (rc = launch_rhs_b(c, p, r[d], d, 1, norm))) return rc;
-    if (is_bound[2 * d + 1] && (rc = launch_rhs_b(c, p, r[d] + plane, d, n[d] - q, norm))) return rc;
+    if (is_bound[2 * d] && (rc = CANS_DISPATCH(c, launch_rhs_b<T>(c, (T*)p, r[d], d, 1, norm)))) return rc;
+    if (is_bound[2 * d + 1] &&
+        (rc = CANS_DISPATCH(c, launch_rhs_b<T>(c, (T*)p, r[d] + plane, d, n[d] - q, norm))))
+      return rc;
   }
   return CANS_OK;
 }
 
-int ensure_tmp_fields(cans_ctx* c, double** t) {
+// three halo'd scratch fields of element type T (room for double), aligned like cans_field_alloc's
+template <typename T>
+int ensure_tmp_fields(cans_ctx* c, T** t) {
   void* base = nullptr;
   const size_t fe = (size_t)c->L.sz * (c->L.n3 + 2);
   int rc = ensure_work(c, 0, 3 * fe * sizeof(double) + 256, &base);
   if (rc) return rc;
-  for (int k = 0; k < 3; k++) t[k] = (double*)((char*)base + 256 - sizeof(double)) + k * fe;
+  for (int k = 0; k < 3; k++) t[k] = (T*)((char*)base + 256 - sizeof(T)) + k * fe;
   return CANS_OK;
 }
 
@@ -167,33 +187,28 @@ const double RKCOEFF[3][2] = {{32. / 60., 0.}, {25. / 60., -17. / 60.}, {45. / 6
 
 }  // namespace
 
-struct cans_rk_state {
-  bool first = true;
-};
-static cans_rk_state* rk_state(cans_ctx* c) {
-  // per-context "first call" flag of rk.f90:72-100 (zeroes RHS_old once)
-  static std::mutex mu;
-  static std::map<cans_ctx*, cans_rk_state> m;
-  std::lock_guard<std::mutex> lk(mu);
-  return &m[c];
-}
-
+// momentum + RK update; the result goes to (uo,vo,wo) or, when eo is given, as
+// increments over (u,v,w) to eo[0..2]
+template <typename S>
 static int rk_core(cans_ctx* c, const double* rkpar, const double* dli, const double* dzci, const double* dzfi,
-                   const double* gvr_c, const double* gvr_f, double dt, double visc, const double* p,
-                   const int* is_forced, const double* velf, const double* bforce, double* ru, double* rv,
-                   double* rw, const double* u, const double* v, const double* w, double* uo, double* vo,
-                   double* wo, int write_ro = 1) {
-  cans_rk_state* st = rk_state(c);
-  if (st->first) {
-    st->first = false;
-    const size_t bytes = (size_t)c->L.cells() * sizeof(double);
+                   const double* gvr_c, const double* gvr_f, double dt, double visc, const S* p,
+                   const int* is_forced, const double* velf, const double* bforce, S* ru, S* rv, S* rw,
+                   const double* u, const double* v, const double* w, double* uo, double* vo, double* wo,
+                   int write_ro = 1, S* const* eo = nullptr, S* divu = nullptr) {
+  if (c->rk_first) {  // rk.f90:72-100: RHS_old starts at zero on the first call
+    c->rk_first = false;
+    const size_t bytes = (size_t)c->L.cells() * sizeof(S);
     CANS_HIP_TRY(hipMemsetAsync(ru, 0, bytes, c->stream));
     CANS_HIP_TRY(hipMemsetAsync(rv, 0, bytes, c->stream));
     CANS_HIP_TRY(hipMemsetAsync(rw, 0, bytes, c->stream));
   }
-  MomLaunch m;
+  MomLaunch<S> m;
   m.u = u; m.v = v; m.w = w; m.p = p;
   m.uo = uo; m.vo = vo; m.wo = wo;
+  if (eo) {
+    m.eu = eo[0]; m.ev = eo[1]; m.ew = eo[2];
+  }
+  m.divu = divu;
   m.ru = ru; m.rv = rv; m.rw = rw;
   m.dzci = dzci; m.dzfi = dzfi; m.gvr_c = gvr_c; m.gvr_f = gvr_f;
   m.dxi = dli[0]; m.dyi = dli[1]; m.visc = visc; m.dt = dt;
@@ -205,7 +220,7 @@ static int rk_core(cans_ctx* c, const double* rkpar, const double* dli, const do
     m.velf[d] = velf[d];
     m.is_forced[d] = is_forced[d];
   }
-  return launch_mom_rk(c, m);
+  return launch_mom_rk<S>(c, m);
 }
 
 extern "C" int cans_rk(cans_ctx* c, const double* rkpar, const int* n, const double* dli, const double* dzci,
@@ -217,16 +232,19 @@ extern "C" int cans_rk(cans_ctx* c, const double* rkpar, const int* n, const dou
   if (!rkpar || !dli || !dzci || !dzfi || !gvr_c || !gvr_f || !p || !is_forced || !velf || !bforce || !ru || !rv ||
       !rw || !u || !v || !w)
     return fail(CANS_EINVAL, "cans_rk: null argument");
-  double* t[3];
-  int rc = ensure_tmp_fields(c, t);
+  int rc = CANS_DISPATCH(c, [&]() -> int {
+    double* t[3];
+    int r = ensure_tmp_fields<double>(c, t);
+    if (r) return r;
+    r = rk_core<T>(c, rkpar, dli, dzci, dzfi, gvr_c, gvr_f, dt, visc, (const T*)p, is_forced, velf, bforce, (T*)ru,
+                   (T*)rv, (T*)rw, (const double*)u, (const double*)v, (const double*)w, t[0], t[1], t[2]);
+    if (r) return r;
+    double* dst[3] = {(double*)u, (double*)v, (double*)w};
+    for (int k = 0; k < 3; k++)
+      if ((r = launch_add_forcing<double>(c, t[k], dst[k], k, is_forced[k] ? 1 : 0))) return r;
+    return CANS_OK;
+  }());
   if (rc) return rc;
-  rc = rk_core(c, rkpar, dli, dzci, dzfi, gvr_c, gvr_f, dt, visc, (const double*)p, is_forced, velf, bforce,
-               (double*)ru, (double*)rv, (double*)rw, (const double*)u, (const double*)v, (const double*)w, t[0],
-               t[1], t[2]);
-  if (rc) return rc;
-  double* dst[3] = {(double*)u, (double*)v, (double*)w};
-  for (int k = 0; k < 3; k++)
-    if ((rc = launch_add_forcing(c, t[k], dst[k], k, is_forced[k] ? 1 : 0))) return rc;
   if (f_h) {
     CANS_HIP_TRY(hipMemcpyAsync(f_h, c->dscal, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     CANS_HIP_TRY(hipStreamSynchronize(c->stream));
@@ -239,7 +257,9 @@ extern "C" int cans_fillps(cans_ctx* c, const int* n, const double* dli, const d
   CHECK_CTX("cans_fillps");
   CHECK_N("cans_fillps", n);
   if (!dli || !dzfi || !u || !v || !w || !p) return fail(CANS_EINVAL, "cans_fillps: null argument");
-  return launch_fillps(c, (const double*)u, (const double*)v, (const double*)w, (double*)p, dzfi, dti, dli);
+  return CANS_DISPATCH(c, launch_fillps<T>(c, (const double*)u, (const double*)v, (const double*)w, (const T*)nullptr,
+                                            (const T*)nullptr, (const T*)nullptr, nullptr, (T*)p, dzfi, dti,
+                                            dli));
 }
 
 extern "C" int cans_correc(cans_ctx* c, const int* n, const double* dli, const double* dzci, double dt,
@@ -247,8 +267,9 @@ extern "C" int cans_correc(cans_ctx* c, const int* n, const double* dli, const d
   CHECK_CTX("cans_correc");
   CHECK_N("cans_correc", n);
   if (!dli || !dzci || !p || !u || !v || !w) return fail(CANS_EINVAL, "cans_correc: null argument");
-  return launch_correc(c, (const double*)p, (double*)u, (double*)v, (double*)w, (double*)u, (double*)v, (double*)w,
-                       dzci, dt, dli);
+  return CANS_DISPATCH(c, launch_correc<T>(c, (const T*)p, (const double*)u, (const double*)v, (const double*)w,
+                                            (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, nullptr,
+                                            (double*)u, (double*)v, (double*)w, dzci, dt, dli, (T*)nullptr));
 }
 
 extern "C" int cans_updatep(cans_ctx* c, const int* n, const double* dli, const double* dzci, const double* dzfi,
@@ -256,7 +277,7 @@ extern "C" int cans_updatep(cans_ctx* c, const int* n, const double* dli, const 
   CHECK_CTX("cans_updatep");
   CHECK_N("cans_updatep", n);
   if (!dli || !dzci || !dzfi || !pp || !p) return fail(CANS_EINVAL, "cans_updatep: null argument");
-  return launch_updatep(c, (const double*)pp, (double*)p, dzci, dzfi, alpha, dli, is_impdiff);
+  return CANS_DISPATCH(c, launch_updatep<T>(c, (const T*)pp, (T*)p, dzci, dzfi, alpha, dli, is_impdiff));
 }
 
 extern "C" int cans_bounduvw(cans_ctx* c, const char* cbc, const int* n, const double* bc, const int* nb,
@@ -266,8 +287,7 @@ extern "C" int cans_bounduvw(cans_ctx* c, const char* cbc, const int* n, const d
   CHECK_N("cans_bounduvw", n);
   if (!cbc || !bc || !nb || !is_bound || !dl || !dzc || !dzf || !u || !v || !w)
     return fail(CANS_EINVAL, "cans_bounduvw: null argument");
-  return plan_bounduvw(c, cbc, bc, nb, is_bound, dl, dzc, dzf, (double*)u, (double*)v, (double*)w, keep_norm,
-                       halos_norm, halos_tang);
+  return plan_bounduvw(c, cbc, bc, nb, is_bound, dl, dzc, dzf, u, v, w, keep_norm, halos_norm, halos_tang);
 }
 
 extern "C" int cans_boundp(cans_ctx* c, const char* cbc, const int* n, const double* bc, const int* nb,
@@ -275,7 +295,7 @@ extern "C" int cans_boundp(cans_ctx* c, const char* cbc, const int* n, const dou
   CHECK_CTX("cans_boundp");
   CHECK_N("cans_boundp", n);
   if (!cbc || !bc || !nb || !is_bound || !dl || !dzc || !p) return fail(CANS_EINVAL, "cans_boundp: null argument");
-  return plan_boundp(c, cbc, bc, nb, is_bound, dl, dzc, (double*)p, halo_side);
+  return plan_boundp(c, cbc, bc, nb, is_bound, dl, dzc, p, halo_side);
 }
 
 extern "C" int cans_updt_rhs_b(cans_ctx* c, const char* c_or_f, const char* cbc, const int* n, const int* is_bound,
@@ -283,7 +303,7 @@ extern "C" int cans_updt_rhs_b(cans_ctx* c, const char* c_or_f, const char* cbc,
   CHECK_CTX("cans_updt_rhs_b");
   CHECK_N("cans_updt_rhs_b", n);
   if (!c_or_f || !cbc || !is_bound || !p) return fail(CANS_EINVAL, "cans_updt_rhs_b: null argument");
-  return do_updt_rhs_b(c, c_or_f, cbc, is_bound, rx, ry, rz, (double*)p, alpha ? *alpha : 1.);
+  return do_updt_rhs_b(c, c_or_f, cbc, is_bound, rx, ry, rz, p, alpha ? *alpha : 1.);
 }
 
 extern "C" int cans_solver(cans_ctx* c, const int* n, const int* ng, const cans_plan* plan, double normfft,
@@ -293,7 +313,7 @@ extern "C" int cans_solver(cans_ctx* c, const int* n, const int* ng, const cans_
   CHECK_N("cans_solver", n);
   CHECK_N("cans_solver", ng);
   if (!plan || !lam || !a || !b || !cc || !bc || !c_or_f || !p) return fail(CANS_EINVAL, "cans_solver: null argument");
-  return run_solver(c, plan, normfft, lam, a, b, cc, bc, c_or_f, (double*)p);
+  return CANS_DISPATCH(c, run_solver<T>(c, plan, normfft, lam, a, b, cc, bc, c_or_f, (T*)p));
 }
 
 extern "C" int cans_solve_helmholtz(cans_ctx* c, const int* n, const int* ng, const cans_plan* plan, double normfft,
@@ -305,7 +325,7 @@ extern "C" int cans_solve_helmholtz(cans_ctx* c, const int* n, const int* ng, co
   if (!plan || !lam || !a || !b || !cc || !is_bound || !cbc || !c_or_f || !p)
     return fail(CANS_EINVAL, "cans_solve_helmholtz: null argument");
   // solve_helmholtz.f90:58-73: rhs boundary terms scaled by alpha, bb = b + 1/alpha, norm/alpha
-  int rc = do_updt_rhs_b(c, c_or_f, cbc, is_bound, rx, ry, rz, (double*)p, alpha);
+  int rc = do_updt_rhs_b(c, c_or_f, cbc, is_bound, rx, ry, rz, p, alpha);
   if (rc) return rc;
   const double alphai = 1. / alpha;
   void* bbv = nullptr;
@@ -317,7 +337,7 @@ extern "C" int cans_solve_helmholtz(cans_ctx* c, const int* n, const int* ng, co
   CANS_HIP_TRY(hipStreamSynchronize(c->stream));
   for (auto& x : hb) x = x + alphai;
   CANS_HIP_TRY(hipMemcpyAsync(bb, hb.data(), hb.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  rc = run_solver(c, plan, normfft * alphai, lam, a, bb, cc, cbc, c_or_f, (double*)p);
+  rc = CANS_DISPATCH(c, run_solver<T>(c, plan, normfft * alphai, lam, a, bb, cc, cbc, c_or_f, (T*)p));
   if (rc) return rc;
   CANS_HIP_TRY(hipStreamSynchronize(c->stream));
   return CANS_OK;
@@ -332,7 +352,7 @@ extern "C" int cans_chkdiv(cans_ctx* c, const int* lo, const int* hi, const doub
   const int n[3] = {hi[0] - lo[0] + 1, hi[1] - lo[1] + 1, hi[2] - lo[2] + 1};
   CHECK_N("cans_chkdiv", n);
   double tot, mx;
-  int rc = run_chkdiv(c, (const double*)u, (const double*)v, (const double*)w, dzfi, dli, &tot, &mx);
+  int rc = run_chkdiv<double>(c, (const double*)u, (const double*)v, (const double*)w, dzfi, dli, &tot, &mx);
   if (rc) return rc;
   *divtot = tot / (l[0] * l[1] * l[2]);
   *divmax = mx;
@@ -346,7 +366,7 @@ extern "C" int cans_chkdt(cans_ctx* c, const int* n, const double* dl, const dou
   CHECK_N("cans_chkdt", n);
   if (!dl || !dzci || !dzfi || !u || !v || !w || !dtmax) return fail(CANS_EINVAL, "cans_chkdt: null argument");
   double dti;
-  int rc = run_chkdt(c, (const double*)u, (const double*)v, (const double*)w, dzci, dzfi, dl, &dti);
+  int rc = run_chkdt<double>(c, (const double*)u, (const double*)v, (const double*)w, dzci, dzfi, dl, &dti);
   if (rc) return rc;
   // dlmin over dl(1:2) and 1/dzfi(0:n3+1) (chkdt.f90:37-44)
   std::vector<double> hz(c->L.n3 + 2);
@@ -366,7 +386,83 @@ extern "C" int cans_bulk_mean(cans_ctx* c, const int* n, const double* gvr, cons
   CHECK_CTX("cans_bulk_mean");
   CHECK_N("cans_bulk_mean", n);
   if (!gvr || !p || !mean) return fail(CANS_EINVAL, "cans_bulk_mean: null argument");
-  return run_bulk_mean(c, (const double*)p, gvr, mean);
+  return run_bulk_mean<double>(c, (const double*)p, gvr, mean);
+}
+
+// One substep.  With S = double (FP64) the updated velocity of rk lives in double
+// scratch fields t, as in the reference.  With S = float (mixed) the scratch holds
+// its increment e over (u,v,w) in FP32 instead: e is ~dt times the right-hand side,
+// so its rounding is far below that of the velocity, which stays FP64 throughout.
+// The intermediate velocity is then u + e + f, with the bulk forcing f kept as a
+// device scalar (no forcing pass), and the momentum kernel also leaves div(u), so
+// the first transform pass forms div(u + e + f) = div(u) + div(e) without reading u.
+template <typename S>
+static int rk3_substep(cans_ctx* c, const cans_step_params* s, int irk, double dt, cans_state* st) {
+  const bool delta = sizeof(S) != sizeof(double);
+  const double* rkpar = RKCOEFF[irk - 1];
+  const double dtrk = (rkpar[0] + rkpar[1]) * dt;  // main.f90:438-439
+  const double dtrki = 1. / dtrk;
+  double* u = (double*)st->u;
+  double* v = (double*)st->v;
+  double* w = (double*)st->w;
+  S* p = (S*)st->p;
+  S* pp = (S*)st->pp;
+  double* t[3];
+  S* e[3];
+  int rc = ensure_tmp_fields<double>(c, t);
+  if (rc) return rc;
+  if ((rc = ensure_tmp_fields<S>(c, e))) return rc;  // same scratch, viewed as S
+  const double* base[3] = {u, v, w};
+  const double* fvec = delta ? c->dscal : nullptr;  // bulk forcing (rk writes it to dscal[0..2])
+  // With homogeneous pressure BCs and a periodic x transform the divergence is formed
+  // inside the first transform pass (fillps -> updt_rhs_b -> solver otherwise)
+  const bool fuse_div = !(s->rhsbx_d || s->rhsby_d || s->rhsbz_d) && s->cbcpre[0] == 'P' && s->cbcpre[1] == 'P' &&
+                        c->L.n1 >= 2 && c->L.n1 % 2 == 0;
+  void* divu_v = nullptr;
+  if (delta && fuse_div && (rc = ensure_work(c, 4, (size_t)c->L.cells() * sizeof(S), &divu_v))) return rc;
+  S* divu = (S*)divu_v;
+  // rk: updated velocity (or its increment) into the scratch; u,v,w keep the old level for the stencil
+  rc = rk_core<S>(c, rkpar, s->dli, s->dzci_d, s->dzfi_d, s->grid_vol_ratio_c_d, s->grid_vol_ratio_f_d, dt,
+                  s->visc, p, s->is_forced, s->velf, s->bforce, (S*)st->dudtrko, (S*)st->dvdtrko, (S*)st->dwdtrko, u,
+                  v, w, t[0], t[1], t[2], irk != 3, delta ? e : nullptr, divu);
+  if (rc) return rc;
+  for (int k = 0; k < 3 && !delta; k++)
+    if (s->is_forced[k] && (rc = launch_add_forcing<double>(c, t[k], t[k], k, 1))) return rc;
+  if (delta)
+    rc = plan_bounduvw(c, s->cbcvel, s->bcvel, s->nb, s->is_bound, s->dl, s->dzc_h, s->dzf_h, e[0], e[1], e[2], 0,
+                       CANS_HALO_LOWER, CANS_HALO_ABSENT, base, fvec);
+  else
+    rc = plan_bounduvw(c, s->cbcvel, s->bcvel, s->nb, s->is_bound, s->dl, s->dzc_h, s->dzf_h, t[0], t[1], t[2], 0,
+                       CANS_HALO_LOWER, CANS_HALO_ABSENT);
+  if (rc) return rc;
+  // the intermediate velocity: t, or (u,v,w) + increments e + forcing f
+  const double* su = delta ? u : t[0];
+  const double* sv = delta ? v : t[1];
+  const double* sw = delta ? w : t[2];
+  const S* eu = delta ? e[0] : nullptr;
+  const S* ev = delta ? e[1] : nullptr;
+  const S* ew = delta ? e[2] : nullptr;
+  if (fuse_div) {
+    DivSrc<S> ds = {su, sv, sw, eu, ev, ew, divu, s->dzfi_d, dtrki, dtrki * s->dli[0], dtrki * s->dli[1],
+                    s->dli[0], s->dli[1]};
+    if ((rc = run_solver<S>(c, s->plan, s->normfft, s->lambdaxy_d, s->a_d, s->b_d, s->c_d, s->cbcpre, "ccc", pp, &ds)))
+      return rc;
+  } else {
+    if ((rc = launch_fillps<S>(c, su, sv, sw, eu, ev, ew, fvec, pp, s->dzfi_d, dtrki, s->dli))) return rc;
+    if (s->rhsbx_d || s->rhsby_d || s->rhsbz_d)
+      if ((rc = do_updt_rhs_b(c, "ccc", s->cbcpre, s->is_bound, s->rhsbx_d, s->rhsby_d, s->rhsbz_d, pp, 1.)))
+        return rc;
+    if ((rc = run_solver<S>(c, s->plan, s->normfft, s->lambdaxy_d, s->a_d, s->b_d, s->c_d, s->cbcpre, "ccc", pp)))
+      return rc;
+  }
+  if ((rc = plan_boundp(c, s->cbcpre, s->bcpre, s->nb, s->is_bound, s->dl, s->dzc_h, pp, CANS_HALO_ABSENT))) return rc;
+  // correc reads the intermediate velocity and writes the caller's fields (whole halo'd
+  // box); updatep (p += pp) is independent of the velocity halos and rides along
+  if ((rc = launch_correc<S>(c, pp, su, sv, sw, eu, ev, ew, fvec, u, v, w, s->dzci_d, dtrk, s->dli, p))) return rc;
+  if ((rc = plan_bounduvw(c, s->cbcvel, s->bcvel, s->nb, s->is_bound, s->dl, s->dzc_h, s->dzf_h, u, v, w, 1,
+                          CANS_HALO_UPPER, CANS_HALO_NONE)))
+    return rc;
+  return plan_boundp(c, s->cbcpre, s->bcpre, s->nb, s->is_bound, s->dl, s->dzc_h, p, CANS_HALO_UPPER);
 }
 
 extern "C" int cans_rk3_substep(cans_ctx* c, const cans_step_params* s, int irk, double dt, cans_state* st) {
@@ -377,49 +473,5 @@ extern "C" int cans_rk3_substep(cans_ctx* c, const cans_step_params* s, int irk,
   if (!s->plan || !s->lambdaxy_d || !s->a_d || !s->b_d || !s->c_d || !s->dzci_d || !s->dzfi_d || !s->dzc_h ||
       !s->dzf_h || !s->grid_vol_ratio_c_d || !s->grid_vol_ratio_f_d)
     return fail(CANS_EINVAL, "cans_rk3_substep: null parameter array");
-  const double* rkpar = RKCOEFF[irk - 1];
-  const double dtrk = (rkpar[0] + rkpar[1]) * dt;  // main.f90:438-439
-  const double dtrki = 1. / dtrk;
-  double* u = (double*)st->u;
-  double* v = (double*)st->v;
-  double* w = (double*)st->w;
-  double* p = (double*)st->p;
-  double* pp = (double*)st->pp;
-  double* t[3];
-  int rc = ensure_tmp_fields(c, t);
-  if (rc) return rc;
-  // rk: updated velocity into the scratch fields t (u,v,w keep the old level for the stencil)
-  rc = rk_core(c, rkpar, s->dli, s->dzci_d, s->dzfi_d, s->grid_vol_ratio_c_d, s->grid_vol_ratio_f_d, dt, s->visc, p,
-               s->is_forced, s->velf, s->bforce, (double*)st->dudtrko, (double*)st->dvdtrko, (double*)st->dwdtrko, u,
-               v, w, t[0], t[1], t[2], irk != 3);
-  if (rc) return rc;
-  for (int k = 0; k < 3; k++)
-    if (s->is_forced[k] && (rc = launch_add_forcing(c, t[k], t[k], k, 1))) return rc;
-  if ((rc = plan_bounduvw(c, s->cbcvel, s->bcvel, s->nb, s->is_bound, s->dl, s->dzc_h, s->dzf_h, t[0], t[1], t[2], 0,
-                          CANS_HALO_LOWER, CANS_HALO_ABSENT)))
-    return rc;
-  // fillps -> updt_rhs_b -> solver.  With homogeneous pressure BCs and a periodic x
-  // transform the divergence is formed inside the first transform pass.
-  const bool fuse_div = !(s->rhsbx_d || s->rhsby_d || s->rhsbz_d) && s->cbcpre[0] == 'P' && s->cbcpre[1] == 'P' &&
-                        c->L.n1 >= 2 && c->L.n1 % 2 == 0;
-  if (fuse_div) {
-    DivSrc ds = {t[0], t[1], t[2], s->dzfi_d, dtrki, dtrki * s->dli[0], dtrki * s->dli[1]};
-    if ((rc = run_solver(c, s->plan, s->normfft, s->lambdaxy_d, s->a_d, s->b_d, s->c_d, s->cbcpre, "ccc", pp, &ds)))
-      return rc;
-  } else {
-    if ((rc = launch_fillps(c, t[0], t[1], t[2], pp, s->dzfi_d, dtrki, s->dli))) return rc;
-    if (s->rhsbx_d || s->rhsby_d || s->rhsbz_d)
-      if ((rc = do_updt_rhs_b(c, "ccc", s->cbcpre, s->is_bound, s->rhsbx_d, s->rhsby_d, s->rhsbz_d, pp, 1.)))
-        return rc;
-    if ((rc = run_solver(c, s->plan, s->normfft, s->lambdaxy_d, s->a_d, s->b_d, s->c_d, s->cbcpre, "ccc", pp)))
-      return rc;
-  }
-  if ((rc = plan_boundp(c, s->cbcpre, s->bcpre, s->nb, s->is_bound, s->dl, s->dzc_h, pp, CANS_HALO_ABSENT))) return rc;
-  // correc reads the scratch velocity and writes the caller's fields (whole halo'd box);
-  // updatep (p += pp) is independent of the velocity halos and rides along
-  if ((rc = launch_correc(c, pp, t[0], t[1], t[2], u, v, w, s->dzci_d, dtrk, s->dli, p))) return rc;
-  if ((rc = plan_bounduvw(c, s->cbcvel, s->bcvel, s->nb, s->is_bound, s->dl, s->dzc_h, s->dzf_h, u, v, w, 1,
-                          CANS_HALO_UPPER, CANS_HALO_NONE)))
-    return rc;
-  return plan_boundp(c, s->cbcpre, s->bcpre, s->nb, s->is_bound, s->dl, s->dzc_h, p, CANS_HALO_UPPER);
+  return CANS_DISPATCH(c, rk3_substep<T>(c, s, irk, dt, st));
 }

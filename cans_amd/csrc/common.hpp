@@ -55,18 +55,28 @@ int fail(int code, const std::string& msg);
 
 inline unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
 
+// Run CALL with T bound to the field storage type of context c: double (CANS_FP64)
+// or float (CANS_MIXED).  Arithmetic stays double in both; only storage differs.
+#define CANS_DISPATCH(c, ...)                                                  \
+  ((c)->precision == CANS_MIXED ? [&]() { using T = float; return __VA_ARGS__; }() \
+                                : [&]() { using T = double; return __VA_ARGS__; }())
+
 }  // namespace cans
 
 // opaque handles
 struct cans_ctx {
   cans::Layout L;
-  int precision;          // CANS_FP64 or CANS_MIXED
+  int precision;          // CANS_FP64, or CANS_MIXED (pressure-like arrays stored as float)
   hipStream_t stream;
   int device;
-  // scratch owned by the context (grown on demand, freed at destroy)
-  void* work[4] = {nullptr, nullptr, nullptr, nullptr};
-  size_t work_bytes[4] = {0, 0, 0, 0};
+  // scratch owned by the context (grown on demand, freed at destroy):
+  // 0 RK intermediate fields, 1 transform work buffer, 2 Thomas checkpoints,
+  // 3 small z tables, 4 divergence of the old velocity (mixed mode)
+  static constexpr int NWORK = 5;
+  void* work[NWORK] = {};
+  size_t work_bytes[NWORK] = {};
   double* dscal = nullptr;     // device scalars: [0..2] forcing f, [3..] reductions
   double* partials = nullptr;  // per-block reduction partials
   size_t partials_n = 0;
+  bool rk_first = true;        // the next rk call is the first on this context
 };

@@ -16,9 +16,19 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
-// origin pointer -> base pointer of every field allocated through a context
+// origin pointer -> (base pointer, element size) of every field allocated through a context
+struct FieldRec {
+  void* base;
+  size_t es;
+};
 static std::mutex g_mu;
-static std::map<void*, void*> g_fields;
+static std::map<void*, FieldRec> g_fields;
+
+// element size of a field kind in a context: velocity is double in every mode;
+// pressure-like arrays are float in the mixed mode
+static size_t kind_size(const cans_ctx* c, int kind) {
+  return (kind == CANS_FIELD_PRESSURE && c->precision == CANS_MIXED) ? sizeof(float) : sizeof(double);
+}
 
 }  // namespace cans
 
@@ -62,7 +72,7 @@ extern "C" int cans_ctx_create(const int* n, int ld, int precision, void* stream
 extern "C" int cans_ctx_destroy(cans_ctx* c) {
   if (!c) return CANS_OK;
   hipStreamSynchronize(c->stream);
-  for (int s = 0; s < 4; s++) hipFree(c->work[s]);
+  for (int s = 0; s < cans_ctx::NWORK; s++) hipFree(c->work[s]);
   hipFree(c->dscal);
   hipFree(c->partials);
   delete c;
@@ -100,17 +110,30 @@ int ensure_work(cans_ctx* c, int slot, size_t bytes, void** ptr) {
     hipError_t e = hipMalloc(&c->work[slot], bytes);
     if (e != hipSuccess) return fail(CANS_ENOMEM, std::string("scratch allocation: ") + hipGetErrorString(e));
     c->work_bytes[slot] = bytes;
+    CANS_HIP_TRY(hipMemsetAsync(c->work[slot], 0, bytes, c->stream));  // defined contents from the start
   }
   *ptr = c->work[slot];
   return CANS_OK;
 }
 }  // namespace cans
 
+extern "C" int cans_ctx_elem_size(const cans_ctx* c, int kind, int* bytes) {
+  if (!c || !bytes) return fail(CANS_EINVAL, "cans_ctx_elem_size: null argument");
+  if (kind != CANS_FIELD_VELOCITY && kind != CANS_FIELD_PRESSURE)
+    return fail(CANS_EINVAL, "cans_ctx_elem_size: unknown field kind");
+  *bytes = (int)kind_size(c, kind);
+  return CANS_OK;
+}
+
 // Fields: the origin (element (0,0,0)) sits one element before a 256-byte
 // boundary so that interior rows start aligned when ld is a multiple of 32.
-extern "C" int cans_field_alloc(cans_ctx* c, void** field) {
+extern "C" int cans_field_alloc(cans_ctx* c, void** field) { return cans_field_alloc_kind(c, CANS_FIELD_VELOCITY, field); }
+
+extern "C" int cans_field_alloc_kind(cans_ctx* c, int kind, void** field) {
   if (!c || !field) return fail(CANS_EINVAL, "cans_field_alloc: null argument");
-  size_t es = sizeof(double);
+  if (kind != CANS_FIELD_VELOCITY && kind != CANS_FIELD_PRESSURE)
+    return fail(CANS_EINVAL, "cans_field_alloc_kind: unknown field kind");
+  const size_t es = kind_size(c, kind);
   size_t bytes = (size_t)c->L.sz * (c->L.n3 + 2) * es + 256;
   void* base = nullptr;
   hipError_t e = hipMalloc(&base, bytes);
@@ -123,7 +146,7 @@ extern "C" int cans_field_alloc(cans_ctx* c, void** field) {
   void* origin = (char*)base + 256 - es;
   {
     std::lock_guard<std::mutex> lk(g_mu);
-    g_fields[origin] = base;
+    g_fields[origin] = FieldRec{base, es};
   }
   *field = origin;
   return CANS_OK;
@@ -136,7 +159,7 @@ extern "C" int cans_field_free(cans_ctx* c, void* field) {
     std::lock_guard<std::mutex> lk(g_mu);
     auto it = g_fields.find(field);
     if (it == g_fields.end()) return fail(CANS_EINVAL, "cans_field_free: not a context field");
-    base = it->second;
+    base = it->second.base;
     g_fields.erase(it);
   }
   if (c) hipStreamSynchronize(c->stream);
@@ -144,12 +167,43 @@ extern "C" int cans_field_free(cans_ctx* c, void* field) {
   return CANS_OK;
 }
 
+// Element size of a field: from the registry for fields of cans_field_alloc*, else
+// double (accepted only where every field is double, i.e. in an FP64 context).
+static int field_es(const cans_ctx* c, const void* field, size_t* es, const char* who) {
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_fields.find(const_cast<void*>(field));
+    if (it != g_fields.end()) {
+      *es = it->second.es;
+      return CANS_OK;
+    }
+  }
+  if (c->precision != CANS_FP64)
+    return fail(CANS_EINVAL, std::string(who) + ": field not from cans_field_alloc_kind (element type unknown)");
+  *es = sizeof(double);
+  return CANS_OK;
+}
+
+// Host arrays are always double; FP32 storage (pressure-like fields of a
+// CANS_MIXED context) rounds on upload (to nearest) and widens exactly on download.
 extern "C" int cans_field_upload(cans_ctx* c, void* field, const double* host) {
   if (!c || !field || !host) return fail(CANS_EINVAL, "cans_field_upload: null argument");
   const Layout& L = c->L;
-  size_t w = (size_t)(L.n1 + 2) * sizeof(double);
-  size_t rows = (size_t)(L.n2 + 2) * (L.n3 + 2);
-  CANS_HIP_TRY(hipMemcpy2DAsync(field, L.sy * sizeof(double), host, w, w, rows, hipMemcpyHostToDevice, c->stream));
+  const size_t rows = (size_t)(L.n2 + 2) * (L.n3 + 2), nx = (size_t)(L.n1 + 2);
+  size_t es;
+  int rc = field_es(c, field, &es, "cans_field_upload");
+  if (rc) return rc;
+  if (es == sizeof(double)) {
+    const size_t w = nx * sizeof(double);
+    CANS_HIP_TRY(hipMemcpy2DAsync(field, L.sy * sizeof(double), host, w, w, rows, hipMemcpyHostToDevice, c->stream));
+  } else {
+    std::vector<float> tmp(nx * rows);
+    for (size_t q = 0; q < tmp.size(); q++) tmp[q] = (float)host[q];
+    const size_t w = nx * sizeof(float);
+    CANS_HIP_TRY(hipMemcpy2DAsync(field, L.sy * sizeof(float), tmp.data(), w, w, rows, hipMemcpyHostToDevice,
+                                  c->stream));
+    CANS_HIP_TRY(hipStreamSynchronize(c->stream));  // tmp must outlive the copy
+  }
   CANS_HIP_TRY(hipStreamSynchronize(c->stream));
   return CANS_OK;
 }
@@ -157,10 +211,22 @@ extern "C" int cans_field_upload(cans_ctx* c, void* field, const double* host) {
 extern "C" int cans_field_download(cans_ctx* c, const void* field, double* host) {
   if (!c || !field || !host) return fail(CANS_EINVAL, "cans_field_download: null argument");
   const Layout& L = c->L;
-  size_t w = (size_t)(L.n1 + 2) * sizeof(double);
-  size_t rows = (size_t)(L.n2 + 2) * (L.n3 + 2);
-  CANS_HIP_TRY(hipMemcpy2DAsync(host, w, field, L.sy * sizeof(double), w, rows, hipMemcpyDeviceToHost, c->stream));
-  CANS_HIP_TRY(hipStreamSynchronize(c->stream));
+  const size_t rows = (size_t)(L.n2 + 2) * (L.n3 + 2), nx = (size_t)(L.n1 + 2);
+  size_t es;
+  int rc = field_es(c, field, &es, "cans_field_download");
+  if (rc) return rc;
+  if (es == sizeof(double)) {
+    const size_t w = nx * sizeof(double);
+    CANS_HIP_TRY(hipMemcpy2DAsync(host, w, field, L.sy * sizeof(double), w, rows, hipMemcpyDeviceToHost, c->stream));
+    CANS_HIP_TRY(hipStreamSynchronize(c->stream));
+  } else {
+    std::vector<float> tmp(nx * rows);
+    const size_t w = nx * sizeof(float);
+    CANS_HIP_TRY(hipMemcpy2DAsync(tmp.data(), w, field, L.sy * sizeof(float), w, rows, hipMemcpyDeviceToHost,
+                                  c->stream));
+    CANS_HIP_TRY(hipStreamSynchronize(c->stream));
+    for (size_t q = 0; q < tmp.size(); q++) host[q] = (double)tmp[q];
+  }
   return CANS_OK;
 }
 

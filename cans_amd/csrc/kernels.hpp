@@ -7,7 +7,9 @@ namespace cans {
 // boundary plane operations (bound.f90:182-463, single-rank halo copies 600-713)
 enum { BT_PERIODIC = 0, BT_CENTERED = 1, BT_STAG_D = 2, BT_STAG_N = 3 };
 struct PlaneOp {
-  double* f;
+  void* f;              // field (element type chosen at launch)
+  const double* base;   // nullptr, or: f holds increments over this velocity field,
+  const double* fadd;   // plus this device scalar (nullptr = 0): value = base + f + *fadd
   int dir;    // 0,1,2 = x,y,z
   int side;   // 0 lower, 1 upper
   int type;   // BT_*
@@ -19,10 +21,14 @@ struct PlaneOps {
   int nops;
 };
 
+template <typename S>
 struct MomLaunch {
-  const double *u, *v, *w, *p;
-  double *uo, *vo, *wo;
-  double *ru, *rv, *rw;
+  const double *u, *v, *w;
+  const S* p;
+  double *uo = nullptr, *vo = nullptr, *wo = nullptr;  // updated velocity, or
+  S *eu = nullptr, *ev = nullptr, *ew = nullptr;       // its increment over (u,v,w)
+  S* divu = nullptr;                                   // optional: div(u) per cell, (n1,n2,n3)
+  S *ru, *rv, *rw;
   const double *dzci, *dzfi, *gvr_c, *gvr_f;
   double dxi, dyi, visc, dt;
   double rkpar[2];
@@ -102,33 +108,54 @@ __device__ __forceinline__ void block_max(double m, double* om) {
 
 int ensure_work(cans_ctx* c, int slot, size_t bytes, void** ptr);
 
-// stencil.hip
-int launch_mom_rk(cans_ctx* c, const MomLaunch& m);
-int launch_add_forcing(cans_ctx* c, const double* src, double* dst, int comp, int apply);
-int launch_fillps(cans_ctx* c, const double* u, const double* v, const double* w, double* p, const double* dzfi,
-                  double dti, const double* dli);
-int launch_correc(cans_ctx* c, const double* p, const double* su, const double* sv, const double* sw, double* du,
-                  double* dv, double* dw, const double* dzci, double dt, const double* dli,
-                  double* p_update = nullptr);
-int launch_updatep(cans_ctx* c, const double* pp, double* p, const double* dzci, const double* dzfi, double alpha,
+// stencil.hip.  Velocity arrays are double; S is the storage type of the
+// pressure-like arrays (double, or float in the mixed-precision mode).
+template <typename S>
+int launch_mom_rk(cans_ctx* c, const MomLaunch<S>& m);
+template <typename T>
+int launch_add_forcing(cans_ctx* c, const T* src, T* dst, int comp, int apply);
+// Where increments e are given, the velocity is u + e + f[comp] with f a device
+// vector of 3 (nullptr = 0): the bulk forcing, folded in instead of stored.
+template <typename S>
+int launch_fillps(cans_ctx* c, const double* u, const double* v, const double* w, const S* eu, const S* ev,
+                  const S* ew, const double* f, S* p, const double* dzfi, double dti, const double* dli);
+template <typename S>
+int launch_correc(cans_ctx* c, const S* p, const double* su, const double* sv, const double* sw, const S* eu,
+                  const S* ev, const S* ew, const double* f, double* du, double* dv, double* dw, const double* dzci,
+                  double dt, const double* dli, S* p_update);
+template <typename S>
+int launch_updatep(cans_ctx* c, const S* pp, S* p, const double* dzci, const double* dzfi, double alpha,
                    const double* dli, int impdiff);
+template <typename T>
 int launch_bound_dir(cans_ctx* c, const PlaneOps& ops);
-int launch_rhs_b(cans_ctx* c, double* p, const double* r, int dir, int plane, double norm);
-int run_chkdiv(cans_ctx* c, const double* u, const double* v, const double* w, const double* dzfi, const double* dli,
-               double* tot, double* mx);
-int run_chkdt(cans_ctx* c, const double* u, const double* v, const double* w, const double* dzci, const double* dzfi,
+template <typename S>
+int launch_rhs_b(cans_ctx* c, S* p, const double* r, int dir, int plane, double norm);
+template <typename T>
+int run_chkdiv(cans_ctx* c, const T* u, const T* v, const T* w, const double* dzfi, const double* dli, double* tot,
+               double* mx);
+template <typename T>
+int run_chkdt(cans_ctx* c, const T* u, const T* v, const T* w, const double* dzci, const double* dzfi,
               const double* dl, double* dti_out);
-int run_bulk_mean(cans_ctx* c, const double* q, const double* gvr, double* mean);
+template <typename T>
+int run_bulk_mean(cans_ctx* c, const T* q, const double* gvr, double* mean);
 
 // solver.hip
-// right-hand side of the pressure equation computed from the velocity inside the
-// first transform pass (fillps.f90:38-50); p then receives only the solution
+// right-hand side of the pressure equation computed from the velocity (plus its
+// increments eu,ev,ew when given) inside the first transform pass
+// (fillps.f90:38-50); p then receives only the solution
+// With divu given (increments mode), the divergence is divu + div(e): u is not read
+// (a constant f has no divergence, and the increments' halos carry the BCs).
+template <typename S>
 struct DivSrc {
-  const double *u, *v, *w, *dzfi;
-  double dti, dtidxi, dtidyi;
+  const double *u, *v, *w;
+  const S *eu, *ev, *ew;
+  const S* divu;
+  const double* dzfi;
+  double dti, dtidxi, dtidyi, dxi, dyi;
 };
+template <typename S>
 int run_solver(cans_ctx* c, const cans_plan* plan, double normfft, const double* lam, const double* a,
-               const double* b, const double* cc, const char* bc, const char* c_or_f, double* p,
-               const DivSrc* div = nullptr);
+               const double* b, const double* cc, const char* bc, const char* c_or_f, S* p,
+               const DivSrc<S>* div = nullptr);
 
 }  // namespace cans

@@ -51,12 +51,21 @@ __device__ __forceinline__ int mk_pos(int e, int N) { return (e & 1) ? N - 1 - (
 //            Y_k = Re(M_k (Z_k + conj Z_{N-k})) with M_k = exp(-i pi k/(2N));
 //   RODFT10: REDFT10 of (-1)^j x_j, output reversed;
 //   HC2R, REDFT01, RODFT01: the inverses (unnormalised, as FFTW).
-// Global accesses move two doubles per lane (16 bytes).
+// Global accesses move two elements per lane (16 bytes FP64, 8 bytes FP32 storage);
+// every value is widened to double on load, the transforms run in double and
+// only the store rounds to the storage type T.
 // ---------------------------------------------------------------------------
 typedef HIP_vector_type<double, 2> d2;
 
-__device__ __forceinline__ d2 ld2(const double* p) { return *(const d2*)p; }
-__device__ __forceinline__ void st2(double* p, double x, double y) { *(d2*)p = d2(x, y); }
+template <typename T>
+__device__ __forceinline__ d2 ld2(const T* p) {
+  const HIP_vector_type<T, 2> v = *(const HIP_vector_type<T, 2>*)p;
+  return d2((double)v.x, (double)v.y);
+}
+template <typename T>
+__device__ __forceinline__ void st2(T* p, double x, double y) {
+  *(HIP_vector_type<T, 2>*)p = HIP_vector_type<T, 2>((T)x, (T)y);
+}
 
 // spectra of A and B at index k from the complex transform of A + iB
 __device__ __forceinline__ void split_pp(cplx<double> zk, cplx<double> zc, double& are, double& aim, double& bre,
@@ -87,33 +96,55 @@ __device__ __forceinline__ cplx<double> merge_pp(double are, double aim, double 
 // pass X: rows of the halo'd field and of the work buffer are contiguous along x.
 // rows r = j + n2*k (0-based); the complex line of team q carries rows 2q, 2q+1.
 // ---------------------------------------------------------------------------
+template <typename T>
 struct XArgs {
   Layout L;
-  double* fld;     // halo'd field (origin)
-  double* W;       // compact work buffer, row r at W + r*n1
+  T* fld;          // halo'd field (origin)
+  T* W;            // compact work buffer, row r at W + r*n1
   int N, logN, Tn, pairs, kind;
   const cplx<double>* tw;
   const cplx<double>* mk;
   int div;         // forward only: build the rhs from the velocity instead of reading fld
-  DivSrc ds;
+  DivSrc<T> ds;
 };
 
 // two consecutive rhs values (elements e, e+1 of row r) of fillps.f90:38-50
-__device__ __forceinline__ d2 div_pair(const XArgs& a, long r, int e) {
+template <typename T>
+__device__ __forceinline__ d2 div_pair(const XArgs<T>& a, long r, int e) {
   const Layout& L = a.L;
   const int j = (int)(r % L.n2) + 1, k = (int)(r / L.n2) + 1;
   const long c = L.at(1, j, k) + e;
-  const d2 u2 = ld2(a.ds.u + c), w2 = ld2(a.ds.w + c), wm = ld2(a.ds.w + c - L.sz);
-  const d2 v2 = ld2(a.ds.v + c), vm = ld2(a.ds.v + c - L.sy);
-  const double um = a.ds.u[c - 1];
+  if (a.ds.divu) {  // dti * (div(u) + div(e)); div(u) was formed by the momentum kernel
+    const d2 dv = ld2(a.ds.divu + r * L.n1 + e);
+    const d2 eu2 = ld2(a.ds.eu + c), ew2 = ld2(a.ds.ew + c), ewm = ld2(a.ds.ew + c - L.sz);
+    const d2 ev2 = ld2(a.ds.ev + c), evm = ld2(a.ds.ev + c - L.sy);
+    const double eum = (double)a.ds.eu[c - 1];
+    const double dzfi = a.ds.dzfi[k], dyi = a.ds.dyi, dxi = a.ds.dxi, dti = a.ds.dti;
+    const double q0 = dti * (dv.x + ((ew2.x - ewm.x) * dzfi + (ev2.x - evm.x) * dyi + (eu2.x - eum) * dxi));
+    const double q1 = dti * (dv.y + ((ew2.y - ewm.y) * dzfi + (ev2.y - evm.y) * dyi + (eu2.y - eu2.x) * dxi));
+    return d2(q0, q1);
+  }
+  d2 u2 = ld2(a.ds.u + c), w2 = ld2(a.ds.w + c), wm = ld2(a.ds.w + c - L.sz);
+  d2 v2 = ld2(a.ds.v + c), vm = ld2(a.ds.v + c - L.sy);
+  double um = a.ds.u[c - 1];
+  if (a.ds.eu) {  // velocity held as u + increment
+    const d2 eu2 = ld2(a.ds.eu + c), ew2 = ld2(a.ds.ew + c), ewm = ld2(a.ds.ew + c - L.sz);
+    const d2 ev2 = ld2(a.ds.ev + c), evm = ld2(a.ds.ev + c - L.sy);
+    u2 = d2(u2.x + eu2.x, u2.y + eu2.y);
+    w2 = d2(w2.x + ew2.x, w2.y + ew2.y);
+    wm = d2(wm.x + ewm.x, wm.y + ewm.y);
+    v2 = d2(v2.x + ev2.x, v2.y + ev2.y);
+    vm = d2(vm.x + evm.x, vm.y + evm.y);
+    um = um + (double)a.ds.eu[c - 1];
+  }
   const double dti = a.ds.dti, dzfi = a.ds.dzfi[k], dy = a.ds.dtidyi, dx = a.ds.dtidxi;
   const double p0 = ((w2.x - wm.x) * dti * dzfi + (v2.x - vm.x) * dy + (u2.x - um) * dx);
   const double p1 = ((w2.y - wm.y) * dti * dzfi + (v2.y - vm.y) * dy + (u2.y - u2.x) * dx);
   return d2(p0, p1);
 }
 
-template <int DIR>  // +1 forward (fld -> W), -1 backward (W -> fld)
-__global__ void k_xpass(XArgs a) {
+template <int DIR, typename T>  // +1 forward (fld -> W), -1 backward (W -> fld)
+__global__ void k_xpass(XArgs<T> a) {
   extern __shared__ unsigned char smem_raw[];
   cplx<double>* smem = (cplx<double>*)smem_raw;
   const int N = a.N, Tn = a.Tn, kind = a.kind;
@@ -126,14 +157,14 @@ __global__ void k_xpass(XArgs a) {
   const Layout& L = a.L;
   auto frow = [&](long r) { return a.fld + L.at(1, (int)(r % L.n2) + 1, (int)(r / L.n2) + 1); };
   auto wrow = [&](long r) { return a.W + r * L.n1; };
-  const double* xa = DIR > 0 ? (va ? frow(ra) : nullptr) : (va ? wrow(ra) : nullptr);
-  const double* xb = DIR > 0 ? (vb ? frow(rb) : nullptr) : (vb ? wrow(rb) : nullptr);
-  double* ya = DIR > 0 ? (va ? wrow(ra) : nullptr) : (va ? frow(ra) : nullptr);
-  double* yb = DIR > 0 ? (vb ? wrow(rb) : nullptr) : (vb ? frow(rb) : nullptr);
+  const T* xa = DIR > 0 ? (va ? frow(ra) : nullptr) : (va ? wrow(ra) : nullptr);
+  const T* xb = DIR > 0 ? (vb ? frow(rb) : nullptr) : (vb ? wrow(rb) : nullptr);
+  T* ya = DIR > 0 ? (va ? wrow(ra) : nullptr) : (va ? frow(ra) : nullptr);
+  T* yb = DIR > 0 ? (vb ? wrow(rb) : nullptr) : (vb ? frow(rb) : nullptr);
   if (DIR > 0) {
     if (va) {
       if (N == 1) {
-        if (lt == 0) smem[ix(0)] = {xa[0], xb ? xb[0] : 0.};
+        if (lt == 0) smem[ix(0)] = {(double)xa[0], xb ? (double)xb[0] : 0.};
       } else {
         for (int e = 2 * lt; e < N; e += 2 * Tn) {
           d2 a2, b2;
@@ -161,8 +192,8 @@ __global__ void k_xpass(XArgs a) {
       const int H = N >> 1;
       if (N == 1) {
         if (lt == 0) {
-          ya[0] = smem[ix(0)].re;
-          if (yb) yb[0] = smem[ix(0)].im;
+          ya[0] = (T)smem[ix(0)].re;
+          if (yb) yb[0] = (T)smem[ix(0)].im;
         }
         return;
       }
@@ -202,13 +233,13 @@ __global__ void k_xpass(XArgs a) {
         for (int k = lt; k < N; k += Tn) {
           double are, aim, bre = 0., bim = 0.;
           if (N == 1 || k == 0) {
-            are = xa[0];
+            are = (double)xa[0];
             aim = 0.;
-            if (xb) bre = xb[0];
+            if (xb) bre = (double)xb[0];
           } else if (k == H) {
-            are = xa[1];
+            are = (double)xa[1];
             aim = 0.;
-            if (xb) bre = xb[1];
+            if (xb) bre = (double)xb[1];
           } else {
             const int kk = k < H ? k : N - k;
             const double sg = k < H ? 1. : -1.;
@@ -227,8 +258,8 @@ __global__ void k_xpass(XArgs a) {
         for (int k = lt; k < N; k += Tn) {
           const bool dd = kind == K_DD;  // RODFT01 reads the reversed input
           const int ik = dd ? N - 1 - k : k, ink = dd ? k - 1 : N - k;
-          const double xka = xa[ik], xnka = k ? xa[ink] : 0.;
-          const double xkb = xb ? xb[ik] : 0., xnkb = (xb && k) ? xb[ink] : 0.;
+          const double xka = (double)xa[ik], xnka = k ? (double)xa[ink] : 0.;
+          const double xkb = xb ? (double)xb[ik] : 0., xnkb = (xb && k) ? (double)xb[ink] : 0.;
           smem[ix(k)] = merge_dct(a.mk[k], xka, xnka, xkb, xnkb);
         }
       }
@@ -238,8 +269,8 @@ __global__ void k_xpass(XArgs a) {
     if (!va) return;
     if (N == 1) {
       if (lt == 0) {
-        ya[0] = smem[ix(0)].re;
-        if (yb) yb[0] = smem[ix(0)].im;
+        ya[0] = (T)smem[ix(0)].re;
+        if (yb) yb[0] = (T)smem[ix(0)].im;
       }
       return;
     }
@@ -261,16 +292,17 @@ __global__ void k_xpass(XArgs a) {
 // thread t -> line c = t % LINES, team thread lt = t / LINES; LDS element (e, c) at [e*LINES + c].
 // The two columns of a line are adjacent, so every global access is one 16-byte pair.
 // ---------------------------------------------------------------------------
+template <typename T>
 struct YArgs {
   int n1, n2, n3;
-  double* W;
+  T* W;
   int N, logN, Tn, lines, kind;
   const cplx<double>* tw;
   const cplx<double>* mk;
 };
 
-template <int DIR>
-__global__ void k_ypass(YArgs a) {
+template <int DIR, typename T>
+__global__ void k_ypass(YArgs<T> a) {
   extern __shared__ unsigned char smem_raw[];
   cplx<double>* smem = (cplx<double>*)smem_raw;
   const int LN = a.lines, N = a.N, Tn = a.Tn, kind = a.kind;
@@ -280,14 +312,14 @@ __global__ void k_ypass(YArgs a) {
   const int i0 = (blockIdx.x % ncolblk) * 2 * LN;
   const int ia = i0 + 2 * c;
   const bool va = ia < a.n1, vb = ia + 1 < a.n1;
-  double* col = a.W + (long)k * a.n1 * a.n2 + ia;
+  T* col = a.W + (long)k * a.n1 * a.n2 + ia;
   const long est = a.n1;  // element stride along y
   const IxStride ix{c, LN};
   // (A, B) pair of element e of this line
-  auto ld = [&](int e) -> d2 { return vb ? ld2(col + e * est) : d2(col[e * est], 0.); };
+  auto ld = [&](int e) -> d2 { return vb ? ld2(col + e * est) : d2((double)col[e * est], 0.); };
   auto st = [&](int e, double x, double y) {
     if (vb) st2(col + e * est, x, y);
-    else col[e * est] = x;
+    else col[e * est] = (T)x;
   };
   if (DIR > 0) {
     if (va) {
@@ -388,12 +420,14 @@ __global__ void k_ypass(YArgs a) {
 // ---------------------------------------------------------------------------
 constexpr int CK = 16;
 
+template <typename T>
 struct ZArgs {
-  double* W;
+  T* W;
   long nl;         // number of lines (n1*n2) = stride between consecutive k
   int n;           // rows of the system (n3 - q)
   int periodic;
   double norm;
+  double pin_eps;  // relative pivot tolerance of the zero-mode pin
   const double *a, *b, *c, *lam;
   double *ckd, *ckp, *ckq;  // checkpoints [nblk][nl]
   double* X2;               // periodic: auxiliary solution p2 [nn][nl]
@@ -405,8 +439,7 @@ struct ZRow {
 };
 
 __device__ __forceinline__ void z_row(int k, int nn, int periodic, double l, double r, double norm, const double* a,
-                                      const double* b, const double* c, ZRow& s) {
-  const double eps = 2.220446049250313e-16;
+                                      const double* b, const double* c, double eps, ZRow& s) {
   if (k == 1) {
     const double zz = 1. / (b[0] + l);
     s.d = c[0] * zz;
@@ -433,11 +466,13 @@ __device__ __forceinline__ void z_row(int k, int nn, int periodic, double l, dou
   }
 }
 
-__global__ void __launch_bounds__(256) k_gaussel(ZArgs z) {
+template <typename T>
+__global__ void __launch_bounds__(256) k_gaussel(ZArgs<T> z) {
   const long line = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (line >= z.nl) return;
   const double l = z.lam[line];
-  double* P = z.W + line;
+  const double eps = z.pin_eps;
+  T* P = z.W + line;
   const long ps = z.nl;
   const int periodic = z.periodic;
   const int nn = periodic ? z.n - 1 : z.n;
@@ -450,17 +485,17 @@ __global__ void __launch_bounds__(256) k_gaussel(ZArgs z) {
   ZRow s = {0., 0., 0.};
   double rr[CK], rn[CK];
 #pragma unroll
-  for (int t = 0; t < CK; t++) rr[t] = (t + 1 <= nn) ? P[(long)t * ps] : 0.;
+  for (int t = 0; t < CK; t++) rr[t] = (t + 1 <= nn) ? (double)P[(long)t * ps] : 0.;
   for (int bb = 0; bb < nblk; bb++) {
     const int k0 = bb * CK + 1;
 #pragma unroll
     for (int t = 0; t < CK; t++) {
       const int k = k0 + CK + t;
-      rn[t] = (k <= nn) ? P[(long)(k - 1) * ps] : 0.;
+      rn[t] = (k <= nn) ? (double)P[(long)(k - 1) * ps] : 0.;
     }
 #pragma unroll
     for (int t = 0; t < CK; t++)
-      if (k0 + t <= nn) z_row(k0 + t, nn, periodic, l, rr[t], norm, a, b, c, s);
+      if (k0 + t <= nn) z_row(k0 + t, nn, periodic, l, rr[t], norm, a, b, c, eps, s);
     const long o = (long)bb * ps + line;
     z.ckd[o] = s.d;
     z.ckp[o] = s.pp;
@@ -474,14 +509,14 @@ __global__ void __launch_bounds__(256) k_gaussel(ZArgs z) {
   {
     const int k0 = (nblk - 1) * CK + 1;
 #pragma unroll
-    for (int t = 0; t < CK; t++) rr[t] = (k0 + t <= nn) ? P[(long)(k0 + t - 1) * ps] : 0.;
+    for (int t = 0; t < CK; t++) rr[t] = (k0 + t <= nn) ? (double)P[(long)(k0 + t - 1) * ps] : 0.;
   }
   for (int bb = nblk - 1; bb >= 0; bb--) {
     const int k0 = bb * CK + 1;
     const int k1 = min(k0 + CK - 1, nn);
     if (bb > 0) {
 #pragma unroll
-      for (int t = 0; t < CK; t++) rn[t] = P[(long)(k0 - CK + t - 1) * ps];
+      for (int t = 0; t < CK; t++) rn[t] = (double)P[(long)(k0 - CK + t - 1) * ps];
     }
     ZRow r0 = {0., 0., 0.};
     if (bb > 0) {
@@ -494,7 +529,7 @@ __global__ void __launch_bounds__(256) k_gaussel(ZArgs z) {
 #pragma unroll
     for (int t = 0; t < CK; t++) {
       if (k0 + t <= k1) {
-        z_row(k0 + t, nn, periodic, l, rr[t], norm, a, b, c, r0);
+        z_row(k0 + t, nn, periodic, l, rr[t], norm, a, b, c, eps, r0);
         dd[t] = r0.d;
         pv[t] = r0.pp;
         qv[t] = r0.q2;
@@ -514,7 +549,7 @@ __global__ void __launch_bounds__(256) k_gaussel(ZArgs z) {
           x = pv[t] - dd[t] * xn;
           if (periodic) x2 = qv[t] - dd[t] * xq;
         }
-        P[(long)(k - 1) * ps] = x;
+        P[(long)(k - 1) * ps] = (T)x;
         if (periodic) z.X2[(long)(k - 1) * ps + line] = x2;
         xn = x;
         xq = x2;
@@ -529,17 +564,16 @@ __global__ void __launch_bounds__(256) k_gaussel(ZArgs z) {
   }
   if (!periodic) return;
   // ---- periodic closure (solver.f90:268-296) and Sherman-Morrison correction ----
-  const double eps = 2.220446049250313e-16;
   const int m = nn + 1;  // closure row, 1-based
   const double den = b[m - 1] + l + c[m - 1] * xq1 + a[m - 1] * xqnn;
   const double tol = eps * fmax(fabs(b[m - 1] + l), fabs(c[m - 1] * xq1 + a[m - 1] * xqnn));
   double xm;
   if (fabs(den) <= tol) xm = 0.;
-  else xm = (P[(long)(m - 1) * ps] * norm - c[m - 1] * x1 - a[m - 1] * xnn) / den;
-  P[(long)(m - 1) * ps] = xm;
+  else xm = ((double)P[(long)(m - 1) * ps] * norm - c[m - 1] * x1 - a[m - 1] * xnn) / den;
+  P[(long)(m - 1) * ps] = (T)xm;
   for (int k = 1; k <= nn; k++) {
     const long o = (long)(k - 1) * ps;
-    P[o] = P[o] + z.X2[o + line] * xm;
+    P[o] = (T)((double)P[o] + z.X2[o + line] * xm);
   }
 }
 
@@ -569,8 +603,9 @@ static int upload_tables(cans_plan* p, int d) {
   return CANS_OK;
 }
 
-static int xpass(cans_ctx* c, const cans_plan* p, double* fld, double* W, int dir, const DivSrc* div = nullptr) {
-  XArgs a;
+template <typename T>
+static int xpass(cans_ctx* c, const cans_plan* p, T* fld, T* W, int dir, const DivSrc<T>* div = nullptr) {
+  XArgs<T> a;
   a.div = div != nullptr && dir > 0;
   if (a.div) a.ds = *div;
   a.L = c->L;
@@ -587,14 +622,15 @@ static int xpass(cans_ctx* c, const cans_plan* p, double* fld, double* W, int di
   const long npairs = (nrows + 1) / 2;
   dim3 grid(cdiv(npairs, a.pairs));
   const size_t shm = (size_t)a.pairs * padded_len(a.N) * sizeof(cplx<double>);
-  if (dir > 0) k_xpass<1><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
-  else k_xpass<-1><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
+  if (dir > 0) k_xpass<1, T><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
+  else k_xpass<-1, T><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
   CANS_LAUNCH_CHECK("k_xpass");
   return CANS_OK;
 }
 
-static int ypass(cans_ctx* c, const cans_plan* p, double* W, int dir) {
-  YArgs a;
+template <typename T>
+static int ypass(cans_ctx* c, const cans_plan* p, T* W, int dir) {
+  YArgs<T> a;
   a.n1 = c->L.n1;
   a.n2 = c->L.n2;
   a.n3 = c->L.n3;
@@ -613,20 +649,21 @@ static int ypass(cans_ctx* c, const cans_plan* p, double* W, int dir) {
   const int ncolblk = (a.n1 + 2 * lines - 1) / (2 * lines);
   dim3 grid((unsigned)ncolblk * a.n3);
   const size_t shm = (size_t)lines * a.N * sizeof(cplx<double>);
-  if (dir > 0) k_ypass<1><<<grid, lines * a.Tn, shm, c->stream>>>(a);
-  else k_ypass<-1><<<grid, lines * a.Tn, shm, c->stream>>>(a);
+  if (dir > 0) k_ypass<1, T><<<grid, lines * a.Tn, shm, c->stream>>>(a);
+  else k_ypass<-1, T><<<grid, lines * a.Tn, shm, c->stream>>>(a);
   CANS_LAUNCH_CHECK("k_ypass");
   return CANS_OK;
 }
 
+template <typename T>
 int run_solver(cans_ctx* c, const cans_plan* p, double normfft, const double* lam, const double* a,
-               const double* b, const double* cc, const char* bc, const char* c_or_f, double* fld,
-               const DivSrc* div) {
+               const double* b, const double* cc, const char* bc, const char* c_or_f, T* fld,
+               const DivSrc<T>* div) {
   const Layout& L = c->L;
   const long N = L.cells();
   const long nl = (long)L.n1 * L.n2;
   void* W = nullptr;
-  int rc = ensure_work(c, 1, (size_t)N * sizeof(double), &W);
+  int rc = ensure_work(c, 1, (size_t)N * sizeof(T), &W);
   if (rc) return rc;
   const int q = (c_or_f[2] == 'f' && bc[5] == 'D') ? 1 : 0;
   const int periodic = bc[4] == 'P' && bc[5] == 'P';
@@ -638,20 +675,27 @@ int run_solver(cans_ctx* c, const cans_plan* p, double normfft, const double* la
   size_t ckbytes = (size_t)3 * nblk * nl * sizeof(double) + (periodic ? (size_t)N * sizeof(double) : 0);
   rc = ensure_work(c, 2, ckbytes, &ck);
   if (rc) return rc;
-  // the transform passes move element pairs: interior rows must start on 16-byte boundaries
-  if (L.n1 % 2 || L.sy % 2 || ((uintptr_t)(fld + 1) & 15))
-    return fail(CANS_EINVAL, "cans_solver: rows of p must start 16-byte aligned (n1 and ld even, field from "
-                             "cans_field_alloc or with element (1,0,0) on a 16-byte boundary)");
+  // the transform passes move element pairs: interior rows must start on pair boundaries
+  if (L.n1 % 2 || L.sy % 2 || ((uintptr_t)(fld + 1) & (2 * sizeof(T) - 1)))
+    return fail(CANS_EINVAL, "cans_solver: rows of p must start aligned to an element pair (n1 and ld even, "
+                             "field from cans_field_alloc or with element (1,0,0) on a pair boundary)");
   if (div && (p->kind[0] != K_PP || p->len[0] < 2))
     return fail(CANS_EINVAL, "run_solver: the velocity-divergence input needs a periodic x transform");
-  if ((rc = xpass(c, p, fld, (double*)W, +1, div))) return rc;
-  if ((rc = ypass(c, p, (double*)W, +1))) return rc;
-  ZArgs z;
-  z.W = (double*)W;
+  if ((rc = xpass<T>(c, p, fld, (T*)W, +1, div))) return rc;
+  if ((rc = ypass<T>(c, p, (T*)W, +1))) return rc;
+  ZArgs<T> z;
+  z.W = (T*)W;
   z.nl = nl;
   z.n = n;
   z.periodic = periodic;
   z.norm = normfft;
+  // The reference pins the last unknown of the zero-wavenumber line when its final
+  // pivot is within one ulp of zero (solver.f90:151-160).  FP64 storage keeps that
+  // test exactly.  With FP32 storage the rhs of that singular line is compatible only
+  // to FP32 rounding, so an unpinned solve would add a constant ~1e8 times the
+  // solution; the tolerance is widened to 4096 ulp (rounding accumulated over the
+  // elimination), still ~1e8 below the smallest regular pivot of any other line.
+  z.pin_eps = 2.220446049250313e-16 * (sizeof(T) == sizeof(double) ? 1. : 4096.);
   z.a = a;
   z.b = b;
   z.c = cc;
@@ -660,12 +704,17 @@ int run_solver(cans_ctx* c, const cans_plan* p, double normfft, const double* la
   z.ckp = z.ckd + nblk * nl;
   z.ckq = z.ckp + nblk * nl;
   z.X2 = periodic ? z.ckq + nblk * nl : nullptr;
-  k_gaussel<<<cdiv(nl, 256), 256, 0, c->stream>>>(z);
+  k_gaussel<T><<<cdiv(nl, 256), 256, 0, c->stream>>>(z);
   CANS_LAUNCH_CHECK("k_gaussel");
-  if ((rc = ypass(c, p, (double*)W, -1))) return rc;
-  if ((rc = xpass(c, p, fld, (double*)W, -1))) return rc;
+  if ((rc = ypass<T>(c, p, (T*)W, -1))) return rc;
+  if ((rc = xpass<T>(c, p, fld, (T*)W, -1))) return rc;
   return CANS_OK;
 }
+
+template int run_solver<double>(cans_ctx*, const cans_plan*, double, const double*, const double*, const double*,
+                                const double*, const char*, const char*, double*, const DivSrc<double>*);
+template int run_solver<float>(cans_ctx*, const cans_plan*, double, const double*, const double*, const double*,
+                               const double*, const char*, const char*, float*, const DivSrc<float>*);
 
 }  // namespace cans
 

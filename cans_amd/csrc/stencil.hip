@@ -5,6 +5,12 @@
 // association and rounding (Fortran evaluates left to right, no fused
 // multiply-add in the reference's -O3 CPU build), so these kernels reproduce the
 // CPU arithmetic bit for bit on identical inputs.
+//
+// Velocity fields are always double.  Pressure-like arrays (p, pp, RHS_old and
+// the RK increment of the mixed mode) have the storage type S: double, or float
+// in the mixed-precision mode.  Values are widened to double on load and all
+// arithmetic is double; only a store to S rounds.  With S = double the code is
+// the plain FP64 path.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -21,11 +27,15 @@ namespace cans {
 // not alias (u,v,w).  Per-block partial sums of the updated velocity weighted
 // by the grid-volume ratio feed the bulk forcing (utils.f90:15-42).
 // ---------------------------------------------------------------------------
+template <typename S>
 struct MomArgs {
   Layout L;
-  const double *u, *v, *w, *p;
-  double *uo, *vo, *wo;
-  double *ru, *rv, *rw;          // dudtrko (1:n1,1:n2,1:n3), in: old RHS, out: new RHS
+  const double *u, *v, *w;
+  const S* p;
+  double *uo, *vo, *wo;          // updated velocity (delta == 0)
+  S *eu, *ev, *ew;               // or its increment over (u,v,w) (delta == 1)
+  S* divu;                       // nullptr, or div(u) at the old level, (1:n1,1:n2,1:n3)
+  S *ru, *rv, *rw;               // dudtrko (1:n1,1:n2,1:n3), in: old RHS, out: new RHS
   const double *dzci, *dzfi;     // (0:n3+1)
   const double *rdzm, *rdzp;     // (0:n3+1): dzci(k)/dzfi(k), dzci(k)/dzfi(k+1)
   const double *gvr_c, *gvr_f;   // grid volume ratios (0:n3+1)
@@ -34,6 +44,7 @@ struct MomArgs {
   double bforce[3];
   int sum_mask;                  // bit c: accumulate the bulk mean of component c
   int read_ro, write_ro;
+  int delta;
   double* partials;              // [3][nblocks]
 };
 
@@ -53,7 +64,8 @@ __global__ void k_rdz(int n3, const double* dzci, const double* dzfi, double* ta
   tab[n3 + 2 + k] = in ? dzci[k] / dzfi[k + 1] : 0.;
 }
 
-__global__ void __launch_bounds__(MOM_TX* MOM_TY) k_mom_rk(MomArgs a) {
+template <typename S>
+__global__ void __launch_bounds__(MOM_TX* MOM_TY) k_mom_rk(MomArgs<S> a) {
   __shared__ double su[3][MOM_PY][MOM_PX], sv[3][MOM_PY][MOM_PX], sw[3][MOM_PY][MOM_PX];
   const Layout& L = a.L;
   const int ntx = (L.n1 + MOM_TX - 1) / MOM_TX, nty = (L.n2 + MOM_TY - 1) / MOM_TY;
@@ -91,7 +103,7 @@ __global__ void __launch_bounds__(MOM_TX* MOM_TY) k_mom_rk(MomArgs a) {
     w_ccm = a.w[c0 - sz];
     w_pcm = a.w[c0 - sz + 1];
     w_cpm = a.w[c0 - sz + sy];
-    p_c = a.p[c0];
+    p_c = (double)a.p[c0];
   }
   stage(kb, kb % 3);
   double s0 = 0., s1 = 0., s2 = 0.;
@@ -150,8 +162,9 @@ __global__ void __launch_bounds__(MOM_TX* MOM_TY) k_mom_rk(MomArgs a) {
       // RK update (rk.f90:156-186) and RHS_old <- RHS (rk.f90:266-276).  With rkpar(2) = 0
       // (first substep) the RHS_old term is factor2*RHS_old = 0 and is not read.
       const long o = (long)(i - 1) + (long)L.n1 * ((j - 1) + (long)L.n2 * (k - 1));
-      const double p_ip = a.p[c + 1], p_jp = a.p[c + sy], p_kp = a.p[c + sz];
-      const double ru = a.read_ro ? a.ru[o] : 0., rv = a.read_ro ? a.rv[o] : 0., rw = a.read_ro ? a.rw[o] : 0.;
+      const double p_ip = (double)a.p[c + 1], p_jp = (double)a.p[c + sy], p_kp = (double)a.p[c + sz];
+      const double ru = a.read_ro ? (double)a.ru[o] : 0., rv = a.read_ro ? (double)a.rv[o] : 0.,
+                   rw = a.read_ro ? (double)a.rw[o] : 0.;
       const double un = a.read_ro ? u_ccc + a.factor1 * dudt + a.factor2 * ru +
                                         a.factor12 * (a.bforce[0] - dxi * (p_ip - p_c))
                                   : u_ccc + a.factor1 * dudt + a.factor12 * (a.bforce[0] - dxi * (p_ip - p_c));
@@ -161,14 +174,22 @@ __global__ void __launch_bounds__(MOM_TX* MOM_TY) k_mom_rk(MomArgs a) {
       const double wn = a.read_ro ? w_ccc + a.factor1 * dwdt + a.factor2 * rw +
                                         a.factor12 * (a.bforce[2] - dzci_k * (p_kp - p_c))
                                   : w_ccc + a.factor1 * dwdt + a.factor12 * (a.bforce[2] - dzci_k * (p_kp - p_c));
-      a.uo[c] = un;
-      a.vo[c] = vn;
-      a.wo[c] = wn;
-      if (a.write_ro) {
-        a.ru[o] = dudt;
-        a.rv[o] = dvdt;
-        a.rw[o] = dwdt;
+      if (a.delta) {
+        a.eu[c] = (S)(un - u_ccc);
+        a.ev[c] = (S)(vn - v_ccc);
+        a.ew[c] = (S)(wn - w_ccc);
+      } else {
+        a.uo[c] = un;
+        a.vo[c] = vn;
+        a.wo[c] = wn;
       }
+      if (a.write_ro) {
+        a.ru[o] = (S)dudt;
+        a.rv[o] = (S)dvdt;
+        a.rw[o] = (S)dwdt;
+      }
+      if (a.divu)  // fillps.f90:38-50 association, without the 1/dt factor
+        a.divu[o] = (S)((w_ccc - w_ccm) * dzfi_k + (v_ccc - v_cmc) * dyi + (u_ccc - u_mcc) * dxi);
       if (a.sum_mask) {
         s0 += un * a.gvr_f[k];
         s1 += vn * a.gvr_f[k];
@@ -270,60 +291,93 @@ static int ensure_partials(cans_ctx* c, size_t n) {
 }
 
 // u(1:n) += f(c) (mom.f90:725-769); src may differ from dst (copy + forcing)
-__global__ void k_add_forcing(Layout L, const double* src, double* dst, const double* f, int comp, int apply) {
+template <typename T>
+__global__ void k_add_forcing(Layout L, const T* src, T* dst, const double* f, int comp, int apply) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int j = blockIdx.y + 1, k = blockIdx.z + 1;
   if (i > L.n1) return;
   const long c = L.at(i, j, k);
-  dst[c] = apply ? src[c] + f[comp] : src[c];
+  dst[c] = apply ? (T)((double)src[c] + f[comp]) : src[c];
 }
 
-// fillps.f90:13-51
-__global__ void k_fillps(Layout L, const double* u, const double* v, const double* w, double* p,
-                         const double* dzfi, double dti, double dtidxi, double dtidyi) {
+// velocity value u[o], or u[o] + e[o] + fc when the field is held as u plus an increment e
+template <typename S>
+__device__ __forceinline__ double vel(const double* u, const S* e, double fc, long o) {
+  return e ? u[o] + (double)e[o] + fc : u[o];
+}
+
+// fillps.f90:13-51 (velocity u,v,w, optionally plus increments eu,ev,ew and forcing f)
+template <typename S>
+__global__ void k_fillps(Layout L, const double* u, const double* v, const double* w, const S* eu, const S* ev,
+                         const S* ew, const double* f, S* p, const double* dzfi, double dti, double dtidxi,
+                         double dtidyi) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int j = blockIdx.y + 1, k = blockIdx.z + 1;
   if (i > L.n1) return;
   const long c = L.at(i, j, k);
-  p[c] = ((w[c] - w[c - L.sz]) * dti * dzfi[k] + (v[c] - v[c - L.sy]) * dtidyi + (u[c] - u[c - 1]) * dtidxi);
+  const double fu = f ? f[0] : 0., fv = f ? f[1] : 0., fw = f ? f[2] : 0.;
+  const double wc = vel(w, ew, fw, c), wm = vel(w, ew, fw, c - L.sz), vc = vel(v, ev, fv, c);
+  const double vm = vel(v, ev, fv, c - L.sy), uc = vel(u, eu, fu, c), um = vel(u, eu, fu, c - 1);
+  p[c] = (S)((wc - wm) * dti * dzfi[k] + (vc - vm) * dtidyi + (uc - um) * dtidxi);
 }
 
 // correc.f90:13-61 over the whole halo'd box; (su,sv,sw) -> (du,dv,dw), which may alias.
 // With pu != nullptr the pressure update p += pp of updatep.f90:78-86 rides along
 // (it needs only pp and p, both final at this point of the substep).
-__global__ void k_correc(Layout L, const double* p, const double* su, const double* sv, const double* sw,
-                         double* du, double* dv, double* dw, const double* dzci, double dt, double factori,
-                         double factorj, double* pu) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y, k = blockIdx.z;
-  if (i > L.n1 + 1) return;
-  const long c = L.at(i, j, k);
-  const double pc = p[c];
-  du[c] = (i <= L.n1) ? su[c] - factori * (p[c + 1] - pc) : su[c];
-  dv[c] = (j <= L.n2) ? sv[c] - factorj * (p[c + L.sy] - pc) : sv[c];
-  dw[c] = (k <= L.n3) ? sw[c] - dt * dzci[k] * (p[c + L.sz] - pc) : sw[c];
-  if (pu && i >= 1 && i <= L.n1 && j >= 1 && j <= L.n2 && k >= 1 && k <= L.n3) pu[c] = pu[c] + pc;
+// The source velocity is (su,sv,sw), plus increments (eu,ev,ew) when those are given;
+// the destination may alias the source (each element is read and written by one thread).
+// A thread owns one (i,j) column of the halo'd box and marches CORREC_KC planes up in
+// z, so p(k+1), loaded for the z-gradient, is reused as p(k) of the next plane; the
+// x and y neighbours of p come from adjacent lanes and rows of the same block.
+constexpr int CORREC_TX = 64, CORREC_TY = 4, CORREC_KC = 32;
+
+template <typename S>
+__global__ void __launch_bounds__(CORREC_TX* CORREC_TY)
+    k_correc(Layout L, const S* p, const double* su, const double* sv, const double* sw, const S* eu, const S* ev,
+             const S* ew, const double* f, double* du, double* dv, double* dw, const double* dzci, double dt,
+             double factori, double factorj, S* pu) {
+  const int i = blockIdx.x * CORREC_TX + threadIdx.x;
+  const int j = blockIdx.y * CORREC_TY + threadIdx.y;
+  if (i > L.n1 + 1 || j > L.n2 + 1) return;
+  const int kb = blockIdx.z * CORREC_KC, ke = min(kb + CORREC_KC - 1, L.n3 + 1);
+  const double fu = f ? f[0] : 0., fv = f ? f[1] : 0., fw = f ? f[2] : 0.;
+  const bool in_ij = i >= 1 && i <= L.n1 && j >= 1 && j <= L.n2;
+  long c = L.at(i, j, kb);
+  double pc = p[c];
+  for (int k = kb; k <= ke; k++, c += L.sz) {
+    const double pk = (k <= L.n3) ? (double)p[c + L.sz] : 0.;
+    const double tu = vel(su, eu, fu, c), tv = vel(sv, ev, fv, c), tw = vel(sw, ew, fw, c);
+    du[c] = (i <= L.n1) ? tu - factori * ((double)p[c + 1] - pc) : tu;
+    dv[c] = (j <= L.n2) ? tv - factorj * ((double)p[c + L.sy] - pc) : tv;
+    dw[c] = (k <= L.n3) ? tw - dt * dzci[k] * (pk - pc) : tw;
+    if (pu && in_ij && k >= 1 && k <= L.n3) pu[c] = (S)((double)pu[c] + pc);
+    pc = pk;
+  }
 }
 
 // updatep.f90:14-88
-__global__ void k_updatep(Layout L, const double* pp, double* p, const double* dzci, const double* dzfi,
-                          double alpha, double dxi2, double dyi2, int impdiff) {
+template <typename T>
+__global__ void k_updatep(Layout L, const T* pp, T* p, const double* dzci, const double* dzfi, double alpha,
+                          double dxi2, double dyi2, int impdiff) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int j = blockIdx.y + 1, k = blockIdx.z + 1;
   if (i > L.n1) return;
   const long c = L.at(i, j, k);
+  const double q = pp[c];
   if (!impdiff) {
-    p[c] = p[c] + pp[c];
+    p[c] = (T)((double)p[c] + q);
     return;
   }
-  const double q = pp[c];
+  const double qxp = pp[c + 1], qxm = pp[c - 1], qyp = pp[c + L.sy], qym = pp[c - L.sy];
+  const double qzp = pp[c + L.sz], qzm = pp[c - L.sz];
   double lap = 0.;
-  lap = lap + (pp[c + 1] - 2. * q + pp[c - 1]) * dxi2 + (pp[c + L.sy] - 2. * q + pp[c - L.sy]) * dyi2;
-  lap = lap + ((pp[c + L.sz] - q) * dzci[k] - (q - pp[c - L.sz]) * dzci[k - 1]) * dzfi[k];
-  p[c] = p[c] + q + alpha * lap;
+  lap = lap + (qxp - 2. * q + qxm) * dxi2 + (qyp - 2. * q + qym) * dyi2;
+  lap = lap + ((qzp - q) * dzci[k] - (q - qzm) * dzci[k - 1]) * dzfi[k];
+  p[c] = (T)((double)p[c] + q + alpha * lap);
 }
 
 // one boundary plane per op (bound.f90:182-463 and the single-rank halo copy of 600-713)
+template <typename T>
 __global__ void k_bound(Layout L, PlaneOps ops) {
   const PlaneOp op = ops.op[blockIdx.y];
   const int e1 = op.dir == 0 ? L.n2 + 2 : L.n1 + 2;
@@ -335,36 +389,68 @@ __global__ void k_bound(Layout L, PlaneOps ops) {
   if (op.dir == 0) { base = L.at(0, a, b); st = 1; }
   else if (op.dir == 1) { base = L.at(a, 0, b); st = L.sy; }
   else { base = L.at(a, b, 0); st = L.sz; }
-  double* f = op.f;
+  T* f = (T*)op.f;
   const int nd = op.nd;
+  if (op.base) {  // f holds increments over op.base (+ fadd): apply the rule to the sum, store the increment
+    const double* g = op.base;
+    const double fc = op.fadd ? *op.fadd : 0.;
+    auto t = [&](long o) { return g[o] + (double)f[o] + fc; };
+    auto put = [&](long o, double x) { f[o] = (T)(x - g[o] - fc); };
+    switch (op.type) {
+      case BT_PERIODIC:
+        if (op.side == 0) put(base, t(base + st * nd));
+        else put(base + st * (nd + 1), t(base + st));
+        break;
+      case BT_CENTERED:
+        if (op.side == 0) put(base, op.factor + op.sgn * t(base + st));
+        else put(base + st * (nd + 1), op.factor + op.sgn * t(base + st * nd));
+        break;
+      case BT_STAG_D:
+        if (op.side == 0) put(base, op.factor);
+        else {
+          put(base + st * (nd + 1), t(base + st * (nd - 1)));
+          put(base + st * nd, op.factor);
+        }
+        break;
+      case BT_STAG_N:
+        if (op.side == 0) put(base, 1. * op.factor + t(base + st));
+        else {
+          put(base + st * (nd + 1), t(base + st * nd));
+          put(base + st * nd, 1. * op.factor + t(base + st * (nd - 1)));
+        }
+        break;
+    }
+    return;
+  }
   switch (op.type) {
     case BT_PERIODIC:
       if (op.side == 0) f[base] = f[base + st * nd];
       else f[base + st * (nd + 1)] = f[base + st];
       break;
     case BT_CENTERED:
-      if (op.side == 0) f[base] = op.factor + op.sgn * f[base + st];
-      else f[base + st * (nd + 1)] = op.factor + op.sgn * f[base + st * nd];
+      if (op.side == 0) f[base] = (T)(op.factor + op.sgn * (double)f[base + st]);
+      else f[base + st * (nd + 1)] = (T)(op.factor + op.sgn * (double)f[base + st * nd]);
       break;
     case BT_STAG_D:
-      if (op.side == 0) f[base] = op.factor;
+      if (op.side == 0) f[base] = (T)op.factor;
       else {
         f[base + st * (nd + 1)] = f[base + st * (nd - 1)];
-        f[base + st * nd] = op.factor;
+        f[base + st * nd] = (T)op.factor;
       }
       break;
     case BT_STAG_N:
-      if (op.side == 0) f[base] = 1. * op.factor + f[base + st];
+      if (op.side == 0) f[base] = (T)(1. * op.factor + (double)f[base + st]);
       else {
         f[base + st * (nd + 1)] = f[base + st * nd];
-        f[base + st * nd] = 1. * op.factor + f[base + st * (nd - 1)];
+        f[base + st * nd] = (T)(1. * op.factor + (double)f[base + st * (nd - 1)]);
       }
       break;
   }
 }
 
 // updt_rhs_b (bound.f90:514-598): p(plane) += rhsb(:,:,side)*norm
-__global__ void k_rhs_b(Layout L, double* p, const double* r, int dir, int plane, double norm) {
+template <typename T>
+__global__ void k_rhs_b(Layout L, T* p, const double* r, int dir, int plane, double norm) {
   const int e1 = dir == 0 ? L.n2 : L.n1;
   const int e2 = dir == 2 ? L.n2 : L.n3;
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -374,18 +460,20 @@ __global__ void k_rhs_b(Layout L, double* p, const double* r, int dir, int plane
   if (dir == 0) c = L.at(plane, a, b);
   else if (dir == 1) c = L.at(a, plane, b);
   else c = L.at(a, b, plane);
-  p[c] = p[c] + r[idx] * norm;
+  p[c] = (T)((double)p[c] + r[idx] * norm);
 }
 
 // divergence norms (chkdiv.f90:15-54): partial sum of |div|*vol and max |div|
-__global__ void k_chkdiv(Layout L, const double* u, const double* v, const double* w, const double* dzfi,
-                         double dxi, double dyi, double* partials, int nb) {
+template <typename T>
+__global__ void k_chkdiv(Layout L, const T* u, const T* v, const T* w, const double* dzfi, double dxi, double dyi,
+                         double* partials, int nb) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int j = blockIdx.y + 1, k = blockIdx.z + 1;
   double tot = 0., mx = 0.;
   if (i <= L.n1) {
     const long c = L.at(i, j, k);
-    const double div = (w[c] - w[c - L.sz]) * dzfi[k] + (v[c] - v[c - L.sy]) * dyi + (u[c] - u[c - 1]) * dxi;
+    const double wc = w[c], wm = w[c - L.sz], vc = v[c], vm = v[c - L.sy], uc = u[c], um = u[c - 1];
+    const double div = (wc - wm) * dzfi[k] + (vc - vm) * dyi + (uc - um) * dxi;
     mx = fabs(div);
     tot = fabs(div) / (dxi * dyi * dzfi[k]);
   }
@@ -394,25 +482,29 @@ __global__ void k_chkdiv(Layout L, const double* u, const double* v, const doubl
 }
 
 // CFL estimate (chkdt.f90:16-78): partial max of dti
-__global__ void k_chkdt(Layout L, const double* u, const double* v, const double* w, const double* dzci,
-                        const double* dzfi, double dxi, double dyi, double* partials) {
+template <typename T>
+__global__ void k_chkdt(Layout L, const T* u, const T* v, const T* w, const double* dzci, const double* dzfi,
+                        double dxi, double dyi, double* partials) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int j = blockIdx.y + 1, k = blockIdx.z + 1;
   double dti = 0.;
   if (i <= L.n1) {
     const long c = L.at(i, j, k);
     const long sy = L.sy, sz = L.sz;
-    const double ux = fabs(u[c]);
-    const double vx = 0.25 * fabs(v[c] + v[c - sy] + v[c + 1] + v[c + 1 - sy]);
-    const double wx = 0.25 * fabs(w[c] + w[c - sz] + w[c + 1] + w[c + 1 - sz]);
+    auto U = [&](long o) { return (double)u[o]; };
+    auto V = [&](long o) { return (double)v[o]; };
+    auto W = [&](long o) { return (double)w[o]; };
+    const double ux = fabs(U(c));
+    const double vx = 0.25 * fabs(V(c) + V(c - sy) + V(c + 1) + V(c + 1 - sy));
+    const double wx = 0.25 * fabs(W(c) + W(c - sz) + W(c + 1) + W(c + 1 - sz));
     const double dtix = ux * dxi + vx * dyi + wx * dzfi[k];
-    const double uy = 0.25 * fabs(u[c] + u[c + sy] + u[c - 1 + sy] + u[c - 1]);
-    const double vy = fabs(v[c]);
-    const double wy = 0.25 * fabs(w[c] + w[c + sy] + w[c + sy - sz] + w[c - sz]);
+    const double uy = 0.25 * fabs(U(c) + U(c + sy) + U(c - 1 + sy) + U(c - 1));
+    const double vy = fabs(V(c));
+    const double wy = 0.25 * fabs(W(c) + W(c + sy) + W(c + sy - sz) + W(c - sz));
     const double dtiy = uy * dxi + vy * dyi + wy * dzfi[k];
-    const double uz = 0.25 * fabs(u[c] + u[c - 1] + u[c - 1 + sz] + u[c + sz]);
-    const double vz = 0.25 * fabs(v[c] + v[c - sy] + v[c - sy + sz] + v[c + sz]);
-    const double wz = fabs(w[c]);
+    const double uz = 0.25 * fabs(U(c) + U(c - 1) + U(c - 1 + sz) + U(c + sz));
+    const double vz = 0.25 * fabs(V(c) + V(c - sy) + V(c - sy + sz) + V(c + sz));
+    const double wz = fabs(W(c));
     const double dtiz = uz * dxi + vz * dyi + wz * dzci[k];
     dti = fmax(dti, fmax(dtix, fmax(dtiy, dtiz)));
   }
@@ -421,11 +513,12 @@ __global__ void k_chkdt(Layout L, const double* u, const double* v, const double
 }
 
 // bulk mean of one field (utils.f90:15-42)
-__global__ void k_bulk_mean(Layout L, const double* q, const double* gvr, double* partials) {
+template <typename T>
+__global__ void k_bulk_mean(Layout L, const T* q, const double* gvr, double* partials) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int j = blockIdx.y + 1, k = blockIdx.z + 1;
   double s = 0.;
-  if (i <= L.n1) s = q[L.at(i, j, k)] * gvr[k];
+  if (i <= L.n1) s = (double)q[L.at(i, j, k)] * gvr[k];
   const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
   double d1, d2;
   block_sum3(s, 0., 0., partials + bid, &d1, &d2, true);
@@ -436,12 +529,16 @@ __global__ void k_bulk_mean(Layout L, const double* q, const double* gvr, double
 // ---------------------------------------------------------------------------
 static dim3 row_grid(const Layout& L, int bx) { return dim3(cdiv(L.n1, bx), L.n2, L.n3); }
 
-int launch_mom_rk(cans_ctx* c, const MomLaunch& m) {
+template <typename S>
+int launch_mom_rk(cans_ctx* c, const MomLaunch<S>& m) {
   const Layout& L = c->L;
-  MomArgs a;
+  MomArgs<S> a;
   a.L = L;
   a.u = m.u; a.v = m.v; a.w = m.w; a.p = m.p;
   a.uo = m.uo; a.vo = m.vo; a.wo = m.wo;
+  a.eu = m.eu; a.ev = m.ev; a.ew = m.ew;
+  a.delta = m.eu != nullptr;
+  a.divu = m.divu;
   a.ru = m.ru; a.rv = m.rv; a.rw = m.rw;
   a.dzci = m.dzci; a.dzfi = m.dzfi;
   a.gvr_c = m.gvr_c; a.gvr_f = m.gvr_f;
@@ -465,7 +562,7 @@ int launch_mom_rk(cans_ctx* c, const MomLaunch& m) {
   a.partials = c->partials;
   a.read_ro = m.read_ro && m.rkpar[1] != 0.;
   a.write_ro = m.write_ro;
-  k_mom_rk<<<grid, dim3(MOM_TX, MOM_TY), 0, c->stream>>>(a);
+  k_mom_rk<S><<<grid, dim3(MOM_TX, MOM_TY), 0, c->stream>>>(a);
   CANS_LAUNCH_CHECK("k_mom_rk");
   if (a.sum_mask)
     return reduce_partials(c, c->partials, (int)nb, 3, 0, c->dscal, c->partials + 3 * nb, 1, a.sum_mask, m.velf);
@@ -474,40 +571,47 @@ int launch_mom_rk(cans_ctx* c, const MomLaunch& m) {
   return CANS_OK;
 }
 
-int launch_add_forcing(cans_ctx* c, const double* src, double* dst, int comp, int apply) {
+template <typename T>
+int launch_add_forcing(cans_ctx* c, const T* src, T* dst, int comp, int apply) {
   const Layout& L = c->L;
-  k_add_forcing<<<row_grid(L, 256), 256, 0, c->stream>>>(L, src, dst, c->dscal, comp, apply);
+  k_add_forcing<T><<<row_grid(L, 256), 256, 0, c->stream>>>(L, src, dst, c->dscal, comp, apply);
   CANS_LAUNCH_CHECK("k_add_forcing");
   return CANS_OK;
 }
 
-int launch_fillps(cans_ctx* c, const double* u, const double* v, const double* w, double* p, const double* dzfi,
-                  double dti, const double* dli) {
+template <typename S>
+int launch_fillps(cans_ctx* c, const double* u, const double* v, const double* w, const S* eu, const S* ev,
+                  const S* ew, const double* f, S* p, const double* dzfi, double dti, const double* dli) {
   const Layout& L = c->L;
-  k_fillps<<<row_grid(L, 256), 256, 0, c->stream>>>(L, u, v, w, p, dzfi, dti, dti * dli[0], dti * dli[1]);
+  k_fillps<S><<<row_grid(L, 256), 256, 0, c->stream>>>(L, u, v, w, eu, ev, ew, f, p, dzfi, dti, dti * dli[0],
+                                                         dti * dli[1]);
   CANS_LAUNCH_CHECK("k_fillps");
   return CANS_OK;
 }
 
-int launch_correc(cans_ctx* c, const double* p, const double* su, const double* sv, const double* sw, double* du,
-                  double* dv, double* dw, const double* dzci, double dt, const double* dli, double* p_update) {
+template <typename S>
+int launch_correc(cans_ctx* c, const S* p, const double* su, const double* sv, const double* sw, const S* eu,
+                  const S* ev, const S* ew, const double* f, double* du, double* dv, double* dw, const double* dzci,
+                  double dt, const double* dli, S* p_update) {
   const Layout& L = c->L;
-  dim3 grid(cdiv(L.n1 + 2, 256), L.n2 + 2, L.n3 + 2);
-  k_correc<<<grid, 256, 0, c->stream>>>(L, p, su, sv, sw, du, dv, dw, dzci, dt, dt * dli[0], dt * dli[1],
-                                         p_update);
+  dim3 grid(cdiv(L.n1 + 2, CORREC_TX), cdiv(L.n2 + 2, CORREC_TY), cdiv(L.n3 + 2, CORREC_KC));
+  k_correc<S><<<grid, dim3(CORREC_TX, CORREC_TY), 0, c->stream>>>(L, p, su, sv, sw, eu, ev, ew, f, du, dv, dw, dzci,
+                                                                   dt, dt * dli[0], dt * dli[1], p_update);
   CANS_LAUNCH_CHECK("k_correc");
   return CANS_OK;
 }
 
-int launch_updatep(cans_ctx* c, const double* pp, double* p, const double* dzci, const double* dzfi, double alpha,
+template <typename T>
+int launch_updatep(cans_ctx* c, const T* pp, T* p, const double* dzci, const double* dzfi, double alpha,
                    const double* dli, int impdiff) {
   const Layout& L = c->L;
-  k_updatep<<<row_grid(L, 256), 256, 0, c->stream>>>(L, pp, p, dzci, dzfi, alpha, dli[0] * dli[0],
-                                                       dli[1] * dli[1], impdiff);
+  k_updatep<T><<<row_grid(L, 256), 256, 0, c->stream>>>(L, pp, p, dzci, dzfi, alpha, dli[0] * dli[0],
+                                                          dli[1] * dli[1], impdiff);
   CANS_LAUNCH_CHECK("k_updatep");
   return CANS_OK;
 }
 
+template <typename T>
 int launch_bound_dir(cans_ctx* c, const PlaneOps& ops) {
   if (ops.nops == 0) return CANS_OK;
   const Layout& L = c->L;
@@ -515,28 +619,30 @@ int launch_bound_dir(cans_ctx* c, const PlaneOps& ops) {
   const long e1 = dir == 0 ? L.n2 + 2 : L.n1 + 2;
   const long e2 = dir == 2 ? L.n2 + 2 : L.n3 + 2;
   dim3 grid(cdiv(e1 * e2, 256), ops.nops);
-  k_bound<<<grid, 256, 0, c->stream>>>(L, ops);
+  k_bound<T><<<grid, 256, 0, c->stream>>>(L, ops);
   CANS_LAUNCH_CHECK("k_bound");
   return CANS_OK;
 }
 
-int launch_rhs_b(cans_ctx* c, double* p, const double* r, int dir, int plane, double norm) {
+template <typename T>
+int launch_rhs_b(cans_ctx* c, T* p, const double* r, int dir, int plane, double norm) {
   const Layout& L = c->L;
   const long e1 = dir == 0 ? L.n2 : L.n1;
   const long e2 = dir == 2 ? L.n2 : L.n3;
-  k_rhs_b<<<cdiv(e1 * e2, 256), 256, 0, c->stream>>>(L, p, r, dir, plane, norm);
+  k_rhs_b<T><<<cdiv(e1 * e2, 256), 256, 0, c->stream>>>(L, p, r, dir, plane, norm);
   CANS_LAUNCH_CHECK("k_rhs_b");
   return CANS_OK;
 }
 
-int run_chkdiv(cans_ctx* c, const double* u, const double* v, const double* w, const double* dzfi, const double* dli,
-               double* tot, double* mx) {
+template <typename T>
+int run_chkdiv(cans_ctx* c, const T* u, const T* v, const T* w, const double* dzfi, const double* dli, double* tot,
+               double* mx) {
   const Layout& L = c->L;
   dim3 grid = row_grid(L, 256);
   const long nb = (long)grid.x * grid.y * grid.z;
   int rc = ensure_partials(c, 2 * nb);
   if (rc) return rc;
-  k_chkdiv<<<grid, 256, 0, c->stream>>>(L, u, v, w, dzfi, dli[0], dli[1], c->partials, (int)nb);
+  k_chkdiv<T><<<grid, 256, 0, c->stream>>>(L, u, v, w, dzfi, dli[0], dli[1], c->partials, (int)nb);
   CANS_LAUNCH_CHECK("k_chkdiv");
   if ((rc = reduce_partials(c, c->partials, (int)nb, 1, 0, c->dscal + 8, c->partials + 2 * nb))) return rc;
   if ((rc = reduce_partials(c, c->partials + nb, (int)nb, 1, 1, c->dscal + 9, c->partials + 2 * nb + 1024)))
@@ -549,14 +655,15 @@ int run_chkdiv(cans_ctx* c, const double* u, const double* v, const double* w, c
   return CANS_OK;
 }
 
-int run_chkdt(cans_ctx* c, const double* u, const double* v, const double* w, const double* dzci, const double* dzfi,
+template <typename T>
+int run_chkdt(cans_ctx* c, const T* u, const T* v, const T* w, const double* dzci, const double* dzfi,
               const double* dl, double* dti_out) {
   const Layout& L = c->L;
   dim3 grid = row_grid(L, 256);
   const long nb = (long)grid.x * grid.y * grid.z;
   int rc = ensure_partials(c, nb);
   if (rc) return rc;
-  k_chkdt<<<grid, 256, 0, c->stream>>>(L, u, v, w, dzci, dzfi, 1. / dl[0], 1. / dl[1], c->partials);
+  k_chkdt<T><<<grid, 256, 0, c->stream>>>(L, u, v, w, dzci, dzfi, 1. / dl[0], 1. / dl[1], c->partials);
   CANS_LAUNCH_CHECK("k_chkdt");
   if ((rc = reduce_partials(c, c->partials, (int)nb, 1, 1, c->dscal + 10, c->partials + nb))) return rc;
   CANS_HIP_TRY(hipMemcpyAsync(dti_out, c->dscal + 10, sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -564,18 +671,39 @@ int run_chkdt(cans_ctx* c, const double* u, const double* v, const double* w, co
   return CANS_OK;
 }
 
-int run_bulk_mean(cans_ctx* c, const double* q, const double* gvr, double* mean) {
+template <typename T>
+int run_bulk_mean(cans_ctx* c, const T* q, const double* gvr, double* mean) {
   const Layout& L = c->L;
   dim3 grid = row_grid(L, 256);
   const long nb = (long)grid.x * grid.y * grid.z;
   int rc = ensure_partials(c, nb);
   if (rc) return rc;
-  k_bulk_mean<<<grid, 256, 0, c->stream>>>(L, q, gvr, c->partials);
+  k_bulk_mean<T><<<grid, 256, 0, c->stream>>>(L, q, gvr, c->partials);
   CANS_LAUNCH_CHECK("k_bulk_mean");
   if ((rc = reduce_partials(c, c->partials, (int)nb, 1, 0, c->dscal + 11, c->partials + nb))) return rc;
   CANS_HIP_TRY(hipMemcpyAsync(mean, c->dscal + 11, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   CANS_HIP_TRY(hipStreamSynchronize(c->stream));
   return CANS_OK;
 }
+
+#define CANS_STENCIL_INSTANTIATE(S)                                                                              \
+  template int launch_mom_rk<S>(cans_ctx*, const MomLaunch<S>&);                                                \
+  template int launch_add_forcing<S>(cans_ctx*, const S*, S*, int, int);                                        \
+  template int launch_fillps<S>(cans_ctx*, const double*, const double*, const double*, const S*, const S*,     \
+                                const S*, const double*, S*, const double*, double, const double*);             \
+  template int launch_correc<S>(cans_ctx*, const S*, const double*, const double*, const double*, const S*,     \
+                                const S*, const S*, const double*, double*, double*, double*, const double*,    \
+                                double, const double*, S*);                                                      \
+  template int launch_updatep<S>(cans_ctx*, const S*, S*, const double*, const double*, double, const double*, int); \
+  template int launch_bound_dir<S>(cans_ctx*, const PlaneOps&);                                                  \
+  template int launch_rhs_b<S>(cans_ctx*, S*, const double*, int, int, double);
+CANS_STENCIL_INSTANTIATE(double)
+CANS_STENCIL_INSTANTIATE(float)
+// reductions read the velocity, which is always double
+template int run_chkdiv<double>(cans_ctx*, const double*, const double*, const double*, const double*,
+                                const double*, double*, double*);
+template int run_chkdt<double>(cans_ctx*, const double*, const double*, const double*, const double*,
+                               const double*, const double*, double*);
+template int run_bulk_mean<double>(cans_ctx*, const double*, const double*, double*);
 
 }  // namespace cans

@@ -10,6 +10,7 @@ module cans_hip
   use, intrinsic :: iso_c_binding
   implicit none
   integer(c_int), parameter :: CANS_FP64 = 0, CANS_MIXED = 1
+  integer(c_int), parameter :: CANS_FIELD_VELOCITY = 0, CANS_FIELD_PRESSURE = 1
   integer(c_int), parameter :: CANS_HALO_ABSENT = -99, CANS_HALO_LOWER = -1, &
                                CANS_HALO_NONE = 0, CANS_HALO_UPPER = 1
   !
@@ -53,9 +54,21 @@ module cans_hip
     type(c_ptr) function cans_last_error() bind(C, name='cans_last_error')
       import :: c_ptr
     end function
+    integer(c_int) function cans_ctx_elem_size(ctx, kind, bytes) bind(C, name='cans_ctx_elem_size')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: kind
+      integer(c_int), intent(out) :: bytes
+    end function
     integer(c_int) function cans_field_alloc(ctx, field) bind(C, name='cans_field_alloc')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
+      type(c_ptr), intent(out) :: field
+    end function
+    integer(c_int) function cans_field_alloc_kind(ctx, kind, field) bind(C, name='cans_field_alloc_kind')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: kind
       type(c_ptr), intent(out) :: field
     end function
     integer(c_int) function cans_field_free(ctx, field) bind(C, name='cans_field_free')

@@ -6,6 +6,8 @@
 ! and writes the fields back in the same format.
 !
 ! usage: drive_step <dir>   (reads <dir>/case.nml and <dir>/fld_{u,v,w,p}.bin)
+! The namelist's optional is_mixed = .true. selects the mixed-precision context
+! (pressure-like arrays in FP32, velocity in FP64); the call sequence is the same.
 program drive_step
   use, intrinsic :: iso_c_binding
   use cans_hip
@@ -18,8 +20,9 @@ program drive_step
   real(dp) :: l(3), gr, visci, dt, time, dtrk, dtrki, alpha
   character(len=1) :: cbcvel(0:1,3,3), cbcpre(0:1,3)
   real(dp) :: bcvel(0:1,3,3), bcpre(0:1,3), bforce(3), velf(3)
-  logical :: is_forced(3)
-  namelist /dns/ ng, l, gtype, gr, visci, dt, nstep, cbcvel, cbcpre, bcvel, bcpre, bforce, is_forced, velf
+  logical :: is_forced(3), is_mixed
+  integer(c_int) :: es_p
+  namelist /dns/ ng, l, gtype, gr, visci, dt, nstep, cbcvel, cbcpre, bcvel, bcpre, bforce, is_forced, velf, is_mixed
   character(len=512) :: dir
   integer(c_int) :: n(3), nb(0:1,3), is_bound(0:1,3), isf(3)
   real(dp) :: dl(3), dli(3), visc, normfft, f(3), divtot, divmax
@@ -34,7 +37,7 @@ program drive_step
   !
   call get_command_argument(1, dir)
   cbcvel = 'P'; cbcpre = 'P'; bcvel = 0.; bcpre = 0.; bforce = 0.; is_forced = .false.; velf = 0.
-  gtype = 1; gr = 0.; nstep = 1
+  gtype = 1; gr = 0.; nstep = 1; is_mixed = .false.
   open(newunit=iunit, file=trim(dir)//'/case.nml', status='old')
   read(iunit, nml=dns)
   close(iunit)
@@ -74,18 +77,21 @@ program drive_step
   end do
   call cans_check(cans_tridmatrix(cbcpre(:,3), ng(3), dzci, dzfi, 'c', a, b, c), 'cans_tridmatrix')
   ! device context, fields and small arrays (what `!$acc enter data` provides, main.f90:388)
-  call cans_check(cans_ctx_create(n, 0, CANS_FP64, c_null_ptr, ctx), 'cans_ctx_create')
+  call cans_check(cans_ctx_create(n, 0, merge(CANS_MIXED, CANS_FP64, is_mixed), c_null_ptr, ctx), 'cans_ctx_create')
   call cans_check(cans_fftini(ctx, ng, cbcpre(:,1:2), ['c','c'], plan, normfft), 'cans_fftini')
   call cans_check(cans_field_alloc(ctx, u), 'alloc'); call cans_check(cans_field_alloc(ctx, v), 'alloc')
-  call cans_check(cans_field_alloc(ctx, w), 'alloc'); call cans_check(cans_field_alloc(ctx, p), 'alloc')
-  call cans_check(cans_field_alloc(ctx, pp), 'alloc')
+  call cans_check(cans_field_alloc(ctx, w), 'alloc')
+  call cans_check(cans_field_alloc_kind(ctx, CANS_FIELD_PRESSURE, p), 'alloc')
+  call cans_check(cans_field_alloc_kind(ctx, CANS_FIELD_PRESSURE, pp), 'alloc')
+  call cans_check(cans_ctx_elem_size(ctx, CANS_FIELD_PRESSURE, es_p), 'cans_ctx_elem_size')
   call dev_copy(dzci, size(dzci), dzci_d); call dev_copy(dzfi, size(dzfi), dzfi_d)
   call dev_copy(gvr_c, size(gvr_c), gvr_c_d); call dev_copy(gvr_f, size(gvr_f), gvr_f_d)
   call dev_copy(lam, size(lam), lam_d)
   call dev_copy(a, ng(3), a_d); call dev_copy(b, ng(3), b_d); call dev_copy(c, ng(3), c_d)
-  call cans_check(cans_device_alloc(ctx, int(8*product(ng), c_long), ru), 'alloc')
-  call cans_check(cans_device_alloc(ctx, int(8*product(ng), c_long), rv), 'alloc')
-  call cans_check(cans_device_alloc(ctx, int(8*product(ng), c_long), rw), 'alloc')
+  ! dudtrko & co. hold the pressure-kind element type
+  call cans_check(cans_device_alloc(ctx, int(es_p, c_long)*product(int(ng, c_long)), ru), 'alloc')
+  call cans_check(cans_device_alloc(ctx, int(es_p, c_long)*product(int(ng, c_long)), rv), 'alloc')
+  call cans_check(cans_device_alloc(ctx, int(es_p, c_long)*product(int(ng, c_long)), rw), 'alloc')
   ! restart (main.f90:382-385) into halo'd host arrays, then upload
   allocate(host(0:ng(1)+1,0:ng(2)+1,0:ng(3)+1))
   flds = [u, v, w, p]

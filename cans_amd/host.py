@@ -93,7 +93,11 @@ def tridmatrix(bc, n, dzci, dzfi, c_or_f):
 
 
 class Context:
-    """cans_ctx: one grid, one HIP stream, one precision mode."""
+    """cans_ctx: one grid, one HIP stream, one precision mode.
+
+    ``precision`` is ``CANS_FP64`` (every array double) or ``CANS_MIXED``
+    (velocity double, pressure-like arrays float; every kernel computes in
+    double)."""
 
     def __init__(self, n, ld=None, precision=LB.CANS_FP64, stream=None, device="cuda"):
         torch = _torch()
@@ -101,6 +105,13 @@ class Context:
         self.device = torch.device(device)
         torch.cuda.init()
         self.n = tuple(int(x) for x in n)
+        self.precision = int(precision)
+        # storage type per field kind (include/cans_hip.h: velocity / pressure)
+        mixed = self.precision == LB.CANS_MIXED
+        self.dtypes = {LB.CANS_FIELD_VELOCITY: torch.float64,
+                       LB.CANS_FIELD_PRESSURE: torch.float32 if mixed else torch.float64}
+        self.p_dtype = self.dtypes[LB.CANS_FIELD_PRESSURE]
+        self.p_es = 4 if mixed else 8
         if ld is None:  # interior rows start 256-byte aligned
             ld = ((self.n[0] + 2 + 31) // 32) * 32
         if stream is None:
@@ -129,8 +140,8 @@ class Context:
         LB.check(LB.lib().cans_ctx_sync(self.h), "cans_ctx_sync")
 
     # halo'd field backed by a torch tensor; .ptr is the address of element (0,0,0)
-    def field(self, fill=0.0):
-        return DeviceField(self, fill)
+    def field(self, fill=0.0, kind=LB.CANS_FIELD_VELOCITY):
+        return DeviceField(self, fill, kind)
 
     def darray(self, a, dtype=None):
         torch = self.torch
@@ -138,12 +149,14 @@ class Context:
 
 
 class DeviceField:
-    def __init__(self, ctx: Context, fill=0.0):
+    """Halo'd field of one kind (velocity or pressure) in the context's storage type for it."""
+
+    def __init__(self, ctx: Context, fill=0.0, kind=LB.CANS_FIELD_VELOCITY):
         torch = ctx.torch
         self.ctx = ctx
-        es = 8
-        off = 256 // es - 1
-        self.buf = torch.full((ctx.elems + 2 * 256 // es,), float(fill), dtype=torch.float64, device=ctx.device)
+        self.dtype = ctx.dtypes[kind]
+        es = 8 if self.dtype == torch.float64 else 4
+        self.buf = torch.full((ctx.elems + 2 * 256 // es,), float(fill), dtype=self.dtype, device=ctx.device)
         base = self.buf.data_ptr()
         # element 1 of every row on a 256-byte boundary
         shift = ((-(base + es)) % 256) // es
@@ -151,17 +164,18 @@ class DeviceField:
         self.ptr = base + es * shift
         n1, n2, n3 = ctx.n
         self.view = self.buf[shift:shift + ctx.elems].view(n3 + 2, n2 + 2, ctx.sy)[:, :, :n1 + 2]
-        del off
 
     def numpy(self):
-        """Fortran-ordered (n1+2, n2+2, n3+2) float64 copy on the host."""
-        return np.asfortranarray(self.view.cpu().numpy().transpose(2, 1, 0))
+        """Fortran-ordered (n1+2, n2+2, n3+2) float64 copy on the host (FP32 storage widens exactly)."""
+        return np.asfortranarray(self.view.cpu().numpy().astype(np.float64).transpose(2, 1, 0))
 
     def set(self, a):
+        """Store a float64 host array (rounded to nearest for FP32 storage)."""
         a = np.asarray(a, dtype=np.float64)
         n1, n2, n3 = self.ctx.n
         assert a.shape == (n1 + 2, n2 + 2, n3 + 2)
-        self.view.copy_(self.ctx.torch.from_numpy(np.ascontiguousarray(a.transpose(2, 1, 0))))
+        t = self.ctx.torch.from_numpy(np.ascontiguousarray(a.transpose(2, 1, 0)))
+        self.view.copy_(t.to(self.dtype))
 
     def interior(self):
         return self.numpy()[1:-1, 1:-1, 1:-1]
@@ -206,17 +220,18 @@ class Plan:
 class Simulation:
     """One case on one GPU: set-up of src/main.f90:241-395 and its RK3 loop."""
 
-    def __init__(self, case: Case, ctx: Context = None):
+    def __init__(self, case: Case, ctx: Context = None, precision=LB.CANS_FP64):
         self.case = case
         self.grid = Grid(case)
-        self.ctx = ctx or Context(case.n)
+        self.ctx = ctx or Context(case.n, precision=precision)
         ctx = self.ctx
         g = self.grid
         self.plan = Plan(ctx, g, case.cbcpre, case.bcpre, "ccc")
-        self.u, self.v, self.w, self.p, self.pp = (ctx.field() for _ in range(5))
+        self.u, self.v, self.w = (ctx.field() for _ in range(3))
+        self.p, self.pp = (ctx.field(kind=LB.CANS_FIELD_PRESSURE) for _ in range(2))
         torch = ctx.torch
         n1, n2, n3 = case.n
-        self.rhs_o = [torch.zeros(n1 * n2 * n3, dtype=torch.float64, device=ctx.device) for _ in range(3)]
+        self.rhs_o = [torch.zeros(n1 * n2 * n3, dtype=ctx.p_dtype, device=ctx.device) for _ in range(3)]
         self.dzci_d = ctx.darray(g.dzci)
         self.dzfi_d = ctx.darray(g.dzfi)
         self.gvr_c_d = ctx.darray(g.grid_vol_ratio_c)

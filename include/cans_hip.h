@@ -17,7 +17,16 @@
  *   - nb(0:1,3): on one GPU a value >= 0 marks a periodic neighbour (this rank),
  *     a negative value (MPI_PROC_NULL) marks none.
  * Device pointers are marked _d, host pointers _h; unmarked scalars are by value.
- * Field storage is double (CANS_FP64); the mixed mode is documented per call.
+ *
+ * Precision modes (cans_ctx_create):
+ *   CANS_FP64  - every array is double.
+ *   CANS_MIXED - velocity fields (u, v, w) stay double; pressure-like arrays
+ *                (p, pp, the field of cans_solver / cans_solve_helmholtz /
+ *                cans_updt_rhs_b / cans_fillps's output, dudtrko, dvdtrko,
+ *                dwdtrko) are float, as are the transform intermediates.  Every
+ *                kernel computes in double; only stores to float arrays round.
+ *                Small arrays (grid, eigenvalues, a/b/c, rhsb, bc values) stay
+ *                double.  cans_ctx_elem_size reports the size per kind.
  */
 #ifndef CANS_HIP_H
 #define CANS_HIP_H
@@ -31,6 +40,7 @@ typedef struct cans_plan cans_plan;
 
 enum { CANS_OK = 0, CANS_EINVAL = -1, CANS_EHIP = -2, CANS_ENOMEM = -3, CANS_EUNSUPPORTED = -4 };
 enum { CANS_FP64 = 0, CANS_MIXED = 1 };
+enum { CANS_FIELD_VELOCITY = 0, CANS_FIELD_PRESSURE = 1 };
 enum { CANS_HALO_ABSENT = -99, CANS_HALO_LOWER = -1, CANS_HALO_NONE = 0, CANS_HALO_UPPER = 1 };
 
 /* ---------------------------------------------------------------------------
@@ -47,10 +57,17 @@ int cans_ctx_layout(const cans_ctx *ctx, long *sy, long *sz, long *field_elems);
 int cans_ctx_sync(cans_ctx *ctx);
 const char *cans_last_error(void);
 
-/* field = device pointer to element (0,0,0) of a halo'd field owned by the context */
+/* bytes per element of a field kind (CANS_FIELD_*) in this context */
+int cans_ctx_elem_size(const cans_ctx *ctx, int kind, int *bytes);
+
+/* field = device pointer to element (0,0,0) of a halo'd field owned by the context;
+ * cans_field_alloc allocates a velocity field, cans_field_alloc_kind either kind */
 int cans_field_alloc(cans_ctx *ctx, void **field_d);
+int cans_field_alloc_kind(cans_ctx *ctx, int kind, void **field_d);
 int cans_field_free(cans_ctx *ctx, void *field_d);
-/* host arrays are the reference's contiguous (0:n1+1,0:n2+1,0:n3+1) Fortran arrays */
+/* host arrays are the reference's contiguous (0:n1+1,0:n2+1,0:n3+1) Fortran arrays
+ * of double; float fields round on upload.  In a CANS_MIXED context the field
+ * must come from cans_field_alloc_kind (its element type is looked up) */
 int cans_field_upload(cans_ctx *ctx, void *field_d, const double *host_h);
 int cans_field_download(cans_ctx *ctx, const void *field_d, double *host_h);
 

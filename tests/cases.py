@@ -25,6 +25,9 @@ def params(name, ng=None):
     elif name == "duct":  # periodic x, walls in y and z
         p = dict(ng=(64, 32, 48), l=(2.0, 1.0, 1.0), gtype=1, gr=1.2, cbcvel="PPDDDD" * 3,
                  cbcpre="PPNNNN", visci=400.0, is_forced=(1, 0, 0), velf=(1.0, 0.0, 0.0))
+    elif name == "box":  # no-slip walls on all six faces (Neumann pressure everywhere, DCT in x and y)
+        p = dict(ng=(16, 16, 12), l=(1.0, 1.0, 1.0), gtype=1, gr=0.5, cbcvel="DDDDDD" * 3,
+                 cbcpre="NNNNNN", visci=100.0)
     elif name == "cavity_dd":  # Dirichlet pressure walls in y (DST), Neumann in z
         p = dict(ng=(32, 32, 24), l=(1.0, 1.0, 1.0), gtype=1, gr=0.5, cbcvel="PPNNDD" * 3,
                  cbcpre="PPDDNN", visci=100.0)

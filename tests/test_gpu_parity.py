@@ -178,8 +178,9 @@ def cases_rk(irk):
     return RKCOEFF[irk - 1]
 
 
+# the walled box (no periodic x) takes the unfused divergence path of the fused substep
 STEP_CASES = [("tgv", (32, 32, 32), 1e-3, 3), ("channel", (64, 32, 48), 2e-4, 3), ("duct", (32, 16, 24), 1e-3, 2),
-              ("cavity_dd", (16, 16, 12), 1e-3, 2)]
+              ("cavity_dd", (16, 16, 12), 1e-3, 2), ("box", (16, 16, 12), 1e-3, 2)]
 
 
 @pytest.mark.parametrize("name,ng,dt,nsteps", STEP_CASES, ids=[c[0] for c in STEP_CASES])
