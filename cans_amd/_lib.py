@@ -59,6 +59,7 @@ SIGNATURES = {
     "cans_memcpy_d2h": (C.c_int, [_vp, _vp, _vp, C.c_long]),
     "cans_initgrid": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp, _dp, _dp]),
     "cans_eigenvalues": (C.c_int, [C.c_int, _cp, C.c_char, _dp]),
+    "cans_bc_rhs": (C.c_int, [_cp, _dp, _dp, _dp, C.c_char, _dp]),
     "cans_tridmatrix": (C.c_int, [_cp, C.c_int, _dp, _dp, C.c_char, _dp, _dp, _dp]),
     "cans_fftini": (C.c_int, [_vp, _ip, _cp, _cp, C.POINTER(_vp), _dp]),
     "cans_fftend": (C.c_int, [_vp]),
@@ -95,6 +96,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIBPATH):
             raise CansError("libcans_hip.so is not built: run cans_amd.build.build() (or __graft_entry__.build())")
+        # One HIP runtime per process: PyTorch bundles its own libamdhip64, and the
+        # Python host hands torch's streams and memory to this library, so torch's
+        # runtime must be the one loaded first (the library then binds to it).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIBPATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

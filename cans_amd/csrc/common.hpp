@@ -18,8 +18,10 @@ struct Layout {
   __host__ __device__ long cells() const { return (long)n1 * n2 * n3; }
 };
 
+// aligned to its full size so that LDS and global accesses move it as one 16-byte
+// (double) or 8-byte (float) operation
 template <typename T>
-struct cplx {
+struct alignas(2 * sizeof(T)) cplx {
   T re, im;
 };
 

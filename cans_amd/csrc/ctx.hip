@@ -345,6 +345,21 @@ extern "C" int cans_eigenvalues(int n, const char* bc, char c_or_f, double* lam)
   return CANS_OK;
 }
 
+extern "C" int cans_bc_rhs(const char* cbc, const double* bc, const double* dlc, const double* dlf, char c_or_f,
+                           double* rhs) {
+  if (!cbc || !bc || !dlc || !dlf || !rhs || (c_or_f != 'c' && c_or_f != 'f'))
+    return fail(CANS_EINVAL, "cans_bc_rhs: bad argument");
+  // initsolver.f90:189-232: plane ib of the boundary rhs is factor(ib)/dlc(ib)/dlf(ib)
+  for (int ib = 0; ib < 2; ib++) {
+    double factor = 0.;
+    if (cbc[ib] == 'D') factor = c_or_f == 'c' ? -2. * bc[ib] : -bc[ib];
+    else if (cbc[ib] == 'N') factor = (ib == 0 ? 1. : -1.) * (c_or_f == 'c' ? dlc[ib] : dlf[ib]) * bc[ib];
+    else if (cbc[ib] != 'P') return fail(CANS_EINVAL, "cans_bc_rhs: unknown boundary type");
+    rhs[ib] = factor / dlc[ib] / dlf[ib];
+  }
+  return CANS_OK;
+}
+
 extern "C" int cans_tridmatrix(const char* bc, int n, const double* dzci, const double* dzfi, char c_or_f,
                                double* a, double* b, double* c) {
   if (n < 1 || !bc || !dzci || !dzfi || !a || !b || !c) return fail(CANS_EINVAL, "cans_tridmatrix: bad argument");

@@ -8,6 +8,7 @@
 // Spectral ordering matches the reference's GPU build: periodic directions are
 // stored packed as (r0, r_{n/2}, r1, i1, ..., r_{n/2-1}, i_{n/2-1}) (fft.f90:402-451,
 // initsolver.f90:98-117); DCT/DST directions in natural order.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -93,14 +94,38 @@ __device__ __forceinline__ cplx<double> merge_pp(double are, double aim, double 
 }
 
 // ---------------------------------------------------------------------------
-// pass X: rows of the halo'd field and of the work buffer are contiguous along x.
+// Work buffer layout (z-tiled).  Spectral line l = i + n1*j (0-based) belongs to tile
+// l / ZT; a tile stores its ZT lines for all n3 rows contiguously, row k at offset
+// k*ZT.  The z pass then streams each wave's tile as one contiguous block, while the
+// x and y passes still move contiguous chunks of ZT (x) or 2*LINES (y) elements.
+// ---------------------------------------------------------------------------
+constexpr int ZT = 64;
+__host__ __device__ __forceinline__ long woff(long l, int k, int n3) {
+  return (l / ZT) * ((long)ZT * n3) + (long)k * ZT + (l % ZT);
+}
+__host__ __device__ inline long wbuf_elems(long nl, int n3) { return (nl + ZT - 1) / ZT * ZT * (long)n3; }
+
+// element e of one row: a contiguous field row, or row (j,k) of the tiled work buffer
+template <typename T>
+struct RowPtr {
+  T* p = nullptr;
+  long l0 = 0;  // first spectral line of the row (tiled rows)
+  int k = 0, n3 = 0, tiled = 0;
+  __device__ __forceinline__ T* operator+(long e) const { return tiled ? p + woff(l0 + e, k, n3) : p + e; }
+  __device__ __forceinline__ T& operator[](long e) const { return *(*this + e); }
+  __device__ __forceinline__ explicit operator bool() const { return p != nullptr; }
+};
+
+// ---------------------------------------------------------------------------
+// pass X: rows of the halo'd field are contiguous along x; rows of the work
+// buffer are contiguous in chunks of ZT.
 // rows r = j + n2*k (0-based); the complex line of team q carries rows 2q, 2q+1.
 // ---------------------------------------------------------------------------
 template <typename T>
 struct XArgs {
   Layout L;
   T* fld;          // halo'd field (origin)
-  T* W;            // compact work buffer, row r at W + r*n1
+  T* W;            // z-tiled work buffer (woff)
   int N, logN, Tn, pairs, kind;
   const cplx<double>* tw;
   const cplx<double>* mk;
@@ -155,12 +180,26 @@ __global__ void k_xpass(XArgs<T> a) {
   const bool va = ra < nrows, vb = rb < nrows;
   const IxPad ix{team * padded_len(N)};
   const Layout& L = a.L;
-  auto frow = [&](long r) { return a.fld + L.at(1, (int)(r % L.n2) + 1, (int)(r / L.n2) + 1); };
-  auto wrow = [&](long r) { return a.W + r * L.n1; };
-  const T* xa = DIR > 0 ? (va ? frow(ra) : nullptr) : (va ? wrow(ra) : nullptr);
-  const T* xb = DIR > 0 ? (vb ? frow(rb) : nullptr) : (vb ? wrow(rb) : nullptr);
-  T* ya = DIR > 0 ? (va ? wrow(ra) : nullptr) : (va ? frow(ra) : nullptr);
-  T* yb = DIR > 0 ? (vb ? wrow(rb) : nullptr) : (vb ? frow(rb) : nullptr);
+  auto frow = [&](bool v, long r) {
+    RowPtr<T> q;
+    if (v) q.p = a.fld + L.at(1, (int)(r % L.n2) + 1, (int)(r / L.n2) + 1);
+    return q;
+  };
+  auto wrow = [&](bool v, long r) {
+    RowPtr<T> q;
+    if (v) {
+      q.p = a.W;
+      q.l0 = (long)L.n1 * (r % L.n2);
+      q.k = (int)(r / L.n2);
+      q.n3 = L.n3;
+      q.tiled = 1;
+    }
+    return q;
+  };
+  const RowPtr<T> xa = DIR > 0 ? frow(va, ra) : wrow(va, ra);
+  const RowPtr<T> xb = DIR > 0 ? frow(vb, rb) : wrow(vb, rb);
+  const RowPtr<T> ya = DIR > 0 ? wrow(va, ra) : frow(va, ra);
+  const RowPtr<T> yb = DIR > 0 ? wrow(vb, rb) : frow(vb, rb);
   if (DIR > 0) {
     if (va) {
       if (N == 1) {
@@ -295,7 +334,7 @@ __global__ void k_xpass(XArgs<T> a) {
 template <typename T>
 struct YArgs {
   int n1, n2, n3;
-  T* W;
+  T* W;            // z-tiled work buffer (woff)
   int N, logN, Tn, lines, kind;
   const cplx<double>* tw;
   const cplx<double>* mk;
@@ -312,14 +351,14 @@ __global__ void k_ypass(YArgs<T> a) {
   const int i0 = (blockIdx.x % ncolblk) * 2 * LN;
   const int ia = i0 + 2 * c;
   const bool va = ia < a.n1, vb = ia + 1 < a.n1;
-  T* col = a.W + (long)k * a.n1 * a.n2 + ia;
-  const long est = a.n1;  // element stride along y
   const IxStride ix{c, LN};
+  // element e (along y) of column ia in the tiled work buffer; columns ia, ia+1 are adjacent
+  auto col = [&](int e) { return a.W + woff(ia + (long)a.n1 * e, k, a.n3); };
   // (A, B) pair of element e of this line
-  auto ld = [&](int e) -> d2 { return vb ? ld2(col + e * est) : d2((double)col[e * est], 0.); };
+  auto ld = [&](int e) -> d2 { return vb ? ld2(col(e)) : d2((double)*col(e), 0.); };
   auto st = [&](int e, double x, double y) {
-    if (vb) st2(col + e * est, x, y);
-    else col[e * est] = (T)x;
+    if (vb) st2(col(e), x, y);
+    else *col(e) = (T)x;
   };
   if (DIR > 0) {
     if (va) {
@@ -422,9 +461,10 @@ constexpr int CK = 16;
 
 template <typename T>
 struct ZArgs {
-  T* W;
-  long nl;         // number of lines (n1*n2) = stride between consecutive k
+  T* W;            // z-tiled work buffer (woff)
+  long nl;         // number of lines (n1*n2)
   int n;           // rows of the system (n3 - q)
+  int n3;          // rows of a work-buffer tile
   int periodic;
   double norm;
   double pin_eps;  // relative pivot tolerance of the zero-mode pin
@@ -438,8 +478,13 @@ struct ZRow {
   double d, pp, q2;
 };
 
-__device__ __forceinline__ void z_row(int k, int nn, int periodic, double l, double r, double norm, const double* a,
-                                      const double* b, const double* c, double eps, ZRow& s) {
+// The tridiagonal coefficients are read-only during the solve and indexed by the row
+// only (the same address in every lane): reading them through the constant address
+// space lets the compiler use scalar loads instead of one vector load per row.
+typedef const __attribute__((address_space(4))) double* kcoef;
+
+__device__ __forceinline__ void z_row(int k, int nn, int periodic, double l, double r, double norm, kcoef a, kcoef b,
+                                      kcoef c, double eps, ZRow& s) {
   if (k == 1) {
     const double zz = 1. / (b[0] + l);
     s.d = c[0] * zz;
@@ -466,20 +511,22 @@ __device__ __forceinline__ void z_row(int k, int nn, int periodic, double l, dou
   }
 }
 
-template <typename T>
+// PER: periodic z (solver.f90:203-307); a compile-time flag so that the regular
+// kernel carries no Sherman-Morrison state in registers
+template <typename T, bool PER>
 __global__ void __launch_bounds__(256) k_gaussel(ZArgs<T> z) {
   const long line = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (line >= z.nl) return;
   const double l = z.lam[line];
   const double eps = z.pin_eps;
-  T* P = z.W + line;
-  const long ps = z.nl;
-  const int periodic = z.periodic;
+  T* P = z.W + woff(line, 0, z.n3);  // this line's row 0; rows follow at stride ZT
+  constexpr long ps = ZT;
+  constexpr int periodic = PER ? 1 : 0;
   const int nn = periodic ? z.n - 1 : z.n;
   const int nblk = (nn + CK - 1) / CK;
-  const double* a = z.a;
-  const double* b = z.b;
-  const double* c = z.c;
+  const kcoef a = (kcoef)z.a;
+  const kcoef b = (kcoef)z.b;
+  const kcoef c = (kcoef)z.c;
   const double norm = z.norm;
   // ---- forward sweep (checkpoints only), next chunk's loads issued before this chunk's recurrence ----
   ZRow s = {0., 0., 0.};
@@ -496,7 +543,7 @@ __global__ void __launch_bounds__(256) k_gaussel(ZArgs<T> z) {
 #pragma unroll
     for (int t = 0; t < CK; t++)
       if (k0 + t <= nn) z_row(k0 + t, nn, periodic, l, rr[t], norm, a, b, c, eps, s);
-    const long o = (long)bb * ps + line;
+    const long o = (long)bb * z.nl + line;
     z.ckd[o] = s.d;
     z.ckp[o] = s.pp;
     if (periodic) z.ckq[o] = s.q2;
@@ -520,7 +567,7 @@ __global__ void __launch_bounds__(256) k_gaussel(ZArgs<T> z) {
     }
     ZRow r0 = {0., 0., 0.};
     if (bb > 0) {
-      const long o = (long)(bb - 1) * ps + line;
+      const long o = (long)(bb - 1) * z.nl + line;
       r0.d = z.ckd[o];
       r0.pp = z.ckp[o];
       r0.q2 = periodic ? z.ckq[o] : 0.;
@@ -550,7 +597,7 @@ __global__ void __launch_bounds__(256) k_gaussel(ZArgs<T> z) {
           if (periodic) x2 = qv[t] - dd[t] * xq;
         }
         P[(long)(k - 1) * ps] = (T)x;
-        if (periodic) z.X2[(long)(k - 1) * ps + line] = x2;
+        if (periodic) z.X2[(long)(k - 1) * z.nl + line] = x2;
         xn = x;
         xq = x2;
         if (k == 1) {
@@ -573,7 +620,7 @@ __global__ void __launch_bounds__(256) k_gaussel(ZArgs<T> z) {
   P[(long)(m - 1) * ps] = (T)xm;
   for (int k = 1; k <= nn; k++) {
     const long o = (long)(k - 1) * ps;
-    P[o] = (T)((double)P[o] + z.X2[o + line] * xm);
+    P[o] = (T)((double)P[o] + z.X2[(long)(k - 1) * z.nl + line] * xm);
   }
 }
 
@@ -663,7 +710,7 @@ int run_solver(cans_ctx* c, const cans_plan* p, double normfft, const double* la
   const long N = L.cells();
   const long nl = (long)L.n1 * L.n2;
   void* W = nullptr;
-  int rc = ensure_work(c, 1, (size_t)N * sizeof(T), &W);
+  int rc = ensure_work(c, 1, (size_t)wbuf_elems(nl, L.n3) * sizeof(T), &W);
   if (rc) return rc;
   const int q = (c_or_f[2] == 'f' && bc[5] == 'D') ? 1 : 0;
   const int periodic = bc[4] == 'P' && bc[5] == 'P';
@@ -687,6 +734,7 @@ int run_solver(cans_ctx* c, const cans_plan* p, double normfft, const double* la
   z.W = (T*)W;
   z.nl = nl;
   z.n = n;
+  z.n3 = L.n3;
   z.periodic = periodic;
   z.norm = normfft;
   // The reference pins the last unknown of the zero-wavenumber line when its final
@@ -704,7 +752,8 @@ int run_solver(cans_ctx* c, const cans_plan* p, double normfft, const double* la
   z.ckp = z.ckd + nblk * nl;
   z.ckq = z.ckp + nblk * nl;
   z.X2 = periodic ? z.ckq + nblk * nl : nullptr;
-  k_gaussel<T><<<cdiv(nl, 256), 256, 0, c->stream>>>(z);
+  if (periodic) k_gaussel<T, true><<<cdiv(nl, 256), 256, 0, c->stream>>>(z);
+  else k_gaussel<T, false><<<cdiv(nl, 256), 256, 0, c->stream>>>(z);
   CANS_LAUNCH_CHECK("k_gaussel");
   if ((rc = ypass<T>(c, p, (T*)W, -1))) return rc;
   if ((rc = xpass<T>(c, p, fld, (T*)W, -1))) return rc;
@@ -731,9 +780,10 @@ extern "C" int cans_fftini(cans_ctx* c, const int* ng, const char* bcxy, const c
   double nf = 1.;
   for (int d = 0; d < 2; d++) {
     const char b0 = bcxy[2 * d], b1 = bcxy[2 * d + 1];
-    if (c_or_f[d] != 'c') {
+    // face-centred periodic directions use the same R2HC/HC2R pair and norm (fft.f90:260-313)
+    if (c_or_f[d] != 'c' && !(c_or_f[d] == 'f' && b0 == 'P' && b1 == 'P')) {
       delete p;
-      return fail(CANS_EUNSUPPORTED, "cans_fftini: face-centred transforms are not implemented on the device");
+      return fail(CANS_EUNSUPPORTED, "cans_fftini: face-centred non-periodic transforms are not implemented on the device");
     }
     if (b0 == 'P' && b1 == 'P') p->kind[d] = K_PP;
     else if (b0 == 'N' && b1 == 'N') p->kind[d] = K_NN;

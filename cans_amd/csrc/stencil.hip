@@ -328,30 +328,54 @@ __global__ void k_fillps(Layout L, const double* u, const double* v, const doubl
 // the destination may alias the source (each element is read and written by one thread).
 // A thread owns one (i,j) column of the halo'd box and marches CORREC_KC planes up in
 // z, so p(k+1), loaded for the z-gradient, is reused as p(k) of the next plane; the
-// x and y neighbours of p come from adjacent lanes and rows of the same block.
+// x and y neighbours of p come from adjacent lanes and rows of the same block.  The
+// loads of plane k+1 are issued before the stores of plane k (two planes in flight).
+// Lane tx of x-block bx takes i = 1 + 64 bx + tx, so a wave's row segment starts at
+// interior element 1 (256-byte aligned in fields of cans_field_alloc); the one index
+// past the box, n1 + 2, wraps to the halo cell i = 0.
 constexpr int CORREC_TX = 64, CORREC_TY = 4, CORREC_KC = 32;
+
+template <typename S>
+struct CorrecIn {
+  double tu, tv, tw, px, py, pz, pu;
+};
 
 template <typename S>
 __global__ void __launch_bounds__(CORREC_TX* CORREC_TY)
     k_correc(Layout L, const S* p, const double* su, const double* sv, const double* sw, const S* eu, const S* ev,
              const S* ew, const double* f, double* du, double* dv, double* dw, const double* dzci, double dt,
              double factori, double factorj, S* pu) {
-  const int i = blockIdx.x * CORREC_TX + threadIdx.x;
+  int i = 1 + blockIdx.x * CORREC_TX + threadIdx.x;
+  if (i == L.n1 + 2) i = 0;
   const int j = blockIdx.y * CORREC_TY + threadIdx.y;
   if (i > L.n1 + 1 || j > L.n2 + 1) return;
   const int kb = blockIdx.z * CORREC_KC, ke = min(kb + CORREC_KC - 1, L.n3 + 1);
   const double fu = f ? f[0] : 0., fv = f ? f[1] : 0., fw = f ? f[2] : 0.;
   const bool in_ij = i >= 1 && i <= L.n1 && j >= 1 && j <= L.n2;
+  const bool xin = i <= L.n1, yin = j <= L.n2;
+  auto fetch = [&](long c, int k) {
+    CorrecIn<S> q;
+    q.tu = vel(su, eu, fu, c);
+    q.tv = vel(sv, ev, fv, c);
+    q.tw = vel(sw, ew, fw, c);
+    q.px = xin ? (double)p[c + 1] : 0.;
+    q.py = yin ? (double)p[c + L.sy] : 0.;
+    q.pz = (k <= L.n3) ? (double)p[c + L.sz] : 0.;
+    q.pu = (pu && in_ij && k >= 1 && k <= L.n3) ? (double)pu[c] : 0.;
+    return q;
+  };
   long c = L.at(i, j, kb);
   double pc = p[c];
+  CorrecIn<S> cur = fetch(c, kb);
   for (int k = kb; k <= ke; k++, c += L.sz) {
-    const double pk = (k <= L.n3) ? (double)p[c + L.sz] : 0.;
-    const double tu = vel(su, eu, fu, c), tv = vel(sv, ev, fv, c), tw = vel(sw, ew, fw, c);
-    du[c] = (i <= L.n1) ? tu - factori * ((double)p[c + 1] - pc) : tu;
-    dv[c] = (j <= L.n2) ? tv - factorj * ((double)p[c + L.sy] - pc) : tv;
-    dw[c] = (k <= L.n3) ? tw - dt * dzci[k] * (pk - pc) : tw;
-    if (pu && in_ij && k >= 1 && k <= L.n3) pu[c] = (S)((double)pu[c] + pc);
-    pc = pk;
+    CorrecIn<S> nxt = cur;
+    if (k < ke) nxt = fetch(c + L.sz, k + 1);
+    du[c] = xin ? cur.tu - factori * (cur.px - pc) : cur.tu;
+    dv[c] = yin ? cur.tv - factorj * (cur.py - pc) : cur.tv;
+    dw[c] = (k <= L.n3) ? cur.tw - dt * dzci[k] * (cur.pz - pc) : cur.tw;
+    if (pu && in_ij && k >= 1 && k <= L.n3) pu[c] = (S)(cur.pu + pc);
+    pc = cur.pz;
+    cur = nxt;
   }
 }
 

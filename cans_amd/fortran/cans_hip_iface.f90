@@ -120,6 +120,13 @@ module cans_hip
       character(kind=c_char), value :: c_or_f
       real(c_double), intent(out) :: lambda(*)
     end function
+    integer(c_int) function cans_bc_rhs(cbc, bc, dlc, dlf, c_or_f, rhs) bind(C, name='cans_bc_rhs')
+      import :: c_int, c_double, c_char
+      character(kind=c_char), intent(in) :: cbc(2)
+      real(c_double), intent(in) :: bc(2), dlc(2), dlf(2)
+      character(kind=c_char), value :: c_or_f
+      real(c_double), intent(out) :: rhs(2)
+    end function
     integer(c_int) function cans_tridmatrix(bc, n, dzci, dzfi, c_or_f, a, b, c) bind(C, name='cans_tridmatrix')
       import :: c_int, c_double, c_char
       character(kind=c_char), intent(in) :: bc(2)

@@ -84,6 +84,16 @@ def eigenvalues(n, bc, c_or_f):
     return lam
 
 
+def bc_rhs(cbc, bc, dlc, dlf, c_or_f):
+    """initsolver.f90:189-232: the two plane values of the boundary rhs."""
+    out = np.zeros(2)
+    LB.check(LB.lib().cans_bc_rhs(cbc.encode(), _dp(np.ascontiguousarray(bc, dtype=np.float64)),
+                                  _dp(np.ascontiguousarray(dlc, dtype=np.float64)),
+                                  _dp(np.ascontiguousarray(dlf, dtype=np.float64)), c_or_f.encode(), _dp(out)),
+             "cans_bc_rhs")
+    return out
+
+
 def tridmatrix(bc, n, dzci, dzfi, c_or_f):
     a, b, c = np.zeros(n), np.zeros(n), np.zeros(n)
     LB.check(LB.lib().cans_tridmatrix(bc.encode(), n, _dp(np.ascontiguousarray(dzci)),
@@ -100,6 +110,8 @@ class Context:
     double)."""
 
     def __init__(self, n, ld=None, precision=LB.CANS_FP64, stream=None, device="cuda"):
+        """stream: None (torch's current stream), or a torch.cuda.Stream the context owns
+        for its lifetime (required to capture its work in a HIP graph)."""
         torch = _torch()
         self.torch = torch
         self.device = torch.device(device)
@@ -114,8 +126,8 @@ class Context:
         self.p_es = 4 if mixed else 8
         if ld is None:  # interior rows start 256-byte aligned
             ld = ((self.n[0] + 2 + 31) // 32) * 32
-        if stream is None:
-            stream = torch.cuda.current_stream(self.device).cuda_stream
+        self.torch_stream = stream if stream is not None else torch.cuda.current_stream(self.device)
+        stream = self.torch_stream.cuda_stream
         self.stream = stream
         h = C.c_void_p()
         LB.check(LB.lib().cans_ctx_create(LB.ivec(self.n), int(ld), int(precision), C.c_void_p(stream),
@@ -185,6 +197,7 @@ class Plan:
     """Transforms + solver arrays of one variable (src/initsolver.f90:15-83)."""
 
     def __init__(self, ctx: Context, grid: Grid, cbc: str, bc, c_or_f="ccc"):
+        self.grid = grid
         self.ctx = ctx
         ng = grid.n
         lx = eigenvalues(ng[0], cbc[0:2], c_or_f[0]) * grid.dli[0] ** 2
@@ -203,11 +216,33 @@ class Plan:
         self.a = ctx.darray(self.a_h)
         self.b = ctx.darray(self.b_h)
         self.c = ctx.darray(self.c_h)
+        # boundary rhs planes (initsolver.f90:66-77): spacings re-derived from their inverses
+        bc = np.asarray(bc, dtype=np.float64)
+        n1, n2, n3 = ng
+        dl = [x ** (-1) for x in grid.dli]
+        dzc, dzf = grid.dzci ** (-1), grid.dzfi ** (-1)
+        fx = bc_rhs(cbc[0:2], bc[0:2], [dl[0], dl[0]], [dl[0], dl[0]], c_or_f[0])
+        fy = bc_rhs(cbc[2:4], bc[2:4], [dl[1], dl[1]], [dl[1], dl[1]], c_or_f[1])
+        if c_or_f[2] == "c":
+            fz = bc_rhs(cbc[4:6], bc[4:6], [dzc[0], dzc[n3]], [dzf[1], dzf[n3]], "c")
+        else:
+            fz = bc_rhs(cbc[4:6], bc[4:6], [dzc[1], dzc[n3 - 1]], [dzf[1], dzf[n3]], "f")
+        # rhsbx(n2,n3,0:1), rhsby(n1,n3,0:1), rhsbz(n1,n2,0:1): constant planes, Fortran order
+        self.rhsb = [ctx.darray(np.repeat(f, m)) for f, m in ((fx, n2 * n3), (fy, n1 * n3), (fz, n1 * n2))]
 
     def close(self):
         if self.h:
             LB.lib().cans_fftend(self.h)
             self.h = None
+
+    def solve_helmholtz(self, f: DeviceField, alpha):
+        """solve_helmholtz.f90:28-75 on f (the rhs in, the solution out)."""
+        n = LB.ivec(self.ctx.n)
+        LB.check(LB.lib().cans_solve_helmholtz(self.ctx.h, n, n, self.h, self.normfft, alpha,
+                                               *[C.c_void_p(t.data_ptr()) for t in (self.lam, self.a, self.b, self.c)],
+                                               *[C.c_void_p(t.data_ptr()) for t in self.rhsb],
+                                               LB.ivec(self.grid.is_bound), self.cbc.encode(), self.c_or_f.encode(),
+                                               C.c_void_p(f.ptr)), "cans_solve_helmholtz")
 
     def solve(self, f: DeviceField):
         n = LB.ivec(self.ctx.n)
@@ -220,10 +255,10 @@ class Plan:
 class Simulation:
     """One case on one GPU: set-up of src/main.f90:241-395 and its RK3 loop."""
 
-    def __init__(self, case: Case, ctx: Context = None, precision=LB.CANS_FP64):
+    def __init__(self, case: Case, ctx: Context = None, precision=LB.CANS_FP64, stream=None):
         self.case = case
         self.grid = Grid(case)
-        self.ctx = ctx or Context(case.n, precision=precision)
+        self.ctx = ctx or Context(case.n, precision=precision, stream=stream)
         ctx = self.ctx
         g = self.grid
         self.plan = Plan(ctx, g, case.cbcpre, case.bcpre, "ccc")
@@ -357,6 +392,24 @@ class Simulation:
     def rk3_step(self, dt):
         for irk in (1, 2, 3):
             self.rk3_substep(irk, dt)
+
+    def capture_step(self, dt, nsteps=1):
+        """Record nsteps fused RK3 steps of size dt into a HIP graph (torch.cuda.CUDAGraph)
+        on the context's own stream; replay() then repeats them without host launch cost.
+
+        The context must have been created with its own stream, and one eager step must
+        have run first (it allocates the scratch buffers and zeroes RHS_old; the fused
+        step itself allocates, copies and synchronises nothing).  dt is baked into the
+        graph; a new dt needs a new capture."""
+        torch = self.ctx.torch
+        if self.ctx.torch_stream == torch.cuda.default_stream(self.ctx.device):
+            raise ValueError("capture_step needs a context created with its own stream")
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=self.ctx.torch_stream):
+            for _ in range(nsteps):
+                self.rk3_step(dt)
+        return g
 
     def rk3_step_unfused(self, dt):
         """The same substeps through the per-routine entry points (main.f90:452-472)."""

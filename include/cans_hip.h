@@ -87,6 +87,10 @@ int cans_initgrid(int gtype, int n, double gr, double lz, int is_periodic,
 /* src/initsolver.f90:85-144 in the device ordering of this library (the
  * reference's GPU ordering, initsolver.f90:98-117, for periodic directions) */
 int cans_eigenvalues(int n, const char *bc, char c_or_f, double *lambda_h);
+/* src/initsolver.f90:189-232: the two plane values of rhs(:,:,0:1) (every entry of
+ * plane ib equals rhs_h[ib]); dlc, dlf: spacings at the two boundaries */
+int cans_bc_rhs(const char *cbc, const double *bc, const double *dlc, const double *dlf, char c_or_f,
+                double *rhs_h);
 /* src/initsolver.f90:146-187; a,b,c (1:n) */
 int cans_tridmatrix(const char *bc, int n, const double *dzci_h, const double *dzfi_h,
                     char c_or_f, double *a_h, double *b_h, double *c_h);

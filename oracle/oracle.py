@@ -222,8 +222,9 @@ class OracleCase:
         lam = np.asfortranarray(lx[:, None] + ly[None, :])
         a, b, c = tridmatrix(cbc[4:6], n3, self.dzci, self.dzfi, c_or_f[2])
         kinds, lens, normfft = fftini(self.ng, cbc[0:4], c_or_f[0:2])
-        dl = self.dl
-        dzc, dzf = self.dzc, self.dzf
+        # initsolver.f90:68-70: the spacings are re-derived from their inverses
+        dl = [x ** (-1) for x in self.dli]
+        dzc, dzf = self.dzci ** (-1), self.dzfi ** (-1)
 
         def bcrhs(cb, bv, dlc, dlf, cf):
             out = np.zeros(2)
@@ -287,6 +288,18 @@ class OracleCase:
                               chars(s["cbc"]), chars(s["c_or_f"]), P(pp))
         if rc:
             raise RuntimeError("orc_solver failed")
+
+    def solve_helmholtz(self, p, alpha, s):
+        """solve_helmholtz.f90:28-75 (3-D branch) with the arrays of initsolver(cbc, bc, c_or_f)."""
+        n, _k = self._n()
+        kinds, _a = IP(s["kinds"])
+        lens, _b = IP(s["lens"])
+        ib, _c = IP(self.is_bound)
+        rc = lib().orc_solve_helmholtz(n, kinds, lens, s["normfft"], alpha, P(s["lam"]), P(s["a"]), P(s["b"]),
+                                       P(s["c"]), VP(s["rhsbx"]), VP(s["rhsby"]), VP(s["rhsbz"]), ib,
+                                       chars(s["cbc"]), chars(s["c_or_f"]), P(p))
+        if rc:
+            raise RuntimeError("orc_solve_helmholtz failed")
 
     def correc(self, pp, u, v, w, dt):
         n, _k = self._n()

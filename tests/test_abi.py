@@ -64,3 +64,60 @@ def test_errors_are_reported():
     assert rc == LB.CANS_EINVAL
     assert b"sizes" in L.cans_last_error()
     assert L.cans_eigenvalues(4, b"PN", b"c", None) == LB.CANS_EINVAL
+
+
+def test_bc_rhs_matches_oracle_and_ghost_cells():
+    """cans_bc_rhs (initsolver.f90:189-232) against the oracle, and against the ghost-cell
+    form of the boundary conditions: with x_0 / x_{n+1} set from the boundary values, the
+    full 1-D second difference equals tridmatrix's operator applied to x minus the
+    boundary rhs added by updt_rhs_b (bound.f90:514-598)."""
+    import ctypes as C
+    from cans_amd import host as H
+    from oracle import oracle as O
+    n = 12
+    dzc, dzf, zc, zf = O.initgrid(1, n, 1.4, 1.0, False)
+    dzci, dzfi = 1 / dzc, 1 / dzf
+    rng = np.random.default_rng(3)
+    for cbc in ("DD", "NN", "DN", "ND", "PP"):
+        for cf in "cf":
+            bv = rng.uniform(-2, 2, 2)
+            if cf == "c":
+                dlc, dlf = [dzc[0], dzc[n]], [dzf[1], dzf[n]]
+            else:
+                dlc, dlf = [dzc[1], dzc[n - 1]], [dzf[1], dzf[n]]
+            got = H.bc_rhs(cbc, bv, dlc, dlf, cf)
+            ref = np.zeros(2)
+            O.lib().orc_bc_rhs(cbc.encode(), bv.ctypes.data_as(C.POINTER(C.c_double)),
+                               np.asarray(dlc).ctypes.data_as(C.POINTER(C.c_double)),
+                               np.asarray(dlf).ctypes.data_as(C.POINTER(C.c_double)), cf.encode(),
+                               ref.ctypes.data_as(C.POINTER(C.c_double)))
+            np.testing.assert_array_equal(got, ref)
+            if cbc == "PP":
+                assert not got.any()
+                continue
+            a, b, c = (np.array(t) for t in H.tridmatrix(cbc, n, dzci, dzfi, cf))
+            x = rng.uniform(-1, 1, n + 2)  # x[1..m] unknowns; x[0], x[m+1] ghosts or wall faces
+            if cf == "c":
+                m = n
+                # centred ghosts: D: x0 = 2v - x1; N: x0 = x1 - dzc0*v, x_{n+1} = x_n + dzc_n*v
+                x[0] = 2 * bv[0] - x[1] if cbc[0] == "D" else x[1] - dzc[0] * bv[0]
+                x[n + 1] = 2 * bv[1] - x[n] if cbc[1] == "D" else x[n] + dzc[n] * bv[1]
+
+                def full(k):
+                    return ((x[k + 1] - x[k]) * dzci[k] - (x[k] - x[k - 1]) * dzci[k - 1]) * dzfi[k]
+                ends = ((1, 0), (m, 1))
+            else:
+                # faces: a D wall face holds the boundary value (the last face is then not
+                # solved, solver.f90:77); face-centred N ends are covered by the oracle equality
+                m = n - 1 if cbc[1] == "D" else n
+                if cbc[0] == "D":
+                    x[0] = bv[0]
+                if cbc[1] == "D":
+                    x[n] = bv[1]
+
+                def full(k):
+                    return ((x[k + 1] - x[k]) * dzfi[k + 1] - (x[k] - x[k - 1]) * dzfi[k]) * dzci[k]
+                ends = tuple(e for e in ((1, 0), (m, 1)) if cbc[e[1]] == "D")
+            for k, ib in ends:
+                t = b[k - 1] * x[k] + (a[k - 1] * x[k - 1] if k > 1 else 0.) + (c[k - 1] * x[k + 1] if k < m else 0.)
+                assert abs(full(k) - (t - got[ib])) <= 1e-9 * max(1.0, abs(full(k))), (cbc, cf, k, full(k), t - got[ib])
