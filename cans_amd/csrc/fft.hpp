@@ -1,7 +1,9 @@
 // LDS-resident batched complex FFT building block (radix 8/4/2 Stockham).
 //
 // One "team" of T threads transforms one line of N = 2^m complex values that
-// sits in LDS, addressed through an element map.  T = max(1, N/8): every stage leaves each
+// sits in LDS, addressed through an element map.  The LDS element type L may be
+// float or double; the butterflies always compute in double (values are widened
+// on load and rounded to L on store).  T = max(1, N/8): every stage leaves each
 // thread exactly 8 (or N, when N < 8) values in registers.  All teams of a
 // work-group run the same N and call this together (it contains barriers).
 //
@@ -88,10 +90,24 @@ struct IxStride {
 };
 __host__ __device__ constexpr int padded_len(int N) { return N + (N >> 4) + 1; }
 
+template <typename L>
+__device__ __forceinline__ cplx<double> widen(cplx<L> x) {
+  return {(double)x.re, (double)x.im};
+}
+template <typename L>
+__device__ __forceinline__ cplx<L> narrow(cplx<double> x) {
+  return {(L)x.re, (L)x.im};
+}
+template <typename L>
+__device__ __forceinline__ cplx<L> narrow(double re, double im) {
+  return {(L)re, (L)im};
+}
+
 // one Stockham stage: R radix, NBF butterflies per thread
-template <typename T, int s, int R, int NBF, class IX>
-__device__ __forceinline__ void fft_stage(cplx<T>* buf, IX ix, int N, int lNs, int lt, int Tn,
-                                          const cplx<T>* __restrict__ tw) {
+template <typename L, int s, int R, int NBF, class IX>
+__device__ __forceinline__ void fft_stage(cplx<L>* buf, IX ix, int N, int lNs, int lt, int Tn,
+                                          const cplx<double>* __restrict__ tw) {
+  typedef double T;
   constexpr int LR = R == 2 ? 1 : R == 4 ? 2 : 3;
   const int NR = N >> LR;
   const int Ns = 1 << lNs;
@@ -100,7 +116,7 @@ __device__ __forceinline__ void fft_stage(cplx<T>* buf, IX ix, int N, int lNs, i
   for (int bb = 0; bb < NBF; bb++) {
     const int j = lt + bb * Tn;
 #pragma unroll
-    for (int r = 0; r < R; r++) v[bb][r] = buf[ix(j + r * NR)];
+    for (int r = 0; r < R; r++) v[bb][r] = widen(buf[ix(j + r * NR)]);
   }
   __syncthreads();
 #pragma unroll
@@ -118,32 +134,33 @@ __device__ __forceinline__ void fft_stage(cplx<T>* buf, IX ix, int N, int lNs, i
     bfly<T, s, R>(v[bb]);
     const int idxD = ((j >> lNs) << (lNs + LR)) + k;
 #pragma unroll
-    for (int r = 0; r < R; r++) buf[ix(idxD + r * Ns)] = v[bb][r];
+    for (int r = 0; r < R; r++) buf[ix(idxD + r * Ns)] = narrow<L>(v[bb][r]);
   }
   __syncthreads();
 }
 
 // full transform of one line (unnormalised), s = -1 forward, +1 inverse
-template <typename T, int s, class IX>
-__device__ void fft_line(cplx<T>* buf, IX ix, int N, int logN, int lt, int Tn, const cplx<T>* __restrict__ tw) {
+template <typename L, int s, class IX>
+__device__ void fft_line(cplx<L>* buf, IX ix, int N, int logN, int lt, int Tn,
+                         const cplx<double>* __restrict__ tw) {
   if (logN == 0) return;
   if (logN == 1) {
-    fft_stage<T, s, 2, 1>(buf, ix, N, 0, lt, Tn, tw);
+    fft_stage<L, s, 2, 1>(buf, ix, N, 0, lt, Tn, tw);
     return;
   }
   if (logN == 2) {
-    fft_stage<T, s, 4, 1>(buf, ix, N, 0, lt, Tn, tw);
+    fft_stage<L, s, 4, 1>(buf, ix, N, 0, lt, Tn, tw);
     return;
   }
   int lNs = 0;
   int rem = logN;
   while (rem >= 3) {
-    fft_stage<T, s, 8, 1>(buf, ix, N, lNs, lt, Tn, tw);
+    fft_stage<L, s, 8, 1>(buf, ix, N, lNs, lt, Tn, tw);
     lNs += 3;
     rem -= 3;
   }
-  if (rem == 2) fft_stage<T, s, 4, 2>(buf, ix, N, lNs, lt, Tn, tw);
-  else if (rem == 1) fft_stage<T, s, 2, 4>(buf, ix, N, lNs, lt, Tn, tw);
+  if (rem == 2) fft_stage<L, s, 4, 2>(buf, ix, N, lNs, lt, Tn, tw);
+  else if (rem == 1) fft_stage<L, s, 2, 4>(buf, ix, N, lNs, lt, Tn, tw);
 }
 
 }  // namespace cans

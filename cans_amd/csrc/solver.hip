@@ -171,7 +171,7 @@ __device__ __forceinline__ d2 div_pair(const XArgs<T>& a, long r, int e) {
 template <int DIR, typename T>  // +1 forward (fld -> W), -1 backward (W -> fld)
 __global__ void k_xpass(XArgs<T> a) {
   extern __shared__ unsigned char smem_raw[];
-  cplx<double>* smem = (cplx<double>*)smem_raw;
+  cplx<T>* smem = (cplx<T>*)smem_raw;  // LDS line in the storage type; arithmetic in double
   const int N = a.N, Tn = a.Tn, kind = a.kind;
   const int team = threadIdx.x / Tn, lt = threadIdx.x % Tn;
   const long nrows = (long)a.L.n2 * a.L.n3;
@@ -203,7 +203,7 @@ __global__ void k_xpass(XArgs<T> a) {
   if (DIR > 0) {
     if (va) {
       if (N == 1) {
-        if (lt == 0) smem[ix(0)] = {(double)xa[0], xb ? (double)xb[0] : 0.};
+        if (lt == 0) smem[ix(0)] = narrow<T>((double)xa[0], xb ? (double)xb[0] : 0.);
       } else {
         for (int e = 2 * lt; e < N; e += 2 * Tn) {
           d2 a2, b2;
@@ -219,33 +219,33 @@ __global__ void k_xpass(XArgs<T> a) {
             b2.y = -b2.y;
           }
           const int p0 = kind == K_PP ? e : mk_pos(e, N), p1 = kind == K_PP ? e + 1 : mk_pos(e + 1, N);
-          smem[ix(p0)] = {a2.x, b2.x};
-          smem[ix(p1)] = {a2.y, b2.y};
+          smem[ix(p0)] = narrow<T>(a2.x, b2.x);
+          smem[ix(p1)] = narrow<T>(a2.y, b2.y);
         }
       }
     }
     __syncthreads();
-    fft_line<double, -1>(smem, ix, N, a.logN, lt, Tn, a.tw);
+    fft_line<T, -1>(smem, ix, N, a.logN, lt, Tn, a.tw);
     if (!va) return;
     if (kind == K_PP) {
       const int H = N >> 1;
       if (N == 1) {
         if (lt == 0) {
-          ya[0] = (T)smem[ix(0)].re;
-          if (yb) yb[0] = (T)smem[ix(0)].im;
+          ya[0] = smem[ix(0)].re;
+          if (yb) yb[0] = smem[ix(0)].im;
         }
         return;
       }
       for (int k = lt; k < H; k += Tn) {
         double are, aim, bre, bim;
         if (k == 0) {
-          const cplx<double> z0 = smem[ix(0)], zh = smem[ix(H)];
+          const cplx<double> z0 = widen(smem[ix(0)]), zh = widen(smem[ix(H)]);
           are = z0.re;
           aim = zh.re;
           bre = z0.im;
           bim = zh.im;
         } else {
-          const cplx<double> zk = smem[ix(k)], zc = smem[ix(N - k)];
+          const cplx<double> zk = widen(smem[ix(k)]), zc = widen(smem[ix(N - k)]);
           split_pp(zk, zc, are, aim, bre, bim);
         }
         st2(ya + 2 * k, are, aim);
@@ -254,8 +254,8 @@ __global__ void k_xpass(XArgs<T> a) {
     } else {
       for (int k = 2 * lt; k < N; k += 2 * Tn) {
         double y0a, y0b, y1a, y1b;
-        split_dct(smem[ix(k)], smem[ix((N - k) & (N - 1))], a.mk[k], y0a, y0b);
-        split_dct(smem[ix(k + 1)], smem[ix(N - k - 1)], a.mk[k + 1], y1a, y1b);
+        split_dct(widen(smem[ix(k)]), widen(smem[ix((N - k) & (N - 1))]), a.mk[k], y0a, y0b);
+        split_dct(widen(smem[ix(k + 1)]), widen(smem[ix(N - k - 1)]), a.mk[k + 1], y1a, y1b);
         if (kind == K_DD) {  // output reversed: element N-1-k
           st2(ya + N - 2 - k, y1a, y0a);
           if (yb) st2(yb + N - 2 - k, y1b, y0b);
@@ -291,7 +291,7 @@ __global__ void k_xpass(XArgs<T> a) {
               bim = sg * b2.y;
             }
           }
-          smem[ix(k)] = merge_pp(are, aim, bre, bim);
+          smem[ix(k)] = narrow<T>(merge_pp(are, aim, bre, bim));
         }
       } else {
         for (int k = lt; k < N; k += Tn) {
@@ -299,23 +299,23 @@ __global__ void k_xpass(XArgs<T> a) {
           const int ik = dd ? N - 1 - k : k, ink = dd ? k - 1 : N - k;
           const double xka = (double)xa[ik], xnka = k ? (double)xa[ink] : 0.;
           const double xkb = xb ? (double)xb[ik] : 0., xnkb = (xb && k) ? (double)xb[ink] : 0.;
-          smem[ix(k)] = merge_dct(a.mk[k], xka, xnka, xkb, xnkb);
+          smem[ix(k)] = narrow<T>(merge_dct(a.mk[k], xka, xnka, xkb, xnkb));
         }
       }
     }
     __syncthreads();
-    fft_line<double, +1>(smem, ix, N, a.logN, lt, Tn, a.tw);
+    fft_line<T, +1>(smem, ix, N, a.logN, lt, Tn, a.tw);
     if (!va) return;
     if (N == 1) {
       if (lt == 0) {
-        ya[0] = (T)smem[ix(0)].re;
-        if (yb) yb[0] = (T)smem[ix(0)].im;
+        ya[0] = smem[ix(0)].re;
+        if (yb) yb[0] = smem[ix(0)].im;
       }
       return;
     }
     for (int e = 2 * lt; e < N; e += 2 * Tn) {
-      cplx<double> z0 = smem[ix(kind == K_PP ? e : mk_pos(e, N))];
-      cplx<double> z1 = smem[ix(kind == K_PP ? e + 1 : mk_pos(e + 1, N))];
+      cplx<double> z0 = widen(smem[ix(kind == K_PP ? e : mk_pos(e, N))]);
+      cplx<double> z1 = widen(smem[ix(kind == K_PP ? e + 1 : mk_pos(e + 1, N))]);
       if (kind == K_DD) {  // (-1)^k on the odd output
         z1.re = -z1.re;
         z1.im = -z1.im;
@@ -343,7 +343,7 @@ struct YArgs {
 template <int DIR, typename T>
 __global__ void k_ypass(YArgs<T> a) {
   extern __shared__ unsigned char smem_raw[];
-  cplx<double>* smem = (cplx<double>*)smem_raw;
+  cplx<T>* smem = (cplx<T>*)smem_raw;  // LDS line in the storage type; arithmetic in double
   const int LN = a.lines, N = a.N, Tn = a.Tn, kind = a.kind;
   const int c = threadIdx.x % LN, lt = threadIdx.x / LN;
   const int ncolblk = (a.n1 + 2 * LN - 1) / (2 * LN);
@@ -368,28 +368,28 @@ __global__ void k_ypass(YArgs<T> a) {
           v.x = -v.x;
           v.y = -v.y;
         }
-        smem[ix(kind == K_PP ? e : mk_pos(e, N))] = {v.x, v.y};
+        smem[ix(kind == K_PP ? e : mk_pos(e, N))] = narrow<T>(v.x, v.y);
       }
     }
     __syncthreads();
-    fft_line<double, -1>(smem, ix, N, a.logN, lt, Tn, a.tw);
+    fft_line<T, -1>(smem, ix, N, a.logN, lt, Tn, a.tw);
     if (!va) return;
     if (kind == K_PP) {
       const int H = N >> 1;
       if (N == 1) {
-        if (lt == 0) st(0, smem[ix(0)].re, smem[ix(0)].im);
+        if (lt == 0) st(0, (double)smem[ix(0)].re, (double)smem[ix(0)].im);
         return;
       }
       for (int kk = lt; kk < H; kk += Tn) {
         double are, aim, bre, bim;
         if (kk == 0) {
-          const cplx<double> z0 = smem[ix(0)], zh = smem[ix(H)];
+          const cplx<double> z0 = widen(smem[ix(0)]), zh = widen(smem[ix(H)]);
           are = z0.re;
           aim = zh.re;
           bre = z0.im;
           bim = zh.im;
         } else {
-          split_pp(smem[ix(kk)], smem[ix(N - kk)], are, aim, bre, bim);
+          split_pp(widen(smem[ix(kk)]), widen(smem[ix(N - kk)]), are, aim, bre, bim);
         }
         st(2 * kk, are, bre);
         st(2 * kk + 1, aim, bim);
@@ -397,7 +397,7 @@ __global__ void k_ypass(YArgs<T> a) {
     } else {
       for (int kk = lt; kk < N; kk += Tn) {
         double ya, yb;
-        split_dct(smem[ix(kk)], smem[ix((N - kk) & (N - 1))], a.mk[kk], ya, yb);
+        split_dct(widen(smem[ix(kk)]), widen(smem[ix((N - kk) & (N - 1))]), a.mk[kk], ya, yb);
         st(kind == K_DD ? N - 1 - kk : kk, ya, yb);
       }
     }
@@ -424,22 +424,22 @@ __global__ void k_ypass(YArgs<T> a) {
             aim = sg * im.x;
             bim = sg * im.y;
           }
-          smem[ix(kk)] = merge_pp(are, aim, bre, bim);
+          smem[ix(kk)] = narrow<T>(merge_pp(are, aim, bre, bim));
         }
       } else {
         for (int kk = lt; kk < N; kk += Tn) {
           const bool dd = kind == K_DD;
           const d2 xk = ld(dd ? N - 1 - kk : kk);
           const d2 xnk = kk ? ld(dd ? kk - 1 : N - kk) : d2(0., 0.);
-          smem[ix(kk)] = merge_dct(a.mk[kk], xk.x, xnk.x, xk.y, xnk.y);
+          smem[ix(kk)] = narrow<T>(merge_dct(a.mk[kk], xk.x, xnk.x, xk.y, xnk.y));
         }
       }
     }
     __syncthreads();
-    fft_line<double, +1>(smem, ix, N, a.logN, lt, Tn, a.tw);
+    fft_line<T, +1>(smem, ix, N, a.logN, lt, Tn, a.tw);
     if (!va) return;
     for (int e = lt; e < N; e += Tn) {
-      cplx<double> z = smem[ix(kind == K_PP ? e : mk_pos(e, N))];
+      cplx<double> z = widen(smem[ix(kind == K_PP ? e : mk_pos(e, N))]);
       if (kind == K_DD && (e & 1)) {
         z.re = -z.re;
         z.im = -z.im;
@@ -668,7 +668,7 @@ static int xpass(cans_ctx* c, const cans_plan* p, T* fld, T* W, int dir, const D
   const long nrows = (long)c->L.n2 * c->L.n3;
   const long npairs = (nrows + 1) / 2;
   dim3 grid(cdiv(npairs, a.pairs));
-  const size_t shm = (size_t)a.pairs * padded_len(a.N) * sizeof(cplx<double>);
+  const size_t shm = (size_t)a.pairs * padded_len(a.N) * sizeof(cplx<T>);
   if (dir > 0) k_xpass<1, T><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
   else k_xpass<-1, T><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
   CANS_LAUNCH_CHECK("k_xpass");
@@ -687,7 +687,8 @@ static int ypass(cans_ctx* c, const cans_plan* p, T* W, int dir) {
   a.Tn = a.N >= 8 ? a.N / 8 : 1;
   int lines = 256 / a.Tn;
   if (lines < 8) lines = 8;
-  while (lines > 1 && (size_t)lines * a.N * sizeof(cplx<double>) > 65536) lines /= 2;
+  if (sizeof(T) < sizeof(double) && lines < 16) lines = 16;  // 128-byte rows of FP32 columns
+  while (lines > 1 && (size_t)lines * a.N * sizeof(cplx<T>) > 65536) lines /= 2;
   while ((long)lines * a.Tn > 1024) lines /= 2;
   a.lines = lines;
   a.kind = p->kind[1];
@@ -695,7 +696,7 @@ static int ypass(cans_ctx* c, const cans_plan* p, T* W, int dir) {
   a.mk = p->mk[1];
   const int ncolblk = (a.n1 + 2 * lines - 1) / (2 * lines);
   dim3 grid((unsigned)ncolblk * a.n3);
-  const size_t shm = (size_t)lines * a.N * sizeof(cplx<double>);
+  const size_t shm = (size_t)lines * a.N * sizeof(cplx<T>);
   if (dir > 0) k_ypass<1, T><<<grid, lines * a.Tn, shm, c->stream>>>(a);
   else k_ypass<-1, T><<<grid, lines * a.Tn, shm, c->stream>>>(a);
   CANS_LAUNCH_CHECK("k_ypass");
