@@ -94,24 +94,27 @@ __device__ __forceinline__ cplx<double> merge_pp(double are, double aim, double 
 }
 
 // ---------------------------------------------------------------------------
-// Work buffer layout (z-tiled).  Spectral line l = i + n1*j (0-based) belongs to tile
-// l / ZT; a tile stores its ZT lines for all n3 rows contiguously, row k at offset
-// k*ZT.  The z pass then streams each wave's tile as one contiguous block, while the
-// x and y passes still move contiguous chunks of ZT (x) or 2*LINES (y) elements.
+// Work buffer layout (z-tiled).  Spectral line (i,j) (0-based) belongs to tile
+// j + n2*(i/ZT), slot i%ZT; a tile stores its ZT lines for all n3 rows contiguously,
+// row k at offset k*ZT.  The z pass then streams each wave's tile as one contiguous
+// block; the x and y passes move contiguous chunks of ZT (x) or 2*LINES (y) elements.
+// Tiles of consecutive j are adjacent, so one y-pass work-group (a column block, all
+// j) spans n2 tiles of one i-chunk, not n2 rows of tiles across the whole buffer.
 // ---------------------------------------------------------------------------
 constexpr int ZT = 64;
-__host__ __device__ __forceinline__ long woff(long l, int k, int n3) {
-  return (l / ZT) * ((long)ZT * n3) + (long)k * ZT + (l % ZT);
+__host__ __device__ __forceinline__ long woff(int i, int j, int k, int n2, int n3) {
+  return ((long)j + (long)n2 * (i / ZT)) * ((long)ZT * n3) + (long)k * ZT + (i % ZT);
 }
-__host__ __device__ inline long wbuf_elems(long nl, int n3) { return (nl + ZT - 1) / ZT * ZT * (long)n3; }
+__host__ __device__ inline long wbuf_elems(int n1, int n2, int n3) {
+  return (long)n2 * ((n1 + ZT - 1) / ZT) * ZT * n3;
+}
 
 // element e of one row: a contiguous field row, or row (j,k) of the tiled work buffer
 template <typename T>
 struct RowPtr {
   T* p = nullptr;
-  long l0 = 0;  // first spectral line of the row (tiled rows)
-  int k = 0, n3 = 0, tiled = 0;
-  __device__ __forceinline__ T* operator+(long e) const { return tiled ? p + woff(l0 + e, k, n3) : p + e; }
+  int j = 0, k = 0, n2 = 0, n3 = 0, tiled = 0;  // (j,k) of a tiled row
+  __device__ __forceinline__ T* operator+(long e) const { return tiled ? p + woff((int)e, j, k, n2, n3) : p + e; }
   __device__ __forceinline__ T& operator[](long e) const { return *(*this + e); }
   __device__ __forceinline__ explicit operator bool() const { return p != nullptr; }
 };
@@ -189,8 +192,9 @@ __global__ void k_xpass(XArgs<T> a) {
     RowPtr<T> q;
     if (v) {
       q.p = a.W;
-      q.l0 = (long)L.n1 * (r % L.n2);
+      q.j = (int)(r % L.n2);
       q.k = (int)(r / L.n2);
+      q.n2 = L.n2;
       q.n3 = L.n3;
       q.tiled = 1;
     }
@@ -353,7 +357,7 @@ __global__ void k_ypass(YArgs<T> a) {
   const bool va = ia < a.n1, vb = ia + 1 < a.n1;
   const IxStride ix{c, LN};
   // element e (along y) of column ia in the tiled work buffer; columns ia, ia+1 are adjacent
-  auto col = [&](int e) { return a.W + woff(ia + (long)a.n1 * e, k, a.n3); };
+  auto col = [&](int e) { return a.W + woff(ia, e, k, a.n2, a.n3); };
   // (A, B) pair of element e of this line
   auto ld = [&](int e) -> d2 { return vb ? ld2(col(e)) : d2((double)*col(e), 0.); };
   auto st = [&](int e, double x, double y) {
@@ -464,7 +468,7 @@ struct ZArgs {
   T* W;            // z-tiled work buffer (woff)
   long nl;         // number of lines (n1*n2)
   int n;           // rows of the system (n3 - q)
-  int n3;          // rows of a work-buffer tile
+  int n1, n2, n3;  // grid (tile geometry of the work buffer)
   int periodic;
   double norm;
   double pin_eps;  // relative pivot tolerance of the zero-mode pin
@@ -519,7 +523,7 @@ __global__ void __launch_bounds__(256) k_gaussel(ZArgs<T> z) {
   if (line >= z.nl) return;
   const double l = z.lam[line];
   const double eps = z.pin_eps;
-  T* P = z.W + woff(line, 0, z.n3);  // this line's row 0; rows follow at stride ZT
+  T* P = z.W + woff((int)(line % z.n1), (int)(line / z.n1), 0, z.n2, z.n3);  // row 0; rows at stride ZT
   constexpr long ps = ZT;
   constexpr int periodic = PER ? 1 : 0;
   const int nn = periodic ? z.n - 1 : z.n;
@@ -711,7 +715,7 @@ int run_solver(cans_ctx* c, const cans_plan* p, double normfft, const double* la
   const long N = L.cells();
   const long nl = (long)L.n1 * L.n2;
   void* W = nullptr;
-  int rc = ensure_work(c, 1, (size_t)wbuf_elems(nl, L.n3) * sizeof(T), &W);
+  int rc = ensure_work(c, 1, (size_t)wbuf_elems(L.n1, L.n2, L.n3) * sizeof(T), &W);
   if (rc) return rc;
   const int q = (c_or_f[2] == 'f' && bc[5] == 'D') ? 1 : 0;
   const int periodic = bc[4] == 'P' && bc[5] == 'P';
@@ -735,6 +739,8 @@ int run_solver(cans_ctx* c, const cans_plan* p, double normfft, const double* la
   z.W = (T*)W;
   z.nl = nl;
   z.n = n;
+  z.n1 = L.n1;
+  z.n2 = L.n2;
   z.n3 = L.n3;
   z.periodic = periodic;
   z.norm = normfft;
