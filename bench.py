@@ -30,6 +30,11 @@ sys.path.insert(0, ROOT)
 METRIC = "ns/cell per RK3 step + Poisson GB/s (% HBM peak), mixed vs FP64, 1 GPU"
 HBM_PEAK_GBS = 8000.0
 DT = 2e-4
+# HBM bytes moved by one standalone Poisson solve, measured with rocprofv3 --pmc
+# (FETCH_SIZE x 2 + WRITE_SIZE x 1, the factors a 1 GiB device copy calibrates);
+# the per-kernel table is profiles/channel_1024x512x384_hbm_traffic.md
+SOLVE_TRAFFIC_BYTES = {((1024, 512, 384), "mixed"): 9.530e9, ((1024, 512, 384), "fp64"): 19.593e9}
+TRAFFIC_SOURCE = "profiles/channel_1024x512x384_hbm_traffic.md"
 
 
 def cpu_baseline(p, init, budget_s=20.0):
@@ -185,7 +190,10 @@ def main():
                                if head_mode == "mixed" else "f64")},
         "poisson": pois,
         "roofline": {"bound": "hbm", "achieved": pois["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": pois["GBps"] / HBM_PEAK_GBS, "traffic": None,
+                     "frac": pois["GBps"] / HBM_PEAK_GBS,
+                     "traffic": SOLVE_TRAFFIC_BYTES.get((tuple(p["ng"]), head_mode)),
+                     "traffic_source": TRAFFIC_SOURCE if (tuple(p["ng"]), head_mode) in SOLVE_TRAFFIC_BYTES else None,
+                     "algorithmic_bytes": pois["algorithmic_bytes"],
                      "kernel": "Poisson solve (x-fwd, y-fwd, z-Thomas, y-bwd, x-bwd passes), %s storage"
                                % ("f32" if head_mode == "mixed" else "f64")},
     }

@@ -270,40 +270,36 @@ __global__ void k_xpass(XArgs<T> a) {
       }
     }
   } else {
+    // spectral indices k and N-k read the same elements: one thread takes both
     if (va) {
-      if (kind == K_PP) {
-        const int H = N >> 1;
-        for (int k = lt; k < N; k += Tn) {
-          double are, aim, bre = 0., bim = 0.;
-          if (N == 1 || k == 0) {
-            are = (double)xa[0];
-            aim = 0.;
-            if (xb) bre = (double)xb[0];
-          } else if (k == H) {
-            are = (double)xa[1];
-            aim = 0.;
-            if (xb) bre = (double)xb[1];
+      const int H = N >> 1;
+      if (N == 1) {
+        if (lt == 0) smem[ix(0)] = narrow<T>(merge_pp((double)xa[0], 0., xb ? (double)xb[0] : 0., 0.));
+      } else if (kind == K_PP) {
+        for (int m = lt; m < H; m += Tn) {
+          const d2 a2 = ld2(xa + 2 * m);  // m = 0: (r0, r_{N/2})
+          const d2 b2 = xb ? ld2(xb + 2 * m) : d2(0., 0.);
+          if (m == 0) {
+            smem[ix(0)] = narrow<T>(merge_pp(a2.x, 0., b2.x, 0.));
+            smem[ix(H)] = narrow<T>(merge_pp(a2.y, 0., b2.y, 0.));
           } else {
-            const int kk = k < H ? k : N - k;
-            const double sg = k < H ? 1. : -1.;
-            const d2 a2 = ld2(xa + 2 * kk);
-            are = a2.x;
-            aim = sg * a2.y;
-            if (xb) {
-              const d2 b2 = ld2(xb + 2 * kk);
-              bre = b2.x;
-              bim = sg * b2.y;
-            }
+            smem[ix(m)] = narrow<T>(merge_pp(a2.x, a2.y, b2.x, b2.y));
+            smem[ix(N - m)] = narrow<T>(merge_pp(a2.x, -a2.y, b2.x, -b2.y));
           }
-          smem[ix(k)] = narrow<T>(merge_pp(are, aim, bre, bim));
         }
       } else {
-        for (int k = lt; k < N; k += Tn) {
-          const bool dd = kind == K_DD;  // RODFT01 reads the reversed input
-          const int ik = dd ? N - 1 - k : k, ink = dd ? k - 1 : N - k;
-          const double xka = (double)xa[ik], xnka = k ? (double)xa[ink] : 0.;
-          const double xkb = xb ? (double)xb[ik] : 0., xnkb = (xb && k) ? (double)xb[ink] : 0.;
-          smem[ix(k)] = narrow<T>(merge_dct(a.mk[k], xka, xnka, xkb, xnkb));
+        const bool dd = kind == K_DD;  // RODFT01 reads the reversed input
+        for (int m = lt; m <= H; m += Tn) {
+          if (m == 0) {
+            const int i0 = dd ? N - 1 : 0;
+            smem[ix(0)] = narrow<T>(merge_dct(a.mk[0], (double)xa[i0], 0., xb ? (double)xb[i0] : 0., 0.));
+            continue;
+          }
+          const int ia = dd ? N - 1 - m : m, ib = dd ? m - 1 : N - m;
+          const double xaa = (double)xa[ia], xab = (double)xa[ib];
+          const double xba = xb ? (double)xb[ia] : 0., xbb = xb ? (double)xb[ib] : 0.;
+          smem[ix(m)] = narrow<T>(merge_dct(a.mk[m], xaa, xab, xba, xbb));
+          if (m != H) smem[ix(N - m)] = narrow<T>(merge_dct(a.mk[N - m], xab, xaa, xbb, xba));
         }
       }
     }
@@ -406,36 +402,36 @@ __global__ void k_ypass(YArgs<T> a) {
       }
     }
   } else {
+    // spectral indices m and N-m read the same two rows: one thread takes both
     if (va) {
-      if (kind == K_PP) {
-        const int H = N >> 1;
-        for (int kk = lt; kk < N; kk += Tn) {
-          double are, aim = 0., bre, bim = 0.;
-          if (N == 1 || kk == 0) {
-            const d2 v = ld(0);
-            are = v.x;
-            bre = v.y;
-          } else if (kk == H) {
-            const d2 v = ld(1);
-            are = v.x;
-            bre = v.y;
+      const int H = N >> 1;
+      if (N == 1) {
+        if (lt == 0) {
+          const d2 v = ld(0);
+          smem[ix(0)] = narrow<T>(merge_pp(v.x, 0., v.y, 0.));
+        }
+      } else if (kind == K_PP) {
+        for (int m = lt; m < H; m += Tn) {
+          const d2 re = ld(2 * m), im = ld(2 * m + 1);  // m = 0: r0 and r_{N/2}
+          if (m == 0) {
+            smem[ix(0)] = narrow<T>(merge_pp(re.x, 0., re.y, 0.));
+            smem[ix(H)] = narrow<T>(merge_pp(im.x, 0., im.y, 0.));
           } else {
-            const int m = kk < H ? kk : N - kk;
-            const double sg = kk < H ? 1. : -1.;
-            const d2 re = ld(2 * m), im = ld(2 * m + 1);
-            are = re.x;
-            bre = re.y;
-            aim = sg * im.x;
-            bim = sg * im.y;
+            smem[ix(m)] = narrow<T>(merge_pp(re.x, im.x, re.y, im.y));
+            smem[ix(N - m)] = narrow<T>(merge_pp(re.x, -im.x, re.y, -im.y));
           }
-          smem[ix(kk)] = narrow<T>(merge_pp(are, aim, bre, bim));
         }
       } else {
-        for (int kk = lt; kk < N; kk += Tn) {
-          const bool dd = kind == K_DD;
-          const d2 xk = ld(dd ? N - 1 - kk : kk);
-          const d2 xnk = kk ? ld(dd ? kk - 1 : N - kk) : d2(0., 0.);
-          smem[ix(kk)] = narrow<T>(merge_dct(a.mk[kk], xk.x, xnk.x, xk.y, xnk.y));
+        const bool dd = kind == K_DD;  // RODFT01 reads the reversed input
+        for (int m = lt; m <= H; m += Tn) {
+          if (m == 0) {
+            const d2 x0 = ld(dd ? N - 1 : 0);
+            smem[ix(0)] = narrow<T>(merge_dct(a.mk[0], x0.x, 0., x0.y, 0.));
+            continue;
+          }
+          const d2 xa = ld(dd ? N - 1 - m : m), xb = ld(dd ? m - 1 : N - m);
+          smem[ix(m)] = narrow<T>(merge_dct(a.mk[m], xa.x, xb.x, xa.y, xb.y));
+          if (m != H) smem[ix(N - m)] = narrow<T>(merge_dct(a.mk[N - m], xb.x, xa.x, xb.y, xa.y));
         }
       }
     }

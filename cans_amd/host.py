@@ -186,8 +186,12 @@ class DeviceField:
         a = np.asarray(a, dtype=np.float64)
         n1, n2, n3 = self.ctx.n
         assert a.shape == (n1 + 2, n2 + 2, n3 + 2)
-        t = self.ctx.torch.from_numpy(np.ascontiguousarray(a.transpose(2, 1, 0)))
-        self.view.copy_(t.to(self.dtype))
+        # pad rows to the leading dimension on the host, then one contiguous transfer
+        sy = self.ctx.sy
+        host = np.zeros((n3 + 2, n2 + 2, sy), dtype=np.float32 if self.dtype != self.ctx.torch.float64 else np.float64)
+        host[:, :, :n1 + 2] = a.transpose(2, 1, 0)
+        t = self.ctx.torch.from_numpy(host.reshape(-1))
+        self.buf[self.off:self.off + host.size].copy_(t)
 
     def interior(self):
         return self.numpy()[1:-1, 1:-1, 1:-1]
