@@ -125,10 +125,25 @@ __device__ __forceinline__ void fft_stage(cplx<L>* buf, IX ix, int N, int lNs, i
     const int k = j & (Ns - 1);
     if (k) {
       const int step = k * (N >> (lNs + LR));  // k * N/(Ns*R)
-#pragma unroll
-      for (int r = 1; r < R; r++) {
-        cplx<T> w = tw[r * step];
-        v[bb][r] = (s < 0) ? cmul(v[bb][r], w) : cmulc(v[bb][r], w);
+      // W^r for r = 1..R-1 from the table entries W^1, W^2, W^4 (at most one product
+      // deep), so only a few twiddles are live at a time
+      const cplx<T> w1 = tw[step];
+      if constexpr (R == 2) {
+        v[bb][1] = (s < 0) ? cmul(v[bb][1], w1) : cmulc(v[bb][1], w1);
+      } else {
+        const cplx<T> w2 = tw[2 * step];
+        const cplx<T> w3 = cmul(w1, w2);
+        v[bb][1] = (s < 0) ? cmul(v[bb][1], w1) : cmulc(v[bb][1], w1);
+        v[bb][2] = (s < 0) ? cmul(v[bb][2], w2) : cmulc(v[bb][2], w2);
+        v[bb][3] = (s < 0) ? cmul(v[bb][3], w3) : cmulc(v[bb][3], w3);
+        if constexpr (R == 8) {
+          const cplx<T> w4 = tw[4 * step];
+          const cplx<T> w5 = cmul(w4, w1), w6 = cmul(w4, w2), w7 = cmul(w4, w3);
+          v[bb][4] = (s < 0) ? cmul(v[bb][4], w4) : cmulc(v[bb][4], w4);
+          v[bb][5] = (s < 0) ? cmul(v[bb][5], w5) : cmulc(v[bb][5], w5);
+          v[bb][6] = (s < 0) ? cmul(v[bb][6], w6) : cmulc(v[bb][6], w6);
+          v[bb][7] = (s < 0) ? cmul(v[bb][7], w7) : cmulc(v[bb][7], w7);
+        }
       }
     }
     bfly<T, s, R>(v[bb]);

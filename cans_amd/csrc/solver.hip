@@ -172,7 +172,7 @@ __device__ __forceinline__ d2 div_pair(const XArgs<T>& a, long r, int e) {
 }
 
 template <int DIR, typename T>  // +1 forward (fld -> W), -1 backward (W -> fld)
-__global__ void k_xpass(XArgs<T> a) {
+__device__ __forceinline__ void xpass_body(const XArgs<T>& a) {
   extern __shared__ unsigned char smem_raw[];
   cplx<T>* smem = (cplx<T>*)smem_raw;  // LDS line in the storage type; arithmetic in double
   const int N = a.N, Tn = a.Tn, kind = a.kind;
@@ -326,6 +326,16 @@ __global__ void k_xpass(XArgs<T> a) {
   }
 }
 
+// one kernel per storage type (the FP32 and FP64 forms differ in LDS and register use)
+template <int DIR>
+__global__ void __launch_bounds__(512) k_xpass_f(XArgs<float> a) {
+  xpass_body<DIR, float>(a);
+}
+template <int DIR>
+__global__ void __launch_bounds__(512) k_xpass_d(XArgs<double> a) {
+  xpass_body<DIR, double>(a);
+}
+
 // ---------------------------------------------------------------------------
 // pass Y: in place on W; a work-group owns 2*LINES adjacent x-columns of one z-plane.
 // thread t -> line c = t % LINES, team thread lt = t / LINES; LDS element (e, c) at [e*LINES + c].
@@ -341,7 +351,7 @@ struct YArgs {
 };
 
 template <int DIR, typename T>
-__global__ void k_ypass(YArgs<T> a) {
+__device__ __forceinline__ void ypass_body(const YArgs<T>& a) {
   extern __shared__ unsigned char smem_raw[];
   cplx<T>* smem = (cplx<T>*)smem_raw;  // LDS line in the storage type; arithmetic in double
   const int LN = a.lines, N = a.N, Tn = a.Tn, kind = a.kind;
@@ -447,6 +457,36 @@ __global__ void k_ypass(YArgs<T> a) {
       st(e, z.re, z.im);
     }
   }
+}
+
+// FP32 lines: 16 lines x N/8 threads = 1024 threads in 64 KB of LDS; at most 64 VGPRs
+// lets two such work-groups share a CU.  FP64 lines: LDS already limits a CU to two
+// 512-thread work-groups, which needs no register cap.
+template <int DIR>
+__global__ void __launch_bounds__(1024, 8) k_ypass_f(YArgs<float> a) {
+  ypass_body<DIR, float>(a);
+}
+template <int DIR>
+__global__ void __launch_bounds__(1024) k_ypass_d(YArgs<double> a) {
+  ypass_body<DIR, double>(a);
+}
+template <int DIR, typename T>
+void launch_ypass(dim3 grid, int threads, size_t shm, hipStream_t st, const YArgs<T>& a);
+template <>
+void launch_ypass<1, float>(dim3 g, int t, size_t shm, hipStream_t st, const YArgs<float>& a) {
+  k_ypass_f<1><<<g, t, shm, st>>>(a);
+}
+template <>
+void launch_ypass<-1, float>(dim3 g, int t, size_t shm, hipStream_t st, const YArgs<float>& a) {
+  k_ypass_f<-1><<<g, t, shm, st>>>(a);
+}
+template <>
+void launch_ypass<1, double>(dim3 g, int t, size_t shm, hipStream_t st, const YArgs<double>& a) {
+  k_ypass_d<1><<<g, t, shm, st>>>(a);
+}
+template <>
+void launch_ypass<-1, double>(dim3 g, int t, size_t shm, hipStream_t st, const YArgs<double>& a) {
+  k_ypass_d<-1><<<g, t, shm, st>>>(a);
 }
 
 // ---------------------------------------------------------------------------
@@ -669,8 +709,13 @@ static int xpass(cans_ctx* c, const cans_plan* p, T* fld, T* W, int dir, const D
   const long npairs = (nrows + 1) / 2;
   dim3 grid(cdiv(npairs, a.pairs));
   const size_t shm = (size_t)a.pairs * padded_len(a.N) * sizeof(cplx<T>);
-  if (dir > 0) k_xpass<1, T><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
-  else k_xpass<-1, T><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
+  if constexpr (sizeof(T) == sizeof(float)) {
+    if (dir > 0) k_xpass_f<1><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
+    else k_xpass_f<-1><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
+  } else {
+    if (dir > 0) k_xpass_d<1><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
+    else k_xpass_d<-1><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
+  }
   CANS_LAUNCH_CHECK("k_xpass");
   return CANS_OK;
 }
@@ -697,8 +742,8 @@ static int ypass(cans_ctx* c, const cans_plan* p, T* W, int dir) {
   const int ncolblk = (a.n1 + 2 * lines - 1) / (2 * lines);
   dim3 grid((unsigned)ncolblk * a.n3);
   const size_t shm = (size_t)lines * a.N * sizeof(cplx<T>);
-  if (dir > 0) k_ypass<1, T><<<grid, lines * a.Tn, shm, c->stream>>>(a);
-  else k_ypass<-1, T><<<grid, lines * a.Tn, shm, c->stream>>>(a);
+  if (dir > 0) launch_ypass<1, T>(grid, lines * a.Tn, shm, c->stream, a);
+  else launch_ypass<-1, T>(grid, lines * a.Tn, shm, c->stream, a);
   CANS_LAUNCH_CHECK("k_ypass");
   return CANS_OK;
 }
