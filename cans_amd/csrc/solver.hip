@@ -523,10 +523,39 @@ struct ZRow {
 // space lets the compiler use scalar loads instead of one vector load per row.
 typedef const __attribute__((address_space(4))) double* kcoef;
 
+// 1/x.  FAST (used when the solve's storage is FP32): the hardware reciprocal
+// refined by two Newton-Raphson steps, r <- r + r(1 - x r), below 1 ulp of the
+// correctly rounded quotient; otherwise the IEEE division, as the reference.
+template <bool FAST>
+__device__ __forceinline__ double rcp64(double x) {
+  if constexpr (FAST) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.);
+    return fma(r, e, r);
+  } else {
+    return 1. / x;
+  }
+}
+
+// interior forward row 1 < k < nn of a non-periodic system: no pin test, same
+// operations and order as z_row
+template <bool FAST>
+__device__ __forceinline__ void z_row_mid(double l, double r, double norm, double ak, double bk, double ck, double& d,
+                                          double& pp) {
+  const double bl = bk + l;
+  const double ad = ak * d;
+  const double zz = rcp64<FAST>(bl - ad);
+  d = ck * zz;
+  pp = (r * norm - ak * pp) * zz;
+}
+
+template <bool FAST = false>
 __device__ __forceinline__ void z_row(int k, int nn, int periodic, double l, double r, double norm, kcoef a, kcoef b,
                                       kcoef c, double eps, ZRow& s) {
   if (k == 1) {
-    const double zz = 1. / (b[0] + l);
+    const double zz = rcp64<FAST>(b[0] + l);
     s.d = c[0] * zz;
     s.pp = r * norm * zz;
     if (periodic) s.q2 = ((nn == 1) ? (-a[0] - c[nn - 1]) : -a[0]) * zz;
@@ -545,7 +574,7 @@ __device__ __forceinline__ void z_row(int k, int nn, int periodic, double l, dou
     s.d = 0.;
     s.pp = 0.;
   } else {
-    const double zz = 1. / den;
+    const double zz = rcp64<FAST>(den);
     s.d = c[k - 1] * zz;
     s.pp = (r * norm - a[k - 1] * s.pp) * zz;
   }
@@ -664,6 +693,132 @@ __global__ void __launch_bounds__(256) k_gaussel(ZArgs<T> z) {
   }
 }
 
+// Non-periodic z (solver.f90:147-200): the same elimination and checkpoints as
+// k_gaussel<T, false>, arranged so that every block but the one holding the last
+// row runs without branches (no row-1, range or pin tests), which lets the
+// compiler batch the coefficient loads and interleave independent work.  The
+// right-hand-side prefetch registers hold the storage type.  Blocks are read
+// whole: rows past the last one (up to CK - 2 of them) fall in the next tile or in
+// the work buffer's tail pad, and their values are never used.  FP64 storage gives
+// the same operations as k_gaussel; FAST replaces the divisions by rcp64<true>.
+template <typename T, bool FAST>
+__global__ void __launch_bounds__(256) k_thomas(ZArgs<T> z) {
+  const long line = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (line >= z.nl) return;
+  const double l = z.lam[line];
+  const double eps = z.pin_eps, norm = z.norm;
+  T* P = z.W + woff((int)(line % z.n1), (int)(line / z.n1), 0, z.n2, z.n3);  // row 0; rows at stride ZT
+  constexpr long ps = ZT;
+  const int nn = z.n;
+  const int nblk = (nn + CK - 1) / CK;
+  const kcoef a = (kcoef)z.a;
+  const kcoef b = (kcoef)z.b;
+  const kcoef c = (kcoef)z.c;
+  // ---- forward sweep: checkpoints (d, p') at the end of every block but the last ----
+  double d = 0., pp = 0.;
+  T rr[CK], rn[CK];
+#pragma unroll
+  for (int t = 0; t < CK; t++) rr[t] = P[(long)t * ps];
+  for (int bb = 0; bb < nblk; bb++) {
+    const int k0 = bb * CK + 1;
+    if (bb + 1 < nblk) {
+#pragma unroll
+      for (int t = 0; t < CK; t++) rn[t] = P[(long)(k0 + CK + t - 1) * ps];
+    }
+    if (bb == nblk - 1) {
+      ZRow s = {d, pp, 0.};
+#pragma unroll
+      for (int t = 0; t < CK; t++)
+        if (k0 + t <= nn) z_row<FAST>(k0 + t, nn, 0, l, (double)rr[t], norm, a, b, c, eps, s);
+      break;
+    }
+    if (bb == 0) {
+      const double zz = rcp64<FAST>(b[0] + l);
+      d = c[0] * zz;
+      pp = (double)rr[0] * norm * zz;
+#pragma unroll
+      for (int t = 1; t < CK; t++) z_row_mid<FAST>(l, (double)rr[t], norm, a[t], b[t], c[t], d, pp);
+    } else {
+#pragma unroll
+      for (int t = 0; t < CK; t++)
+        z_row_mid<FAST>(l, (double)rr[t], norm, a[k0 + t - 1], b[k0 + t - 1], c[k0 + t - 1], d, pp);
+    }
+    const long o = (long)bb * z.nl + line;
+    z.ckd[o] = d;
+    z.ckp[o] = pp;
+#pragma unroll
+    for (int t = 0; t < CK; t++) rr[t] = rn[t];
+  }
+  // ---- backward sweep: recompute each block from its checkpoint, then substitute ----
+  double xn = 0.;
+  {
+    const int k0 = (nblk - 1) * CK + 1;
+#pragma unroll
+    for (int t = 0; t < CK; t++) rr[t] = P[(long)(k0 + t - 1) * ps];
+  }
+  for (int bb = nblk - 1; bb >= 0; bb--) {
+    const int k0 = bb * CK + 1;
+    double d0 = 0., p0 = 0.;
+    if (bb > 0) {
+#pragma unroll
+      for (int t = 0; t < CK; t++) rn[t] = P[(long)(k0 - CK + t - 1) * ps];
+      const long o = (long)(bb - 1) * z.nl + line;
+      d0 = z.ckd[o];
+      p0 = z.ckp[o];
+    }
+    double dd[CK], pv[CK];
+    if (bb == nblk - 1) {
+      ZRow s = {d0, p0, 0.};
+#pragma unroll
+      for (int t = 0; t < CK; t++) {
+        if (k0 + t <= nn) {
+          z_row<FAST>(k0 + t, nn, 0, l, (double)rr[t], norm, a, b, c, eps, s);
+          dd[t] = s.d;
+          pv[t] = s.pp;
+        }
+      }
+#pragma unroll
+      for (int t = CK - 1; t >= 0; t--) {
+        const int k = k0 + t;
+        if (k <= nn) {
+          const double x = (k == nn) ? pv[t] : pv[t] - dd[t] * xn;
+          P[(long)(k - 1) * ps] = (T)x;
+          xn = x;
+        }
+      }
+    } else {
+      if (bb == 0) {
+        const double zz = rcp64<FAST>(b[0] + l);
+        d0 = c[0] * zz;
+        p0 = (double)rr[0] * norm * zz;
+        dd[0] = d0;
+        pv[0] = p0;
+#pragma unroll
+        for (int t = 1; t < CK; t++) {
+          z_row_mid<FAST>(l, (double)rr[t], norm, a[t], b[t], c[t], d0, p0);
+          dd[t] = d0;
+          pv[t] = p0;
+        }
+      } else {
+#pragma unroll
+        for (int t = 0; t < CK; t++) {
+          z_row_mid<FAST>(l, (double)rr[t], norm, a[k0 + t - 1], b[k0 + t - 1], c[k0 + t - 1], d0, p0);
+          dd[t] = d0;
+          pv[t] = p0;
+        }
+      }
+#pragma unroll
+      for (int t = CK - 1; t >= 0; t--) {
+        const double x = pv[t] - dd[t] * xn;
+        P[(long)(k0 + t - 1) * ps] = (T)x;
+        xn = x;
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < CK; t++) rr[t] = rn[t];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -756,7 +911,8 @@ int run_solver(cans_ctx* c, const cans_plan* p, double normfft, const double* la
   const long N = L.cells();
   const long nl = (long)L.n1 * L.n2;
   void* W = nullptr;
-  int rc = ensure_work(c, 1, (size_t)wbuf_elems(L.n1, L.n2, L.n3) * sizeof(T), &W);
+  // tail pad: the z pass reads whole CK-row blocks of the last tile
+  int rc = ensure_work(c, 1, (size_t)(wbuf_elems(L.n1, L.n2, L.n3) + (long)CK * ZT) * sizeof(T), &W);
   if (rc) return rc;
   const int q = (c_or_f[2] == 'f' && bc[5] == 'D') ? 1 : 0;
   const int periodic = bc[4] == 'P' && bc[5] == 'P';
@@ -801,7 +957,7 @@ int run_solver(cans_ctx* c, const cans_plan* p, double normfft, const double* la
   z.ckq = z.ckp + nblk * nl;
   z.X2 = periodic ? z.ckq + nblk * nl : nullptr;
   if (periodic) k_gaussel<T, true><<<cdiv(nl, 256), 256, 0, c->stream>>>(z);
-  else k_gaussel<T, false><<<cdiv(nl, 256), 256, 0, c->stream>>>(z);
+  else k_thomas<T, sizeof(T) == sizeof(float)><<<cdiv(nl, 256), 256, 0, c->stream>>>(z);
   CANS_LAUNCH_CHECK("k_gaussel");
   if ((rc = ypass<T>(c, p, (T*)W, -1))) return rc;
   if ((rc = xpass<T>(c, p, fld, (T*)W, -1))) return rc;
