@@ -2,9 +2,10 @@
 //
 // One "team" of T threads transforms one line of N = 2^m complex values that
 // sits in LDS, addressed through an element map.  The LDS element type L may be
-// float or double; the butterflies always compute in double (values are widened
-// on load and rounded to L on store).  T = max(1, N/8): every stage leaves each
-// thread exactly 8 (or N, when N < 8) values in registers.  All teams of a
+// float or double, and the butterflies compute in L: FP32 lines give FP32
+// transforms (twiddles come from a double table, rounded once), FP64 lines FP64
+// ones.  Twiddle products use fused multiply-adds.  T = max(1, N/8): every stage
+// leaves each thread exactly 8 (or N, when N < 8) values in registers.  All teams of a
 // work-group run the same N and call this together (it contains barriers).
 //
 // Formulation: the Stockham autosort iteration of Govindaraju et al., "High
@@ -103,11 +104,23 @@ __device__ __forceinline__ cplx<L> narrow(double re, double im) {
   return {(L)re, (L)im};
 }
 
-// one Stockham stage: R radix, NBF butterflies per thread
+// twiddle products with fused multiply-adds
+template <typename T>
+__device__ __forceinline__ cplx<T> tmul(cplx<T> a, cplx<T> b) {
+#pragma clang fp contract(fast)
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> tmulc(cplx<T> a, cplx<T> b) {
+#pragma clang fp contract(fast)
+  return {a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im};
+}
+
+// one Stockham stage: R radix, NBF butterflies per thread, arithmetic in L
 template <typename L, int s, int R, int NBF, class IX>
 __device__ __forceinline__ void fft_stage(cplx<L>* buf, IX ix, int N, int lNs, int lt, int Tn,
                                           const cplx<double>* __restrict__ tw) {
-  typedef double T;
+  typedef L T;
   constexpr int LR = R == 2 ? 1 : R == 4 ? 2 : 3;
   const int NR = N >> LR;
   const int Ns = 1 << lNs;
@@ -116,7 +129,7 @@ __device__ __forceinline__ void fft_stage(cplx<L>* buf, IX ix, int N, int lNs, i
   for (int bb = 0; bb < NBF; bb++) {
     const int j = lt + bb * Tn;
 #pragma unroll
-    for (int r = 0; r < R; r++) v[bb][r] = widen(buf[ix(j + r * NR)]);
+    for (int r = 0; r < R; r++) v[bb][r] = buf[ix(j + r * NR)];
   }
   __syncthreads();
 #pragma unroll
@@ -127,36 +140,36 @@ __device__ __forceinline__ void fft_stage(cplx<L>* buf, IX ix, int N, int lNs, i
       const int step = k * (N >> (lNs + LR));  // k * N/(Ns*R)
       // W^r for r = 1..R-1 from the table entries W^1, W^2, W^4 (at most one product
       // deep), so only a few twiddles are live at a time
-      const cplx<T> w1 = tw[step];
+      const cplx<T> w1 = {(T)tw[step].re, (T)tw[step].im};
       if constexpr (R == 2) {
-        v[bb][1] = (s < 0) ? cmul(v[bb][1], w1) : cmulc(v[bb][1], w1);
+        v[bb][1] = (s < 0) ? tmul(v[bb][1], w1) : tmulc(v[bb][1], w1);
       } else {
-        const cplx<T> w2 = tw[2 * step];
-        const cplx<T> w3 = cmul(w1, w2);
-        v[bb][1] = (s < 0) ? cmul(v[bb][1], w1) : cmulc(v[bb][1], w1);
-        v[bb][2] = (s < 0) ? cmul(v[bb][2], w2) : cmulc(v[bb][2], w2);
-        v[bb][3] = (s < 0) ? cmul(v[bb][3], w3) : cmulc(v[bb][3], w3);
+        const cplx<T> w2 = {(T)tw[2 * step].re, (T)tw[2 * step].im};
+        const cplx<T> w3 = tmul(w1, w2);
+        v[bb][1] = (s < 0) ? tmul(v[bb][1], w1) : tmulc(v[bb][1], w1);
+        v[bb][2] = (s < 0) ? tmul(v[bb][2], w2) : tmulc(v[bb][2], w2);
+        v[bb][3] = (s < 0) ? tmul(v[bb][3], w3) : tmulc(v[bb][3], w3);
         if constexpr (R == 8) {
-          const cplx<T> w4 = tw[4 * step];
-          const cplx<T> w5 = cmul(w4, w1), w6 = cmul(w4, w2), w7 = cmul(w4, w3);
-          v[bb][4] = (s < 0) ? cmul(v[bb][4], w4) : cmulc(v[bb][4], w4);
-          v[bb][5] = (s < 0) ? cmul(v[bb][5], w5) : cmulc(v[bb][5], w5);
-          v[bb][6] = (s < 0) ? cmul(v[bb][6], w6) : cmulc(v[bb][6], w6);
-          v[bb][7] = (s < 0) ? cmul(v[bb][7], w7) : cmulc(v[bb][7], w7);
+          const cplx<T> w4 = {(T)tw[4 * step].re, (T)tw[4 * step].im};
+          const cplx<T> w5 = tmul(w4, w1), w6 = tmul(w4, w2), w7 = tmul(w4, w3);
+          v[bb][4] = (s < 0) ? tmul(v[bb][4], w4) : tmulc(v[bb][4], w4);
+          v[bb][5] = (s < 0) ? tmul(v[bb][5], w5) : tmulc(v[bb][5], w5);
+          v[bb][6] = (s < 0) ? tmul(v[bb][6], w6) : tmulc(v[bb][6], w6);
+          v[bb][7] = (s < 0) ? tmul(v[bb][7], w7) : tmulc(v[bb][7], w7);
         }
       }
     }
     bfly<T, s, R>(v[bb]);
     const int idxD = ((j >> lNs) << (lNs + LR)) + k;
 #pragma unroll
-    for (int r = 0; r < R; r++) buf[ix(idxD + r * Ns)] = narrow<L>(v[bb][r]);
+    for (int r = 0; r < R; r++) buf[ix(idxD + r * Ns)] = v[bb][r];
   }
   __syncthreads();
 }
 
 // full transform of one line (unnormalised), s = -1 forward, +1 inverse
 template <typename L, int s, class IX>
-__device__ void fft_line(cplx<L>* buf, IX ix, int N, int logN, int lt, int Tn,
+__device__ __forceinline__ void fft_line(cplx<L>* buf, IX ix, int N, int logN, int lt, int Tn,
                          const cplx<double>* __restrict__ tw) {
   if (logN == 0) return;
   if (logN == 1) {

@@ -52,9 +52,9 @@ __device__ __forceinline__ int mk_pos(int e, int N) { return (e & 1) ? N - 1 - (
 //            Y_k = Re(M_k (Z_k + conj Z_{N-k})) with M_k = exp(-i pi k/(2N));
 //   RODFT10: REDFT10 of (-1)^j x_j, output reversed;
 //   HC2R, REDFT01, RODFT01: the inverses (unnormalised, as FFTW).
-// Global accesses move two elements per lane (16 bytes FP64, 8 bytes FP32 storage);
-// every value is widened to double on load, the transforms run in double and
-// only the store rounds to the storage type T.
+// Global accesses move two elements per lane (16 bytes FP64, 8 bytes FP32 storage).
+// The pre/post-processing widens to double and rounds once on the store to T; the
+// FFT stages in between compute in the storage type (fft.hpp).
 // ---------------------------------------------------------------------------
 typedef HIP_vector_type<double, 2> d2;
 
@@ -95,26 +95,36 @@ __device__ __forceinline__ cplx<double> merge_pp(double are, double aim, double 
 
 // ---------------------------------------------------------------------------
 // Work buffer layout (z-tiled).  Spectral line (i,j) (0-based) belongs to tile
-// j + n2*(i/ZT), slot i%ZT; a tile stores its ZT lines for all n3 rows contiguously,
-// row k at offset k*ZT.  The z pass then streams each wave's tile as one contiguous
-// block; the x and y passes move contiguous chunks of ZT (x) or 2*LINES (y) elements.
-// Tiles of consecutive j are adjacent, so one y-pass work-group (a column block, all
-// j) spans n2 tiles of one i-chunk, not n2 rows of tiles across the whole buffer.
+// column (i/ZT, j), slot i%ZT.  Rows are grouped in blocks of ZK: tile block
+// (kb, i/ZT, j) stores its ZT lines for rows kb*ZK .. kb*ZK+ZK-1 contiguously, row k
+// at offset (k%ZK)*ZT, and the blocks of one row block kb are ordered j fastest.
+// The z pass streams each wave's lines as contiguous ZK-row blocks; the x and y
+// passes move contiguous chunks of ZT (x) or 2*LINES (y) elements, and a y-pass
+// work-group (one plane, one column chunk, all j) stays inside n2 adjacent blocks.
+// nic = number of ZT-line chunks along x; PS = elements per row block (all tiles).
 // ---------------------------------------------------------------------------
 constexpr int ZT = 64;
-__host__ __device__ __forceinline__ long woff(int i, int j, int k, int n2, int n3) {
-  return ((long)j + (long)n2 * (i / ZT)) * ((long)ZT * n3) + (long)k * ZT + (i % ZT);
+constexpr int ZK = 16;
+constexpr int CK = 16;  // z pass: rows per checkpoint block (ZK is a multiple)
+static_assert(ZK % CK == 0, "tile row blocks hold whole checkpoint blocks");
+__host__ __device__ __forceinline__ int wnic(int n1) { return (n1 + ZT - 1) / ZT; }
+__host__ __device__ __forceinline__ long wplane(int n1, int n2) { return (long)wnic(n1) * n2 * ZT * ZK; }
+__host__ __device__ __forceinline__ long woff(int i, int j, int k, int n2, int nic) {
+  return (((long)(k / ZK) * nic + i / ZT) * n2 + j) * (ZT * ZK) + (k % ZK) * ZT + (i % ZT);
 }
+// offset of row r of a line from that line's row 0
+__host__ __device__ __forceinline__ long zrow(int r, long PS) { return (long)(r / ZK) * PS + (r % ZK) * ZT; }
+// elements of the buffer: the z pass reads whole CK-row blocks, up to CK - 2 rows past n3
 __host__ __device__ inline long wbuf_elems(int n1, int n2, int n3) {
-  return (long)n2 * ((n1 + ZT - 1) / ZT) * ZT * n3;
+  return (long)((n3 + CK + ZK - 1) / ZK) * wplane(n1, n2);
 }
 
 // element e of one row: a contiguous field row, or row (j,k) of the tiled work buffer
 template <typename T>
 struct RowPtr {
   T* p = nullptr;
-  int j = 0, k = 0, n2 = 0, n3 = 0, tiled = 0;  // (j,k) of a tiled row
-  __device__ __forceinline__ T* operator+(long e) const { return tiled ? p + woff((int)e, j, k, n2, n3) : p + e; }
+  int j = 0, k = 0, n2 = 0, nic = 0, tiled = 0;  // (j,k) of a tiled row
+  __device__ __forceinline__ T* operator+(long e) const { return tiled ? p + woff((int)e, j, k, n2, nic) : p + e; }
   __device__ __forceinline__ T& operator[](long e) const { return *(*this + e); }
   __device__ __forceinline__ explicit operator bool() const { return p != nullptr; }
 };
@@ -171,10 +181,13 @@ __device__ __forceinline__ d2 div_pair(const XArgs<T>& a, long r, int e) {
   return d2(p0, p1);
 }
 
-template <int DIR, typename T>  // +1 forward (fld -> W), -1 backward (W -> fld)
+// DIR +1 forward (fld -> W), -1 backward (W -> fld); DV (forward only): the rhs is
+// formed from the velocity (a.ds) instead of read from fld.  DV is a template flag
+// so that the plain forward pass carries none of the divergence code's registers.
+template <int DIR, typename T, int DV = 0>
 __device__ __forceinline__ void xpass_body(const XArgs<T>& a) {
   extern __shared__ unsigned char smem_raw[];
-  cplx<T>* smem = (cplx<T>*)smem_raw;  // LDS line in the storage type; arithmetic in double
+  cplx<T>* smem = (cplx<T>*)smem_raw;  // LDS line in the storage type
   const int N = a.N, Tn = a.Tn, kind = a.kind;
   const int team = threadIdx.x / Tn, lt = threadIdx.x % Tn;
   const long nrows = (long)a.L.n2 * a.L.n3;
@@ -195,7 +208,7 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& a) {
       q.j = (int)(r % L.n2);
       q.k = (int)(r / L.n2);
       q.n2 = L.n2;
-      q.n3 = L.n3;
+      q.nic = wnic(L.n1);
       q.tiled = 1;
     }
     return q;
@@ -211,7 +224,7 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& a) {
       } else {
         for (int e = 2 * lt; e < N; e += 2 * Tn) {
           d2 a2, b2;
-          if (a.div) {
+          if constexpr (DV != 0) {
             a2 = div_pair(a, ra, e);
             b2 = xb ? div_pair(a, rb, e) : d2(0., 0.);
           } else {
@@ -327,13 +340,13 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& a) {
 }
 
 // one kernel per storage type (the FP32 and FP64 forms differ in LDS and register use)
-template <int DIR>
+template <int DIR, int DV>
 __global__ void __launch_bounds__(512) k_xpass_f(XArgs<float> a) {
-  xpass_body<DIR, float>(a);
+  xpass_body<DIR, float, DV>(a);
 }
-template <int DIR>
+template <int DIR, int DV>
 __global__ void __launch_bounds__(512) k_xpass_d(XArgs<double> a) {
-  xpass_body<DIR, double>(a);
+  xpass_body<DIR, double, DV>(a);
 }
 
 // ---------------------------------------------------------------------------
@@ -353,7 +366,7 @@ struct YArgs {
 template <int DIR, typename T>
 __device__ __forceinline__ void ypass_body(const YArgs<T>& a) {
   extern __shared__ unsigned char smem_raw[];
-  cplx<T>* smem = (cplx<T>*)smem_raw;  // LDS line in the storage type; arithmetic in double
+  cplx<T>* smem = (cplx<T>*)smem_raw;  // LDS line in the storage type
   const int LN = a.lines, N = a.N, Tn = a.Tn, kind = a.kind;
   const int c = threadIdx.x % LN, lt = threadIdx.x / LN;
   const int ncolblk = (a.n1 + 2 * LN - 1) / (2 * LN);
@@ -363,7 +376,8 @@ __device__ __forceinline__ void ypass_body(const YArgs<T>& a) {
   const bool va = ia < a.n1, vb = ia + 1 < a.n1;
   const IxStride ix{c, LN};
   // element e (along y) of column ia in the tiled work buffer; columns ia, ia+1 are adjacent
-  auto col = [&](int e) { return a.W + woff(ia, e, k, a.n2, a.n3); };
+  const int nic = wnic(a.n1);
+  auto col = [&](int e) { return a.W + woff(ia, e, k, a.n2, nic); };
   // (A, B) pair of element e of this line
   auto ld = [&](int e) -> d2 { return vb ? ld2(col(e)) : d2((double)*col(e), 0.); };
   auto st = [&](int e, double x, double y) {
@@ -497,14 +511,13 @@ void launch_ypass<-1, double>(dim3 g, int t, size_t shm, hipStream_t st, const Y
 // the backward sweep re-reads the rhs of one block, recomputes the block's
 // d/rhs from its checkpoint (bit-identical: same operations) and substitutes.
 // ---------------------------------------------------------------------------
-constexpr int CK = 16;
-
 template <typename T>
 struct ZArgs {
   T* W;            // z-tiled work buffer (woff)
   long nl;         // number of lines (n1*n2)
   int n;           // rows of the system (n3 - q)
-  int n1, n2, n3;  // grid (tile geometry of the work buffer)
+  int n1, n2, nic;  // tile geometry of the work buffer
+  long PS;          // elements per row block
   int periodic;
   double norm;
   double pin_eps;  // relative pivot tolerance of the zero-mode pin
@@ -588,8 +601,8 @@ __global__ void __launch_bounds__(256) k_gaussel(ZArgs<T> z) {
   if (line >= z.nl) return;
   const double l = z.lam[line];
   const double eps = z.pin_eps;
-  T* P = z.W + woff((int)(line % z.n1), (int)(line / z.n1), 0, z.n2, z.n3);  // row 0; rows at stride ZT
-  constexpr long ps = ZT;
+  T* P = z.W + woff((int)(line % z.n1), (int)(line / z.n1), 0, z.n2, z.nic);  // row 0
+  const long PS = z.PS;
   constexpr int periodic = PER ? 1 : 0;
   const int nn = periodic ? z.n - 1 : z.n;
   const int nblk = (nn + CK - 1) / CK;
@@ -601,13 +614,13 @@ __global__ void __launch_bounds__(256) k_gaussel(ZArgs<T> z) {
   ZRow s = {0., 0., 0.};
   double rr[CK], rn[CK];
 #pragma unroll
-  for (int t = 0; t < CK; t++) rr[t] = (t + 1 <= nn) ? (double)P[(long)t * ps] : 0.;
+  for (int t = 0; t < CK; t++) rr[t] = (t + 1 <= nn) ? (double)P[zrow(t, PS)] : 0.;
   for (int bb = 0; bb < nblk; bb++) {
     const int k0 = bb * CK + 1;
 #pragma unroll
     for (int t = 0; t < CK; t++) {
       const int k = k0 + CK + t;
-      rn[t] = (k <= nn) ? (double)P[(long)(k - 1) * ps] : 0.;
+      rn[t] = (k <= nn) ? (double)P[zrow(k - 1, PS)] : 0.;
     }
 #pragma unroll
     for (int t = 0; t < CK; t++)
@@ -625,14 +638,14 @@ __global__ void __launch_bounds__(256) k_gaussel(ZArgs<T> z) {
   {
     const int k0 = (nblk - 1) * CK + 1;
 #pragma unroll
-    for (int t = 0; t < CK; t++) rr[t] = (k0 + t <= nn) ? (double)P[(long)(k0 + t - 1) * ps] : 0.;
+    for (int t = 0; t < CK; t++) rr[t] = (k0 + t <= nn) ? (double)P[zrow(k0 + t - 1, PS)] : 0.;
   }
   for (int bb = nblk - 1; bb >= 0; bb--) {
     const int k0 = bb * CK + 1;
     const int k1 = min(k0 + CK - 1, nn);
     if (bb > 0) {
 #pragma unroll
-      for (int t = 0; t < CK; t++) rn[t] = (double)P[(long)(k0 - CK + t - 1) * ps];
+      for (int t = 0; t < CK; t++) rn[t] = (double)P[zrow(k0 - CK + t - 1, PS)];
     }
     ZRow r0 = {0., 0., 0.};
     if (bb > 0) {
@@ -665,7 +678,7 @@ __global__ void __launch_bounds__(256) k_gaussel(ZArgs<T> z) {
           x = pv[t] - dd[t] * xn;
           if (periodic) x2 = qv[t] - dd[t] * xq;
         }
-        P[(long)(k - 1) * ps] = (T)x;
+        P[zrow(k - 1, PS)] = (T)x;
         if (periodic) z.X2[(long)(k - 1) * z.nl + line] = x2;
         xn = x;
         xq = x2;
@@ -685,10 +698,10 @@ __global__ void __launch_bounds__(256) k_gaussel(ZArgs<T> z) {
   const double tol = eps * fmax(fabs(b[m - 1] + l), fabs(c[m - 1] * xq1 + a[m - 1] * xqnn));
   double xm;
   if (fabs(den) <= tol) xm = 0.;
-  else xm = ((double)P[(long)(m - 1) * ps] * norm - c[m - 1] * x1 - a[m - 1] * xnn) / den;
-  P[(long)(m - 1) * ps] = (T)xm;
+  else xm = ((double)P[zrow(m - 1, PS)] * norm - c[m - 1] * x1 - a[m - 1] * xnn) / den;
+  P[zrow(m - 1, PS)] = (T)xm;
   for (int k = 1; k <= nn; k++) {
-    const long o = (long)(k - 1) * ps;
+    const long o = zrow(k - 1, PS);
     P[o] = (T)((double)P[o] + z.X2[(long)(k - 1) * z.nl + line] * xm);
   }
 }
@@ -698,8 +711,8 @@ __global__ void __launch_bounds__(256) k_gaussel(ZArgs<T> z) {
 // row runs without branches (no row-1, range or pin tests), which lets the
 // compiler batch the coefficient loads and interleave independent work.  The
 // right-hand-side prefetch registers hold the storage type.  Blocks are read
-// whole: rows past the last one (up to CK - 2 of them) fall in the next tile or in
-// the work buffer's tail pad, and their values are never used.  FP64 storage gives
+// whole: rows past the last one (up to CK - 2 of them) lie inside the work buffer
+// (wbuf_elems) and their values are never used.  FP64 storage gives
 // the same operations as k_gaussel; FAST replaces the divisions by rcp64<true>.
 template <typename T, bool FAST>
 __global__ void __launch_bounds__(256) k_thomas(ZArgs<T> z) {
@@ -707,8 +720,10 @@ __global__ void __launch_bounds__(256) k_thomas(ZArgs<T> z) {
   if (line >= z.nl) return;
   const double l = z.lam[line];
   const double eps = z.pin_eps, norm = z.norm;
-  T* P = z.W + woff((int)(line % z.n1), (int)(line / z.n1), 0, z.n2, z.n3);  // row 0; rows at stride ZT
-  constexpr long ps = ZT;
+  T* P = z.W + woff((int)(line % z.n1), (int)(line / z.n1), 0, z.n2, z.nic);  // row 0
+  const long PS = z.PS;
+  // first row of checkpoint block bb; its rows follow at stride ZT
+  auto blk = [&](int bb) { return P + zrow(bb * CK, PS); };
   const int nn = z.n;
   const int nblk = (nn + CK - 1) / CK;
   const kcoef a = (kcoef)z.a;
@@ -718,12 +733,13 @@ __global__ void __launch_bounds__(256) k_thomas(ZArgs<T> z) {
   double d = 0., pp = 0.;
   T rr[CK], rn[CK];
 #pragma unroll
-  for (int t = 0; t < CK; t++) rr[t] = P[(long)t * ps];
+  for (int t = 0; t < CK; t++) rr[t] = P[t * ZT];
   for (int bb = 0; bb < nblk; bb++) {
     const int k0 = bb * CK + 1;
     if (bb + 1 < nblk) {
+      const T* B = blk(bb + 1);
 #pragma unroll
-      for (int t = 0; t < CK; t++) rn[t] = P[(long)(k0 + CK + t - 1) * ps];
+      for (int t = 0; t < CK; t++) rn[t] = B[t * ZT];
     }
     if (bb == nblk - 1) {
       ZRow s = {d, pp, 0.};
@@ -752,16 +768,18 @@ __global__ void __launch_bounds__(256) k_thomas(ZArgs<T> z) {
   // ---- backward sweep: recompute each block from its checkpoint, then substitute ----
   double xn = 0.;
   {
-    const int k0 = (nblk - 1) * CK + 1;
+    const T* B = blk(nblk - 1);
 #pragma unroll
-    for (int t = 0; t < CK; t++) rr[t] = P[(long)(k0 + t - 1) * ps];
+    for (int t = 0; t < CK; t++) rr[t] = B[t * ZT];
   }
   for (int bb = nblk - 1; bb >= 0; bb--) {
     const int k0 = bb * CK + 1;
+    T* const B = blk(bb);
     double d0 = 0., p0 = 0.;
     if (bb > 0) {
+      const T* Bp = blk(bb - 1);
 #pragma unroll
-      for (int t = 0; t < CK; t++) rn[t] = P[(long)(k0 - CK + t - 1) * ps];
+      for (int t = 0; t < CK; t++) rn[t] = Bp[t * ZT];
       const long o = (long)(bb - 1) * z.nl + line;
       d0 = z.ckd[o];
       p0 = z.ckp[o];
@@ -782,7 +800,7 @@ __global__ void __launch_bounds__(256) k_thomas(ZArgs<T> z) {
         const int k = k0 + t;
         if (k <= nn) {
           const double x = (k == nn) ? pv[t] : pv[t] - dd[t] * xn;
-          P[(long)(k - 1) * ps] = (T)x;
+          B[t * ZT] = (T)x;
           xn = x;
         }
       }
@@ -810,7 +828,7 @@ __global__ void __launch_bounds__(256) k_thomas(ZArgs<T> z) {
 #pragma unroll
       for (int t = CK - 1; t >= 0; t--) {
         const double x = pv[t] - dd[t] * xn;
-        P[(long)(k0 + t - 1) * ps] = (T)x;
+        B[t * ZT] = (T)x;
         xn = x;
       }
     }
@@ -865,11 +883,13 @@ static int xpass(cans_ctx* c, const cans_plan* p, T* fld, T* W, int dir, const D
   dim3 grid(cdiv(npairs, a.pairs));
   const size_t shm = (size_t)a.pairs * padded_len(a.N) * sizeof(cplx<T>);
   if constexpr (sizeof(T) == sizeof(float)) {
-    if (dir > 0) k_xpass_f<1><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
-    else k_xpass_f<-1><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
+    if (dir < 0) k_xpass_f<-1, 0><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
+    else if (a.div) k_xpass_f<1, 1><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
+    else k_xpass_f<1, 0><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
   } else {
-    if (dir > 0) k_xpass_d<1><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
-    else k_xpass_d<-1><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
+    if (dir < 0) k_xpass_d<-1, 0><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
+    else if (a.div) k_xpass_d<1, 1><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
+    else k_xpass_d<1, 0><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
   }
   CANS_LAUNCH_CHECK("k_xpass");
   return CANS_OK;
@@ -911,8 +931,7 @@ int run_solver(cans_ctx* c, const cans_plan* p, double normfft, const double* la
   const long N = L.cells();
   const long nl = (long)L.n1 * L.n2;
   void* W = nullptr;
-  // tail pad: the z pass reads whole CK-row blocks of the last tile
-  int rc = ensure_work(c, 1, (size_t)(wbuf_elems(L.n1, L.n2, L.n3) + (long)CK * ZT) * sizeof(T), &W);
+  int rc = ensure_work(c, 1, (size_t)wbuf_elems(L.n1, L.n2, L.n3) * sizeof(T), &W);
   if (rc) return rc;
   const int q = (c_or_f[2] == 'f' && bc[5] == 'D') ? 1 : 0;
   const int periodic = bc[4] == 'P' && bc[5] == 'P';
@@ -938,7 +957,8 @@ int run_solver(cans_ctx* c, const cans_plan* p, double normfft, const double* la
   z.n = n;
   z.n1 = L.n1;
   z.n2 = L.n2;
-  z.n3 = L.n3;
+  z.nic = wnic(L.n1);
+  z.PS = wplane(L.n1, L.n2);
   z.periodic = periodic;
   z.norm = normfft;
   // The reference pins the last unknown of the zero-wavenumber line when its final

@@ -23,9 +23,11 @@
  *   CANS_MIXED - velocity fields (u, v, w) stay double; pressure-like arrays
  *                (p, pp, the field of cans_solver / cans_solve_helmholtz /
  *                cans_updt_rhs_b / cans_fillps's output, dudtrko, dvdtrko,
- *                dwdtrko) are float, as are the transform intermediates.  Every
- *                kernel computes in double; only stores to float arrays round.
- *                Small arrays (grid, eigenvalues, a/b/c, rhsb, bc values) stay
+ *                dwdtrko) are float, as are the transform intermediates.  The
+ *                FFT butterflies of the transforms compute in float; every
+ *                other kernel (stencils, RK update, projection, the DCT/DST
+ *                pre/post-processing and the z elimination) computes in
+ *                double, and only stores to float arrays round.  Small arrays (grid, eigenvalues, a/b/c, rhsb, bc values) stay
  *                double.  cans_ctx_elem_size reports the size per kind.
  */
 #ifndef CANS_HIP_H
