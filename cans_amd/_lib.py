@@ -31,7 +31,15 @@ class StepParams(C.Structure):
                 ("dzc_h", _vp), ("dzf_h", _vp), ("dzci_d", _vp), ("dzfi_d", _vp),
                 ("grid_vol_ratio_c_d", _vp), ("grid_vol_ratio_f_d", _vp), ("plan", _vp),
                 ("normfft", C.c_double), ("lambdaxy_d", _vp), ("a_d", _vp), ("b_d", _vp), ("c_d", _vp),
-                ("rhsbx_d", _vp), ("rhsby_d", _vp), ("rhsbz_d", _vp)]
+                ("rhsbx_d", _vp), ("rhsby_d", _vp), ("rhsbz_d", _vp),
+                ("nscal", C.c_int), ("scal", _vp), ("is_buoyancy", C.c_int), ("gacc", C.c_double * 3),
+                ("beta", C.c_double)]
+
+
+class Scalar(C.Structure):
+    """cans_scalar (include/cans_hip.h)."""
+    _fields_ = [("val", _vp), ("dsdtrko", _vp), ("alpha", C.c_double), ("cbc", C.c_char * 6),
+                ("bc", C.c_double * 6), ("source", C.c_double), ("is_forced", C.c_int), ("scalf", C.c_double)]
 
 
 class State(C.Structure):
@@ -51,6 +59,7 @@ SIGNATURES = {
     "cans_field_alloc": (C.c_int, [_vp, C.POINTER(_vp)]),
     "cans_field_alloc_kind": (C.c_int, [_vp, C.c_int, C.POINTER(_vp)]),
     "cans_field_free": (C.c_int, [_vp, _vp]),
+    "cans_field_register": (C.c_int, [_vp, _vp, C.c_int]),
     "cans_field_upload": (C.c_int, [_vp, _vp, _dp]),
     "cans_field_download": (C.c_int, [_vp, _vp, _dp]),
     "cans_device_alloc": (C.c_int, [_vp, C.c_long, C.POINTER(_vp)]),
@@ -67,7 +76,10 @@ SIGNATURES = {
     "cans_solve_helmholtz": (C.c_int, [_vp, _ip, _ip, _vp, C.c_double, C.c_double, _vp, _vp, _vp, _vp,
                                        _vp, _vp, _vp, _ip, _cp, _cp, _vp]),
     "cans_rk": (C.c_int, [_vp, _dp, _ip, _dp, _vp, _vp, _vp, _vp, C.c_double, C.c_double, _vp, _ip, _dp,
-                          _dp, _vp, _vp, _vp, _vp, _vp, _vp, _dp]),
+                          _dp, _dp, C.c_double, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _dp]),
+    "cans_rk_scal": (C.c_int, [_vp, _dp, _ip, _dp, _dp, _vp, _vp, _vp, C.c_double, C.c_double, _ip, _vp, _vp, _vp,
+                               C.c_int, C.c_double, C.c_double, _dp, _vp, _vp, _dp]),
+    "cans_bulk_forcing_s": (C.c_int, [_vp, _ip, C.c_int, C.c_double, _vp]),
     "cans_fillps": (C.c_int, [_vp, _ip, _dp, _vp, C.c_double, _vp, _vp, _vp, _vp]),
     "cans_correc": (C.c_int, [_vp, _ip, _dp, _vp, C.c_double, _vp, _vp, _vp, _vp]),
     "cans_updatep": (C.c_int, [_vp, _ip, _dp, _vp, _vp, C.c_double, _vp, _vp, C.c_int]),

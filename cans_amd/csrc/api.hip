@@ -132,7 +132,7 @@ int plan_bounduvw(cans_ctx* c, const char* cbc, const double* bc, const int* nb,
 }
 
 int plan_boundp(cans_ctx* c, const char* cbc, const double* bc, const int* nb, const int* is_bound, const double* dl,
-                const double* dzc, void* p, int halo_side) {
+                const double* dzc, void* p, int halo_side, int es = 0) {
   const Layout& L = c->L;
   const int n[3] = {L.n1, L.n2, L.n3};
   bool update_halo[2] = {true, true};
@@ -148,7 +148,9 @@ int plan_boundp(cans_ctx* c, const char* cbc, const double* bc, const int* nb, c
       const double dr = d < 2 ? dl[d] : (ib == 0 ? dzc[0] : dzc[L.n3]);
       push(ops, make_setbc(p, cbc[ib + 2 * d], ib, d, true, bc[ib + 2 * d], dr, n[d]));
     }
-    int rc = CANS_DISPATCH(c, launch_bound_dir<T>(c, ops));
+    int rc = es == (int)sizeof(double) ? launch_bound_dir<double>(c, ops)
+             : es == (int)sizeof(float) ? launch_bound_dir<float>(c, ops)
+                                        : CANS_DISPATCH(c, launch_bound_dir<T>(c, ops));
     if (rc) return rc;
   }
   return CANS_OK;
@@ -194,7 +196,8 @@ static int rk_core(cans_ctx* c, const double* rkpar, const double* dli, const do
                    const double* gvr_c, const double* gvr_f, double dt, double visc, const S* p,
                    const int* is_forced, const double* velf, const double* bforce, S* ru, S* rv, S* rw,
                    const double* u, const double* v, const double* w, double* uo, double* vo, double* wo,
-                   int write_ro = 1, S* const* eo = nullptr, S* divu = nullptr) {
+                   int write_ro = 1, S* const* eo = nullptr, S* divu = nullptr, const double* sb = nullptr,
+                   const double* gacc = nullptr, double beta = 0.) {
   if (c->rk_first) {  // rk.f90:72-100: RHS_old starts at zero on the first call
     c->rk_first = false;
     const size_t bytes = (size_t)c->L.cells() * sizeof(S);
@@ -220,24 +223,55 @@ static int rk_core(cans_ctx* c, const double* rkpar, const double* dli, const do
     m.velf[d] = velf[d];
     m.is_forced[d] = is_forced[d];
   }
+  if (sb) {
+    m.sb = sb;
+    for (int d = 0; d < 3; d++) m.gacc[d] = gacc[d];
+    m.beta = beta;
+  }
   return launch_mom_rk<S>(c, m);
+}
+
+// rk_scal (rk.f90:335-491) of one scalar: the update goes to the double scratch t,
+// f = scalf - <s> (or 0) to dscal[DSCAL_SCAL + slot]; then t -> s, plus f when
+// apply_forcing (bulk_forcing_s, scal.f90:313-329, fused)
+static int scal_core(cans_ctx* c, const double* rkpar, const double* dli, const double* dzci, const double* dzfi,
+                     const double* gvr_f, double alpha, double dt, const double* u, const double* v,
+                     const double* w, int is_forced, double scalf, double ssource, double* dsdtrko, double* s,
+                     int slot, int apply_forcing) {
+  double* t[3];
+  int rc = ensure_tmp_fields<double>(c, t);
+  if (rc) return rc;
+  ScalLaunch m;
+  m.u = u; m.v = v; m.w = w; m.s = s;
+  m.so = t[0];
+  m.ro = dsdtrko;
+  m.dzci = dzci; m.dzfi = dzfi; m.gvr_f = gvr_f;
+  m.dxi = dli[0]; m.dyi = dli[1]; m.alpha = alpha; m.dt = dt;
+  m.source = ssource; m.scalf = scalf;
+  m.rkpar[0] = rkpar[0]; m.rkpar[1] = rkpar[1];
+  m.is_forced = is_forced ? 1 : 0;
+  m.fslot = DSCAL_SCAL + slot;
+  if ((rc = launch_scal_rk(c, m))) return rc;
+  return launch_add_forcing<double>(c, t[0], s, DSCAL_SCAL + slot, apply_forcing && is_forced ? 1 : 0);
 }
 
 extern "C" int cans_rk(cans_ctx* c, const double* rkpar, const int* n, const double* dli, const double* dzci,
                        const double* dzfi, const double* gvr_c, const double* gvr_f, double dt, double visc,
-                       const void* p, const int* is_forced, const double* velf, const double* bforce, void* ru,
-                       void* rv, void* rw, void* u, void* v, void* w, double* f_h) {
+                       const void* p, const int* is_forced, const double* velf, const double* bforce,
+                       const double* gacc, double beta, const void* s, void* ru, void* rv, void* rw, void* u, void* v,
+                       void* w, double* f_h) {
   CHECK_CTX("cans_rk");
   CHECK_N("cans_rk", n);
   if (!rkpar || !dli || !dzci || !dzfi || !gvr_c || !gvr_f || !p || !is_forced || !velf || !bforce || !ru || !rv ||
-      !rw || !u || !v || !w)
+      !rw || !u || !v || !w || (s && !gacc))
     return fail(CANS_EINVAL, "cans_rk: null argument");
   int rc = CANS_DISPATCH(c, [&]() -> int {
     double* t[3];
     int r = ensure_tmp_fields<double>(c, t);
     if (r) return r;
     r = rk_core<T>(c, rkpar, dli, dzci, dzfi, gvr_c, gvr_f, dt, visc, (const T*)p, is_forced, velf, bforce, (T*)ru,
-                   (T*)rv, (T*)rw, (const double*)u, (const double*)v, (const double*)w, t[0], t[1], t[2]);
+                   (T*)rv, (T*)rw, (const double*)u, (const double*)v, (const double*)w, t[0], t[1], t[2], 1,
+                   nullptr, nullptr, (const double*)s, gacc, beta);
     if (r) return r;
     double* dst[3] = {(double*)u, (double*)v, (double*)w};
     for (int k = 0; k < 3; k++)
@@ -250,6 +284,38 @@ extern "C" int cans_rk(cans_ctx* c, const double* rkpar, const int* n, const dou
     CANS_HIP_TRY(hipStreamSynchronize(c->stream));
   }
   return CANS_OK;
+}
+
+extern "C" int cans_rk_scal(cans_ctx* c, const double* rkpar, const int* n, const double* dli, const double* l,
+                            const double* dzci, const double* dzfi, const double* gvr_f, double alpha, double dt,
+                            const int* is_bound, const void* u, const void* v, const void* w, int is_forced,
+                            double scalf, double ssource, double* fluxo, void* dsdtrko, void* s, double* f_h) {
+  (void)l;
+  (void)is_bound;
+  (void)fluxo;
+  CHECK_CTX("cans_rk_scal");
+  CHECK_N("cans_rk_scal", n);
+  if (!rkpar || !dli || !dzci || !dzfi || !gvr_f || !u || !v || !w || !dsdtrko || !s)
+    return fail(CANS_EINVAL, "cans_rk_scal: null argument");
+  int rc = scal_core(c, rkpar, dli, dzci, dzfi, gvr_f, alpha, dt, (const double*)u, (const double*)v,
+                     (const double*)w, is_forced, scalf, ssource, (double*)dsdtrko, (double*)s, 0, 0);
+  if (rc) return rc;
+  if (f_h) {
+    CANS_HIP_TRY(hipMemcpyAsync(f_h, c->dscal + DSCAL_SCAL, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CANS_HIP_TRY(hipStreamSynchronize(c->stream));
+  }
+  return CANS_OK;
+}
+
+extern "C" int cans_bulk_forcing_s(cans_ctx* c, const int* n, int is_forced, double ff, void* s) {
+  CHECK_CTX("cans_bulk_forcing_s");
+  CHECK_N("cans_bulk_forcing_s", n);
+  if (!s) return fail(CANS_EINVAL, "cans_bulk_forcing_s: null argument");
+  if (!is_forced) return CANS_OK;
+  // ff through a device slot, so the kernel is the one of the fused forcing
+  int rc = launch_fill(c, c->dscal + DSCAL_SCAL, 1, ff);
+  if (rc) return rc;
+  return launch_add_forcing<double>(c, (const double*)s, (double*)s, DSCAL_SCAL, 1);
 }
 
 extern "C" int cans_fillps(cans_ctx* c, const int* n, const double* dli, const double* dzfi, double dti,
@@ -295,7 +361,9 @@ extern "C" int cans_boundp(cans_ctx* c, const char* cbc, const int* n, const dou
   CHECK_CTX("cans_boundp");
   CHECK_N("cans_boundp", n);
   if (!cbc || !bc || !nb || !is_bound || !dl || !dzc || !p) return fail(CANS_EINVAL, "cans_boundp: null argument");
-  return plan_boundp(c, cbc, bc, nb, is_bound, dl, dzc, p, halo_side);
+  // a field from cans_field_alloc_kind carries its element type (a scalar is FP64 in
+  // every mode); other pointers are taken as pressure-like
+  return plan_boundp(c, cbc, bc, nb, is_bound, dl, dzc, p, halo_side, field_elem_size(c, p));
 }
 
 extern "C" int cans_updt_rhs_b(cans_ctx* c, const char* c_or_f, const char* cbc, const int* n, const int* is_bound,
@@ -421,10 +489,21 @@ static int rk3_substep(cans_ctx* c, const cans_step_params* s, int irk, double d
   void* divu_v = nullptr;
   if (delta && fuse_div && (rc = ensure_work(c, 4, (size_t)c->L.cells() * sizeof(S), &divu_v))) return rc;
   S* divu = (S*)divu_v;
+  // scalars first, from the old velocity (main.f90:440-450): rk_scal, bulk_forcing_s, boundp
+  for (int i = 0; i < s->nscal; i++) {
+    const cans_scalar& sc = s->scal[i];
+    if ((rc = scal_core(c, rkpar, s->dli, s->dzci_d, s->dzfi_d, s->grid_vol_ratio_f_d, sc.alpha, dt, u, v, w,
+                        sc.is_forced, sc.scalf, sc.source, (double*)sc.dsdtrko, (double*)sc.val, i, 1)))
+      return rc;
+    if ((rc = plan_boundp(c, sc.cbc, sc.bc, s->nb, s->is_bound, s->dl, s->dzc_h, sc.val, CANS_HALO_ABSENT,
+                          (int)sizeof(double))))
+      return rc;
+  }
+  const double* sb = (s->is_buoyancy && s->nscal > 0) ? (const double*)s->scal[0].val : nullptr;
   // rk: updated velocity (or its increment) into the scratch; u,v,w keep the old level for the stencil
   rc = rk_core<S>(c, rkpar, s->dli, s->dzci_d, s->dzfi_d, s->grid_vol_ratio_c_d, s->grid_vol_ratio_f_d, dt,
                   s->visc, p, s->is_forced, s->velf, s->bforce, (S*)st->dudtrko, (S*)st->dvdtrko, (S*)st->dwdtrko, u,
-                  v, w, t[0], t[1], t[2], irk != 3, delta ? e : nullptr, divu);
+                  v, w, t[0], t[1], t[2], irk != 3, delta ? e : nullptr, divu, sb, s->gacc, s->beta);
   if (rc) return rc;
   for (int k = 0; k < 3 && !delta; k++)
     if (s->is_forced[k] && (rc = launch_add_forcing<double>(c, t[k], t[k], k, 1))) return rc;

@@ -66,6 +66,8 @@ inline unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
 }  // namespace cans
 
 // opaque handles
+constexpr int DSCAL_SCAL = 16;  // first device-scalar slot of the scalars' bulk forcing (CANS_MAX_SCALARS of them)
+
 struct cans_ctx {
   cans::Layout L;
   int precision;          // CANS_FP64, or CANS_MIXED (pressure-like arrays stored as float)
@@ -77,7 +79,8 @@ struct cans_ctx {
   static constexpr int NWORK = 5;
   void* work[NWORK] = {};
   size_t work_bytes[NWORK] = {};
-  double* dscal = nullptr;     // device scalars: [0..2] forcing f, [3..] reductions
+  double* dscal = nullptr;     // device scalars (64): [0..2] forcing f, [8..11] reductions,
+                               // [DSCAL_SCAL + i] forcing of scalar i
   double* partials = nullptr;  // per-block reduction partials
   size_t partials_n = 0;
   bool rk_first = true;        // the next rk call is the first on this context

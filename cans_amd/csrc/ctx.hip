@@ -162,10 +162,33 @@ extern "C" int cans_field_free(cans_ctx* c, void* field) {
     base = it->second.base;
     g_fields.erase(it);
   }
+  if (!base) return CANS_OK;  // registered by the caller (cans_field_register): not ours to free
   if (c) hipStreamSynchronize(c->stream);
   CANS_HIP_TRY(hipFree(base));
   return CANS_OK;
 }
+
+extern "C" int cans_field_register(cans_ctx* c, void* field, int kind) {
+  if (!c || !field) return fail(CANS_EINVAL, "cans_field_register: null argument");
+  if (kind != CANS_FIELD_VELOCITY && kind != CANS_FIELD_PRESSURE)
+    return fail(CANS_EINVAL, "cans_field_register: unknown field kind");
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_fields.find(field);
+  if (it != g_fields.end() && it->second.base)
+    return fail(CANS_EINVAL, "cans_field_register: field allocated by cans_field_alloc_kind");
+  g_fields[field] = FieldRec{nullptr, kind_size(c, kind)};
+  return CANS_OK;
+}
+
+namespace cans {
+// element size of a registered field (cans_field_alloc*), 0 if unknown
+int field_elem_size(const cans_ctx* c, const void* field) {
+  (void)c;
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_fields.find(const_cast<void*>(field));
+  return it == g_fields.end() ? 0 : (int)it->second.es;
+}
+}  // namespace cans
 
 // Element size of a field: from the registry for fields of cans_field_alloc*, else
 // double (accepted only where every field is double, i.e. in an FP64 context).

@@ -35,7 +35,25 @@ struct MomLaunch {
   double bforce[3], velf[3];
   int is_forced[3];
   int read_ro = 1, write_ro = 1;  // RHS_old input/output needed (see launch_mom_rk)
+  const double* sb = nullptr;     // Boussinesq buoyancy: the first scalar (halo'd), or none
+  double gacc[3] = {0., 0., 0.}, beta = 0.;
 };
+
+// scalar transport + RK update of one scalar (rk.f90:335-491, explicit diffusion);
+// the bulk forcing f (scalf - mean, or 0) goes to c->dscal[fslot]
+struct ScalLaunch {
+  const double *u, *v, *w, *s;
+  double* so;  // updated scalar
+  double* ro;  // dsdtrko
+  const double *dzci, *dzfi, *gvr_f;
+  double dxi, dyi, alpha, dt, source, scalf;
+  double rkpar[2];
+  int is_forced;
+  int fslot;
+};
+int launch_scal_rk(cans_ctx* c, const ScalLaunch& m);
+int launch_fill(cans_ctx* c, double* p, int n, double v);  // p[0..n) = v, on the context stream
+int field_elem_size(const cans_ctx* c, const void* field);
 
 // XCD-aware tile order: the hardware hands work-group b to XCD b % 8; remap so that
 // each XCD walks a contiguous range of tiles (neighbouring tiles then share its L2)

@@ -46,6 +46,8 @@ struct MomArgs {
   int read_ro, write_ro;
   int delta;
   double* partials;              // [3][nblocks]
+  const double* sb;              // nullptr, or the buoyancy scalar (halo'd)
+  double bq[3];                  // factor12*gacc(d)*beta*0.5
 };
 
 // Tiling of the momentum kernel: a work-group owns a TX x TY column of cells and
@@ -165,15 +167,21 @@ __global__ void __launch_bounds__(MOM_TX* MOM_TY) k_mom_rk(MomArgs<S> a) {
       const double p_ip = (double)a.p[c + 1], p_jp = (double)a.p[c + sy], p_kp = (double)a.p[c + sz];
       const double ru = a.read_ro ? (double)a.ru[o] : 0., rv = a.read_ro ? (double)a.rv[o] : 0.,
                    rw = a.read_ro ? (double)a.rw[o] : 0.;
-      const double un = a.read_ro ? u_ccc + a.factor1 * dudt + a.factor2 * ru +
-                                        a.factor12 * (a.bforce[0] - dxi * (p_ip - p_c))
-                                  : u_ccc + a.factor1 * dudt + a.factor12 * (a.bforce[0] - dxi * (p_ip - p_c));
-      const double vn = a.read_ro ? v_ccc + a.factor1 * dvdt + a.factor2 * rv +
-                                        a.factor12 * (a.bforce[1] - dyi * (p_jp - p_c))
-                                  : v_ccc + a.factor1 * dvdt + a.factor12 * (a.bforce[1] - dyi * (p_jp - p_c));
-      const double wn = a.read_ro ? w_ccc + a.factor1 * dwdt + a.factor2 * rw +
-                                        a.factor12 * (a.bforce[2] - dzci_k * (p_kp - p_c))
-                                  : w_ccc + a.factor1 * dwdt + a.factor12 * (a.bforce[2] - dzci_k * (p_kp - p_c));
+      double un = a.read_ro ? u_ccc + a.factor1 * dudt + a.factor2 * ru +
+                                  a.factor12 * (a.bforce[0] - dxi * (p_ip - p_c))
+                            : u_ccc + a.factor1 * dudt + a.factor12 * (a.bforce[0] - dxi * (p_ip - p_c));
+      double vn = a.read_ro ? v_ccc + a.factor1 * dvdt + a.factor2 * rv +
+                                  a.factor12 * (a.bforce[1] - dyi * (p_jp - p_c))
+                            : v_ccc + a.factor1 * dvdt + a.factor12 * (a.bforce[1] - dyi * (p_jp - p_c));
+      double wn = a.read_ro ? w_ccc + a.factor1 * dwdt + a.factor2 * rw +
+                                  a.factor12 * (a.bforce[2] - dzci_k * (p_kp - p_c))
+                            : w_ccc + a.factor1 * dwdt + a.factor12 * (a.bforce[2] - dzci_k * (p_kp - p_c));
+      if (a.sb) {  // Boussinesq buoyancy of the first scalar (rk.f90:163-177)
+        const double s_c = a.sb[c];
+        un = un - a.bq[0] * (a.sb[c + 1] + s_c);
+        vn = vn - a.bq[1] * (a.sb[c + sy] + s_c);
+        wn = wn - a.bq[2] * (a.sb[c + sz] + s_c);
+      }
       if (a.delta) {
         a.eu[c] = (S)(un - u_ccc);
         a.ev[c] = (S)(vn - v_ccc);
@@ -288,6 +296,62 @@ static int ensure_partials(cans_ctx* c, size_t n) {
   c->partials_n = n;
   CANS_HIP_TRY(hipMalloc((void**)&c->partials, n * sizeof(double)));
   return CANS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Scalar transport, explicit diffusion (scal.f90:41-98, the default loop), fused
+// with the RK update of rk_scal (rk.f90:427-441) and RHS_old <- RHS (rk.f90:481-489):
+//   so = s + factor1*dsdt + factor2*dsdtrko + factor12*ssource,  dsdtrko <- dsdt.
+// With a forcing, block partials of so*grid_vol_ratio_f feed the bulk mean
+// (rk.f90:471-474).  One thread per cell of a row; neighbours come through the caches.
+// ---------------------------------------------------------------------------
+struct ScalArgs {
+  Layout L;
+  const double *u, *v, *w, *s;
+  double* so;                 // updated scalar (interior written)
+  double* ro;                 // dsdtrko (1:n1,1:n2,1:n3)
+  const double *dzci, *dzfi, *gvr_f;
+  double dxi, dyi, alpha, factor1, factor2, factor12, source;
+  int read_ro;
+  double* partials;           // nullptr: no bulk mean
+};
+
+__global__ void __launch_bounds__(256) k_scal_rk(ScalArgs a) {
+  const Layout& L = a.L;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y + 1, k = blockIdx.z + 1;
+  double acc = 0.;
+  if (i <= L.n1) {
+    const long c = L.at(i, j, k), sy = L.sy, sz = L.sz;
+    const double* s = a.s;
+    const double sc = s[c], sxm = s[c - 1], sxp = s[c + 1], sym = s[c - sy], syp = s[c + sy];
+    const double szm = s[c - sz], szp = s[c + sz];
+    const double dxi = a.dxi, dyi = a.dyi, alpha = a.alpha, dzfi_k = a.dzfi[k];
+    const double usim = 0.5 * (sxm + sc) * a.u[c - 1];
+    const double usip = 0.5 * (sxp + sc) * a.u[c];
+    const double vsjm = 0.5 * (sym + sc) * a.v[c - sy];
+    const double vsjp = 0.5 * (syp + sc) * a.v[c];
+    const double wskm = 0.5 * (szm + sc) * a.w[c - sz];
+    const double wskp = 0.5 * (szp + sc) * a.w[c];
+    const double dsdxp = (sxp - sc) * dxi, dsdxm = (sc - sxm) * dxi;
+    const double dsdyp = (syp - sc) * dyi, dsdym = (sc - sym) * dyi;
+    const double dsdzp = (szp - sc) * a.dzci[k], dsdzm = (sc - szm) * a.dzci[k - 1];
+    double dsdt = dxi * (-usip + usim) + dyi * (-vsjp + vsjm) + dzfi_k * (-wskp + wskm);
+    const double dxy = (dsdxp - dsdxm) * alpha * dxi + (dsdyp - dsdym) * alpha * dyi;
+    const double dz = (dsdzp - dsdzm) * alpha * dzfi_k;
+    dsdt = dsdt + dxy + dz;
+    const long o = (long)(i - 1) + (long)L.n1 * ((j - 1) + (long)L.n2 * (k - 1));
+    const double sn = a.read_ro ? sc + a.factor1 * dsdt + a.factor2 * a.ro[o] + a.factor12 * a.source
+                                : sc + a.factor1 * dsdt + a.factor12 * a.source;
+    a.so[c] = sn;
+    a.ro[o] = dsdt;
+    acc = sn * a.gvr_f[k];
+  }
+  if (a.partials) {
+    const long b = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+    double d0 = 0., d1 = 0.;
+    block_sum3<256>(acc, d0, d1, a.partials + b, nullptr, nullptr, true);
+  }
 }
 
 // u(1:n) += f(c) (mom.f90:725-769); src may differ from dst (copy + forcing)
@@ -586,6 +650,9 @@ int launch_mom_rk(cans_ctx* c, const MomLaunch<S>& m) {
   a.partials = c->partials;
   a.read_ro = m.read_ro && m.rkpar[1] != 0.;
   a.write_ro = m.write_ro;
+  // the buoyancy coefficient in the reference's left-to-right order: factor12*gacc*beta*0.5
+  a.sb = m.sb;
+  for (int d = 0; d < 3; d++) a.bq[d] = a.factor12 * m.gacc[d] * m.beta * 0.5;
   k_mom_rk<S><<<grid, dim3(MOM_TX, MOM_TY), 0, c->stream>>>(a);
   CANS_LAUNCH_CHECK("k_mom_rk");
   if (a.sum_mask)
@@ -600,6 +667,42 @@ int launch_add_forcing(cans_ctx* c, const T* src, T* dst, int comp, int apply) {
   const Layout& L = c->L;
   k_add_forcing<T><<<row_grid(L, 256), 256, 0, c->stream>>>(L, src, dst, c->dscal, comp, apply);
   CANS_LAUNCH_CHECK("k_add_forcing");
+  return CANS_OK;
+}
+
+int launch_fill(cans_ctx* c, double* p, int n, double v) {
+  k_fill<<<cdiv(n, 64), 64, 0, c->stream>>>(p, n, v);
+  CANS_LAUNCH_CHECK("k_fill");
+  return CANS_OK;
+}
+
+int launch_scal_rk(cans_ctx* c, const ScalLaunch& m) {
+  const Layout& L = c->L;
+  ScalArgs a;
+  a.L = L;
+  a.u = m.u; a.v = m.v; a.w = m.w; a.s = m.s;
+  a.so = m.so; a.ro = m.ro;
+  a.dzci = m.dzci; a.dzfi = m.dzfi; a.gvr_f = m.gvr_f;
+  a.dxi = m.dxi; a.dyi = m.dyi; a.alpha = m.alpha;
+  a.factor1 = m.rkpar[0] * m.dt;
+  a.factor2 = m.rkpar[1] * m.dt;
+  a.factor12 = a.factor1 + a.factor2;
+  a.source = m.source;
+  a.read_ro = m.rkpar[1] != 0.;
+  const dim3 grid = row_grid(L, 256);
+  const long nb = (long)grid.x * grid.y * grid.z;
+  int rc;
+  if (m.is_forced && (rc = ensure_partials(c, nb))) return rc;
+  a.partials = m.is_forced ? c->partials : nullptr;
+  k_scal_rk<<<grid, 256, 0, c->stream>>>(a);
+  CANS_LAUNCH_CHECK("k_scal_rk");
+  // f = scalf - mean (rk.f90:471-474), or 0, into dscal[m.fslot]
+  if (m.is_forced) {
+    const double sf[3] = {m.scalf, 0., 0.};
+    return reduce_partials(c, c->partials, (int)nb, 1, 0, c->dscal + m.fslot, c->partials + nb, 1, 1, sf);
+  }
+  k_fill<<<1, 64, 0, c->stream>>>(c->dscal + m.fslot, 1, 0.);
+  CANS_LAUNCH_CHECK("k_fill");
   return CANS_OK;
 }
 

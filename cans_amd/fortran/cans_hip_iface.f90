@@ -29,7 +29,21 @@ module cans_hip
     type(c_ptr)            :: dzc_h, dzf_h, dzci_d, dzfi_d, grid_vol_ratio_c_d, grid_vol_ratio_f_d, plan
     real(c_double)         :: normfft
     type(c_ptr)            :: lambdaxy_d, a_d, b_d, c_d, rhsbx_d, rhsby_d, rhsbz_d
+    integer(c_int)         :: nscal = 0
+    type(c_ptr)            :: scal = c_null_ptr      ! nscal entries of type(cans_scalar)
+    integer(c_int)         :: is_buoyancy = 0
+    real(c_double)         :: gacc(3) = 0., beta = 0.
   end type cans_step_params
+  ! the members of src/scal.f90's scalar type that the step uses
+  type, bind(C) :: cans_scalar
+    type(c_ptr)            :: val, dsdtrko
+    real(c_double)         :: alpha
+    character(kind=c_char) :: cbc(6)
+    real(c_double)         :: bc(6)
+    real(c_double)         :: source
+    integer(c_int)         :: is_forced
+    real(c_double)         :: scalf
+  end type cans_scalar
   type, bind(C) :: cans_state
     type(c_ptr) :: u, v, w, p, pp, dudtrko, dvdtrko, dwdtrko
   end type cans_state
@@ -74,6 +88,11 @@ module cans_hip
     integer(c_int) function cans_field_free(ctx, field) bind(C, name='cans_field_free')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, field
+    end function
+    integer(c_int) function cans_field_register(ctx, field, kind) bind(C, name='cans_field_register')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, field
+      integer(c_int), value :: kind
     end function
     integer(c_int) function cans_field_upload(ctx, field, host) bind(C, name='cans_field_upload')
       import :: c_int, c_ptr, c_double
@@ -166,15 +185,36 @@ module cans_hip
       real(c_double), value :: normfft, alpha
       character(kind=c_char), intent(in) :: cbc(6), c_or_f(3)
     end function
+    ! s: the first scalar's field with Boussinesq buoyancy (rk.f90:69-72), else c_null_ptr
     integer(c_int) function cans_rk(ctx, rkpar, n, dli, dzci, dzfi, grid_vol_ratio_c, grid_vol_ratio_f, dt, visc, &
-                                    p, is_forced, velf, bforce, dudtrko, dvdtrko, dwdtrko, u, v, w, f) &
-                   bind(C, name='cans_rk')
+                                    p, is_forced, velf, bforce, gacc, beta, s, dudtrko, dvdtrko, dwdtrko, &
+                                    u, v, w, f) bind(C, name='cans_rk')
       import :: c_int, c_ptr, c_double
-      type(c_ptr), value :: ctx, dzci, dzfi, grid_vol_ratio_c, grid_vol_ratio_f, p, dudtrko, dvdtrko, dwdtrko, u, v, w
-      real(c_double), intent(in) :: rkpar(2), dli(3), velf(3), bforce(3)
+      type(c_ptr), value :: ctx, dzci, dzfi, grid_vol_ratio_c, grid_vol_ratio_f, p, s, dudtrko, dvdtrko, dwdtrko, &
+                            u, v, w
+      real(c_double), intent(in) :: rkpar(2), dli(3), velf(3), bforce(3), gacc(3)
       integer(c_int), intent(in) :: n(3), is_forced(3)
-      real(c_double), value :: dt, visc
+      real(c_double), value :: dt, visc, beta
       real(c_double), intent(out) :: f(3)
+    end function
+    integer(c_int) function cans_rk_scal(ctx, rkpar, n, dli, l, dzci, dzfi, grid_vol_ratio_f, alpha, dt, is_bound, &
+                                         u, v, w, is_forced, scalf, ssource, fluxo, dsdtrko, s, f) &
+                   bind(C, name='cans_rk_scal')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, dzci, dzfi, grid_vol_ratio_f, u, v, w, dsdtrko, s
+      real(c_double), intent(in) :: rkpar(2), dli(3), l(3)
+      integer(c_int), intent(in) :: n(3), is_bound(6)
+      real(c_double), value :: alpha, dt, scalf, ssource
+      integer(c_int), value :: is_forced
+      real(c_double), intent(inout) :: fluxo(6)
+      real(c_double), intent(out) :: f
+    end function
+    integer(c_int) function cans_bulk_forcing_s(ctx, n, is_forced, ff, s) bind(C, name='cans_bulk_forcing_s')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, s
+      integer(c_int), intent(in) :: n(3)
+      integer(c_int), value :: is_forced
+      real(c_double), value :: ff
     end function
     integer(c_int) function cans_fillps(ctx, n, dli, dzfi, dti, u, v, w, p) bind(C, name='cans_fillps')
       import :: c_int, c_ptr, c_double

@@ -19,7 +19,7 @@ program drive_step
   integer :: ng(3), gtype, nstep, istep, irk, k, idir, ib
   real(dp) :: l(3), gr, visci, dt, time, dtrk, dtrki, alpha
   character(len=1) :: cbcvel(0:1,3,3), cbcpre(0:1,3)
-  real(dp) :: bcvel(0:1,3,3), bcpre(0:1,3), bforce(3), velf(3)
+  real(dp) :: bcvel(0:1,3,3), bcpre(0:1,3), bforce(3), velf(3), gacc(3) = 0.
   logical :: is_forced(3), is_mixed
   integer(c_int) :: es_p
   namelist /dns/ ng, l, gtype, gr, visci, dt, nstep, cbcvel, cbcpre, bcvel, bcpre, bforce, is_forced, velf, is_mixed
@@ -113,7 +113,7 @@ program drive_step
       dtrk = sum(rkcoeff(:,irk))*dt
       dtrki = dtrk**(-1)
       call cans_check(cans_rk(ctx, rkcoeff(:,irk), n, dli, dzci_d, dzfi_d, gvr_c_d, gvr_f_d, dt, visc, p, &
-                              isf, velf, bforce, ru, rv, rw, u, v, w, f), 'cans_rk')
+                              isf, velf, bforce, gacc, 0._c_double, c_null_ptr, ru, rv, rw, u, v, w, f), 'cans_rk')
       call cans_check(cans_bounduvw(ctx, cbcvel, n, bcvel, nb, is_bound, dl, dzc, dzf, u, v, w, 0, &
                                     CANS_HALO_LOWER, CANS_HALO_ABSENT), 'cans_bounduvw')
       call cans_check(cans_fillps(ctx, n, dli, dzfi_d, dtrki, u, v, w, pp), 'cans_fillps')

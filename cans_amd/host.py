@@ -38,6 +38,12 @@ class Case:
     bforce: tuple = (0.0, 0.0, 0.0)
     is_forced: tuple = (False, False, False)
     velf: tuple = (0.0, 0.0, 0.0)
+    # &scalar namelist (src/param.f90:140-152), one dict per scalar with keys alphai, cbc
+    # (6 chars, cbcscal(0:1,1:3)), bc (6), source, is_forced, scalf; gacc/beta of &dns
+    scalars: tuple = ()
+    gacc: tuple = (0.0, 0.0, 0.0)
+    beta: float = 0.0
+    is_boussinesq_buoyancy: bool = False
 
     @property
     def n(self):
@@ -176,6 +182,8 @@ class DeviceField:
         self.ptr = base + es * shift
         n1, n2, n3 = ctx.n
         self.view = self.buf[shift:shift + ctx.elems].view(n3 + 2, n2 + 2, ctx.sy)[:, :, :n1 + 2]
+        # tell the library the element type behind this address (see cans_field_register)
+        LB.check(LB.lib().cans_field_register(ctx.h, C.c_void_p(self.ptr), kind), "cans_field_register")
 
     def numpy(self):
         """Fortran-ordered (n1+2, n2+2, n3+2) float64 copy on the host (FP32 storage widens exactly)."""
@@ -306,21 +314,46 @@ class Simulation:
         if any(abs(x) > 0 for x in case.bcpre):
             raise NotImplementedError("non-homogeneous pressure boundary values")
         self.prm = prm
+        # scalars: FP64 fields in every precision mode (scal.f90:17-41)
+        self.scal_spec = [dict(alpha=float(sc.get("alphai", 1.0)) ** (-1), cbc=sc["cbc"],
+                               bc=tuple(float(x) for x in sc.get("bc", (0.0,) * 6)),
+                               source=float(sc.get("source", 0.0)), is_forced=int(bool(sc.get("is_forced", False))),
+                               scalf=float(sc.get("scalf", 0.0))) for sc in case.scalars]
+        if len(self.scal_spec) > 32:
+            raise ValueError("at most CANS_MAX_SCALARS (32) scalars")
+        self.s = [ctx.field() for _ in self.scal_spec]
+        self.dsdtrko = [torch.zeros(n1 * n2 * n3, dtype=torch.float64, device=ctx.device) for _ in self.scal_spec]
+        # param.f90:308-310: the largest scalar diffusivity for chkdt (1/huge without scalars)
+        self.alpha_max = max((sp["alpha"] for sp in self.scal_spec), default=0.0)
+        self.is_buoyancy = bool(case.is_boussinesq_buoyancy) and len(self.s) > 0
+        self._scal_c = (LB.Scalar * max(1, len(self.s)))()
+        for i, sp in enumerate(self.scal_spec):
+            e = self._scal_c[i]
+            e.val, e.dsdtrko = self.s[i].ptr, self.dsdtrko[i].data_ptr()
+            e.alpha, e.cbc, e.bc = sp["alpha"], sp["cbc"].encode(), LB.dvec(sp["bc"])
+            e.source, e.is_forced, e.scalf = sp["source"], sp["is_forced"], sp["scalf"]
+        prm.nscal = len(self.s)
+        prm.scal = C.cast(self._scal_c, C.c_void_p).value if self.s else None
+        prm.is_buoyancy = int(self.is_buoyancy)
+        prm.gacc = LB.dvec(case.gacc)
+        prm.beta = float(case.beta)
         st = LB.State()
         st.u, st.v, st.w, st.p, st.pp = (f.ptr for f in (self.u, self.v, self.w, self.p, self.pp))
         st.dudtrko, st.dvdtrko, st.dwdtrko = (t.data_ptr() for t in self.rhs_o)
         self.st = st
 
     # ---- state ----
-    def set_state(self, u=None, v=None, w=None, p=None):
-        for f, a in ((self.u, u), (self.v, v), (self.w, w), (self.p, p)):
+    def set_state(self, u=None, v=None, w=None, p=None, s=()):
+        for f, a in ((self.u, u), (self.v, v), (self.w, w), (self.p, p)) + tuple(zip(self.s, s)):
             if a is not None:
                 f.set(a)
 
     def init_bounds(self, restart=False):
-        """main.f90:389-390."""
+        """main.f90:389-395."""
         self.bounduvw(self.u, self.v, self.w, keep_norm=1 if restart else 0)
         self.boundp(self.p, LB.HALO_UPPER)
+        for i in range(len(self.s)):
+            self.boundp_s(i)
 
     # ---- operators (reference argument lists; single GPU) ----
     def bounduvw(self, u, v, w, keep_norm=-1, halos_norm=LB.HALO_ABSENT, halos_tang=LB.HALO_ABSENT):
@@ -339,15 +372,42 @@ class Simulation:
     def rk(self, irk, dt, with_f=True):
         c, g = self.case, self.grid
         f = (C.c_double * 3)()
+        sb = C.c_void_p(self.s[0].ptr) if self.is_buoyancy else None
         LB.check(LB.lib().cans_rk(self.ctx.h, LB.dvec(RKCOEFF[irk - 1]), LB.ivec(c.n), LB.dvec(g.dli),
                                   C.c_void_p(self.dzci_d.data_ptr()), C.c_void_p(self.dzfi_d.data_ptr()),
                                   C.c_void_p(self.gvr_c_d.data_ptr()), C.c_void_p(self.gvr_f_d.data_ptr()),
                                   dt, g.visc, C.c_void_p(self.p.ptr),
                                   LB.ivec([int(bool(x)) for x in c.is_forced]), LB.dvec(c.velf),
-                                  LB.dvec(c.bforce), *[C.c_void_p(t.data_ptr()) for t in self.rhs_o],
+                                  LB.dvec(c.bforce), LB.dvec(c.gacc), float(c.beta), sb,
+                                  *[C.c_void_p(t.data_ptr()) for t in self.rhs_o],
                                   C.c_void_p(self.u.ptr), C.c_void_p(self.v.ptr), C.c_void_p(self.w.ptr),
                                   f if with_f else None), "cans_rk")
         return np.array(f[:]) if with_f else None
+
+    def rk_scal(self, i, irk, dt):
+        """rk.f90:335 for scalar i; returns its bulk forcing f."""
+        c, g, sp = self.case, self.grid, self.scal_spec[i]
+        f = C.c_double()
+        fluxo = (C.c_double * 6)()
+        LB.check(LB.lib().cans_rk_scal(self.ctx.h, LB.dvec(RKCOEFF[irk - 1]), LB.ivec(c.n), LB.dvec(g.dli),
+                                       LB.dvec(g.l), C.c_void_p(self.dzci_d.data_ptr()),
+                                       C.c_void_p(self.dzfi_d.data_ptr()), C.c_void_p(self.gvr_f_d.data_ptr()),
+                                       sp["alpha"], dt, LB.ivec(g.is_bound), C.c_void_p(self.u.ptr),
+                                       C.c_void_p(self.v.ptr), C.c_void_p(self.w.ptr), sp["is_forced"], sp["scalf"],
+                                       sp["source"], fluxo, C.c_void_p(self.dsdtrko[i].data_ptr()),
+                                       C.c_void_p(self.s[i].ptr), C.byref(f)), "cans_rk_scal")
+        return f.value
+
+    def bulk_forcing_s(self, i, ff):
+        LB.check(LB.lib().cans_bulk_forcing_s(self.ctx.h, LB.ivec(self.case.n), self.scal_spec[i]["is_forced"], ff,
+                                              C.c_void_p(self.s[i].ptr)), "cans_bulk_forcing_s")
+
+    def boundp_s(self, i):
+        """boundp of scalar i with its own cbc/bc (main.f90:449)."""
+        c, g, sp = self.case, self.grid, self.scal_spec[i]
+        LB.check(LB.lib().cans_boundp(self.ctx.h, sp["cbc"].encode(), LB.ivec(c.n), LB.dvec(sp["bc"]),
+                                      LB.ivec(g.nb), LB.ivec(g.is_bound), LB.dvec(g.dl), _dp(self._dzc),
+                                      C.c_void_p(self.s[i].ptr), LB.HALO_ABSENT), "cans_boundp")
 
     def fillps(self, dti):
         g = self.grid
@@ -374,7 +434,8 @@ class Simulation:
         g = self.grid
         out = C.c_double()
         LB.check(LB.lib().cans_chkdt(self.ctx.h, LB.ivec(g.n), LB.dvec(g.dl), C.c_void_p(self.dzci_d.data_ptr()),
-                                     C.c_void_p(self.dzfi_d.data_ptr()), g.visc, 0.0, C.c_void_p(self.u.ptr),
+                                     C.c_void_p(self.dzfi_d.data_ptr()), g.visc, self.alpha_max,
+                                     C.c_void_p(self.u.ptr),
                                      C.c_void_p(self.v.ptr), C.c_void_p(self.w.ptr), 0, 0, C.byref(out)),
                  "cans_chkdt")
         return out.value
@@ -421,6 +482,10 @@ class Simulation:
             rkpar = RKCOEFF[irk - 1]
             dtrk = (rkpar[0] + rkpar[1]) * dt
             dtrki = dtrk ** (-1)
+            for i in range(len(self.s)):  # main.f90:440-450
+                fs = self.rk_scal(i, irk, dt)
+                self.bulk_forcing_s(i, fs)
+                self.boundp_s(i)
             self.rk(irk, dt, with_f=False)
             self.bounduvw(self.u, self.v, self.w, 0, LB.HALO_LOWER)
             self.fillps(dtrki)

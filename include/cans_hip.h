@@ -67,6 +67,12 @@ int cans_ctx_elem_size(const cans_ctx *ctx, int kind, int *bytes);
 int cans_field_alloc(cans_ctx *ctx, void **field_d);
 int cans_field_alloc_kind(cans_ctx *ctx, int kind, void **field_d);
 int cans_field_free(cans_ctx *ctx, void *field_d);
+/* Declares the kind of a field the caller allocated itself (origin = element (0,0,0)),
+ * so that entry points whose element type depends on the field (cans_boundp on a
+ * scalar in a CANS_MIXED context, cans_field_upload/download) read it right.
+ * Registering an address again replaces its kind; cans_field_free only forgets it.
+ * No counterpart in the reference (its fields are typed Fortran arrays). */
+int cans_field_register(cans_ctx *ctx, void *field_d, int kind);
 /* host arrays are the reference's contiguous (0:n1+1,0:n2+1,0:n3+1) Fortran arrays
  * of double; float fields round on upload.  In a CANS_MIXED context the field
  * must come from cans_field_alloc_kind (its element type is looked up) */
@@ -120,15 +126,31 @@ int cans_solve_helmholtz(cans_ctx *ctx, const int *n, const int *ng, const cans_
                          const double *a_d, const double *b_d, const double *c_d,
                          const double *rhsbx_d, const double *rhsby_d, const double *rhsbz_d,
                          const int *is_bound, const char *cbc, const char *c_or_f, void *p_d);
-/* src/rk.f90:24 (no scalars, explicit diffusion) -- f_h (3) receives the bulk
- * forcing like the reference's intent(out) f; pass NULL to keep it on the
- * device (no host synchronisation) */
+/* src/rk.f90:24 (explicit diffusion) -- s_d is the first scalar's halo'd FP64 field
+ * when Boussinesq buoyancy is on (rk.f90:69-72, 163-177; the reference passes the
+ * scalars array), NULL otherwise (gacc and beta are then not read).  f_h (3)
+ * receives the bulk forcing like the reference's intent(out) f; pass NULL to keep
+ * it on the device (no host synchronisation) */
 int cans_rk(cans_ctx *ctx, const double *rkpar, const int *n, const double *dli,
             const double *dzci_d, const double *dzfi_d, const double *grid_vol_ratio_c_d,
             const double *grid_vol_ratio_f_d, double dt, double visc, const void *p_d,
             const int *is_forced, const double *velf, const double *bforce,
+            const double *gacc, double beta, const void *s_d,
             void *dudtrko_d, void *dvdtrko_d, void *dwdtrko_d,
             void *u_d, void *v_d, void *w_d, double *f_h);
+/* src/rk.f90:335 (rk_scal), explicit diffusion.  l, is_bound and fluxo serve the
+ * wall-flux branch, which the reference compiles out (is_cmpt_wallflux is a .false.
+ * parameter there); they are accepted and not read.  The scalar and dsdtrko are
+ * FP64 fields in both precision modes (cans_field_alloc_kind with
+ * CANS_FIELD_VELOCITY; dsdtrko holds n1*n2*n3 doubles).  *f_h (if not NULL)
+ * receives the bulk forcing f = scalf - <s> when is_forced, else 0. */
+int cans_rk_scal(cans_ctx *ctx, const double *rkpar, const int *n, const double *dli,
+                 const double *l, const double *dzci_d, const double *dzfi_d,
+                 const double *grid_vol_ratio_f_d, double alpha, double dt, const int *is_bound,
+                 const void *u_d, const void *v_d, const void *w_d, int is_forced, double scalf,
+                 double ssource, double *fluxo, void *dsdtrko_d, void *s_d, double *f_h);
+/* src/scal.f90:313 (bulk_forcing_s): s(1:n) += ff when is_forced */
+int cans_bulk_forcing_s(cans_ctx *ctx, const int *n, int is_forced, double ff, void *s_d);
 /* src/fillps.f90:13 */
 int cans_fillps(cans_ctx *ctx, const int *n, const double *dli, const double *dzfi_d, double dti,
                 const void *u_d, const void *v_d, const void *w_d, void *p_d);
@@ -185,7 +207,25 @@ typedef struct cans_step_params {
   double normfft;
   const double *lambdaxy_d, *a_d, *b_d, *c_d;  /* pressure solver arrays */
   const double *rhsbx_d, *rhsby_d, *rhsbz_d;   /* pressure BC rhs, NULL if homogeneous */
+  /* scalars (main.f90:440-450, at most CANS_MAX_SCALARS) and the Boussinesq buoyancy
+   * of scal[0] (rk.f90:69-72); nscal = 0 and is_buoyancy = 0 give the velocity-only step */
+  int nscal;
+  const struct cans_scalar *scal;
+  int is_buoyancy;
+  double gacc[3], beta;
 } cans_step_params;
+
+/* the members of src/scal.f90:17-41's scalar type that the step uses */
+typedef struct cans_scalar {
+  void *val, *dsdtrko;                         /* FP64: halo'd field; (1:n1,1:n2,1:n3) */
+  double alpha;                                /* diffusivity 1/alphai */
+  char cbc[6];                                 /* cbcscal(0:1,1:3) */
+  double bc[6];                                /* bcscal(0:1,1:3) */
+  double source;                               /* ssource */
+  int is_forced;                               /* is_sforced */
+  double scalf;                                /* scalf */
+} cans_scalar;
+#define CANS_MAX_SCALARS 32
 
 typedef struct cans_state {
   void *u, *v, *w, *p, *pp;                    /* halo'd fields */

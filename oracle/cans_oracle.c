@@ -429,6 +429,7 @@ void orc_rk(const double *rkpar, const int *n, const double *dli, const double *
             const double *dzfi, const double *grid_vol_ratio_c,
             const double *grid_vol_ratio_f, double dt, double visc, const double *p,
             const int *is_forced, const double *velf, const double *bforce,
+            const double *gacc, double beta, const double *s,
             double *dudtrko, double *dvdtrko, double *dwdtrko,
             double *u, double *v, double *w, double *f, double *work,
             int is_impdiff, int is_impdiff_1d) {
@@ -453,6 +454,11 @@ void orc_rk(const double *rkpar, const int *n, const double *dli, const double *
                factor12 * (bforce[1] - dli[1] * (p[HX(i, j + 1, k)] - p[h]));
         w[h] = w[h] + factor1 * dwdtrk[o] + factor2 * dwdtrko[o] +
                factor12 * (bforce[2] - dzci[k] * (p[HX(i, j, k + 1)] - p[h]));
+        if (s) { /* Boussinesq buoyancy of the first scalar, rk.f90:163-177 */
+          u[h] = u[h] - factor12 * gacc[0] * beta * 0.5 * (s[HX(i + 1, j, k)] + s[h]);
+          v[h] = v[h] - factor12 * gacc[1] * beta * 0.5 * (s[HX(i, j + 1, k)] + s[h]);
+          w[h] = w[h] - factor12 * gacc[2] * beta * 0.5 * (s[HX(i, j, k + 1)] + s[h]);
+        }
         if (is_impdiff) {
           u[h] = u[h] + factor12 * dudtrkd[o];
           v[h] = v[h] + factor12 * dvdtrkd[o];
@@ -491,6 +497,92 @@ void orc_rk(const double *rkpar, const int *n, const double *dli, const double *
           for (int i = 1; i <= n1; i++) q[HX(i, j, k)] = q[HX(i, j, k)] + ff;
     }
   }
+}
+
+/* ------------------------------------------------------------------------ */
+/* scalar transport: src/scal.f90:41-98 (the default, non-unswitched loop)    */
+/* ------------------------------------------------------------------------ */
+void orc_scal(const int *n, double dxi, double dyi, const double *dzci, const double *dzfi,
+              double visc, const double *u, const double *v, const double *w, const double *s,
+              double *dsdt, double *dsdtd, int is_impdiff, int is_impdiff_1d) {
+  DIMS
+  #pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 1; k <= n3; k++)
+    for (int j = 1; j <= n2; j++)
+      for (int i = 1; i <= n1; i++) {
+        const size_t h = HX(i, j, k), o = IX(i, j, k);
+        const double sc = s[h];
+        const double usim = 0.5 * (s[HX(i - 1, j, k)] + sc) * u[HX(i - 1, j, k)];
+        const double usip = 0.5 * (s[HX(i + 1, j, k)] + sc) * u[h];
+        const double vsjm = 0.5 * (s[HX(i, j - 1, k)] + sc) * v[HX(i, j - 1, k)];
+        const double vsjp = 0.5 * (s[HX(i, j + 1, k)] + sc) * v[h];
+        const double wskm = 0.5 * (s[HX(i, j, k - 1)] + sc) * w[HX(i, j, k - 1)];
+        const double wskp = 0.5 * (s[HX(i, j, k + 1)] + sc) * w[h];
+        const double dsdxp = (s[HX(i + 1, j, k)] - sc) * dxi;
+        const double dsdxm = (sc - s[HX(i - 1, j, k)]) * dxi;
+        const double dsdyp = (s[HX(i, j + 1, k)] - sc) * dyi;
+        const double dsdym = (sc - s[HX(i, j - 1, k)]) * dyi;
+        const double dsdzp = (s[HX(i, j, k + 1)] - sc) * dzci[k];
+        const double dsdzm = (sc - s[HX(i, j, k - 1)]) * dzci[k - 1];
+        double r = dxi * (-usip + usim) + dyi * (-vsjp + vsjm) + dzfi[k] * (-wskp + wskm);
+        const double dxy = (dsdxp - dsdxm) * visc * dxi + (dsdyp - dsdym) * visc * dyi;
+        const double dz = (dsdzp - dsdzm) * visc * dzfi[k];
+        if (is_impdiff) {
+          if (is_impdiff_1d) {
+            r = r + dxy;
+            dsdtd[o] = dz;
+          } else {
+            dsdtd[o] = dxy + dz;
+          }
+        } else {
+          r = r + dxy + dz;
+        }
+        dsdt[o] = r;
+      }
+}
+
+/* src/rk.f90:335-491 (is_cmpt_wallflux is a .false. parameter there: no wall flux) */
+void orc_rk_scal(const double *rkpar, const int *n, const double *dli, const double *dzci,
+                 const double *dzfi, const double *grid_vol_ratio_f, double alpha, double dt,
+                 const double *u, const double *v, const double *w, int is_forced, double scalf,
+                 double ssource, double *dsdtrko, double *s, double *f, double *work,
+                 int is_impdiff, int is_impdiff_1d) {
+  DIMS
+  const size_t N = (size_t)n1 * n2 * n3;
+  const double factor1 = rkpar[0] * dt, factor2 = rkpar[1] * dt, factor12 = factor1 + factor2;
+  double *dsdtrk = work, *dsdtrkd = is_impdiff ? work + N : NULL;
+  orc_scal(n, dli[0], dli[1], dzci, dzfi, alpha, u, v, w, s, dsdtrk, dsdtrkd, is_impdiff, is_impdiff_1d);
+  #pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 1; k <= n3; k++)
+    for (int j = 1; j <= n2; j++)
+      for (int i = 1; i <= n1; i++) {
+        const size_t o = IX(i, j, k), h = HX(i, j, k);
+        s[h] = s[h] + factor1 * dsdtrk[o] + factor2 * dsdtrko[o] + factor12 * ssource;
+        if (is_impdiff) s[h] = s[h] + factor12 * dsdtrkd[o];
+      }
+  *f = 0.;
+  if (is_forced) *f = scalf - orc_bulk_mean(n, grid_vol_ratio_f, s);
+  if (is_impdiff) {
+    #pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 1; k <= n3; k++)
+      for (int j = 1; j <= n2; j++)
+        for (int i = 1; i <= n1; i++) {
+          const size_t o = IX(i, j, k), h = HX(i, j, k);
+          s[h] = s[h] - .5 * factor12 * dsdtrkd[o];
+        }
+  }
+  #pragma omp parallel for schedule(static)
+  for (size_t o = 0; o < N; o++) dsdtrko[o] = dsdtrk[o];
+}
+
+/* src/scal.f90:313-329 */
+void orc_bulk_forcing_s(const int *n, int is_forced, double ff, double *p) {
+  DIMS
+  if (!is_forced) return;
+  #pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 1; k <= n3; k++)
+    for (int j = 1; j <= n2; j++)
+      for (int i = 1; i <= n1; i++) p[HX(i, j, k)] = p[HX(i, j, k)] + ff;
 }
 
 /* ------------------------------------------------------------------------ */

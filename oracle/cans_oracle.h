@@ -73,14 +73,28 @@ void orc_mom_xyz_ad(const int *n, double dxi, double dyi, const double *dzci,
                     double *dudtd, double *dvdtd, double *dwdtd,
                     int is_impdiff, int is_impdiff_1d);
 /* src/rk.f90:24-333 (+ cmpt_bulk_forcing 493-518, bulk_forcing mom.f90:725-769);
+ * s = NULL: no buoyancy, else the Boussinesq term -gacc*beta*s of the first scalar;
  * work holds 3*n1*n2*n3 doubles (6* with implicit diffusion) */
 void orc_rk(const double *rkpar, const int *n, const double *dli, const double *dzci,
             const double *dzfi, const double *grid_vol_ratio_c,
             const double *grid_vol_ratio_f, double dt, double visc, const double *p,
             const int *is_forced, const double *velf, const double *bforce,
+            const double *gacc, double beta, const double *s,
             double *dudtrko, double *dvdtrko, double *dwdtrko,
             double *u, double *v, double *w, double *f, double *work,
             int is_impdiff, int is_impdiff_1d);
+/* src/scal.f90:41-98: dsdt (and dsdtd with implicit diffusion) of scalar s */
+void orc_scal(const int *n, double dxi, double dyi, const double *dzci, const double *dzfi,
+              double visc, const double *u, const double *v, const double *w, const double *s,
+              double *dsdt, double *dsdtd, int is_impdiff, int is_impdiff_1d);
+/* src/rk.f90:335-491; work holds n1*n2*n3 doubles (2* with implicit diffusion) */
+void orc_rk_scal(const double *rkpar, const int *n, const double *dli, const double *dzci,
+                 const double *dzfi, const double *grid_vol_ratio_f, double alpha, double dt,
+                 const double *u, const double *v, const double *w, int is_forced, double scalf,
+                 double ssource, double *dsdtrko, double *s, double *f, double *work,
+                 int is_impdiff, int is_impdiff_1d);
+/* src/scal.f90:313-329 */
+void orc_bulk_forcing_s(const int *n, int is_forced, double ff, double *p);
 /* src/utils.f90:15-42 */
 double orc_bulk_mean(const int *n, const double *grid_vol_ratio, const double *p);
 /* src/fillps.f90:13-51 */

@@ -62,7 +62,13 @@ def _declare(L):
     L.orc_mom_xyz_ad.argtypes = [_ip, C.c_double, C.c_double, _dp, _dp, C.c_double, _dp, _dp, _dp,
                                  _dp, _dp, _dp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int]
     L.orc_rk.argtypes = [_dp, _ip, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_double, _dp, _ip, _dp, _dp,
+                         _dp, C.c_double, C.c_void_p,
                          _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int, C.c_int]
+    L.orc_scal.argtypes = [_ip, C.c_double, C.c_double, _dp, _dp, C.c_double, _dp, _dp, _dp, _dp, _dp,
+                           C.c_void_p, C.c_int, C.c_int]
+    L.orc_rk_scal.argtypes = [_dp, _ip, _dp, _dp, _dp, _dp, C.c_double, C.c_double, _dp, _dp, _dp, C.c_int,
+                              C.c_double, C.c_double, _dp, _dp, _dp, _dp, C.c_int, C.c_int]
+    L.orc_bulk_forcing_s.argtypes = [_ip, C.c_int, C.c_double, _dp]
     L.orc_bulk_mean.argtypes = [_ip, _dp, _dp]
     L.orc_bulk_mean.restype = C.c_double
     L.orc_fillps.argtypes = [_ip, _dp, _dp, C.c_double, _dp, _dp, _dp, _dp]
@@ -181,7 +187,8 @@ class OracleCase:
 
     def __init__(self, ng, l, gtype=1, gr=0.0, cbcvel="PP" * 9, cbcpre="PP" * 3,
                  bcvel=None, bcpre=None, visci=1.0, bforce=(0, 0, 0), is_forced=(0, 0, 0),
-                 velf=(0, 0, 0), is_impdiff=False, is_impdiff_1d=False):
+                 velf=(0, 0, 0), is_impdiff=False, is_impdiff_1d=False, scalars=(), gacc=(0, 0, 0),
+                 beta=0.0, is_boussinesq_buoyancy=False):
         L = lib()
         self.ng = [int(x) for x in ng]
         self.n = list(self.ng)
@@ -213,6 +220,19 @@ class OracleCase:
         self.grid_vol_ratio_f = self.dl[0] * self.dl[1] * self.dzf / vol
         self.p_solver = self.initsolver(cbcpre, self.bcpre, "ccc")
         self.work = np.zeros(3 * int(np.prod(self.n)) * (2 if is_impdiff else 1))
+        # scalars (scal.f90:277-311): alphai, cbc (6 chars), bc (6), source, is_forced, scalf
+        self.scalars = []
+        for sc in scalars:
+            self.scalars.append(dict(alpha=float(sc.get("alphai", 1.0)) ** (-1), cbc=sc["cbc"],
+                                     bc=np.asarray(sc.get("bc", np.zeros(6)), dtype=np.float64),
+                                     source=float(sc.get("source", 0.0)), is_forced=int(bool(sc.get("is_forced", 0))),
+                                     scalf=float(sc.get("scalf", 0.0))))
+        self.gacc = [float(x) for x in gacc]
+        self.beta = float(beta)
+        self.is_buoyancy = bool(is_boussinesq_buoyancy) and len(self.scalars) > 0
+        # param.f90:308-310: the largest scalar diffusivity (1/huge without scalars)
+        self.alpha_max = (min(float(sc.get("alphai", 1.0)) for sc in scalars) ** (-1)) if len(scalars) else 0.0
+        self.swork = np.zeros(int(np.prod(self.n)) * (2 if is_impdiff else 1))
 
     # initsolver.f90:15-83 (single rank)
     def initsolver(self, cbc, bc, c_or_f):
@@ -312,19 +332,45 @@ class OracleCase:
         lib().orc_updatep(n, dli, P(self.dzci), P(self.dzfi), alpha, P(pp), P(p), self.is_impdiff,
                           self.is_impdiff_1d)
 
-    def rk(self, rkpar, dt, p, dudtrko, dvdtrko, dwdtrko, u, v, w):
+    def rk(self, rkpar, dt, p, dudtrko, dvdtrko, dwdtrko, u, v, w, s=None):
+        """rk.f90:24-333; s: the first scalar's field when buoyancy is on (rk.f90:69-72)."""
         n, _k = self._n()
         rk_, _a = DP(rkpar)
         dli, _b = DP(self.dli)
         isf, _c = IP(self.is_forced)
         velf, _d = DP(self.velf)
         bfo, _e = DP(self.bforce)
+        gacc, _g = DP(self.gacc)
         f = np.zeros(3)
+        sb = VP(s) if (s is not None and self.is_buoyancy) else None
         lib().orc_rk(rk_, n, dli, P(self.dzci), P(self.dzfi), P(self.grid_vol_ratio_c),
-                     P(self.grid_vol_ratio_f), dt, self.visc, P(p), isf, velf, bfo,
+                     P(self.grid_vol_ratio_f), dt, self.visc, P(p), isf, velf, bfo, gacc, self.beta, sb,
                      P(dudtrko), P(dvdtrko), P(dwdtrko), P(u), P(v), P(w), P(f), P(self.work),
                      self.is_impdiff, self.is_impdiff_1d)
         return f
+
+    def rk_scal(self, rkpar, dt, sc, u, v, w, s, dsdtrko):
+        """rk.f90:335-491 for scalar description sc; returns the forcing f."""
+        n, _k = self._n()
+        rk_, _a = DP(rkpar)
+        dli, _b = DP(self.dli)
+        f = C.c_double()
+        lib().orc_rk_scal(rk_, n, dli, P(self.dzci), P(self.dzfi), P(self.grid_vol_ratio_f), sc["alpha"], dt,
+                          P(u), P(v), P(w), sc["is_forced"], sc["scalf"], sc["source"], P(dsdtrko), P(s),
+                          C.byref(f), P(self.swork), self.is_impdiff, self.is_impdiff_1d)
+        return f.value
+
+    def scal(self, u, v, w, s, alpha):
+        """scal.f90:41-98 (explicit diffusion): dsdt as an (n1,n2,n3) array."""
+        n, _k = self._n()
+        dsdt = np.zeros(tuple(self.n), order="F")
+        lib().orc_scal(n, self.dli[0], self.dli[1], P(self.dzci), P(self.dzfi), alpha, P(u), P(v), P(w), P(s),
+                       P(dsdt), None, 0, 0)
+        return dsdt
+
+    def bulk_forcing_s(self, is_forced, ff, s):
+        n, _k = self._n()
+        lib().orc_bulk_forcing_s(n, int(is_forced), ff, P(s))
 
     def chkdiv(self, u, v, w):
         n, _k = self._n()
@@ -334,10 +380,11 @@ class OracleCase:
         lib().orc_chkdiv(n, l, dli, P(self.dzfi), P(u), P(v), P(w), C.byref(tot), C.byref(mx))
         return tot.value, mx.value
 
-    def chkdt(self, u, v, w, alpha=0.0):
+    def chkdt(self, u, v, w, alpha=None):
         n, _k = self._n()
         dl, _a = DP(self.dl)
-        # with no scalars alpha_max = 1/huge (param.f90:308-310), i.e. max(visc,alpha) = visc
+        # alpha_max (param.f90:308-310); with no scalars 1/huge, i.e. max(visc,alpha) = visc
+        alpha = self.alpha_max if alpha is None else alpha
         return lib().orc_chkdt(n, dl, P(self.dzci), P(self.dzfi), self.visc, alpha, P(u), P(v), P(w),
                                self.is_impdiff, self.is_impdiff_1d)
 
@@ -346,16 +393,22 @@ class OracleCase:
         g = self.grid_vol_ratio_f if ratio == "f" else self.grid_vol_ratio_c
         return lib().orc_bulk_mean(n, P(g), P(q))
 
-    # ---- main.f90:437-473 (no scalars, explicit diffusion) ----
+    # ---- main.f90:437-473 (explicit diffusion) ----
     def rk3_step(self, st, dt):
-        """One time step on a state dict with u,v,w,p,pp,dudtrko,dvdtrko,dwdtrko."""
+        """One time step on a state dict with u,v,w,p,pp,dudtrko,dvdtrko,dwdtrko (+ s<i>, dsdtrko<i>)."""
         u, v, w, p, pp = st["u"], st["v"], st["w"], st["p"], st["pp"]
         dpdl = np.zeros(3)
         for irk in range(3):
             rkpar = RKCOEFF[irk]
             dtrk = (rkpar[0] + rkpar[1]) * dt
             dtrki = dtrk ** (-1)
-            f = self.rk(rkpar, dt, p, st["dudtrko"], st["dvdtrko"], st["dwdtrko"], u, v, w)
+            for i, sc in enumerate(self.scalars):  # main.f90:440-450
+                s = st["s%d" % i]
+                fs = self.rk_scal(rkpar, dt, sc, u, v, w, s, st["dsdtrko%d" % i])
+                self.bulk_forcing_s(sc["is_forced"], fs, s)
+                self.boundp(s, cbc=sc["cbc"], bc=sc["bc"])
+            f = self.rk(rkpar, dt, p, st["dudtrko"], st["dvdtrko"], st["dwdtrko"], u, v, w,
+                        st.get("s0"))
             dpdl += f
             self.bounduvw(u, v, w, 0, -1)
             self.fillps(u, v, w, pp, dtrki)
@@ -373,9 +426,14 @@ class OracleCase:
         st = {k: field(n) for k in ("u", "v", "w", "p", "pp")}
         for k in ("dudtrko", "dvdtrko", "dwdtrko"):
             st[k] = np.zeros(tuple(n), order="F")
+        for i in range(len(self.scalars)):
+            st["s%d" % i] = field(n)
+            st["dsdtrko%d" % i] = np.zeros(tuple(n), order="F")
         return st
 
     def init_bounds(self, st):
-        """main.f90:389-390."""
+        """main.f90:389-390 (and the scalars' boundp of main.f90:391-395)."""
         self.bounduvw(st["u"], st["v"], st["w"])
         self.boundp(st["p"], 1)
+        for i, sc in enumerate(self.scalars):
+            self.boundp(st["s%d" % i], cbc=sc["cbc"], bc=sc["bc"])

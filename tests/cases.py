@@ -31,6 +31,18 @@ def params(name, ng=None):
     elif name == "cavity_dd":  # Dirichlet pressure walls in y (DST), Neumann in z
         p = dict(ng=(32, 32, 24), l=(1.0, 1.0, 1.0), gtype=1, gr=0.5, cbcvel="PPNNDD" * 3,
                  cbcpre="PPDDNN", visci=100.0)
+    elif name == "dhc":  # the reference's differentially heated cavity (tests/differentially_heated_cavity)
+        p = dict(ng=(128, 2, 128), l=(1.0, 0.015625, 1.0), gtype=1, gr=0.0, cbcvel="DDPPDD" * 3,
+                 cbcpre="NNPPNN", visci=1186.78165819,
+                 scalars=(dict(alphai=842.614977318, cbc="DDPPNN", bc=(-0.5, 0.5, 0.0, 0.0, 0.0, 0.0)),),
+                 gacc=(0.0, 0.0, -1.0), beta=1.0, is_boussinesq_buoyancy=True)
+    elif name == "rb_channel":  # heated channel: scalar walls in z (Dirichlet), buoyancy, two scalars, one forced
+        p = dict(ng=(32, 16, 24), l=(2.0, 1.0, 1.0), gtype=1, gr=1.2, cbcvel="PPPPDD" * 3,
+                 cbcpre="PPPPNN", visci=300.0, is_forced=(1, 0, 0), velf=(1.0, 0.0, 0.0),
+                 scalars=(dict(alphai=250.0, cbc="PPPPDD", bc=(0.0, 0.0, 0.0, 0.0, 1.0, -1.0), source=0.3),
+                          dict(alphai=400.0, cbc="PPPPNN", bc=(0.0, 0.0, 0.0, 0.0, 0.5, -0.25), is_forced=True,
+                               scalf=0.7)),
+                 gacc=(0.1, 0.0, -2.0), beta=0.8, is_boussinesq_buoyancy=True)
     else:
         raise KeyError(name)
     if ng is not None:
@@ -42,7 +54,9 @@ def oracle_case(p):
     from oracle import oracle as O
     return O.OracleCase(ng=p["ng"], l=p["l"], gtype=p["gtype"], gr=p["gr"], cbcvel=p["cbcvel"],
                         cbcpre=p["cbcpre"], bcvel=p.get("bcvel"), visci=p["visci"],
-                        is_forced=p.get("is_forced", (0, 0, 0)), velf=p.get("velf", (0, 0, 0)))
+                        is_forced=p.get("is_forced", (0, 0, 0)), velf=p.get("velf", (0, 0, 0)),
+                        scalars=p.get("scalars", ()), gacc=p.get("gacc", (0, 0, 0)), beta=p.get("beta", 0.0),
+                        is_boussinesq_buoyancy=p.get("is_boussinesq_buoyancy", False))
 
 
 def host_case(p):
@@ -50,7 +64,9 @@ def host_case(p):
     return Case(ng=p["ng"], l=p["l"], gtype=p["gtype"], gr=p["gr"], cbcvel=p["cbcvel"], cbcpre=p["cbcpre"],
                 bcvel=tuple(p.get("bcvel", (0.0,) * 18)), visci=p["visci"],
                 is_forced=tuple(bool(x) for x in p.get("is_forced", (0, 0, 0))),
-                velf=tuple(p.get("velf", (0.0, 0.0, 0.0))))
+                velf=tuple(p.get("velf", (0.0, 0.0, 0.0))), scalars=tuple(p.get("scalars", ())),
+                gacc=tuple(p.get("gacc", (0.0, 0.0, 0.0))), beta=float(p.get("beta", 0.0)),
+                is_boussinesq_buoyancy=bool(p.get("is_boussinesq_buoyancy", False)))
 
 
 def initial_state(name, p, grid, seed=2024):
@@ -74,3 +90,27 @@ def initial_state(name, p, grid, seed=2024):
         u, v, w = (np.asfortranarray(g.uniform(-0.1, 0.1, tuple(n))) for _ in range(3))
     pr = np.zeros(tuple(n), order="F")
     return u, v, w, pr
+
+
+def initial_scalars(name, p, grid, seed=2025):
+    """Interior fields of the case's scalars.  dhc: the linear profile between the
+    x walls (initflow.f90:327-337, with zero velocity); otherwise seeded noise."""
+    n = p["ng"]
+    out = []
+    g = IF.rng(seed)
+    for sc in p.get("scalars", ()):
+        if name == "dhc":
+            xx = (np.arange(1, n[0] + 1) - 0.5) * grid.dl[0] / grid.l[0]
+            prof = (1.0 - xx) * sc["bc"][0] + xx * sc["bc"][1]
+            out.append(np.asfortranarray(np.broadcast_to(prof[:, None, None], tuple(n)).copy()))
+        else:
+            out.append(np.asfortranarray(g.uniform(-0.5, 0.5, tuple(n))))
+    return out
+
+
+def dhc_nusselt(s_interior, dl):
+    """Nusselt number of tests/differentially_heated_cavity/test.py:7-14 from s(1,1,:):
+    tw = -0.5, l = 1, dx = 2*xp(1), dz = 2*zp(1) on the uniform grid."""
+    tw, l = -0.5, 1.0
+    dx, dz = dl[0], dl[2]
+    return float((1.0 / l) * np.sum(((s_interior[0, 0, :] - tw) / dx) * dz) * l / (-tw))
