@@ -22,6 +22,13 @@ _dp = C.POINTER(C.c_double)
 _cp = C.c_char_p
 
 
+class Helmholtz(C.Structure):
+    """cans_helmholtz (include/cans_hip.h)."""
+    _fields_ = [("plan", _vp), ("normfft", C.c_double), ("lambdaxy_d", _vp), ("a_d", _vp), ("b_d", _vp),
+                ("c_d", _vp), ("rhsbx_d", _vp), ("rhsby_d", _vp), ("rhsbz_d", _vp), ("cbc", C.c_char * 6),
+                ("c_or_f", C.c_char * 3)]
+
+
 class StepParams(C.Structure):
     """cans_step_params (include/cans_hip.h)."""
     _fields_ = [("cbcvel", C.c_char * 18), ("bcvel", C.c_double * 18), ("cbcpre", C.c_char * 6),
@@ -31,7 +38,7 @@ class StepParams(C.Structure):
                 ("dzc_h", _vp), ("dzf_h", _vp), ("dzci_d", _vp), ("dzfi_d", _vp),
                 ("grid_vol_ratio_c_d", _vp), ("grid_vol_ratio_f_d", _vp), ("plan", _vp),
                 ("normfft", C.c_double), ("lambdaxy_d", _vp), ("a_d", _vp), ("b_d", _vp), ("c_d", _vp),
-                ("rhsbx_d", _vp), ("rhsby_d", _vp), ("rhsbz_d", _vp),
+                ("rhsbx_d", _vp), ("rhsby_d", _vp), ("rhsbz_d", _vp), ("hvel", Helmholtz * 3),
                 ("nscal", C.c_int), ("scal", _vp), ("is_buoyancy", C.c_int), ("gacc", C.c_double * 3),
                 ("beta", C.c_double)]
 
@@ -39,7 +46,8 @@ class StepParams(C.Structure):
 class Scalar(C.Structure):
     """cans_scalar (include/cans_hip.h)."""
     _fields_ = [("val", _vp), ("dsdtrko", _vp), ("alpha", C.c_double), ("cbc", C.c_char * 6),
-                ("bc", C.c_double * 6), ("source", C.c_double), ("is_forced", C.c_int), ("scalf", C.c_double)]
+                ("bc", C.c_double * 6), ("source", C.c_double), ("is_forced", C.c_int), ("scalf", C.c_double),
+                ("h", Helmholtz)]
 
 
 class State(C.Structure):
@@ -60,6 +68,7 @@ SIGNATURES = {
     "cans_field_alloc_kind": (C.c_int, [_vp, C.c_int, C.POINTER(_vp)]),
     "cans_field_free": (C.c_int, [_vp, _vp]),
     "cans_field_register": (C.c_int, [_vp, _vp, C.c_int]),
+    "cans_ctx_set_numerics": (C.c_int, [_vp, C.c_int, C.c_int]),
     "cans_field_upload": (C.c_int, [_vp, _vp, _dp]),
     "cans_field_download": (C.c_int, [_vp, _vp, _dp]),
     "cans_device_alloc": (C.c_int, [_vp, C.c_long, C.POINTER(_vp)]),

@@ -157,7 +157,7 @@ int plan_boundp(cans_ctx* c, const char* cbc, const double* bc, const int* nb, c
 }
 
 int do_updt_rhs_b(cans_ctx* c, const char* c_or_f, const char* cbc, const int* is_bound, const double* rx,
-                  const double* ry, const double* rz, void* p, double norm) {
+                  const double* ry, const double* rz, void* p, double norm, int es = 0) {
   const Layout& L = c->L;
   const int n[3] = {L.n1, L.n2, L.n3};
   const double* r[3] = {rx, ry, rz};
@@ -166,10 +166,14 @@ int do_updt_rhs_b(cans_ctx* c, const char* c_or_f, const char* cbc, const int* i
     const int q = (c_or_f[d] == 'f' && cbc[1 + 2 * d] == 'D') ? 1 : 0;
     const long plane = (long)(d == 0 ? L.n2 : L.n1) * (d == 2 ? L.n2 : L.n3);
     int rc;
-    if (is_bound[2 * d] && (rc = CANS_DISPATCH(c, launch_rhs_b<T>(c, (T*)p, r[d], d, 1, norm)))) return rc;
-    if (is_bound[2 * d + 1] &&
-        (rc = CANS_DISPATCH(c, launch_rhs_b<T>(c, (T*)p, r[d] + plane, d, n[d] - q, norm))))
-      return rc;
+    // es: element size of p (0: the context's pressure-like type)
+    auto one = [&](const double* rr, int pl) -> int {
+      if (es == (int)sizeof(double)) return launch_rhs_b<double>(c, (double*)p, rr, d, pl, norm);
+      if (es == (int)sizeof(float)) return launch_rhs_b<float>(c, (float*)p, rr, d, pl, norm);
+      return CANS_DISPATCH(c, launch_rhs_b<T>(c, (T*)p, rr, d, pl, norm));
+    };
+    if (is_bound[2 * d] && (rc = one(r[d], 1))) return rc;
+    if (is_bound[2 * d + 1] && (rc = one(r[d] + plane, n[d] - q))) return rc;
   }
   return CANS_OK;
 }
@@ -218,6 +222,7 @@ static int rk_core(cans_ctx* c, const double* rkpar, const double* dli, const do
   m.rkpar[0] = rkpar[0]; m.rkpar[1] = rkpar[1];
   m.read_ro = 1;
   m.write_ro = write_ro;
+  m.imp = c->imp;
   for (int d = 0; d < 3; d++) {
     m.bforce[d] = bforce[d];
     m.velf[d] = velf[d];
@@ -251,6 +256,7 @@ static int scal_core(cans_ctx* c, const double* rkpar, const double* dli, const 
   m.rkpar[0] = rkpar[0]; m.rkpar[1] = rkpar[1];
   m.is_forced = is_forced ? 1 : 0;
   m.fslot = DSCAL_SCAL + slot;
+  m.imp = c->imp;
   if ((rc = launch_scal_rk(c, m))) return rc;
   return launch_add_forcing<double>(c, t[0], s, DSCAL_SCAL + slot, apply_forcing && is_forced ? 1 : 0);
 }
@@ -343,7 +349,9 @@ extern "C" int cans_updatep(cans_ctx* c, const int* n, const double* dli, const 
   CHECK_CTX("cans_updatep");
   CHECK_N("cans_updatep", n);
   if (!dli || !dzci || !dzfi || !pp || !p) return fail(CANS_EINVAL, "cans_updatep: null argument");
-  return CANS_DISPATCH(c, launch_updatep<T>(c, (const T*)pp, (T*)p, dzci, dzfi, alpha, dli, is_impdiff));
+  // updatep.f90:16-76: with is_impdiff_1d (context numerics) only the z part of the Laplacian
+  const int mode = is_impdiff ? (c->imp == 2 ? 2 : 1) : 0;
+  return CANS_DISPATCH(c, launch_updatep<T>(c, (const T*)pp, (T*)p, dzci, dzfi, alpha, dli, mode));
 }
 
 extern "C" int cans_bounduvw(cans_ctx* c, const char* cbc, const int* n, const double* bc, const int* nb,
@@ -390,25 +398,22 @@ extern "C" int cans_solve_helmholtz(cans_ctx* c, const int* n, const int* ng, co
                                     const int* is_bound, const char* cbc, const char* c_or_f, void* p) {
   CHECK_CTX("cans_solve_helmholtz");
   CHECK_N("cans_solve_helmholtz", n);
-  if (!plan || !lam || !a || !b || !cc || !is_bound || !cbc || !c_or_f || !p)
+  if (!a || !b || !cc || !is_bound || !cbc || !c_or_f || !p)
     return fail(CANS_EINVAL, "cans_solve_helmholtz: null argument");
+  // the field's element type: a registered field's own, else the context's pressure-like type
+  int es = field_elem_size(c, p);
+  if (!es) es = c->precision == CANS_MIXED ? (int)sizeof(float) : (int)sizeof(double);
   // solve_helmholtz.f90:58-73: rhs boundary terms scaled by alpha, bb = b + 1/alpha, norm/alpha
-  int rc = do_updt_rhs_b(c, c_or_f, cbc, is_bound, rx, ry, rz, p, alpha);
+  int rc = do_updt_rhs_b(c, c_or_f, cbc, is_bound, rx, ry, rz, p, alpha, es);
   if (rc) return rc;
-  const double alphai = 1. / alpha;
-  void* bbv = nullptr;
-  rc = ensure_work(c, 3, 2 * (c->L.n3 + 2) * sizeof(double) + (size_t)c->L.n3 * sizeof(double), &bbv);
-  if (rc) return rc;
-  double* bb = (double*)bbv + 2 * (c->L.n3 + 2);
-  std::vector<double> hb(c->L.n3);
-  CANS_HIP_TRY(hipMemcpyAsync(hb.data(), b, hb.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  CANS_HIP_TRY(hipStreamSynchronize(c->stream));
-  for (auto& x : hb) x = x + alphai;
-  CANS_HIP_TRY(hipMemcpyAsync(bb, hb.data(), hb.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  rc = CANS_DISPATCH(c, run_solver<T>(c, plan, normfft * alphai, lam, a, bb, cc, cbc, c_or_f, (T*)p));
-  if (rc) return rc;
-  CANS_HIP_TRY(hipStreamSynchronize(c->stream));
-  return CANS_OK;
+  if (c->imp == 2) {  // is_impdiff_1d: solver_gaussel_z
+    if (es != (int)sizeof(double)) return fail(CANS_EUNSUPPORTED, "cans_solve_helmholtz: z-only form needs an FP64 field");
+    return run_helmholtz_z(c, a, b, cc, alpha, cbc + 4, c_or_f[2], (double*)p);
+  }
+  if (!plan || !lam) return fail(CANS_EINVAL, "cans_solve_helmholtz: null plan or eigenvalues");
+  if (es == (int)sizeof(double))
+    return run_helmholtz_3d<double>(c, plan, normfft, alpha, lam, a, b, cc, cbc, c_or_f, (double*)p);
+  return run_helmholtz_3d<float>(c, plan, normfft, alpha, lam, a, b, cc, cbc, c_or_f, (float*)p);
 }
 
 extern "C" int cans_chkdiv(cans_ctx* c, const int* lo, const int* hi, const double* l, const double* dli,
@@ -457,6 +462,19 @@ extern "C" int cans_bulk_mean(cans_ctx* c, const int* n, const double* gvr, cons
   return run_bulk_mean<double>(c, (const double*)p, gvr, mean);
 }
 
+// solve_helmholtz (solve_helmholtz.f90:28-75) of one FP64 field with the system h
+static int helmholtz(cans_ctx* c, const cans_helmholtz& h, const int* is_bound, double alpha, double* f) {
+  const bool z_only = c->imp == 2;
+  // the z-only form has no x/y boundary planes (main.f90:317-329 deallocates them)
+  int rc = do_updt_rhs_b(c, h.c_or_f, h.cbc, is_bound, z_only ? nullptr : h.rhsbx_d, z_only ? nullptr : h.rhsby_d,
+                         h.rhsbz_d, f, alpha, (int)sizeof(double));
+  if (rc) return rc;
+  if (z_only) return run_helmholtz_z(c, h.a_d, h.b_d, h.c_d, alpha, h.cbc + 4, h.c_or_f[2], f);
+  if (!h.plan || !h.lambdaxy_d) return fail(CANS_EINVAL, "cans_rk3_substep: Helmholtz system without a plan");
+  return run_helmholtz_3d<double>(c, h.plan, h.normfft, alpha, h.lambdaxy_d, h.a_d, h.b_d, h.c_d, h.cbc, h.c_or_f,
+                                  f);
+}
+
 // One substep.  With S = double (FP64) the updated velocity of rk lives in double
 // scratch fields t, as in the reference.  With S = float (mixed) the scratch holds
 // its increment e over (u,v,w) in FP32 instead: e is ~dt times the right-hand side,
@@ -489,12 +507,16 @@ static int rk3_substep(cans_ctx* c, const cans_step_params* s, int irk, double d
   void* divu_v = nullptr;
   if (delta && fuse_div && (rc = ensure_work(c, 4, (size_t)c->L.cells() * sizeof(S), &divu_v))) return rc;
   S* divu = (S*)divu_v;
-  // scalars first, from the old velocity (main.f90:440-450): rk_scal, bulk_forcing_s, boundp
+  if (c->imp && delta)
+    return fail(CANS_EUNSUPPORTED, "cans_rk3_substep: implicit diffusion needs a CANS_FP64 context");
+  // scalars first, from the old velocity (main.f90:440-450): rk_scal, bulk_forcing_s,
+  // [solve_helmholtz], boundp
   for (int i = 0; i < s->nscal; i++) {
     const cans_scalar& sc = s->scal[i];
     if ((rc = scal_core(c, rkpar, s->dli, s->dzci_d, s->dzfi_d, s->grid_vol_ratio_f_d, sc.alpha, dt, u, v, w,
                         sc.is_forced, sc.scalf, sc.source, (double*)sc.dsdtrko, (double*)sc.val, i, 1)))
       return rc;
+    if (c->imp && (rc = helmholtz(c, sc.h, s->is_bound, -0.5 * sc.alpha * dtrk, (double*)sc.val))) return rc;
     if ((rc = plan_boundp(c, sc.cbc, sc.bc, s->nb, s->is_bound, s->dl, s->dzc_h, sc.val, CANS_HALO_ABSENT,
                           (int)sizeof(double))))
       return rc;
@@ -507,6 +529,9 @@ static int rk3_substep(cans_ctx* c, const cans_step_params* s, int irk, double d
   if (rc) return rc;
   for (int k = 0; k < 3 && !delta; k++)
     if (s->is_forced[k] && (rc = launch_add_forcing<double>(c, t[k], t[k], k, 1))) return rc;
+  // implicit diffusion (main.f90:453-462): one Helmholtz solve per component, alpha = -visc*dtrk/2
+  for (int k = 0; k < 3 && c->imp; k++)
+    if ((rc = helmholtz(c, s->hvel[k], s->is_bound, -.5 * s->visc * dtrk, t[k]))) return rc;
   if (delta)
     rc = plan_bounduvw(c, s->cbcvel, s->bcvel, s->nb, s->is_bound, s->dl, s->dzc_h, s->dzf_h, e[0], e[1], e[2], 0,
                        CANS_HALO_LOWER, CANS_HALO_ABSENT, base, fvec);
@@ -537,7 +562,12 @@ static int rk3_substep(cans_ctx* c, const cans_step_params* s, int irk, double d
   if ((rc = plan_boundp(c, s->cbcpre, s->bcpre, s->nb, s->is_bound, s->dl, s->dzc_h, pp, CANS_HALO_ABSENT))) return rc;
   // correc reads the intermediate velocity and writes the caller's fields (whole halo'd
   // box); updatep (p += pp) is independent of the velocity halos and rides along
-  if ((rc = launch_correc<S>(c, pp, su, sv, sw, eu, ev, ew, fvec, u, v, w, s->dzci_d, dtrk, s->dli, p))) return rc;
+  // with implicit diffusion updatep also adds alpha*lap(pp) (updatep.f90:16-33): its own pass
+  if ((rc = launch_correc<S>(c, pp, su, sv, sw, eu, ev, ew, fvec, u, v, w, s->dzci_d, dtrk, s->dli,
+                             c->imp ? nullptr : p)))
+    return rc;
+  if (c->imp && (rc = launch_updatep<S>(c, pp, p, s->dzci_d, s->dzfi_d, -.5 * s->visc * dtrk, s->dli, c->imp)))
+    return rc;
   if ((rc = plan_bounduvw(c, s->cbcvel, s->bcvel, s->nb, s->is_bound, s->dl, s->dzc_h, s->dzf_h, u, v, w, 1,
                           CANS_HALO_UPPER, CANS_HALO_NONE)))
     return rc;

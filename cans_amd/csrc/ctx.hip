@@ -168,6 +168,13 @@ extern "C" int cans_field_free(cans_ctx* c, void* field) {
   return CANS_OK;
 }
 
+extern "C" int cans_ctx_set_numerics(cans_ctx* c, int is_impdiff, int is_impdiff_1d) {
+  if (!c) return fail(CANS_EINVAL, "cans_ctx_set_numerics: null context");
+  // param.f90:331: is_impdiff_1d implies is_impdiff
+  c->imp = is_impdiff_1d ? 2 : is_impdiff ? 1 : 0;
+  return CANS_OK;
+}
+
 extern "C" int cans_field_register(cans_ctx* c, void* field, int kind) {
   if (!c || !field) return fail(CANS_EINVAL, "cans_field_register: null argument");
   if (kind != CANS_FIELD_VELOCITY && kind != CANS_FIELD_PRESSURE)

@@ -37,6 +37,7 @@ struct MomLaunch {
   int read_ro = 1, write_ro = 1;  // RHS_old input/output needed (see launch_mom_rk)
   const double* sb = nullptr;     // Boussinesq buoyancy: the first scalar (halo'd), or none
   double gacc[3] = {0., 0., 0.}, beta = 0.;
+  int imp = 0;                    // 0 explicit, 1 implicit diffusion, 2 implicit in z only
 };
 
 // scalar transport + RK update of one scalar (rk.f90:335-491, explicit diffusion);
@@ -50,8 +51,16 @@ struct ScalLaunch {
   double rkpar[2];
   int is_forced;
   int fslot;
+  int imp = 0;  // 0 explicit, 1 implicit diffusion, 2 implicit in z only
 };
 int launch_scal_rk(cans_ctx* c, const ScalLaunch& m);
+// implicit-diffusion Helmholtz solves of an FP64 field (solver.hip)
+int run_helmholtz_z(cans_ctx* c, const double* a, const double* b, const double* cc, double alpha, const char* bcz,
+                    char c_or_f_z, double* fld);
+template <typename T>
+int run_helmholtz_3d(cans_ctx* c, const cans_plan* p, double normfft, double alpha, const double* lam,
+                     const double* a, const double* b, const double* cc, const char* cbc, const char* c_or_f,
+                     T* fld);
 int launch_fill(cans_ctx* c, double* p, int n, double v);  // p[0..n) = v, on the context stream
 int field_elem_size(const cans_ctx* c, const void* field);
 

@@ -984,6 +984,95 @@ int run_solver(cans_ctx* c, const cans_plan* p, double normfft, const double* la
   return CANS_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Helmholtz solves of the implicit diffusion (solve_helmholtz.f90:28-75) on an
+// FP64 field: bb = b + 1/alpha, then the 3-D solver with normfft/alpha, or the
+// z-only elimination of solver_gaussel_z (solver.f90:547-616).  The z-only system
+// has no eigenvalue, so gaussel's lambda-absent branch computes z(k) = 1/(bb(k) -
+// a(k)*d(k-1)) and d(k) = c(k)*z(k) once for all lines (solver.f90:170-185): one
+// thread builds those tables, then one thread per (i,j) line sweeps the field in
+// place (rhs -> modified rhs on the way up, solution on the way down).
+// ---------------------------------------------------------------------------
+__global__ void k_shift_diag(const double* b, double alphai, double* bb, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) bb[k] = b[k] + alphai;
+}
+
+__global__ void k_z1_tables(const double* a, const double* bb, const double* c, int nn, double* zt, double* dt) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double z = 1. / bb[0];
+  zt[0] = z;
+  dt[0] = c[0] * z;
+  for (int k = 2; k <= nn; k++) {
+    z = 1. / (bb[k - 1] - a[k - 1] * dt[k - 2]);
+    zt[k - 1] = z;
+    dt[k - 1] = c[k - 1] * z;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_z1_sweep(Layout L, double* p, const double* a, const double* zt,
+                                                  const double* dt, int nn, double norm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y + 1;
+  if (i > L.n1) return;
+  const kcoef ka = (kcoef)a, kz = (kcoef)zt, kd = (kcoef)dt;
+  double* P = p + L.at(i, j, 1);  // row k at P + (k-1)*sz
+  const long sz = L.sz;
+  double prev = P[0] * norm * kz[0];
+  P[0] = prev;
+  for (int k = 2; k <= nn; k++) {
+    double* q = P + (long)(k - 1) * sz;
+    prev = (*q * norm - ka[k - 1] * prev) * kz[k - 1];
+    *q = prev;
+  }
+  for (int k = nn - 1; k >= 1; k--) {
+    double* q = P + (long)(k - 1) * sz;
+    prev = *q - kd[k - 1] * prev;
+    *q = prev;
+  }
+}
+
+int run_helmholtz_z(cans_ctx* c, const double* a, const double* b, const double* cc, double alpha, const char* bcz,
+                    char c_or_f_z, double* fld) {
+  const Layout& L = c->L;
+  if (bcz[0] == 'P' && bcz[1] == 'P')
+    return fail(CANS_EUNSUPPORTED, "implicit z diffusion with periodic z is not built on the device");
+  const int q = (c_or_f_z == 'f' && bcz[1] == 'D') ? 1 : 0;
+  const int nn = L.n3 - q;
+  void* tab = nullptr;
+  int rc = ensure_work(c, 5, (size_t)3 * L.n3 * sizeof(double), &tab);
+  if (rc) return rc;
+  double* bb = (double*)tab;
+  double* zt = bb + L.n3;
+  double* dt = zt + L.n3;
+  k_shift_diag<<<cdiv(L.n3, 256), 256, 0, c->stream>>>(b, 1. / alpha, bb, L.n3);
+  CANS_LAUNCH_CHECK("k_shift_diag");
+  k_z1_tables<<<1, 64, 0, c->stream>>>(a, bb, cc, nn, zt, dt);
+  CANS_LAUNCH_CHECK("k_z1_tables");
+  k_z1_sweep<<<dim3(cdiv(L.n1, 256), L.n2), 256, 0, c->stream>>>(L, fld, a, zt, dt, nn, 1. / alpha);
+  CANS_LAUNCH_CHECK("k_z1_sweep");
+  return CANS_OK;
+}
+
+template <typename T>
+int run_helmholtz_3d(cans_ctx* c, const cans_plan* p, double normfft, double alpha, const double* lam,
+                     const double* a, const double* b, const double* cc, const char* cbc, const char* c_or_f,
+                     T* fld) {
+  const Layout& L = c->L;
+  void* tab = nullptr;
+  int rc = ensure_work(c, 5, (size_t)3 * L.n3 * sizeof(double), &tab);
+  if (rc) return rc;
+  double* bb = (double*)tab;
+  const double alphai = 1. / alpha;
+  k_shift_diag<<<cdiv(L.n3, 256), 256, 0, c->stream>>>(b, alphai, bb, L.n3);
+  CANS_LAUNCH_CHECK("k_shift_diag");
+  return run_solver<T>(c, p, normfft * alphai, lam, a, bb, cc, cbc, c_or_f, fld);
+}
+template int run_helmholtz_3d<double>(cans_ctx*, const cans_plan*, double, double, const double*, const double*,
+                                      const double*, const double*, const char*, const char*, double*);
+template int run_helmholtz_3d<float>(cans_ctx*, const cans_plan*, double, double, const double*, const double*,
+                                     const double*, const double*, const char*, const char*, float*);
+
 template int run_solver<double>(cans_ctx*, const cans_plan*, double, const double*, const double*, const double*,
                                 const double*, const char*, const char*, double*, const DivSrc<double>*);
 template int run_solver<float>(cans_ctx*, const cans_plan*, double, const double*, const double*, const double*,

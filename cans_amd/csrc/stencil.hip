@@ -48,6 +48,8 @@ struct MomArgs {
   double* partials;              // [3][nblocks]
   const double* sb;              // nullptr, or the buoyancy scalar (halo'd)
   double bq[3];                  // factor12*gacc(d)*beta*0.5
+  int imp;                       // 0 explicit, 1 implicit diffusion, 2 implicit in z only
+  double hf12;                   // 0.5*factor12
 };
 
 // Tiling of the momentum kernel: a work-group owns a TX x TY column of cells and
@@ -137,8 +139,13 @@ __global__ void __launch_bounds__(MOM_TX* MOM_TY) k_mom_rk(MomArgs<S> a) {
       const double vujp = 0.25 * (v_ccc + v_pcc) * u_cpc, vujm = 0.25 * (v_cmc + v_pmc) * u_cmc;
       const double wukp = 0.25 * (w_ccc + w_pcc) * u_ccp, wukm = 0.25 * (w_ccm + w_pcm) * u_ccm;
       double dudt = -(uuip - uuim) * dxi - (vujp - vujm) * dyi - (wukp - wukm) * dzfi_k;
-      dudt = dudt + (visc * (dudxp - dudxm) * dxi + visc * (dudyp - dudym) * dyi) +
-             visc * (dudzp - dudzm) * dzfi_k;
+      // diffusion (mom.f90:990-1016): explicit, or split off for the implicit solve
+      const double dudxy = visc * (dudxp - dudxm) * dxi + visc * (dudyp - dudym) * dyi;
+      const double dudz = visc * (dudzp - dudzm) * dzfi_k;
+      double dudtd = 0.;
+      if (a.imp == 0) dudt = dudt + dudxy + dudz;
+      else if (a.imp == 2) dudt = dudt + dudxy, dudtd = dudz;
+      else dudtd = dudxy + dudz;
       // y
       const double dvdxp = (v_pcc - v_ccc) * dxi, dvdxm = (v_ccc - v_mcc) * dxi;
       const double dvdyp = (v_cpc - v_ccc) * dyi, dvdym = (v_ccc - v_cmc) * dyi;
@@ -147,8 +154,12 @@ __global__ void __launch_bounds__(MOM_TX* MOM_TY) k_mom_rk(MomArgs<S> a) {
       const double vvjp = 0.25 * (v_ccc + v_cpc) * v_cpc, vvjm = 0.25 * (v_ccc + v_cmc) * v_cmc;
       const double wvkp = 0.25 * (w_ccc + w_cpc) * v_ccp, wvkm = 0.25 * (w_ccm + w_cpm) * v_ccm;
       double dvdt = -(uvip - uvim) * dxi - (vvjp - vvjm) * dyi - (wvkp - wvkm) * dzfi_k;
-      dvdt = dvdt + (visc * (dvdxp - dvdxm) * dxi + visc * (dvdyp - dvdym) * dyi) +
-             visc * (dvdzp - dvdzm) * dzfi_k;
+      const double dvdxy = visc * (dvdxp - dvdxm) * dxi + visc * (dvdyp - dvdym) * dyi;
+      const double dvdz = visc * (dvdzp - dvdzm) * dzfi_k;
+      double dvdtd = 0.;
+      if (a.imp == 0) dvdt = dvdt + dvdxy + dvdz;
+      else if (a.imp == 2) dvdt = dvdt + dvdxy, dvdtd = dvdz;
+      else dvdtd = dvdxy + dvdz;
       // z
       const double dwdxp = (w_pcc - w_ccc) * dxi, dwdxm = (w_ccc - w_mcc) * dxi;
       const double dwdyp = (w_cpc - w_ccc) * dyi, dwdym = (w_ccc - w_cmc) * dyi;
@@ -159,8 +170,12 @@ __global__ void __launch_bounds__(MOM_TX* MOM_TY) k_mom_rk(MomArgs<S> a) {
       const double vwjm = 0.25 * (rdzm * v_cmc + rdzp * v_cmp) * w_cmc;
       const double wwkp = 0.25 * (w_ccc + w_ccp) * w_ccp, wwkm = 0.25 * (w_ccc + w_ccm) * w_ccm;
       double dwdt = -(uwip - uwim) * dxi - (vwjp - vwjm) * dyi - (wwkp - wwkm) * dzci_k;
-      dwdt = dwdt + (visc * (dwdxp - dwdxm) * dxi + visc * (dwdyp - dwdym) * dyi) +
-             visc * (dwdzp - dwdzm) * dzci_k;
+      const double dwdxy = visc * (dwdxp - dwdxm) * dxi + visc * (dwdyp - dwdym) * dyi;
+      const double dwdz = visc * (dwdzp - dwdzm) * dzci_k;
+      double dwdtd = 0.;
+      if (a.imp == 0) dwdt = dwdt + dwdxy + dwdz;
+      else if (a.imp == 2) dwdt = dwdt + dwdxy, dwdtd = dwdz;
+      else dwdtd = dwdxy + dwdz;
       // RK update (rk.f90:156-186) and RHS_old <- RHS (rk.f90:266-276).  With rkpar(2) = 0
       // (first substep) the RHS_old term is factor2*RHS_old = 0 and is not read.
       const long o = (long)(i - 1) + (long)L.n1 * ((j - 1) + (long)L.n2 * (k - 1));
@@ -182,14 +197,25 @@ __global__ void __launch_bounds__(MOM_TX* MOM_TY) k_mom_rk(MomArgs<S> a) {
         vn = vn - a.bq[1] * (a.sb[c + sy] + s_c);
         wn = wn - a.bq[2] * (a.sb[c + sz] + s_c);
       }
+      // implicit diffusion (rk.f90:180-184, 312-326): the update takes + factor12*dudtd; the
+      // bulk mean is of that velocity; the stored value is the Helmholtz rhs u - 0.5*factor12*dudtd
+      double uout = un, vout = vn, wout = wn;
+      if (a.imp) {
+        un = un + a.factor12 * dudtd;
+        vn = vn + a.factor12 * dvdtd;
+        wn = wn + a.factor12 * dwdtd;
+        uout = un - a.hf12 * dudtd;
+        vout = vn - a.hf12 * dvdtd;
+        wout = wn - a.hf12 * dwdtd;
+      }
       if (a.delta) {
-        a.eu[c] = (S)(un - u_ccc);
-        a.ev[c] = (S)(vn - v_ccc);
-        a.ew[c] = (S)(wn - w_ccc);
+        a.eu[c] = (S)(uout - u_ccc);
+        a.ev[c] = (S)(vout - v_ccc);
+        a.ew[c] = (S)(wout - w_ccc);
       } else {
-        a.uo[c] = un;
-        a.vo[c] = vn;
-        a.wo[c] = wn;
+        a.uo[c] = uout;
+        a.vo[c] = vout;
+        a.wo[c] = wout;
       }
       if (a.write_ro) {
         a.ru[o] = (S)dudt;
@@ -314,6 +340,8 @@ struct ScalArgs {
   double dxi, dyi, alpha, factor1, factor2, factor12, source;
   int read_ro;
   double* partials;           // nullptr: no bulk mean
+  int imp;                    // 0 explicit, 1 implicit diffusion, 2 implicit in z only
+  double hf12;                // 0.5*factor12
 };
 
 __global__ void __launch_bounds__(256) k_scal_rk(ScalArgs a) {
@@ -339,11 +367,16 @@ __global__ void __launch_bounds__(256) k_scal_rk(ScalArgs a) {
     double dsdt = dxi * (-usip + usim) + dyi * (-vsjp + vsjm) + dzfi_k * (-wskp + wskm);
     const double dxy = (dsdxp - dsdxm) * alpha * dxi + (dsdyp - dsdym) * alpha * dyi;
     const double dz = (dsdzp - dsdzm) * alpha * dzfi_k;
-    dsdt = dsdt + dxy + dz;
+    double dsdtd = 0.;  // scal.f90:86-97: the implicit part, if any
+    if (a.imp == 0) dsdt = dsdt + dxy + dz;
+    else if (a.imp == 2) dsdt = dsdt + dxy, dsdtd = dz;
+    else dsdtd = dxy + dz;
     const long o = (long)(i - 1) + (long)L.n1 * ((j - 1) + (long)L.n2 * (k - 1));
-    const double sn = a.read_ro ? sc + a.factor1 * dsdt + a.factor2 * a.ro[o] + a.factor12 * a.source
-                                : sc + a.factor1 * dsdt + a.factor12 * a.source;
-    a.so[c] = sn;
+    double sn = a.read_ro ? sc + a.factor1 * dsdt + a.factor2 * a.ro[o] + a.factor12 * a.source
+                          : sc + a.factor1 * dsdt + a.factor12 * a.source;
+    if (a.imp) sn = sn + a.factor12 * dsdtd;  // rk.f90:437-440
+    // the bulk mean is of sn; with implicit diffusion the stored value is the Helmholtz rhs (rk.f90:476-486)
+    a.so[c] = a.imp ? sn - a.hf12 * dsdtd : sn;
     a.ro[o] = dsdt;
     acc = sn * a.gvr_f[k];
   }
@@ -459,7 +492,8 @@ __global__ void k_updatep(Layout L, const T* pp, T* p, const double* dzci, const
   const double qxp = pp[c + 1], qxm = pp[c - 1], qyp = pp[c + L.sy], qym = pp[c - L.sy];
   const double qzp = pp[c + L.sz], qzm = pp[c - L.sz];
   double lap = 0.;
-  lap = lap + (qxp - 2. * q + qxm) * dxi2 + (qyp - 2. * q + qym) * dyi2;
+  if (impdiff != 2)  // is_impdiff_1d: the z part only
+    lap = lap + (qxp - 2. * q + qxm) * dxi2 + (qyp - 2. * q + qym) * dyi2;
   lap = lap + ((qzp - q) * dzci[k] - (q - qzm) * dzci[k - 1]) * dzfi[k];
   p[c] = (T)((double)p[c] + q + alpha * lap);
 }
@@ -653,6 +687,8 @@ int launch_mom_rk(cans_ctx* c, const MomLaunch<S>& m) {
   // the buoyancy coefficient in the reference's left-to-right order: factor12*gacc*beta*0.5
   a.sb = m.sb;
   for (int d = 0; d < 3; d++) a.bq[d] = a.factor12 * m.gacc[d] * m.beta * 0.5;
+  a.imp = m.imp;
+  a.hf12 = 0.5 * a.factor12;
   k_mom_rk<S><<<grid, dim3(MOM_TX, MOM_TY), 0, c->stream>>>(a);
   CANS_LAUNCH_CHECK("k_mom_rk");
   if (a.sum_mask)
@@ -689,6 +725,8 @@ int launch_scal_rk(cans_ctx* c, const ScalLaunch& m) {
   a.factor12 = a.factor1 + a.factor2;
   a.source = m.source;
   a.read_ro = m.rkpar[1] != 0.;
+  a.imp = m.imp;
+  a.hf12 = 0.5 * a.factor12;
   const dim3 grid = row_grid(L, 256);
   const long nb = (long)grid.x * grid.y * grid.z;
   int rc;

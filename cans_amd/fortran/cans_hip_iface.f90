@@ -16,6 +16,15 @@ module cans_hip
   !
   ! cans_step_params (the argument bundle of one fused RK3 substep)
   !
+  ! one implicit-diffusion system (what initsolver returns for a variable)
+  type, bind(C) :: cans_helmholtz
+    type(c_ptr)            :: plan = c_null_ptr
+    real(c_double)         :: normfft = 1.
+    type(c_ptr)            :: lambdaxy_d = c_null_ptr, a_d = c_null_ptr, b_d = c_null_ptr, c_d = c_null_ptr
+    type(c_ptr)            :: rhsbx_d = c_null_ptr, rhsby_d = c_null_ptr, rhsbz_d = c_null_ptr
+    character(kind=c_char) :: cbc(6) = 'P'
+    character(kind=c_char) :: c_or_f(3) = 'c'
+  end type cans_helmholtz
   type, bind(C) :: cans_step_params
     character(kind=c_char) :: cbcvel(18)
     real(c_double)         :: bcvel(18)
@@ -29,6 +38,7 @@ module cans_hip
     type(c_ptr)            :: dzc_h, dzf_h, dzci_d, dzfi_d, grid_vol_ratio_c_d, grid_vol_ratio_f_d, plan
     real(c_double)         :: normfft
     type(c_ptr)            :: lambdaxy_d, a_d, b_d, c_d, rhsbx_d, rhsby_d, rhsbz_d
+    type(cans_helmholtz)   :: hvel(3)
     integer(c_int)         :: nscal = 0
     type(c_ptr)            :: scal = c_null_ptr      ! nscal entries of type(cans_scalar)
     integer(c_int)         :: is_buoyancy = 0
@@ -43,6 +53,7 @@ module cans_hip
     real(c_double)         :: source
     integer(c_int)         :: is_forced
     real(c_double)         :: scalf
+    type(cans_helmholtz)   :: h
   end type cans_scalar
   type, bind(C) :: cans_state
     type(c_ptr) :: u, v, w, p, pp, dudtrko, dvdtrko, dwdtrko
@@ -88,6 +99,12 @@ module cans_hip
     integer(c_int) function cans_field_free(ctx, field) bind(C, name='cans_field_free')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, field
+    end function
+    integer(c_int) function cans_ctx_set_numerics(ctx, is_impdiff, is_impdiff_1d) &
+                   bind(C, name='cans_ctx_set_numerics')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: is_impdiff, is_impdiff_1d
     end function
     integer(c_int) function cans_field_register(ctx, field, kind) bind(C, name='cans_field_register')
       import :: c_int, c_ptr

@@ -44,6 +44,9 @@ class Case:
     gacc: tuple = (0.0, 0.0, 0.0)
     beta: float = 0.0
     is_boussinesq_buoyancy: bool = False
+    # &numerics (src/param.f90:96,331): implicit diffusion, in 3-D or in z only
+    is_impdiff: bool = False
+    is_impdiff_1d: bool = False
 
     @property
     def n(self):
@@ -208,18 +211,20 @@ class DeviceField:
 class Plan:
     """Transforms + solver arrays of one variable (src/initsolver.f90:15-83)."""
 
-    def __init__(self, ctx: Context, grid: Grid, cbc: str, bc, c_or_f="ccc"):
+    def __init__(self, ctx: Context, grid: Grid, cbc: str, bc, c_or_f="ccc", transforms=True):
         self.grid = grid
         self.ctx = ctx
+        self.transforms = transforms  # False: z-only systems (is_impdiff_1d), no x/y plans
         ng = grid.n
         lx = eigenvalues(ng[0], cbc[0:2], c_or_f[0]) * grid.dli[0] ** 2
         ly = eigenvalues(ng[1], cbc[2:4], c_or_f[1]) * grid.dli[1] ** 2
         self.lam_h = np.asfortranarray(lx[:, None] + ly[None, :])  # lambdaxy(i,j), device ordering
         self.a_h, self.b_h, self.c_h = tridmatrix(cbc[4:6], ng[2], grid.dzci, grid.dzfi, c_or_f[2])
         h = C.c_void_p()
-        nf = C.c_double()
-        LB.check(LB.lib().cans_fftini(ctx.h, LB.ivec(ng), cbc[0:4].encode(), c_or_f[0:2].encode(), C.byref(h),
-                                      C.byref(nf)), "cans_fftini")
+        nf = C.c_double(1.0)
+        if transforms:
+            LB.check(LB.lib().cans_fftini(ctx.h, LB.ivec(ng), cbc[0:4].encode(), c_or_f[0:2].encode(), C.byref(h),
+                                          C.byref(nf)), "cans_fftini")
         self.h = h
         self.normfft = nf.value
         self.cbc = cbc
@@ -247,14 +252,28 @@ class Plan:
             LB.lib().cans_fftend(self.h)
             self.h = None
 
+    def rhsb_ptrs(self):
+        """Boundary rhs planes; the z-only form has no x/y ones (main.f90:317-329)."""
+        ptrs = [t.data_ptr() for t in self.rhsb]
+        return ptrs if self.transforms else [None, None, ptrs[2]]
+
     def solve_helmholtz(self, f: DeviceField, alpha):
         """solve_helmholtz.f90:28-75 on f (the rhs in, the solution out)."""
         n = LB.ivec(self.ctx.n)
         LB.check(LB.lib().cans_solve_helmholtz(self.ctx.h, n, n, self.h, self.normfft, alpha,
                                                *[C.c_void_p(t.data_ptr()) for t in (self.lam, self.a, self.b, self.c)],
-                                               *[C.c_void_p(t.data_ptr()) for t in self.rhsb],
+                                               *[C.c_void_p(x) for x in self.rhsb_ptrs()],
                                                LB.ivec(self.grid.is_bound), self.cbc.encode(), self.c_or_f.encode(),
                                                C.c_void_p(f.ptr)), "cans_solve_helmholtz")
+
+    def fill(self, h):
+        """Fill a cans_helmholtz struct with this system."""
+        h.plan = self.h.value if self.h else None
+        h.normfft = self.normfft
+        h.lambdaxy_d, h.a_d, h.b_d, h.c_d = (t.data_ptr() for t in (self.lam, self.a, self.b, self.c))
+        h.rhsbx_d, h.rhsby_d, h.rhsbz_d = self.rhsb_ptrs()
+        h.cbc = self.cbc.encode()
+        h.c_or_f = self.c_or_f.encode()
 
     def solve(self, f: DeviceField):
         n = LB.ivec(self.ctx.n)
@@ -273,7 +292,16 @@ class Simulation:
         self.ctx = ctx or Context(case.n, precision=precision, stream=stream)
         ctx = self.ctx
         g = self.grid
+        self.imp = 2 if case.is_impdiff_1d else 1 if case.is_impdiff else 0
+        LB.check(LB.lib().cans_ctx_set_numerics(ctx.h, int(self.imp > 0), int(self.imp == 2)),
+                 "cans_ctx_set_numerics")
         self.plan = Plan(ctx, g, case.cbcpre, case.bcpre, "ccc")
+        # implicit diffusion: one Helmholtz system per velocity component (main.f90:304-309)
+        self.hplans = []
+        if self.imp:
+            for c, cf in enumerate(("fcc", "cfc", "ccf")):
+                self.hplans.append(Plan(ctx, g, case.cbcvel[6 * c:6 * c + 6], case.bcvel[6 * c:6 * c + 6], cf,
+                                        transforms=self.imp == 1))
         self.u, self.v, self.w = (ctx.field() for _ in range(3))
         self.p, self.pp = (ctx.field(kind=LB.CANS_FIELD_PRESSURE) for _ in range(2))
         torch = ctx.torch
@@ -327,11 +355,17 @@ class Simulation:
         self.alpha_max = max((sp["alpha"] for sp in self.scal_spec), default=0.0)
         self.is_buoyancy = bool(case.is_boussinesq_buoyancy) and len(self.s) > 0
         self._scal_c = (LB.Scalar * max(1, len(self.s)))()
+        self.splans = [Plan(ctx, g, sp["cbc"], sp["bc"], "ccc", transforms=self.imp == 1) if self.imp else None
+                       for sp in self.scal_spec]
         for i, sp in enumerate(self.scal_spec):
             e = self._scal_c[i]
             e.val, e.dsdtrko = self.s[i].ptr, self.dsdtrko[i].data_ptr()
             e.alpha, e.cbc, e.bc = sp["alpha"], sp["cbc"].encode(), LB.dvec(sp["bc"])
             e.source, e.is_forced, e.scalf = sp["source"], sp["is_forced"], sp["scalf"]
+            if self.imp:
+                self.splans[i].fill(e.h)
+        for k, hp in enumerate(self.hplans):
+            hp.fill(prm.hvel[k])
         prm.nscal = len(self.s)
         prm.scal = C.cast(self._scal_c, C.c_void_p).value if self.s else None
         prm.is_buoyancy = int(self.is_buoyancy)
@@ -421,11 +455,12 @@ class Simulation:
                                       dt, C.c_void_p(self.pp.ptr), C.c_void_p(self.u.ptr), C.c_void_p(self.v.ptr),
                                       C.c_void_p(self.w.ptr)), "cans_correc")
 
-    def updatep(self):
+    def updatep(self, alpha=0.0):
         g = self.grid
         LB.check(LB.lib().cans_updatep(self.ctx.h, LB.ivec(g.n), LB.dvec(g.dli),
-                                       C.c_void_p(self.dzci_d.data_ptr()), C.c_void_p(self.dzfi_d.data_ptr()), 0.0,
-                                       C.c_void_p(self.pp.ptr), C.c_void_p(self.p.ptr), 0), "cans_updatep")
+                                       C.c_void_p(self.dzci_d.data_ptr()), C.c_void_p(self.dzfi_d.data_ptr()), alpha,
+                                       C.c_void_p(self.pp.ptr), C.c_void_p(self.p.ptr), int(self.imp > 0)),
+                 "cans_updatep")
 
     def solver(self, f=None):
         self.plan.solve(f or self.pp)
@@ -436,7 +471,8 @@ class Simulation:
         LB.check(LB.lib().cans_chkdt(self.ctx.h, LB.ivec(g.n), LB.dvec(g.dl), C.c_void_p(self.dzci_d.data_ptr()),
                                      C.c_void_p(self.dzfi_d.data_ptr()), g.visc, self.alpha_max,
                                      C.c_void_p(self.u.ptr),
-                                     C.c_void_p(self.v.ptr), C.c_void_p(self.w.ptr), 0, 0, C.byref(out)),
+                                     C.c_void_p(self.v.ptr), C.c_void_p(self.w.ptr), int(self.imp > 0),
+                                     int(self.imp == 2), C.byref(out)),
                  "cans_chkdt")
         return out.value
 
@@ -485,17 +521,26 @@ class Simulation:
             for i in range(len(self.s)):  # main.f90:440-450
                 fs = self.rk_scal(i, irk, dt)
                 self.bulk_forcing_s(i, fs)
+                if self.imp:
+                    self.splans[i].solve_helmholtz(self.s[i], -0.5 * self.scal_spec[i]["alpha"] * dtrk)
                 self.boundp_s(i)
             self.rk(irk, dt, with_f=False)
+            alpha = 0.0
+            if self.imp:  # main.f90:453-462
+                alpha = -.5 * self.grid.visc * dtrk
+                for f, hp in zip((self.u, self.v, self.w), self.hplans):
+                    hp.solve_helmholtz(f, alpha)
             self.bounduvw(self.u, self.v, self.w, 0, LB.HALO_LOWER)
             self.fillps(dtrki)
             self.solver()
             self.boundp(self.pp)
             self.correc(dtrk)
             self.bounduvw(self.u, self.v, self.w, 1, LB.HALO_UPPER, LB.HALO_NONE)
-            self.updatep()
+            self.updatep(alpha)
             self.boundp(self.p, LB.HALO_UPPER)
 
     def close(self):
         self.plan.close()
+        for pl in self.hplans + [x for x in self.splans if x is not None]:
+            pl.close()
         self.ctx.sync()

@@ -73,6 +73,13 @@ int cans_field_free(cans_ctx *ctx, void *field_d);
  * Registering an address again replaces its kind; cans_field_free only forgets it.
  * No counterpart in the reference (its fields are typed Fortran arrays). */
 int cans_field_register(cans_ctx *ctx, void *field_d, int kind);
+/* the &numerics switches the reference's routines read from mod_param (param.f90:96,331):
+ * is_impdiff (implicit diffusion, solve_helmholtz of u, v, w and the scalars) and
+ * is_impdiff_1d (implicit in z only, which implies is_impdiff).  They change cans_rk,
+ * cans_rk_scal, cans_updatep, cans_solve_helmholtz and cans_rk3_substep as in the
+ * reference.  Implicit diffusion runs with FP64 velocity and scalar fields; the fused
+ * substep supports it in CANS_FP64 contexts. */
+int cans_ctx_set_numerics(cans_ctx *ctx, int is_impdiff, int is_impdiff_1d);
 /* host arrays are the reference's contiguous (0:n1+1,0:n2+1,0:n3+1) Fortran arrays
  * of double; float fields round on upload.  In a CANS_MIXED context the field
  * must come from cans_field_alloc_kind (its element type is looked up) */
@@ -188,6 +195,16 @@ int cans_bulk_mean(cans_ctx *ctx, const int *n, const double *grid_vol_ratio_d, 
  * scalars, explicit diffusion) in one stream-ordered call with no host
  * synchronisation.  Same results as calling the operators above in order.
  * ------------------------------------------------------------------------- */
+/* one implicit-diffusion system: what initsolver returns for a variable (initsolver.f90:15-83) */
+typedef struct cans_helmholtz {
+  const cans_plan *plan;                       /* NULL with is_impdiff_1d */
+  double normfft;
+  const double *lambdaxy_d, *a_d, *b_d, *c_d;
+  const double *rhsbx_d, *rhsby_d, *rhsbz_d;   /* boundary rhs planes, NULL if absent */
+  char cbc[6];                                 /* the variable's cbc(0:1,1:3) */
+  char c_or_f[3];
+} cans_helmholtz;
+
 typedef struct cans_step_params {
   char cbcvel[18];
   double bcvel[18];
@@ -207,6 +224,9 @@ typedef struct cans_step_params {
   double normfft;
   const double *lambdaxy_d, *a_d, *b_d, *c_d;  /* pressure solver arrays */
   const double *rhsbx_d, *rhsby_d, *rhsbz_d;   /* pressure BC rhs, NULL if homogeneous */
+  /* implicit diffusion (cans_ctx_set_numerics): the Helmholtz systems of u, v, w
+   * (initsolver with c_or_f 'fcc', 'cfc', 'ccf', main.f90:304-309) */
+  cans_helmholtz hvel[3];
   /* scalars (main.f90:440-450, at most CANS_MAX_SCALARS) and the Boussinesq buoyancy
    * of scal[0] (rk.f90:69-72); nscal = 0 and is_buoyancy = 0 give the velocity-only step */
   int nscal;
@@ -224,6 +244,7 @@ typedef struct cans_scalar {
   double source;                               /* ssource */
   int is_forced;                               /* is_sforced */
   double scalf;                                /* scalf */
+  cans_helmholtz h;                            /* implicit diffusion: initsolver(..., 'ccc', ...) */
 } cans_scalar;
 #define CANS_MAX_SCALARS 32
 

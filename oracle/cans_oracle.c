@@ -321,6 +321,44 @@ int orc_solve_helmholtz(const int *n, const int *kinds, const int *lens, double 
   return rc;
 }
 
+/* src/solver.f90:547-616 on one rank (z not decomposed): gaussel along z of the
+ * halo'd field, without eigenvalues (the lambda-absent branch of gaussel) */
+int orc_solver_gaussel_z(const int *n, const double *a, const double *b, const double *c,
+                         const char *bcz, char c_or_f_z, double norm, double *p) {
+  DIMS
+  const int q = (c_or_f_z == 'f' && bcz[1] == 'D') ? 1 : 0;
+  const int is_periodic_z = bcz[0] == 'P' && bcz[1] == 'P';
+  const size_t N = (size_t)n1 * n2 * n3;
+  double *px = (double *)malloc(sizeof(double) * (N ? N : 1));
+  if (!px) return -1;
+  #pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 1; k <= n3; k++)
+    for (int j = 1; j <= n2; j++)
+      for (int i = 1; i <= n1; i++) px[IX(i, j, k)] = p[HX(i, j, k)];
+  orc_gaussel(n1, n2, n3 - q, a, b, c, is_periodic_z, norm, px, NULL);
+  #pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 1; k <= n3 - q; k++)
+    for (int j = 1; j <= n2; j++)
+      for (int i = 1; i <= n1; i++) p[HX(i, j, k)] = px[IX(i, j, k)];
+  free(px);
+  return 0;
+}
+
+/* solve_helmholtz.f90:28-75, is_impdiff_1d branch: z boundary terms only (the x/y
+ * ones are deallocated for the 1-D form, main.f90:317-329) */
+int orc_solve_helmholtz_1d(const int *n, double alpha, const double *a, const double *b,
+                           const double *c, const double *rhsbz, const int *is_bound,
+                           const char *cbc, const char *c_or_f, double *p) {
+  int nz = n[2];
+  orc_updt_rhs_b(c_or_f, cbc, n, is_bound, NULL, NULL, rhsbz, p, alpha, 1);
+  double alphai = 1. / alpha;
+  double *bb = (double *)malloc(sizeof(double) * (size_t)nz);
+  for (int k = 0; k < nz; k++) bb[k] = b[k] + alphai;
+  int rc = orc_solver_gaussel_z(n, a, bb, c, cbc + 4, c_or_f[2], alphai, p);
+  free(bb);
+  return rc;
+}
+
 /* ------------------------------------------------------------------------ */
 /* momentum right-hand side: src/mom.f90:771-1019                            */
 /* ------------------------------------------------------------------------ */

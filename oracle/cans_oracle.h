@@ -66,6 +66,13 @@ int orc_solve_helmholtz(const int *n, const int *kinds, const int *lens, double 
                         const double *rhsby, const double *rhsbz, const int *is_bound,
                         const char *cbc, const char *c_or_f, double *p);
 
+/* src/solver.f90:547-616 (one rank): z-only elimination of the halo'd field p */
+int orc_solver_gaussel_z(const int *n, const double *a, const double *b, const double *c,
+                         const char *bcz, char c_or_f_z, double norm, double *p);
+/* src/solve_helmholtz.f90:28-75, is_impdiff_1d branch */
+int orc_solve_helmholtz_1d(const int *n, double alpha, const double *a, const double *b,
+                           const double *c, const double *rhsbz, const int *is_bound,
+                           const char *cbc, const char *c_or_f, double *p);
 /* src/mom.f90:771-1019 */
 void orc_mom_xyz_ad(const int *n, double dxi, double dyi, const double *dzci,
                     const double *dzfi, double visc, const double *u, const double *v,

@@ -69,6 +69,9 @@ def _declare(L):
     L.orc_rk_scal.argtypes = [_dp, _ip, _dp, _dp, _dp, _dp, C.c_double, C.c_double, _dp, _dp, _dp, C.c_int,
                               C.c_double, C.c_double, _dp, _dp, _dp, _dp, C.c_int, C.c_int]
     L.orc_bulk_forcing_s.argtypes = [_ip, C.c_int, C.c_double, _dp]
+    L.orc_solver_gaussel_z.argtypes = [_ip, _dp, _dp, _dp, C.c_char_p, C.c_char, C.c_double, _dp]
+    L.orc_solve_helmholtz_1d.argtypes = [_ip, C.c_double, _dp, _dp, _dp, C.c_void_p, _ip, C.c_char_p,
+                                         C.c_char_p, _dp]
     L.orc_bulk_mean.argtypes = [_ip, _dp, _dp]
     L.orc_bulk_mean.restype = C.c_double
     L.orc_fillps.argtypes = [_ip, _dp, _dp, C.c_double, _dp, _dp, _dp, _dp]
@@ -233,6 +236,13 @@ class OracleCase:
         # param.f90:308-310: the largest scalar diffusivity (1/huge without scalars)
         self.alpha_max = (min(float(sc.get("alphai", 1.0)) for sc in scalars) ** (-1)) if len(scalars) else 0.0
         self.swork = np.zeros(int(np.prod(self.n)) * (2 if is_impdiff else 1))
+        # implicit diffusion: one Helmholtz set-up per velocity component and scalar (main.f90:304-316)
+        self.h_vel = []
+        if self.is_impdiff:
+            for c, cf in enumerate(("fcc", "cfc", "ccf")):
+                self.h_vel.append(self.initsolver(cbcvel[6 * c:6 * c + 6], self.bcvel[6 * c:6 * c + 6], cf))
+            for sc in self.scalars:
+                sc["h"] = self.initsolver(sc["cbc"], sc["bc"], "ccc")
 
     # initsolver.f90:15-83 (single rank)
     def initsolver(self, cbc, bc, c_or_f):
@@ -310,8 +320,16 @@ class OracleCase:
             raise RuntimeError("orc_solver failed")
 
     def solve_helmholtz(self, p, alpha, s):
-        """solve_helmholtz.f90:28-75 (3-D branch) with the arrays of initsolver(cbc, bc, c_or_f)."""
+        """solve_helmholtz.f90:28-75 with the arrays of initsolver(cbc, bc, c_or_f): the 3-D branch, or
+        the z-only one (solver_gaussel_z) when is_impdiff_1d."""
         n, _k = self._n()
+        if self.is_impdiff_1d:
+            ib, _c = IP(self.is_bound)
+            rc = lib().orc_solve_helmholtz_1d(n, alpha, P(s["a"]), P(s["b"]), P(s["c"]), VP(s["rhsbz"]), ib,
+                                              chars(s["cbc"]), chars(s["c_or_f"]), P(p))
+            if rc:
+                raise RuntimeError("orc_solve_helmholtz_1d failed")
+            return
         kinds, _a = IP(s["kinds"])
         lens, _b = IP(s["lens"])
         ib, _c = IP(self.is_bound)
@@ -406,9 +424,16 @@ class OracleCase:
                 s = st["s%d" % i]
                 fs = self.rk_scal(rkpar, dt, sc, u, v, w, s, st["dsdtrko%d" % i])
                 self.bulk_forcing_s(sc["is_forced"], fs, s)
+                if self.is_impdiff:  # main.f90:446-448
+                    self.solve_helmholtz(s, -0.5 * sc["alpha"] * dtrk, sc["h"])
                 self.boundp(s, cbc=sc["cbc"], bc=sc["bc"])
             f = self.rk(rkpar, dt, p, st["dudtrko"], st["dvdtrko"], st["dwdtrko"], u, v, w,
                         st.get("s0"))
+            alpha = 0.0
+            if self.is_impdiff:  # main.f90:453-462
+                alpha = -.5 * self.visc * dtrk
+                for q, h in zip((u, v, w), self.h_vel):
+                    self.solve_helmholtz(q, alpha, h)
             dpdl += f
             self.bounduvw(u, v, w, 0, -1)
             self.fillps(u, v, w, pp, dtrki)
@@ -417,7 +442,7 @@ class OracleCase:
             self.boundp(pp)
             self.correc(pp, u, v, w, dtrk)
             self.bounduvw(u, v, w, 1, 1, 0)
-            self.updatep(pp, p)
+            self.updatep(pp, p, alpha)
             self.boundp(p, 1)
         return dpdl
 

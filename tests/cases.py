@@ -56,7 +56,9 @@ def oracle_case(p):
                         cbcpre=p["cbcpre"], bcvel=p.get("bcvel"), visci=p["visci"],
                         is_forced=p.get("is_forced", (0, 0, 0)), velf=p.get("velf", (0, 0, 0)),
                         scalars=p.get("scalars", ()), gacc=p.get("gacc", (0, 0, 0)), beta=p.get("beta", 0.0),
-                        is_boussinesq_buoyancy=p.get("is_boussinesq_buoyancy", False))
+                        is_boussinesq_buoyancy=p.get("is_boussinesq_buoyancy", False),
+                        is_impdiff=p.get("is_impdiff", False) or p.get("is_impdiff_1d", False),
+                        is_impdiff_1d=p.get("is_impdiff_1d", False))
 
 
 def host_case(p):
@@ -66,7 +68,9 @@ def host_case(p):
                 is_forced=tuple(bool(x) for x in p.get("is_forced", (0, 0, 0))),
                 velf=tuple(p.get("velf", (0.0, 0.0, 0.0))), scalars=tuple(p.get("scalars", ())),
                 gacc=tuple(p.get("gacc", (0.0, 0.0, 0.0))), beta=float(p.get("beta", 0.0)),
-                is_boussinesq_buoyancy=bool(p.get("is_boussinesq_buoyancy", False)))
+                is_boussinesq_buoyancy=bool(p.get("is_boussinesq_buoyancy", False)),
+                is_impdiff=bool(p.get("is_impdiff", False) or p.get("is_impdiff_1d", False)),
+                is_impdiff_1d=bool(p.get("is_impdiff_1d", False)))
 
 
 def initial_state(name, p, grid, seed=2024):

@@ -164,3 +164,28 @@ def test_dhc_nusselt_fixture():
     assert divmax < 2.2e-9
     nu = cases.dhc_nusselt(st["s0"][1:-1, 1:-1, 1:-1], case.dl)
     np.testing.assert_allclose([nu], [8.8252], rtol=1e-2, atol=1e-2)
+
+
+def test_gaussel_z_dense():
+    """solver_gaussel_z (gaussel's eigenvalue-free branch) against dense solves of the same tridiagonal
+    systems, for a c-grid system and an f-grid one with its last row dropped (q = 1)."""
+    rng = np.random.default_rng(21)
+    n = (3, 2, 12)
+    for cf, bcz in (("c", "DD"), ("f", "DD"), ("c", "NN")):
+        n3 = n[2]
+        q = 1 if (cf == "f" and bcz[1] == "D") else 0
+        nn = n3 - q
+        a = rng.uniform(0.5, 1.0, n3)
+        c = rng.uniform(0.5, 1.0, n3)
+        b = -(a + c) - rng.uniform(1.0, 2.0, n3)  # diagonally dominant, like b + 1/alpha with alpha < 0
+        norm = 0.37
+        p = np.asfortranarray(rng.uniform(-1, 1, (n[0] + 2, n[1] + 2, n3 + 2)))
+        ref = p.copy(order="F")
+        A = np.diag(b[:nn]) + np.diag(a[1:nn], -1) + np.diag(c[:nn - 1], 1)
+        for i in range(1, n[0] + 1):
+            for j in range(1, n[1] + 1):
+                ref[i, j, 1:nn + 1] = np.linalg.solve(A, p[i, j, 1:nn + 1] * norm)
+        nn_, _k = O.IP(list(n))
+        rc = O.lib().orc_solver_gaussel_z(nn_, O.P(a), O.P(b), O.P(c), bcz.encode(), cf.encode(), norm, O.P(p))
+        assert rc == 0
+        np.testing.assert_allclose(p, ref, rtol=1e-13, atol=1e-14)
