@@ -15,8 +15,11 @@ B_P = 5 passes x N cells x (s_in + s_out) = 40 N bytes with FP32 storage, 80 N
 with FP64 storage.  The oracle (C/OpenMP restatement) is timed on the host on a
 bounded number of full steps of the same case as the CPU baseline.
 
+A secondary workload (the 256^3 heated cube with a scalar, buoyancy and implicit z
+diffusion, FP64) is timed after the headline one and reported under "secondary".
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--ng N1 N2 N3]
-                       [--precision both|mixed|fp64] [--solves S] [--no-cpu]
+                       [--precision both|mixed|fp64] [--solves S] [--no-cpu] [--no-secondary]
 """
 import argparse
 import json
@@ -117,6 +120,46 @@ def run(p, init, precision, args):
     return res, final
 
 
+def secondary_cavity(args):
+    """A second workload beside the headline: the 256^3 heated cube (cases.params("dhc3d")): scalar
+    transport with buoyancy, implicit z diffusion (four z-only Helmholtz solves per substep: u, v, w and
+    the scalar) and a DCT x DCT pressure solve, FP64, fixed dt."""
+    import numpy as np
+    import torch
+    from cans_amd import _lib as LB
+    from cans_amd import initflow as IF
+    from cans_amd.host import Simulation
+    from tests import cases
+    p = cases.params("dhc3d")
+    print("[bench] heated cube %s: set-up" % (p["ng"],), file=sys.stderr, flush=True)
+    sim = Simulation(cases.host_case(p), precision=LB.CANS_FP64)
+    u, v, w, pr = cases.initial_state("dhc3d", p, sim.grid, seed=3)
+    sc = cases.initial_scalars("dhc3d", p, sim.grid)
+    sim.set_state(*(IF.halo(p["ng"], q) for q in (u, v, w, pr)), s=[IF.halo(p["ng"], q) for q in sc])
+    sim.init_bounds()
+    dt = 1e-3
+    for _ in range(2):
+        sim.rk3_step(dt)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    nst = max(1, args.steps // 2)
+    e0.record()
+    for _ in range(nst):
+        sim.rk3_step(dt)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / nst
+    _, divmax = sim.chkdiv()
+    ncell = int(np.prod(p["ng"]))
+    sim.close()
+    del sim
+    torch.cuda.empty_cache()
+    print("[bench] heated cube: %.3f ms/step" % ms, file=sys.stderr, flush=True)
+    return {"workload": "heated cube %dx%dx%d, no-slip walls, scalar with buoyancy, implicit z diffusion "
+                        "(is_impdiff_1d), DCT x DCT pressure, dt=1e-3, FP64" % tuple(p["ng"]),
+            "ms_per_step": ms, "ns_per_cell_step": ms * 1e-3 / ncell * 1e9, "steps": nst, "divmax": divmax}
+
+
 def copy_bandwidth():
     """Device copy of a known size: the box's streaming bandwidth."""
     import torch
@@ -146,6 +189,7 @@ def main():
     ap.add_argument("--precision", choices=("both", "mixed", "fp64"), default="both")
     ap.add_argument("--solves", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs (--gpus %d) are not part of it" % args.gpus,
@@ -213,6 +257,8 @@ def main():
         out["mixed_vs_fp64_rel_linf"] = err
     del final
     out["copy_GBps"] = copy_bandwidth()
+    if not args.no_secondary:
+        out["secondary"] = [secondary_cavity(args)]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, init)
     print(json.dumps(out))

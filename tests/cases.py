@@ -36,6 +36,11 @@ def params(name, ng=None):
                  cbcpre="NNPPNN", visci=1186.78165819,
                  scalars=(dict(alphai=842.614977318, cbc="DDPPNN", bc=(-0.5, 0.5, 0.0, 0.0, 0.0, 0.0)),),
                  gacc=(0.0, 0.0, -1.0), beta=1.0, is_boussinesq_buoyancy=True)
+    elif name == "dhc3d":  # heated cube: no-slip walls, hot/cold x walls, adiabatic y/z walls, implicit z diffusion
+        p = dict(ng=(256, 256, 256), l=(1.0, 1.0, 1.0), gtype=1, gr=0.0, cbcvel="DDDDDD" * 3,
+                 cbcpre="NNNNNN", visci=1186.78165819,
+                 scalars=(dict(alphai=842.614977318, cbc="DDNNNN", bc=(-0.5, 0.5, 0.0, 0.0, 0.0, 0.0)),),
+                 gacc=(0.0, 0.0, -1.0), beta=1.0, is_boussinesq_buoyancy=True, is_impdiff_1d=True)
     elif name == "rb_channel":  # heated channel: scalar walls in z (Dirichlet), buoyancy, two scalars, one forced
         p = dict(ng=(32, 16, 24), l=(2.0, 1.0, 1.0), gtype=1, gr=1.2, cbcvel="PPPPDD" * 3,
                  cbcpre="PPPPNN", visci=300.0, is_forced=(1, 0, 0), velf=(1.0, 0.0, 0.0),
@@ -97,13 +102,13 @@ def initial_state(name, p, grid, seed=2024):
 
 
 def initial_scalars(name, p, grid, seed=2025):
-    """Interior fields of the case's scalars.  dhc: the linear profile between the
-    x walls (initflow.f90:327-337, with zero velocity); otherwise seeded noise."""
+    """Interior fields of the case's scalars.  dhc, dhc3d: the linear profile between the
+    x walls (initflow.f90:327-337); otherwise seeded noise."""
     n = p["ng"]
     out = []
     g = IF.rng(seed)
     for sc in p.get("scalars", ()):
-        if name == "dhc":
+        if name in ("dhc", "dhc3d"):
             xx = (np.arange(1, n[0] + 1) - 0.5) * grid.dl[0] / grid.l[0]
             prof = (1.0 - xx) * sc["bc"][0] + xx * sc["bc"][1]
             out.append(np.asfortranarray(np.broadcast_to(prof[:, None, None], tuple(n)).copy()))
