@@ -998,37 +998,69 @@ __global__ void k_shift_diag(const double* b, double alphai, double* bb, int n) 
   if (k < n) bb[k] = b[k] + alphai;
 }
 
-__global__ void k_z1_tables(const double* a, const double* bb, const double* c, int nn, double* zt, double* dt) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  double z = 1. / bb[0];
-  zt[0] = z;
-  dt[0] = c[0] * z;
-  for (int k = 2; k <= nn; k++) {
-    z = 1. / (bb[k - 1] - a[k - 1] * dt[k - 2]);
-    zt[k - 1] = z;
-    dt[k - 1] = c[k - 1] * z;
-  }
-}
+// One work-group per row of 256 lines: the coefficients of bb = b + 1/alpha, a and c
+// are staged in LDS, one thread runs the (line-independent) recurrence for z(k) and
+// d(k) from them, then every thread sweeps its line in batches of ZB rows, the
+// batch's loads issued before its dependent arithmetic.
+constexpr int ZB = 8;
 
-__global__ void __launch_bounds__(256) k_z1_sweep(Layout L, double* p, const double* a, const double* zt,
-                                                  const double* dt, int nn, double norm) {
+__global__ void __launch_bounds__(256) k_z1_solve(Layout L, double* p, const double* a, const double* b,
+                                                  const double* c, double alphai, int nn, double norm) {
+  extern __shared__ double zsh[];
+  double* la = zsh;           // a(k)
+  double* lb = la + nn;       // bb(k), then z(k)
+  double* lc = lb + nn;       // c(k), then d(k)
+  for (int k = threadIdx.x; k < nn; k += blockDim.x) {
+    la[k] = a[k];
+    lb[k] = b[k] + alphai;
+    lc[k] = c[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {  // solver.f90:170-185 (lambda absent): z and d once for all lines
+    double z = 1. / lb[0];
+    lb[0] = z;
+    double d = lc[0] * z;
+    lc[0] = d;
+    for (int k = 1; k < nn; k++) {
+      z = 1. / (lb[k] - la[k] * d);
+      lb[k] = z;
+      d = lc[k] * z;
+      lc[k] = d;
+    }
+  }
+  __syncthreads();
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int j = blockIdx.y + 1;
   if (i > L.n1) return;
-  const kcoef ka = (kcoef)a, kz = (kcoef)zt, kd = (kcoef)dt;
-  double* P = p + L.at(i, j, 1);  // row k at P + (k-1)*sz
+  double* P = p + L.at(i, j, 1);  // row k (1-based) at P + (k-1)*sz
   const long sz = L.sz;
-  double prev = P[0] * norm * kz[0];
-  P[0] = prev;
-  for (int k = 2; k <= nn; k++) {
-    double* q = P + (long)(k - 1) * sz;
-    prev = (*q * norm - ka[k - 1] * prev) * kz[k - 1];
-    *q = prev;
+  double prev = 0.;
+  for (int k0 = 0; k0 < nn; k0 += ZB) {
+    double r[ZB];
+#pragma unroll
+    for (int u = 0; u < ZB; u++) r[u] = (k0 + u < nn) ? P[(long)(k0 + u) * sz] : 0.;
+#pragma unroll
+    for (int u = 0; u < ZB; u++) {
+      const int k = k0 + u;
+      if (k < nn) {
+        prev = (k == 0) ? r[u] * norm * lb[0] : (r[u] * norm - la[k] * prev) * lb[k];
+        P[(long)k * sz] = prev;
+      }
+    }
   }
-  for (int k = nn - 1; k >= 1; k--) {
-    double* q = P + (long)(k - 1) * sz;
-    prev = *q - kd[k - 1] * prev;
-    *q = prev;
+  // backward substitution from row nn-1 (1-based) down: p(k) = p(k) - d(k)*p(k+1)
+  for (int k1 = nn - 2; k1 >= 0; k1 -= ZB) {
+    double r[ZB];
+#pragma unroll
+    for (int u = 0; u < ZB; u++) r[u] = (k1 - u >= 0) ? P[(long)(k1 - u) * sz] : 0.;
+#pragma unroll
+    for (int u = 0; u < ZB; u++) {
+      const int k = k1 - u;
+      if (k >= 0) {
+        prev = r[u] - lc[k] * prev;
+        P[(long)k * sz] = prev;
+      }
+    }
   }
 }
 
@@ -1039,18 +1071,10 @@ int run_helmholtz_z(cans_ctx* c, const double* a, const double* b, const double*
     return fail(CANS_EUNSUPPORTED, "implicit z diffusion with periodic z is not built on the device");
   const int q = (c_or_f_z == 'f' && bcz[1] == 'D') ? 1 : 0;
   const int nn = L.n3 - q;
-  void* tab = nullptr;
-  int rc = ensure_work(c, 5, (size_t)3 * L.n3 * sizeof(double), &tab);
-  if (rc) return rc;
-  double* bb = (double*)tab;
-  double* zt = bb + L.n3;
-  double* dt = zt + L.n3;
-  k_shift_diag<<<cdiv(L.n3, 256), 256, 0, c->stream>>>(b, 1. / alpha, bb, L.n3);
-  CANS_LAUNCH_CHECK("k_shift_diag");
-  k_z1_tables<<<1, 64, 0, c->stream>>>(a, bb, cc, nn, zt, dt);
-  CANS_LAUNCH_CHECK("k_z1_tables");
-  k_z1_sweep<<<dim3(cdiv(L.n1, 256), L.n2), 256, 0, c->stream>>>(L, fld, a, zt, dt, nn, 1. / alpha);
-  CANS_LAUNCH_CHECK("k_z1_sweep");
+  const size_t shm = (size_t)3 * nn * sizeof(double);
+  if (shm > 160 * 1024) return fail(CANS_EUNSUPPORTED, "implicit z diffusion: too many z rows for LDS tables");
+  k_z1_solve<<<dim3(cdiv(L.n1, 256), L.n2), 256, shm, c->stream>>>(L, fld, a, b, cc, 1. / alpha, nn, 1. / alpha);
+  CANS_LAUNCH_CHECK("k_z1_solve");
   return CANS_OK;
 }
 
