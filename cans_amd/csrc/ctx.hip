@@ -175,6 +175,13 @@ extern "C" int cans_ctx_set_numerics(cans_ctx* c, int is_impdiff, int is_impdiff
   return CANS_OK;
 }
 
+extern "C" int cans_ctx_set_dtdma(cans_ctx* c, int nslab) {
+  if (!c) return fail(CANS_EINVAL, "cans_ctx_set_dtdma: null context");
+  if (nslab < 0) return fail(CANS_EINVAL, "cans_ctx_set_dtdma: negative slab count");
+  c->dtdma = nslab;
+  return CANS_OK;
+}
+
 extern "C" int cans_field_register(cans_ctx* c, void* field, int kind) {
   if (!c || !field) return fail(CANS_EINVAL, "cans_field_register: null argument");
   if (kind != CANS_FIELD_VELOCITY && kind != CANS_FIELD_PRESSURE)

@@ -838,6 +838,144 @@ __global__ void __launch_bounds__(256) k_thomas(ZArgs<T> z) {
 }
 
 // ---------------------------------------------------------------------------
+// Distributed tridiagonal solve (gaussel_dtdma, solver.f90:309-517) for a
+// non-periodic z: the rows are split into nslab consecutive slabs, as the ranks
+// of a z decomposition hold them (slab s: n/nslab rows, the first n%nslab slabs
+// one more).  On several GPUs each slab's elimination and back-substitution run
+// on its own GPU and only the 2*nslab boundary rows per line travel; on one GPU
+// the three phases show those kernels at their real shape, and the slabs give
+// nslab times more independent sweeps.  The arithmetic is the reference's; the
+// modified rhs and the aa/cc factors are kept in FP64 scratch ([row][line]).
+// ---------------------------------------------------------------------------
+template <typename T>
+struct DtArgs {
+  T* W;
+  long nl;
+  int n1, n2, nic;
+  long PS;
+  int n, nslab;
+  double norm;
+  const double *a, *b, *c, *lam;
+  double *pd, *aa, *cc;  // [n][nl]
+  double *ra, *rc, *rp;  // [2*nslab][nl]
+};
+
+__device__ __forceinline__ void slab_range(int n, int nslab, int s, int& lo, int& cnt) {
+  const int m = n / nslab, r = n % nslab;
+  cnt = m + (s < r ? 1 : 0);
+  lo = s * m + (s < r ? s : r);
+}
+
+// phase 1, one thread per (line, slab): eliminate the slab down to its first and last unknowns
+template <typename T>
+__global__ void __launch_bounds__(256) k_dtdma_local(DtArgs<T> z) {
+  const long line = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (line >= z.nl) return;
+  const int s = blockIdx.y;
+  int lo, ns;
+  slab_range(z.n, z.nslab, s, lo, ns);
+  const T* P = z.W + woff((int)(line % z.n1), (int)(line / z.n1), 0, z.n2, z.nic);
+  const long nl = z.nl, PS = z.PS;
+  const double l = z.lam ? z.lam[line] : 0., norm = z.norm;
+  const kcoef a = (kcoef)(z.a + lo), b = (kcoef)(z.b + lo), c = (kcoef)(z.c + lo);
+  double* pd = z.pd + (long)lo * nl + line;
+  double* aa = z.aa + (long)lo * nl + line;
+  double* cc = z.cc + (long)lo * nl + line;
+  // lambda present: bb = b + lambda; absent: b itself (solver.f90:344-347, 395-398)
+  auto bbk = [&](int k) { return z.lam ? b[k] + l : b[k]; };
+  for (int k = 0; k < 2; k++) {
+    const double zz = 1. / bbk(k);
+    aa[k * nl] = a[k] * zz;
+    cc[k * nl] = c[k] * zz;
+    pd[k * nl] = (double)P[zrow(lo + k, PS)] * norm * zz;
+  }
+  double pprev = pd[nl], aprev = aa[nl], cprev = cc[nl];
+  for (int k = 2; k < ns; k++) {  // elimination of lower diagonals
+    const double zk = 1. / (bbk(k) - a[k] * cprev);
+    pprev = ((double)P[zrow(lo + k, PS)] * norm - a[k] * pprev) * zk;
+    aprev = -a[k] * aprev * zk;
+    cprev = c[k] * zk;
+    pd[k * nl] = pprev;
+    aa[k * nl] = aprev;
+    cc[k * nl] = cprev;
+  }
+  // elimination of upper diagonals, k = ns-3 .. 1: pprev/aprev/cprev hold row k+1
+  pprev = pd[(long)(ns - 2) * nl];
+  aprev = aa[(long)(ns - 2) * nl];
+  cprev = cc[(long)(ns - 2) * nl];
+  for (int k = ns - 3; k >= 1; k--) {
+    const double ck = cc[(long)k * nl];
+    pprev = pd[(long)k * nl] - ck * pprev;
+    aprev = aa[(long)k * nl] - ck * aprev;
+    cprev = -ck * cprev;
+    pd[(long)k * nl] = pprev;
+    aa[(long)k * nl] = aprev;
+    cc[(long)k * nl] = cprev;
+  }
+  // first row; row 1 now holds the upper-eliminated values (pprev, aprev, cprev when ns > 3)
+  const double p1 = pd[nl], a1 = aa[nl], c1 = cc[nl];
+  const double c0 = cc[0];
+  const double zf = 1. / (1. - a1 * c0);
+  const double p0 = (pd[0] - c0 * p1) * zf;
+  const double a0 = aa[0] * zf;
+  const double cn0 = -c0 * c1 * zf;
+  pd[0] = p0;
+  aa[0] = a0;
+  cc[0] = cn0;
+  const long r0 = (long)(2 * s) * nl + line, r1 = r0 + nl;
+  z.ra[r0] = a0;
+  z.rc[r0] = cn0;
+  z.rp[r0] = p0;
+  z.ra[r1] = aa[(long)(ns - 1) * nl];
+  z.rc[r1] = cc[(long)(ns - 1) * nl];
+  z.rp[r1] = pd[(long)(ns - 1) * nl];
+}
+
+// phase 2, one thread per line: the reduced system of the 2*nslab boundary rows (unit diagonal)
+template <typename T>
+__global__ void __launch_bounds__(256) k_dtdma_reduced(DtArgs<T> z) {
+  const long line = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (line >= z.nl) return;
+  const long nl = z.nl;
+  const int nn = 2 * z.nslab;
+  double* ra = z.ra + line;
+  double* rc = z.rc + line;
+  double* rp = z.rp + line;
+  double pprev = rp[0], cprev = rc[0];
+  for (int k = 1; k < nn; k++) {
+    const double ak = ra[(long)k * nl];
+    const double zk = 1. / (1. - ak * cprev);
+    pprev = (rp[(long)k * nl] - ak * pprev) * zk;
+    cprev = rc[(long)k * nl] * zk;
+    rp[(long)k * nl] = pprev;
+    rc[(long)k * nl] = cprev;
+  }
+  for (int k = nn - 2; k >= 0; k--) {
+    pprev = rp[(long)k * nl] - rc[(long)k * nl] * pprev;
+    rp[(long)k * nl] = pprev;
+  }
+}
+
+// phase 3, one thread per (line, slab): boundary unknowns in, interior rows back-substituted
+template <typename T>
+__global__ void __launch_bounds__(256) k_dtdma_back(DtArgs<T> z) {
+  const long line = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (line >= z.nl) return;
+  const int s = blockIdx.y;
+  int lo, ns;
+  slab_range(z.n, z.nslab, s, lo, ns);
+  T* P = z.W + woff((int)(line % z.n1), (int)(line / z.n1), 0, z.n2, z.nic);
+  const long nl = z.nl, PS = z.PS;
+  const double p1 = z.rp[(long)(2 * s) * nl + line], pn = z.rp[(long)(2 * s + 1) * nl + line];
+  P[zrow(lo, PS)] = (T)p1;
+  P[zrow(lo + ns - 1, PS)] = (T)pn;
+  for (int k = 1; k < ns - 1; k++) {
+    const long o = (long)(lo + k) * nl + line;
+    P[zrow(lo + k, PS)] = (T)(z.pd[o] - z.aa[o] * p1 - z.cc[o] * pn);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 static int upload_tables(cans_plan* p, int d) {
@@ -976,9 +1114,45 @@ int run_solver(cans_ctx* c, const cans_plan* p, double normfft, const double* la
   z.ckp = z.ckd + nblk * nl;
   z.ckq = z.ckp + nblk * nl;
   z.X2 = periodic ? z.ckq + nblk * nl : nullptr;
-  if (periodic) k_gaussel<T, true><<<cdiv(nl, 256), 256, 0, c->stream>>>(z);
-  else k_thomas<T, sizeof(T) == sizeof(float)><<<cdiv(nl, 256), 256, 0, c->stream>>>(z);
-  CANS_LAUNCH_CHECK("k_gaussel");
+  if (periodic) {
+    k_gaussel<T, true><<<cdiv(nl, 256), 256, 0, c->stream>>>(z);
+    CANS_LAUNCH_CHECK("k_gaussel");
+  } else if (c->dtdma > 1) {
+    if (nn / c->dtdma < 3) return fail(CANS_EINVAL, "cans_solver: distributed z solve needs 3 rows per slab");
+    DtArgs<T> d;
+    d.W = (T*)W;
+    d.nl = nl;
+    d.n1 = L.n1;
+    d.n2 = L.n2;
+    d.nic = z.nic;
+    d.PS = z.PS;
+    d.n = nn;
+    d.nslab = c->dtdma;
+    d.norm = normfft;
+    d.a = a;
+    d.b = b;
+    d.c = cc;
+    d.lam = lam;
+    void* dt = nullptr;
+    rc = ensure_work(c, 6, ((size_t)3 * nn + (size_t)6 * d.nslab) * nl * sizeof(double), &dt);
+    if (rc) return rc;
+    d.pd = (double*)dt;
+    d.aa = d.pd + (long)nn * nl;
+    d.cc = d.aa + (long)nn * nl;
+    d.ra = d.cc + (long)nn * nl;
+    d.rc = d.ra + (long)2 * d.nslab * nl;
+    d.rp = d.rc + (long)2 * d.nslab * nl;
+    const dim3 gs(cdiv(nl, 256), d.nslab);
+    k_dtdma_local<T><<<gs, 256, 0, c->stream>>>(d);
+    CANS_LAUNCH_CHECK("k_dtdma_local");
+    k_dtdma_reduced<T><<<cdiv(nl, 256), 256, 0, c->stream>>>(d);
+    CANS_LAUNCH_CHECK("k_dtdma_reduced");
+    k_dtdma_back<T><<<gs, 256, 0, c->stream>>>(d);
+    CANS_LAUNCH_CHECK("k_dtdma_back");
+  } else {
+    k_thomas<T, sizeof(T) == sizeof(float)><<<cdiv(nl, 256), 256, 0, c->stream>>>(z);
+    CANS_LAUNCH_CHECK("k_thomas");
+  }
   if ((rc = ypass<T>(c, p, (T*)W, -1))) return rc;
   if ((rc = xpass<T>(c, p, fld, (T*)W, -1))) return rc;
   return CANS_OK;

@@ -106,6 +106,11 @@ module cans_hip
       type(c_ptr), value :: ctx
       integer(c_int), value :: is_impdiff, is_impdiff_1d
     end function
+    integer(c_int) function cans_ctx_set_dtdma(ctx, nslab) bind(C, name='cans_ctx_set_dtdma')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nslab
+    end function
     integer(c_int) function cans_field_register(ctx, field, kind) bind(C, name='cans_field_register')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, field

@@ -47,6 +47,8 @@ class Case:
     # &numerics (src/param.f90:96,331): implicit diffusion, in 3-D or in z only
     is_impdiff: bool = False
     is_impdiff_1d: bool = False
+    # is_poisson_dtdma: z slabs of the distributed tridiagonal solve (0: the plain sweep)
+    dtdma_slabs: int = 0
 
     @property
     def n(self):
@@ -295,6 +297,7 @@ class Simulation:
         self.imp = 2 if case.is_impdiff_1d else 1 if case.is_impdiff else 0
         LB.check(LB.lib().cans_ctx_set_numerics(ctx.h, int(self.imp > 0), int(self.imp == 2)),
                  "cans_ctx_set_numerics")
+        LB.check(LB.lib().cans_ctx_set_dtdma(ctx.h, int(case.dtdma_slabs)), "cans_ctx_set_dtdma")
         self.plan = Plan(ctx, g, case.cbcpre, case.bcpre, "ccc")
         # implicit diffusion: one Helmholtz system per velocity component (main.f90:304-309)
         self.hplans = []

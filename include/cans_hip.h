@@ -80,6 +80,12 @@ int cans_field_register(cans_ctx *ctx, void *field_d, int kind);
  * reference.  Implicit diffusion runs with FP64 velocity and scalar fields; the fused
  * substep supports it in CANS_FP64 contexts. */
 int cans_ctx_set_numerics(cans_ctx *ctx, int is_impdiff, int is_impdiff_1d);
+/* is_poisson_dtdma (param.f90, solver.f90:309-517): the z elimination of the pressure
+ * solve (and of the Helmholtz solves' 3-D form) as the distributed tridiagonal algorithm
+ * with the z rows split into nslab slabs, the ranks of a z decomposition (each slab at
+ * least 3 rows).  0 or 1: the plain sweep.  Non-periodic z only; like the reference's
+ * gaussel_dtdma it has no zero-mode pin. */
+int cans_ctx_set_dtdma(cans_ctx *ctx, int nslab);
 /* host arrays are the reference's contiguous (0:n1+1,0:n2+1,0:n3+1) Fortran arrays
  * of double; float fields round on upload.  In a CANS_MIXED context the field
  * must come from cans_field_alloc_kind (its element type is looked up) */

@@ -274,6 +274,98 @@ void orc_gaussel(int nx, int ny, int n, const double *a, const double *b, const 
 }
 
 /* ------------------------------------------------------------------------ */
+/* distributed tridiagonal solve: src/solver.f90:309-517 (gaussel_dtdma),     */
+/* non-periodic z, with the z rows split into nslab consecutive slabs (the    */
+/* ranks of the z decomposition; slab s has n/nslab rows, the first n%nslab   */
+/* slabs one more).  Every slab eliminates its rows down to its first and     */
+/* last unknown, the 2*nslab boundary rows form the reduced system (unit      */
+/* diagonal) solved per line, and the slab interiors are back-substituted.    */
+/* ------------------------------------------------------------------------ */
+void orc_slab_range(int n, int nslab, int s, int *lo, int *cnt) {
+  const int m = n / nslab, r = n % nslab;
+  *cnt = m + (s < r ? 1 : 0);
+  *lo = s * m + (s < r ? s : r);
+}
+
+int orc_gaussel_dtdma(int nx, int ny, int n, int nslab, const double *a, const double *b,
+                      const double *c, double norm, double *p, const double *lambdaxy) {
+  const size_t pl = (size_t)nx * ny;
+  if (nslab < 1 || n / nslab < 3) return -1;  /* the boundary-row step needs three rows per slab */
+  int ok = 0;
+  #pragma omp parallel
+  {
+    double *aa = (double *)malloc(sizeof(double) * (size_t)n);
+    double *cc = (double *)malloc(sizeof(double) * (size_t)n);
+    double *bb = (double *)malloc(sizeof(double) * (size_t)n);
+    double *ra = (double *)malloc(sizeof(double) * 2 * (size_t)nslab);
+    double *rc = (double *)malloc(sizeof(double) * 2 * (size_t)nslab);
+    double *rp = (double *)malloc(sizeof(double) * 2 * (size_t)nslab);
+    #pragma omp for collapse(2) schedule(static)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        double *pl_ = p + (size_t)i + (size_t)nx * j;
+        #define PK(k) pl_[(size_t)(k) * pl]
+        const double lxy = lambdaxy ? lambdaxy[(size_t)i + (size_t)nx * j] : 0.;
+        for (int s = 0; s < nslab; s++) {
+          int lo, ns;
+          orc_slab_range(n, nslab, s, &lo, &ns);
+          /* local rows 0..ns-1 are global rows lo..lo+ns-1 (0-based) */
+          #define A(k) a[lo + (k)]
+          #define C(k) c[lo + (k)]
+          for (int k = 0; k < ns; k++) bb[k] = lambdaxy ? b[lo + k] + lxy : b[lo + k];
+          for (int k = 0; k < 2; k++) {
+            const double zz = 1. / bb[k];
+            aa[lo + k] = A(k) * zz;
+            cc[lo + k] = C(k) * zz;
+            PK(lo + k) = PK(lo + k) * norm * zz;
+          }
+          for (int k = 2; k < ns; k++) {  /* elimination of lower diagonals */
+            const double z = 1. / (bb[k] - A(k) * cc[lo + k - 1]);
+            PK(lo + k) = (PK(lo + k) * norm - A(k) * PK(lo + k - 1)) * z;
+            aa[lo + k] = -A(k) * aa[lo + k - 1] * z;
+            cc[lo + k] = C(k) * z;
+          }
+          for (int k = ns - 3; k >= 1; k--) {  /* elimination of upper diagonals */
+            PK(lo + k) = PK(lo + k) - cc[lo + k] * PK(lo + k + 1);
+            aa[lo + k] = aa[lo + k] - cc[lo + k] * aa[lo + k + 1];
+            cc[lo + k] = -cc[lo + k] * cc[lo + k + 1];
+          }
+          {
+            const double z = 1. / (1. - aa[lo + 1] * cc[lo]);
+            PK(lo) = (PK(lo) - cc[lo] * PK(lo + 1)) * z;
+            aa[lo] = aa[lo] * z;
+            cc[lo] = -cc[lo] * cc[lo + 1] * z;
+          }
+          ra[2 * s] = aa[lo];          rc[2 * s] = cc[lo];          rp[2 * s] = PK(lo);
+          ra[2 * s + 1] = aa[lo + ns - 1]; rc[2 * s + 1] = cc[lo + ns - 1]; rp[2 * s + 1] = PK(lo + ns - 1);
+          #undef A
+          #undef C
+        }
+        /* reduced system (solver.f90:456-468) */
+        const int nn = 2 * nslab;
+        for (int k = 1; k < nn; k++) {
+          const double z = 1. / (1. - ra[k] * rc[k - 1]);
+          rp[k] = (rp[k] - ra[k] * rp[k - 1]) * z;
+          rc[k] = rc[k] * z;
+        }
+        for (int k = nn - 2; k >= 0; k--) rp[k] = rp[k] - rc[k] * rp[k + 1];
+        /* back to the slabs (solver.f90:503-514) */
+        for (int s = 0; s < nslab; s++) {
+          int lo, ns;
+          orc_slab_range(n, nslab, s, &lo, &ns);
+          const double p1 = rp[2 * s], pn = rp[2 * s + 1];
+          PK(lo) = p1;
+          PK(lo + ns - 1) = pn;
+          for (int k = 1; k < ns - 1; k++) PK(lo + k) = PK(lo + k) - aa[lo + k] * p1 - cc[lo + k] * pn;
+        }
+        #undef PK
+      }
+    free(aa); free(cc); free(bb); free(ra); free(rc); free(rp);
+  }
+  return ok;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Poisson/Helmholtz solver: src/solver.f90:17-112                           */
 /* ------------------------------------------------------------------------ */
 int orc_solver(const int *n, const int *kinds, const int *lens, double normfft,

@@ -75,7 +75,7 @@ def host_case(p):
                 gacc=tuple(p.get("gacc", (0.0, 0.0, 0.0))), beta=float(p.get("beta", 0.0)),
                 is_boussinesq_buoyancy=bool(p.get("is_boussinesq_buoyancy", False)),
                 is_impdiff=bool(p.get("is_impdiff", False) or p.get("is_impdiff_1d", False)),
-                is_impdiff_1d=bool(p.get("is_impdiff_1d", False)))
+                is_impdiff_1d=bool(p.get("is_impdiff_1d", False)), dtdma_slabs=int(p.get("dtdma_slabs", 0)))
 
 
 def initial_state(name, p, grid, seed=2024):
