@@ -462,17 +462,23 @@ extern "C" int cans_bulk_mean(cans_ctx* c, const int* n, const double* gvr, cons
   return run_bulk_mean<double>(c, (const double*)p, gvr, mean);
 }
 
-// solve_helmholtz (solve_helmholtz.f90:28-75) of one FP64 field with the system h
-static int helmholtz(cans_ctx* c, const cans_helmholtz& h, const int* is_bound, double alpha, double* f) {
+// solve_helmholtz (solve_helmholtz.f90:28-75) of n FP64 fields f[q] with the systems h[q]:
+// each system's boundary rhs, then one batched solve (the passes carry all systems)
+static int helmholtz(cans_ctx* c, int n, const cans_helmholtz* const* h, const int* is_bound, const double* alpha,
+                     double* const* f) {
   const bool z_only = c->imp == 2;
-  // the z-only form has no x/y boundary planes (main.f90:317-329 deallocates them)
-  int rc = do_updt_rhs_b(c, h.c_or_f, h.cbc, is_bound, z_only ? nullptr : h.rhsbx_d, z_only ? nullptr : h.rhsby_d,
-                         h.rhsbz_d, f, alpha, (int)sizeof(double));
-  if (rc) return rc;
-  if (z_only) return run_helmholtz_z(c, h.a_d, h.b_d, h.c_d, alpha, h.cbc + 4, h.c_or_f[2], f);
-  if (!h.plan || !h.lambdaxy_d) return fail(CANS_EINVAL, "cans_rk3_substep: Helmholtz system without a plan");
-  return run_helmholtz_3d<double>(c, h.plan, h.normfft, alpha, h.lambdaxy_d, h.a_d, h.b_d, h.c_d, h.cbc, h.c_or_f,
-                                  f);
+  HelmSys hs[CANS_MAX_SCALARS + 3];
+  for (int q = 0; q < n; q++) {
+    // the z-only form has no x/y boundary planes (main.f90:317-329 deallocates them)
+    int rc = do_updt_rhs_b(c, h[q]->c_or_f, h[q]->cbc, is_bound, z_only ? nullptr : h[q]->rhsbx_d,
+                           z_only ? nullptr : h[q]->rhsby_d, h[q]->rhsbz_d, f[q], alpha[q], (int)sizeof(double));
+    if (rc) return rc;
+    if (!z_only && (!h[q]->plan || !h[q]->lambdaxy_d))
+      return fail(CANS_EINVAL, "cans_rk3_substep: Helmholtz system without a plan");
+    hs[q] = HelmSys{h[q]->plan, h[q]->normfft, h[q]->lambdaxy_d, h[q]->a_d, h[q]->b_d, h[q]->c_d,
+                    h[q]->cbc, h[q]->c_or_f, f[q], alpha[q]};
+  }
+  return run_helmholtz_batch(c, n, hs);
 }
 
 // One substep.  With S = double (FP64) the updated velocity of rk lives in double
@@ -510,13 +516,24 @@ static int rk3_substep(cans_ctx* c, const cans_step_params* s, int irk, double d
   if (c->imp && delta)
     return fail(CANS_EUNSUPPORTED, "cans_rk3_substep: implicit diffusion needs a CANS_FP64 context");
   // scalars first, from the old velocity (main.f90:440-450): rk_scal, bulk_forcing_s,
-  // [solve_helmholtz], boundp
+  // [solve_helmholtz], boundp.  A scalar's step reads only the velocity and itself, so
+  // all of them run rk_scal first and then share one batched Helmholtz solve.
+  if (s->nscal > CANS_MAX_SCALARS) return fail(CANS_EINVAL, "cans_rk3_substep: too many scalars");
+  const cans_helmholtz* hsc[CANS_MAX_SCALARS];
+  double asc[CANS_MAX_SCALARS];
+  double* fsc[CANS_MAX_SCALARS];
   for (int i = 0; i < s->nscal; i++) {
     const cans_scalar& sc = s->scal[i];
     if ((rc = scal_core(c, rkpar, s->dli, s->dzci_d, s->dzfi_d, s->grid_vol_ratio_f_d, sc.alpha, dt, u, v, w,
                         sc.is_forced, sc.scalf, sc.source, (double*)sc.dsdtrko, (double*)sc.val, i, 1)))
       return rc;
-    if (c->imp && (rc = helmholtz(c, sc.h, s->is_bound, -0.5 * sc.alpha * dtrk, (double*)sc.val))) return rc;
+    hsc[i] = &sc.h;
+    asc[i] = -0.5 * sc.alpha * dtrk;
+    fsc[i] = (double*)sc.val;
+  }
+  if (c->imp && s->nscal && (rc = helmholtz(c, s->nscal, hsc, s->is_bound, asc, fsc))) return rc;
+  for (int i = 0; i < s->nscal; i++) {
+    const cans_scalar& sc = s->scal[i];
     if ((rc = plan_boundp(c, sc.cbc, sc.bc, s->nb, s->is_bound, s->dl, s->dzc_h, sc.val, CANS_HALO_ABSENT,
                           (int)sizeof(double))))
       return rc;
@@ -529,9 +546,14 @@ static int rk3_substep(cans_ctx* c, const cans_step_params* s, int irk, double d
   if (rc) return rc;
   for (int k = 0; k < 3 && !delta; k++)
     if (s->is_forced[k] && (rc = launch_add_forcing<double>(c, t[k], t[k], k, 1))) return rc;
-  // implicit diffusion (main.f90:453-462): one Helmholtz solve per component, alpha = -visc*dtrk/2
-  for (int k = 0; k < 3 && c->imp; k++)
-    if ((rc = helmholtz(c, s->hvel[k], s->is_bound, -.5 * s->visc * dtrk, t[k]))) return rc;
+  // implicit diffusion (main.f90:453-462): the three components' Helmholtz systems, alpha = -visc*dtrk/2,
+  // solved as one batch
+  if (c->imp) {
+    const cans_helmholtz* hv[3] = {&s->hvel[0], &s->hvel[1], &s->hvel[2]};
+    const double av = -.5 * s->visc * dtrk;
+    const double aa3[3] = {av, av, av};
+    if ((rc = helmholtz(c, 3, hv, s->is_bound, aa3, t))) return rc;
+  }
   if (delta)
     rc = plan_bounduvw(c, s->cbcvel, s->bcvel, s->nb, s->is_bound, s->dl, s->dzc_h, s->dzf_h, e[0], e[1], e[2], 0,
                        CANS_HALO_LOWER, CANS_HALO_ABSENT, base, fvec);

@@ -54,7 +54,16 @@ struct ScalLaunch {
   int imp = 0;  // 0 explicit, 1 implicit diffusion, 2 implicit in z only
 };
 int launch_scal_rk(cans_ctx* c, const ScalLaunch& m);
-// implicit-diffusion Helmholtz solves of an FP64 field (solver.hip)
+// implicit-diffusion Helmholtz solves of FP64 fields (solver.hip): one system each, or a batch
+struct HelmSys {
+  const cans_plan* plan;  // nullptr in the z-only form
+  double normfft;
+  const double *lam, *a, *b, *c;
+  const char *cbc, *c_or_f;
+  double* fld;
+  double alpha;
+};
+int run_helmholtz_batch(cans_ctx* c, int nsys, const HelmSys* hs);
 int run_helmholtz_z(cans_ctx* c, const double* a, const double* b, const double* cc, double alpha, const char* bcz,
                     char c_or_f_z, double* fld);
 template <typename T>

@@ -105,6 +105,7 @@ __device__ __forceinline__ cplx<double> merge_pp(double are, double aim, double 
 // ---------------------------------------------------------------------------
 constexpr int ZT = 64;
 constexpr int ZK = 16;
+constexpr int NBATCH = 4;  // systems one batched solve carries (blockIdx.y of the passes)
 constexpr int CK = 16;  // z pass: rows per checkpoint block (ZK is a multiple)
 static_assert(ZK % CK == 0, "tile row blocks hold whole checkpoint blocks");
 __host__ __device__ __forceinline__ int wnic(int n1) { return (n1 + ZT - 1) / ZT; }
@@ -137,8 +138,8 @@ struct RowPtr {
 template <typename T>
 struct XArgs {
   Layout L;
-  T* fld;          // halo'd field (origin)
-  T* W;            // z-tiled work buffer (woff)
+  T* fld[NBATCH];  // halo'd fields (origin), one per system of a batch (blockIdx.y)
+  T* W[NBATCH];    // their z-tiled work buffers (woff)
   int N, logN, Tn, pairs, kind;
   const cplx<double>* tw;
   const cplx<double>* mk;
@@ -186,6 +187,8 @@ __device__ __forceinline__ d2 div_pair(const XArgs<T>& a, long r, int e) {
 // so that the plain forward pass carries none of the divergence code's registers.
 template <int DIR, typename T, int DV = 0>
 __device__ __forceinline__ void xpass_body(const XArgs<T>& a) {
+  T* const fld = a.fld[blockIdx.y];
+  T* const W = a.W[blockIdx.y];
   extern __shared__ unsigned char smem_raw[];
   cplx<T>* smem = (cplx<T>*)smem_raw;  // LDS line in the storage type
   const int N = a.N, Tn = a.Tn, kind = a.kind;
@@ -198,13 +201,13 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& a) {
   const Layout& L = a.L;
   auto frow = [&](bool v, long r) {
     RowPtr<T> q;
-    if (v) q.p = a.fld + L.at(1, (int)(r % L.n2) + 1, (int)(r / L.n2) + 1);
+    if (v) q.p = fld + L.at(1, (int)(r % L.n2) + 1, (int)(r / L.n2) + 1);
     return q;
   };
   auto wrow = [&](bool v, long r) {
     RowPtr<T> q;
     if (v) {
-      q.p = a.W;
+      q.p = W;
       q.j = (int)(r % L.n2);
       q.k = (int)(r / L.n2);
       q.n2 = L.n2;
@@ -357,7 +360,7 @@ __global__ void __launch_bounds__(512) k_xpass_d(XArgs<double> a) {
 template <typename T>
 struct YArgs {
   int n1, n2, n3;
-  T* W;            // z-tiled work buffer (woff)
+  T* W[NBATCH];    // z-tiled work buffers (woff), one per system of a batch (blockIdx.y)
   int N, logN, Tn, lines, kind;
   const cplx<double>* tw;
   const cplx<double>* mk;
@@ -377,7 +380,8 @@ __device__ __forceinline__ void ypass_body(const YArgs<T>& a) {
   const IxStride ix{c, LN};
   // element e (along y) of column ia in the tiled work buffer; columns ia, ia+1 are adjacent
   const int nic = wnic(a.n1);
-  auto col = [&](int e) { return a.W + woff(ia, e, k, a.n2, nic); };
+  T* const W = a.W[blockIdx.y];
+  auto col = [&](int e) { return W + woff(ia, e, k, a.n2, nic); };
   // (A, B) pair of element e of this line
   auto ld = [&](int e) -> d2 { return vb ? ld2(col(e)) : d2((double)*col(e), 0.); };
   auto st = [&](int e, double x, double y) {
@@ -511,19 +515,35 @@ void launch_ypass<-1, double>(dim3 g, int t, size_t shm, hipStream_t st, const Y
 // the backward sweep re-reads the rhs of one block, recomputes the block's
 // d/rhs from its checkpoint (bit-identical: same operations) and substitutes.
 // ---------------------------------------------------------------------------
+// one system of a (batched) solve, as the caller gives it
 template <typename T>
-struct ZArgs {
-  T* W;            // z-tiled work buffer (woff)
-  long nl;         // number of lines (n1*n2)
-  int n;           // rows of the system (n3 - q)
-  int n1, n2, nic;  // tile geometry of the work buffer
-  long PS;          // elements per row block
-  int periodic;
+struct SolveSys {
+  T* fld;
+  double normfft;
+  const double *lam, *a, *b, *c;
+  const char* bc;      // cbc(0:1,1:3) of the variable
+  const char* c_or_f;  // (3)
+};
+
+// one system of a (batched) z pass: its work buffer, rows, normalisation, coefficients
+template <typename T>
+struct ZSys {
+  T* W;                     // z-tiled work buffer (woff)
+  int n;                    // rows of the system (n3 - q)
   double norm;
-  double pin_eps;  // relative pivot tolerance of the zero-mode pin
   const double *a, *b, *c, *lam;
   double *ckd, *ckp, *ckq;  // checkpoints [nblk][nl]
-  double* X2;               // periodic: auxiliary solution p2 [nn][nl]
+};
+
+template <typename T>
+struct ZArgs {
+  ZSys<T> sys[NBATCH];      // blockIdx.y selects the system
+  long nl;                  // number of lines (n1*n2)
+  int n1, n2, nic;          // tile geometry of the work buffer
+  long PS;                  // elements per row block
+  int periodic;
+  double pin_eps;           // relative pivot tolerance of the zero-mode pin
+  double* X2;               // periodic (single system): auxiliary solution p2 [nn][nl]
 };
 
 // one forward elimination row (solver.f90:147-170, with the p2 recurrence of 235-245)
@@ -597,19 +617,20 @@ __device__ __forceinline__ void z_row(int k, int nn, int periodic, double l, dou
 // kernel carries no Sherman-Morrison state in registers
 template <typename T, bool PER>
 __global__ void __launch_bounds__(256) k_gaussel(ZArgs<T> z) {
+  const ZSys<T>& zs = z.sys[blockIdx.y];
   const long line = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (line >= z.nl) return;
-  const double l = z.lam[line];
+  const double l = zs.lam[line];
   const double eps = z.pin_eps;
-  T* P = z.W + woff((int)(line % z.n1), (int)(line / z.n1), 0, z.n2, z.nic);  // row 0
+  T* P = zs.W + woff((int)(line % z.n1), (int)(line / z.n1), 0, z.n2, z.nic);  // row 0
   const long PS = z.PS;
   constexpr int periodic = PER ? 1 : 0;
-  const int nn = periodic ? z.n - 1 : z.n;
+  const int nn = periodic ? zs.n - 1 : zs.n;
   const int nblk = (nn + CK - 1) / CK;
-  const kcoef a = (kcoef)z.a;
-  const kcoef b = (kcoef)z.b;
-  const kcoef c = (kcoef)z.c;
-  const double norm = z.norm;
+  const kcoef a = (kcoef)zs.a;
+  const kcoef b = (kcoef)zs.b;
+  const kcoef c = (kcoef)zs.c;
+  const double norm = zs.norm;
   // ---- forward sweep (checkpoints only), next chunk's loads issued before this chunk's recurrence ----
   ZRow s = {0., 0., 0.};
   double rr[CK], rn[CK];
@@ -626,9 +647,9 @@ __global__ void __launch_bounds__(256) k_gaussel(ZArgs<T> z) {
     for (int t = 0; t < CK; t++)
       if (k0 + t <= nn) z_row(k0 + t, nn, periodic, l, rr[t], norm, a, b, c, eps, s);
     const long o = (long)bb * z.nl + line;
-    z.ckd[o] = s.d;
-    z.ckp[o] = s.pp;
-    if (periodic) z.ckq[o] = s.q2;
+    zs.ckd[o] = s.d;
+    zs.ckp[o] = s.pp;
+    if (periodic) zs.ckq[o] = s.q2;
 #pragma unroll
     for (int t = 0; t < CK; t++) rr[t] = rn[t];
   }
@@ -650,9 +671,9 @@ __global__ void __launch_bounds__(256) k_gaussel(ZArgs<T> z) {
     ZRow r0 = {0., 0., 0.};
     if (bb > 0) {
       const long o = (long)(bb - 1) * z.nl + line;
-      r0.d = z.ckd[o];
-      r0.pp = z.ckp[o];
-      r0.q2 = periodic ? z.ckq[o] : 0.;
+      r0.d = zs.ckd[o];
+      r0.pp = zs.ckp[o];
+      r0.q2 = periodic ? zs.ckq[o] : 0.;
     }
     double dd[CK], pv[CK], qv[CK];
 #pragma unroll
@@ -716,19 +737,20 @@ __global__ void __launch_bounds__(256) k_gaussel(ZArgs<T> z) {
 // the same operations as k_gaussel; FAST replaces the divisions by rcp64<true>.
 template <typename T, bool FAST>
 __global__ void __launch_bounds__(256) k_thomas(ZArgs<T> z) {
+  const ZSys<T>& zs = z.sys[blockIdx.y];
   const long line = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (line >= z.nl) return;
-  const double l = z.lam[line];
-  const double eps = z.pin_eps, norm = z.norm;
-  T* P = z.W + woff((int)(line % z.n1), (int)(line / z.n1), 0, z.n2, z.nic);  // row 0
+  const double l = zs.lam[line];
+  const double eps = z.pin_eps, norm = zs.norm;
+  T* P = zs.W + woff((int)(line % z.n1), (int)(line / z.n1), 0, z.n2, z.nic);  // row 0
   const long PS = z.PS;
   // first row of checkpoint block bb; its rows follow at stride ZT
   auto blk = [&](int bb) { return P + zrow(bb * CK, PS); };
-  const int nn = z.n;
+  const int nn = zs.n;
   const int nblk = (nn + CK - 1) / CK;
-  const kcoef a = (kcoef)z.a;
-  const kcoef b = (kcoef)z.b;
-  const kcoef c = (kcoef)z.c;
+  const kcoef a = (kcoef)zs.a;
+  const kcoef b = (kcoef)zs.b;
+  const kcoef c = (kcoef)zs.c;
   // ---- forward sweep: checkpoints (d, p') at the end of every block but the last ----
   double d = 0., pp = 0.;
   T rr[CK], rn[CK];
@@ -760,8 +782,8 @@ __global__ void __launch_bounds__(256) k_thomas(ZArgs<T> z) {
         z_row_mid<FAST>(l, (double)rr[t], norm, a[k0 + t - 1], b[k0 + t - 1], c[k0 + t - 1], d, pp);
     }
     const long o = (long)bb * z.nl + line;
-    z.ckd[o] = d;
-    z.ckp[o] = pp;
+    zs.ckd[o] = d;
+    zs.ckp[o] = pp;
 #pragma unroll
     for (int t = 0; t < CK; t++) rr[t] = rn[t];
   }
@@ -781,8 +803,8 @@ __global__ void __launch_bounds__(256) k_thomas(ZArgs<T> z) {
 #pragma unroll
       for (int t = 0; t < CK; t++) rn[t] = Bp[t * ZT];
       const long o = (long)(bb - 1) * z.nl + line;
-      d0 = z.ckd[o];
-      p0 = z.ckp[o];
+      d0 = zs.ckd[o];
+      p0 = zs.ckp[o];
     }
     double dd[CK], pv[CK];
     if (bb == nblk - 1) {
@@ -1002,13 +1024,16 @@ static int upload_tables(cans_plan* p, int d) {
 }
 
 template <typename T>
-static int xpass(cans_ctx* c, const cans_plan* p, T* fld, T* W, int dir, const DivSrc<T>* div = nullptr) {
+static int xpass(cans_ctx* c, const cans_plan* p, T* const* fld, T* const* W, int nsys, int dir,
+                 const DivSrc<T>* div = nullptr) {
   XArgs<T> a;
   a.div = div != nullptr && dir > 0;
   if (a.div) a.ds = *div;
   a.L = c->L;
-  a.fld = fld;
-  a.W = W;
+  for (int q = 0; q < nsys; q++) {
+    a.fld[q] = fld[q];
+    a.W[q] = W[q];
+  }
   a.N = p->len[0];
   a.logN = p->logn[0];
   a.Tn = a.N >= 8 ? a.N / 8 : 1;
@@ -1018,7 +1043,7 @@ static int xpass(cans_ctx* c, const cans_plan* p, T* fld, T* W, int dir, const D
   a.mk = p->mk[0];
   const long nrows = (long)c->L.n2 * c->L.n3;
   const long npairs = (nrows + 1) / 2;
-  dim3 grid(cdiv(npairs, a.pairs));
+  dim3 grid(cdiv(npairs, a.pairs), nsys);
   const size_t shm = (size_t)a.pairs * padded_len(a.N) * sizeof(cplx<T>);
   if constexpr (sizeof(T) == sizeof(float)) {
     if (dir < 0) k_xpass_f<-1, 0><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
@@ -1034,12 +1059,12 @@ static int xpass(cans_ctx* c, const cans_plan* p, T* fld, T* W, int dir, const D
 }
 
 template <typename T>
-static int ypass(cans_ctx* c, const cans_plan* p, T* W, int dir) {
+static int ypass(cans_ctx* c, const cans_plan* p, T* const* W, int nsys, int dir) {
   YArgs<T> a;
   a.n1 = c->L.n1;
   a.n2 = c->L.n2;
   a.n3 = c->L.n3;
-  a.W = W;
+  for (int q = 0; q < nsys; q++) a.W[q] = W[q];
   a.N = p->len[1];
   a.logN = p->logn[1];
   a.Tn = a.N >= 8 ? a.N / 8 : 1;
@@ -1053,7 +1078,7 @@ static int ypass(cans_ctx* c, const cans_plan* p, T* W, int dir) {
   a.tw = p->tw[1];
   a.mk = p->mk[1];
   const int ncolblk = (a.n1 + 2 * lines - 1) / (2 * lines);
-  dim3 grid((unsigned)ncolblk * a.n3);
+  dim3 grid((unsigned)ncolblk * a.n3, nsys);
   const size_t shm = (size_t)lines * a.N * sizeof(cplx<T>);
   if (dir > 0) launch_ypass<1, T>(grid, lines * a.Tn, shm, c->stream, a);
   else launch_ypass<-1, T>(grid, lines * a.Tn, shm, c->stream, a);
@@ -1061,44 +1086,51 @@ static int ypass(cans_ctx* c, const cans_plan* p, T* W, int dir) {
   return CANS_OK;
 }
 
+// Solve of nsys systems that share the x and y transforms (the plan p): each pass
+// is one launch for all of them (blockIdx.y = system).  The z systems may differ
+// (coefficients, c/f grid, normalisation); their z boundary pairs must agree on
+// periodicity.  nsys = 1 is the plain solver.
 template <typename T>
-int run_solver(cans_ctx* c, const cans_plan* p, double normfft, const double* lam, const double* a,
-               const double* b, const double* cc, const char* bc, const char* c_or_f, T* fld,
-               const DivSrc<T>* div) {
+int run_solver_batch(cans_ctx* c, const cans_plan* p, int nsys, const SolveSys<T>* sy, const DivSrc<T>* div) {
   const Layout& L = c->L;
   const long N = L.cells();
   const long nl = (long)L.n1 * L.n2;
+  if (nsys < 1 || nsys > NBATCH) return fail(CANS_EINVAL, "run_solver: batch size out of range");
+  const int periodic = sy[0].bc[4] == 'P' && sy[0].bc[5] == 'P';
+  if (nsys > 1 && (periodic || div || c->dtdma > 1))
+    return fail(CANS_EUNSUPPORTED, "run_solver: batches need a non-periodic z and no fused divergence or slabs");
+  const long wel = wbuf_elems(L.n1, L.n2, L.n3);
   void* W = nullptr;
-  int rc = ensure_work(c, 1, (size_t)wbuf_elems(L.n1, L.n2, L.n3) * sizeof(T), &W);
+  int rc = ensure_work(c, 1, (size_t)nsys * wel * sizeof(T), &W);
   if (rc) return rc;
-  const int q = (c_or_f[2] == 'f' && bc[5] == 'D') ? 1 : 0;
-  const int periodic = bc[4] == 'P' && bc[5] == 'P';
-  const int n = L.n3 - q;
-  const int nn = periodic ? n - 1 : n;
-  if (nn < 1) return fail(CANS_EUNSUPPORTED, "cans_solver: fewer than 2 z rows");
-  const long nblk = (nn + CK - 1) / CK;
+  if ((periodic ? L.n3 - 1 : L.n3) < 1) return fail(CANS_EUNSUPPORTED, "cans_solver: fewer than 2 z rows");
+  const long nblk = (L.n3 + CK - 1) / CK;  // enough checkpoint blocks for any system of the batch
   void* ck = nullptr;
-  size_t ckbytes = (size_t)3 * nblk * nl * sizeof(double) + (periodic ? (size_t)N * sizeof(double) : 0);
+  const size_t ckper = (size_t)3 * nblk * nl;
+  size_t ckbytes = (size_t)nsys * ckper * sizeof(double) + (periodic ? (size_t)N * sizeof(double) : 0);
   rc = ensure_work(c, 2, ckbytes, &ck);
   if (rc) return rc;
-  // the transform passes move element pairs: interior rows must start on pair boundaries
-  if (L.n1 % 2 || L.sy % 2 || ((uintptr_t)(fld + 1) & (2 * sizeof(T) - 1)))
-    return fail(CANS_EINVAL, "cans_solver: rows of p must start aligned to an element pair (n1 and ld even, "
-                             "field from cans_field_alloc or with element (1,0,0) on a pair boundary)");
+  T* fld[NBATCH];
+  T* Wq[NBATCH];
+  for (int q = 0; q < nsys; q++) {
+    fld[q] = sy[q].fld;
+    Wq[q] = (T*)W + q * wel;
+    // the transform passes move element pairs: interior rows must start on pair boundaries
+    if (L.n1 % 2 || L.sy % 2 || ((uintptr_t)(fld[q] + 1) & (2 * sizeof(T) - 1)))
+      return fail(CANS_EINVAL, "cans_solver: rows of p must start aligned to an element pair (n1 and ld even, "
+                               "field from cans_field_alloc or with element (1,0,0) on a pair boundary)");
+  }
   if (div && (p->kind[0] != K_PP || p->len[0] < 2))
     return fail(CANS_EINVAL, "run_solver: the velocity-divergence input needs a periodic x transform");
-  if ((rc = xpass<T>(c, p, fld, (T*)W, +1, div))) return rc;
-  if ((rc = ypass<T>(c, p, (T*)W, +1))) return rc;
+  if ((rc = xpass<T>(c, p, fld, Wq, nsys, +1, div))) return rc;
+  if ((rc = ypass<T>(c, p, Wq, nsys, +1))) return rc;
   ZArgs<T> z;
-  z.W = (T*)W;
   z.nl = nl;
-  z.n = n;
   z.n1 = L.n1;
   z.n2 = L.n2;
   z.nic = wnic(L.n1);
   z.PS = wplane(L.n1, L.n2);
   z.periodic = periodic;
-  z.norm = normfft;
   // The reference pins the last unknown of the zero-wavenumber line when its final
   // pivot is within one ulp of zero (solver.f90:151-160).  FP64 storage keeps that
   // test exactly.  With FP32 storage the rhs of that singular line is compatible only
@@ -1106,21 +1138,29 @@ int run_solver(cans_ctx* c, const cans_plan* p, double normfft, const double* la
   // solution; the tolerance is widened to 4096 ulp (rounding accumulated over the
   // elimination), still ~1e8 below the smallest regular pivot of any other line.
   z.pin_eps = 2.220446049250313e-16 * (sizeof(T) == sizeof(double) ? 1. : 4096.);
-  z.a = a;
-  z.b = b;
-  z.c = cc;
-  z.lam = lam;
-  z.ckd = (double*)ck;
-  z.ckp = z.ckd + nblk * nl;
-  z.ckq = z.ckp + nblk * nl;
-  z.X2 = periodic ? z.ckq + nblk * nl : nullptr;
+  for (int q = 0; q < nsys; q++) {
+    ZSys<T>& zs = z.sys[q];
+    const int qq = (sy[q].c_or_f[2] == 'f' && sy[q].bc[5] == 'D') ? 1 : 0;
+    zs.W = Wq[q];
+    zs.n = L.n3 - qq;
+    zs.norm = sy[q].normfft;
+    zs.a = sy[q].a;
+    zs.b = sy[q].b;
+    zs.c = sy[q].c;
+    zs.lam = sy[q].lam;
+    zs.ckd = (double*)ck + q * ckper;
+    zs.ckp = zs.ckd + nblk * nl;
+    zs.ckq = zs.ckp + nblk * nl;
+  }
+  z.X2 = periodic ? z.sys[0].ckq + nblk * nl : nullptr;
+  const int nn = periodic ? z.sys[0].n - 1 : z.sys[0].n;
   if (periodic) {
     k_gaussel<T, true><<<cdiv(nl, 256), 256, 0, c->stream>>>(z);
     CANS_LAUNCH_CHECK("k_gaussel");
   } else if (c->dtdma > 1) {
     if (nn / c->dtdma < 3) return fail(CANS_EINVAL, "cans_solver: distributed z solve needs 3 rows per slab");
     DtArgs<T> d;
-    d.W = (T*)W;
+    d.W = Wq[0];
     d.nl = nl;
     d.n1 = L.n1;
     d.n2 = L.n2;
@@ -1128,11 +1168,11 @@ int run_solver(cans_ctx* c, const cans_plan* p, double normfft, const double* la
     d.PS = z.PS;
     d.n = nn;
     d.nslab = c->dtdma;
-    d.norm = normfft;
-    d.a = a;
-    d.b = b;
-    d.c = cc;
-    d.lam = lam;
+    d.norm = sy[0].normfft;
+    d.a = sy[0].a;
+    d.b = sy[0].b;
+    d.c = sy[0].c;
+    d.lam = sy[0].lam;
     void* dt = nullptr;
     rc = ensure_work(c, 6, ((size_t)3 * nn + (size_t)6 * d.nslab) * nl * sizeof(double), &dt);
     if (rc) return rc;
@@ -1150,12 +1190,20 @@ int run_solver(cans_ctx* c, const cans_plan* p, double normfft, const double* la
     k_dtdma_back<T><<<gs, 256, 0, c->stream>>>(d);
     CANS_LAUNCH_CHECK("k_dtdma_back");
   } else {
-    k_thomas<T, sizeof(T) == sizeof(float)><<<cdiv(nl, 256), 256, 0, c->stream>>>(z);
+    k_thomas<T, sizeof(T) == sizeof(float)><<<dim3(cdiv(nl, 256), nsys), 256, 0, c->stream>>>(z);
     CANS_LAUNCH_CHECK("k_thomas");
   }
-  if ((rc = ypass<T>(c, p, (T*)W, -1))) return rc;
-  if ((rc = xpass<T>(c, p, fld, (T*)W, -1))) return rc;
+  if ((rc = ypass<T>(c, p, Wq, nsys, -1))) return rc;
+  if ((rc = xpass<T>(c, p, fld, Wq, nsys, -1))) return rc;
   return CANS_OK;
+}
+
+template <typename T>
+int run_solver(cans_ctx* c, const cans_plan* p, double normfft, const double* lam, const double* a,
+               const double* b, const double* cc, const char* bc, const char* c_or_f, T* fld,
+               const DivSrc<T>* div) {
+  SolveSys<T> s1 = {fld, normfft, lam, a, b, cc, bc, c_or_f};
+  return run_solver_batch<T>(c, p, 1, &s1, div);
 }
 
 // ---------------------------------------------------------------------------
@@ -1178,8 +1226,19 @@ __global__ void k_shift_diag(const double* b, double alphai, double* bb, int n) 
 // batch's loads issued before its dependent arithmetic.
 constexpr int ZB = 8;
 
-__global__ void __launch_bounds__(256) k_z1_solve(Layout L, double* p, const double* a, const double* b,
-                                                  const double* c, double alphai, int nn, double norm) {
+struct Z1Args {
+  Layout L;
+  double* p[NBATCH];
+  const double *a[NBATCH], *b[NBATCH], *c[NBATCH];
+  double alphai[NBATCH];
+  int nn[NBATCH];
+};
+
+__global__ void __launch_bounds__(256) k_z1_solve(Z1Args z) {
+  const int sys = blockIdx.z;
+  const int nn = z.nn[sys];
+  const double alphai = z.alphai[sys], norm = alphai;
+  const double *a = z.a[sys], *b = z.b[sys], *c = z.c[sys];
   extern __shared__ double zsh[];
   double* la = zsh;           // a(k)
   double* lb = la + nn;       // bb(k), then z(k)
@@ -1191,22 +1250,23 @@ __global__ void __launch_bounds__(256) k_z1_solve(Layout L, double* p, const dou
   }
   __syncthreads();
   if (threadIdx.x == 0) {  // solver.f90:170-185 (lambda absent): z and d once for all lines
-    double z = 1. / lb[0];
-    lb[0] = z;
-    double d = lc[0] * z;
+    double zz = 1. / lb[0];
+    lb[0] = zz;
+    double d = lc[0] * zz;
     lc[0] = d;
     for (int k = 1; k < nn; k++) {
-      z = 1. / (lb[k] - la[k] * d);
-      lb[k] = z;
-      d = lc[k] * z;
+      zz = 1. / (lb[k] - la[k] * d);
+      lb[k] = zz;
+      d = lc[k] * zz;
       lc[k] = d;
     }
   }
   __syncthreads();
+  const Layout& L = z.L;
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int j = blockIdx.y + 1;
   if (i > L.n1) return;
-  double* P = p + L.at(i, j, 1);  // row k (1-based) at P + (k-1)*sz
+  double* P = z.p[sys] + L.at(i, j, 1);  // row k (1-based) at P + (k-1)*sz
   const long sz = L.sz;
   double prev = 0.;
   for (int k0 = 0; k0 < nn; k0 += ZB) {
@@ -1238,18 +1298,74 @@ __global__ void __launch_bounds__(256) k_z1_solve(Layout L, double* p, const dou
   }
 }
 
+// batched Helmholtz solves of FP64 fields (the rhs in, the solution out): the z-only
+// form (is_impdiff_1d) as one k_z1_solve launch for all systems; the 3-D form as one
+// batched solver when the systems share their x/y transform kinds, else one by one
+int run_helmholtz_batch(cans_ctx* c, int nsys, const HelmSys* hs) {
+  const Layout& L = c->L;
+  if (nsys < 1) return CANS_OK;
+  if (nsys > NBATCH) {
+    int rc = run_helmholtz_batch(c, NBATCH, hs);
+    return rc ? rc : run_helmholtz_batch(c, nsys - NBATCH, hs + NBATCH);
+  }
+  if (c->imp == 2) {
+    Z1Args z;
+    z.L = L;
+    int nnmax = 0;
+    for (int q = 0; q < nsys; q++) {
+      const char* bcz = hs[q].cbc + 4;
+      if (bcz[0] == 'P' && bcz[1] == 'P')
+        return fail(CANS_EUNSUPPORTED, "implicit z diffusion with periodic z is not built on the device");
+      const int qq = (hs[q].c_or_f[2] == 'f' && bcz[1] == 'D') ? 1 : 0;
+      z.p[q] = hs[q].fld;
+      z.a[q] = hs[q].a;
+      z.b[q] = hs[q].b;
+      z.c[q] = hs[q].c;
+      z.alphai[q] = 1. / hs[q].alpha;
+      z.nn[q] = L.n3 - qq;
+      nnmax = std::max(nnmax, z.nn[q]);
+    }
+    const size_t shm = (size_t)3 * nnmax * sizeof(double);
+    if (shm > 160 * 1024) return fail(CANS_EUNSUPPORTED, "implicit z diffusion: too many z rows for LDS tables");
+    k_z1_solve<<<dim3(cdiv(L.n1, 256), L.n2, nsys), 256, shm, c->stream>>>(z);
+    CANS_LAUNCH_CHECK("k_z1_solve");
+    return CANS_OK;
+  }
+  bool same = true;
+  for (int q = 1; q < nsys; q++)
+    same = same && hs[q].plan && hs[0].plan && hs[q].plan->kind[0] == hs[0].plan->kind[0] &&
+           hs[q].plan->kind[1] == hs[0].plan->kind[1] && hs[q].plan->len[0] == hs[0].plan->len[0] &&
+           hs[q].plan->len[1] == hs[0].plan->len[1] && hs[q].cbc[4] != 'P';
+  if (!same || hs[0].cbc[4] == 'P' || c->dtdma > 1) {
+    for (int q = 0; q < nsys; q++) {
+      int rc = run_helmholtz_3d<double>(c, hs[q].plan, hs[q].normfft, hs[q].alpha, hs[q].lam, hs[q].a, hs[q].b,
+                                        hs[q].c, hs[q].cbc, hs[q].c_or_f, hs[q].fld);
+      if (rc) return rc;
+    }
+    return CANS_OK;
+  }
+  void* tab = nullptr;
+  int rc = ensure_work(c, 5, (size_t)NBATCH * L.n3 * sizeof(double), &tab);
+  if (rc) return rc;
+  SolveSys<double> sy[NBATCH];
+  for (int q = 0; q < nsys; q++) {
+    double* bb = (double*)tab + (long)q * L.n3;
+    const double alphai = 1. / hs[q].alpha;
+    k_shift_diag<<<cdiv(L.n3, 256), 256, 0, c->stream>>>(hs[q].b, alphai, bb, L.n3);
+    CANS_LAUNCH_CHECK("k_shift_diag");
+    sy[q] = SolveSys<double>{hs[q].fld, hs[q].normfft * alphai, hs[q].lam, hs[q].a, bb, hs[q].c, hs[q].cbc,
+                             hs[q].c_or_f};
+  }
+  return run_solver_batch<double>(c, hs[0].plan, nsys, sy, nullptr);
+}
+
 int run_helmholtz_z(cans_ctx* c, const double* a, const double* b, const double* cc, double alpha, const char* bcz,
                     char c_or_f_z, double* fld) {
-  const Layout& L = c->L;
-  if (bcz[0] == 'P' && bcz[1] == 'P')
-    return fail(CANS_EUNSUPPORTED, "implicit z diffusion with periodic z is not built on the device");
-  const int q = (c_or_f_z == 'f' && bcz[1] == 'D') ? 1 : 0;
-  const int nn = L.n3 - q;
-  const size_t shm = (size_t)3 * nn * sizeof(double);
-  if (shm > 160 * 1024) return fail(CANS_EUNSUPPORTED, "implicit z diffusion: too many z rows for LDS tables");
-  k_z1_solve<<<dim3(cdiv(L.n1, 256), L.n2), 256, shm, c->stream>>>(L, fld, a, b, cc, 1. / alpha, nn, 1. / alpha);
-  CANS_LAUNCH_CHECK("k_z1_solve");
-  return CANS_OK;
+  char cbc[6] = {'P', 'P', 'P', 'P', bcz[0], bcz[1]};
+  char cf[3] = {'c', 'c', c_or_f_z};
+  HelmSys h = {nullptr, 1., nullptr, a, b, cc, cbc, cf, fld, alpha};
+  if (c->imp != 2) return fail(CANS_EINVAL, "run_helmholtz_z: z-only form needs is_impdiff_1d");
+  return run_helmholtz_batch(c, 1, &h);
 }
 
 template <typename T>
