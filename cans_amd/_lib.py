@@ -82,6 +82,7 @@ SIGNATURES = {
     "cans_tridmatrix": (C.c_int, [_cp, C.c_int, _dp, _dp, C.c_char, _dp, _dp, _dp]),
     "cans_fftini": (C.c_int, [_vp, _ip, _cp, _cp, C.POINTER(_vp), _dp]),
     "cans_fftend": (C.c_int, [_vp]),
+    "cans_fft_tuning": (C.c_int, [_vp, _ip]),
     "cans_solver": (C.c_int, [_vp, _ip, _ip, _vp, C.c_double, _vp, _vp, _vp, _vp, _cp, _cp, _vp]),
     "cans_solve_helmholtz": (C.c_int, [_vp, _ip, _ip, _vp, C.c_double, C.c_double, _vp, _vp, _vp, _vp,
                                        _vp, _vp, _vp, _ip, _cp, _cp, _vp]),

@@ -30,6 +30,9 @@ struct cans_plan {
   double normfft;
   cans::cplx<double>* tw[2];  // W_N^t = exp(-2 pi i t/N), t < N
   cans::cplx<double>* mk[2];  // exp(-i pi k/(2N)), k <= N (DCT/DST pre/post twiddles)
+  // launch shapes chosen by the tuner at cans_fftini, [0] FP32 storage, [1] FP64 (0: heuristic)
+  int xthreads[2] = {0, 0};   // x pass: threads per work-group (lines = threads / (N/8))
+  int ylines[2] = {0, 0};     // y pass: complex lines (2 columns each) per work-group
 };
 
 namespace cans {
@@ -1037,7 +1040,8 @@ static int xpass(cans_ctx* c, const cans_plan* p, T* const* fld, T* const* W, in
   a.N = p->len[0];
   a.logN = p->logn[0];
   a.Tn = a.N >= 8 ? a.N / 8 : 1;
-  a.pairs = a.Tn >= 256 ? 1 : 256 / a.Tn;
+  const int xt = p->xthreads[sizeof(T) == sizeof(double)] ? p->xthreads[sizeof(T) == sizeof(double)] : 256;
+  a.pairs = a.Tn >= xt ? 1 : xt / a.Tn;
   a.kind = p->kind[0];
   a.tw = p->tw[0];
   a.mk = p->mk[0];
@@ -1073,6 +1077,7 @@ static int ypass(cans_ctx* c, const cans_plan* p, T* const* W, int nsys, int dir
   if (sizeof(T) < sizeof(double) && lines < 16) lines = 16;  // 128-byte rows of FP32 columns
   while (lines > 1 && (size_t)lines * a.N * sizeof(cplx<T>) > 65536) lines /= 2;
   while ((long)lines * a.Tn > 1024) lines /= 2;
+  if (p->ylines[sizeof(T) == sizeof(double)]) lines = p->ylines[sizeof(T) == sizeof(double)];
   a.lines = lines;
   a.kind = p->kind[1];
   a.tw = p->tw[1];
@@ -1396,6 +1401,89 @@ template int run_solver<float>(cans_ctx*, const cans_plan*, double, const double
 
 using namespace cans;
 
+// ---------------------------------------------------------------------------
+// Launch-shape tuner.  The x pass's work-group size (lines per group) and the y
+// pass's lines per group trade LDS, registers and access width; the best choice
+// depends on the transform lengths, the storage type and the part.  cans_fftini
+// times the candidates on a zero-filled problem of the context's x-y shape with at
+// most 128 z-planes (the passes treat planes independently) and keeps the fastest
+// forward+backward pair per storage type.
+// ---------------------------------------------------------------------------
+template <typename T>
+static int tune_type(cans_ctx* c, cans_plan* p) {
+  const int slot = sizeof(T) == sizeof(double);
+  cans_ctx cs = *c;  // the context's layout with fewer planes; its scratch is not used
+  cs.L.n3 = std::min(c->L.n3, 128);
+  const Layout& L = cs.L;
+  const size_t fbytes = (size_t)L.sz * (L.n3 + 2) * sizeof(T) + 256;
+  const size_t wbytes = (size_t)wbuf_elems(L.n1, L.n2, L.n3) * sizeof(T);
+  void *fb = nullptr, *wb = nullptr;
+  CANS_HIP_TRY(hipMalloc(&fb, fbytes));
+  if (hipMalloc(&wb, wbytes) != hipSuccess) {
+    hipFree(fb);
+    return fail(CANS_ENOMEM, "cans_fftini: tuner scratch");
+  }
+  hipEvent_t e0, e1;
+  int rc = CANS_OK;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = fail(CANS_EHIP, "cans_fftini: events");
+  T* fld = (T*)((char*)fb + 256 - sizeof(T));  // element (1,0,0) on a 256-byte boundary
+  T* W = (T*)wb;
+  if (!rc && (hipMemsetAsync(fb, 0, fbytes, cs.stream) != hipSuccess ||
+              hipMemsetAsync(wb, 0, wbytes, cs.stream) != hipSuccess))
+    rc = fail(CANS_EHIP, "cans_fftini: tuner memset");
+  auto timed = [&](auto&& run) -> float {
+    float best = 1e30f;
+    for (int rep = 0; rep < 3 && !rc; rep++) {
+      hipEventRecord(e0, cs.stream);
+      if ((rc = run())) break;
+      hipEventRecord(e1, cs.stream);
+      hipEventSynchronize(e1);
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, e0, e1);
+      if (rep > 0) best = std::min(best, ms);  // the first run warms up
+    }
+    return best;
+  };
+  // x pass
+  const int tnx = p->len[0] >= 8 ? p->len[0] / 8 : 1;
+  float bx = 1e30f;
+  int xbest = 256;
+  for (int xt : {128, 256, 512}) {
+    if (xt < tnx && xt != 512) continue;
+    p->xthreads[slot] = xt;
+    const float ms = timed([&]() -> int {
+      int r = xpass<T>(&cs, p, &fld, &W, 1, +1);
+      return r ? r : xpass<T>(&cs, p, &fld, &W, 1, -1);
+    });
+    if (ms < bx) bx = ms, xbest = xt;
+  }
+  p->xthreads[slot] = xbest;
+  // y pass
+  const int tny = p->len[1] >= 8 ? p->len[1] / 8 : 1;
+  float by = 1e30f;
+  int ybest = 0;
+  for (int yl : {4, 8, 16, 32}) {
+    if ((long)yl * tny > 1024 || (size_t)yl * p->len[1] * sizeof(cplx<T>) > 65536) continue;
+    p->ylines[slot] = yl;
+    const float ms = timed([&]() -> int {
+      int r = ypass<T>(&cs, p, &W, 1, +1);
+      return r ? r : ypass<T>(&cs, p, &W, 1, -1);
+    });
+    if (ms < by) by = ms, ybest = yl;
+  }
+  p->ylines[slot] = ybest;
+  hipStreamSynchronize(cs.stream);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(fb);
+  hipFree(wb);
+  if (rc) {  // keep the heuristics
+    p->xthreads[slot] = 0;
+    p->ylines[slot] = 0;
+  }
+  return rc;
+}
+
 extern "C" int cans_fftini(cans_ctx* c, const int* ng, const char* bcxy, const char* c_or_f, cans_plan** out,
                            double* normfft) {
   if (!c || !ng || !bcxy || !c_or_f || !out) return fail(CANS_EINVAL, "cans_fftini: null argument");
@@ -1434,8 +1522,25 @@ extern "C" int cans_fftini(cans_ctx* c, const int* ng, const char* bcxy, const c
     }
   }
   p->normfft = 1. / nf;
+  // tune the pass shapes for the storage types this context uses (FP32 for pressure-like
+  // fields in the mixed mode; FP64 for everything else)
+  int rc = tune_type<double>(c, p);
+  if (!rc && c->precision == CANS_MIXED) rc = tune_type<float>(c, p);
+  if (rc) {
+    cans_fftend(p);
+    return rc;
+  }
   if (normfft) *normfft = p->normfft;
   *out = p;
+  return CANS_OK;
+}
+
+extern "C" int cans_fft_tuning(const cans_plan* p, int* shapes) {
+  if (!p || !shapes) return fail(CANS_EINVAL, "cans_fft_tuning: null argument");
+  shapes[0] = p->xthreads[0];
+  shapes[1] = p->xthreads[1];
+  shapes[2] = p->ylines[0];
+  shapes[3] = p->ylines[1];
   return CANS_OK;
 }
 

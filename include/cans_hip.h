@@ -124,6 +124,11 @@ int cans_tridmatrix(const char *bc, int n, const double *dzci_h, const double *d
 int cans_fftini(cans_ctx *ctx, const int *ng, const char *bcxy, const char *c_or_f,
                 cans_plan **arrplan, double *normfft);
 int cans_fftend(cans_plan *arrplan);
+/* The launch shapes the plan's tuner chose at cans_fftini (no counterpart in the
+ * reference, whose FFT libraries tune themselves): shapes[0..1] x-pass threads per
+ * work-group for FP32/FP64 storage, shapes[2..3] y-pass lines per work-group (0: not
+ * tuned for that type; a heuristic applies). */
+int cans_fft_tuning(const cans_plan *arrplan, int *shapes);
 
 /* ---------------------------------------------------------------------------
  * Operators.  Each replaces the reference routine cited; argument meaning and

@@ -55,3 +55,23 @@ def test_graph_replays_match_eager(precision):
             got = getattr(sim, k).numpy()
             assert np.array_equal(got, ref[r][k]), (precision, r, k, float(np.abs(got - ref[r][k]).max()))
         print("replay %d identical to eager" % r, flush=True)
+
+
+def test_plan_tuner_shapes(torch):
+    """cans_fftini times the pass launch shapes and keeps valid ones for the storage types in use."""
+    import ctypes as C
+    from cans_amd import _lib as LB
+    from cans_amd.host import Simulation
+    for prec in (LB.CANS_FP64, LB.CANS_MIXED):
+        p = cases.params("channel", ng=(64, 32, 48))
+        sim = Simulation(cases.host_case(p), precision=prec)
+        sh = (C.c_int * 4)()
+        LB.check(LB.lib().cans_fft_tuning(sim.plan.h, sh), "cans_fft_tuning")
+        xf, xd, yf, yd = list(sh)
+        print("[tuner] precision %d: x %d/%d y %d/%d" % (prec, xf, xd, yf, yd), flush=True)
+        assert xd in (128, 256, 512) and yd in (4, 8, 16, 32)
+        if prec == LB.CANS_MIXED:
+            assert xf in (128, 256, 512) and yf in (4, 8, 16, 32)
+        else:
+            assert xf == 0 and yf == 0
+        sim.close()
