@@ -50,6 +50,12 @@ class Scalar(C.Structure):
                 ("h", Helmholtz)]
 
 
+class PoissonBox(C.Structure):
+    """cans_poisson_box (include/cans_hip.h)."""
+    _fields_ = [("n", C.c_int * 3), ("sy", C.c_long), ("sz", C.c_long), ("cbc", C.c_char * 6),
+                ("lambdaxy_h", _vp), ("a_h", _vp), ("b_h", _vp), ("c_h", _vp), ("p_d", _vp)]
+
+
 class State(C.Structure):
     """cans_state (include/cans_hip.h)."""
     _fields_ = [("u", _vp), ("v", _vp), ("w", _vp), ("p", _vp), ("pp", _vp),
@@ -84,6 +90,9 @@ SIGNATURES = {
     "cans_fftend": (C.c_int, [_vp]),
     "cans_fft_tuning": (C.c_int, [_vp, _ip]),
     "cans_solver": (C.c_int, [_vp, _ip, _ip, _vp, C.c_double, _vp, _vp, _vp, _vp, _cp, _cp, _vp]),
+    "cans_poisson_batch_create": (C.c_int, [_vp, C.c_int, C.POINTER(PoissonBox), _dp, C.POINTER(_vp)]),
+    "cans_poisson_batch_solve": (C.c_int, [_vp]),
+    "cans_poisson_batch_destroy": (C.c_int, [_vp]),
     "cans_solve_helmholtz": (C.c_int, [_vp, _ip, _ip, _vp, C.c_double, C.c_double, _vp, _vp, _vp, _vp,
                                        _vp, _vp, _vp, _ip, _cp, _cp, _vp]),
     "cans_rk": (C.c_int, [_vp, _dp, _ip, _dp, _vp, _vp, _vp, _vp, C.c_double, C.c_double, _vp, _ip, _dp,

@@ -1,6 +1,6 @@
-// LDS-resident batched complex FFT building block (radix 8/4/2 Stockham).
+// LDS-resident batched complex FFT building block (radix 8/4/2/3 Stockham).
 //
-// One "team" of T threads transforms one line of N = 2^m complex values that
+// One "team" of T threads transforms one line of N = 2^m (or 3 * 2^m) complex values that
 // sits in LDS, addressed through an element map.  The LDS element type L may be
 // float or double, and the butterflies compute in L: FP32 lines give FP32
 // transforms (twiddles come from a double table, rounded once), FP64 lines FP64
@@ -117,78 +117,136 @@ __device__ __forceinline__ cplx<T> tmulc(cplx<T> a, cplx<T> b) {
 }
 
 // one Stockham stage: R radix, NBF butterflies per thread, arithmetic in L
+// act = false: a lane outside every team (work-groups whose size is not a multiple
+// of the team size); it takes part in the barriers only
 template <typename L, int s, int R, int NBF, class IX>
 __device__ __forceinline__ void fft_stage(cplx<L>* buf, IX ix, int N, int lNs, int lt, int Tn,
-                                          const cplx<double>* __restrict__ tw) {
+                                          const cplx<double>* __restrict__ tw, bool act = true) {
   typedef L T;
   constexpr int LR = R == 2 ? 1 : R == 4 ? 2 : 3;
   const int NR = N >> LR;
   const int Ns = 1 << lNs;
   cplx<T> v[NBF][R];
+  if (act) {
 #pragma unroll
-  for (int bb = 0; bb < NBF; bb++) {
-    const int j = lt + bb * Tn;
+    for (int bb = 0; bb < NBF; bb++) {
+      const int j = lt + bb * Tn;
 #pragma unroll
-    for (int r = 0; r < R; r++) v[bb][r] = buf[ix(j + r * NR)];
+      for (int r = 0; r < R; r++) v[bb][r] = buf[ix(j + r * NR)];
+    }
   }
   __syncthreads();
+  if (act) {
 #pragma unroll
-  for (int bb = 0; bb < NBF; bb++) {
-    const int j = lt + bb * Tn;
-    const int k = j & (Ns - 1);
-    if (k) {
-      const int step = k * (N >> (lNs + LR));  // k * N/(Ns*R)
-      // W^r for r = 1..R-1 from the table entries W^1, W^2, W^4 (at most one product
-      // deep), so only a few twiddles are live at a time
-      const cplx<T> w1 = {(T)tw[step].re, (T)tw[step].im};
-      if constexpr (R == 2) {
-        v[bb][1] = (s < 0) ? tmul(v[bb][1], w1) : tmulc(v[bb][1], w1);
-      } else {
-        const cplx<T> w2 = {(T)tw[2 * step].re, (T)tw[2 * step].im};
-        const cplx<T> w3 = tmul(w1, w2);
-        v[bb][1] = (s < 0) ? tmul(v[bb][1], w1) : tmulc(v[bb][1], w1);
-        v[bb][2] = (s < 0) ? tmul(v[bb][2], w2) : tmulc(v[bb][2], w2);
-        v[bb][3] = (s < 0) ? tmul(v[bb][3], w3) : tmulc(v[bb][3], w3);
-        if constexpr (R == 8) {
-          const cplx<T> w4 = {(T)tw[4 * step].re, (T)tw[4 * step].im};
-          const cplx<T> w5 = tmul(w4, w1), w6 = tmul(w4, w2), w7 = tmul(w4, w3);
-          v[bb][4] = (s < 0) ? tmul(v[bb][4], w4) : tmulc(v[bb][4], w4);
-          v[bb][5] = (s < 0) ? tmul(v[bb][5], w5) : tmulc(v[bb][5], w5);
-          v[bb][6] = (s < 0) ? tmul(v[bb][6], w6) : tmulc(v[bb][6], w6);
-          v[bb][7] = (s < 0) ? tmul(v[bb][7], w7) : tmulc(v[bb][7], w7);
+    for (int bb = 0; bb < NBF; bb++) {
+      const int j = lt + bb * Tn;
+      const int k = j & (Ns - 1);
+      if (k) {
+        const int step = k * (N >> (lNs + LR));  // k * N/(Ns*R)
+        // W^r for r = 1..R-1 from the table entries W^1, W^2, W^4 (at most one product
+        // deep), so only a few twiddles are live at a time
+        const cplx<T> w1 = {(T)tw[step].re, (T)tw[step].im};
+        if constexpr (R == 2) {
+          v[bb][1] = (s < 0) ? tmul(v[bb][1], w1) : tmulc(v[bb][1], w1);
+        } else {
+          const cplx<T> w2 = {(T)tw[2 * step].re, (T)tw[2 * step].im};
+          const cplx<T> w3 = tmul(w1, w2);
+          v[bb][1] = (s < 0) ? tmul(v[bb][1], w1) : tmulc(v[bb][1], w1);
+          v[bb][2] = (s < 0) ? tmul(v[bb][2], w2) : tmulc(v[bb][2], w2);
+          v[bb][3] = (s < 0) ? tmul(v[bb][3], w3) : tmulc(v[bb][3], w3);
+          if constexpr (R == 8) {
+            const cplx<T> w4 = {(T)tw[4 * step].re, (T)tw[4 * step].im};
+            const cplx<T> w5 = tmul(w4, w1), w6 = tmul(w4, w2), w7 = tmul(w4, w3);
+            v[bb][4] = (s < 0) ? tmul(v[bb][4], w4) : tmulc(v[bb][4], w4);
+            v[bb][5] = (s < 0) ? tmul(v[bb][5], w5) : tmulc(v[bb][5], w5);
+            v[bb][6] = (s < 0) ? tmul(v[bb][6], w6) : tmulc(v[bb][6], w6);
+            v[bb][7] = (s < 0) ? tmul(v[bb][7], w7) : tmulc(v[bb][7], w7);
+          }
         }
       }
-    }
-    bfly<T, s, R>(v[bb]);
-    const int idxD = ((j >> lNs) << (lNs + LR)) + k;
+      bfly<T, s, R>(v[bb]);
+      const int idxD = ((j >> lNs) << (lNs + LR)) + k;
 #pragma unroll
-    for (int r = 0; r < R; r++) buf[ix(idxD + r * Ns)] = v[bb][r];
+      for (int r = 0; r < R; r++) buf[ix(idxD + r * Ns)] = v[bb][r];
+    }
   }
   __syncthreads();
 }
 
-// full transform of one line (unnormalised), s = -1 forward, +1 inverse
+// radix 3: y_r = sum_q x_q W3^{r q}, W3 = exp(s 2 pi i/3) = -1/2 + s i sqrt(3)/2
+template <typename T, int s>
+__device__ __forceinline__ void bfly3(cplx<T>* v) {
+  const T h = (T)0.86602540378443864676;  // sqrt(3)/2
+  const cplx<T> t1 = cadd(v[1], v[2]);
+  const cplx<T> t2 = {v[0].re - (T)0.5 * t1.re, v[0].im - (T)0.5 * t1.im};
+  const cplx<T> d = csub(v[1], v[2]);
+  const cplx<T> t3 = mul_si<T, s>(cplx<T>{h * d.re, h * d.im});
+  v[0] = cadd(v[0], t1);
+  v[1] = cadd(t2, t3);
+  v[2] = csub(t2, t3);
+}
+
+// Last stage of a line of N = 3 * 2^m (m >= 3): radix 3 with span Ns = N/3.  The
+// N/3 butterflies do not split evenly over the team's N/8 threads; each thread takes
+// up to three of them (j = lt, lt + Tn, lt + 2 Tn below N/3).
+template <typename L, int s, class IX>
+__device__ __forceinline__ void fft_stage3(cplx<L>* buf, IX ix, int N, int lt, int Tn,
+                                           const cplx<double>* __restrict__ tw, bool act = true) {
+  typedef L T;
+  const int NR = N / 3;
+  cplx<T> v[3][3];
+#pragma unroll
+  for (int bb = 0; bb < 3; bb++) {
+    const int j = lt + bb * Tn;
+    if (act && j < NR) {
+#pragma unroll
+      for (int r = 0; r < 3; r++) v[bb][r] = buf[ix(j + r * NR)];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int bb = 0; bb < 3; bb++) {
+    const int j = lt + bb * Tn;
+    if (act && j < NR) {
+      if (j) {  // W_N^{r j}, r = 1, 2 (span Ns = N/3: k = j, step = j)
+        const cplx<T> w1 = {(T)tw[j].re, (T)tw[j].im};
+        const cplx<T> w2 = {(T)tw[2 * j].re, (T)tw[2 * j].im};
+        v[bb][1] = (s < 0) ? tmul(v[bb][1], w1) : tmulc(v[bb][1], w1);
+        v[bb][2] = (s < 0) ? tmul(v[bb][2], w2) : tmulc(v[bb][2], w2);
+      }
+      bfly3<T, s>(v[bb]);
+#pragma unroll
+      for (int r = 0; r < 3; r++) buf[ix(j + r * NR)] = v[bb][r];
+    }
+  }
+  __syncthreads();
+}
+
+// full transform of one line (unnormalised), s = -1 forward, +1 inverse.
+// N = 2^logN, or N = 3 * 2^logN with logN >= 3 when r3 is set (the power-of-two
+// stages run first with spans 1, 8, ..., then one radix-3 stage).
 template <typename L, int s, class IX>
 __device__ __forceinline__ void fft_line(cplx<L>* buf, IX ix, int N, int logN, int lt, int Tn,
-                         const cplx<double>* __restrict__ tw) {
-  if (logN == 0) return;
-  if (logN == 1) {
-    fft_stage<L, s, 2, 1>(buf, ix, N, 0, lt, Tn, tw);
+                         const cplx<double>* __restrict__ tw, int r3 = 0, bool act = true) {
+  if (logN == 0 && !r3) return;
+  if (logN == 1 && !r3) {
+    fft_stage<L, s, 2, 1>(buf, ix, N, 0, lt, Tn, tw, act);
     return;
   }
-  if (logN == 2) {
-    fft_stage<L, s, 4, 1>(buf, ix, N, 0, lt, Tn, tw);
+  if (logN == 2 && !r3) {
+    fft_stage<L, s, 4, 1>(buf, ix, N, 0, lt, Tn, tw, act);
     return;
   }
   int lNs = 0;
   int rem = logN;
   while (rem >= 3) {
-    fft_stage<L, s, 8, 1>(buf, ix, N, lNs, lt, Tn, tw);
+    fft_stage<L, s, 8, 1>(buf, ix, N, lNs, lt, Tn, tw, act);
     lNs += 3;
     rem -= 3;
   }
-  if (rem == 2) fft_stage<L, s, 4, 2>(buf, ix, N, lNs, lt, Tn, tw);
-  else if (rem == 1) fft_stage<L, s, 2, 4>(buf, ix, N, lNs, lt, Tn, tw);
+  if (rem == 2) fft_stage<L, s, 4, 2>(buf, ix, N, lNs, lt, Tn, tw, act);
+  else if (rem == 1) fft_stage<L, s, 2, 4>(buf, ix, N, lNs, lt, Tn, tw, act);
+  if (r3) fft_stage3<L, s>(buf, ix, N, lt, Tn, tw, act);
 }
 
 }  // namespace cans

@@ -25,7 +25,8 @@ struct cans_plan {
   cans_ctx* ctx;
   int ng[3];
   int len[2];        // transform lengths (x, y)
-  int logn[2];
+  int logn[2];       // len = 2^logn, or 3 * 2^logn when r3 is set
+  int r3[2];
   int kind[2];
   double normfft;
   cans::cplx<double>* tw[2];  // W_N^t = exp(-2 pi i t/N), t < N
@@ -41,6 +42,21 @@ static int ilog2(int n) {
   int l = 0;
   while ((1 << l) < n) l++;
   return (1 << l) == n ? l : -1;
+}
+
+// transform length n = 2^m, or 3 * 2^m with m >= 3 (the radix-3 stage needs a team of
+// n/8 threads): m and the radix-3 flag; false for other lengths
+static bool fft_factor(int n, int* m, int* r3) {
+  if (n < 1 || n > 4096) return false;
+  *m = ilog2(n);
+  *r3 = 0;
+  if (*m >= 0) return true;
+  if (n % 3 == 0) {
+    *m = ilog2(n / 3);
+    *r3 = 1;
+    return *m >= 3;
+  }
+  return false;
 }
 
 // position of input element e in Makhoul's even/odd reordering (v_m = x_{2m}, v_{N-1-m} = x_{2m+1})
@@ -143,7 +159,7 @@ struct XArgs {
   Layout L;
   T* fld[NBATCH];  // halo'd fields (origin), one per system of a batch (blockIdx.y)
   T* W[NBATCH];    // their z-tiled work buffers (woff)
-  int N, logN, Tn, pairs, kind;
+  int N, logN, r3, Tn, pairs, kind;
   const cplx<double>* tw;
   const cplx<double>* mk;
   int div;         // forward only: build the rhs from the velocity instead of reading fld
@@ -188,18 +204,21 @@ __device__ __forceinline__ d2 div_pair(const XArgs<T>& a, long r, int e) {
 // DIR +1 forward (fld -> W), -1 backward (W -> fld); DV (forward only): the rhs is
 // formed from the velocity (a.ds) instead of read from fld.  DV is a template flag
 // so that the plain forward pass carries none of the divergence code's registers.
+// bx: work-group index among this system's; sys: system of a batch.  Lanes past the
+// last whole team (a.pairs teams of a.Tn) only take part in the barriers.
 template <int DIR, typename T, int DV = 0>
-__device__ __forceinline__ void xpass_body(const XArgs<T>& a) {
-  T* const fld = a.fld[blockIdx.y];
-  T* const W = a.W[blockIdx.y];
+__device__ __forceinline__ void xpass_body(const XArgs<T>& a, int bx, int sys) {
+  T* const fld = a.fld[sys];
+  T* const W = a.W[sys];
   extern __shared__ unsigned char smem_raw[];
   cplx<T>* smem = (cplx<T>*)smem_raw;  // LDS line in the storage type
   const int N = a.N, Tn = a.Tn, kind = a.kind;
   const int team = threadIdx.x / Tn, lt = threadIdx.x % Tn;
+  const bool act = team < a.pairs;
   const long nrows = (long)a.L.n2 * a.L.n3;
-  const long q = (long)blockIdx.x * a.pairs + team;
+  const long q = (long)bx * a.pairs + team;
   const long ra = 2 * q, rb = 2 * q + 1;
-  const bool va = ra < nrows, vb = rb < nrows;
+  const bool va = act && ra < nrows, vb = act && rb < nrows;
   const IxPad ix{team * padded_len(N)};
   const Layout& L = a.L;
   auto frow = [&](bool v, long r) {
@@ -248,7 +267,7 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& a) {
       }
     }
     __syncthreads();
-    fft_line<T, -1>(smem, ix, N, a.logN, lt, Tn, a.tw);
+    fft_line<T, -1>(smem, ix, N, a.logN, lt, Tn, a.tw, a.r3, act);
     if (!va) return;
     if (kind == K_PP) {
       const int H = N >> 1;
@@ -277,7 +296,7 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& a) {
     } else {
       for (int k = 2 * lt; k < N; k += 2 * Tn) {
         double y0a, y0b, y1a, y1b;
-        split_dct(widen(smem[ix(k)]), widen(smem[ix((N - k) & (N - 1))]), a.mk[k], y0a, y0b);
+        split_dct(widen(smem[ix(k)]), widen(smem[ix(k ? N - k : 0)]), a.mk[k], y0a, y0b);
         split_dct(widen(smem[ix(k + 1)]), widen(smem[ix(N - k - 1)]), a.mk[k + 1], y1a, y1b);
         if (kind == K_DD) {  // output reversed: element N-1-k
           st2(ya + N - 2 - k, y1a, y0a);
@@ -323,7 +342,7 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& a) {
       }
     }
     __syncthreads();
-    fft_line<T, +1>(smem, ix, N, a.logN, lt, Tn, a.tw);
+    fft_line<T, +1>(smem, ix, N, a.logN, lt, Tn, a.tw, a.r3, act);
     if (!va) return;
     if (N == 1) {
       if (lt == 0) {
@@ -348,11 +367,11 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& a) {
 // one kernel per storage type (the FP32 and FP64 forms differ in LDS and register use)
 template <int DIR, int DV>
 __global__ void __launch_bounds__(512) k_xpass_f(XArgs<float> a) {
-  xpass_body<DIR, float, DV>(a);
+  xpass_body<DIR, float, DV>(a, blockIdx.x, blockIdx.y);
 }
 template <int DIR, int DV>
 __global__ void __launch_bounds__(512) k_xpass_d(XArgs<double> a) {
-  xpass_body<DIR, double, DV>(a);
+  xpass_body<DIR, double, DV>(a, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -364,26 +383,29 @@ template <typename T>
 struct YArgs {
   int n1, n2, n3;
   T* W[NBATCH];    // z-tiled work buffers (woff), one per system of a batch (blockIdx.y)
-  int N, logN, Tn, lines, kind;
+  int N, logN, r3, Tn, lines, kind;
   const cplx<double>* tw;
   const cplx<double>* mk;
 };
 
+// bx: work-group index among this system's; sys: system of a batch.  Lanes past
+// a.lines * a.Tn only take part in the barriers.
 template <int DIR, typename T>
-__device__ __forceinline__ void ypass_body(const YArgs<T>& a) {
+__device__ __forceinline__ void ypass_body(const YArgs<T>& a, int bx, int sys) {
   extern __shared__ unsigned char smem_raw[];
   cplx<T>* smem = (cplx<T>*)smem_raw;  // LDS line in the storage type
   const int LN = a.lines, N = a.N, Tn = a.Tn, kind = a.kind;
   const int c = threadIdx.x % LN, lt = threadIdx.x / LN;
+  const bool act = lt < Tn;
   const int ncolblk = (a.n1 + 2 * LN - 1) / (2 * LN);
-  const int k = blockIdx.x / ncolblk;
-  const int i0 = (blockIdx.x % ncolblk) * 2 * LN;
+  const int k = bx / ncolblk;
+  const int i0 = (bx % ncolblk) * 2 * LN;
   const int ia = i0 + 2 * c;
-  const bool va = ia < a.n1, vb = ia + 1 < a.n1;
+  const bool va = act && ia < a.n1, vb = act && ia + 1 < a.n1;
   const IxStride ix{c, LN};
   // element e (along y) of column ia in the tiled work buffer; columns ia, ia+1 are adjacent
   const int nic = wnic(a.n1);
-  T* const W = a.W[blockIdx.y];
+  T* const W = a.W[sys];
   auto col = [&](int e) { return W + woff(ia, e, k, a.n2, nic); };
   // (A, B) pair of element e of this line
   auto ld = [&](int e) -> d2 { return vb ? ld2(col(e)) : d2((double)*col(e), 0.); };
@@ -403,7 +425,7 @@ __device__ __forceinline__ void ypass_body(const YArgs<T>& a) {
       }
     }
     __syncthreads();
-    fft_line<T, -1>(smem, ix, N, a.logN, lt, Tn, a.tw);
+    fft_line<T, -1>(smem, ix, N, a.logN, lt, Tn, a.tw, a.r3, act);
     if (!va) return;
     if (kind == K_PP) {
       const int H = N >> 1;
@@ -428,7 +450,7 @@ __device__ __forceinline__ void ypass_body(const YArgs<T>& a) {
     } else {
       for (int kk = lt; kk < N; kk += Tn) {
         double ya, yb;
-        split_dct(widen(smem[ix(kk)]), widen(smem[ix((N - kk) & (N - 1))]), a.mk[kk], ya, yb);
+        split_dct(widen(smem[ix(kk)]), widen(smem[ix(kk ? N - kk : 0)]), a.mk[kk], ya, yb);
         st(kind == K_DD ? N - 1 - kk : kk, ya, yb);
       }
     }
@@ -467,7 +489,7 @@ __device__ __forceinline__ void ypass_body(const YArgs<T>& a) {
       }
     }
     __syncthreads();
-    fft_line<T, +1>(smem, ix, N, a.logN, lt, Tn, a.tw);
+    fft_line<T, +1>(smem, ix, N, a.logN, lt, Tn, a.tw, a.r3, act);
     if (!va) return;
     for (int e = lt; e < N; e += Tn) {
       cplx<double> z = widen(smem[ix(kind == K_PP ? e : mk_pos(e, N))]);
@@ -485,11 +507,11 @@ __device__ __forceinline__ void ypass_body(const YArgs<T>& a) {
 // 512-thread work-groups, which needs no register cap.
 template <int DIR>
 __global__ void __launch_bounds__(1024, 8) k_ypass_f(YArgs<float> a) {
-  ypass_body<DIR, float>(a);
+  ypass_body<DIR, float>(a, blockIdx.x, blockIdx.y);
 }
 template <int DIR>
 __global__ void __launch_bounds__(1024) k_ypass_d(YArgs<double> a) {
-  ypass_body<DIR, double>(a);
+  ypass_body<DIR, double>(a, blockIdx.x, blockIdx.y);
 }
 template <int DIR, typename T>
 void launch_ypass(dim3 grid, int threads, size_t shm, hipStream_t st, const YArgs<T>& a);
@@ -739,10 +761,7 @@ __global__ void __launch_bounds__(256) k_gaussel(ZArgs<T> z) {
 // (wbuf_elems) and their values are never used.  FP64 storage gives
 // the same operations as k_gaussel; FAST replaces the divisions by rcp64<true>.
 template <typename T, bool FAST>
-__global__ void __launch_bounds__(256) k_thomas(ZArgs<T> z) {
-  const ZSys<T>& zs = z.sys[blockIdx.y];
-  const long line = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (line >= z.nl) return;
+__device__ __forceinline__ void thomas_body(const ZArgs<T>& z, const ZSys<T>& zs, long line) {
   const double l = zs.lam[line];
   const double eps = z.pin_eps, norm = zs.norm;
   T* P = zs.W + woff((int)(line % z.n1), (int)(line / z.n1), 0, z.n2, z.nic);  // row 0
@@ -860,6 +879,13 @@ __global__ void __launch_bounds__(256) k_thomas(ZArgs<T> z) {
 #pragma unroll
     for (int t = 0; t < CK; t++) rr[t] = rn[t];
   }
+}
+
+template <typename T, bool FAST>
+__global__ void __launch_bounds__(256) k_thomas(ZArgs<T> z) {
+  const long line = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (line >= z.nl) return;
+  thomas_body<T, FAST>(z, z.sys[blockIdx.y], line);
 }
 
 // ---------------------------------------------------------------------------
@@ -1003,9 +1029,10 @@ __global__ void __launch_bounds__(256) k_dtdma_back(DtArgs<T> z) {
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-static int upload_tables(cans_plan* p, int d) {
-  const int N = p->len[d];
-  std::vector<cplx<double>> tw(N > 0 ? N : 1), mk(N + 1);
+// W_N^t = exp(-2 pi i t/N), t < N, and the DCT/DST twiddles exp(-i pi k/(2N)), k <= N
+static void make_tables(int N, std::vector<cplx<double>>& tw, std::vector<cplx<double>>& mk) {
+  tw.assign(N > 0 ? N : 1, cplx<double>{1., 0.});
+  mk.assign(N + 1, cplx<double>{1., 0.});
   const long double pi = 3.141592653589793238462643383279502884L;
   for (int t = 0; t < N; t++) {
     long double ang = 2.0L * pi * (long double)t / (long double)N;
@@ -1019,6 +1046,11 @@ static int upload_tables(cans_plan* p, int d) {
     long double ang = pi * (long double)k / (2.0L * (long double)N);
     mk[k] = {(double)cosl(ang), -(double)sinl(ang)};
   }
+}
+
+static int upload_tables(cans_plan* p, int d) {
+  std::vector<cplx<double>> tw, mk;
+  make_tables(p->len[d], tw, mk);
   CANS_HIP_TRY(hipMalloc((void**)&p->tw[d], tw.size() * sizeof(cplx<double>)));
   CANS_HIP_TRY(hipMalloc((void**)&p->mk[d], mk.size() * sizeof(cplx<double>)));
   CANS_HIP_TRY(hipMemcpy(p->tw[d], tw.data(), tw.size() * sizeof(cplx<double>), hipMemcpyHostToDevice));
@@ -1039,6 +1071,7 @@ static int xpass(cans_ctx* c, const cans_plan* p, T* const* fld, T* const* W, in
   }
   a.N = p->len[0];
   a.logN = p->logn[0];
+  a.r3 = p->r3[0];
   a.Tn = a.N >= 8 ? a.N / 8 : 1;
   const int xt = p->xthreads[sizeof(T) == sizeof(double)] ? p->xthreads[sizeof(T) == sizeof(double)] : 256;
   a.pairs = a.Tn >= xt ? 1 : xt / a.Tn;
@@ -1071,6 +1104,7 @@ static int ypass(cans_ctx* c, const cans_plan* p, T* const* W, int nsys, int dir
   for (int q = 0; q < nsys; q++) a.W[q] = W[q];
   a.N = p->len[1];
   a.logN = p->logn[1];
+  a.r3 = p->r3[1];
   a.Tn = a.N >= 8 ? a.N / 8 : 1;
   int lines = 256 / a.Tn;
   if (lines < 8) lines = 8;
@@ -1401,6 +1435,329 @@ template int run_solver<float>(cans_ctx*, const cans_plan*, double, const double
 
 using namespace cans;
 
+namespace cans {
+
+// ---------------------------------------------------------------------------
+// Independent Poisson boxes of different sizes: one solver.f90:17-112 call per box
+// (each with its own arrplan, normfft, lambdaxy, a, b, c), all boxes solved together.
+// Each pass is one launch over every box: the work-groups of box b are
+// off[b] .. off[b+1]-1 of the grid, found by a binary search over off, and run the
+// single solve's pass body with that box's shapes.  Work-groups have BOX_THREADS
+// lanes; a box uses as many whole transform teams as fit and the lanes left over
+// only take part in the barriers.  Boxes are cell-centred with a non-periodic z.
+// ---------------------------------------------------------------------------
+constexpr int BOX_THREADS = 256;
+
+template <typename T>
+struct BoxDesc {
+  Layout L;
+  T* fld;                        // halo'd field (origin)
+  T* W;                          // z-tiled work buffer (woff)
+  int N[2], logN[2], r3[2], Tn[2], kind[2];
+  int pairs, lines;              // x-pass teams, y-pass lines per work-group
+  const cplx<double>* tw[2];
+  const cplx<double>* mk[2];
+  double norm;                   // normfft
+  const double *lam, *a, *b, *c;
+  double *ckd, *ckp;             // Thomas checkpoints [nblk][n1*n2]
+};
+
+template <typename T>
+struct BoxArgs {
+  const BoxDesc<T>* __restrict__ box;
+  const int* __restrict__ off;   // [nbox + 1]: first work-group of each box in this pass
+  int nbox;
+  double pin_eps;
+};
+
+__device__ __forceinline__ int find_box(const int* __restrict__ off, int nbox, int g) {
+  int lo = 0, hi = nbox;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= g) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <int DIR, typename T>
+__global__ void __launch_bounds__(BOX_THREADS) k_box_xpass(BoxArgs<T> ba) {
+  const int b = find_box(ba.off, ba.nbox, blockIdx.x);
+  const BoxDesc<T>& d = ba.box[b];
+  XArgs<T> a;
+  a.L = d.L;
+  a.fld[0] = d.fld;
+  a.W[0] = d.W;
+  a.N = d.N[0];
+  a.logN = d.logN[0];
+  a.r3 = d.r3[0];
+  a.Tn = d.Tn[0];
+  a.pairs = d.pairs;
+  a.kind = d.kind[0];
+  a.tw = d.tw[0];
+  a.mk = d.mk[0];
+  a.div = 0;
+  xpass_body<DIR, T, 0>(a, blockIdx.x - ba.off[b], 0);
+}
+
+template <int DIR, typename T>
+__global__ void __launch_bounds__(BOX_THREADS) k_box_ypass(BoxArgs<T> ba) {
+  const int b = find_box(ba.off, ba.nbox, blockIdx.x);
+  const BoxDesc<T>& d = ba.box[b];
+  YArgs<T> a;
+  a.n1 = d.L.n1;
+  a.n2 = d.L.n2;
+  a.n3 = d.L.n3;
+  a.W[0] = d.W;
+  a.N = d.N[1];
+  a.logN = d.logN[1];
+  a.r3 = d.r3[1];
+  a.Tn = d.Tn[1];
+  a.lines = d.lines;
+  a.kind = d.kind[1];
+  a.tw = d.tw[1];
+  a.mk = d.mk[1];
+  ypass_body<DIR, T>(a, blockIdx.x - ba.off[b], 0);
+}
+
+template <typename T, bool FAST>
+__global__ void __launch_bounds__(256) k_box_thomas(BoxArgs<T> ba) {
+  const int b = find_box(ba.off, ba.nbox, blockIdx.x);
+  const BoxDesc<T>& d = ba.box[b];
+  const long nl = (long)d.L.n1 * d.L.n2;
+  const long line = (long)(blockIdx.x - ba.off[b]) * blockDim.x + threadIdx.x;
+  if (line >= nl) return;
+  ZArgs<T> z;
+  z.nl = nl;
+  z.n1 = d.L.n1;
+  z.n2 = d.L.n2;
+  z.nic = wnic(d.L.n1);
+  z.PS = wplane(d.L.n1, d.L.n2);
+  z.periodic = 0;
+  z.pin_eps = ba.pin_eps;
+  z.X2 = nullptr;
+  ZSys<T> zs;
+  zs.W = d.W;
+  zs.n = d.L.n3;
+  zs.norm = d.norm;
+  zs.a = d.a;
+  zs.b = d.b;
+  zs.c = d.c;
+  zs.lam = d.lam;
+  zs.ckd = d.ckd;
+  zs.ckp = d.ckp;
+  zs.ckq = nullptr;
+  thomas_body<T, FAST>(z, zs, line);
+}
+
+}  // namespace cans
+
+struct cans_poisson_batch {
+  cans_ctx* ctx;
+  int nbox = 0;
+  int es = 8;                    // element size of the boxes' fields
+  void* mem = nullptr;           // one device allocation holding everything below
+  const void* desc = nullptr;    // BoxDesc<T>[nbox]
+  const int* off[3] = {};        // first work-group of each box: x passes, y passes, z pass
+  int blocks[3] = {};            // work-groups per launch of each pass
+  size_t shm[2] = {};            // LDS of the x and y launches
+};
+
+namespace cans {
+
+// host staging of one device allocation: chunks at 256-byte aligned offsets
+struct Arena {
+  std::vector<unsigned char> host;
+  size_t take(size_t bytes) {
+    const size_t o = (host.size() + 255) & ~(size_t)255;
+    host.resize(o + bytes);
+    return o;
+  }
+};
+
+template <typename T>
+static int batch_create(cans_ctx* c, int nbox, const cans_poisson_box* bx, double* normfft, cans_poisson_batch* B) {
+  Arena ar;
+  struct Fix {  // device addresses filled in once the allocation exists
+    size_t desc, lam, a, b, c, tw[2], mk[2], W, ck;
+    long nl, nblk;
+  };
+  std::vector<Fix> fx(nbox);
+  std::vector<BoxDesc<T>> d(nbox);
+  std::vector<int> off[3];
+  for (int q = 0; q < 3; q++) off[q].assign(nbox + 1, 0);
+  const size_t desc_at = ar.take(sizeof(BoxDesc<T>) * nbox);
+  size_t off_at[3];
+  for (int q = 0; q < 3; q++) off_at[q] = ar.take(sizeof(int) * (nbox + 1));
+  long wtot = 0, cktot = 0;
+  for (int i = 0; i < nbox; i++) {
+    const cans_poisson_box& x = bx[i];
+    const std::string who = "cans_poisson_batch_create: box " + std::to_string(i) + ": ";
+    const int n1 = x.n[0], n2 = x.n[1], n3 = x.n[2];
+    if (n1 < 2 || n2 < 1 || n3 < 1) return fail(CANS_EINVAL, who + "bad sizes");
+    const long sy = x.sy ? x.sy : n1 + 2, sz = x.sz ? x.sz : sy * (n2 + 2);
+    if (!x.p_d || !x.lambdaxy_h || !x.a_h || !x.b_h || !x.c_h) return fail(CANS_EINVAL, who + "null array");
+    if (sy < n1 + 2 || sz < sy * (n2 + 2)) return fail(CANS_EINVAL, who + "strides smaller than the halo'd rows/planes");
+    // the x pass moves element pairs: interior rows must start on pair boundaries
+    if (n1 % 2 || sy % 2 || ((uintptr_t)((T*)x.p_d + 1) & (2 * sizeof(T) - 1)))
+      return fail(CANS_EINVAL, who + "rows of p must start aligned to an element pair (n1 and sy even, element "
+                                     "(1,0,0) on a pair boundary)");
+    if (x.cbc[4] == 'P' || x.cbc[5] == 'P' || !std::strchr("ND", x.cbc[4]) || !std::strchr("ND", x.cbc[5]) ||
+        !x.cbc[4] || !x.cbc[5])
+      return fail(CANS_EUNSUPPORTED, who + "z must be non-periodic (N or D at each end)");
+    BoxDesc<T>& b = d[i];
+    b.L.n1 = n1;
+    b.L.n2 = n2;
+    b.L.n3 = n3;
+    b.L.sy = sy;
+    b.L.sz = sz;
+    b.fld = (T*)x.p_d;
+    double nf = 1.;
+    for (int q = 0; q < 2; q++) {
+      const char b0 = x.cbc[2 * q], b1 = x.cbc[2 * q + 1];
+      if (b0 == 'P' && b1 == 'P') b.kind[q] = K_PP;
+      else if (b0 == 'N' && b1 == 'N') b.kind[q] = K_NN;
+      else if (b0 == 'D' && b1 == 'D') b.kind[q] = K_DD;
+      else return fail(CANS_EUNSUPPORTED, who + "x/y boundary pairs must be PP, NN or DD");
+      b.N[q] = x.n[q];
+      if (!fft_factor(x.n[q], &b.logN[q], &b.r3[q]) || x.n[q] > 8 * BOX_THREADS)
+        return fail(CANS_EUNSUPPORTED, who + "x/y lengths must be 2^m or 3*2^m (m >= 3), at most 2048");
+      b.Tn[q] = x.n[q] >= 8 ? x.n[q] / 8 : 1;
+      nf = nf * (b.kind[q] == K_PP ? 1. : 2.) * (x.n[q] + 0.);  // fft.f90:75,140
+      std::vector<cplx<double>> tw, mk;
+      make_tables(x.n[q], tw, mk);
+      fx[i].tw[q] = ar.take(tw.size() * sizeof(cplx<double>));
+      std::memcpy(&ar.host[fx[i].tw[q]], tw.data(), tw.size() * sizeof(cplx<double>));
+      fx[i].mk[q] = ar.take(mk.size() * sizeof(cplx<double>));
+      std::memcpy(&ar.host[fx[i].mk[q]], mk.data(), mk.size() * sizeof(cplx<double>));
+    }
+    b.norm = 1. / nf;
+    if (normfft) normfft[i] = b.norm;
+    b.pairs = BOX_THREADS / b.Tn[0];
+    b.lines = BOX_THREADS / b.Tn[1];
+    const long nl = (long)n1 * n2;
+    fx[i].nl = nl;
+    fx[i].lam = ar.take(nl * sizeof(double));
+    std::memcpy(&ar.host[fx[i].lam], x.lambdaxy_h, nl * sizeof(double));
+    const double* abc[3] = {x.a_h, x.b_h, x.c_h};
+    size_t* at[3] = {&fx[i].a, &fx[i].b, &fx[i].c};
+    for (int q = 0; q < 3; q++) {
+      *at[q] = ar.take(n3 * sizeof(double));
+      std::memcpy(&ar.host[*at[q]], abc[q], n3 * sizeof(double));
+    }
+    fx[i].W = wtot;
+    wtot += (wbuf_elems(n1, n2, n3) + 63) & ~63L;
+    fx[i].nblk = (n3 + CK - 1) / CK;
+    fx[i].ck = cktot;
+    cktot += 2 * fx[i].nblk * nl;
+    // work-groups of each pass
+    const long npairs = ((long)n2 * n3 + 1) / 2;
+    const long gx = (npairs + b.pairs - 1) / b.pairs;
+    const long gy = (long)n3 * ((n1 + 2 * b.lines - 1) / (2 * b.lines));
+    const long gz = (nl + 255) / 256;
+    off[0][i + 1] = off[0][i] + (int)gx;
+    off[1][i + 1] = off[1][i] + (int)gy;
+    off[2][i + 1] = off[2][i] + (int)gz;
+    if ((long)off[0][i + 1] + gx > INT32_MAX / 2 || (long)off[1][i + 1] > INT32_MAX / 2)
+      return fail(CANS_EINVAL, "cans_poisson_batch_create: too many work-groups");
+    B->shm[0] = std::max(B->shm[0], (size_t)b.pairs * padded_len(b.N[0]) * sizeof(cplx<T>));
+    B->shm[1] = std::max(B->shm[1], (size_t)b.lines * b.N[1] * sizeof(cplx<T>));
+  }
+  // the work buffer and checkpoints follow the tables in the device allocation only
+  const size_t W_at = (ar.host.size() + 255) & ~(size_t)255;
+  const size_t ck_at = (W_at + (size_t)wtot * sizeof(T) + 255) & ~(size_t)255;
+  const size_t total = ck_at + (size_t)cktot * sizeof(double);
+  void* mem = nullptr;
+  hipError_t e = hipMalloc(&mem, total);
+  if (e != hipSuccess) return fail(CANS_ENOMEM, std::string("cans_poisson_batch_create: ") + hipGetErrorString(e));
+  unsigned char* base = (unsigned char*)mem;
+  for (int i = 0; i < nbox; i++) {
+    BoxDesc<T>& b = d[i];
+    for (int q = 0; q < 2; q++) {
+      b.tw[q] = (const cplx<double>*)(base + fx[i].tw[q]);
+      b.mk[q] = (const cplx<double>*)(base + fx[i].mk[q]);
+    }
+    b.lam = (const double*)(base + fx[i].lam);
+    b.a = (const double*)(base + fx[i].a);
+    b.b = (const double*)(base + fx[i].b);
+    b.c = (const double*)(base + fx[i].c);
+    b.W = (T*)(base + W_at) + fx[i].W;
+    b.ckd = (double*)(base + ck_at) + fx[i].ck;
+    b.ckp = b.ckd + fx[i].nblk * fx[i].nl;
+  }
+  std::memcpy(&ar.host[desc_at], d.data(), sizeof(BoxDesc<T>) * nbox);
+  for (int q = 0; q < 3; q++) std::memcpy(&ar.host[off_at[q]], off[q].data(), sizeof(int) * (nbox + 1));
+  // the work buffer and checkpoints need no initial values
+  e = hipMemcpy(mem, ar.host.data(), ar.host.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    hipFree(mem);
+    return fail(CANS_EHIP, std::string("cans_poisson_batch_create: ") + hipGetErrorString(e));
+  }
+  B->mem = mem;
+  B->desc = base + desc_at;
+  for (int q = 0; q < 3; q++) {
+    B->off[q] = (const int*)(base + off_at[q]);
+    B->blocks[q] = off[q][nbox];
+  }
+  return CANS_OK;
+}
+
+template <typename T>
+static int batch_solve(cans_poisson_batch* B) {
+  cans_ctx* c = B->ctx;
+  BoxArgs<T> a[3];
+  for (int q = 0; q < 3; q++) {
+    a[q].box = (const BoxDesc<T>*)B->desc;
+    a[q].off = B->off[q];
+    a[q].nbox = B->nbox;
+    // the zero-wavenumber pin of run_solver_batch
+    a[q].pin_eps = 2.220446049250313e-16 * (sizeof(T) == sizeof(double) ? 1. : 4096.);
+  }
+  k_box_xpass<1, T><<<B->blocks[0], BOX_THREADS, B->shm[0], c->stream>>>(a[0]);
+  CANS_LAUNCH_CHECK("k_box_xpass");
+  k_box_ypass<1, T><<<B->blocks[1], BOX_THREADS, B->shm[1], c->stream>>>(a[1]);
+  CANS_LAUNCH_CHECK("k_box_ypass");
+  k_box_thomas<T, sizeof(T) == sizeof(float)><<<B->blocks[2], 256, 0, c->stream>>>(a[2]);
+  CANS_LAUNCH_CHECK("k_box_thomas");
+  k_box_ypass<-1, T><<<B->blocks[1], BOX_THREADS, B->shm[1], c->stream>>>(a[1]);
+  CANS_LAUNCH_CHECK("k_box_ypass");
+  k_box_xpass<-1, T><<<B->blocks[0], BOX_THREADS, B->shm[0], c->stream>>>(a[0]);
+  CANS_LAUNCH_CHECK("k_box_xpass");
+  return CANS_OK;
+}
+
+}  // namespace cans
+
+extern "C" int cans_poisson_batch_create(cans_ctx* c, int nbox, const cans_poisson_box* boxes, double* normfft,
+                                         cans_poisson_batch** out) {
+  using namespace cans;
+  if (!c || !boxes || !out || nbox < 1) return fail(CANS_EINVAL, "cans_poisson_batch_create: bad argument");
+  cans_poisson_batch* B = new cans_poisson_batch();
+  B->ctx = c;
+  B->nbox = nbox;
+  B->es = c->precision == CANS_MIXED ? 4 : 8;
+  int rc = B->es == 4 ? batch_create<float>(c, nbox, boxes, normfft, B) : batch_create<double>(c, nbox, boxes, normfft, B);
+  if (rc) {
+    delete B;
+    return rc;
+  }
+  *out = B;
+  return CANS_OK;
+}
+
+extern "C" int cans_poisson_batch_solve(cans_poisson_batch* B) {
+  using namespace cans;
+  if (!B) return fail(CANS_EINVAL, "cans_poisson_batch_solve: null batch");
+  return B->es == 4 ? batch_solve<float>(B) : batch_solve<double>(B);
+}
+
+extern "C" int cans_poisson_batch_destroy(cans_poisson_batch* B) {
+  if (!B) return CANS_OK;
+  hipFree(B->mem);
+  delete B;
+  return CANS_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Launch-shape tuner.  The x pass's work-group size (lines per group) and the y
 // pass's lines per group trade LDS, registers and access width; the best choice
@@ -1508,10 +1865,10 @@ extern "C" int cans_fftini(cans_ctx* c, const int* ng, const char* bcxy, const c
       return fail(CANS_EUNSUPPORTED, "cans_fftini: mixed N/D transforms are not implemented on the device");
     }
     p->len[d] = ng[d];
-    p->logn[d] = ilog2(ng[d]);
-    if (p->logn[d] < 0 || ng[d] > 4096) {
+    if (!fft_factor(ng[d], &p->logn[d], &p->r3[d])) {
       delete p;
-      return fail(CANS_EUNSUPPORTED, "cans_fftini: transform lengths must be powers of two <= 4096");
+      return fail(CANS_EUNSUPPORTED,
+                  "cans_fftini: transform lengths must be 2^m or 3*2^m (m >= 3), at most 4096");
     }
     // fft.f90:75,140: normfft = 1/prod(norm(1)*(ng+norm(2)-i)), norm = [1,0] (PP) or [2,0] (NN/DD, c-grid)
     nf = nf * (p->kind[d] == K_PP ? 1. : 2.) * (ng[d] + 0.);

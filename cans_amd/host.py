@@ -285,6 +285,84 @@ class Plan:
                                       C.c_void_p(f.ptr)), "cans_solver")
 
 
+class PoissonBatch:
+    """Independent Poisson problems of different sizes solved together
+    (cans_poisson_batch_*): per box, the arrays of initsolver (src/initsolver.f90:15-83,
+    c_or_f = 'ccc') and one src/solver.f90:17 call, all boxes in five launches.
+
+    boxes: dicts with n=(n1, n2, n3), l=(lx, ly, lz), cbc (cbcpre, 6 chars, z
+    non-periodic) and optionally gtype (default 1) and gr (default 0, uniform z).
+    Each box's field is halo'd, dense (leading dimension n1+2), in the context's
+    pressure storage type."""
+
+    def __init__(self, ctx: Context, boxes):
+        torch = ctx.torch
+        self.ctx = ctx
+        es = ctx.p_es
+        self.n, self.views, self.bufs, self.dzfi = [], [], [], []
+        arr = (LB.PoissonBox * len(boxes))()
+        keep = []
+        for i, bx in enumerate(boxes):
+            n1, n2, n3 = (int(v) for v in bx["n"])
+            lx, ly, lz = (float(v) for v in bx["l"])
+            cbc = bx["cbc"]
+            dli = [(x / float(np.float32(m))) ** (-1) for x, m in ((lx, n1), (ly, n2))]
+            dzc, dzf, zc, zf = (np.zeros(n3 + 2) for _ in range(4))
+            LB.check(LB.lib().cans_initgrid(int(bx.get("gtype", 1)), n3, float(bx.get("gr", 0.0)), lz, 0, _dp(dzc),
+                                            _dp(dzf), _dp(zc), _dp(zf)), "cans_initgrid")
+            dzci, dzfi = dzc ** (-1), dzf ** (-1)
+            lam = (eigenvalues(n1, cbc[0:2], "c")[:, None] * dli[0] ** 2 +
+                   eigenvalues(n2, cbc[2:4], "c")[None, :] * dli[1] ** 2).ravel(order="F")
+            a, b, c = tridmatrix(cbc[4:6], n3, dzci, dzfi, "c")
+            sy, sz = n1 + 2, (n1 + 2) * (n2 + 2)
+            elems = sz * (n3 + 2)
+            buf = torch.zeros(elems + 2 * 256 // es, dtype=ctx.p_dtype, device=ctx.device)
+            shift = ((-(buf.data_ptr() + es)) % 256) // es  # element (1,0,0) on a 256-byte boundary
+            self.bufs.append(buf)
+            self.views.append(buf[shift:shift + elems].view(n3 + 2, n2 + 2, sy))
+            self.n.append((n1, n2, n3))
+            self.dzfi.append(dzfi)
+            arrays = [np.ascontiguousarray(q, dtype=np.float64) for q in (lam, a, b, c)]
+            keep.append(arrays)
+            bs = arr[i]
+            bs.n[:] = [n1, n2, n3]
+            bs.sy, bs.sz = sy, sz
+            bs.cbc = cbc.encode()
+            bs.lambdaxy_h, bs.a_h, bs.b_h, bs.c_h = (q.ctypes.data for q in arrays)
+            bs.p_d = buf.data_ptr() + es * shift
+        nf = np.zeros(len(boxes))
+        h = C.c_void_p()
+        LB.check(LB.lib().cans_poisson_batch_create(ctx.h, len(boxes), arr, _dp(nf), C.byref(h)),
+                 "cans_poisson_batch_create")
+        self.h = h
+        self.normfft = nf
+
+    def close(self):
+        if self.h:
+            LB.lib().cans_poisson_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set(self, i, a):
+        """Store box i from a Fortran-shaped (n1+2, n2+2, n3+2) float64 host array."""
+        a = np.asarray(a, dtype=np.float64)
+        assert a.shape == tuple(m + 2 for m in self.n[i])
+        host = np.ascontiguousarray(a.transpose(2, 1, 0)).astype(np.float32 if self.ctx.p_es == 4 else np.float64)
+        self.views[i].copy_(self.ctx.torch.from_numpy(host))
+
+    def get(self, i):
+        """Box i as a Fortran-ordered (n1+2, n2+2, n3+2) float64 host array."""
+        return np.asfortranarray(self.views[i].cpu().numpy().astype(np.float64).transpose(2, 1, 0))
+
+    def solve(self):
+        LB.check(LB.lib().cans_poisson_batch_solve(self.h), "cans_poisson_batch_solve")
+
+
 class Simulation:
     """One case on one GPU: set-up of src/main.f90:241-395 and its RK3 loop."""
 

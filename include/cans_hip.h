@@ -138,6 +138,29 @@ int cans_fft_tuning(const cans_plan *arrplan, int *shapes);
 int cans_solver(cans_ctx *ctx, const int *n, const int *ng, const cans_plan *arrplan,
                 double normfft, const double *lambdaxy_d, const double *a_d, const double *b_d,
                 const double *c_d, const char *bc, const char *c_or_f, void *p_d);
+/* Many independent Poisson problems of different sizes, solved together: each box
+ * carries the arguments of one src/solver.f90:17 call, and cans_poisson_batch_create
+ * does each box's fftini (src/fft.f90:25) as well.  A solve is five launches for
+ * all boxes (x, y, z, y, x passes) in place on the boxes' fields, on the context's stream,
+ * with no allocation or host synchronisation (it can be captured in a HIP graph).
+ * The boxes are cell-centred (c_or_f = 'ccc'); x and y pairs PP, NN or DD with
+ * lengths 2^m or 3*2^m (m >= 3) up to 2048, n1 even; z non-periodic.  The fields
+ * are in the context's pressure storage type (float under CANS_MIXED) with element
+ * (1,0,0) on an element-pair boundary.  normfft (nbox, may be NULL) receives each
+ * box's normfft.  The batch keeps the field addresses given at creation. */
+typedef struct cans_poisson_batch cans_poisson_batch;
+typedef struct cans_poisson_box {
+  int n[3];                   /* cells */
+  long sy, sz;                /* strides of p in elements; 0: n1+2 and sy*(n2+2) */
+  char cbc[6];                /* cbcpre(0:1,1:3) of the box */
+  const double *lambdaxy_h;   /* (n1,n2), device ordering (cans_eigenvalues) */
+  const double *a_h, *b_h, *c_h; /* (n3) (cans_tridmatrix) */
+  void *p_d;                  /* &p(0,0,0): rhs in, solution out */
+} cans_poisson_box;
+int cans_poisson_batch_create(cans_ctx *ctx, int nbox, const cans_poisson_box *boxes,
+                              double *normfft, cans_poisson_batch **batch);
+int cans_poisson_batch_solve(cans_poisson_batch *batch);
+int cans_poisson_batch_destroy(cans_poisson_batch *batch);
 /* src/solve_helmholtz.f90:28 (3-D branch) -- rhsb*_d may be NULL */
 int cans_solve_helmholtz(cans_ctx *ctx, const int *n, const int *ng, const cans_plan *arrplan,
                          double normfft, double alpha, const double *lambdaxy_d,

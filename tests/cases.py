@@ -123,3 +123,26 @@ def dhc_nusselt(s_interior, dl):
     tw, l = -0.5, 1.0
     dx, dz = dl[0], dl[2]
     return float((1.0 / l) * np.sum(((s_interior[0, 0, :] - tw) / dx) * dz) * l / (-tw))
+
+
+def poisson_boxes(nbox, seed, sizes=(32, 48, 64, 96, 128), nz=(32, 160), gr=(0.0, 2.0), cbcs=("PPPPNN",)):
+    """Seeded independent Poisson boxes: nx, ny drawn from `sizes`, nz uniform in `nz`
+    (inclusive), tanh stretching factor uniform in `gr`, pressure BCs drawn from `cbcs`.
+    Each box spans (2*pi, 2*pi, 1) like a channel cell."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(nbox):
+        n1, n2 = (int(rng.choice(sizes)) for _ in range(2))
+        n3 = int(rng.integers(nz[0], nz[1] + 1))
+        out.append(dict(n=(n1, n2, n3), l=(2 * np.pi, 2 * np.pi, 1.0), gtype=1, gr=float(rng.uniform(*gr)),
+                        cbc=cbcs[int(rng.integers(len(cbcs)))]))
+    return out
+
+
+def box_rhs(box, dzf, rng):
+    """uniform[-1, 1] rhs of a box with its volume-weighted mean removed when no wall is Dirichlet."""
+    rhs = rng.uniform(-1, 1, tuple(box["n"]))
+    if "D" not in box["cbc"]:
+        wgt = np.broadcast_to(dzf, rhs.shape)
+        rhs -= (rhs * wgt).sum() / wgt.sum()
+    return rhs

@@ -47,7 +47,9 @@ def pair(name, ng, seed=7):
 
 SOLVER_CASES = [("tgv", (16, 16, 16)), ("tgv", (32, 8, 12)), ("channel", (32, 16, 24)),
                 ("ldc", (2, 64, 64)), ("duct", (16, 32, 20)), ("cavity_dd", (16, 32, 24)),
-                ("channel", (128, 64, 48))]
+                ("channel", (128, 64, 48)),
+                # radix-3 transform lengths (3 * 2^m) along x and y
+                ("channel", (48, 96, 40)), ("cavity_dd", (96, 24, 20)), ("box", (24, 48, 16))]
 
 
 @pytest.mark.parametrize("name,ng", SOLVER_CASES, ids=["%s-%dx%dx%d" % ((c,) + g) for c, g in SOLVER_CASES])
@@ -64,9 +66,18 @@ def test_solver_matches_oracle(torch, name, ng):
     sim.ctx.sync()
     st["pp"][...] = f
     orc.solver(st["pp"])
-    got = sim.pp.numpy()
-    err = relerr(got[1:-1, 1:-1, 1:-1], st["pp"][1:-1, 1:-1, 1:-1])
-    print("solver rel err %.3e" % err, flush=True)
+    got = sim.pp.numpy()[1:-1, 1:-1, 1:-1]
+    ref = st["pp"][1:-1, 1:-1, 1:-1]
+    raw = relerr(got, ref)
+    # with no Dirichlet wall the zero-wavenumber line is pinned only when its last pivot
+    # is within one ulp of zero (solver.f90:151-160); otherwise the solution carries a
+    # constant set by the transforms' rounding, which the comparison removes
+    if "D" not in p["cbcpre"]:
+        wgt = np.broadcast_to(1 / sim.grid.dzfi[1:-1], got.shape)
+        got = got - (got * wgt).sum() / wgt.sum()
+        ref = ref - (ref * wgt).sum() / wgt.sum()
+    err = relerr(got, ref)
+    print("solver rel err %.3e (before removing the mean %.3e)" % (err, raw), flush=True)
     assert err < RTOL
 
 
@@ -180,10 +191,11 @@ def cases_rk(irk):
 
 # the walled box (no periodic x) takes the unfused divergence path of the fused substep
 STEP_CASES = [("tgv", (32, 32, 32), 1e-3, 3), ("channel", (64, 32, 48), 2e-4, 3), ("duct", (32, 16, 24), 1e-3, 2),
-              ("cavity_dd", (16, 16, 12), 1e-3, 2), ("box", (16, 16, 12), 1e-3, 2)]
+              ("cavity_dd", (16, 16, 12), 1e-3, 2), ("box", (16, 16, 12), 1e-3, 2),
+              ("channel", (96, 48, 32), 2e-4, 2)]
 
 
-@pytest.mark.parametrize("name,ng,dt,nsteps", STEP_CASES, ids=[c[0] for c in STEP_CASES])
+@pytest.mark.parametrize("name,ng,dt,nsteps", STEP_CASES, ids=["%s-%dx%dx%d" % ((c[0],) + c[1]) for c in STEP_CASES])
 @pytest.mark.parametrize("fused", [True, False], ids=["fused", "per-routine"])
 def test_rk3_steps_match_oracle(torch, name, ng, dt, nsteps, fused):
     """Velocity at 1e-12; pressure (gauge-fixed) at 1e-12 or, where the stretched
