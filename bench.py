@@ -15,8 +15,10 @@ B_P = 5 passes x N cells x (s_in + s_out) = 40 N bytes with FP32 storage, 80 N
 with FP64 storage.  The oracle (C/OpenMP restatement) is timed on the host on a
 bounded number of full steps of the same case as the CPU baseline.
 
-A secondary workload (the 256^3 heated cube with a scalar, buoyancy and implicit z
-diffusion, FP64) is timed after the headline one and reported under "secondary".
+Secondary workloads, timed after the headline one and reported under "secondary":
+the 256^3 heated cube with a scalar, buoyancy and implicit z diffusion (FP64), and
+64 independent Poisson boxes of different sizes solved as one batch (FP64 and mixed
+storage, beside the same boxes solved one at a time).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--ng N1 N2 N3]
                        [--precision both|mixed|fp64] [--solves S] [--no-cpu] [--no-secondary]
@@ -160,6 +162,70 @@ def secondary_cavity(args):
             "ms_per_step": ms, "ns_per_cell_step": ms * 1e-3 / ncell * 1e9, "steps": nst, "divmax": divmax}
 
 
+def secondary_boxes(args):
+    """64 independent Poisson boxes (nx, ny from {32, 48, 64, 96, 128}, nz uniform in [32, 160],
+    tanh stretching in [0, 2], periodic x and y, Neumann z, seed 2024) solved as one batch
+    (cans_poisson_batch_solve: five launches for all boxes), FP64 and mixed storage, beside
+    the same boxes solved one cans_solver call at a time (five launches per box)."""
+    import numpy as np
+    import torch
+    from cans_amd import _lib as LB
+    from cans_amd import initflow as IF
+    from cans_amd.host import Context, Grid, Plan, PoissonBatch
+    from tests import cases
+    boxes = cases.poisson_boxes(64, seed=2024)
+    ncell = sum(int(np.prod(b["n"])) for b in boxes)
+    big = tuple(max(b["n"][d] for b in boxes) for d in range(3))
+    nrep = max(5, args.solves)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(nrep):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / nrep
+
+    out = {"workload": "64 independent Poisson boxes, nx,ny in {32,48,64,96,128}, nz in [32,160], "
+                       "stretching gr in [0,2], PP x/y, NN z (seed 2024)", "cells": ncell, "boxes": len(boxes)}
+    for name, prec, es in (("fp64", LB.CANS_FP64, 8), ("mixed", LB.CANS_MIXED, 4)):
+        print("[bench] %d Poisson boxes (%d cells), %s: set-up" % (len(boxes), ncell, name), file=sys.stderr,
+              flush=True)
+        ctx = Context(big, precision=prec)
+        batch = PoissonBatch(ctx, boxes)
+        rng = np.random.default_rng(6)
+        for i, b in enumerate(boxes):
+            batch.set(i, IF.halo(b["n"], cases.box_rhs(b, 1 / batch.dzfi[i][1:-1], rng)))
+        ms = timed(batch.solve)
+        nbytes = 10 * es * ncell  # 5 passes x (s_in + s_out) per cell
+        out[name] = {"ms_per_batch_solve": ms, "algorithmic_bytes": nbytes, "GBps": nbytes / (ms * 1e-3) / 1e9,
+                     "frac_hbm_peak": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        batch.close()
+        del batch
+        if name == "fp64":  # the same boxes, one cans_solver call each
+            single = []
+            for b in boxes:
+                p = dict(ng=b["n"], l=b["l"], gtype=1, gr=b["gr"], cbcvel="PP" * 9, cbcpre=b["cbc"], visci=100.0)
+                c1 = Context(b["n"], precision=prec)
+                pl = Plan(c1, Grid(cases.host_case(p)), b["cbc"], np.zeros(6))
+                f = c1.field(kind=LB.CANS_FIELD_PRESSURE)
+                single.append((c1, pl, f))
+            ms1 = timed(lambda: [pl.solve(f) for _, pl, f in single])
+            out[name]["ms_one_box_at_a_time"] = ms1
+            for c1, pl, f in single:
+                pl.close()
+                c1.close()
+            del single
+        ctx.close()
+        torch.cuda.empty_cache()
+        print("[bench] boxes %s: %.3f ms per batch solve (%.0f GB/s)" % (name, ms, out[name]["GBps"]),
+              file=sys.stderr, flush=True)
+    return out
+
+
 def copy_bandwidth():
     """Device copy of a known size: the box's streaming bandwidth."""
     import torch
@@ -258,7 +324,7 @@ def main():
     del final
     out["copy_GBps"] = copy_bandwidth()
     if not args.no_secondary:
-        out["secondary"] = [secondary_cavity(args)]
+        out["secondary"] = [secondary_cavity(args), secondary_boxes(args)]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, init)
     print(json.dumps(out))
