@@ -22,6 +22,7 @@ storage, beside the same boxes solved one at a time).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--ng N1 N2 N3]
                        [--precision both|mixed|fp64] [--solves S] [--no-cpu] [--no-secondary]
+                       [--boxes-only]
 """
 import argparse
 import json
@@ -38,7 +39,7 @@ DT = 2e-4
 # HBM bytes moved by one standalone Poisson solve, measured with rocprofv3 --pmc
 # (FETCH_SIZE x 2 + WRITE_SIZE x 1, the factors a 1 GiB device copy calibrates);
 # the per-kernel table is profiles/channel_1024x512x384_hbm_traffic.md
-SOLVE_TRAFFIC_BYTES = {((1024, 512, 384), "mixed"): 9.188e9, ((1024, 512, 384), "fp64"): 18.047e9}
+SOLVE_TRAFFIC_BYTES = {((1024, 512, 384), "mixed"): 9.185e9, ((1024, 512, 384), "fp64"): 18.046e9}
 TRAFFIC_SOURCE = "profiles/channel_1024x512x384_hbm_traffic.md"
 
 
@@ -256,11 +257,16 @@ def main():
     ap.add_argument("--solves", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--boxes-only", action="store_true", help="time only the 64 Poisson boxes (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs (--gpus %d) are not part of it" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+
+    if args.boxes_only:
+        print(json.dumps({"secondary": [secondary_boxes(args)]}))
+        return
 
     import numpy as np
     from cans_amd import _lib as LB

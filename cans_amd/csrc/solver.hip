@@ -3,7 +3,7 @@
 //
 // Data stay x-contiguous for the whole solve, so no pencil transposes exist:
 //   pass X  : rows along x are contiguous; one complex line carries two rows.
-//   pass Y  : a work-group holds a tile of x-columns x all y of one z-plane in LDS.
+//   pass Y  : a work-group holds a tile of x-columns x all y of a z-plane in LDS.
 //   pass Z  : one thread per (kx,ky) line, lines adjacent in x -> coalesced rows.
 // Spectral ordering matches the reference's GPU build: periodic directions are
 // stored packed as (r0, r_{n/2}, r1, i1, ..., r_{n/2-1}, i_{n/2-1}) (fft.f90:402-451,
@@ -119,7 +119,8 @@ __device__ __forceinline__ cplx<double> merge_pp(double are, double aim, double 
 // at offset (k%ZK)*ZT, and the blocks of one row block kb are ordered j fastest.
 // The z pass streams each wave's lines as contiguous ZK-row blocks; the x and y
 // passes move contiguous chunks of ZT (x) or 2*LINES (y) elements, and a y-pass
-// work-group (one plane, one column chunk, all j) stays inside n2 adjacent blocks.
+// work-group (consecutive column pairs of one plane, all j) stays inside n2 adjacent
+// blocks unless its lines run on into the next plane (narrow x).
 // nic = number of ZT-line chunks along x; PS = elements per row block (all tiles).
 // ---------------------------------------------------------------------------
 constexpr int ZT = 64;
@@ -375,7 +376,8 @@ __global__ void __launch_bounds__(512) k_xpass_d(XArgs<double> a) {
 }
 
 // ---------------------------------------------------------------------------
-// pass Y: in place on W; a work-group owns 2*LINES adjacent x-columns of one z-plane.
+// pass Y: in place on W; a work-group owns LINES consecutive column pairs (2*LINES
+// x-columns; after the last pair of a z-plane come the first of the next).
 // thread t -> line c = t % LINES, team thread lt = t / LINES; LDS element (e, c) at [e*LINES + c].
 // The two columns of a line are adjacent, so every global access is one 16-byte pair.
 // ---------------------------------------------------------------------------
@@ -397,11 +399,12 @@ __device__ __forceinline__ void ypass_body(const YArgs<T>& a, int bx, int sys) {
   const int LN = a.lines, N = a.N, Tn = a.Tn, kind = a.kind;
   const int c = threadIdx.x % LN, lt = threadIdx.x / LN;
   const bool act = lt < Tn;
-  const int ncolblk = (a.n1 + 2 * LN - 1) / (2 * LN);
-  const int k = bx / ncolblk;
-  const int i0 = (bx % ncolblk) * 2 * LN;
-  const int ia = i0 + 2 * c;
-  const bool va = act && ia < a.n1, vb = act && ia + 1 < a.n1;
+  // column pairs are numbered plane by plane; a work-group's lines may span planes
+  const int ncp = (a.n1 + 1) / 2;
+  const long g = (long)bx * LN + c;
+  const int k = (int)(g / ncp);
+  const int ia = 2 * (int)(g % ncp);
+  const bool va = act && k < a.n3, vb = va && ia + 1 < a.n1;
   const IxStride ix{c, LN};
   // element e (along y) of column ia in the tiled work buffer; columns ia, ia+1 are adjacent
   const int nic = wnic(a.n1);
@@ -1116,8 +1119,7 @@ static int ypass(cans_ctx* c, const cans_plan* p, T* const* W, int nsys, int dir
   a.kind = p->kind[1];
   a.tw = p->tw[1];
   a.mk = p->mk[1];
-  const int ncolblk = (a.n1 + 2 * lines - 1) / (2 * lines);
-  dim3 grid((unsigned)ncolblk * a.n3, nsys);
+  dim3 grid(cdiv((long)a.n3 * ((a.n1 + 1) / 2), lines), nsys);
   const size_t shm = (size_t)lines * a.N * sizeof(cplx<T>);
   if (dir > 0) launch_ypass<1, T>(grid, lines * a.Tn, shm, c->stream, a);
   else launch_ypass<-1, T>(grid, lines * a.Tn, shm, c->stream, a);
@@ -1653,7 +1655,7 @@ static int batch_create(cans_ctx* c, int nbox, const cans_poisson_box* bx, doubl
     // work-groups of each pass
     const long npairs = ((long)n2 * n3 + 1) / 2;
     const long gx = (npairs + b.pairs - 1) / b.pairs;
-    const long gy = (long)n3 * ((n1 + 2 * b.lines - 1) / (2 * b.lines));
+    const long gy = ((long)n3 * ((n1 + 1) / 2) + b.lines - 1) / b.lines;
     const long gz = (nl + 255) / 256;
     off[0][i + 1] = off[0][i] + (int)gx;
     off[1][i + 1] = off[1][i] + (int)gy;
