@@ -58,6 +58,13 @@ module cans_hip
   type, bind(C) :: cans_state
     type(c_ptr) :: u, v, w, p, pp, dudtrko, dvdtrko, dwdtrko
   end type cans_state
+  ! one independent Poisson problem of a batch: the arguments of one solver call
+  type, bind(C) :: cans_poisson_box
+    integer(c_int)         :: n(3)
+    integer(c_long)        :: sy, sz
+    character(kind=c_char) :: cbc(6)
+    type(c_ptr)            :: lambdaxy_h, a_h, b_h, c_h, p_d
+  end type cans_poisson_box
   !
   interface
     ! ---- context and memory (replace initmpi/workspaces device setup) ----
@@ -197,6 +204,24 @@ module cans_hip
       integer(c_int), intent(in) :: n(3), ng(3)
       real(c_double), value :: normfft
       character(kind=c_char), intent(in) :: bc(6), c_or_f(3)
+    end function
+    ! many independent Poisson problems of different sizes, five launches per solve for all of them
+    integer(c_int) function cans_poisson_batch_create(ctx, nbox, boxes, normfft, batch) &
+                   bind(C, name='cans_poisson_batch_create')
+      import :: c_int, c_ptr, c_double, cans_poisson_box
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nbox
+      type(cans_poisson_box), intent(in) :: boxes(*)
+      real(c_double), intent(out) :: normfft(*)
+      type(c_ptr), intent(out) :: batch
+    end function
+    integer(c_int) function cans_poisson_batch_solve(batch) bind(C, name='cans_poisson_batch_solve')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+    end function
+    integer(c_int) function cans_poisson_batch_destroy(batch) bind(C, name='cans_poisson_batch_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
     end function
     integer(c_int) function cans_solve_helmholtz(ctx, n, ng, arrplan, normfft, alpha, lambdaxy, a, b, c, &
                                                  rhsbx, rhsby, rhsbz, is_bound, cbc, c_or_f, p) &
