@@ -1,0 +1,82 @@
+"""Gradients through RK3 steps on the GPU (tri-periodic uniform grids, FP64).
+
+``kinetic_energy_after_steps(sim, u0, v0, w0, dt, nsteps, every)`` runs nsteps fused
+RK3 steps (cans_rk3_substep) from the interior velocity (u0, v0, w0) with zero
+pressure and returns the volume-averaged kinetic energy of the final velocity as a
+torch scalar whose backward pass runs the adjoint substeps
+(cans_rk3_substep_adjoint).  The forward pass keeps the velocity and pressure every
+`every` steps; the backward pass restarts from each checkpoint in turn (last first),
+recomputes its steps while keeping the velocity at the start of every substep, and
+runs their adjoints in reverse.  Recomputed steps repeat the forward ones bit for bit
+(the fused substep is deterministic), so the gradient is that of the computed map.
+
+Tensors are the fields' interiors in torch's (k, j, i) order: shape (n3, n2, n1),
+float64, on the context's device.
+"""
+import torch
+
+from cans_amd import _lib as LB
+
+
+def _interior(f):
+    return f.view[1:-1, 1:-1, 1:-1]
+
+
+def _addr(f, buf):
+    """Address of element (0,0,0) of a copy of field f's buffer."""
+    return buf.data_ptr() + buf.element_size() * f.off
+
+
+class _KineticEnergyAfterSteps(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, u0, v0, w0, sim, dt, nsteps, every):
+        if sim.ctx.precision != LB.CANS_FP64:
+            raise LB.CansError("the adjoint needs a CANS_FP64 simulation")
+        for f, a in zip((sim.u, sim.v, sim.w), (u0, v0, w0)):
+            f.buf.zero_()
+            _interior(f).copy_(a)
+        for f in (sim.p, sim.pp):
+            f.buf.zero_()
+        sim.init_bounds()
+        ckpt = {}
+        for s in range(nsteps):
+            if s % every == 0:
+                ckpt[s] = [f.buf.clone() for f in (sim.u, sim.v, sim.w, sim.p)]
+            sim.rk3_step(dt)
+        fin = [_interior(f).clone() for f in (sim.u, sim.v, sim.w)]
+        E = 0.5 * sum((x * x).mean() for x in fin)
+        ctx.sim, ctx.dt, ctx.nsteps, ctx.every, ctx.ckpt, ctx.fin = sim, dt, nsteps, every, ckpt, fin
+        return E
+
+    @staticmethod
+    def backward(ctx, gE):
+        sim, dt, nsteps, every = ctx.sim, ctx.dt, ctx.nsteps, ctx.every
+        ncell = ctx.fin[0].numel()
+        ub = [sim.ctx.field() for _ in range(3)]
+        fb = [sim.ctx.field() for _ in range(3)]
+        pp = sim.ctx.field(kind=LB.CANS_FIELD_PRESSURE)
+        for f, x in zip(ub, ctx.fin):  # dE/du = u / N
+            _interior(f).copy_(x * (gE / ncell))
+        fields = (sim.u, sim.v, sim.w, sim.p)
+        for s0 in sorted(ctx.ckpt, reverse=True):
+            for f, b in zip(fields, ctx.ckpt[s0]):
+                f.buf.copy_(b)
+            steps = []
+            for _ in range(s0, min(s0 + every, nsteps)):
+                starts = []
+                for irk in (1, 2, 3):
+                    starts.append([f.buf.clone() for f in (sim.u, sim.v, sim.w)])
+                    sim.rk3_substep(irk, dt)
+                steps.append(starts)
+            for starts in reversed(steps):
+                for irk in (3, 2, 1):
+                    q = [_addr(f, b) for f, b in zip((sim.u, sim.v, sim.w), starts[irk - 1])]
+                    sim.rk3_substep_adjoint(irk, dt, q, ub, fb, pp)
+            del steps
+        grads = [_interior(f).clone() for f in ub]
+        return grads[0], grads[1], grads[2], None, None, None, None
+
+
+def kinetic_energy_after_steps(sim, u0, v0, w0, dt, nsteps, every=4):
+    """Volume-averaged kinetic energy after nsteps RK3 steps from (u0, v0, w0); differentiable in them."""
+    return _KineticEnergyAfterSteps.apply(u0, v0, w0, sim, dt, nsteps, every)

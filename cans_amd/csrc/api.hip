@@ -606,3 +606,97 @@ extern "C" int cans_rk3_substep(cans_ctx* c, const cans_step_params* s, int irk,
     return fail(CANS_EINVAL, "cans_rk3_substep: null parameter array");
   return CANS_DISPATCH(c, rk3_substep<T>(c, s, irk, dt, st));
 }
+
+// ---------------------------------------------------------------------------
+// Adjoint of the RK3 substep (no counterpart in the reference; for gradients of a
+// loss on the final velocity).  Scope: FP64 contexts, tri-periodic uniform grids,
+// explicit diffusion, no forcing, scalars or buoyancy.  There the substep maps
+// (u, Fo) to (P(u + f1 F(u) + f2 Fo), F(u)) (main.f90:437-473, rk.f90:24-333), and
+// P = I - G L^-1 D is symmetric (G = -D^T), so its adjoint is
+//   wb = P ub,   ub <- wb + J(u)^T (f1 wb + fb),   fb <- f2 wb.
+// ---------------------------------------------------------------------------
+namespace {
+
+int adjoint_scope(cans_ctx* c, const cans_step_params* s, const char* who) {
+  const std::string w(who);
+  if (c->precision != CANS_FP64) return fail(CANS_EUNSUPPORTED, w + ": the adjoint needs a CANS_FP64 context");
+  if (c->imp) return fail(CANS_EUNSUPPORTED, w + ": the adjoint covers explicit diffusion");
+  if (!s) return CANS_OK;
+  for (int q = 0; q < 6; q++)
+    if (s->cbcpre[q] != 'P') return fail(CANS_EUNSUPPORTED, w + ": the adjoint covers tri-periodic boxes");
+  for (int q = 0; q < 18; q++)
+    if (s->cbcvel[q] != 'P') return fail(CANS_EUNSUPPORTED, w + ": the adjoint covers tri-periodic boxes");
+  for (int q = 0; q < 3; q++)
+    if (s->is_forced[q]) return fail(CANS_EUNSUPPORTED, w + ": the adjoint covers unforced flows");
+  if (s->nscal) return fail(CANS_EUNSUPPORTED, w + ": the adjoint covers flows without scalars");
+  const double dz = s->dzc_h[1];
+  for (int k = 0; k <= c->L.n3 + 1; k++)
+    if (std::fabs(s->dzc_h[k] - dz) > 1e-12 * dz || std::fabs(s->dzf_h[k] - dz) > 1e-12 * dz)
+      return fail(CANS_EUNSUPPORTED, w + ": the adjoint covers uniform grids");
+  return CANS_OK;
+}
+
+}  // namespace
+
+extern "C" int cans_mom_jt(cans_ctx* c, const int* n, const double* dli, double visc, const void* u, const void* v,
+                           const void* w, const void* gu, const void* gv, const void* gw, const void* bu,
+                           const void* bv, const void* bw, void* ou, void* ov, void* ow) {
+  CHECK_CTX("cans_mom_jt");
+  CHECK_N("cans_mom_jt", n);
+  int rc = adjoint_scope(c, nullptr, "cans_mom_jt");
+  if (rc) return rc;
+  if (!dli || !u || !v || !w || !gu || !gv || !gw || !bu || !bv || !bw || !ou || !ov || !ow)
+    return fail(CANS_EINVAL, "cans_mom_jt: null argument");
+  const void* rd[6] = {u, v, w, gu, gv, gw};
+  void* wr[3] = {ou, ov, ow};
+  for (void* o : wr)
+    for (const void* r : rd)
+      if (o == r) return fail(CANS_EINVAL, "cans_mom_jt: the output may not be a velocity or g field");
+  const double* q[3] = {(const double*)u, (const double*)v, (const double*)w};
+  const double* g[3] = {(const double*)gu, (const double*)gv, (const double*)gw};
+  const double* b[3] = {(const double*)bu, (const double*)bv, (const double*)bw};
+  double* o[3] = {(double*)ou, (double*)ov, (double*)ow};
+  return launch_mom_jt(c, q, g, b, o, dli, visc);
+}
+
+extern "C" int cans_rk3_substep_adjoint(cans_ctx* c, const cans_step_params* s, int irk, double dt, const void* u,
+                                        const void* v, const void* w, void* ub, void* vb, void* wb, void* fu,
+                                        void* fv, void* fw, void* pp) {
+  CHECK_CTX("cans_rk3_substep_adjoint");
+  if (!s || irk < 1 || irk > 3) return fail(CANS_EINVAL, "cans_rk3_substep_adjoint: bad argument");
+  if (!u || !v || !w || !ub || !vb || !wb || !fu || !fv || !fw || !pp)
+    return fail(CANS_EINVAL, "cans_rk3_substep_adjoint: null field");
+  if (!s->plan || !s->lambdaxy_d || !s->a_d || !s->b_d || !s->c_d || !s->dzci_d || !s->dzfi_d || !s->dzc_h ||
+      !s->dzf_h)
+    return fail(CANS_EINVAL, "cans_rk3_substep_adjoint: null parameter array");
+  int rc = adjoint_scope(c, s, "cans_rk3_substep_adjoint");
+  if (rc) return rc;
+  const double* rkpar = RKCOEFF[irk - 1];
+  const double f1 = rkpar[0] * dt, f2 = rkpar[1] * dt;
+  double* a[3] = {(double*)ub, (double*)vb, (double*)wb};
+  double* fb[3] = {(double*)fu, (double*)fv, (double*)fw};
+  double* P = (double*)pp;
+  // wb = P ub: fillps with dti = 1, solver, boundp, correc with dt = 1 (in place: correc reads each
+  // cell's input before it writes that cell)
+  if ((rc = plan_bounduvw(c, s->cbcvel, s->bcvel, s->nb, s->is_bound, s->dl, s->dzc_h, s->dzf_h, a[0], a[1], a[2],
+                          -1, CANS_HALO_ABSENT, CANS_HALO_ABSENT)))
+    return rc;
+  if ((rc = launch_fillps<double>(c, a[0], a[1], a[2], nullptr, nullptr, nullptr, nullptr, P, s->dzfi_d, 1., s->dli)))
+    return rc;
+  if ((rc = run_solver<double>(c, s->plan, s->normfft, s->lambdaxy_d, s->a_d, s->b_d, s->c_d, s->cbcpre, "ccc", P)))
+    return rc;
+  if ((rc = plan_boundp(c, s->cbcpre, s->bcpre, s->nb, s->is_bound, s->dl, s->dzc_h, P, CANS_HALO_ABSENT))) return rc;
+  if ((rc = launch_correc<double>(c, P, a[0], a[1], a[2], nullptr, nullptr, nullptr, nullptr, a[0], a[1], a[2],
+                                  s->dzci_d, 1., s->dli, nullptr)))
+    return rc;
+  // g = f1 wb + fb (with periodic ghosts), fb = f2 wb; then ub = wb + J(u)^T g
+  double* t[3];
+  if ((rc = ensure_tmp_fields<double>(c, t))) return rc;
+  if ((rc = launch_adj_combine(c, a, t, fb, f1, f2))) return rc;
+  if ((rc = plan_bounduvw(c, s->cbcvel, s->bcvel, s->nb, s->is_bound, s->dl, s->dzc_h, s->dzf_h, t[0], t[1], t[2],
+                          -1, CANS_HALO_ABSENT, CANS_HALO_ABSENT)))
+    return rc;
+  const double h[3] = {s->dli[0], s->dli[1], 1. / s->dzc_h[1]};
+  const double* q[3] = {(const double*)u, (const double*)v, (const double*)w};
+  return launch_mom_jt(c, q, t, a, a, h, s->visc);
+}

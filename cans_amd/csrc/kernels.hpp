@@ -175,6 +175,14 @@ int run_chkdt(cans_ctx* c, const T* u, const T* v, const T* w, const double* dzc
 template <typename T>
 int run_bulk_mean(cans_ctx* c, const T* q, const double* gvr, double* mean);
 
+// adjoint.hip (tri-periodic uniform grids, FP64)
+// out = base + J(q)^T g, J the Jacobian of mom_xyz_ad's right-hand side at q; h = (1/dx, 1/dy, 1/dz)
+int launch_mom_jt(cans_ctx* c, const double* const* q, const double* const* g, const double* const* base,
+                  double* const* out, const double* h, double visc);
+// g = f1 wb + fb, then fb = f2 wb (interior cells)
+int launch_adj_combine(cans_ctx* c, const double* const* wb, double* const* g, double* const* fb, double f1,
+                       double f2);
+
 // solver.hip
 // right-hand side of the pressure equation computed from the velocity (plus its
 // increments eu,ev,ew when given) inside the first transform pass

@@ -575,6 +575,16 @@ class Simulation:
         for irk in (1, 2, 3):
             self.rk3_substep(irk, dt)
 
+    def rk3_substep_adjoint(self, irk, dt, q, ub, fb, pp):
+        """cans_rk3_substep_adjoint: q = three device addresses of the velocity the forward
+        substep irk started from; ub, fb, pp: DeviceFields (adjoint velocity in/out, adjoint of
+        the stored right-hand side in/out, scratch)."""
+        LB.check(LB.lib().cans_rk3_substep_adjoint(self.ctx.h, C.byref(self.prm), irk, dt,
+                                                   *[C.c_void_p(x) for x in q],
+                                                   *[C.c_void_p(f.ptr) for f in ub],
+                                                   *[C.c_void_p(f.ptr) for f in fb], C.c_void_p(pp.ptr)),
+                 "cans_rk3_substep_adjoint")
+
     def capture_step(self, dt, nsteps=1):
         """Record nsteps fused RK3 steps of size dt into a HIP graph (torch.cuda.CUDAGraph)
         on the context's own stream; replay() then repeats them without host launch cost.

@@ -295,6 +295,30 @@ int cans_rk3_substep(cans_ctx *ctx, const cans_step_params *prm, int irk, double
 /* forcing of the last substep, device pointer to 3 doubles (f(1:3) of rk.f90) */
 int cans_ctx_forcing(cans_ctx *ctx, const double **f_d);
 
+/* ---------------------------------------------------------------------------
+ * Adjoint of the RK3 substep (no counterpart in the reference; for gradients of a
+ * loss on the velocity).  Scope: CANS_FP64 contexts, tri-periodic uniform grids,
+ * explicit diffusion, no forcing, scalars or buoyancy.  There substep irk maps
+ * (u, Fo) to (P(u + f1 F(u) + f2 Fo), F(u)), F the right-hand side of mom_xyz_ad
+ * (src/mom.f90:771-1019), f1, f2 = rkcoeff(1:2,irk)*dt, P the projection of
+ * fillps/solver/correc; the pressure drops out (P G p = 0).
+ * ------------------------------------------------------------------------- */
+/* out = base + J(q)^T g: J the Jacobian of mom_xyz_ad's right-hand side at the
+ * velocity q = (u,v,w); dli = (1/dx, 1/dy, 1/dz).  q and g are halo'd with their
+ * periodic ghost cells set; out may be base but no q or g field. */
+int cans_mom_jt(cans_ctx *ctx, const int *n, const double *dli, double visc, const void *u_d,
+                const void *v_d, const void *w_d, const void *gu_d, const void *gv_d, const void *gw_d,
+                const void *bu_d, const void *bv_d, const void *bw_d, void *ou_d, void *ov_d, void *ow_d);
+/* One substep backwards: u_d, v_d, w_d is the velocity the forward substep irk started
+ * from (halo'd, ghosts set); (ub, vb, wb) holds the adjoint of the substep's output
+ * velocity and receives that of its input; (fu, fv, fw) holds the adjoint of the
+ * right-hand side this substep stored for the next one (zero after substep 3) and
+ * receives that of the one it read.  pp_d is a halo'd scratch field.  prm as for
+ * cans_rk3_substep (plan, lambdaxy, grid arrays). */
+int cans_rk3_substep_adjoint(cans_ctx *ctx, const cans_step_params *prm, int irk, double dt,
+                             const void *u_d, const void *v_d, const void *w_d, void *ub_d,
+                             void *vb_d, void *wb_d, void *fu_d, void *fv_d, void *fw_d, void *pp_d);
+
 #ifdef __cplusplus
 }
 #endif
