@@ -16,13 +16,14 @@ with FP64 storage.  The oracle (C/OpenMP restatement) is timed on the host on a
 bounded number of full steps of the same case as the CPU baseline.
 
 Secondary workloads, timed after the headline one and reported under "secondary":
-the 256^3 heated cube with a scalar, buoyancy and implicit z diffusion (FP64), and
+the 256^3 heated cube with a scalar, buoyancy and implicit z diffusion (FP64),
 64 independent Poisson boxes of different sizes solved as one batch (FP64 and mixed
-storage, beside the same boxes solved one at a time).
+storage, beside the same boxes solved one at a time), and the gradient of the kinetic
+energy after 32 steps of the 256^3 Taylor-Green flow (forward and adjoint, FP64).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--ng N1 N2 N3]
                        [--precision both|mixed|fp64] [--solves S] [--no-cpu] [--no-secondary]
-                       [--boxes-only]
+                       [--boxes-only | --adjoint-only]
 """
 import argparse
 import json
@@ -227,6 +228,47 @@ def secondary_boxes(args):
     return out
 
 
+def secondary_adjoint(args):
+    """Gradient of the kinetic energy after 32 RK3 steps of the 256^3 Taylor-Green flow (nu = 1/1600,
+    dt = 5e-4, FP64) w.r.t. the initial velocity: forward with checkpoints every 4 steps, backward with
+    recompute and the adjoint substeps (cans_amd.adjoint)."""
+    import numpy as np
+    import torch
+    from cans_amd.adjoint import kinetic_energy_after_steps
+    from cans_amd.host import Simulation
+    from tests import cases
+    p = cases.params("tgv", ng=(256, 256, 256))
+    print("[bench] adjoint %s: set-up" % (p["ng"],), file=sys.stderr, flush=True)
+    sim = Simulation(cases.host_case(p))
+    u0 = cases.initial_state("tgv", p, sim.grid, seed=1)[:3]
+    dt, nsteps, every = 5e-4, 32, 4
+    res = {}
+    for rep in range(2):  # the first run warms up (allocations, code loading)
+        t0 = [torch.tensor(np.ascontiguousarray(a.transpose(2, 1, 0)), device="cuda", requires_grad=True)
+              for a in u0]
+        torch.cuda.synchronize()
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        e[0].record()
+        E = kinetic_energy_after_steps(sim, *t0, dt, nsteps, every)
+        e[1].record()
+        E.backward()
+        e[2].record()
+        torch.cuda.synchronize()
+        res = {"forward_ms": e[0].elapsed_time(e[1]), "backward_ms": e[1].elapsed_time(e[2])}
+    ncell = int(np.prod(p["ng"]))
+    sim.close()
+    del sim, t0, E
+    torch.cuda.empty_cache()
+    out = {"workload": "gradient of the kinetic energy after 32 RK3 steps, Taylor-Green 256^3, nu=1/1600, "
+                       "dt=5e-4, FP64, checkpoints every 4 steps", "steps": nsteps,
+           "forward_ms": res["forward_ms"], "backward_ms": res["backward_ms"],
+           "backward_over_forward": res["backward_ms"] / res["forward_ms"],
+           "ns_per_cell_step_forward": res["forward_ms"] * 1e-3 / nsteps / ncell * 1e9}
+    print("[bench] adjoint: forward %.1f ms, backward %.1f ms" % (res["forward_ms"], res["backward_ms"]),
+          file=sys.stderr, flush=True)
+    return out
+
+
 def copy_bandwidth():
     """Device copy of a known size: the box's streaming bandwidth."""
     import torch
@@ -258,14 +300,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--boxes-only", action="store_true", help="time only the 64 Poisson boxes (for profiling)")
+    ap.add_argument("--adjoint-only", action="store_true", help="time only the 256^3 adjoint (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs (--gpus %d) are not part of it" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
 
-    if args.boxes_only:
-        print(json.dumps({"secondary": [secondary_boxes(args)]}))
+    if args.boxes_only or args.adjoint_only:
+        print(json.dumps({"secondary": [secondary_boxes(args) if args.boxes_only else secondary_adjoint(args)]}))
         return
 
     import numpy as np
@@ -330,7 +373,7 @@ def main():
     del final
     out["copy_GBps"] = copy_bandwidth()
     if not args.no_secondary:
-        out["secondary"] = [secondary_cavity(args), secondary_boxes(args)]
+        out["secondary"] = [secondary_cavity(args), secondary_boxes(args), secondary_adjoint(args)]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, init)
     print(json.dumps(out))
