@@ -114,7 +114,7 @@ SIGNATURES = {
     "cans_rk3_substep": (C.c_int, [_vp, C.POINTER(StepParams), C.c_int, C.c_double, C.POINTER(State)]),
     "cans_ctx_forcing": (C.c_int, [_vp, C.POINTER(_vp)]),
     "cans_mom_jt": (C.c_int, [_vp, _ip, _dp, C.c_double] + [_vp] * 12),
-    "cans_rk3_substep_adjoint": (C.c_int, [_vp, C.POINTER(StepParams), C.c_int, C.c_double] + [_vp] * 10),
+    "cans_rk3_substep_adjoint": (C.c_int, [_vp, C.POINTER(StepParams), C.c_int, C.c_double] + [_vp] * 16),
 }
 
 _lib = None

@@ -52,8 +52,8 @@ class _KineticEnergyAfterSteps(torch.autograd.Function):
     def backward(ctx, gE):
         sim, dt, nsteps, every = ctx.sim, ctx.dt, ctx.nsteps, ctx.every
         ncell = ctx.fin[0].numel()
-        ub = [sim.ctx.field() for _ in range(3)]
-        fb = [sim.ctx.field() for _ in range(3)]
+        # adjoint velocity and right-hand side, each with a second set the substeps write into
+        ub, ubo, fb, fbo = ([sim.ctx.field() for _ in range(3)] for _ in range(4))
         pp = sim.ctx.field(kind=LB.CANS_FIELD_PRESSURE)
         for f, x in zip(ub, ctx.fin):  # dE/du = u / N
             _interior(f).copy_(x * (gE / ncell))
@@ -71,7 +71,8 @@ class _KineticEnergyAfterSteps(torch.autograd.Function):
             for starts in reversed(steps):
                 for irk in (3, 2, 1):
                     q = [_addr(f, b) for f, b in zip((sim.u, sim.v, sim.w), starts[irk - 1])]
-                    sim.rk3_substep_adjoint(irk, dt, q, ub, fb, pp)
+                    sim.rk3_substep_adjoint(irk, dt, q, ub, fb, ubo, fbo, pp)
+                    ub, ubo, fb, fbo = ubo, ub, fbo, fb
             del steps
         grads = [_interior(f).clone() for f in ub]
         return grads[0], grads[1], grads[2], None, None, None, None

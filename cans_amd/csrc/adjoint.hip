@@ -17,14 +17,18 @@ namespace cans {
 struct JtArgs {
   Layout L;
   const double* q[3];     // velocity at which the Jacobian is taken (halo'd, ghosts set)
-  const double* g[3];     // vector the transpose applies to (halo'd, ghosts set)
+  const double* g[3];     // vector the transpose applies to (halo'd, ghosts set); with COMB: wb
+  const double* fb[3];    // COMB: g = f1 wb + fb (fb halo'd, ghosts set)
   const double* base[3];  // added to the result (own cell only: may be out)
   double* out[3];
+  double* fbo[3];         // COMB: receives f2 wb
   double h[3];            // 1/dx, 1/dy, 1/dz
-  double visc;
+  double visc, f1, f2;
 };
 
-// one thread per cell, all three components
+// one thread per cell, all three components.  COMB: the adjoint substep's g = f1 wb + fb
+// is formed where it is read, and f2 wb is stored beside the result (no separate pass)
+template <bool COMB>
 __global__ void __launch_bounds__(256) k_mom_jt(JtArgs a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
@@ -32,13 +36,17 @@ __global__ void __launch_bounds__(256) k_mom_jt(JtArgs a) {
   if (i > a.L.n1 || j > a.L.n2) return;
   const long o = a.L.at(i, j, k);
   const long st[3] = {1, a.L.sy, a.L.sz};
+  auto G = [&](int c, long off) {
+    if constexpr (COMB) return a.f1 * a.g[c][o + off] + a.fb[c][o + off];
+    else return a.g[c][o + off];
+  };
 #pragma unroll
   for (int m = 0; m < 3; m++) {
     const long em = st[m];
-    const double* gm = a.g[m];
+    const double g0 = G(m, 0);
     double lap = 0.;
 #pragma unroll
-    for (int d = 0; d < 3; d++) lap += a.h[d] * a.h[d] * (gm[o + st[d]] - 2. * gm[o] + gm[o - st[d]]);
+    for (int d = 0; d < 3; d++) lap += a.h[d] * a.h[d] * (G(m, st[d]) - 2. * g0 + G(m, -st[d]));
     double acc = a.visc * lap;
     // g_m as the advected quantity: the coefficients K^m_d at y - e_d and y
 #pragma unroll
@@ -47,7 +55,7 @@ __global__ void __launch_bounds__(256) k_mom_jt(JtArgs a) {
       const double* qd = a.q[d];
       const double kmm = 0.25 * (qd[o - ed] + qd[o - ed + em]);
       const double km0 = 0.25 * (qd[o] + qd[o + em]);
-      acc -= a.h[d] * (kmm * gm[o - ed] - km0 * gm[o + ed]);
+      acc -= a.h[d] * (kmm * G(m, -ed) - km0 * G(m, ed));
     }
     // q_m as the advecting velocity of every component c (direction m)
     double s = 0.;
@@ -55,34 +63,12 @@ __global__ void __launch_bounds__(256) k_mom_jt(JtArgs a) {
     for (int c = 0; c < 3; c++) {
       const long ec = st[c];
       const double* qc = a.q[c];
-      const double* gc = a.g[c];
-      s += gc[o] * qc[o + em] - gc[o + em] * qc[o];
-      s += gc[o - ec] * qc[o - ec + em] - gc[o - ec + em] * qc[o - ec];
+      s += G(c, 0) * qc[o + em] - G(c, em) * qc[o];
+      s += G(c, -ec) * qc[o - ec + em] - G(c, -ec + em) * qc[o - ec];
     }
     acc -= 0.25 * a.h[m] * s;
     a.out[m][o] = a.base[m][o] + acc;
-  }
-}
-
-struct CombineArgs {
-  Layout L;
-  const double* wb[3];  // projected adjoint velocity
-  double* g[3];         // out: f1 wb + fb
-  double* fb[3];        // in: adjoint of the stored right-hand side; out: f2 wb
-  double f1, f2;
-};
-
-__global__ void __launch_bounds__(256) k_adj_combine(CombineArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-  const int k = blockIdx.z + 1;
-  if (i > a.L.n1 || j > a.L.n2) return;
-  const long o = a.L.at(i, j, k);
-#pragma unroll
-  for (int m = 0; m < 3; m++) {
-    const double w = a.wb[m][o];
-    a.g[m][o] = a.f1 * w + a.fb[m][o];
-    a.fb[m][o] = a.f2 * w;
+    if constexpr (COMB) a.fbo[m][o] = a.f2 * a.g[m][o];
   }
 }
 
@@ -95,29 +81,37 @@ int launch_mom_jt(cans_ctx* c, const double* const* q, const double* const* g, c
   for (int m = 0; m < 3; m++) {
     a.q[m] = q[m];
     a.g[m] = g[m];
+    a.fb[m] = nullptr;
     a.base[m] = base[m];
     a.out[m] = out[m];
+    a.fbo[m] = nullptr;
     a.h[m] = h[m];
   }
   a.visc = visc;
-  k_mom_jt<<<jt_grid(c->L), dim3(64, 4), 0, c->stream>>>(a);
+  a.f1 = a.f2 = 0.;
+  k_mom_jt<false><<<jt_grid(c->L), dim3(64, 4), 0, c->stream>>>(a);
   CANS_LAUNCH_CHECK("k_mom_jt");
   return CANS_OK;
 }
 
-int launch_adj_combine(cans_ctx* c, const double* const* wb, double* const* g, double* const* fb, double f1,
-                       double f2) {
-  CombineArgs a;
+int launch_mom_jt_adj(cans_ctx* c, const double* const* q, const double* const* wb, const double* const* fb,
+                      double* const* out, double* const* fbo, const double* h, double visc, double f1, double f2) {
+  JtArgs a;
   a.L = c->L;
   for (int m = 0; m < 3; m++) {
-    a.wb[m] = wb[m];
-    a.g[m] = g[m];
+    a.q[m] = q[m];
+    a.g[m] = wb[m];
     a.fb[m] = fb[m];
+    a.base[m] = wb[m];
+    a.out[m] = out[m];
+    a.fbo[m] = fbo[m];
+    a.h[m] = h[m];
   }
+  a.visc = visc;
   a.f1 = f1;
   a.f2 = f2;
-  k_adj_combine<<<jt_grid(c->L), dim3(64, 4), 0, c->stream>>>(a);
-  CANS_LAUNCH_CHECK("k_adj_combine");
+  k_mom_jt<true><<<jt_grid(c->L), dim3(64, 4), 0, c->stream>>>(a);
+  CANS_LAUNCH_CHECK("k_mom_jt");
   return CANS_OK;
 }
 

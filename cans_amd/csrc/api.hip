@@ -661,11 +661,16 @@ extern "C" int cans_mom_jt(cans_ctx* c, const int* n, const double* dli, double 
 
 extern "C" int cans_rk3_substep_adjoint(cans_ctx* c, const cans_step_params* s, int irk, double dt, const void* u,
                                         const void* v, const void* w, void* ub, void* vb, void* wb, void* fu,
-                                        void* fv, void* fw, void* pp) {
+                                        void* fv, void* fw, void* ubo, void* vbo, void* wbo, void* fuo, void* fvo,
+                                        void* fwo, void* pp) {
   CHECK_CTX("cans_rk3_substep_adjoint");
   if (!s || irk < 1 || irk > 3) return fail(CANS_EINVAL, "cans_rk3_substep_adjoint: bad argument");
-  if (!u || !v || !w || !ub || !vb || !wb || !fu || !fv || !fw || !pp)
-    return fail(CANS_EINVAL, "cans_rk3_substep_adjoint: null field");
+  const void* all[16] = {u, v, w, ub, vb, wb, fu, fv, fw, ubo, vbo, wbo, fuo, fvo, fwo, pp};
+  for (int q = 0; q < 16; q++) {
+    if (!all[q]) return fail(CANS_EINVAL, "cans_rk3_substep_adjoint: null field");
+    for (int r = 0; r < q; r++)
+      if (all[q] == all[r]) return fail(CANS_EINVAL, "cans_rk3_substep_adjoint: every field must be distinct");
+  }
   if (!s->plan || !s->lambdaxy_d || !s->a_d || !s->b_d || !s->c_d || !s->dzci_d || !s->dzfi_d || !s->dzc_h ||
       !s->dzf_h)
     return fail(CANS_EINVAL, "cans_rk3_substep_adjoint: null parameter array");
@@ -676,11 +681,13 @@ extern "C" int cans_rk3_substep_adjoint(cans_ctx* c, const cans_step_params* s, 
   double* a[3] = {(double*)ub, (double*)vb, (double*)wb};
   double* fb[3] = {(double*)fu, (double*)fv, (double*)fw};
   double* P = (double*)pp;
-  // wb = P ub: fillps with dti = 1, solver, boundp, correc with dt = 1 (in place: correc reads each
+  auto periodic = [&](double* const* f) {
+    return plan_bounduvw(c, s->cbcvel, s->bcvel, s->nb, s->is_bound, s->dl, s->dzc_h, s->dzf_h, f[0], f[1], f[2],
+                         -1, CANS_HALO_ABSENT, CANS_HALO_ABSENT);
+  };
+  // wb = P ub in place: fillps with dti = 1, solver, boundp, correc with dt = 1 (correc reads each
   // cell's input before it writes that cell)
-  if ((rc = plan_bounduvw(c, s->cbcvel, s->bcvel, s->nb, s->is_bound, s->dl, s->dzc_h, s->dzf_h, a[0], a[1], a[2],
-                          -1, CANS_HALO_ABSENT, CANS_HALO_ABSENT)))
-    return rc;
+  if ((rc = periodic(a))) return rc;
   if ((rc = launch_fillps<double>(c, a[0], a[1], a[2], nullptr, nullptr, nullptr, nullptr, P, s->dzfi_d, 1., s->dli)))
     return rc;
   if ((rc = run_solver<double>(c, s->plan, s->normfft, s->lambdaxy_d, s->a_d, s->b_d, s->c_d, s->cbcpre, "ccc", P)))
@@ -689,14 +696,11 @@ extern "C" int cans_rk3_substep_adjoint(cans_ctx* c, const cans_step_params* s, 
   if ((rc = launch_correc<double>(c, P, a[0], a[1], a[2], nullptr, nullptr, nullptr, nullptr, a[0], a[1], a[2],
                                   s->dzci_d, 1., s->dli, nullptr)))
     return rc;
-  // g = f1 wb + fb (with periodic ghosts), fb = f2 wb; then ub = wb + J(u)^T g
-  double* t[3];
-  if ((rc = ensure_tmp_fields<double>(c, t))) return rc;
-  if ((rc = launch_adj_combine(c, a, t, fb, f1, f2))) return rc;
-  if ((rc = plan_bounduvw(c, s->cbcvel, s->bcvel, s->nb, s->is_bound, s->dl, s->dzc_h, s->dzf_h, t[0], t[1], t[2],
-                          -1, CANS_HALO_ABSENT, CANS_HALO_ABSENT)))
-    return rc;
+  // (ubo, fbo) = (wb + J(u)^T (f1 wb + fb), f2 wb): one pass over wb and fb with periodic ghosts
+  if ((rc = periodic(a)) || (rc = periodic(fb))) return rc;
   const double h[3] = {s->dli[0], s->dli[1], 1. / s->dzc_h[1]};
   const double* q[3] = {(const double*)u, (const double*)v, (const double*)w};
-  return launch_mom_jt(c, q, t, a, a, h, s->visc);
+  double* o[3] = {(double*)ubo, (double*)vbo, (double*)wbo};
+  double* fo[3] = {(double*)fuo, (double*)fvo, (double*)fwo};
+  return launch_mom_jt_adj(c, q, a, fb, o, fo, h, s->visc, f1, f2);
 }

@@ -179,9 +179,10 @@ int run_bulk_mean(cans_ctx* c, const T* q, const double* gvr, double* mean);
 // out = base + J(q)^T g, J the Jacobian of mom_xyz_ad's right-hand side at q; h = (1/dx, 1/dy, 1/dz)
 int launch_mom_jt(cans_ctx* c, const double* const* q, const double* const* g, const double* const* base,
                   double* const* out, const double* h, double visc);
-// g = f1 wb + fb, then fb = f2 wb (interior cells)
-int launch_adj_combine(cans_ctx* c, const double* const* wb, double* const* g, double* const* fb, double f1,
-                       double f2);
+// the adjoint substep's last part: out = wb + J(q)^T (f1 wb + fb), fbo = f2 wb (wb, fb halo'd with
+// periodic ghosts; out and fbo distinct from every input)
+int launch_mom_jt_adj(cans_ctx* c, const double* const* q, const double* const* wb, const double* const* fb,
+                      double* const* out, double* const* fbo, const double* h, double visc, double f1, double f2);
 
 // solver.hip
 // right-hand side of the pressure equation computed from the velocity (plus its

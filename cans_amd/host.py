@@ -575,14 +575,15 @@ class Simulation:
         for irk in (1, 2, 3):
             self.rk3_substep(irk, dt)
 
-    def rk3_substep_adjoint(self, irk, dt, q, ub, fb, pp):
+    def rk3_substep_adjoint(self, irk, dt, q, ub, fb, ubo, fbo, pp):
         """cans_rk3_substep_adjoint: q = three device addresses of the velocity the forward
-        substep irk started from; ub, fb, pp: DeviceFields (adjoint velocity in/out, adjoint of
-        the stored right-hand side in/out, scratch)."""
+        substep irk started from; ub, fb: DeviceFields holding the adjoints of the substep's
+        output velocity and stored right-hand side (ub is overwritten); ubo, fbo receive those of
+        its input velocity and the right-hand side it read; pp: scratch DeviceField."""
         LB.check(LB.lib().cans_rk3_substep_adjoint(self.ctx.h, C.byref(self.prm), irk, dt,
                                                    *[C.c_void_p(x) for x in q],
-                                                   *[C.c_void_p(f.ptr) for f in ub],
-                                                   *[C.c_void_p(f.ptr) for f in fb], C.c_void_p(pp.ptr)),
+                                                   *[C.c_void_p(f.ptr) for f in ub + fb + ubo + fbo],
+                                                   C.c_void_p(pp.ptr)),
                  "cans_rk3_substep_adjoint")
 
     def capture_step(self, dt, nsteps=1):

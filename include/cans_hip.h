@@ -311,13 +311,16 @@ int cans_mom_jt(cans_ctx *ctx, const int *n, const double *dli, double visc, con
                 const void *bu_d, const void *bv_d, const void *bw_d, void *ou_d, void *ov_d, void *ow_d);
 /* One substep backwards: u_d, v_d, w_d is the velocity the forward substep irk started
  * from (halo'd, ghosts set); (ub, vb, wb) holds the adjoint of the substep's output
- * velocity and receives that of its input; (fu, fv, fw) holds the adjoint of the
- * right-hand side this substep stored for the next one (zero after substep 3) and
- * receives that of the one it read.  pp_d is a halo'd scratch field.  prm as for
- * cans_rk3_substep (plan, lambdaxy, grid arrays). */
+ * velocity, (fu, fv, fw) that of the right-hand side this substep stored for the next
+ * one (zero after substep 3).  (ubo, vbo, wbo) and (fuo, fvo, fwo) receive the adjoints of
+ * the substep's input velocity and of the right-hand side it read; ub..wb are
+ * overwritten (they receive P ub).  pp_d is a halo'd scratch field; all sixteen fields
+ * are distinct.  prm as for cans_rk3_substep (plan, lambdaxy, grid arrays). */
 int cans_rk3_substep_adjoint(cans_ctx *ctx, const cans_step_params *prm, int irk, double dt,
                              const void *u_d, const void *v_d, const void *w_d, void *ub_d,
-                             void *vb_d, void *wb_d, void *fu_d, void *fv_d, void *fw_d, void *pp_d);
+                             void *vb_d, void *wb_d, void *fu_d, void *fv_d, void *fw_d,
+                             void *ubo_d, void *vbo_d, void *wbo_d, void *fuo_d, void *fvo_d,
+                             void *fwo_d, void *pp_d);
 
 #ifdef __cplusplus
 }

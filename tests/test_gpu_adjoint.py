@@ -79,16 +79,17 @@ def test_substep_adjoint_matches_oracle(torch):
     rng = np.random.default_rng(2)
     q, ub, fb = ([rng.uniform(-1, 1, tuple(orc.n)) for _ in range(3)] for _ in range(3))
     fq, fub, ffb = put(sim, q), put(sim, ub), put(sim, fb)
+    fubo, ffbo = [sim.ctx.field() for _ in range(3)], [sim.ctx.field() for _ in range(3)]
     pp = sim.ctx.field()
     dt, irk = 3e-3, 2
-    sim.rk3_substep_adjoint(irk, dt, [f.ptr for f in fq], fub, ffb, pp)
+    sim.rk3_substep_adjoint(irk, dt, [f.ptr for f in fq], fub, ffb, fubo, ffbo, pp)
     sim.ctx.sync()
     f1, f2 = RKCOEFF[irk - 1][0] * dt, RKCOEFF[irk - 1][1] * dt
     wb = A.project(orc, ub)
     g = [f1 * a + c for a, c in zip(wb, fb)]
     ref_ub = [a + c for a, c in zip(wb, A.mom_jt(orc, q, g))]
     ref_fb = [f2 * a for a in wb]
-    e1, e2 = relerr([f.interior() for f in fub], ref_ub), relerr([f.interior() for f in ffb], ref_fb)
+    e1, e2 = relerr([f.interior() for f in fubo], ref_ub), relerr([f.interior() for f in ffbo], ref_fb)
     print("adjoint substep rel err: velocity %.3e, rhs %.3e" % (e1, e2), flush=True)
     assert e1 < 1e-12 and e2 < 1e-12
 
