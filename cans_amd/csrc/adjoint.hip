@@ -26,53 +26,129 @@ struct JtArgs {
   double visc, f1, f2;
 };
 
-// one thread per cell, all three components.  COMB: the adjoint substep's g = f1 wb + fb
-// is formed where it is read, and f2 wb is stored beside the result (no separate pass)
+// A work-group owns a JT_X x JT_Y tile of (i, j) and marches over JT_K planes of k.  The
+// planes k-1, k, k+1 of q and g (with one ghost cell around the tile) sit in LDS in three
+// rotating slots; each new plane is staged once, so every value is read from global
+// memory about (JT_X+2)(JT_Y+2)/(JT_X JT_Y) times instead of once per use.  COMB: the
+// adjoint substep's g = f1 wb + fb is formed while it is staged, and f2 wb is stored
+// beside the result.
+constexpr int JT_X = 64, JT_Y = 4, JT_K = 16;
+constexpr int JT_PX = JT_X + 2, JT_PLANE = (JT_X + 2) * (JT_Y + 2);
+
 template <bool COMB>
-__global__ void __launch_bounds__(256) k_mom_jt(JtArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-  const int k = blockIdx.z + 1;
-  if (i > a.L.n1 || j > a.L.n2) return;
-  const long o = a.L.at(i, j, k);
-  const long st[3] = {1, a.L.sy, a.L.sz};
-  auto G = [&](int c, long off) {
-    if constexpr (COMB) return a.f1 * a.g[c][o + off] + a.fb[c][o + off];
-    else return a.g[c][o + off];
+__global__ void __launch_bounds__(JT_X * JT_Y) k_mom_jt(JtArgs a) {
+  __shared__ double sq[3][3][JT_PLANE];  // [component][slot][cell of the padded plane]
+  __shared__ double sg[3][3][JT_PLANE];
+  const Layout& L = a.L;
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * JT_X + tx;
+  const int i0 = blockIdx.x * JT_X, j0 = blockIdx.y * JT_Y;  // padded cell (0, 0) is (i0, j0)
+  const int kb = blockIdx.z * JT_K, ke = min(kb + JT_K, L.n3);
+  auto stage = [&](int kk, int slot) {
+    for (int e = tid; e < JT_PLANE; e += JT_X * JT_Y) {
+      const int i = i0 + e % JT_PX, j = j0 + e / JT_PX;
+      const bool in = i <= L.n1 + 1 && j <= L.n2 + 1;
+      const long o = L.at(i, j, kk);
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        sq[c][slot][e] = in ? a.q[c][o] : 0.;
+        if constexpr (COMB) sg[c][slot][e] = in ? a.f1 * a.g[c][o] + a.fb[c][o] : 0.;
+        else sg[c][slot][e] = in ? a.g[c][o] : 0.;
+      }
+    }
   };
+  // the plane after next is loaded into registers before the current plane's arithmetic and
+  // stored to LDS after it, so its global-memory latency overlaps that work
+  constexpr int NE = (JT_PLANE + JT_X * JT_Y - 1) / (JT_X * JT_Y);  // plane cells per lane
+  double rq[NE][3], rg[NE][3];
+  auto load = [&](int kk) {
 #pragma unroll
-  for (int m = 0; m < 3; m++) {
-    const long em = st[m];
-    const double g0 = G(m, 0);
-    double lap = 0.;
+    for (int r = 0; r < NE; r++) {
+      const int e = tid + r * JT_X * JT_Y;
+      const int i = i0 + e % JT_PX, j = j0 + e / JT_PX;
+      const bool in = e < JT_PLANE && i <= L.n1 + 1 && j <= L.n2 + 1;
+      const long o = in ? L.at(i, j, kk) : 0;
 #pragma unroll
-    for (int d = 0; d < 3; d++) lap += a.h[d] * a.h[d] * (G(m, st[d]) - 2. * g0 + G(m, -st[d]));
-    double acc = a.visc * lap;
-    // g_m as the advected quantity: the coefficients K^m_d at y - e_d and y
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-      const long ed = st[d];
-      const double* qd = a.q[d];
-      const double kmm = 0.25 * (qd[o - ed] + qd[o - ed + em]);
-      const double km0 = 0.25 * (qd[o] + qd[o + em]);
-      acc -= a.h[d] * (kmm * G(m, -ed) - km0 * G(m, ed));
+      for (int c = 0; c < 3; c++) {
+        rq[r][c] = in ? a.q[c][o] : 0.;
+        if constexpr (COMB) rg[r][c] = in ? a.f1 * a.g[c][o] + a.fb[c][o] : 0.;
+        else rg[r][c] = in ? a.g[c][o] : 0.;
+      }
     }
-    // q_m as the advecting velocity of every component c (direction m)
-    double s = 0.;
+  };
+  auto store = [&](int slot) {
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-      const long ec = st[c];
-      const double* qc = a.q[c];
-      s += G(c, 0) * qc[o + em] - G(c, em) * qc[o];
-      s += G(c, -ec) * qc[o - ec + em] - G(c, -ec + em) * qc[o - ec];
+    for (int r = 0; r < NE; r++) {
+      const int e = tid + r * JT_X * JT_Y;
+      if (e < JT_PLANE) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          sq[c][slot][e] = rq[r][c];
+          sg[c][slot][e] = rg[r][c];
+        }
+      }
     }
-    acc -= 0.25 * a.h[m] * s;
-    a.out[m][o] = a.base[m][o] + acc;
-    if constexpr (COMB) a.fbo[m][o] = a.f2 * a.g[m][o];
+  };
+  stage(kb, 0);
+  stage(kb + 1, 1);
+  stage(kb + 2, 2);  // kb < n3, so kb + 2 <= n3 + 1
+  __syncthreads();
+  const int i = i0 + tx + 1, j = j0 + ty + 1;
+  const bool own = i <= L.n1 && j <= L.n2;
+  const int ec[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  for (int k = kb + 1; k <= ke; k++) {
+    const bool next = k + 2 <= ke + 1;
+    if (next) load(k + 2);
+    if (own) {
+      const int sl[3] = {(k - 1 - kb) % 3, (k - kb) % 3, (k + 1 - kb) % 3};  // slots of k-1, k, k+1
+      auto at = [&](int di, int dj) { return (ty + 1 + dj) * JT_PX + tx + 1 + di; };
+      auto Q = [&](int c, const int* d) { return sq[c][sl[1 + d[2]]][at(d[0], d[1])]; };
+      auto G = [&](int c, const int* d) { return sg[c][sl[1 + d[2]]][at(d[0], d[1])]; };
+      const long o = L.at(i, j, k);
+#pragma unroll
+      for (int m = 0; m < 3; m++) {
+        const int* em = ec[m];
+        const int z0[3] = {0, 0, 0};
+        const double g0 = G(m, z0);
+        double lap = 0.;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          const int p[3] = {ec[d][0], ec[d][1], ec[d][2]}, n[3] = {-ec[d][0], -ec[d][1], -ec[d][2]};
+          lap += a.h[d] * a.h[d] * (G(m, p) - 2. * g0 + G(m, n));
+        }
+        double acc = a.visc * lap;
+        // g_m as the advected quantity: the coefficients K^m_d at y - e_d and y
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          const int p[3] = {ec[d][0], ec[d][1], ec[d][2]}, n[3] = {-ec[d][0], -ec[d][1], -ec[d][2]};
+          const int nm[3] = {n[0] + em[0], n[1] + em[1], n[2] + em[2]};
+          const double kmm = 0.25 * (Q(d, n) + Q(d, nm));
+          const double km0 = 0.25 * (Q(d, z0) + Q(d, em));
+          acc -= a.h[d] * (kmm * G(m, n) - km0 * G(m, p));
+        }
+        // q_m as the advecting velocity of every component c (direction m)
+        double s = 0.;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          const int n[3] = {-ec[c][0], -ec[c][1], -ec[c][2]};
+          const int nm[3] = {n[0] + em[0], n[1] + em[1], n[2] + em[2]};
+          s += G(c, z0) * Q(c, em) - G(c, em) * Q(c, z0);
+          s += G(c, n) * Q(c, nm) - G(c, nm) * Q(c, n);
+        }
+        acc -= 0.25 * a.h[m] * s;
+        const double b = a.base[m][o];
+        a.out[m][o] = b + acc;
+        if constexpr (COMB) a.fbo[m][o] = a.f2 * b;  // base is wb
+      }
+    }
+    __syncthreads();  // every lane is done with plane k-1
+    if (next) {
+      store((k - 1 - kb) % 3);
+      __syncthreads();
+    }
   }
 }
 
-static dim3 jt_grid(const Layout& L) { return dim3(cdiv(L.n1, 64), cdiv(L.n2, 4), L.n3); }
+static dim3 jt_grid(const Layout& L) { return dim3(cdiv(L.n1, JT_X), cdiv(L.n2, JT_Y), cdiv(L.n3, JT_K)); }
 
 int launch_mom_jt(cans_ctx* c, const double* const* q, const double* const* g, const double* const* base,
                   double* const* out, const double* h, double visc) {
@@ -89,7 +165,7 @@ int launch_mom_jt(cans_ctx* c, const double* const* q, const double* const* g, c
   }
   a.visc = visc;
   a.f1 = a.f2 = 0.;
-  k_mom_jt<false><<<jt_grid(c->L), dim3(64, 4), 0, c->stream>>>(a);
+  k_mom_jt<false><<<jt_grid(c->L), dim3(JT_X, JT_Y), 0, c->stream>>>(a);
   CANS_LAUNCH_CHECK("k_mom_jt");
   return CANS_OK;
 }
@@ -110,7 +186,7 @@ int launch_mom_jt_adj(cans_ctx* c, const double* const* q, const double* const* 
   a.visc = visc;
   a.f1 = f1;
   a.f2 = f2;
-  k_mom_jt<true><<<jt_grid(c->L), dim3(64, 4), 0, c->stream>>>(a);
+  k_mom_jt<true><<<jt_grid(c->L), dim3(JT_X, JT_Y), 0, c->stream>>>(a);
   CANS_LAUNCH_CHECK("k_mom_jt");
   return CANS_OK;
 }
