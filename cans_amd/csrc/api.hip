@@ -704,3 +704,52 @@ extern "C" int cans_rk3_substep_adjoint(cans_ctx* c, const cans_step_params* s, 
   double* fo[3] = {(double*)fuo, (double*)fvo, (double*)fwo};
   return launch_mom_jt_adj(c, q, a, fb, o, fo, h, s->visc, f1, f2);
 }
+
+// ---------------------------------------------------------------------------
+// Poisson solve with iterative refinement (no counterpart in the reference, whose
+// solves run in one precision): p (FP64) holds b on entry and receives x with
+// L x = b.  The inner solves run in the context's pressure storage type (float
+// under CANS_MIXED: FP32 transforms and storage, FP64 elimination), the residuals
+// b - L x and the accumulation of x in FP64:
+//   x = S(b);  repeat sweeps times: x += S(b - L x)
+// with S the solver (solver.f90:17-112) and L its 7-point operator.  Cell-centred
+// (c_or_f "ccc").
+// ---------------------------------------------------------------------------
+extern "C" int cans_solver_refined(cans_ctx* c, const int* n, const int* ng, const cans_plan* plan, double normfft,
+                                   const double* lambdaxy, const double* a, const double* b, const double* cc,
+                                   const char* bc, const char* c_or_f, const double* dli, const double* dzci,
+                                   const double* dzfi, int sweeps, void* p) {
+  CHECK_CTX("cans_solver_refined");
+  CHECK_N("cans_solver_refined", n);
+  if (!ng || !plan || !lambdaxy || !a || !b || !cc || !bc || !c_or_f || !dli || !dzci || !dzfi || !p)
+    return fail(CANS_EINVAL, "cans_solver_refined: null argument");
+  if (std::strncmp(c_or_f, "ccc", 3)) return fail(CANS_EUNSUPPORTED, "cans_solver_refined: cell-centred systems only");
+  if (sweeps < 0 || sweeps > 8) return fail(CANS_EINVAL, "cans_solver_refined: sweeps must be in 0..8");
+  int bt[6];
+  for (int q = 0; q < 6; q++) {
+    if (bc[q] == 'P') bt[q] = 0;
+    else if (bc[q] == 'N') bt[q] = 1;
+    else if (bc[q] == 'D') bt[q] = 2;
+    else return fail(CANS_EINVAL, "cans_solver_refined: boundary types are P, N or D");
+  }
+  // scratch: the FP64 rhs b in tmp field 0; the inner solve's field in tmp field 2, viewed in the
+  // storage type with element 1 on a pair boundary (the float view lies inside tmp field 1's bytes)
+  double* td[3];
+  int rc = ensure_tmp_fields<double>(c, td);
+  if (rc) return rc;
+  const size_t fbytes = (size_t)c->L.sz * (c->L.n3 + 2) * sizeof(double);
+  double* x = (double*)p;
+  CANS_HIP_TRY(hipMemcpyAsync(td[0], x, fbytes, hipMemcpyDeviceToDevice, c->stream));
+  return CANS_DISPATCH(c, [&]() -> int {
+    T* tt[3];
+    int r = ensure_tmp_fields<T>(c, tt);
+    if (r) return r;
+    T* f = tt[2];
+    for (int s = 0; s <= sweeps; s++) {
+      if ((r = launch_residual<T>(c, td[0], s ? x : nullptr, f, dzci, dzfi, dli, bt))) return r;
+      if ((r = run_solver<T>(c, plan, normfft, lambdaxy, a, b, cc, bc, "ccc", f))) return r;
+      if ((r = launch_accumulate<T>(c, x, f, s == 0))) return r;
+    }
+    return CANS_OK;
+  }());
+}

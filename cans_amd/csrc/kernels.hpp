@@ -166,6 +166,13 @@ template <typename T>
 int launch_bound_dir(cans_ctx* c, const PlaneOps& ops);
 template <typename S>
 int launch_rhs_b(cans_ctx* c, S* p, const double* r, int dir, int plane, double norm);
+// iterative refinement of a Poisson solve: r = b - L x (x = nullptr: r = b) with homogeneous
+// cell-centred ghost values by boundary type bt[6] (0 P, 1 N, 2 D); x = (first ? 0 : x) + d
+template <typename S>
+int launch_residual(cans_ctx* c, const double* b, const double* x, S* r, const double* dzci, const double* dzfi,
+                    const double* dli, const int* bt);
+template <typename S>
+int launch_accumulate(cans_ctx* c, double* x, const S* d, int first);
 template <typename T>
 int run_chkdiv(cans_ctx* c, const T* u, const T* v, const T* w, const double* dzfi, const double* dli, double* tot,
                double* mx);

@@ -851,6 +851,71 @@ int run_bulk_mean(cans_ctx* c, const T* q, const double* gvr, double* mean) {
   return CANS_OK;
 }
 
+// Residual of the pressure equation for the iterative refinement of a Poisson solve:
+// r = b - L x at interior cells, L the 7-point operator whose eigenvalues and
+// tridiagonal coefficients the solver uses (initsolver.f90:85-187: second differences
+// with dxi^2, dyi^2 along x, y; dzfi(k) (dzci(k) (x(k+1) - x(k)) - dzci(k-1) (x(k) -
+// x(k-1))) along z), the ghost values homogeneous by boundary type (bound.f90:193-206
+// for cell-centred BCs): P the periodic image, N x(0) = x(1), D x(0) = -x(1).  x = nullptr
+// gives r = b.  bt[2d+s]: 0 P, 1 N, 2 D for direction d, side s.
+template <typename S>
+__global__ void k_residual(Layout L, const double* b, const double* x, S* r, const double* dzci,
+                           const double* dzfi, double dxi2, double dyi2, int4 bt0, int2 bt1) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y + 1, k = blockIdx.z + 1;
+  if (i > L.n1) return;
+  const long o = L.at(i, j, k);
+  if (!x) {
+    r[o] = (S)b[o];
+    return;
+  }
+  const int bt[6] = {bt0.x, bt0.y, bt0.z, bt0.w, bt1.x, bt1.y};
+  const double xc = x[o];
+  // neighbour along direction d (index pos in 1..n, stride st) on side s: interior, periodic image or ghost
+  auto nbd = [&](int d, int pos, int n, long st, int side) {
+    const int t = bt[2 * d + side];
+    if (side == 0 && pos > 1) return x[o - st];
+    if (side == 1 && pos < n) return x[o + st];
+    if (t == 0) return side == 0 ? x[o + (long)(n - 1) * st] : x[o - (long)(n - 1) * st];
+    return t == 1 ? xc : -xc;
+  };
+  const double xm = nbd(0, i, L.n1, 1, 0), xp = nbd(0, i, L.n1, 1, 1);
+  const double ym = nbd(1, j, L.n2, L.sy, 0), yp = nbd(1, j, L.n2, L.sy, 1);
+  const double zm = nbd(2, k, L.n3, L.sz, 0), zp = nbd(2, k, L.n3, L.sz, 1);
+  const double lap = (xp - 2. * xc + xm) * dxi2 + (yp - 2. * xc + ym) * dyi2 +
+                     ((zp - xc) * dzci[k] - (xc - zm) * dzci[k - 1]) * dzfi[k];
+  r[o] = (S)(b[o] - lap);
+}
+
+// x = (first ? 0 : x) + d at interior cells
+template <typename S>
+__global__ void k_accumulate(Layout L, double* x, const S* d, int first) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y + 1, k = blockIdx.z + 1;
+  if (i > L.n1) return;
+  const long o = L.at(i, j, k);
+  x[o] = first ? (double)d[o] : x[o] + (double)d[o];
+}
+
+template <typename S>
+int launch_residual(cans_ctx* c, const double* b, const double* x, S* r, const double* dzci, const double* dzfi,
+                    const double* dli, const int* bt) {
+  const Layout& L = c->L;
+  k_residual<S><<<row_grid(L, 256), 256, 0, c->stream>>>(L, b, x, r, dzci, dzfi, dli[0] * dli[0], dli[1] * dli[1],
+                                                          make_int4(bt[0], bt[1], bt[2], bt[3]),
+                                                          make_int2(bt[4], bt[5]));
+  CANS_LAUNCH_CHECK("k_residual");
+  return CANS_OK;
+}
+
+template <typename S>
+int launch_accumulate(cans_ctx* c, double* x, const S* d, int first) {
+  const Layout& L = c->L;
+  k_accumulate<S><<<row_grid(L, 256), 256, 0, c->stream>>>(L, x, d, first);
+  CANS_LAUNCH_CHECK("k_accumulate");
+  return CANS_OK;
+}
+
 #define CANS_STENCIL_INSTANTIATE(S)                                                                              \
   template int launch_mom_rk<S>(cans_ctx*, const MomLaunch<S>&);                                                \
   template int launch_add_forcing<S>(cans_ctx*, const S*, S*, int, int);                                        \
@@ -861,7 +926,10 @@ int run_bulk_mean(cans_ctx* c, const T* q, const double* gvr, double* mean) {
                                 double, const double*, S*);                                                      \
   template int launch_updatep<S>(cans_ctx*, const S*, S*, const double*, const double*, double, const double*, int); \
   template int launch_bound_dir<S>(cans_ctx*, const PlaneOps&);                                                  \
-  template int launch_rhs_b<S>(cans_ctx*, S*, const double*, int, int, double);
+  template int launch_rhs_b<S>(cans_ctx*, S*, const double*, int, int, double);                               \
+  template int launch_residual<S>(cans_ctx*, const double*, const double*, S*, const double*, const double*,   \
+                                  const double*, const int*);                                                  \
+  template int launch_accumulate<S>(cans_ctx*, double*, const S*, int);
 CANS_STENCIL_INSTANTIATE(double)
 CANS_STENCIL_INSTANTIATE(float)
 // reductions read the velocity, which is always double

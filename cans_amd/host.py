@@ -546,6 +546,17 @@ class Simulation:
     def solver(self, f=None):
         self.plan.solve(f or self.pp)
 
+    def solver_refined(self, f, sweeps):
+        """cans_solver_refined on an FP64 DeviceField f (velocity kind): rhs in, solution out; the inner
+        solves in the context's pressure storage type, the residual and the sum in FP64."""
+        g, pl = self.grid, self.plan
+        n = LB.ivec(g.n)
+        LB.check(LB.lib().cans_solver_refined(self.ctx.h, n, n, pl.h, pl.normfft,
+                                              *[C.c_void_p(t.data_ptr()) for t in (pl.lam, pl.a, pl.b, pl.c)],
+                                              pl.cbc.encode(), pl.c_or_f.encode(), LB.dvec(g.dli[:2]),
+                                              C.c_void_p(self.dzci_d.data_ptr()), C.c_void_p(self.dzfi_d.data_ptr()),
+                                              int(sweeps), C.c_void_p(f.ptr)), "cans_solver_refined")
+
     def chkdt(self):
         g = self.grid
         out = C.c_double()

@@ -161,6 +161,17 @@ int cans_poisson_batch_create(cans_ctx *ctx, int nbox, const cans_poisson_box *b
                               double *normfft, cans_poisson_batch **batch);
 int cans_poisson_batch_solve(cans_poisson_batch *batch);
 int cans_poisson_batch_destroy(cans_poisson_batch *batch);
+/* Poisson solve with iterative refinement (no counterpart in the reference): p_d is an
+ * FP64 halo'd field holding b on entry and x with L x = b on return, whatever the
+ * context's precision.  The inner solves (src/solver.f90:17, same arguments) run in the
+ * context's pressure storage type -- FP32 transforms and storage under CANS_MIXED --
+ * and the residuals b - L x and the sum x in FP64: x = S(b), then `sweeps` times
+ * x += S(b - L x), L the solver's 7-point operator with homogeneous cell-centred
+ * ghost values (c_or_f "ccc").  dli = (1/dx, 1/dy); dzci_d, dzfi_d (0:n3+1). */
+int cans_solver_refined(cans_ctx *ctx, const int *n, const int *ng, const cans_plan *arrplan,
+                        double normfft, const double *lambdaxy_d, const double *a_d, const double *b_d,
+                        const double *c_d, const char *bc, const char *c_or_f, const double *dli,
+                        const double *dzci_d, const double *dzfi_d, int sweeps, void *p_d);
 /* src/solve_helmholtz.f90:28 (3-D branch) -- rhsb*_d may be NULL */
 int cans_solve_helmholtz(cans_ctx *ctx, const int *n, const int *ng, const cans_plan *arrplan,
                          double normfft, double alpha, const double *lambdaxy_d,
