@@ -64,6 +64,17 @@ def test_errors_are_reported():
     assert rc == LB.CANS_EINVAL
     assert b"sizes" in L.cans_last_error()
     assert L.cans_eigenvalues(4, b"PN", b"c", None) == LB.CANS_EINVAL
+    # entry points added beside the reference's: argument checks before any device work
+    box = (LB.PoissonBox * 1)()
+    assert L.cans_poisson_batch_create(None, 1, box, None, ctypes.byref(h)) == LB.CANS_EINVAL
+    assert b"cans_poisson_batch_create" in L.cans_last_error()
+    assert L.cans_poisson_batch_solve(None) == LB.CANS_EINVAL
+    assert L.cans_poisson_batch_destroy(None) == LB.CANS_OK
+    assert L.cans_rk3_substep_adjoint(None, None, 1, 1e-3, *([None] * 16)) == LB.CANS_EINVAL
+    assert b"cans_rk3_substep_adjoint" in L.cans_last_error()
+    assert L.cans_mom_jt(None, None, None, 1.0, *([None] * 12)) == LB.CANS_EINVAL
+    assert L.cans_solver_refined(None, None, None, None, 1.0, None, None, None, None, b"PPPPNN", b"ccc", None,
+                                 None, None, 2, None) == LB.CANS_EINVAL
 
 
 def test_bc_rhs_matches_oracle_and_ghost_cells():
