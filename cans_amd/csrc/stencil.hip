@@ -53,9 +53,9 @@ struct MomArgs {
 };
 
 // Tiling of the momentum kernel: a work-group owns a TX x TY column of cells and
-// marches up KC planes in z.  Plane k+1 of u,v,w (with a one-cell ring) is staged
-// in LDS once per tile; three rotating plane buffers leave one barrier per plane;
-// the z-neighbours at k-1 travel in registers.
+// marches up KC planes in z.  Each plane of u,v,w (with a one-cell ring) is staged
+// in LDS once per tile, two planes ahead of the arithmetic; three rotating plane
+// buffers leave one barrier per plane; the z-neighbours at k-1 travel in registers.
 constexpr int MOM_TX = 64, MOM_TY = 8, MOM_KC = 32;
 constexpr int MOM_PX = MOM_TX + 2, MOM_PY = MOM_TY + 2;
 
@@ -109,11 +109,43 @@ __global__ void __launch_bounds__(MOM_TX* MOM_TY) k_mom_rk(MomArgs<S> a) {
     w_cpm = a.w[c0 - sz + sy];
     p_c = (double)a.p[c0];
   }
+  // plane k+2 is loaded into registers before plane k's arithmetic and stored to its slot
+  // after it (the slot plane k-1 used, free since the previous barrier), so the loads'
+  // latency overlaps that work; one barrier per plane
+  constexpr int NQ = (MOM_PX * MOM_PY + MOM_TX * MOM_TY - 1) / (MOM_TX * MOM_TY);  // ring cells per lane
+  double nu[NQ], nv[NQ], nw[NQ];
+  auto load = [&](int kk) {
+#pragma unroll
+    for (int r = 0; r < NQ; r++) {
+      const int q = t + r * MOM_TX * MOM_TY;
+      const int gi = i0 + q % MOM_PX, gj = j0 + q / MOM_PX;
+      if (q < MOM_PX * MOM_PY && gi <= L.n1 + 1 && gj <= L.n2 + 1) {
+        const long o = L.at(gi, gj, kk);
+        nu[r] = a.u[o];
+        nv[r] = a.v[o];
+        nw[r] = a.w[o];
+      }
+    }
+  };
+  auto store = [&](int slot) {
+#pragma unroll
+    for (int r = 0; r < NQ; r++) {
+      const int q = t + r * MOM_TX * MOM_TY;
+      const int pi = q % MOM_PX, pj = q / MOM_PX;
+      if (q < MOM_PX * MOM_PY && i0 + pi <= L.n1 + 1 && j0 + pj <= L.n2 + 1) {
+        su[slot][pj][pi] = nu[r];
+        sv[slot][pj][pi] = nv[r];
+        sw[slot][pj][pi] = nw[r];
+      }
+    }
+  };
   stage(kb, kb % 3);
+  stage(kb + 1, (kb + 1) % 3);  // kb <= n3, so kb + 1 <= n3 + 1
+  __syncthreads();
   double s0 = 0., s1 = 0., s2 = 0.;
   for (int k = kb; k <= ke; k++) {
-    stage(k + 1, (k + 1) % 3);
-    __syncthreads();
+    const bool next = k + 2 <= ke + 1;
+    if (next) load(k + 2);
     const int b0 = k % 3, b1 = (k + 1) % 3;
     const int pi = li + 1, pj = lj + 1;
     if (in) {
@@ -237,6 +269,8 @@ __global__ void __launch_bounds__(MOM_TX* MOM_TY) k_mom_rk(MomArgs<S> a) {
       w_cpm = w_cpc;
       p_c = p_kp;
     }
+    if (next) store((k + 2) % 3);
+    __syncthreads();
   }
   if (a.sum_mask) {
     __syncthreads();
