@@ -3,8 +3,8 @@
 // One "team" of T threads transforms one line of N = 2^m (or 3 * 2^m) complex values that
 // sits in LDS, addressed through an element map.  The LDS element type L may be
 // float or double, and the butterflies compute in L: FP32 lines give FP32
-// transforms (twiddles come from a double table, rounded once), FP64 lines FP64
-// ones.  Twiddle products use fused multiply-adds.  T = max(1, N/8): every stage
+// transforms (twiddles from a table in L: the double table rounded once), FP64
+// lines FP64 ones.  Twiddle products use fused multiply-adds.  T = max(1, N/8): every stage
 // leaves each thread exactly 8 (or N, when N < 8) values in registers.  All teams of a
 // work-group run the same N and call this together (it contains barriers).
 //
@@ -121,7 +121,7 @@ __device__ __forceinline__ cplx<T> tmulc(cplx<T> a, cplx<T> b) {
 // of the team size); it takes part in the barriers only
 template <typename L, int s, int R, int NBF, class IX>
 __device__ __forceinline__ void fft_stage(cplx<L>* buf, IX ix, int N, int lNs, int lt, int Tn,
-                                          const cplx<double>* __restrict__ tw, bool act = true) {
+                                          const cplx<L>* __restrict__ tw, bool act = true) {
   typedef L T;
   constexpr int LR = R == 2 ? 1 : R == 4 ? 2 : 3;
   const int NR = N >> LR;
@@ -145,17 +145,17 @@ __device__ __forceinline__ void fft_stage(cplx<L>* buf, IX ix, int N, int lNs, i
         const int step = k * (N >> (lNs + LR));  // k * N/(Ns*R)
         // W^r for r = 1..R-1 from the table entries W^1, W^2, W^4 (at most one product
         // deep), so only a few twiddles are live at a time
-        const cplx<T> w1 = {(T)tw[step].re, (T)tw[step].im};
+        const cplx<T> w1 = tw[step];
         if constexpr (R == 2) {
           v[bb][1] = (s < 0) ? tmul(v[bb][1], w1) : tmulc(v[bb][1], w1);
         } else {
-          const cplx<T> w2 = {(T)tw[2 * step].re, (T)tw[2 * step].im};
+          const cplx<T> w2 = tw[2 * step];
           const cplx<T> w3 = tmul(w1, w2);
           v[bb][1] = (s < 0) ? tmul(v[bb][1], w1) : tmulc(v[bb][1], w1);
           v[bb][2] = (s < 0) ? tmul(v[bb][2], w2) : tmulc(v[bb][2], w2);
           v[bb][3] = (s < 0) ? tmul(v[bb][3], w3) : tmulc(v[bb][3], w3);
           if constexpr (R == 8) {
-            const cplx<T> w4 = {(T)tw[4 * step].re, (T)tw[4 * step].im};
+            const cplx<T> w4 = tw[4 * step];
             const cplx<T> w5 = tmul(w4, w1), w6 = tmul(w4, w2), w7 = tmul(w4, w3);
             v[bb][4] = (s < 0) ? tmul(v[bb][4], w4) : tmulc(v[bb][4], w4);
             v[bb][5] = (s < 0) ? tmul(v[bb][5], w5) : tmulc(v[bb][5], w5);
@@ -191,7 +191,7 @@ __device__ __forceinline__ void bfly3(cplx<T>* v) {
 // up to three of them (j = lt, lt + Tn, lt + 2 Tn below N/3).
 template <typename L, int s, class IX>
 __device__ __forceinline__ void fft_stage3(cplx<L>* buf, IX ix, int N, int lt, int Tn,
-                                           const cplx<double>* __restrict__ tw, bool act = true) {
+                                           const cplx<L>* __restrict__ tw, bool act = true) {
   typedef L T;
   const int NR = N / 3;
   cplx<T> v[3][3];
@@ -209,8 +209,8 @@ __device__ __forceinline__ void fft_stage3(cplx<L>* buf, IX ix, int N, int lt, i
     const int j = lt + bb * Tn;
     if (act && j < NR) {
       if (j) {  // W_N^{r j}, r = 1, 2 (span Ns = N/3: k = j, step = j)
-        const cplx<T> w1 = {(T)tw[j].re, (T)tw[j].im};
-        const cplx<T> w2 = {(T)tw[2 * j].re, (T)tw[2 * j].im};
+        const cplx<T> w1 = tw[j];
+        const cplx<T> w2 = tw[2 * j];
         v[bb][1] = (s < 0) ? tmul(v[bb][1], w1) : tmulc(v[bb][1], w1);
         v[bb][2] = (s < 0) ? tmul(v[bb][2], w2) : tmulc(v[bb][2], w2);
       }
@@ -227,7 +227,7 @@ __device__ __forceinline__ void fft_stage3(cplx<L>* buf, IX ix, int N, int lt, i
 // stages run first with spans 1, 8, ..., then one radix-3 stage).
 template <typename L, int s, class IX>
 __device__ __forceinline__ void fft_line(cplx<L>* buf, IX ix, int N, int logN, int lt, int Tn,
-                         const cplx<double>* __restrict__ tw, int r3 = 0, bool act = true) {
+                         const cplx<L>* __restrict__ tw, int r3 = 0, bool act = true) {
   if (logN == 0 && !r3) return;
   if (logN == 1 && !r3) {
     fft_stage<L, s, 2, 1>(buf, ix, N, 0, lt, Tn, tw, act);

@@ -29,7 +29,7 @@ struct cans_plan {
   int r3[2];
   int kind[2];
   double normfft;
-  cans::cplx<double>* tw[2];  // W_N^t = exp(-2 pi i t/N), t < N
+  cans::cplx<double>* tw[2];  // W_N^t = exp(-2 pi i t/N), t < N; then the same N values rounded to float
   cans::cplx<double>* mk[2];  // exp(-i pi k/(2N)), k <= N (DCT/DST pre/post twiddles)
   // launch shapes chosen by the tuner at cans_fftini, [0] FP32 storage, [1] FP64 (0: heuristic)
   int xthreads[2] = {0, 0};   // x pass: threads per work-group (lines = threads / (N/8))
@@ -63,6 +63,13 @@ static bool fft_factor(int n, int* m, int* r3) {
 __device__ __forceinline__ int mk_pos(int e, int N) { return (e & 1) ? N - 1 - (e >> 1) : (e >> 1); }
 
 // ---------------------------------------------------------------------------
+// the twiddle table of an N-point line in the line's type (the float copy follows the N doubles)
+template <typename T>
+__device__ __forceinline__ const cplx<T>* line_tw(const cplx<double>* tw, int N) {
+  if constexpr (sizeof(T) == sizeof(double)) return (const cplx<T>*)tw;
+  else return (const cplx<T>*)(tw + N);
+}
+
 // Pre/post-processing around the complex FFT.  Two real lines A, B travel as
 // one complex line z = A + iB:
 //   R2HC   : A_k = (Z_k + conj Z_{N-k})/2, B_k = -i (Z_k - conj Z_{N-k})/2,
@@ -268,7 +275,7 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& a, int bx, int sys) {
       }
     }
     __syncthreads();
-    fft_line<T, -1>(smem, ix, N, a.logN, lt, Tn, a.tw, a.r3, act);
+    fft_line<T, -1>(smem, ix, N, a.logN, lt, Tn, line_tw<T>(a.tw, N), a.r3, act);
     if (!va) return;
     if (kind == K_PP) {
       const int H = N >> 1;
@@ -343,7 +350,7 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& a, int bx, int sys) {
       }
     }
     __syncthreads();
-    fft_line<T, +1>(smem, ix, N, a.logN, lt, Tn, a.tw, a.r3, act);
+    fft_line<T, +1>(smem, ix, N, a.logN, lt, Tn, line_tw<T>(a.tw, N), a.r3, act);
     if (!va) return;
     if (N == 1) {
       if (lt == 0) {
@@ -428,7 +435,7 @@ __device__ __forceinline__ void ypass_body(const YArgs<T>& a, int bx, int sys) {
       }
     }
     __syncthreads();
-    fft_line<T, -1>(smem, ix, N, a.logN, lt, Tn, a.tw, a.r3, act);
+    fft_line<T, -1>(smem, ix, N, a.logN, lt, Tn, line_tw<T>(a.tw, N), a.r3, act);
     if (!va) return;
     if (kind == K_PP) {
       const int H = N >> 1;
@@ -492,7 +499,7 @@ __device__ __forceinline__ void ypass_body(const YArgs<T>& a, int bx, int sys) {
       }
     }
     __syncthreads();
-    fft_line<T, +1>(smem, ix, N, a.logN, lt, Tn, a.tw, a.r3, act);
+    fft_line<T, +1>(smem, ix, N, a.logN, lt, Tn, line_tw<T>(a.tw, N), a.r3, act);
     if (!va) return;
     for (int e = lt; e < N; e += Tn) {
       cplx<double> z = widen(smem[ix(kind == K_PP ? e : mk_pos(e, N))]);
@@ -1051,12 +1058,22 @@ static void make_tables(int N, std::vector<cplx<double>>& tw, std::vector<cplx<d
   }
 }
 
+// bytes of a twiddle table as the device holds it: N double values, then N float ones
+static size_t tw_bytes(size_t n) { return n * (sizeof(cplx<double>) + sizeof(cplx<float>)); }
+static void tw_pack(const std::vector<cplx<double>>& tw, unsigned char* out) {
+  std::memcpy(out, tw.data(), tw.size() * sizeof(cplx<double>));
+  cplx<float>* f = (cplx<float>*)(out + tw.size() * sizeof(cplx<double>));
+  for (size_t t = 0; t < tw.size(); t++) f[t] = {(float)tw[t].re, (float)tw[t].im};
+}
+
 static int upload_tables(cans_plan* p, int d) {
   std::vector<cplx<double>> tw, mk;
   make_tables(p->len[d], tw, mk);
-  CANS_HIP_TRY(hipMalloc((void**)&p->tw[d], tw.size() * sizeof(cplx<double>)));
+  std::vector<unsigned char> twp(tw_bytes(tw.size()));
+  tw_pack(tw, twp.data());
+  CANS_HIP_TRY(hipMalloc((void**)&p->tw[d], twp.size()));
   CANS_HIP_TRY(hipMalloc((void**)&p->mk[d], mk.size() * sizeof(cplx<double>)));
-  CANS_HIP_TRY(hipMemcpy(p->tw[d], tw.data(), tw.size() * sizeof(cplx<double>), hipMemcpyHostToDevice));
+  CANS_HIP_TRY(hipMemcpy(p->tw[d], twp.data(), twp.size(), hipMemcpyHostToDevice));
   CANS_HIP_TRY(hipMemcpy(p->mk[d], mk.data(), mk.size() * sizeof(cplx<double>), hipMemcpyHostToDevice));
   return CANS_OK;
 }
@@ -1628,8 +1645,8 @@ static int batch_create(cans_ctx* c, int nbox, const cans_poisson_box* bx, doubl
       nf = nf * (b.kind[q] == K_PP ? 1. : 2.) * (x.n[q] + 0.);  // fft.f90:75,140
       std::vector<cplx<double>> tw, mk;
       make_tables(x.n[q], tw, mk);
-      fx[i].tw[q] = ar.take(tw.size() * sizeof(cplx<double>));
-      std::memcpy(&ar.host[fx[i].tw[q]], tw.data(), tw.size() * sizeof(cplx<double>));
+      fx[i].tw[q] = ar.take(tw_bytes(tw.size()));
+      tw_pack(tw, &ar.host[fx[i].tw[q]]);
       fx[i].mk[q] = ar.take(mk.size() * sizeof(cplx<double>));
       std::memcpy(&ar.host[fx[i].mk[q]], mk.data(), mk.size() * sizeof(cplx<double>));
     }
