@@ -23,7 +23,7 @@ energy after 32 steps of the 256^3 Taylor-Green flow (forward and adjoint, FP64)
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--ng N1 N2 N3]
                        [--precision both|mixed|fp64] [--solves S] [--no-cpu] [--no-secondary]
-                       [--boxes-only | --adjoint-only]
+                       [--cavity-only] [--boxes-only] [--adjoint-only]
 """
 import argparse
 import json
@@ -325,14 +325,17 @@ def main():
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--boxes-only", action="store_true", help="time only the 64 Poisson boxes (for profiling)")
     ap.add_argument("--adjoint-only", action="store_true", help="time only the 256^3 adjoint (for profiling)")
+    ap.add_argument("--cavity-only", action="store_true", help="time only the 256^3 heated cube (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs (--gpus %d) are not part of it" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
 
-    if args.boxes_only or args.adjoint_only:
-        print(json.dumps({"secondary": [secondary_boxes(args) if args.boxes_only else secondary_adjoint(args)]}))
+    only = [f for flag, f in ((args.cavity_only, secondary_cavity), (args.boxes_only, secondary_boxes),
+                              (args.adjoint_only, secondary_adjoint)) if flag]
+    if only:
+        print(json.dumps({"secondary": [f(args) for f in only]}))
         return
 
     import numpy as np
