@@ -68,7 +68,11 @@ __global__ void k_rdz(int n3, const double* dzci, const double* dzfi, double* ta
   tab[n3 + 2 + k] = in ? dzci[k] / dzfi[k + 1] : 0.;
 }
 
-template <typename S>
+// DELTA: the output is the increment over (u,v,w); DIVU: div(u) is written too; BUOY: the
+// Boussinesq term of the scalar sb.  Compile-time
+// choices, so that each form keeps only the pointers it uses in scalar registers (the
+// arguments otherwise spill to vector-register lanes inside the plane loop).
+template <typename S, bool DELTA, bool DIVU, bool BUOY>
 __global__ void __launch_bounds__(MOM_TX* MOM_TY) k_mom_rk(MomArgs<S> a) {
   __shared__ double su[3][MOM_PY][MOM_PX], sv[3][MOM_PY][MOM_PX], sw[3][MOM_PY][MOM_PX];
   const Layout& L = a.L;
@@ -223,7 +227,7 @@ __global__ void __launch_bounds__(MOM_TX* MOM_TY) k_mom_rk(MomArgs<S> a) {
       double wn = a.read_ro ? w_ccc + a.factor1 * dwdt + a.factor2 * rw +
                                   a.factor12 * (a.bforce[2] - dzci_k * (p_kp - p_c))
                             : w_ccc + a.factor1 * dwdt + a.factor12 * (a.bforce[2] - dzci_k * (p_kp - p_c));
-      if (a.sb) {  // Boussinesq buoyancy of the first scalar (rk.f90:163-177)
+      if constexpr (BUOY) {  // Boussinesq buoyancy of the first scalar (rk.f90:163-177)
         const double s_c = a.sb[c];
         un = un - a.bq[0] * (a.sb[c + 1] + s_c);
         vn = vn - a.bq[1] * (a.sb[c + sy] + s_c);
@@ -240,7 +244,7 @@ __global__ void __launch_bounds__(MOM_TX* MOM_TY) k_mom_rk(MomArgs<S> a) {
         vout = vn - a.hf12 * dvdtd;
         wout = wn - a.hf12 * dwdtd;
       }
-      if (a.delta) {
+      if constexpr (DELTA) {
         a.eu[c] = (S)(uout - u_ccc);
         a.ev[c] = (S)(vout - v_ccc);
         a.ew[c] = (S)(wout - w_ccc);
@@ -254,7 +258,7 @@ __global__ void __launch_bounds__(MOM_TX* MOM_TY) k_mom_rk(MomArgs<S> a) {
         a.rv[o] = (S)dvdt;
         a.rw[o] = (S)dwdt;
       }
-      if (a.divu)  // fillps.f90:38-50 association, without the 1/dt factor
+      if constexpr (DIVU)  // fillps.f90:38-50 association, without the 1/dt factor
         a.divu[o] = (S)((w_ccc - w_ccm) * dzfi_k + (v_ccc - v_cmc) * dyi + (u_ccc - u_mcc) * dxi);
       if (a.sum_mask) {
         s0 += un * a.gvr_f[k];
@@ -723,7 +727,18 @@ int launch_mom_rk(cans_ctx* c, const MomLaunch<S>& m) {
   for (int d = 0; d < 3; d++) a.bq[d] = a.factor12 * m.gacc[d] * m.beta * 0.5;
   a.imp = m.imp;
   a.hf12 = 0.5 * a.factor12;
-  k_mom_rk<S><<<grid, dim3(MOM_TX, MOM_TY), 0, c->stream>>>(a);
+  const dim3 blk(MOM_TX, MOM_TY);
+  const int form = (a.delta ? 4 : 0) | (a.divu ? 2 : 0) | (a.sb ? 1 : 0);
+  switch (form) {
+    case 0: k_mom_rk<S, false, false, false><<<grid, blk, 0, c->stream>>>(a); break;
+    case 1: k_mom_rk<S, false, false, true><<<grid, blk, 0, c->stream>>>(a); break;
+    case 2: k_mom_rk<S, false, true, false><<<grid, blk, 0, c->stream>>>(a); break;
+    case 3: k_mom_rk<S, false, true, true><<<grid, blk, 0, c->stream>>>(a); break;
+    case 4: k_mom_rk<S, true, false, false><<<grid, blk, 0, c->stream>>>(a); break;
+    case 5: k_mom_rk<S, true, false, true><<<grid, blk, 0, c->stream>>>(a); break;
+    case 6: k_mom_rk<S, true, true, false><<<grid, blk, 0, c->stream>>>(a); break;
+    default: k_mom_rk<S, true, true, true><<<grid, blk, 0, c->stream>>>(a); break;
+  }
   CANS_LAUNCH_CHECK("k_mom_rk");
   if (a.sum_mask)
     return reduce_partials(c, c->partials, (int)nb, 3, 0, c->dscal, c->partials + 3 * nb, 1, a.sum_mask, m.velf);
