@@ -143,6 +143,25 @@ __global__ void __launch_bounds__(MOM_TX* MOM_TY) k_mom_rk(MomArgs<S> a) {
       }
     }
   };
+  // the z-grid values of the tile's planes, read from LDS into vector registers in the plane
+  // loop (as uniform scalar loads they would take ~16 scalar registers, which this kernel's
+  // arguments already fill)
+  __shared__ double zg[MOM_KC][8];
+  for (int q = t; q < (ke - kb + 1) * 8; q += MOM_TX * MOM_TY) {
+    const int kk = kb + q / 8;
+    double v;
+    switch (q % 8) {
+      case 0: v = a.dzci[kk]; break;
+      case 1: v = a.dzci[kk - 1]; break;
+      case 2: v = a.dzfi[kk]; break;
+      case 3: v = a.dzfi[kk + 1]; break;
+      case 4: v = a.rdzm[kk]; break;
+      case 5: v = a.rdzp[kk]; break;
+      case 6: v = a.gvr_f[kk]; break;
+      default: v = a.gvr_c[kk]; break;
+    }
+    zg[q / 8][q % 8] = v;
+  }
   stage(kb, kb % 3);
   stage(kb + 1, (kb + 1) % 3);  // kb <= n3, so kb + 1 <= n3 + 1
   __syncthreads();
@@ -165,8 +184,9 @@ __global__ void __launch_bounds__(MOM_TX* MOM_TY) k_mom_rk(MomArgs<S> a) {
       const double u_ccp = su[b1][pj][pi], u_mcp = su[b1][pj][pi - 1];
       const double v_ccp = sv[b1][pj][pi], v_cmp = sv[b1][pj - 1][pi];
       const double w_ccp = sw[b1][pj][pi];
-      const double dzci_k = a.dzci[k], dzci_km = a.dzci[k - 1], dzfi_k = a.dzfi[k], dzfi_kp = a.dzfi[k + 1];
-      const double rdzm = a.rdzm[k], rdzp = a.rdzp[k];
+      const double* zk = zg[k - kb];
+      const double dzci_k = zk[0], dzci_km = zk[1], dzfi_k = zk[2], dzfi_kp = zk[3];
+      const double rdzm = zk[4], rdzp = zk[5];
       // x
       const double dudxp = (u_pcc - u_ccc) * dxi, dudxm = (u_ccc - u_mcc) * dxi;
       const double dudyp = (u_cpc - u_ccc) * dyi, dudym = (u_ccc - u_cmc) * dyi;
@@ -261,9 +281,9 @@ __global__ void __launch_bounds__(MOM_TX* MOM_TY) k_mom_rk(MomArgs<S> a) {
       if constexpr (DIVU)  // fillps.f90:38-50 association, without the 1/dt factor
         a.divu[o] = (S)((w_ccc - w_ccm) * dzfi_k + (v_ccc - v_cmc) * dyi + (u_ccc - u_mcc) * dxi);
       if (a.sum_mask) {
-        s0 += un * a.gvr_f[k];
-        s1 += vn * a.gvr_f[k];
-        s2 += wn * a.gvr_c[k];
+        s0 += un * zk[6];
+        s1 += vn * zk[6];
+        s2 += wn * zk[7];
       }
       // march: plane k becomes plane k-1
       u_ccm = u_ccc;
