@@ -1674,11 +1674,12 @@ static int batch_create(cans_ctx* c, int nbox, const cans_poisson_box* bx, doubl
     const long gx = (npairs + b.pairs - 1) / b.pairs;
     const long gy = ((long)n3 * ((n1 + 1) / 2) + b.lines - 1) / b.lines;
     const long gz = (nl + 255) / 256;
-    off[0][i + 1] = off[0][i] + (int)gx;
-    off[1][i + 1] = off[1][i] + (int)gy;
-    off[2][i + 1] = off[2][i] + (int)gz;
-    if ((long)off[0][i + 1] + gx > INT32_MAX / 2 || (long)off[1][i + 1] > INT32_MAX / 2)
-      return fail(CANS_EINVAL, "cans_poisson_batch_create: too many work-groups");
+    const long g[3] = {gx, gy, gz};
+    for (int q = 0; q < 3; q++) {
+      const long tot = (long)off[q][i] + g[q];
+      if (tot > INT32_MAX) return fail(CANS_EINVAL, "cans_poisson_batch_create: too many work-groups");
+      off[q][i + 1] = (int)tot;
+    }
     B->shm[0] = std::max(B->shm[0], (size_t)b.pairs * padded_len(b.N[0]) * sizeof(cplx<T>));
     B->shm[1] = std::max(B->shm[1], (size_t)b.lines * b.N[1] * sizeof(cplx<T>));
   }
