@@ -228,7 +228,7 @@ def secondary_boxes(args):
         ms = timed(batch.solve)
         nbytes = 10 * es * ncell  # 5 passes x (s_in + s_out) per cell
         out[name] = {"ms_per_batch_solve": ms, "algorithmic_bytes": nbytes, "GBps": nbytes / (ms * 1e-3) / 1e9,
-                     "frac_hbm_peak": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+                     "frac_hbm_peak": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "threads": batch.threads()}
         batch.close()
         del batch
         if name == "fp64":  # the same boxes, one cans_solver call each

@@ -92,6 +92,7 @@ SIGNATURES = {
     "cans_solver": (C.c_int, [_vp, _ip, _ip, _vp, C.c_double, _vp, _vp, _vp, _vp, _cp, _cp, _vp]),
     "cans_poisson_batch_create": (C.c_int, [_vp, C.c_int, C.POINTER(PoissonBox), _dp, C.POINTER(_vp)]),
     "cans_poisson_batch_solve": (C.c_int, [_vp]),
+    "cans_poisson_batch_tuning": (C.c_int, [_vp, _ip]),
     "cans_poisson_batch_destroy": (C.c_int, [_vp]),
     "cans_solver_refined": (C.c_int, [_vp, _ip, _ip, _vp, C.c_double, _vp, _vp, _vp, _vp, _cp, _cp, _dp, _vp, _vp,
                                       C.c_int, _vp]),

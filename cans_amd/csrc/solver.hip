@@ -1461,11 +1461,12 @@ namespace cans {
 // (each with its own arrplan, normfft, lambdaxy, a, b, c), all boxes solved together.
 // Each pass is one launch over every box: the work-groups of box b are
 // off[b] .. off[b+1]-1 of the grid, found by a binary search over off, and run the
-// single solve's pass body with that box's shapes.  Work-groups have BOX_THREADS
+// single solve's pass body with that box's shapes.  Work-groups have one size for all
+// boxes (128, 256 or 512 lanes, the fastest forward x and y passes at creation)
 // lanes; a box uses as many whole transform teams as fit and the lanes left over
 // only take part in the barriers.  Boxes are cell-centred with a non-periodic z.
 // ---------------------------------------------------------------------------
-constexpr int BOX_THREADS = 256;
+constexpr int BOX_THREADS = 512;  // the largest work-group size
 
 template <typename T>
 struct BoxDesc {
@@ -1580,6 +1581,7 @@ struct cans_poisson_batch {
   const int* off[3] = {};        // first work-group of each box: x passes, y passes, z pass
   int blocks[3] = {};            // work-groups per launch of each pass
   size_t shm[2] = {};            // LDS of the x and y launches
+  int threads = 256;             // lanes per x/y work-group (the tuner's choice)
 };
 
 namespace cans {
@@ -1593,6 +1595,28 @@ struct Arena {
     return o;
   }
 };
+
+template <typename T>
+static BoxArgs<T> box_args(const cans_poisson_batch* B, int pass) {
+  BoxArgs<T> a;
+  a.box = (const BoxDesc<T>*)B->desc;
+  a.off = B->off[pass];
+  a.nbox = B->nbox;
+  // the zero-wavenumber pin of run_solver_batch
+  a.pin_eps = 2.220446049250313e-16 * (sizeof(T) == sizeof(double) ? 1. : 4096.);
+  return a;
+}
+
+// the forward x and y passes of a batch solve
+template <typename T>
+static int batch_passes_fwd(cans_poisson_batch* B) {
+  cans_ctx* c = B->ctx;
+  k_box_xpass<1, T><<<B->blocks[0], B->threads, B->shm[0], c->stream>>>(box_args<T>(B, 0));
+  CANS_LAUNCH_CHECK("k_box_xpass");
+  k_box_ypass<1, T><<<B->blocks[1], B->threads, B->shm[1], c->stream>>>(box_args<T>(B, 1));
+  CANS_LAUNCH_CHECK("k_box_ypass");
+  return CANS_OK;
+}
 
 template <typename T>
 static int batch_create(cans_ctx* c, int nbox, const cans_poisson_box* bx, double* normfft, cans_poisson_batch* B) {
@@ -1652,8 +1676,6 @@ static int batch_create(cans_ctx* c, int nbox, const cans_poisson_box* bx, doubl
     }
     b.norm = 1. / nf;
     if (normfft) normfft[i] = b.norm;
-    b.pairs = BOX_THREADS / b.Tn[0];
-    b.lines = BOX_THREADS / b.Tn[1];
     const long nl = (long)n1 * n2;
     fx[i].nl = nl;
     fx[i].lam = ar.take(nl * sizeof(double));
@@ -1669,20 +1691,34 @@ static int batch_create(cans_ctx* c, int nbox, const cans_poisson_box* bx, doubl
     fx[i].nblk = (n3 + CK - 1) / CK;
     fx[i].ck = cktot;
     cktot += 2 * fx[i].nblk * nl;
-    // work-groups of each pass
-    const long npairs = ((long)n2 * n3 + 1) / 2;
-    const long gx = (npairs + b.pairs - 1) / b.pairs;
-    const long gy = ((long)n3 * ((n1 + 1) / 2) + b.lines - 1) / b.lines;
-    const long gz = (nl + 255) / 256;
-    const long g[3] = {gx, gy, gz};
-    for (int q = 0; q < 3; q++) {
-      const long tot = (long)off[q][i] + g[q];
-      if (tot > INT32_MAX) return fail(CANS_EINVAL, "cans_poisson_batch_create: too many work-groups");
-      off[q][i + 1] = (int)tot;
-    }
-    B->shm[0] = std::max(B->shm[0], (size_t)b.pairs * padded_len(b.N[0]) * sizeof(cplx<T>));
-    B->shm[1] = std::max(B->shm[1], (size_t)b.lines * b.N[1] * sizeof(cplx<T>));
+    // work-groups of the z pass (those of the x and y passes depend on the work-group size)
+    const long tot = (long)off[2][i] + (nl + 255) / 256;
+    if (tot > INT32_MAX) return fail(CANS_EINVAL, "cans_poisson_batch_create: too many work-groups");
+    off[2][i + 1] = (int)tot;
   }
+  // x and y passes with work-groups of bt lanes: teams, lines, work-groups and LDS per launch;
+  // false when a box's team does not fit or a launch would need more than 64 KB of LDS
+  auto shape = [&](int bt) {
+    B->shm[0] = B->shm[1] = 0;
+    for (int i = 0; i < nbox; i++) {
+      BoxDesc<T>& b = d[i];
+      if (b.Tn[0] > bt || b.Tn[1] > bt) return false;
+      b.pairs = bt / b.Tn[0];
+      b.lines = bt / b.Tn[1];
+      const long npairs = ((long)b.L.n2 * b.L.n3 + 1) / 2;
+      const long g[2] = {(npairs + b.pairs - 1) / b.pairs,
+                         ((long)b.L.n3 * ((b.L.n1 + 1) / 2) + b.lines - 1) / b.lines};
+      for (int q = 0; q < 2; q++) {
+        const long tot = (long)off[q][i] + g[q];
+        if (tot > INT32_MAX) return false;
+        off[q][i + 1] = (int)tot;
+      }
+      B->shm[0] = std::max(B->shm[0], (size_t)b.pairs * padded_len(b.N[0]) * sizeof(cplx<T>));
+      B->shm[1] = std::max(B->shm[1], (size_t)b.lines * b.N[1] * sizeof(cplx<T>));
+    }
+    return B->shm[0] <= 65536 && B->shm[1] <= 65536;
+  };
+  if (!shape(256)) return fail(CANS_EINVAL, "cans_poisson_batch_create: transform lengths too large");
   // the work buffer and checkpoints follow the tables in the device allocation only
   const size_t W_at = (ar.host.size() + 255) & ~(size_t)255;
   const size_t ck_at = (W_at + (size_t)wtot * sizeof(T) + 255) & ~(size_t)255;
@@ -1705,19 +1741,57 @@ static int batch_create(cans_ctx* c, int nbox, const cans_poisson_box* bx, doubl
     b.ckd = (double*)(base + ck_at) + fx[i].ck;
     b.ckp = b.ckd + fx[i].nblk * fx[i].nl;
   }
-  std::memcpy(&ar.host[desc_at], d.data(), sizeof(BoxDesc<T>) * nbox);
-  for (int q = 0; q < 3; q++) std::memcpy(&ar.host[off_at[q]], off[q].data(), sizeof(int) * (nbox + 1));
   // the work buffer and checkpoints need no initial values
-  e = hipMemcpy(mem, ar.host.data(), ar.host.size(), hipMemcpyHostToDevice);
-  if (e != hipSuccess) {
-    hipFree(mem);
-    return fail(CANS_EHIP, std::string("cans_poisson_batch_create: ") + hipGetErrorString(e));
-  }
+  auto upload = [&]() -> int {
+    std::memcpy(&ar.host[desc_at], d.data(), sizeof(BoxDesc<T>) * nbox);
+    for (int q = 0; q < 3; q++) std::memcpy(&ar.host[off_at[q]], off[q].data(), sizeof(int) * (nbox + 1));
+    CANS_HIP_TRY(hipMemcpy(mem, ar.host.data(), ar.host.size(), hipMemcpyHostToDevice));
+    for (int q = 0; q < 3; q++) B->blocks[q] = off[q][nbox];
+    return CANS_OK;
+  };
   B->mem = mem;
   B->desc = base + desc_at;
-  for (int q = 0; q < 3; q++) {
-    B->off[q] = (const int*)(base + off_at[q]);
-    B->blocks[q] = off[q][nbox];
+  for (int q = 0; q < 3; q++) B->off[q] = (const int*)(base + off_at[q]);
+  // launch-shape tuner: time the forward x and y passes (they read the fields and write only the
+  // batch's own work buffers) for each work-group size and keep the fastest
+  auto tune = [&]() -> int {
+    hipEvent_t ev[2];
+    CANS_HIP_TRY(hipEventCreate(&ev[0]));
+    if (hipEventCreate(&ev[1]) != hipSuccess) {
+      hipEventDestroy(ev[0]);
+      return fail(CANS_EHIP, "cans_poisson_batch_create: hipEventCreate");
+    }
+    float best_ms = 0.f;
+    int best = 0, r = CANS_OK;
+    for (int bt : {128, 256, 512}) {
+      if (!shape(bt)) continue;
+      if ((r = upload())) break;
+      B->threads = bt;
+      float ms = 0.f;
+      for (int rep = 0; rep < 2 && !r; rep++) {  // the first run warms up
+        if (hipEventRecord(ev[0], c->stream) != hipSuccess) r = fail(CANS_EHIP, "cans_poisson_batch_create: timing");
+        if (!r) r = batch_passes_fwd<T>(B);
+        if (!r && (hipEventRecord(ev[1], c->stream) != hipSuccess || hipEventSynchronize(ev[1]) != hipSuccess ||
+                   hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess))
+          r = fail(CANS_EHIP, "cans_poisson_batch_create: timing");
+      }
+      if (r) break;
+      if (!best || ms < best_ms) {
+        best = bt;
+        best_ms = ms;
+      }
+    }
+    hipEventDestroy(ev[0]);
+    hipEventDestroy(ev[1]);
+    if (r) return r;
+    if (!best || !shape(best)) return fail(CANS_EINVAL, "cans_poisson_batch_create: no launch shape fits");
+    B->threads = best;
+    return upload();
+  };
+  if (int rc = tune()) {
+    hipFree(mem);
+    B->mem = nullptr;
+    return rc;
   }
   return CANS_OK;
 }
@@ -1725,23 +1799,13 @@ static int batch_create(cans_ctx* c, int nbox, const cans_poisson_box* bx, doubl
 template <typename T>
 static int batch_solve(cans_poisson_batch* B) {
   cans_ctx* c = B->ctx;
-  BoxArgs<T> a[3];
-  for (int q = 0; q < 3; q++) {
-    a[q].box = (const BoxDesc<T>*)B->desc;
-    a[q].off = B->off[q];
-    a[q].nbox = B->nbox;
-    // the zero-wavenumber pin of run_solver_batch
-    a[q].pin_eps = 2.220446049250313e-16 * (sizeof(T) == sizeof(double) ? 1. : 4096.);
-  }
-  k_box_xpass<1, T><<<B->blocks[0], BOX_THREADS, B->shm[0], c->stream>>>(a[0]);
-  CANS_LAUNCH_CHECK("k_box_xpass");
-  k_box_ypass<1, T><<<B->blocks[1], BOX_THREADS, B->shm[1], c->stream>>>(a[1]);
-  CANS_LAUNCH_CHECK("k_box_ypass");
-  k_box_thomas<T, sizeof(T) == sizeof(float)><<<B->blocks[2], 256, 0, c->stream>>>(a[2]);
+  int rc = batch_passes_fwd<T>(B);
+  if (rc) return rc;
+  k_box_thomas<T, sizeof(T) == sizeof(float)><<<B->blocks[2], 256, 0, c->stream>>>(box_args<T>(B, 2));
   CANS_LAUNCH_CHECK("k_box_thomas");
-  k_box_ypass<-1, T><<<B->blocks[1], BOX_THREADS, B->shm[1], c->stream>>>(a[1]);
+  k_box_ypass<-1, T><<<B->blocks[1], B->threads, B->shm[1], c->stream>>>(box_args<T>(B, 1));
   CANS_LAUNCH_CHECK("k_box_ypass");
-  k_box_xpass<-1, T><<<B->blocks[0], BOX_THREADS, B->shm[0], c->stream>>>(a[0]);
+  k_box_xpass<-1, T><<<B->blocks[0], B->threads, B->shm[0], c->stream>>>(box_args<T>(B, 0));
   CANS_LAUNCH_CHECK("k_box_xpass");
   return CANS_OK;
 }
@@ -1769,6 +1833,12 @@ extern "C" int cans_poisson_batch_solve(cans_poisson_batch* B) {
   using namespace cans;
   if (!B) return fail(CANS_EINVAL, "cans_poisson_batch_solve: null batch");
   return B->es == 4 ? batch_solve<float>(B) : batch_solve<double>(B);
+}
+
+extern "C" int cans_poisson_batch_tuning(const cans_poisson_batch* B, int* threads) {
+  if (!B || !threads) return fail(CANS_EINVAL, "cans_poisson_batch_tuning: null argument");
+  *threads = B->threads;
+  return CANS_OK;
 }
 
 extern "C" int cans_poisson_batch_destroy(cans_poisson_batch* B) {

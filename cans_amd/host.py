@@ -362,6 +362,12 @@ class PoissonBatch:
     def solve(self):
         LB.check(LB.lib().cans_poisson_batch_solve(self.h), "cans_poisson_batch_solve")
 
+    def threads(self):
+        """Lanes per x/y work-group the batch's tuner chose."""
+        t = C.c_int()
+        LB.check(LB.lib().cans_poisson_batch_tuning(self.h, C.byref(t)), "cans_poisson_batch_tuning")
+        return t.value
+
 
 class Simulation:
     """One case on one GPU: set-up of src/main.f90:241-395 and its RK3 loop."""

@@ -160,6 +160,9 @@ typedef struct cans_poisson_box {
 int cans_poisson_batch_create(cans_ctx *ctx, int nbox, const cans_poisson_box *boxes,
                               double *normfft, cans_poisson_batch **batch);
 int cans_poisson_batch_solve(cans_poisson_batch *batch);
+/* lanes per work-group of the batch's x and y passes (128, 256 or 512), chosen at
+ * creation by timing the forward passes (no counterpart in the reference) */
+int cans_poisson_batch_tuning(const cans_poisson_batch *batch, int *threads);
 int cans_poisson_batch_destroy(cans_poisson_batch *batch);
 /* Poisson solve with iterative refinement (no counterpart in the reference): p_d is an
  * FP64 halo'd field holding b on entry and x with L x = b on return, whatever the

@@ -136,7 +136,8 @@ def test_boxes_64_workload(torch):
         err = relerr(gauge(batch.get(i)[1:-1, 1:-1, 1:-1], b, dzf), gauge(ref[1:-1, 1:-1, 1:-1], b, dzf))
         worst = max(worst, err)
         assert err < RTOL, (i, b, err)
-    print("64 boxes: worst rel err %.3e" % worst, flush=True)
+    print("64 boxes: worst rel err %.3e, x/y work-groups of %d lanes" % (worst, batch.threads()), flush=True)
+    assert batch.threads() in (128, 256, 512)
     batch.close()
 
 
