@@ -1,11 +1,12 @@
-// LDS-resident batched complex FFT building block (radix 8/4/2/3 Stockham).
+// LDS-resident batched complex FFT building block (radix 8/4/2, then 3/5, Stockham).
 //
-// One "team" of T threads transforms one line of N = 2^m (or 3 * 2^m) complex values that
+// One "team" of T threads transforms one line of N = 2^m (or 2^m 3^a 5^b) complex values that
 // sits in LDS, addressed through an element map.  The LDS element type L may be
 // float or double, and the butterflies compute in L: FP32 lines give FP32
 // transforms (twiddles from a table in L: the double table rounded once), FP64
-// lines FP64 ones.  Twiddle products use fused multiply-adds.  T = max(1, N/8): every stage
-// leaves each thread exactly 8 (or N, when N < 8) values in registers.  All teams of a
+// lines FP64 ones.  Twiddle products use fused multiply-adds.  T = max(1, N/8): every
+// power-of-two stage leaves each thread exactly 8 (or N, when N < 8) values in registers, an
+// odd stage of radix P up to P * ceil(8/P).  All teams of a
 // work-group run the same N and call this together (it contains barriers).
 //
 // Formulation: the Stockham autosort iteration of Govindaraju et al., "High
@@ -186,54 +187,87 @@ __device__ __forceinline__ void bfly3(cplx<T>* v) {
   v[2] = csub(t2, t3);
 }
 
-// Last stage of a line of N = 3 * 2^m (m >= 3): radix 3 with span Ns = N/3.  The
-// N/3 butterflies do not split evenly over the team's N/8 threads; each thread takes
-// up to three of them (j = lt, lt + Tn, lt + 2 Tn below N/3).
-template <typename L, int s, class IX>
-__device__ __forceinline__ void fft_stage3(cplx<L>* buf, IX ix, int N, int lt, int Tn,
-                                           const cplx<L>* __restrict__ tw, bool act = true) {
+// radix 5: y_r = sum_q x_q W5^{r q}, W5 = exp(s 2 pi i/5).  With t1 = x1 + x4, t2 = x2 + x3,
+// t3 = x1 - x4, t4 = x2 - x3, c_n = cos(2 pi n/5), s_n = sin(2 pi n/5):
+// y_{1,4} = x0 + c1 t1 + c2 t2 +- s i (s1 t3 + s2 t4), y_{2,3} = x0 + c2 t1 + c1 t2 +- s i (s2 t3 - s1 t4)
+template <typename T, int s>
+__device__ __forceinline__ void bfly5(cplx<T>* v) {
+  const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;
+  const T s1 = (T)0.95105651629515357212, s2 = (T)0.58778525229247312917;
+  const cplx<T> t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+  const cplx<T> t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+  const cplx<T> a1 = {v[0].re + c1 * t1.re + c2 * t2.re, v[0].im + c1 * t1.im + c2 * t2.im};
+  const cplx<T> a2 = {v[0].re + c2 * t1.re + c1 * t2.re, v[0].im + c2 * t1.im + c1 * t2.im};
+  const cplx<T> b1 = mul_si<T, s>(cplx<T>{s1 * t3.re + s2 * t4.re, s1 * t3.im + s2 * t4.im});
+  const cplx<T> b2 = mul_si<T, s>(cplx<T>{s2 * t3.re - s1 * t4.re, s2 * t3.im - s1 * t4.im});
+  v[0] = cadd(v[0], cadd(t1, t2));
+  v[1] = cadd(a1, b1);
+  v[4] = csub(a1, b1);
+  v[2] = cadd(a2, b2);
+  v[3] = csub(a2, b2);
+}
+
+// One Stockham stage of odd radix P (3 or 5) with span Ns (Ns * P divides N), after the
+// power-of-two stages of a line of N = 2^m * (odd part), m >= 3.  The N/P butterflies do
+// not split evenly over the team's N/8 threads; each thread takes up to ceil(8/P) of them
+// (j = lt, lt + Tn, ... below N/P).
+template <typename L, int s, int P, class IX>
+__device__ __forceinline__ void fft_stage_odd(cplx<L>* buf, IX ix, int N, int Ns, int lt, int Tn,
+                                              const cplx<L>* __restrict__ tw, bool act = true) {
   typedef L T;
-  const int NR = N / 3;
-  cplx<T> v[3][3];
+  constexpr int NBF = (8 + P - 1) / P;
+  const int NR = N / P;
+  const int step = N / (Ns * P);  // W_{Ns P}^{r k} = W_N^{r k step}
+  cplx<T> v[NBF][P];
 #pragma unroll
-  for (int bb = 0; bb < 3; bb++) {
+  for (int bb = 0; bb < NBF; bb++) {
     const int j = lt + bb * Tn;
     if (act && j < NR) {
 #pragma unroll
-      for (int r = 0; r < 3; r++) v[bb][r] = buf[ix(j + r * NR)];
+      for (int r = 0; r < P; r++) v[bb][r] = buf[ix(j + r * NR)];
     }
   }
   __syncthreads();
 #pragma unroll
-  for (int bb = 0; bb < 3; bb++) {
+  for (int bb = 0; bb < NBF; bb++) {
     const int j = lt + bb * Tn;
     if (act && j < NR) {
-      if (j) {  // W_N^{r j}, r = 1, 2 (span Ns = N/3: k = j, step = j)
-        const cplx<T> w1 = tw[j];
-        const cplx<T> w2 = tw[2 * j];
+      const int q = j / Ns, k = j - q * Ns;
+      if (k) {
+        const cplx<T> w1 = tw[k * step];
+        const cplx<T> w2 = tw[2 * k * step];
         v[bb][1] = (s < 0) ? tmul(v[bb][1], w1) : tmulc(v[bb][1], w1);
         v[bb][2] = (s < 0) ? tmul(v[bb][2], w2) : tmulc(v[bb][2], w2);
+        if constexpr (P == 5) {
+          const cplx<T> w3 = tw[3 * k * step];
+          const cplx<T> w4 = tw[4 * k * step];
+          v[bb][3] = (s < 0) ? tmul(v[bb][3], w3) : tmulc(v[bb][3], w3);
+          v[bb][4] = (s < 0) ? tmul(v[bb][4], w4) : tmulc(v[bb][4], w4);
+        }
       }
-      bfly3<T, s>(v[bb]);
+      if constexpr (P == 3) bfly3<T, s>(v[bb]);
+      else bfly5<T, s>(v[bb]);
+      const int idxD = q * Ns * P + k;
 #pragma unroll
-      for (int r = 0; r < 3; r++) buf[ix(j + r * NR)] = v[bb][r];
+      for (int r = 0; r < P; r++) buf[ix(idxD + r * Ns)] = v[bb][r];
     }
   }
   __syncthreads();
 }
 
 // full transform of one line (unnormalised), s = -1 forward, +1 inverse.
-// N = 2^logN, or N = 3 * 2^logN with logN >= 3 when r3 is set (the power-of-two
-// stages run first with spans 1, 8, ..., then one radix-3 stage).
+// N = 2^logN * odd, where odd (the odd part of N) is 1 or a product of at most three
+// factors 3 and 5, and logN >= 3 when odd > 1.  The power-of-two stages run first with
+// spans 1, 8, ..., then one radix-3 stage per factor 3 and one radix-5 stage per factor 5.
 template <typename L, int s, class IX>
 __device__ __forceinline__ void fft_line(cplx<L>* buf, IX ix, int N, int logN, int lt, int Tn,
-                         const cplx<L>* __restrict__ tw, int r3 = 0, bool act = true) {
-  if (logN == 0 && !r3) return;
-  if (logN == 1 && !r3) {
+                         const cplx<L>* __restrict__ tw, int odd = 1, bool act = true) {
+  if (logN == 0 && odd == 1) return;
+  if (logN == 1 && odd == 1) {
     fft_stage<L, s, 2, 1>(buf, ix, N, 0, lt, Tn, tw, act);
     return;
   }
-  if (logN == 2 && !r3) {
+  if (logN == 2 && odd == 1) {
     fft_stage<L, s, 4, 1>(buf, ix, N, 0, lt, Tn, tw, act);
     return;
   }
@@ -246,7 +280,9 @@ __device__ __forceinline__ void fft_line(cplx<L>* buf, IX ix, int N, int logN, i
   }
   if (rem == 2) fft_stage<L, s, 4, 2>(buf, ix, N, lNs, lt, Tn, tw, act);
   else if (rem == 1) fft_stage<L, s, 2, 4>(buf, ix, N, lNs, lt, Tn, tw, act);
-  if (r3) fft_stage3<L, s>(buf, ix, N, lt, Tn, tw, act);
+  int Ns = 1 << logN;
+  for (; odd % 3 == 0; odd /= 3, Ns *= 3) fft_stage_odd<L, s, 3>(buf, ix, N, Ns, lt, Tn, tw, act);
+  for (; odd % 5 == 0; odd /= 5, Ns *= 5) fft_stage_odd<L, s, 5>(buf, ix, N, Ns, lt, Tn, tw, act);
 }
 
 }  // namespace cans

@@ -25,8 +25,8 @@ struct cans_plan {
   cans_ctx* ctx;
   int ng[3];
   int len[2];        // transform lengths (x, y)
-  int logn[2];       // len = 2^logn, or 3 * 2^logn when r3 is set
-  int r3[2];
+  int logn[2];       // len = 2^logn * odd
+  int odd[2];        // odd part of len: 1, or a product of at most three factors 3 and 5
   int kind[2];
   double normfft;
   cans::cplx<double>* tw[2];  // W_N^t = exp(-2 pi i t/N), t < N; then the same N values rounded to float
@@ -44,19 +44,18 @@ static int ilog2(int n) {
   return (1 << l) == n ? l : -1;
 }
 
-// transform length n = 2^m, or 3 * 2^m with m >= 3 (the radix-3 stage needs a team of
-// n/8 threads): m and the radix-3 flag; false for other lengths
-static bool fft_factor(int n, int* m, int* r3) {
+// n = 2^m * odd with odd a product of at most three factors 3 and 5 (1 for a power of two);
+// with odd > 1 the line needs m >= 3 (its team of n/8 threads)
+static bool fft_factor(int n, int* m, int* odd) {
   if (n < 1 || n > 4096) return false;
-  *m = ilog2(n);
-  *r3 = 0;
-  if (*m >= 0) return true;
-  if (n % 3 == 0) {
-    *m = ilog2(n / 3);
-    *r3 = 1;
-    return *m >= 3;
-  }
-  return false;
+  int o = n, e = 0;
+  while (o % 2 == 0) o /= 2, e++;
+  int q = o, nf = 0;
+  while (q % 3 == 0) q /= 3, nf++;
+  while (q % 5 == 0) q /= 5, nf++;
+  *m = e;
+  *odd = o;
+  return q == 1 && nf <= 3 && (o == 1 || e >= 3);
 }
 
 // position of input element e in Makhoul's even/odd reordering (v_m = x_{2m}, v_{N-1-m} = x_{2m+1})
@@ -167,7 +166,7 @@ struct XArgs {
   Layout L;
   T* fld[NBATCH];  // halo'd fields (origin), one per system of a batch (blockIdx.y)
   T* W[NBATCH];    // their z-tiled work buffers (woff)
-  int N, logN, r3, Tn, pairs, kind;
+  int N, logN, odd, Tn, pairs, kind;
   const cplx<double>* tw;
   const cplx<double>* mk;
   int div;         // forward only: build the rhs from the velocity instead of reading fld
@@ -275,7 +274,7 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& a, int bx, int sys) {
       }
     }
     __syncthreads();
-    fft_line<T, -1>(smem, ix, N, a.logN, lt, Tn, line_tw<T>(a.tw, N), a.r3, act);
+    fft_line<T, -1>(smem, ix, N, a.logN, lt, Tn, line_tw<T>(a.tw, N), a.odd, act);
     if (!va) return;
     if (kind == K_PP) {
       const int H = N >> 1;
@@ -350,7 +349,7 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& a, int bx, int sys) {
       }
     }
     __syncthreads();
-    fft_line<T, +1>(smem, ix, N, a.logN, lt, Tn, line_tw<T>(a.tw, N), a.r3, act);
+    fft_line<T, +1>(smem, ix, N, a.logN, lt, Tn, line_tw<T>(a.tw, N), a.odd, act);
     if (!va) return;
     if (N == 1) {
       if (lt == 0) {
@@ -392,7 +391,7 @@ template <typename T>
 struct YArgs {
   int n1, n2, n3;
   T* W[NBATCH];    // z-tiled work buffers (woff), one per system of a batch (blockIdx.y)
-  int N, logN, r3, Tn, lines, kind;
+  int N, logN, odd, Tn, lines, kind;
   const cplx<double>* tw;
   const cplx<double>* mk;
 };
@@ -435,7 +434,7 @@ __device__ __forceinline__ void ypass_body(const YArgs<T>& a, int bx, int sys) {
       }
     }
     __syncthreads();
-    fft_line<T, -1>(smem, ix, N, a.logN, lt, Tn, line_tw<T>(a.tw, N), a.r3, act);
+    fft_line<T, -1>(smem, ix, N, a.logN, lt, Tn, line_tw<T>(a.tw, N), a.odd, act);
     if (!va) return;
     if (kind == K_PP) {
       const int H = N >> 1;
@@ -499,7 +498,7 @@ __device__ __forceinline__ void ypass_body(const YArgs<T>& a, int bx, int sys) {
       }
     }
     __syncthreads();
-    fft_line<T, +1>(smem, ix, N, a.logN, lt, Tn, line_tw<T>(a.tw, N), a.r3, act);
+    fft_line<T, +1>(smem, ix, N, a.logN, lt, Tn, line_tw<T>(a.tw, N), a.odd, act);
     if (!va) return;
     for (int e = lt; e < N; e += Tn) {
       cplx<double> z = widen(smem[ix(kind == K_PP ? e : mk_pos(e, N))]);
@@ -1091,7 +1090,7 @@ static int xpass(cans_ctx* c, const cans_plan* p, T* const* fld, T* const* W, in
   }
   a.N = p->len[0];
   a.logN = p->logn[0];
-  a.r3 = p->r3[0];
+  a.odd = p->odd[0];
   a.Tn = a.N >= 8 ? a.N / 8 : 1;
   const int xt = p->xthreads[sizeof(T) == sizeof(double)] ? p->xthreads[sizeof(T) == sizeof(double)] : 256;
   a.pairs = a.Tn >= xt ? 1 : xt / a.Tn;
@@ -1124,7 +1123,7 @@ static int ypass(cans_ctx* c, const cans_plan* p, T* const* W, int nsys, int dir
   for (int q = 0; q < nsys; q++) a.W[q] = W[q];
   a.N = p->len[1];
   a.logN = p->logn[1];
-  a.r3 = p->r3[1];
+  a.odd = p->odd[1];
   a.Tn = a.N >= 8 ? a.N / 8 : 1;
   int lines = 256 / a.Tn;
   if (lines < 8) lines = 8;
@@ -1473,7 +1472,7 @@ struct BoxDesc {
   Layout L;
   T* fld;                        // halo'd field (origin)
   T* W;                          // z-tiled work buffer (woff)
-  int N[2], logN[2], r3[2], Tn[2], kind[2];
+  int N[2], logN[2], odd[2], Tn[2], kind[2];
   int pairs, lines;              // x-pass teams, y-pass lines per work-group
   const cplx<double>* tw[2];
   const cplx<double>* mk[2];
@@ -1510,7 +1509,7 @@ __global__ void __launch_bounds__(BOX_THREADS) k_box_xpass(BoxArgs<T> ba) {
   a.W[0] = d.W;
   a.N = d.N[0];
   a.logN = d.logN[0];
-  a.r3 = d.r3[0];
+  a.odd = d.odd[0];
   a.Tn = d.Tn[0];
   a.pairs = d.pairs;
   a.kind = d.kind[0];
@@ -1531,7 +1530,7 @@ __global__ void __launch_bounds__(BOX_THREADS) k_box_ypass(BoxArgs<T> ba) {
   a.W[0] = d.W;
   a.N = d.N[1];
   a.logN = d.logN[1];
-  a.r3 = d.r3[1];
+  a.odd = d.odd[1];
   a.Tn = d.Tn[1];
   a.lines = d.lines;
   a.kind = d.kind[1];
@@ -1663,8 +1662,9 @@ static int batch_create(cans_ctx* c, int nbox, const cans_poisson_box* bx, doubl
       else if (b0 == 'D' && b1 == 'D') b.kind[q] = K_DD;
       else return fail(CANS_EUNSUPPORTED, who + "x/y boundary pairs must be PP, NN or DD");
       b.N[q] = x.n[q];
-      if (!fft_factor(x.n[q], &b.logN[q], &b.r3[q]) || x.n[q] > 8 * BOX_THREADS)
-        return fail(CANS_EUNSUPPORTED, who + "x/y lengths must be 2^m or 3*2^m (m >= 3), at most 2048");
+      if (!fft_factor(x.n[q], &b.logN[q], &b.odd[q]) || x.n[q] > 2048)
+        return fail(CANS_EUNSUPPORTED, who + "x/y lengths must be 2^m, or 2^m (m >= 3) times at most three "
+                                             "factors 3 and 5, at most 2048");
       b.Tn[q] = x.n[q] >= 8 ? x.n[q] / 8 : 1;
       nf = nf * (b.kind[q] == K_PP ? 1. : 2.) * (x.n[q] + 0.);  // fft.f90:75,140
       std::vector<cplx<double>> tw, mk;
@@ -1955,10 +1955,11 @@ extern "C" int cans_fftini(cans_ctx* c, const int* ng, const char* bcxy, const c
       return fail(CANS_EUNSUPPORTED, "cans_fftini: mixed N/D transforms are not implemented on the device");
     }
     p->len[d] = ng[d];
-    if (!fft_factor(ng[d], &p->logn[d], &p->r3[d])) {
+    if (!fft_factor(ng[d], &p->logn[d], &p->odd[d])) {
       delete p;
       return fail(CANS_EUNSUPPORTED,
-                  "cans_fftini: transform lengths must be 2^m or 3*2^m (m >= 3), at most 4096");
+                  "cans_fftini: transform lengths must be 2^m, or 2^m (m >= 3) times at most three factors 3 "
+                  "and 5, at most 4096");
     }
     // fft.f90:75,140: normfft = 1/prod(norm(1)*(ng+norm(2)-i)), norm = [1,0] (PP) or [2,0] (NN/DD, c-grid)
     nf = nf * (p->kind[d] == K_PP ? 1. : 2.) * (ng[d] + 0.);

@@ -119,7 +119,8 @@ int cans_tridmatrix(const char *bc, int n, const double *dzci_h, const double *d
 /* ---------------------------------------------------------------------------
  * Transforms: twins of fftini/fftend (src/fft.f90:25-245).  arrplan holds the
  * four batched transforms ((fwd,bwd) x (x,y)); normfft is returned as the
- * reference computes it (fft.f90:75,140,202,208).
+ * reference computes it (fft.f90:75,140,202,208).  Device transform lengths: 2^m, or
+ * 2^m (m >= 3) times at most three factors 3 and 5 (e.g. 24, 40, 72, 120), at most 4096.
  * ------------------------------------------------------------------------- */
 int cans_fftini(cans_ctx *ctx, const int *ng, const char *bcxy, const char *c_or_f,
                 cans_plan **arrplan, double *normfft);
@@ -144,7 +145,8 @@ int cans_solver(cans_ctx *ctx, const int *n, const int *ng, const cans_plan *arr
  * all boxes (x, y, z, y, x passes) in place on the boxes' fields, on the context's stream,
  * with no allocation or host synchronisation (it can be captured in a HIP graph).
  * The boxes are cell-centred (c_or_f = 'ccc'); x and y pairs PP, NN or DD with
- * lengths 2^m or 3*2^m (m >= 3) up to 2048, n1 even; z non-periodic.  The fields
+ * lengths 2^m, or 2^m (m >= 3) times at most three factors 3 and 5, up to 2048;
+ * n1 even; z non-periodic.  The fields
  * are in the context's pressure storage type (float under CANS_MIXED) with element
  * (1,0,0) on an element-pair boundary.  normfft (nbox, may be NULL) receives each
  * box's normfft.  The batch keeps the field addresses given at creation. */

@@ -1,7 +1,8 @@
 """Independent Poisson boxes of different sizes solved together (cans_poisson_batch_*),
 box by box against the oracle's solver (solver.f90:17-112 on each box).
 
-* lengths 2^m and 3*2^m (radix-3 transforms) along x and y, PP/NN/DD pairs, stretched z;
+* lengths 2^m, 3*2^m, and (second set) 5*2^m, 9*2^m, 15*2^m (radix-3 and radix-5 stages)
+  along x and y, PP/NN/DD pairs, stretched z;
 * FP64: every box at 1e-12 relative L-inf against the oracle, and bit for bit against
   the single-box cans_solver of the same box (the same per-line arithmetic in other
   launch shapes);
@@ -57,6 +58,9 @@ def oracle_box(box):
 
 MIX_BOXES = cases.poisson_boxes(10, seed=5, sizes=(24, 32, 48, 64, 96), nz=(8, 40),
                                 cbcs=("PPPPNN", "NNPPDN", "PPDDNN", "DDNNNN", "PPPPDD"))
+ODD_BOXES = cases.poisson_boxes(8, seed=6, sizes=(40, 72, 80, 120, 160), nz=(8, 32),
+                                cbcs=("PPPPNN", "NNPPDN", "PPDDNN", "DDNNNN", "PPPPDD"))
+BOX_SETS = {"mix": MIX_BOXES, "odd": ODD_BOXES}
 
 
 def run_batch(boxes, precision, seed):
@@ -78,10 +82,11 @@ def run_batch(boxes, precision, seed):
     return ctx, batch, rhs
 
 
-def test_boxes_fp64_match_oracle_and_single_solver(torch):
+@pytest.mark.parametrize("which", list(BOX_SETS))
+def test_boxes_fp64_match_oracle_and_single_solver(torch, which):
     from cans_amd import _lib as LB
     from cans_amd.host import Simulation
-    boxes = MIX_BOXES
+    boxes = BOX_SETS[which]
     print("\n[case] %d boxes fp64: %s" % (len(boxes), [(b["n"], b["cbc"]) for b in boxes]), flush=True)
     ctx, batch, rhs = run_batch(boxes, LB.CANS_FP64, seed=3)
     for i, b in enumerate(boxes):
@@ -106,9 +111,10 @@ def test_boxes_fp64_match_oracle_and_single_solver(torch):
     batch.close()
 
 
-def test_boxes_mixed(torch):
+@pytest.mark.parametrize("which", list(BOX_SETS))
+def test_boxes_mixed(torch, which):
     from cans_amd import _lib as LB
-    boxes = MIX_BOXES
+    boxes = BOX_SETS[which]
     print("\n[case] %d boxes mixed" % len(boxes), flush=True)
     ctx, batch, rhs = run_batch(boxes, LB.CANS_MIXED, seed=4)
     for i, b in enumerate(boxes):
