@@ -19,13 +19,16 @@
 #include "kernels.hpp"
 
 // transform family per direction
-enum { K_PP = 0, K_NN = 1, K_DD = 2 };
+// K_ND, K_DN: REDFT11 and RODFT11 (cell-centred N-D and D-N pairs, fft.f90:280-287), computed on a
+// line of twice the data length (below)
+enum { K_PP = 0, K_NN = 1, K_DD = 2, K_ND = 3, K_DN = 4 };
 
 struct cans_plan {
   cans_ctx* ctx;
   int ng[3];
   int len[2];        // transform lengths (x, y)
-  int logn[2];       // len = 2^logn * odd
+  int nfft[2];       // complex line lengths: len, or 2 len for K_ND / K_DN
+  int logn[2];       // nfft = 2^logn * odd
   int odd[2];        // odd part of len: 1, or a product of at most three factors 3 and 5
   int kind[2];
   double normfft;
@@ -76,7 +79,12 @@ __device__ __forceinline__ const cplx<T>* line_tw(const cplx<double>* tw, int N)
 //   REDFT10: Makhoul's reorder v = (x0, x2, ..., x3, x1), then
 //            Y_k = Re(M_k (Z_k + conj Z_{N-k})) with M_k = exp(-i pi k/(2N));
 //   RODFT10: REDFT10 of (-1)^j x_j, output reversed;
-//   HC2R, REDFT01, RODFT01: the inverses (unnormalised, as FFTW).
+//   HC2R, REDFT01, RODFT01: the inverses (unnormalised, as FFTW);
+//   REDFT11 (length n): Y_k = 2 sum_j x_j cos(pi (2j+1)(2k+1)/(4n)) is output 2k+1 of the
+//            REDFT10 of length N = 2n of x padded with n zeros, so the line has N = 2n
+//            elements and Makhoul positions (n+1)/2 .. N-1-n/2 hold zeros;
+//   RODFT11: (-1)^k REDFT11 of the reversed input.  Both are their own inverses (up to the
+//            factor 2n), so the backward passes run the forward code for them.
 // Global accesses move two elements per lane (16 bytes FP64, 8 bytes FP32 storage).
 // The pre/post-processing widens to double and rounds once on the store to T; the
 // FFT stages in between compute in the storage type (fft.hpp).
@@ -167,6 +175,7 @@ struct XArgs {
   T* fld[NBATCH];  // halo'd fields (origin), one per system of a batch (blockIdx.y)
   T* W[NBATCH];    // their z-tiled work buffers (woff)
   int N, logN, odd, Tn, pairs, kind;
+  int nd;          // data length along x: N, or N/2 for K_ND / K_DN
   const cplx<double>* tw;
   const cplx<double>* mk;
   int div;         // forward only: build the rhs from the velocity instead of reading fld
@@ -249,10 +258,19 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& a, int bx, int sys) {
   const RowPtr<T> xb = DIR > 0 ? frow(vb, rb) : wrow(vb, rb);
   const RowPtr<T> ya = DIR > 0 ? wrow(va, ra) : frow(va, ra);
   const RowPtr<T> yb = DIR > 0 ? wrow(vb, rb) : frow(vb, rb);
-  if (DIR > 0) {
+  if (DIR > 0 || kind >= K_ND) {
     if (va) {
       if (N == 1) {
         if (lt == 0) smem[ix(0)] = narrow<T>((double)xa[0], xb ? (double)xb[0] : 0.);
+      } else if (kind >= K_ND) {
+        const int nd = a.nd;
+        const bool dn = kind == K_DN;  // RODFT11 reads the reversed input
+        for (int p = (nd + 1) / 2 + lt; p < N - nd / 2; p += Tn) smem[ix(p)] = narrow<T>(0., 0.);
+        for (int e = 2 * lt; e < nd; e += 2 * Tn) {
+          const d2 a2 = ld2(xa + e), b2 = xb ? ld2(xb + e) : d2(0., 0.);
+          smem[ix(mk_pos(dn ? nd - 1 - e : e, N))] = narrow<T>(a2.x, b2.x);
+          smem[ix(mk_pos(dn ? nd - 2 - e : e + 1, N))] = narrow<T>(a2.y, b2.y);
+        }
       } else {
         for (int e = 2 * lt; e < N; e += 2 * Tn) {
           d2 a2, b2;
@@ -299,6 +317,19 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& a, int bx, int sys) {
         }
         st2(ya + 2 * k, are, aim);
         if (yb) st2(yb + 2 * k, bre, bim);
+      }
+    } else if (kind >= K_ND) {
+      // outputs m, m+1 (m even) are spectral indices 2m+1, 2m+3 of the N-point REDFT10
+      for (int m = 2 * lt; m < a.nd; m += 2 * Tn) {
+        double y0a, y0b, y1a, y1b;
+        split_dct(widen(smem[ix(2 * m + 1)]), widen(smem[ix(N - 2 * m - 1)]), a.mk[2 * m + 1], y0a, y0b);
+        split_dct(widen(smem[ix(2 * m + 3)]), widen(smem[ix(N - 2 * m - 3)]), a.mk[2 * m + 3], y1a, y1b);
+        if (kind == K_DN) {  // (-1)^k on the odd output
+          y1a = -y1a;
+          y1b = -y1b;
+        }
+        st2(ya + m, y0a, y1a);
+        if (yb) st2(yb + m, y0b, y1b);
       }
     } else {
       for (int k = 2 * lt; k < N; k += 2 * Tn) {
@@ -392,6 +423,7 @@ struct YArgs {
   int n1, n2, n3;
   T* W[NBATCH];    // z-tiled work buffers (woff), one per system of a batch (blockIdx.y)
   int N, logN, odd, Tn, lines, kind;
+  int nd;          // data length along y: N, or N/2 for K_ND / K_DN
   const cplx<double>* tw;
   const cplx<double>* mk;
 };
@@ -422,8 +454,15 @@ __device__ __forceinline__ void ypass_body(const YArgs<T>& a, int bx, int sys) {
     if (vb) st2(col(e), x, y);
     else *col(e) = (T)x;
   };
-  if (DIR > 0) {
-    if (va) {
+  if (DIR > 0 || kind >= K_ND) {
+    if (va && kind >= K_ND) {
+      const int nd = a.nd;
+      for (int p = (nd + 1) / 2 + lt; p < N - nd / 2; p += Tn) smem[ix(p)] = narrow<T>(0., 0.);
+      for (int e = lt; e < nd; e += Tn) {
+        const d2 v = ld(e);
+        smem[ix(mk_pos(kind == K_DN ? nd - 1 - e : e, N))] = narrow<T>(v.x, v.y);
+      }
+    } else if (va) {
       for (int e = lt; e < N; e += Tn) {
         d2 v = ld(e);
         if (kind == K_DD && (e & 1)) {
@@ -455,6 +494,16 @@ __device__ __forceinline__ void ypass_body(const YArgs<T>& a, int bx, int sys) {
         }
         st(2 * kk, are, bre);
         st(2 * kk + 1, aim, bim);
+      }
+    } else if (kind >= K_ND) {
+      for (int m = lt; m < a.nd; m += Tn) {
+        double ya, yb;
+        split_dct(widen(smem[ix(2 * m + 1)]), widen(smem[ix(N - 2 * m - 1)]), a.mk[2 * m + 1], ya, yb);
+        if (kind == K_DN && (m & 1)) {
+          ya = -ya;
+          yb = -yb;
+        }
+        st(m, ya, yb);
       }
     } else {
       for (int kk = lt; kk < N; kk += Tn) {
@@ -1067,7 +1116,7 @@ static void tw_pack(const std::vector<cplx<double>>& tw, unsigned char* out) {
 
 static int upload_tables(cans_plan* p, int d) {
   std::vector<cplx<double>> tw, mk;
-  make_tables(p->len[d], tw, mk);
+  make_tables(p->nfft[d], tw, mk);
   std::vector<unsigned char> twp(tw_bytes(tw.size()));
   tw_pack(tw, twp.data());
   CANS_HIP_TRY(hipMalloc((void**)&p->tw[d], twp.size()));
@@ -1088,7 +1137,8 @@ static int xpass(cans_ctx* c, const cans_plan* p, T* const* fld, T* const* W, in
     a.fld[q] = fld[q];
     a.W[q] = W[q];
   }
-  a.N = p->len[0];
+  a.N = p->nfft[0];
+  a.nd = p->len[0];
   a.logN = p->logn[0];
   a.odd = p->odd[0];
   a.Tn = a.N >= 8 ? a.N / 8 : 1;
@@ -1121,7 +1171,8 @@ static int ypass(cans_ctx* c, const cans_plan* p, T* const* W, int nsys, int dir
   a.n2 = c->L.n2;
   a.n3 = c->L.n3;
   for (int q = 0; q < nsys; q++) a.W[q] = W[q];
-  a.N = p->len[1];
+  a.N = p->nfft[1];
+  a.nd = p->len[1];
   a.logN = p->logn[1];
   a.odd = p->odd[1];
   a.Tn = a.N >= 8 ? a.N / 8 : 1;
@@ -1508,6 +1559,7 @@ __global__ void __launch_bounds__(BOX_THREADS) k_box_xpass(BoxArgs<T> ba) {
   a.fld[0] = d.fld;
   a.W[0] = d.W;
   a.N = d.N[0];
+  a.nd = d.N[0];
   a.logN = d.logN[0];
   a.odd = d.odd[0];
   a.Tn = d.Tn[0];
@@ -1529,6 +1581,7 @@ __global__ void __launch_bounds__(BOX_THREADS) k_box_ypass(BoxArgs<T> ba) {
   a.n3 = d.L.n3;
   a.W[0] = d.W;
   a.N = d.N[1];
+  a.nd = d.N[1];
   a.logN = d.logN[1];
   a.odd = d.odd[1];
   a.Tn = d.Tn[1];
@@ -1892,7 +1945,7 @@ static int tune_type(cans_ctx* c, cans_plan* p) {
     return best;
   };
   // x pass
-  const int tnx = p->len[0] >= 8 ? p->len[0] / 8 : 1;
+  const int tnx = p->nfft[0] >= 8 ? p->nfft[0] / 8 : 1;
   float bx = 1e30f;
   int xbest = 256;
   for (int xt : {128, 256, 512}) {
@@ -1906,11 +1959,11 @@ static int tune_type(cans_ctx* c, cans_plan* p) {
   }
   p->xthreads[slot] = xbest;
   // y pass
-  const int tny = p->len[1] >= 8 ? p->len[1] / 8 : 1;
+  const int tny = p->nfft[1] >= 8 ? p->nfft[1] / 8 : 1;
   float by = 1e30f;
   int ybest = 0;
   for (int yl : {4, 8, 16, 32}) {
-    if ((long)yl * tny > 1024 || (size_t)yl * p->len[1] * sizeof(cplx<T>) > 65536) continue;
+    if ((long)yl * tny > 1024 || (size_t)yl * p->nfft[1] * sizeof(cplx<T>) > 65536) continue;
     p->ylines[slot] = yl;
     const float ms = timed([&]() -> int {
       int r = ypass<T>(&cs, p, &W, 1, +1);
@@ -1950,18 +2003,21 @@ extern "C" int cans_fftini(cans_ctx* c, const int* ng, const char* bcxy, const c
     if (b0 == 'P' && b1 == 'P') p->kind[d] = K_PP;
     else if (b0 == 'N' && b1 == 'N') p->kind[d] = K_NN;
     else if (b0 == 'D' && b1 == 'D') p->kind[d] = K_DD;
+    else if (b0 == 'N' && b1 == 'D') p->kind[d] = K_ND;
+    else if (b0 == 'D' && b1 == 'N') p->kind[d] = K_DN;
     else {
       delete p;
-      return fail(CANS_EUNSUPPORTED, "cans_fftini: mixed N/D transforms are not implemented on the device");
+      return fail(CANS_EINVAL, "cans_fftini: boundary pairs must be PP, NN, DD, ND or DN");
     }
     p->len[d] = ng[d];
-    if (!fft_factor(ng[d], &p->logn[d], &p->odd[d])) {
+    p->nfft[d] = p->kind[d] >= K_ND ? 2 * ng[d] : ng[d];
+    if (!fft_factor(p->nfft[d], &p->logn[d], &p->odd[d])) {
       delete p;
       return fail(CANS_EUNSUPPORTED,
                   "cans_fftini: transform lengths must be 2^m, or 2^m (m >= 3) times at most three factors 3 "
-                  "and 5, at most 4096");
+                  "and 5, at most 4096 (ND and DN pairs: the doubled length)");
     }
-    // fft.f90:75,140: normfft = 1/prod(norm(1)*(ng+norm(2)-i)), norm = [1,0] (PP) or [2,0] (NN/DD, c-grid)
+    // fft.f90:75,140: normfft = 1/prod(norm(1)*(ng+norm(2)-i)), norm = [1,0] (PP) or [2,0] (NN/DD/ND/DN, c-grid)
     nf = nf * (p->kind[d] == K_PP ? 1. : 2.) * (ng[d] + 0.);
     int rc = upload_tables(p, d);
     if (rc) {

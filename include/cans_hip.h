@@ -121,6 +121,8 @@ int cans_tridmatrix(const char *bc, int n, const double *dzci_h, const double *d
  * four batched transforms ((fwd,bwd) x (x,y)); normfft is returned as the
  * reference computes it (fft.f90:75,140,202,208).  Device transform lengths: 2^m, or
  * 2^m (m >= 3) times at most three factors 3 and 5 (e.g. 24, 40, 72, 120), at most 4096.
+ * Boundary pairs along x and y: PP (also face-centred), and cell-centred NN, DD, ND and DN
+ * (fft.f90:260-287; ND/DN run on a line of twice the length, which must be such a length).
  * ------------------------------------------------------------------------- */
 int cans_fftini(cans_ctx *ctx, const int *ng, const char *bcxy, const char *c_or_f,
                 cans_plan **arrplan, double *normfft);

@@ -28,6 +28,10 @@ def params(name, ng=None):
     elif name == "box":  # no-slip walls on all six faces (Neumann pressure everywhere, DCT in x and y)
         p = dict(ng=(16, 16, 12), l=(1.0, 1.0, 1.0), gtype=1, gr=0.5, cbcvel="DDDDDD" * 3,
                  cbcpre="NNNNNN", visci=100.0)
+    elif name == "open_nd":  # outflow faces at x = lx and y = 0 (velocity N, pressure D): pressure ND in x
+        # (REDFT11), DN in y (RODFT11), walls in z
+        p = dict(ng=(16, 24, 12), l=(1.0, 1.0, 1.0), gtype=1, gr=0.5, cbcvel="DNNDDD" * 3,
+                 cbcpre="NDDNNN", visci=100.0)
     elif name == "cavity_dd":  # Dirichlet pressure walls in y (DST), Neumann in z
         p = dict(ng=(32, 32, 24), l=(1.0, 1.0, 1.0), gtype=1, gr=0.5, cbcvel="PPNNDD" * 3,
                  cbcpre="PPDDNN", visci=100.0)

@@ -51,7 +51,9 @@ SOLVER_CASES = [("tgv", (16, 16, 16)), ("tgv", (32, 8, 12)), ("channel", (32, 16
                 # radix-3 transform lengths (3 * 2^m) along x and y
                 ("channel", (48, 96, 40)), ("cavity_dd", (96, 24, 20)), ("box", (24, 48, 16)),
                 # radix-5 and mixed odd factors (5 * 2^m, 15 * 2^m, 9 * 2^m, 25 * 2^m)
-                ("channel", (40, 120, 24)), ("cavity_dd", (72, 80, 16)), ("box", (200, 40, 12))]
+                ("channel", (40, 120, 24)), ("cavity_dd", (72, 80, 16)), ("box", (200, 40, 12)),
+                # REDFT11 along x and RODFT11 along y (pressure ND, DN)
+                ("open_nd", (16, 24, 12)), ("open_nd", (64, 32, 20)), ("open_nd", (40, 12, 10))]
 
 
 @pytest.mark.parametrize("name,ng", SOLVER_CASES, ids=["%s-%dx%dx%d" % ((c,) + g) for c, g in SOLVER_CASES])
@@ -194,7 +196,8 @@ def cases_rk(irk):
 # the walled box (no periodic x) takes the unfused divergence path of the fused substep
 STEP_CASES = [("tgv", (32, 32, 32), 1e-3, 3), ("channel", (64, 32, 48), 2e-4, 3), ("duct", (32, 16, 24), 1e-3, 2),
               ("cavity_dd", (16, 16, 12), 1e-3, 2), ("box", (16, 16, 12), 1e-3, 2),
-              ("channel", (96, 48, 32), 2e-4, 2), ("channel", (80, 120, 24), 2e-4, 2)]
+              ("channel", (96, 48, 32), 2e-4, 2), ("channel", (80, 120, 24), 2e-4, 2),
+              ("open_nd", (16, 24, 12), 1e-3, 2)]
 
 
 @pytest.mark.parametrize("name,ng,dt,nsteps", STEP_CASES, ids=["%s-%dx%dx%d" % ((c[0],) + c[1]) for c in STEP_CASES])
