@@ -1524,6 +1524,7 @@ struct BoxDesc {
   T* fld;                        // halo'd field (origin)
   T* W;                          // z-tiled work buffer (woff)
   int N[2], logN[2], odd[2], Tn[2], kind[2];
+  int nd[2];                     // data lengths (N[q], or N[q]/2 for K_ND / K_DN)
   int pairs, lines;              // x-pass teams, y-pass lines per work-group
   const cplx<double>* tw[2];
   const cplx<double>* mk[2];
@@ -1559,7 +1560,7 @@ __global__ void __launch_bounds__(BOX_THREADS) k_box_xpass(BoxArgs<T> ba) {
   a.fld[0] = d.fld;
   a.W[0] = d.W;
   a.N = d.N[0];
-  a.nd = d.N[0];
+  a.nd = d.nd[0];
   a.logN = d.logN[0];
   a.odd = d.odd[0];
   a.Tn = d.Tn[0];
@@ -1581,7 +1582,7 @@ __global__ void __launch_bounds__(BOX_THREADS) k_box_ypass(BoxArgs<T> ba) {
   a.n3 = d.L.n3;
   a.W[0] = d.W;
   a.N = d.N[1];
-  a.nd = d.N[1];
+  a.nd = d.nd[1];
   a.logN = d.logN[1];
   a.odd = d.odd[1];
   a.Tn = d.Tn[1];
@@ -1713,15 +1714,18 @@ static int batch_create(cans_ctx* c, int nbox, const cans_poisson_box* bx, doubl
       if (b0 == 'P' && b1 == 'P') b.kind[q] = K_PP;
       else if (b0 == 'N' && b1 == 'N') b.kind[q] = K_NN;
       else if (b0 == 'D' && b1 == 'D') b.kind[q] = K_DD;
-      else return fail(CANS_EUNSUPPORTED, who + "x/y boundary pairs must be PP, NN or DD");
-      b.N[q] = x.n[q];
-      if (!fft_factor(x.n[q], &b.logN[q], &b.odd[q]) || x.n[q] > 2048)
+      else if (b0 == 'N' && b1 == 'D') b.kind[q] = K_ND;
+      else if (b0 == 'D' && b1 == 'N') b.kind[q] = K_DN;
+      else return fail(CANS_EINVAL, who + "x/y boundary pairs must be PP, NN, DD, ND or DN");
+      b.nd[q] = x.n[q];
+      b.N[q] = b.kind[q] >= K_ND ? 2 * x.n[q] : x.n[q];
+      if (!fft_factor(b.N[q], &b.logN[q], &b.odd[q]) || b.N[q] > 2048)
         return fail(CANS_EUNSUPPORTED, who + "x/y lengths must be 2^m, or 2^m (m >= 3) times at most three "
-                                             "factors 3 and 5, at most 2048");
-      b.Tn[q] = x.n[q] >= 8 ? x.n[q] / 8 : 1;
+                                             "factors 3 and 5, at most 2048 (ND and DN pairs: the doubled length)");
+      b.Tn[q] = b.N[q] >= 8 ? b.N[q] / 8 : 1;
       nf = nf * (b.kind[q] == K_PP ? 1. : 2.) * (x.n[q] + 0.);  // fft.f90:75,140
       std::vector<cplx<double>> tw, mk;
-      make_tables(x.n[q], tw, mk);
+      make_tables(b.N[q], tw, mk);
       fx[i].tw[q] = ar.take(tw_bytes(tw.size()));
       tw_pack(tw, &ar.host[fx[i].tw[q]]);
       fx[i].mk[q] = ar.take(mk.size() * sizeof(cplx<double>));

@@ -146,8 +146,9 @@ int cans_solver(cans_ctx *ctx, const int *n, const int *ng, const cans_plan *arr
  * does each box's fftini (src/fft.f90:25) as well.  A solve is five launches for
  * all boxes (x, y, z, y, x passes) in place on the boxes' fields, on the context's stream,
  * with no allocation or host synchronisation (it can be captured in a HIP graph).
- * The boxes are cell-centred (c_or_f = 'ccc'); x and y pairs PP, NN or DD with
- * lengths 2^m, or 2^m (m >= 3) times at most three factors 3 and 5, up to 2048;
+ * The boxes are cell-centred (c_or_f = 'ccc'); x and y pairs PP, NN, DD, ND or DN with
+ * lengths 2^m, or 2^m (m >= 3) times at most three factors 3 and 5, up to 2048 (ND, DN:
+ * the doubled length);
  * n1 even; z non-periodic.  The fields
  * are in the context's pressure storage type (float under CANS_MIXED) with element
  * (1,0,0) on an element-pair boundary.  normfft (nbox, may be NULL) receives each

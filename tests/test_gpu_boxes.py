@@ -2,7 +2,7 @@
 box by box against the oracle's solver (solver.f90:17-112 on each box).
 
 * lengths 2^m, 3*2^m, and (second set) 5*2^m, 9*2^m, 15*2^m (radix-3 and radix-5 stages)
-  along x and y, PP/NN/DD pairs, stretched z;
+  along x and y, PP/NN/DD pairs, stretched z; a third set with ND/DN pairs along x and y;
 * FP64: every box at 1e-12 relative L-inf against the oracle, and bit for bit against
   the single-box cans_solver of the same box (the same per-line arithmetic in other
   launch shapes);
@@ -60,7 +60,10 @@ MIX_BOXES = cases.poisson_boxes(10, seed=5, sizes=(24, 32, 48, 64, 96), nz=(8, 4
                                 cbcs=("PPPPNN", "NNPPDN", "PPDDNN", "DDNNNN", "PPPPDD"))
 ODD_BOXES = cases.poisson_boxes(8, seed=6, sizes=(40, 72, 80, 120, 160), nz=(8, 32),
                                 cbcs=("PPPPNN", "NNPPDN", "PPDDNN", "DDNNNN", "PPPPDD"))
-BOX_SETS = {"mix": MIX_BOXES, "odd": ODD_BOXES}
+# REDFT11 / RODFT11 along x and y (pressure ND, DN)
+NDDN_BOXES = cases.poisson_boxes(6, seed=13, sizes=(16, 24, 32, 40, 64), nz=(8, 24),
+                                 cbcs=("NDPPNN", "PPDNDD", "DNNDNN", "NDNDDD"))
+BOX_SETS = {"mix": MIX_BOXES, "odd": ODD_BOXES, "nd": NDDN_BOXES}
 
 
 def run_batch(boxes, precision, seed):
