@@ -505,7 +505,13 @@ static int rk3_substep(cans_ctx* c, const cans_step_params* s, int irk, double d
   if (rc) return rc;
   if ((rc = ensure_tmp_fields<S>(c, e))) return rc;  // same scratch, viewed as S
   const double* base[3] = {u, v, w};
-  const double* fvec = delta ? c->dscal : nullptr;  // bulk forcing (rk writes it to dscal[0..2])
+  // bulk forcing (rk writes it to dscal[0..2], 0 for unforced components): added where the
+  // intermediate velocity is read (divergence, correc) rather than stored, except before the
+  // implicit-diffusion solve, which needs the forced field.  Forcing acts only along periodic
+  // directions (sanity.f90:chk_forcing), so the ghost cells that bounduvw fills from the unforced
+  // field and the divergence reads (u_d along d) differ from the forced ones by exactly f; the
+  // other ghosts correc reads are rewritten by the final bounduvw.
+  const double* fvec = (delta || !c->imp) ? c->dscal : nullptr;
   // With homogeneous pressure BCs and a periodic x transform the divergence is formed
   // inside the first transform pass (fillps -> updt_rhs_b -> solver otherwise)
   const bool fuse_div = !(s->rhsbx_d || s->rhsby_d || s->rhsbz_d) && s->cbcpre[0] == 'P' && s->cbcpre[1] == 'P' &&
@@ -544,7 +550,7 @@ static int rk3_substep(cans_ctx* c, const cans_step_params* s, int irk, double d
                   s->visc, p, s->is_forced, s->velf, s->bforce, (S*)st->dudtrko, (S*)st->dvdtrko, (S*)st->dwdtrko, u,
                   v, w, t[0], t[1], t[2], irk != 3, delta ? e : nullptr, divu, sb, s->gacc, s->beta);
   if (rc) return rc;
-  for (int k = 0; k < 3 && !delta; k++)
+  for (int k = 0; k < 3 && !fvec; k++)
     if (s->is_forced[k] && (rc = launch_add_forcing<double>(c, t[k], t[k], k, 1))) return rc;
   // implicit diffusion (main.f90:453-462): the three components' Helmholtz systems, alpha = -visc*dtrk/2,
   // solved as one batch
@@ -571,6 +577,7 @@ static int rk3_substep(cans_ctx* c, const cans_step_params* s, int irk, double d
   if (fuse_div) {
     DivSrc<S> ds = {su, sv, sw, eu, ev, ew, divu, s->dzfi_d, dtrki, dtrki * s->dli[0], dtrki * s->dli[1],
                     s->dli[0], s->dli[1]};
+    if (!delta) ds.f = fvec;
     if ((rc = run_solver<S>(c, s->plan, s->normfft, s->lambdaxy_d, s->a_d, s->b_d, s->c_d, s->cbcpre, "ccc", pp, &ds)))
       return rc;
   } else {

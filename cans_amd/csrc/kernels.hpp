@@ -204,6 +204,7 @@ struct DivSrc {
   const S* divu;
   const double* dzfi;
   double dti, dtidxi, dtidyi, dxi, dyi;
+  const double* f = nullptr;  // FP64 velocity (no increments): bulk forcing added where the velocity is read
 };
 template <typename S>
 int run_solver(cans_ctx* c, const cans_plan* plan, double normfft, const double* lam, const double* a,

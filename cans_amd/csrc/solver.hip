@@ -201,6 +201,15 @@ __device__ __forceinline__ d2 div_pair(const XArgs<T>& a, long r, int e) {
   d2 u2 = ld2(a.ds.u + c), w2 = ld2(a.ds.w + c), wm = ld2(a.ds.w + c - L.sz);
   d2 v2 = ld2(a.ds.v + c), vm = ld2(a.ds.v + c - L.sy);
   double um = a.ds.u[c - 1];
+  if (a.ds.f) {  // the forced velocity u + f (rk.f90:307-311), rounded as the stored one would be
+    const double fu = a.ds.f[0], fv = a.ds.f[1], fw = a.ds.f[2];
+    u2 = d2(u2.x + fu, u2.y + fu);
+    um = um + fu;
+    v2 = d2(v2.x + fv, v2.y + fv);
+    vm = d2(vm.x + fv, vm.y + fv);
+    w2 = d2(w2.x + fw, w2.y + fw);
+    wm = d2(wm.x + fw, wm.y + fw);
+  }
   if (a.ds.eu) {  // velocity held as u + increment
     const d2 eu2 = ld2(a.ds.eu + c), ew2 = ld2(a.ds.ew + c), ewm = ld2(a.ds.ew + c - L.sz);
     const d2 ev2 = ld2(a.ds.ev + c), evm = ld2(a.ds.ev + c - L.sy);

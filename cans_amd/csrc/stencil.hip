@@ -455,10 +455,11 @@ __global__ void k_add_forcing(Layout L, const T* src, T* dst, const double* f, i
   dst[c] = apply ? (T)((double)src[c] + f[comp]) : src[c];
 }
 
-// velocity value u[o], or u[o] + e[o] + fc when the field is held as u plus an increment e
+// velocity value u[o] + fc, or u[o] + e[o] + fc when the field is held as u plus an increment e
+// (fc: the bulk forcing of rk.f90:307-311 not yet added to the stored field, or 0)
 template <typename S>
 __device__ __forceinline__ double vel(const double* u, const S* e, double fc, long o) {
-  return e ? u[o] + (double)e[o] + fc : u[o];
+  return e ? u[o] + (double)e[o] + fc : u[o] + fc;
 }
 
 // fillps.f90:13-51 (velocity u,v,w, optionally plus increments eu,ev,ew and forcing f)
