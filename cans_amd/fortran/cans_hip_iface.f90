@@ -343,6 +343,58 @@ module cans_hip
       real(c_double), value :: dt
       type(cans_state), intent(inout) :: st
     end function
+    ! device address of the bulk forcing f(1:3) of the last rk (no counterpart: rk.f90 keeps it on the host)
+    integer(c_int) function cans_ctx_forcing(ctx, f_d) bind(C, name='cans_ctx_forcing')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      type(c_ptr), intent(out) :: f_d
+    end function
+    integer(c_int) function cans_ctx_layout(ctx, sy, sz, field_elems) bind(C, name='cans_ctx_layout')
+      import :: c_int, c_ptr, c_long
+      type(c_ptr), value :: ctx
+      integer(c_long), intent(out) :: sy, sz, field_elems
+    end function
+    ! launch shapes the plan's tuner chose: x threads (FP32, FP64), y lines (FP32, FP64)
+    integer(c_int) function cans_fft_tuning(arrplan, shapes) bind(C, name='cans_fft_tuning')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: arrplan
+      integer(c_int), intent(out) :: shapes(4)
+    end function
+    integer(c_int) function cans_poisson_batch_tuning(batch, threads) bind(C, name='cans_poisson_batch_tuning')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+      integer(c_int), intent(out) :: threads
+    end function
+    ! solver with FP64 iterative refinement (p FP64; inner solves in the context's pressure storage)
+    integer(c_int) function cans_solver_refined(ctx, n, ng, arrplan, normfft, lambdaxy, a, b, c, bc, c_or_f, &
+                                                dli, dzci, dzfi, sweeps, p) bind(C, name='cans_solver_refined')
+      import :: c_int, c_ptr, c_double, c_char
+      type(c_ptr), value :: ctx, arrplan, lambdaxy, a, b, c, dzci, dzfi, p
+      integer(c_int), intent(in) :: n(3), ng(3)
+      real(c_double), value :: normfft
+      character(kind=c_char), intent(in) :: bc(6), c_or_f(3)
+      real(c_double), intent(in) :: dli(3)
+      integer(c_int), value :: sweeps
+    end function
+    ! adjoint (tri-periodic uniform grids, FP64): o = J(u)^T g + b of the momentum right-hand side
+    integer(c_int) function cans_mom_jt(ctx, n, dli, visc, u, v, w, gu, gv, gw, bu, bv, bw, ou, ov, ow) &
+                   bind(C, name='cans_mom_jt')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, u, v, w, gu, gv, gw, bu, bv, bw, ou, ov, ow
+      integer(c_int), intent(in) :: n(3)
+      real(c_double), intent(in) :: dli(3)
+      real(c_double), value :: visc
+    end function
+    ! one RK3 substep run backwards: adjoints (ub, fb) of its outputs -> (ubo, fbo) of its inputs
+    integer(c_int) function cans_rk3_substep_adjoint(ctx, prm, irk, dt, u, v, w, ub, vb, wb, fu, fv, fw, &
+                                                     ubo, vbo, wbo, fuo, fvo, fwo, pp) &
+                   bind(C, name='cans_rk3_substep_adjoint')
+      import :: c_int, c_ptr, c_double, cans_step_params
+      type(c_ptr), value :: ctx, u, v, w, ub, vb, wb, fu, fv, fw, ubo, vbo, wbo, fuo, fvo, fwo, pp
+      type(cans_step_params), intent(in) :: prm
+      integer(c_int), value :: irk
+      real(c_double), value :: dt
+    end function
   end interface
 contains
   subroutine cans_check(istat, what)

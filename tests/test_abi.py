@@ -24,6 +24,15 @@ def test_library_exports_every_declared_symbol():
         assert n in LB.SIGNATURES, "no ctypes signature for " + n
 
 
+def test_fortran_module_binds_every_declared_symbol():
+    """module cans_hip (cans_amd/fortran/cans_hip_iface.f90) has a bind(C) interface for each
+    entry point of the header, under the C name."""
+    txt = open(os.path.join(ROOT, "cans_amd", "fortran", "cans_hip_iface.f90")).read().lower()
+    bound = set(re.findall(r"bind\(c,\s*name='(cans_[a-z0-9_]+)'\)", txt))
+    missing = [n for n in header_functions() if n not in bound]
+    assert not missing, missing
+
+
 def test_host_setup_matches_oracle():
     """The library's host helpers (initgrid, tridmatrix, eigenvalues) agree with the oracle."""
     from cans_amd import host as H
