@@ -31,8 +31,9 @@ struct JtArgs {
 // rotating slots; each new plane is staged once, so every value is read from global
 // memory about (JT_X+2)(JT_Y+2)/(JT_X JT_Y) times instead of once per use.  COMB: the
 // adjoint substep's g = f1 wb + fb is formed while it is staged, and f2 wb is stored
-// beside the result.
-constexpr int JT_X = 64, JT_Y = 4, JT_K = 16;
+// beside the result.  32 x 8 cells: 49 KB of LDS, so three work-groups share a CU (64 x 4
+// needs 57 KB and allows two) and the ring re-reads are 1.33x instead of 1.55x.
+constexpr int JT_X = 32, JT_Y = 8, JT_K = 16;
 constexpr int JT_PX = JT_X + 2, JT_PLANE = (JT_X + 2) * (JT_Y + 2);
 
 template <bool COMB>
