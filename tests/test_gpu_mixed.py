@@ -83,10 +83,10 @@ def test_mixed_solver(torch, name, ng):
 
 # the walled box (no periodic x) takes the unfused divergence path (fillps, then the solve)
 STEP_CASES = [("channel", (64, 32, 48), 2e-4, 10), ("tgv", (32, 32, 32), 1e-3, 10), ("duct", (32, 16, 24), 1e-3, 5),
-              ("box", (16, 16, 12), 1e-3, 5)]
+              ("box", (16, 16, 12), 1e-3, 5), ("open_nd", (16, 24, 12), 1e-3, 5), ("channel", (80, 40, 24), 2e-4, 5)]
 
 
-@pytest.mark.parametrize("name,ng,dt,nsteps", STEP_CASES, ids=[c[0] for c in STEP_CASES])
+@pytest.mark.parametrize("name,ng,dt,nsteps", STEP_CASES, ids=["%s-%dx%dx%d" % ((c[0],) + c[1]) for c in STEP_CASES])
 @pytest.mark.parametrize("fused", [True, False], ids=["fused", "per-routine"])
 def test_mixed_steps(torch, name, ng, dt, nsteps, fused):
     p, sim, orc, st = mixed_pair(name, ng)
