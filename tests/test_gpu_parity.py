@@ -270,3 +270,35 @@ def test_fused_substep_bitwise_per_routine(torch, name, ng, dt):
     for k in ("u", "v", "w", "p"):
         np.testing.assert_array_equal(getattr(sim, k).numpy(), getattr(sim2, k).numpy(), err_msg=k)
     print("fused and per-routine steps identical", flush=True)
+
+
+def test_transform_lengths_and_pairs_refused(torch):
+    """cans_fftini accepts x/y lengths 2^m, or 2^m (m >= 3) times at most three factors 3 and 5
+    (ND/DN pairs: twice the length), and refuses others with CANS_EUNSUPPORTED; boundary pairs
+    other than PP, NN, DD, ND, DN with CANS_EINVAL.  The checks run before any device work."""
+    import ctypes as C
+    from cans_amd import _lib as LB
+    L = LB.lib()
+
+    def fftini(ng, bc):
+        h = C.c_void_p()
+        rc = L.cans_ctx_create(LB.ivec(ng), 0, LB.CANS_FP64, None, C.byref(h))
+        assert rc == LB.CANS_OK
+        plan, nf = C.c_void_p(), C.c_double()
+        rc = L.cans_fftini(h, LB.ivec(ng), bc, b"ccc", C.byref(plan), C.byref(nf))
+        msg = L.cans_last_error()
+        if rc == LB.CANS_OK:
+            L.cans_fftend(plan)
+        L.cans_ctx_destroy(h)
+        return rc, msg
+
+    for ng, bc, want in [((14, 16, 8), b"PPPPNN", LB.CANS_EUNSUPPORTED),   # 7 * 2
+                         ((12, 16, 8), b"PPPPNN", LB.CANS_EUNSUPPORTED),   # 3 * 4: m < 3
+                         ((16, 270, 8), b"PPPPNN", LB.CANS_EUNSUPPORTED),  # 27 * 10: four odd factors
+                         ((16, 8, 8), b"PNPPNN", LB.CANS_EINVAL),
+                         ((12, 16, 8), b"NDPPNN", LB.CANS_OK),             # REDFT11 on a line of 24
+                         ((16, 120, 8), b"PPDNNN", LB.CANS_OK),            # RODFT11 on a line of 240
+                         ((40, 72, 8), b"NNDDNN", LB.CANS_OK)]:
+        rc, msg = fftini(ng, bc)
+        print("[case] fftini %s %s -> %d %s" % (ng, bc.decode(), rc, msg.decode() if rc else ""), flush=True)
+        assert rc == want, (ng, bc, rc, msg)
