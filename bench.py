@@ -40,7 +40,7 @@ DT = 2e-4
 # HBM bytes moved by one standalone Poisson solve, measured with rocprofv3 --pmc
 # (FETCH_SIZE x 2 + WRITE_SIZE x 1, the factors a 1 GiB device copy calibrates);
 # the per-kernel table is profiles/channel_1024x512x384_hbm_traffic.md
-SOLVE_TRAFFIC_BYTES = {((1024, 512, 384), "mixed"): 9.185e9, ((1024, 512, 384), "fp64"): 18.046e9}
+SOLVE_TRAFFIC_BYTES = {((1024, 512, 384), "mixed"): 9.184e9, ((1024, 512, 384), "fp64"): 18.045e9}
 TRAFFIC_SOURCE = "profiles/channel_1024x512x384_hbm_traffic.md"
 
 
