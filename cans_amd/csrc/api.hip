@@ -521,6 +521,19 @@ static int rk3_substep(cans_ctx* c, const cans_step_params* s, int irk, double d
   S* divu = (S*)divu_v;
   if (c->imp && delta)
     return fail(CANS_EUNSUPPORTED, "cans_rk3_substep: implicit diffusion needs a CANS_FP64 context");
+  // refinement sweeps of this substep's pressure solve (cans_ctx_set_refinement; mixed mode):
+  // their corrections live in two halo'd fields of S, each field's element 1 256-byte aligned
+  const int nsweep = (delta && ((c->refine_mask >> (irk - 1)) & 1)) ? c->refine_sweeps : 0;
+  S* corr[2] = {nullptr, nullptr};
+  if (nsweep) {
+    if (s->rhsbx_d || s->rhsby_d || s->rhsbz_d)
+      return fail(CANS_EUNSUPPORTED, "cans_rk3_substep: refinement needs homogeneous pressure boundary values");
+    const size_t fe = ((size_t)c->L.sz * (c->L.n3 + 2) + 63) & ~(size_t)63;
+    void* cb = nullptr;
+    if ((rc = ensure_work(c, 7, 2 * fe * sizeof(S) + 256, &cb))) return rc;
+    corr[0] = (S*)((char*)cb + 256 - sizeof(S));
+    corr[1] = corr[0] + fe;
+  }
   // scalars first, from the old velocity (main.f90:440-450): rk_scal, bulk_forcing_s,
   // [solve_helmholtz], boundp.  A scalar's step reads only the velocity and itself, so
   // all of them run rk_scal first and then share one batched Helmholtz solve.
@@ -589,11 +602,39 @@ static int rk3_substep(cans_ctx* c, const cans_step_params* s, int irk, double d
       return rc;
   }
   if ((rc = plan_boundp(c, s->cbcpre, s->bcpre, s->nb, s->is_bound, s->dl, s->dzc_h, pp, CANS_HALO_ABSENT))) return rc;
+  // Refinement sweeps: solve L x_q = b - L(pp + x_1 + ... + x_{q-1}) with FP32 storage, the
+  // residual formed in FP64 from the velocity (b is fillps's rhs) and the boundp'd corrections
+  // so far.  correc then projects with pp + x_1 + x_2, so div(u) after this substep is dtrk times
+  // the residual left by the last sweep; pp keeps the first solve's part only.
+  for (int q = 1; q <= nsweep; q++) {
+    S* xq = corr[q - 1];
+    if (fuse_div) {
+      DivSrc<S> ds = {su, sv, sw, eu, ev, ew, divu, s->dzfi_d, dtrki, dtrki * s->dli[0], dtrki * s->dli[1],
+                      s->dli[0], s->dli[1]};
+      ds.corr[0] = pp;
+      ds.corr[1] = q > 1 ? corr[0] : nullptr;
+      ds.ncorr = q;
+      ds.dzci = s->dzci_d;
+      ds.dxi2 = s->dli[0] * s->dli[0];
+      ds.dyi2 = s->dli[1] * s->dli[1];
+      if ((rc = run_solver<S>(c, s->plan, s->normfft, s->lambdaxy_d, s->a_d, s->b_d, s->c_d, s->cbcpre, "ccc", xq,
+                              &ds)))
+        return rc;
+    } else {
+      if ((rc = launch_fillps_residual<S>(c, su, sv, sw, eu, ev, ew, fvec, xq, s->dzfi_d, dtrki, s->dli, pp,
+                                          q > 1 ? corr[0] : nullptr, s->dzci_d)))
+        return rc;
+      if ((rc = run_solver<S>(c, s->plan, s->normfft, s->lambdaxy_d, s->a_d, s->b_d, s->c_d, s->cbcpre, "ccc", xq)))
+        return rc;
+    }
+    if ((rc = plan_boundp(c, s->cbcpre, s->bcpre, s->nb, s->is_bound, s->dl, s->dzc_h, xq, CANS_HALO_ABSENT)))
+      return rc;
+  }
   // correc reads the intermediate velocity and writes the caller's fields (whole halo'd
   // box); updatep (p += pp) is independent of the velocity halos and rides along
   // with implicit diffusion updatep also adds alpha*lap(pp) (updatep.f90:16-33): its own pass
   if ((rc = launch_correc<S>(c, pp, su, sv, sw, eu, ev, ew, fvec, u, v, w, s->dzci_d, dtrk, s->dli,
-                             c->imp ? nullptr : p)))
+                             c->imp ? nullptr : p, nsweep > 0 ? corr[0] : nullptr, nsweep > 1 ? corr[1] : nullptr)))
     return rc;
   if (c->imp && (rc = launch_updatep<S>(c, pp, p, s->dzci_d, s->dzfi_d, -.5 * s->visc * dtrk, s->dli, c->imp)))
     return rc;
@@ -601,6 +642,17 @@ static int rk3_substep(cans_ctx* c, const cans_step_params* s, int irk, double d
                           CANS_HALO_UPPER, CANS_HALO_NONE)))
     return rc;
   return plan_boundp(c, s->cbcpre, s->bcpre, s->nb, s->is_bound, s->dl, s->dzc_h, p, CANS_HALO_UPPER);
+}
+
+extern "C" int cans_ctx_set_refinement(cans_ctx* c, int sweeps, int substep_mask) {
+  CHECK_CTX("cans_ctx_set_refinement");
+  if (sweeps < 0 || sweeps > 2) return fail(CANS_EINVAL, "cans_ctx_set_refinement: sweeps must be 0, 1 or 2");
+  if (substep_mask < 0 || substep_mask > 7) return fail(CANS_EINVAL, "cans_ctx_set_refinement: substep_mask is 3 bits");
+  if (sweeps && c->precision != CANS_MIXED)
+    return fail(CANS_EINVAL, "cans_ctx_set_refinement: refinement applies to CANS_MIXED contexts");
+  c->refine_sweeps = sweeps;
+  c->refine_mask = sweeps ? substep_mask : 0;
+  return CANS_OK;
 }
 
 extern "C" int cans_rk3_substep(cans_ctx* c, const cans_step_params* s, int irk, double dt, cans_state* st) {

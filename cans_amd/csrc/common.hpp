@@ -76,8 +76,8 @@ struct cans_ctx {
   // scratch owned by the context (grown on demand, freed at destroy):
   // 0 RK intermediate fields, 1 transform work buffer, 2 Thomas checkpoints,
   // 3 small z tables, 4 divergence of the old velocity (mixed mode), 5 Helmholtz z tables,
-  // 6 distributed z solve (factors and reduced systems)
-  static constexpr int NWORK = 7;
+  // 6 distributed z solve (factors and reduced systems), 7 refinement corrections (mixed mode)
+  static constexpr int NWORK = 8;
   void* work[NWORK] = {};
   size_t work_bytes[NWORK] = {};
   double* dscal = nullptr;     // device scalars (64): [0..2] forcing f, [8..11] reductions,
@@ -87,4 +87,6 @@ struct cans_ctx {
   bool rk_first = true;        // the next rk call is the first on this context
   int imp = 0;                 // &numerics: 0 explicit diffusion, 1 is_impdiff, 2 is_impdiff_1d
   int dtdma = 0;               // &numerics is_poisson_dtdma: z slabs of the distributed solve (<= 1: off)
+  int refine_sweeps = 0;       // refinement sweeps of the fused mixed substep's pressure solve
+  int refine_mask = 0;         // substeps (bit irk-1) that refine
 };

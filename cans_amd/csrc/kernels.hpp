@@ -155,10 +155,17 @@ int launch_add_forcing(cans_ctx* c, const T* src, T* dst, int comp, int apply);
 template <typename S>
 int launch_fillps(cans_ctx* c, const double* u, const double* v, const double* w, const S* eu, const S* ev,
                   const S* ew, const double* f, S* p, const double* dzfi, double dti, const double* dli);
+// p1, p2 (optional): further pressure corrections; the gradient and the update of p then use
+// p + p1 + p2, summed in FP64 (the corrections of a refined solve)
 template <typename S>
 int launch_correc(cans_ctx* c, const S* p, const double* su, const double* sv, const double* sw, const S* eu,
                   const S* ev, const S* ew, const double* f, double* du, double* dv, double* dw, const double* dzci,
-                  double dt, const double* dli, S* p_update);
+                  double dt, const double* dli, S* p_update, const S* p1 = nullptr, const S* p2 = nullptr);
+// fillps of the velocity minus L(c0 + c1) (the residual of a refined solve; c1 may be nullptr)
+template <typename S>
+int launch_fillps_residual(cans_ctx* c, const double* u, const double* v, const double* w, const S* eu,
+                           const S* ev, const S* ew, const double* f, S* p, const double* dzfi, double dti,
+                           const double* dli, const S* c0, const S* c1, const double* dzci);
 template <typename S>
 int launch_updatep(cans_ctx* c, const S* pp, S* p, const double* dzci, const double* dzfi, double alpha,
                    const double* dli, int impdiff);
@@ -205,10 +212,26 @@ struct DivSrc {
   const double* dzfi;
   double dti, dtidxi, dtidyi, dxi, dyi;
   const double* f = nullptr;  // FP64 velocity (no increments): bulk forcing added where the velocity is read
+  // Iterative refinement: the rhs becomes the residual b - L(corr[0] + ... + corr[ncorr-1]), L the
+  // solver's 7-point operator (second differences with dxi2, dyi2; dzci/dzfi along z) applied to
+  // the sum of the earlier corrections, halo'd fields of S with their ghost cells set by boundp.
+  const S* corr[2] = {nullptr, nullptr};
+  int ncorr = 0;
+  const double* dzci = nullptr;
+  double dxi2 = 0., dyi2 = 0.;
 };
+// A caller-provided workspace for one solve: the transform work buffer and the z pass's
+// checkpoints are carved from [base, base + bytes) instead of the context's scratch slots,
+// so the solve allocates nothing.
+struct SolveWork {
+  void* base;
+  size_t bytes;
+};
+// bytes of workspace one solve of the context's grid needs with element size es
+size_t solver_work_bytes(const cans_ctx* c, size_t es, bool periodic_z);
 template <typename S>
 int run_solver(cans_ctx* c, const cans_plan* plan, double normfft, const double* lam, const double* a,
                const double* b, const double* cc, const char* bc, const char* c_or_f, S* p,
-               const DivSrc<S>* div = nullptr);
+               const DivSrc<S>* div = nullptr, const SolveWork* ws = nullptr);
 
 }  // namespace cans

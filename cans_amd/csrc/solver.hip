@@ -20,14 +20,19 @@
 
 // transform family per direction
 // K_ND, K_DN: REDFT11 and RODFT11 (cell-centred N-D and D-N pairs, fft.f90:280-287), computed on a
-// line of twice the data length (below)
-enum { K_PP = 0, K_NN = 1, K_DD = 2, K_ND = 3, K_DN = 4 };
+// line of twice the data length (below).  Face-centred pairs (fft.f90:289-311): PP as K_PP, ND as
+// K_NN (REDFT10 forward, REDFT01 backward), DN as K_DNF (RODFT01 forward, RODFT10 backward: the
+// K_DD pair with the directions swapped), DD as K_DDF (RODFT00 on the n-1 interior faces,
+// fft.f90:80-82, by the odd extension of the line to 2n points; the boundary face n passes through).
+enum { K_PP = 0, K_NN = 1, K_DD = 2, K_ND = 3, K_DN = 4, K_DNF = 5, K_DDF = 6 };
+// kinds whose lines hold twice the data length
+__host__ __device__ __forceinline__ bool k_double(int kind) { return kind == K_ND || kind == K_DN || kind == K_DDF; }
 
 struct cans_plan {
   cans_ctx* ctx;
   int ng[3];
   int len[2];        // transform lengths (x, y)
-  int nfft[2];       // complex line lengths: len, or 2 len for K_ND / K_DN
+  int nfft[2];       // complex line lengths: len, or 2 len for K_ND / K_DN / K_DDF
   int logn[2];       // nfft = 2^logn * odd
   int odd[2];        // odd part of len: 1, or a product of at most three factors 3 and 5
   int kind[2];
@@ -175,15 +180,44 @@ struct XArgs {
   T* fld[NBATCH];  // halo'd fields (origin), one per system of a batch (blockIdx.y)
   T* W[NBATCH];    // their z-tiled work buffers (woff)
   int N, logN, odd, Tn, pairs, kind;
-  int nd;          // data length along x: N, or N/2 for K_ND / K_DN
+  int nd;          // data length along x: N, or N/2 for K_ND / K_DN / K_DDF
   const cplx<double>* tw;
   const cplx<double>* mk;
   int div;         // forward only: build the rhs from the velocity instead of reading fld
   DivSrc<T> ds;
 };
 
-// two consecutive rhs values (elements e, e+1 of row r) of fillps.f90:38-50
+// L(corr[0] + ... ) at the cells c, c+1 of row (j,k): the 7-point operator of the solver
+// (the eigenvalues and tridiagonal coefficients of initsolver.f90:85-187) on the sum of a
+// refined solve's earlier corrections, whose ghost cells boundp has set
 template <typename T>
+__device__ __forceinline__ d2 corr_lap_pair(const DivSrc<T>& ds, const Layout& L, long c, int k) {
+  auto sum2 = [&](long o) -> d2 {
+    d2 v = ld2(ds.corr[0] + o);
+    if (ds.ncorr > 1) {
+      const d2 w = ld2(ds.corr[1] + o);
+      v = d2(v.x + w.x, v.y + w.y);
+    }
+    return v;
+  };
+  auto sum1 = [&](long o) -> double {
+    double v = (double)ds.corr[0][o];
+    if (ds.ncorr > 1) v += (double)ds.corr[1][o];
+    return v;
+  };
+  const d2 x0 = sum2(c), ym = sum2(c - L.sy), yp = sum2(c + L.sy), zm = sum2(c - L.sz), zp = sum2(c + L.sz);
+  const double xm = sum1(c - 1), xp = sum1(c + 2);
+  const double dzc = ds.dzci[k], dzm = ds.dzci[k - 1], dzf = ds.dzfi[k];
+  const double l0 = (x0.y - 2. * x0.x + xm) * ds.dxi2 + (yp.x - 2. * x0.x + ym.x) * ds.dyi2 +
+                    ((zp.x - x0.x) * dzc - (x0.x - zm.x) * dzm) * dzf;
+  const double l1 = (xp - 2. * x0.y + x0.x) * ds.dxi2 + (yp.y - 2. * x0.y + ym.y) * ds.dyi2 +
+                    ((zp.y - x0.y) * dzc - (x0.y - zm.y) * dzm) * dzf;
+  return d2(l0, l1);
+}
+
+// two consecutive rhs values (elements e, e+1 of row r) of fillps.f90:38-50; RES: minus
+// L(corrections), the residual of a refinement sweep
+template <typename T, bool RES = false>
 __device__ __forceinline__ d2 div_pair(const XArgs<T>& a, long r, int e) {
   const Layout& L = a.L;
   const int j = (int)(r % L.n2) + 1, k = (int)(r / L.n2) + 1;
@@ -194,8 +228,13 @@ __device__ __forceinline__ d2 div_pair(const XArgs<T>& a, long r, int e) {
     const d2 ev2 = ld2(a.ds.ev + c), evm = ld2(a.ds.ev + c - L.sy);
     const double eum = (double)a.ds.eu[c - 1];
     const double dzfi = a.ds.dzfi[k], dyi = a.ds.dyi, dxi = a.ds.dxi, dti = a.ds.dti;
-    const double q0 = dti * (dv.x + ((ew2.x - ewm.x) * dzfi + (ev2.x - evm.x) * dyi + (eu2.x - eum) * dxi));
-    const double q1 = dti * (dv.y + ((ew2.y - ewm.y) * dzfi + (ev2.y - evm.y) * dyi + (eu2.y - eu2.x) * dxi));
+    double q0 = dti * (dv.x + ((ew2.x - ewm.x) * dzfi + (ev2.x - evm.x) * dyi + (eu2.x - eum) * dxi));
+    double q1 = dti * (dv.y + ((ew2.y - ewm.y) * dzfi + (ev2.y - evm.y) * dyi + (eu2.y - eu2.x) * dxi));
+    if constexpr (RES) {
+      const d2 lp = corr_lap_pair(a.ds, L, c, k);
+      q0 = q0 - lp.x;
+      q1 = q1 - lp.y;
+    }
     return d2(q0, q1);
   }
   d2 u2 = ld2(a.ds.u + c), w2 = ld2(a.ds.w + c), wm = ld2(a.ds.w + c - L.sz);
@@ -221,23 +260,32 @@ __device__ __forceinline__ d2 div_pair(const XArgs<T>& a, long r, int e) {
     um = um + (double)a.ds.eu[c - 1];
   }
   const double dti = a.ds.dti, dzfi = a.ds.dzfi[k], dy = a.ds.dtidyi, dx = a.ds.dtidxi;
-  const double p0 = ((w2.x - wm.x) * dti * dzfi + (v2.x - vm.x) * dy + (u2.x - um) * dx);
-  const double p1 = ((w2.y - wm.y) * dti * dzfi + (v2.y - vm.y) * dy + (u2.y - u2.x) * dx);
+  double p0 = ((w2.x - wm.x) * dti * dzfi + (v2.x - vm.x) * dy + (u2.x - um) * dx);
+  double p1 = ((w2.y - wm.y) * dti * dzfi + (v2.y - vm.y) * dy + (u2.y - u2.x) * dx);
+  if constexpr (RES) {
+    const d2 lp = corr_lap_pair(a.ds, L, c, k);
+    p0 = p0 - lp.x;
+    p1 = p1 - lp.y;
+  }
   return d2(p0, p1);
 }
 
 // DIR +1 forward (fld -> W), -1 backward (W -> fld); DV (forward only): the rhs is
-// formed from the velocity (a.ds) instead of read from fld.  DV is a template flag
+// formed from the velocity (a.ds) instead of read from fld (DV = 2: minus L(corrections),
+// the residual of a refinement sweep).  DV is a template flag
 // so that the plain forward pass carries none of the divergence code's registers.
 // bx: work-group index among this system's; sys: system of a batch.  Lanes past the
 // last whole team (a.pairs teams of a.Tn) only take part in the barriers.
-template <int DIR, typename T, int DV = 0>
+// FLIP (face-centred DN, K_DNF): the K_DD pair with swapped directions, so the forward pass
+// runs the RODFT01 code and the backward pass the RODFT10 code.
+template <int DIR, typename T, int DV = 0, bool FLIP = false>
 __device__ __forceinline__ void xpass_body(const XArgs<T>& a, int bx, int sys) {
   T* const fld = a.fld[sys];
   T* const W = a.W[sys];
   extern __shared__ unsigned char smem_raw[];
   cplx<T>* smem = (cplx<T>*)smem_raw;  // LDS line in the storage type
-  const int N = a.N, Tn = a.Tn, kind = a.kind;
+  const int N = a.N, Tn = a.Tn, kind = FLIP ? K_DD : a.kind;
+  constexpr bool FWD = (DIR > 0) != FLIP;  // forward-type code (R2HC, REDFT10, RODFT10)
   const int team = threadIdx.x / Tn, lt = threadIdx.x % Tn;
   const bool act = team < a.pairs;
   const long nrows = (long)a.L.n2 * a.L.n3;
@@ -267,11 +315,31 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& a, int bx, int sys) {
   const RowPtr<T> xb = DIR > 0 ? frow(vb, rb) : wrow(vb, rb);
   const RowPtr<T> ya = DIR > 0 ? wrow(va, ra) : frow(va, ra);
   const RowPtr<T> yb = DIR > 0 ? wrow(vb, rb) : frow(vb, rb);
-  if (DIR > 0 || kind >= K_ND) {
+  if (FWD || k_double(kind)) {
     if (va) {
       if (N == 1) {
         if (lt == 0) smem[ix(0)] = narrow<T>((double)xa[0], xb ? (double)xb[0] : 0.);
-      } else if (kind >= K_ND) {
+      } else if (kind == K_DDF) {
+        // odd extension: data element e (< nd-1) at point e+1 and its negative at N-1-e; points 0
+        // and nd are zero; the boundary face nd-1 is not transformed and passes through
+        const int nd = a.nd;
+        if (lt == 0) {
+          smem[ix(0)] = narrow<T>(0., 0.);
+          smem[ix(nd)] = narrow<T>(0., 0.);
+        }
+        for (int e = 2 * lt; e < nd; e += 2 * Tn) {
+          const d2 a2 = ld2(xa + e), b2 = xb ? ld2(xb + e) : d2(0., 0.);
+          smem[ix(e + 1)] = narrow<T>(a2.x, b2.x);
+          smem[ix(N - 1 - e)] = narrow<T>(-a2.x, -b2.x);
+          if (e + 1 < nd - 1) {
+            smem[ix(e + 2)] = narrow<T>(a2.y, b2.y);
+            smem[ix(N - 2 - e)] = narrow<T>(-a2.y, -b2.y);
+          } else {
+            ya[e + 1] = (T)a2.y;
+            if (yb) yb[e + 1] = (T)b2.y;
+          }
+        }
+      } else if (kind == K_ND || kind == K_DN) {
         const int nd = a.nd;
         const bool dn = kind == K_DN;  // RODFT11 reads the reversed input
         for (int p = (nd + 1) / 2 + lt; p < N - nd / 2; p += Tn) smem[ix(p)] = narrow<T>(0., 0.);
@@ -284,8 +352,8 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& a, int bx, int sys) {
         for (int e = 2 * lt; e < N; e += 2 * Tn) {
           d2 a2, b2;
           if constexpr (DV != 0) {
-            a2 = div_pair(a, ra, e);
-            b2 = xb ? div_pair(a, rb, e) : d2(0., 0.);
+            a2 = div_pair<T, DV == 2>(a, ra, e);
+            b2 = xb ? div_pair<T, DV == 2>(a, rb, e) : d2(0., 0.);
           } else {
             a2 = ld2(xa + e);
             b2 = xb ? ld2(xb + e) : d2(0., 0.);
@@ -327,7 +395,21 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& a, int bx, int sys) {
         st2(ya + 2 * k, are, aim);
         if (yb) st2(yb + 2 * k, bre, bim);
       }
-    } else if (kind >= K_ND) {
+    } else if (kind == K_DDF) {
+      // RODFT00 of the two lines: Y_A(k-1) = -Im Z_k, Y_B(k-1) = Re Z_k, k = 1 .. nd-1
+      const int nd = a.nd;
+      for (int m = 2 * lt; m < nd - 1; m += 2 * Tn) {
+        const cplx<double> z0 = widen(smem[ix(m + 1)]);
+        if (m + 1 < nd - 1) {
+          const cplx<double> z1 = widen(smem[ix(m + 2)]);
+          st2(ya + m, -z0.im, -z1.im);
+          if (yb) st2(yb + m, z0.re, z1.re);
+        } else {
+          ya[m] = (T)(-z0.im);
+          if (yb) yb[m] = (T)z0.re;
+        }
+      }
+    } else if (kind == K_ND || kind == K_DN) {
       // outputs m, m+1 (m even) are spectral indices 2m+1, 2m+3 of the N-point REDFT10
       for (int m = 2 * lt; m < a.nd; m += 2 * Tn) {
         double y0a, y0b, y1a, y1b;
@@ -412,13 +494,13 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& a, int bx, int sys) {
 }
 
 // one kernel per storage type (the FP32 and FP64 forms differ in LDS and register use)
-template <int DIR, int DV>
+template <int DIR, int DV, bool FLIP = false>
 __global__ void __launch_bounds__(512) k_xpass_f(XArgs<float> a) {
-  xpass_body<DIR, float, DV>(a, blockIdx.x, blockIdx.y);
+  xpass_body<DIR, float, DV, FLIP>(a, blockIdx.x, blockIdx.y);
 }
-template <int DIR, int DV>
+template <int DIR, int DV, bool FLIP = false>
 __global__ void __launch_bounds__(512) k_xpass_d(XArgs<double> a) {
-  xpass_body<DIR, double, DV>(a, blockIdx.x, blockIdx.y);
+  xpass_body<DIR, double, DV, FLIP>(a, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -432,18 +514,19 @@ struct YArgs {
   int n1, n2, n3;
   T* W[NBATCH];    // z-tiled work buffers (woff), one per system of a batch (blockIdx.y)
   int N, logN, odd, Tn, lines, kind;
-  int nd;          // data length along y: N, or N/2 for K_ND / K_DN
+  int nd;          // data length along y: N, or N/2 for K_ND / K_DN / K_DDF
   const cplx<double>* tw;
   const cplx<double>* mk;
 };
 
 // bx: work-group index among this system's; sys: system of a batch.  Lanes past
-// a.lines * a.Tn only take part in the barriers.
-template <int DIR, typename T>
+// a.lines * a.Tn only take part in the barriers.  FLIP: as in xpass_body.
+template <int DIR, typename T, bool FLIP = false>
 __device__ __forceinline__ void ypass_body(const YArgs<T>& a, int bx, int sys) {
   extern __shared__ unsigned char smem_raw[];
   cplx<T>* smem = (cplx<T>*)smem_raw;  // LDS line in the storage type
-  const int LN = a.lines, N = a.N, Tn = a.Tn, kind = a.kind;
+  const int LN = a.lines, N = a.N, Tn = a.Tn, kind = FLIP ? K_DD : a.kind;
+  constexpr bool FWD = (DIR > 0) != FLIP;
   const int c = threadIdx.x % LN, lt = threadIdx.x / LN;
   const bool act = lt < Tn;
   // column pairs are numbered plane by plane; a work-group's lines may span planes
@@ -463,8 +546,20 @@ __device__ __forceinline__ void ypass_body(const YArgs<T>& a, int bx, int sys) {
     if (vb) st2(col(e), x, y);
     else *col(e) = (T)x;
   };
-  if (DIR > 0 || kind >= K_ND) {
-    if (va && kind >= K_ND) {
+  if (FWD || k_double(kind)) {
+    if (va && kind == K_DDF) {
+      // odd extension (as in the x pass); the boundary face nd-1 stays in place
+      const int nd = a.nd;
+      if (lt == 0) {
+        smem[ix(0)] = narrow<T>(0., 0.);
+        smem[ix(nd)] = narrow<T>(0., 0.);
+      }
+      for (int e = lt; e < nd - 1; e += Tn) {
+        const d2 v = ld(e);
+        smem[ix(e + 1)] = narrow<T>(v.x, v.y);
+        smem[ix(N - 1 - e)] = narrow<T>(-v.x, -v.y);
+      }
+    } else if (va && (kind == K_ND || kind == K_DN)) {
       const int nd = a.nd;
       for (int p = (nd + 1) / 2 + lt; p < N - nd / 2; p += Tn) smem[ix(p)] = narrow<T>(0., 0.);
       for (int e = lt; e < nd; e += Tn) {
@@ -504,7 +599,12 @@ __device__ __forceinline__ void ypass_body(const YArgs<T>& a, int bx, int sys) {
         st(2 * kk, are, bre);
         st(2 * kk + 1, aim, bim);
       }
-    } else if (kind >= K_ND) {
+    } else if (kind == K_DDF) {
+      for (int m = lt; m < a.nd - 1; m += Tn) {
+        const cplx<double> z = widen(smem[ix(m + 1)]);
+        st(m, -z.im, z.re);
+      }
+    } else if (kind == K_ND || kind == K_DN) {
       for (int m = lt; m < a.nd; m += Tn) {
         double ya, yb;
         split_dct(widen(smem[ix(2 * m + 1)]), widen(smem[ix(N - 2 * m - 1)]), a.mk[2 * m + 1], ya, yb);
@@ -572,31 +672,24 @@ __device__ __forceinline__ void ypass_body(const YArgs<T>& a, int bx, int sys) {
 // FP32 lines: 16 lines x N/8 threads = 1024 threads in 64 KB of LDS; at most 64 VGPRs
 // lets two such work-groups share a CU.  FP64 lines: LDS already limits a CU to two
 // 512-thread work-groups, which needs no register cap.
-template <int DIR>
+template <int DIR, bool FLIP = false>
 __global__ void __launch_bounds__(1024, 8) k_ypass_f(YArgs<float> a) {
-  ypass_body<DIR, float>(a, blockIdx.x, blockIdx.y);
+  ypass_body<DIR, float, FLIP>(a, blockIdx.x, blockIdx.y);
 }
-template <int DIR>
+template <int DIR, bool FLIP = false>
 __global__ void __launch_bounds__(1024) k_ypass_d(YArgs<double> a) {
-  ypass_body<DIR, double>(a, blockIdx.x, blockIdx.y);
+  ypass_body<DIR, double, FLIP>(a, blockIdx.x, blockIdx.y);
 }
 template <int DIR, typename T>
-void launch_ypass(dim3 grid, int threads, size_t shm, hipStream_t st, const YArgs<T>& a);
-template <>
-void launch_ypass<1, float>(dim3 g, int t, size_t shm, hipStream_t st, const YArgs<float>& a) {
-  k_ypass_f<1><<<g, t, shm, st>>>(a);
-}
-template <>
-void launch_ypass<-1, float>(dim3 g, int t, size_t shm, hipStream_t st, const YArgs<float>& a) {
-  k_ypass_f<-1><<<g, t, shm, st>>>(a);
-}
-template <>
-void launch_ypass<1, double>(dim3 g, int t, size_t shm, hipStream_t st, const YArgs<double>& a) {
-  k_ypass_d<1><<<g, t, shm, st>>>(a);
-}
-template <>
-void launch_ypass<-1, double>(dim3 g, int t, size_t shm, hipStream_t st, const YArgs<double>& a) {
-  k_ypass_d<-1><<<g, t, shm, st>>>(a);
+void launch_ypass(dim3 g, int t, size_t shm, hipStream_t st, const YArgs<T>& a) {
+  const bool flip = a.kind == K_DNF;
+  if constexpr (sizeof(T) == sizeof(float)) {
+    if (flip) k_ypass_f<DIR, true><<<g, t, shm, st>>>(a);
+    else k_ypass_f<DIR, false><<<g, t, shm, st>>>(a);
+  } else {
+    if (flip) k_ypass_d<DIR, true><<<g, t, shm, st>>>(a);
+    else k_ypass_d<DIR, false><<<g, t, shm, st>>>(a);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1160,11 +1253,27 @@ static int xpass(cans_ctx* c, const cans_plan* p, T* const* fld, T* const* W, in
   const long npairs = (nrows + 1) / 2;
   dim3 grid(cdiv(npairs, a.pairs), nsys);
   const size_t shm = (size_t)a.pairs * padded_len(a.N) * sizeof(cplx<T>);
+  const bool res = a.div && a.ds.ncorr > 0;
+  const int nt = a.pairs * a.Tn;
+  if (a.kind == K_DNF) {  // face-centred DN: the K_DD code with the directions swapped
+    if (a.div) return fail(CANS_EINVAL, "k_xpass: the divergence input needs a periodic x transform");
+    if constexpr (sizeof(T) == sizeof(float)) {
+      if (dir < 0) k_xpass_f<-1, 0, true><<<grid, nt, shm, c->stream>>>(a);
+      else k_xpass_f<1, 0, true><<<grid, nt, shm, c->stream>>>(a);
+    } else {
+      if (dir < 0) k_xpass_d<-1, 0, true><<<grid, nt, shm, c->stream>>>(a);
+      else k_xpass_d<1, 0, true><<<grid, nt, shm, c->stream>>>(a);
+    }
+    CANS_LAUNCH_CHECK("k_xpass");
+    return CANS_OK;
+  }
   if constexpr (sizeof(T) == sizeof(float)) {
     if (dir < 0) k_xpass_f<-1, 0><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
+    else if (res) k_xpass_f<1, 2><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
     else if (a.div) k_xpass_f<1, 1><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
     else k_xpass_f<1, 0><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
   } else {
+    if (res) return fail(CANS_EUNSUPPORTED, "k_xpass: refinement residuals are formed for FP32 storage only");
     if (dir < 0) k_xpass_d<-1, 0><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
     else if (a.div) k_xpass_d<1, 1><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
     else k_xpass_d<1, 0><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
@@ -1207,26 +1316,47 @@ static int ypass(cans_ctx* c, const cans_plan* p, T* const* W, int nsys, int dir
 // is one launch for all of them (blockIdx.y = system).  The z systems may differ
 // (coefficients, c/f grid, normalisation); their z boundary pairs must agree on
 // periodicity.  nsys = 1 is the plain solver.
+// work buffer (rounded up to 256 bytes) and checkpoint bytes of nsys systems
+static void solver_work_split(const Layout& L, int nsys, size_t es, bool periodic, size_t* wbytes,
+                              size_t* ckbytes) {
+  const long nl = (long)L.n1 * L.n2;
+  const long nblk = (L.n3 + CK - 1) / CK;  // enough checkpoint blocks for any system of the batch
+  *wbytes = ((size_t)nsys * wbuf_elems(L.n1, L.n2, L.n3) * es + 255) & ~(size_t)255;
+  *ckbytes = (size_t)nsys * 3 * nblk * nl * sizeof(double) + (periodic ? (size_t)L.cells() * sizeof(double) : 0);
+}
+
+size_t solver_work_bytes(const cans_ctx* c, size_t es, bool periodic_z) {
+  size_t w, k;
+  solver_work_split(c->L, 1, es, periodic_z, &w, &k);
+  return w + k;
+}
+
 template <typename T>
-int run_solver_batch(cans_ctx* c, const cans_plan* p, int nsys, const SolveSys<T>* sy, const DivSrc<T>* div) {
+int run_solver_batch(cans_ctx* c, const cans_plan* p, int nsys, const SolveSys<T>* sy, const DivSrc<T>* div,
+                     const SolveWork* ws = nullptr) {
   const Layout& L = c->L;
-  const long N = L.cells();
   const long nl = (long)L.n1 * L.n2;
   if (nsys < 1 || nsys > NBATCH) return fail(CANS_EINVAL, "run_solver: batch size out of range");
   const int periodic = sy[0].bc[4] == 'P' && sy[0].bc[5] == 'P';
   if (nsys > 1 && (periodic || div || c->dtdma > 1))
     return fail(CANS_EUNSUPPORTED, "run_solver: batches need a non-periodic z and no fused divergence or slabs");
-  const long wel = wbuf_elems(L.n1, L.n2, L.n3);
-  void* W = nullptr;
-  int rc = ensure_work(c, 1, (size_t)nsys * wel * sizeof(T), &W);
-  if (rc) return rc;
+  if (ws && c->dtdma > 1) return fail(CANS_EUNSUPPORTED, "run_solver: a caller workspace does not cover DTDMA slabs");
   if ((periodic ? L.n3 - 1 : L.n3) < 1) return fail(CANS_EUNSUPPORTED, "cans_solver: fewer than 2 z rows");
-  const long nblk = (L.n3 + CK - 1) / CK;  // enough checkpoint blocks for any system of the batch
-  void* ck = nullptr;
+  const long wel = wbuf_elems(L.n1, L.n2, L.n3);
+  const long nblk = (L.n3 + CK - 1) / CK;
   const size_t ckper = (size_t)3 * nblk * nl;
-  size_t ckbytes = (size_t)nsys * ckper * sizeof(double) + (periodic ? (size_t)N * sizeof(double) : 0);
-  rc = ensure_work(c, 2, ckbytes, &ck);
-  if (rc) return rc;
+  size_t wbytes, ckbytes;
+  solver_work_split(L, nsys, sizeof(T), periodic, &wbytes, &ckbytes);
+  void *W = nullptr, *ck = nullptr;
+  int rc;
+  if (ws) {
+    if (ws->bytes < wbytes + ckbytes) return fail(CANS_EINVAL, "run_solver: workspace too small");
+    W = ws->base;
+    ck = (char*)ws->base + wbytes;
+  } else {
+    if ((rc = ensure_work(c, 1, wbytes, &W))) return rc;
+    if ((rc = ensure_work(c, 2, ckbytes, &ck))) return rc;
+  }
   T* fld[NBATCH];
   T* Wq[NBATCH];
   for (int q = 0; q < nsys; q++) {
@@ -1318,9 +1448,9 @@ int run_solver_batch(cans_ctx* c, const cans_plan* p, int nsys, const SolveSys<T
 template <typename T>
 int run_solver(cans_ctx* c, const cans_plan* p, double normfft, const double* lam, const double* a,
                const double* b, const double* cc, const char* bc, const char* c_or_f, T* fld,
-               const DivSrc<T>* div) {
+               const DivSrc<T>* div, const SolveWork* ws) {
   SolveSys<T> s1 = {fld, normfft, lam, a, b, cc, bc, c_or_f};
-  return run_solver_batch<T>(c, p, 1, &s1, div);
+  return run_solver_batch<T>(c, p, 1, &s1, div, ws);
 }
 
 // ---------------------------------------------------------------------------
@@ -1505,9 +1635,11 @@ template int run_helmholtz_3d<float>(cans_ctx*, const cans_plan*, double, double
                                      const double*, const double*, const char*, const char*, float*);
 
 template int run_solver<double>(cans_ctx*, const cans_plan*, double, const double*, const double*, const double*,
-                                const double*, const char*, const char*, double*, const DivSrc<double>*);
+                                const double*, const char*, const char*, double*, const DivSrc<double>*,
+                                const SolveWork*);
 template int run_solver<float>(cans_ctx*, const cans_plan*, double, const double*, const double*, const double*,
-                               const double*, const char*, const char*, float*, const DivSrc<float>*);
+                               const double*, const char*, const char*, float*, const DivSrc<float>*,
+                               const SolveWork*);
 
 }  // namespace cans
 
@@ -1727,7 +1859,7 @@ static int batch_create(cans_ctx* c, int nbox, const cans_poisson_box* bx, doubl
       else if (b0 == 'D' && b1 == 'N') b.kind[q] = K_DN;
       else return fail(CANS_EINVAL, who + "x/y boundary pairs must be PP, NN, DD, ND or DN");
       b.nd[q] = x.n[q];
-      b.N[q] = b.kind[q] >= K_ND ? 2 * x.n[q] : x.n[q];
+      b.N[q] = k_double(b.kind[q]) ? 2 * x.n[q] : x.n[q];
       if (!fft_factor(b.N[q], &b.logN[q], &b.odd[q]) || b.N[q] > 2048)
         return fail(CANS_EUNSUPPORTED, who + "x/y lengths must be 2^m, or 2^m (m >= 3) times at most three "
                                              "factors 3 and 5, at most 2048 (ND and DN pairs: the doubled length)");
@@ -2008,30 +2140,40 @@ extern "C" int cans_fftini(cans_ctx* c, const int* ng, const char* bcxy, const c
   double nf = 1.;
   for (int d = 0; d < 2; d++) {
     const char b0 = bcxy[2 * d], b1 = bcxy[2 * d + 1];
-    // face-centred periodic directions use the same R2HC/HC2R pair and norm (fft.f90:260-313)
-    if (c_or_f[d] != 'c' && !(c_or_f[d] == 'f' && b0 == 'P' && b1 == 'P')) {
+    const bool face = c_or_f[d] == 'f';
+    if (c_or_f[d] != 'c' && !face) {
       delete p;
-      return fail(CANS_EUNSUPPORTED, "cans_fftini: face-centred non-periodic transforms are not implemented on the device");
+      return fail(CANS_EINVAL, "cans_fftini: c_or_f must be 'c' or 'f'");
     }
-    if (b0 == 'P' && b1 == 'P') p->kind[d] = K_PP;
-    else if (b0 == 'N' && b1 == 'N') p->kind[d] = K_NN;
-    else if (b0 == 'D' && b1 == 'D') p->kind[d] = K_DD;
-    else if (b0 == 'N' && b1 == 'D') p->kind[d] = K_ND;
-    else if (b0 == 'D' && b1 == 'N') p->kind[d] = K_DN;
-    else {
+    // kinds of fft.f90:260-313; the normalisation factor norm(1)*(n+norm(2)-ix) of fft.f90:75,140
+    double nfac;
+    if (b0 == 'P' && b1 == 'P') p->kind[d] = K_PP, nfac = ng[d];
+    else if (b0 == 'N' && b1 == 'N' && !face) p->kind[d] = K_NN, nfac = 2. * ng[d];
+    else if (b0 == 'D' && b1 == 'D') p->kind[d] = face ? K_DDF : K_DD, nfac = 2. * ng[d];
+    else if (b0 == 'N' && b1 == 'D') p->kind[d] = face ? K_NN : K_ND, nfac = 2. * ng[d];
+    else if (b0 == 'D' && b1 == 'N') p->kind[d] = face ? K_DNF : K_DN, nfac = 2. * ng[d];
+    else if (b0 == 'N' && b1 == 'N') {
+      // REDFT00: the reference forbids face-centred NN velocity walls with implicit diffusion
+      // (sanity.f90:151-162), the only user of face-centred transforms
+      delete p;
+      return fail(CANS_EUNSUPPORTED, "cans_fftini: face-centred NN (REDFT00) is not built on the device");
+    } else {
       delete p;
       return fail(CANS_EINVAL, "cans_fftini: boundary pairs must be PP, NN, DD, ND or DN");
     }
+    if (p->kind[d] == K_DDF && ng[d] < 2) {
+      delete p;
+      return fail(CANS_EINVAL, "cans_fftini: face-centred DD needs at least 2 points");
+    }
     p->len[d] = ng[d];
-    p->nfft[d] = p->kind[d] >= K_ND ? 2 * ng[d] : ng[d];
+    p->nfft[d] = k_double(p->kind[d]) ? 2 * ng[d] : ng[d];
     if (!fft_factor(p->nfft[d], &p->logn[d], &p->odd[d])) {
       delete p;
       return fail(CANS_EUNSUPPORTED,
                   "cans_fftini: transform lengths must be 2^m, or 2^m (m >= 3) times at most three factors 3 "
                   "and 5, at most 4096 (ND and DN pairs: the doubled length)");
     }
-    // fft.f90:75,140: normfft = 1/prod(norm(1)*(ng+norm(2)-i)), norm = [1,0] (PP) or [2,0] (NN/DD/ND/DN, c-grid)
-    nf = nf * (p->kind[d] == K_PP ? 1. : 2.) * (ng[d] + 0.);
+    nf = nf * nfac;
     int rc = upload_tables(p, d);
     if (rc) {
       cans_fftend(p);

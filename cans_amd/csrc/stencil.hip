@@ -496,11 +496,22 @@ struct CorrecIn {
   double tu, tv, tw, px, py, pz, pu;
 };
 
-template <typename S>
+// NP pressure fields p, p1, p2 (NP > 1: the corrections of a refined solve), read as their FP64 sum
+template <typename S, int NP>
 __global__ void __launch_bounds__(CORREC_TX* CORREC_TY)
-    k_correc(Layout L, const S* p, const double* su, const double* sv, const double* sw, const S* eu, const S* ev,
+    k_correc(Layout L, const S* p0, const double* su, const double* sv, const double* sw, const S* eu, const S* ev,
              const S* ew, const double* f, double* du, double* dv, double* dw, const double* dzci, double dt,
-             double factori, double factorj, S* pu) {
+             double factori, double factorj, S* pu, const S* p1, const S* p2) {
+  struct PSum {
+    const S *a, *b, *c;
+    __device__ __forceinline__ double operator[](long o) const {
+      double v = (double)a[o];
+      if constexpr (NP > 1) v += (double)b[o];
+      if constexpr (NP > 2) v += (double)c[o];
+      return v;
+    }
+  };
+  const PSum p = {p0, p1, p2};
   int i = 1 + blockIdx.x * CORREC_TX + threadIdx.x;
   if (i == L.n1 + 2) i = 0;
   const int j = blockIdx.y * CORREC_TY + threadIdx.y;
@@ -514,9 +525,9 @@ __global__ void __launch_bounds__(CORREC_TX* CORREC_TY)
     q.tu = vel(su, eu, fu, c);
     q.tv = vel(sv, ev, fv, c);
     q.tw = vel(sw, ew, fw, c);
-    q.px = xin ? (double)p[c + 1] : 0.;
-    q.py = yin ? (double)p[c + L.sy] : 0.;
-    q.pz = (k <= L.n3) ? (double)p[c + L.sz] : 0.;
+    q.px = xin ? p[c + 1] : 0.;
+    q.py = yin ? p[c + L.sy] : 0.;
+    q.pz = (k <= L.n3) ? p[c + L.sz] : 0.;
     q.pu = (pu && in_ij && k >= 1 && k <= L.n3) ? (double)pu[c] : 0.;
     return q;
   };
@@ -827,12 +838,55 @@ int launch_fillps(cans_ctx* c, const double* u, const double* v, const double* w
 template <typename S>
 int launch_correc(cans_ctx* c, const S* p, const double* su, const double* sv, const double* sw, const S* eu,
                   const S* ev, const S* ew, const double* f, double* du, double* dv, double* dw, const double* dzci,
-                  double dt, const double* dli, S* p_update) {
+                  double dt, const double* dli, S* p_update, const S* p1, const S* p2) {
   const Layout& L = c->L;
   dim3 grid(cdiv(L.n1 + 2, CORREC_TX), cdiv(L.n2 + 2, CORREC_TY), cdiv(L.n3 + 2, CORREC_KC));
-  k_correc<S><<<grid, dim3(CORREC_TX, CORREC_TY), 0, c->stream>>>(L, p, su, sv, sw, eu, ev, ew, f, du, dv, dw, dzci,
-                                                                   dt, dt * dli[0], dt * dli[1], p_update);
+  const dim3 blk(CORREC_TX, CORREC_TY);
+  if (p2 && !p1) return fail(CANS_EINVAL, "launch_correc: p2 without p1");
+  if (p2)
+    k_correc<S, 3><<<grid, blk, 0, c->stream>>>(L, p, su, sv, sw, eu, ev, ew, f, du, dv, dw, dzci, dt, dt * dli[0],
+                                                dt * dli[1], p_update, p1, p2);
+  else if (p1)
+    k_correc<S, 2><<<grid, blk, 0, c->stream>>>(L, p, su, sv, sw, eu, ev, ew, f, du, dv, dw, dzci, dt, dt * dli[0],
+                                                dt * dli[1], p_update, p1, p2);
+  else
+    k_correc<S, 1><<<grid, blk, 0, c->stream>>>(L, p, su, sv, sw, eu, ev, ew, f, du, dv, dw, dzci, dt, dt * dli[0],
+                                                dt * dli[1], p_update, p1, p2);
   CANS_LAUNCH_CHECK("k_correc");
+  return CANS_OK;
+}
+
+// fillps.f90:13-51 minus the solver's 7-point operator applied to c0 + c1 (c1 may be nullptr):
+// the residual of a refinement sweep when the divergence is not formed inside the first x pass
+template <typename S>
+__global__ void k_fillps_residual(Layout L, const double* u, const double* v, const double* w, const S* eu,
+                                  const S* ev, const S* ew, const double* f, S* p, const double* dzfi, double dti,
+                                  double dtidxi, double dtidyi, const S* c0, const S* c1, const double* dzci,
+                                  double dxi2, double dyi2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y + 1, k = blockIdx.z + 1;
+  if (i > L.n1) return;
+  const long c = L.at(i, j, k);
+  const double fu = f ? f[0] : 0., fv = f ? f[1] : 0., fw = f ? f[2] : 0.;
+  const double wc = vel(w, ew, fw, c), wm = vel(w, ew, fw, c - L.sz), vc = vel(v, ev, fv, c);
+  const double vm = vel(v, ev, fv, c - L.sy), uc = vel(u, eu, fu, c), um = vel(u, eu, fu, c - 1);
+  const double b = (wc - wm) * dti * dzfi[k] + (vc - vm) * dtidyi + (uc - um) * dtidxi;
+  auto x = [&](long o) { return c1 ? (double)c0[o] + (double)c1[o] : (double)c0[o]; };
+  const double xc = x(c);
+  const double lap = (x(c + 1) - 2. * xc + x(c - 1)) * dxi2 + (x(c + L.sy) - 2. * xc + x(c - L.sy)) * dyi2 +
+                     ((x(c + L.sz) - xc) * dzci[k] - (xc - x(c - L.sz)) * dzci[k - 1]) * dzfi[k];
+  p[c] = (S)(b - lap);
+}
+
+template <typename S>
+int launch_fillps_residual(cans_ctx* c, const double* u, const double* v, const double* w, const S* eu,
+                           const S* ev, const S* ew, const double* f, S* p, const double* dzfi, double dti,
+                           const double* dli, const S* c0, const S* c1, const double* dzci) {
+  const Layout& L = c->L;
+  k_fillps_residual<S><<<row_grid(L, 256), 256, 0, c->stream>>>(L, u, v, w, eu, ev, ew, f, p, dzfi, dti,
+                                                                  dti * dli[0], dti * dli[1], c0, c1, dzci,
+                                                                  dli[0] * dli[0], dli[1] * dli[1]);
+  CANS_LAUNCH_CHECK("k_fillps_residual");
   return CANS_OK;
 }
 
@@ -993,7 +1047,10 @@ int launch_accumulate(cans_ctx* c, double* x, const S* d, int first) {
                                 const S*, const double*, S*, const double*, double, const double*);             \
   template int launch_correc<S>(cans_ctx*, const S*, const double*, const double*, const double*, const S*,     \
                                 const S*, const S*, const double*, double*, double*, double*, const double*,    \
-                                double, const double*, S*);                                                      \
+                                double, const double*, S*, const S*, const S*);                                  \
+  template int launch_fillps_residual<S>(cans_ctx*, const double*, const double*, const double*, const S*,       \
+                                         const S*, const S*, const double*, S*, const double*, double,           \
+                                         const double*, const S*, const S*, const double*);                      \
   template int launch_updatep<S>(cans_ctx*, const S*, S*, const double*, const double*, double, const double*, int); \
   template int launch_bound_dir<S>(cans_ctx*, const PlaneOps&);                                                  \
   template int launch_rhs_b<S>(cans_ctx*, S*, const double*, int, int, double);                               \

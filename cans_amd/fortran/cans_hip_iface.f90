@@ -118,6 +118,12 @@ module cans_hip
       type(c_ptr), value :: ctx
       integer(c_int), value :: nslab
     end function
+    ! FP64 refinement sweeps of the mixed fused substep's pressure solve (substep_mask: bit irk-1)
+    integer(c_int) function cans_ctx_set_refinement(ctx, sweeps, substep_mask) bind(C, name='cans_ctx_set_refinement')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: sweeps, substep_mask
+    end function
     integer(c_int) function cans_field_register(ctx, field, kind) bind(C, name='cans_field_register')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, field

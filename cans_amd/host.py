@@ -463,6 +463,12 @@ class Simulation:
         st.dudtrko, st.dvdtrko, st.dwdtrko = (t.data_ptr() for t in self.rhs_o)
         self.st = st
 
+    def set_refinement(self, sweeps, substep_mask=4):
+        """cans_ctx_set_refinement: FP64 refinement sweeps of the fused mixed substep's pressure
+        solve on the substeps of substep_mask (bit irk-1; 4 = the last substep of each step)."""
+        LB.check(LB.lib().cans_ctx_set_refinement(self.ctx.h, int(sweeps), int(substep_mask)),
+                 "cans_ctx_set_refinement")
+
     # ---- state ----
     def set_state(self, u=None, v=None, w=None, p=None, s=()):
         for f, a in ((self.u, u), (self.v, v), (self.w, w), (self.p, p)) + tuple(zip(self.s, s)):

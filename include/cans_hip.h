@@ -86,6 +86,17 @@ int cans_ctx_set_numerics(cans_ctx *ctx, int is_impdiff, int is_impdiff_1d);
  * least 3 rows).  0 or 1: the plain sweep.  Non-periodic z only; like the reference's
  * gaussel_dtdma it has no zero-mode pin. */
 int cans_ctx_set_dtdma(cans_ctx *ctx, int nslab);
+/* Iterative refinement of the pressure solve inside cans_rk3_substep (no counterpart in the
+ * reference, whose solves run in one precision; CANS_MIXED contexts only).  Each substep irk
+ * whose bit (irk-1) is set in substep_mask follows its FP32-storage solve with `sweeps` (0-2)
+ * sweeps: the residual r = b - L(pp + x_1 + ...) of the pressure equation is formed in FP64
+ * from the intermediate velocity (b, the rhs of fillps) and the corrections so far, L the
+ * solver's 7-point operator, and solved again with FP32 storage into x_q; correc and the
+ * pressure update then use pp + x_1 + x_2 summed in FP64 (pp itself keeps the first solve's
+ * part).  After the substep div(u) = dtrk * (b - L(pp + x_1 + x_2)), the FP64 residual, so a
+ * refined last substep (mask 4) leaves every step's velocity divergence-free to FP64 level.
+ * Homogeneous pressure boundary values only.  Default: 0 sweeps (the unrefined mixed step). */
+int cans_ctx_set_refinement(cans_ctx *ctx, int sweeps, int substep_mask);
 /* host arrays are the reference's contiguous (0:n1+1,0:n2+1,0:n3+1) Fortran arrays
  * of double; float fields round on upload.  In a CANS_MIXED context the field
  * must come from cans_field_alloc_kind (its element type is looked up) */
@@ -121,8 +132,11 @@ int cans_tridmatrix(const char *bc, int n, const double *dzci_h, const double *d
  * four batched transforms ((fwd,bwd) x (x,y)); normfft is returned as the
  * reference computes it (fft.f90:75,140,202,208).  Device transform lengths: 2^m, or
  * 2^m (m >= 3) times at most three factors 3 and 5 (e.g. 24, 40, 72, 120), at most 4096.
- * Boundary pairs along x and y: PP (also face-centred), and cell-centred NN, DD, ND and DN
- * (fft.f90:260-287; ND/DN run on a line of twice the length, which must be such a length).
+ * Boundary pairs along x and y (fft.f90:260-313): PP; cell-centred NN, DD, ND and DN (ND/DN
+ * run on a line of twice the length, which must be such a length); face-centred ND and DN
+ * (REDFT10/01, RODFT01/10) and DD (RODFT00 on the n-1 interior faces, fft.f90:80-82, through a
+ * line of 2n points).  Face-centred NN (REDFT00) is refused: the reference forbids it with
+ * implicit diffusion (sanity.f90:151-162), the only user of face-centred transforms.
  * ------------------------------------------------------------------------- */
 int cans_fftini(cans_ctx *ctx, const int *ng, const char *bcxy, const char *c_or_f,
                 cans_plan **arrplan, double *normfft);

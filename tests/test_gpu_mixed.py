@@ -112,3 +112,52 @@ def test_mixed_steps(torch, name, ng, dt, nsteps, fused):
     scale = vmax[0] * g.dli[0] + vmax[1] * g.dli[1] + vmax[2] * g.dzfi.max()
     print("divmax mixed %.3e oracle %.3e (relative to the gradient scale %.3e)" % (mx, omx, mx / scale), flush=True)
     assert mx <= 1e-6 * scale
+
+
+# FP64 refinement of the mixed pressure solve (cans_ctx_set_refinement): every step's velocity
+# must come out divergence-free to the reference's FP64 abort threshold `small` (param.f90:18,
+# main.f90:501-510), not only to FP32 level.  mask 4 refines the last substep of each step,
+# whose projection alone sets the divergence the step leaves; mask 7 refines every substep.
+SMALL = 2.220446049250313e-9
+REFINED = [("channel", (64, 32, 48), 2e-4, 5, 2, 4), ("channel", (64, 32, 48), 2e-4, 5, 1, 7),
+           ("tgv", (32, 32, 32), 1e-3, 5, 2, 7), ("duct", (32, 16, 24), 1e-3, 4, 2, 4),
+           ("box", (16, 16, 12), 1e-3, 4, 2, 4), ("open_nd", (16, 24, 12), 1e-3, 4, 2, 7)]
+
+
+@pytest.mark.parametrize("name,ng,dt,nsteps,sweeps,mask", REFINED,
+                         ids=["%s-%dx%dx%d-s%d-m%d" % ((c[0],) + c[1] + c[4:]) for c in REFINED])
+def test_mixed_refined_steps(torch, name, ng, dt, nsteps, sweeps, mask):
+    p, sim, orc, st = mixed_pair(name, ng)
+    sim.set_refinement(sweeps, mask)
+    sim.init_bounds()
+    orc.init_bounds(st)
+    g = sim.grid
+    for it in range(nsteps):
+        sim.rk3_step(dt)
+        orc.rk3_step(st, dt)
+        tot, mx = sim.chkdiv()
+        vmax = [np.abs(getattr(sim, k).interior()).max() for k in "uvw"]
+        scale = vmax[0] * g.dli[0] + vmax[1] * g.dli[1] + vmax[2] * g.dzfi.max()
+        print("step %d divmax %.3e (%.2e of the gradient scale)" % (it + 1, mx, mx / scale), flush=True)
+        assert mx < SMALL
+        assert mx <= 1e-12 * scale
+    for k in "uvw":
+        err = relerr(getattr(sim, k).interior(), st[k][1:-1, 1:-1, 1:-1])
+        print("%s rel err %.3e" % (k, err), flush=True)
+        assert err < MIXED_TOL, (k, err)
+    singular = "D" not in p["cbcpre"]
+    dzf = 1 / g.dzfi[1:-1]
+    err = relerr(gauge_fixed(sim.p.interior(), dzf, singular), gauge_fixed(st["p"][1:-1, 1:-1, 1:-1], dzf, singular))
+    print("p rel err (gauge-fixed) %.3e" % err, flush=True)
+    assert err < MIXED_TOL, ("p", err)
+
+
+def test_refinement_setting_refused_outside_mixed(torch):
+    from cans_amd import _lib as LB
+    from cans_amd.host import Simulation
+    p = cases.params("tgv", ng=(16, 16, 16))
+    sim = Simulation(cases.host_case(p))
+    with pytest.raises(LB.CansError):
+        sim.set_refinement(2, 4)
+    sim.set_refinement(0, 0)
+    sim.close()
