@@ -49,6 +49,11 @@ def cpu_baseline(p, init, budget_s=20.0):
     import numpy as np
     from tests import cases
     from oracle import oracle as O
+    try:  # built with -march=native on this host, as the reference's CPU build would be
+        march = O.use_native()
+    except Exception as e:  # noqa: BLE001 - fall back to the portable build, and say so
+        print("[cpu] native oracle build failed (%s); portable x86-64-v3 build" % e, file=sys.stderr, flush=True)
+        march = "x86-64-v3"
     orc = cases.oracle_case(p)
     st = orc.new_state()
     for k, q in zip("uvwp", init):
@@ -66,23 +71,35 @@ def cpu_baseline(p, init, budget_s=20.0):
             break
     n = int(np.prod(p["ng"]))
     med = float(np.median(times))
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next(ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
     return {"value": med / n * 1e9, "unit": "ns/cell/step", "cores": O.lib().orc_num_threads(),
             "kind": "port",
             "sample": "%d full RK3 steps of the %dx%dx%d channel (median), oracle C/OpenMP, FP64"
-                      % (len(times), *p["ng"])}
+                      % (len(times), *p["ng"]),
+            "cpu_model": model, "nproc": len(os.sched_getaffinity(0)), "os_cpu_count": os.cpu_count(),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "march": march,
+            "min_s": float(np.min(times)), "max_s": float(np.max(times)), "avg_s": float(np.mean(times))}
 
 
-def run(p, init, precision, args):
-    """Time K steps and S Poisson solves in one precision mode; return figures and final fields."""
+def run(p, init, precision, args, sweeps=0, solves=True):
+    """Time K steps (and, with solves, S Poisson solves) in one precision mode; return figures and
+    final fields.  sweeps: FP64 refinement sweeps of the last substep's pressure solve (mixed mode)."""
     import numpy as np
     import torch
     from cans_amd import _lib as LB
     from cans_amd.host import Simulation
     from cans_amd import initflow as IF
     from tests import cases
-    label = "mixed" if precision == LB.CANS_MIXED else "fp64"
+    label = ("mixed, %d refinement sweeps" % sweeps) if precision == LB.CANS_MIXED else "fp64"
     print("[bench] %s: set-up" % label, file=sys.stderr, flush=True)
     sim = Simulation(cases.host_case(p), precision=precision)
+    if sweeps:
+        sim.set_refinement(sweeps, 4)  # the last substep of each step, whose projection sets div(u)
     sim.set_state(*(IF.halo(p["ng"], q) for q in init))
     sim.init_bounds()
     ncell = int(np.prod(p["ng"]))
@@ -102,6 +119,15 @@ def run(p, init, precision, args):
     print("[bench] %s: %.3f ms/step" % (label, ms_step), file=sys.stderr, flush=True)
     final = {k: getattr(sim, k).interior() for k in "uvwp"}
     tot, divmax = sim.chkdiv()
+    res = {"ms_per_step": ms_step, "ns_per_cell_step": ms_step * 1e-3 / ncell * 1e9, "wall_s": wall,
+           "divmax": divmax}
+    if precision == LB.CANS_MIXED:
+        res["refinement_sweeps"] = sweeps
+    if not solves:
+        sim.close()
+        del sim
+        torch.cuda.empty_cache()
+        return res, final
     # Poisson solve alone (same stream), repeated on the last right-hand side
     s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     sim.solver()
@@ -139,10 +165,8 @@ def run(p, init, precision, args):
     sim.close()
     del sim
     torch.cuda.empty_cache()
-    res = {"ms_per_step": ms_step, "ns_per_cell_step": ms_step * 1e-3 / ncell * 1e9, "wall_s": wall,
-           "divmax": divmax,
-           "poisson": {"ms": t_p * 1e3, "GBps": gbs, "pct_hbm_peak": 100.0 * gbs / HBM_PEAK_GBS,
-                       "algorithmic_bytes": b_p}}
+    res["poisson"] = {"ms": t_p * 1e3, "GBps": gbs, "pct_hbm_peak": 100.0 * gbs / HBM_PEAK_GBS,
+                      "algorithmic_bytes": b_p}
     if refined:
         res["poisson_refined"] = refined
     return res, final
@@ -348,15 +372,20 @@ def main():
     grid = Grid(cases.host_case(p))
     # one seeded state, FP32-representable so that both modes start from the same values
     init = [q.astype(np.float32).astype(np.float64) for q in cases.initial_state("channel", p, grid, seed=1)]
-    modes = {"both": ("mixed", "fp64"), "mixed": ("mixed",), "fp64": ("fp64",)}[args.precision]
+    # mixed: config 2 as stated, two FP64 refinement sweeps of the last substep's pressure solve;
+    # mixed_plain: the same step with a single FP32-storage solve (a labelled variant)
+    modes = {"both": ("mixed", "mixed_plain", "fp64"), "mixed": ("mixed", "mixed_plain"),
+             "fp64": ("fp64",)}[args.precision]
     res, final = {}, {}
     for m in modes:
-        res[m], final[m] = run(p, init, LB.CANS_MIXED if m == "mixed" else LB.CANS_FP64, args)
+        res[m], final[m] = run(p, init, LB.CANS_FP64 if m == "fp64" else LB.CANS_MIXED, args,
+                               sweeps=2 if m == "mixed" else 0, solves=m != "mixed_plain")
     head = res[modes[0]]
     head_mode = modes[0]
     wl = ("periodic channel %dx%dx%d, stretched z (gr=1.8), Re_b=5600, dt=2e-4; %s"
-          % (tuple(p["ng"]) + ("mixed precision (FP32 pressure-side storage and FFTs, FP64 velocity and stencils) vs FP64"
-                               if len(modes) == 2 else head_mode,)))
+          % (tuple(p["ng"]) + ("mixed precision (FP32 pressure-side storage and FFTs, FP64 velocity, stencils, "
+                               "z elimination and divergence, 2 FP64 refinement sweeps of the last substep's "
+                               "pressure solve) vs FP64" if head_mode == "mixed" else head_mode,)))
     pois = head["poisson"]
     out = {
         "metric": METRIC,
@@ -385,18 +414,21 @@ def main():
     }
     for m in modes:
         out[m] = res[m]
-    if len(modes) == 2:
+    if "fp64" in modes and "mixed" in modes:
         out["speedup_mixed_vs_fp64"] = res["fp64"]["ms_per_step"] / res["mixed"]["ms_per_step"]
-        err = {}
-        a, b = final["mixed"], final["fp64"]
-        for k in "uvw":
-            err[k] = float(np.abs(a[k] - b[k]).max() / np.abs(b[k]).max())
-        wgt = np.broadcast_to(1 / grid.dzfi[1:-1], a["p"].shape)
-        da = a["p"] - (a["p"] * wgt).sum() / wgt.sum()
+        out["speedup_mixed_plain_vs_fp64"] = res["fp64"]["ms_per_step"] / res["mixed_plain"]["ms_per_step"]
+        b = final["fp64"]
+        wgt = np.broadcast_to(1 / grid.dzfi[1:-1], b["p"].shape)
         db = b["p"] - (b["p"] * wgt).sum() / wgt.sum()
-        err["p_gauge_fixed"] = float(np.abs(da - db).max() / np.abs(db).max())
-        err["steps"] = args.warmup + args.steps
-        out["mixed_vs_fp64_rel_linf"] = err
+        for m in ("mixed", "mixed_plain"):
+            err = {}
+            a = final[m]
+            for k in "uvw":
+                err[k] = float(np.abs(a[k] - b[k]).max() / np.abs(b[k]).max())
+            da = a["p"] - (a["p"] * wgt).sum() / wgt.sum()
+            err["p_gauge_fixed"] = float(np.abs(da - db).max() / np.abs(db).max())
+            err["steps"] = args.warmup + args.steps
+            out[m + "_vs_fp64_rel_linf"] = err
     del final
     out["copy_GBps"] = copy_bandwidth()
     if not args.no_secondary:

@@ -34,6 +34,17 @@ def build():
     subprocess.check_call(["make", "-s", "-C", _HERE])
 
 
+def use_native():
+    """Build the oracle with -march=native for this host and load that build (the CPU
+    baseline's build; call before the first lib()).  Returns the compiler flags' -march."""
+    global _LIBPATH
+    if _lib is not None:
+        raise RuntimeError("use_native() must come before the oracle is loaded")
+    subprocess.check_call(["make", "-s", "-B", "-C", _HERE, "liboracle_native.so"])
+    _LIBPATH = os.path.join(_HERE, "liboracle_native.so")
+    return "native"
+
+
 def lib():
     global _lib
     if _lib is None:
