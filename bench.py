@@ -16,7 +16,7 @@ with FP64 storage.  The oracle (C/OpenMP restatement) is timed on the host on a
 bounded number of full steps of the same case as the CPU baseline.
 
 Secondary workloads, timed after the headline one and reported under "secondary":
-the 256^3 heated cube with a scalar, buoyancy and implicit z diffusion (FP64),
+the 256^3 heated cube with a scalar, buoyancy and 3-D implicit diffusion (FP64 and mixed),
 64 independent Poisson boxes of different sizes solved as one batch (FP64 and mixed
 storage, beside the same boxes solved one at a time), and the gradient of the kinetic
 energy after 32 steps of the 256^3 Taylor-Green flow (forward and adjoint, FP64).
@@ -172,10 +172,12 @@ def run(p, init, precision, args, sweeps=0, solves=True):
     return res, final
 
 
-def secondary_cavity(args):
-    """A second workload beside the headline: the 256^3 heated cube (cases.params("dhc3d")): scalar
-    transport with buoyancy, implicit z diffusion (four z-only Helmholtz solves per substep: u, v, w and
-    the scalar) and a DCT x DCT pressure solve, FP64, fixed dt."""
+def secondary_cavity(args, precision="fp64"):
+    """A second workload beside the headline: the 256^3 heated cube (cases.params("dhc3d")) with no-slip
+    walls on all faces, a scalar with buoyancy and 3-D implicit diffusion (is_impdiff: per substep the
+    scalar's Helmholtz system, then the three velocity systems, whose transforms along each component's
+    wall-normal direction are face-centred DST-I), a DCT x DCT pressure solve, fixed dt.  FP64, or the
+    mixed mode (FP32 pressure side; the velocity and its Helmholtz solves FP64)."""
     import numpy as np
     import torch
     from cans_amd import _lib as LB
@@ -183,8 +185,10 @@ def secondary_cavity(args):
     from cans_amd.host import Simulation
     from tests import cases
     p = cases.params("dhc3d")
-    print("[bench] heated cube %s: set-up" % (p["ng"],), file=sys.stderr, flush=True)
-    sim = Simulation(cases.host_case(p), precision=LB.CANS_FP64)
+    p["is_impdiff_1d"] = False
+    p["is_impdiff"] = True
+    print("[bench] heated cube %s, %s: set-up" % (p["ng"], precision), file=sys.stderr, flush=True)
+    sim = Simulation(cases.host_case(p), precision=LB.CANS_MIXED if precision == "mixed" else LB.CANS_FP64)
     u, v, w, pr = cases.initial_state("dhc3d", p, sim.grid, seed=3)
     sc = cases.initial_scalars("dhc3d", p, sim.grid)
     sim.set_state(*(IF.halo(p["ng"], q) for q in (u, v, w, pr)), s=[IF.halo(p["ng"], q) for q in sc])
@@ -206,9 +210,10 @@ def secondary_cavity(args):
     sim.close()
     del sim
     torch.cuda.empty_cache()
-    print("[bench] heated cube: %.3f ms/step" % ms, file=sys.stderr, flush=True)
-    return {"workload": "heated cube %dx%dx%d, no-slip walls, scalar with buoyancy, implicit z diffusion "
-                        "(is_impdiff_1d), DCT x DCT pressure, dt=1e-3, FP64" % tuple(p["ng"]),
+    print("[bench] heated cube %s: %.3f ms/step" % (precision, ms), file=sys.stderr, flush=True)
+    return {"workload": "heated cube %dx%dx%d, no-slip walls, scalar with buoyancy, 3-D implicit diffusion "
+                        "(is_impdiff; velocity Helmholtz with face-centred DST-I along the wall normal), "
+                        "DCT x DCT pressure, dt=1e-3, %s" % (tuple(p["ng"]) + (precision,)),
             "ms_per_step": ms, "ns_per_cell_step": ms * 1e-3 / ncell * 1e9, "steps": nst, "divmax": divmax}
 
 
@@ -432,7 +437,8 @@ def main():
     del final
     out["copy_GBps"] = copy_bandwidth()
     if not args.no_secondary:
-        out["secondary"] = [secondary_cavity(args), secondary_boxes(args), secondary_adjoint(args)]
+        out["secondary"] = [secondary_cavity(args), secondary_cavity(args, "mixed"), secondary_boxes(args),
+                            secondary_adjoint(args)]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, init)
     print(json.dumps(out))

@@ -488,9 +488,13 @@ static int helmholtz(cans_ctx* c, int n, const cans_helmholtz* const* h, const i
 // The intermediate velocity is then u + e + f, with the bulk forcing f kept as a
 // device scalar (no forcing pass), and the momentum kernel also leaves div(u), so
 // the first transform pass forms div(u + e + f) = div(u) + div(e) without reading u.
+// With implicit diffusion the mixed mode keeps the FP64 form of the velocity (the forced
+// intermediate velocity is the rhs of its Helmholtz systems, solved in FP64 on the FP64
+// velocity fields); the pressure side (p, pp, RHS_old, the Poisson solve) stays FP32.
 template <typename S>
 static int rk3_substep(cans_ctx* c, const cans_step_params* s, int irk, double dt, cans_state* st) {
-  const bool delta = sizeof(S) != sizeof(double);
+  const bool mixed = sizeof(S) != sizeof(double);
+  const bool delta = mixed && !c->imp;
   const double* rkpar = RKCOEFF[irk - 1];
   const double dtrk = (rkpar[0] + rkpar[1]) * dt;  // main.f90:438-439
   const double dtrki = 1. / dtrk;
@@ -519,11 +523,9 @@ static int rk3_substep(cans_ctx* c, const cans_step_params* s, int irk, double d
   void* divu_v = nullptr;
   if (delta && fuse_div && (rc = ensure_work(c, 4, (size_t)c->L.cells() * sizeof(S), &divu_v))) return rc;
   S* divu = (S*)divu_v;
-  if (c->imp && delta)
-    return fail(CANS_EUNSUPPORTED, "cans_rk3_substep: implicit diffusion needs a CANS_FP64 context");
   // refinement sweeps of this substep's pressure solve (cans_ctx_set_refinement; mixed mode):
   // their corrections live in two halo'd fields of S, each field's element 1 256-byte aligned
-  const int nsweep = (delta && ((c->refine_mask >> (irk - 1)) & 1)) ? c->refine_sweeps : 0;
+  const int nsweep = (mixed && ((c->refine_mask >> (irk - 1)) & 1)) ? c->refine_sweeps : 0;
   S* corr[2] = {nullptr, nullptr};
   if (nsweep) {
     if (s->rhsbx_d || s->rhsby_d || s->rhsbz_d)

@@ -159,7 +159,9 @@ __host__ __device__ inline long wbuf_elems(int n1, int n2, int n3) {
   return (long)((n3 + CK + ZK - 1) / ZK) * wplane(n1, n2);
 }
 
-// element e of one row: a contiguous field row, or row (j,k) of the tiled work buffer
+// element e of one row: a contiguous field row, or row (j,k) of the tiled work buffer.
+// `r + e` maps the whole element index e through the tiling and returns a plain pointer, so
+// an index expression must be parenthesised: r + (a - b), never r + a - b.
 template <typename T>
 struct RowPtr {
   T* p = nullptr;
@@ -428,8 +430,8 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& a, int bx, int sys) {
         split_dct(widen(smem[ix(k)]), widen(smem[ix(k ? N - k : 0)]), a.mk[k], y0a, y0b);
         split_dct(widen(smem[ix(k + 1)]), widen(smem[ix(N - k - 1)]), a.mk[k + 1], y1a, y1b);
         if (kind == K_DD) {  // output reversed: element N-1-k
-          st2(ya + N - 2 - k, y1a, y0a);
-          if (yb) st2(yb + N - 2 - k, y1b, y0b);
+          st2(ya + (N - 2 - k), y1a, y0a);
+          if (yb) st2(yb + (N - 2 - k), y1b, y0b);
         } else {
           st2(ya + k, y0a, y1a);
           if (yb) st2(yb + k, y0b, y1b);

@@ -77,8 +77,9 @@ int cans_field_register(cans_ctx *ctx, void *field_d, int kind);
  * is_impdiff (implicit diffusion, solve_helmholtz of u, v, w and the scalars) and
  * is_impdiff_1d (implicit in z only, which implies is_impdiff).  They change cans_rk,
  * cans_rk_scal, cans_updatep, cans_solve_helmholtz and cans_rk3_substep as in the
- * reference.  Implicit diffusion runs with FP64 velocity and scalar fields; the fused
- * substep supports it in CANS_FP64 contexts. */
+ * reference.  Implicit diffusion runs with FP64 velocity and scalar fields and FP64
+ * Helmholtz solves; the fused substep supports it in both modes (under CANS_MIXED the
+ * pressure side -- p, pp, RHS_old and the Poisson solve -- stays FP32). */
 int cans_ctx_set_numerics(cans_ctx *ctx, int is_impdiff, int is_impdiff_1d);
 /* is_poisson_dtdma (param.f90, solver.f90:309-517): the z elimination of the pressure
  * solve (and of the Helmholtz solves' 3-D form) as the distributed tridiagonal algorithm

@@ -32,6 +32,9 @@ def params(name, ng=None):
         # (REDFT11), DN in y (RODFT11), walls in z
         p = dict(ng=(16, 24, 12), l=(1.0, 1.0, 1.0), gtype=1, gr=0.5, cbcvel="DNNDDD" * 3,
                  cbcpre="NDDNNN", visci=100.0)
+    elif name == "open_x":  # outflow faces at both x ends (velocity N, pressure D: DST-II along x), walls in z
+        p = dict(ng=(64, 16, 12), l=(1.0, 1.0, 1.0), gtype=1, gr=0.5, cbcvel="NNPPDD" * 3,
+                 cbcpre="DDPPNN", visci=100.0)
     elif name == "cavity_dd":  # Dirichlet pressure walls in y (DST), Neumann in z
         p = dict(ng=(32, 32, 24), l=(1.0, 1.0, 1.0), gtype=1, gr=0.5, cbcvel="PPNNDD" * 3,
                  cbcpre="PPDDNN", visci=100.0)

@@ -55,7 +55,7 @@ def mixed_pair(name, ng, seed=7):
 
 SOLVER_CASES = [("tgv", (32, 32, 32)), ("channel", (64, 32, 48)), ("duct", (16, 32, 20)),
                 ("cavity_dd", (16, 32, 24)), ("channel", (128, 64, 96)), ("channel", (96, 48, 40)),
-                ("channel", (160, 120, 40)), ("open_nd", (64, 32, 20))]
+                ("channel", (160, 120, 40)), ("open_nd", (64, 32, 20)), ("open_x", (128, 16, 12))]
 
 
 @pytest.mark.parametrize("name,ng", SOLVER_CASES, ids=["%s-%dx%dx%d" % ((c,) + g) for c, g in SOLVER_CASES])

@@ -53,7 +53,10 @@ SOLVER_CASES = [("tgv", (16, 16, 16)), ("tgv", (32, 8, 12)), ("channel", (32, 16
                 # radix-5 and mixed odd factors (5 * 2^m, 15 * 2^m, 9 * 2^m, 25 * 2^m)
                 ("channel", (40, 120, 24)), ("cavity_dd", (72, 80, 16)), ("box", (200, 40, 12)),
                 # REDFT11 along x and RODFT11 along y (pressure ND, DN)
-                ("open_nd", (16, 24, 12)), ("open_nd", (64, 32, 20)), ("open_nd", (40, 12, 10))]
+                ("open_nd", (16, 24, 12)), ("open_nd", (64, 32, 20)), ("open_nd", (40, 12, 10)),
+                # DST-II/III along x on lines of 64 elements and more (more than one tile chunk of the
+                # work buffer)
+                ("open_x", (64, 16, 12)), ("open_x", (96, 24, 10)), ("open_x", (256, 8, 8))]
 
 
 @pytest.mark.parametrize("name,ng", SOLVER_CASES, ids=["%s-%dx%dx%d" % ((c,) + g) for c, g in SOLVER_CASES])
