@@ -91,6 +91,7 @@ SIGNATURES = {
     "cans_fftend": (C.c_int, [_vp]),
     "cans_fft_tuning": (C.c_int, [_vp, _ip]),
     "cans_solver": (C.c_int, [_vp, _ip, _ip, _vp, C.c_double, _vp, _vp, _vp, _vp, _cp, _cp, _vp]),
+    "cans_solver_noalloc": (C.c_int, [_vp, _ip, _ip, _vp, C.c_double, _vp, _vp, _vp, _vp, _cp, _cp, _vp]),
     "cans_poisson_batch_create": (C.c_int, [_vp, C.c_int, C.POINTER(PoissonBox), _dp, C.POINTER(_vp)]),
     "cans_poisson_batch_solve": (C.c_int, [_vp]),
     "cans_poisson_batch_tuning": (C.c_int, [_vp, _ip]),

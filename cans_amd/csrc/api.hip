@@ -392,6 +392,30 @@ extern "C" int cans_solver(cans_ctx* c, const int* n, const int* ng, const cans_
   return CANS_DISPATCH(c, run_solver<T>(c, plan, normfft, lam, a, b, cc, bc, c_or_f, (T*)p));
 }
 
+// cans_solver on the fixed device scratch: the same passes, their work buffer and
+// checkpoints carved from this device's array; no allocation on any path of the call
+extern "C" int cans_solver_noalloc(cans_ctx* c, const int* n, const int* ng, const cans_plan* plan, double normfft,
+                                   const double* lam, const double* a, const double* b, const double* cc,
+                                   const char* bc, const char* c_or_f, void* p) {
+  CHECK_CTX("cans_solver_noalloc");
+  CHECK_N("cans_solver_noalloc", n);
+  CHECK_N("cans_solver_noalloc", ng);
+  if (!plan || !lam || !a || !b || !cc || !bc || !c_or_f || !p)
+    return fail(CANS_EINVAL, "cans_solver_noalloc: null argument");
+  if (c->dtdma > 1) return fail(CANS_EUNSUPPORTED, "cans_solver_noalloc: the distributed z solve is not covered");
+  const bool per = bc[4] == 'P' && bc[5] == 'P';
+  const size_t es = c->precision == CANS_MIXED ? sizeof(float) : sizeof(double);
+  const size_t need = solver_work_bytes(c, es, per);
+  if (need > (size_t)CANS_NOALLOC_SCRATCH_BYTES)
+    return fail(CANS_EINVAL, "cans_solver_noalloc: this grid's solve needs " + std::to_string(need) +
+                                 " bytes of scratch, above the fixed " +
+                                 std::to_string((long)CANS_NOALLOC_SCRATCH_BYTES));
+  void* base = noalloc_scratch(c->device);
+  if (!base) return fail(CANS_EHIP, "cans_solver_noalloc: no scratch address for the context's device");
+  const SolveWork ws{base, (size_t)CANS_NOALLOC_SCRATCH_BYTES};
+  return CANS_DISPATCH(c, run_solver<T>(c, plan, normfft, lam, a, b, cc, bc, c_or_f, (T*)p, nullptr, &ws));
+}
+
 extern "C" int cans_solve_helmholtz(cans_ctx* c, const int* n, const int* ng, const cans_plan* plan, double normfft,
                                     double alpha, const double* lam, const double* a, const double* b,
                                     const double* cc, const double* rx, const double* ry, const double* rz,

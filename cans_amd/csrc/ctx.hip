@@ -16,6 +16,36 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
+// Fixed device scratch of the allocation-free solve (cans_solver_noalloc): the transform
+// work buffer and the z pass's checkpoints of one solve.  Every GPU the library runs on
+// holds its own copy, at its own address, until the process ends.  Host code never
+// passes the array's name to a kernel (on the host it only names a stand-in): the
+// device address is looked up once per device, with that device current, when the
+// first context on it is created, and kept in g_scratch_ptr.
+static __device__ __align__(256) unsigned char scratch[CANS_NOALLOC_SCRATCH_BYTES];
+constexpr int MAX_DEVICES = 64;
+static void* g_scratch_ptr[MAX_DEVICES] = {};
+static std::mutex g_scratch_mu;
+
+// device address of this device's scratch (the device must be current)
+static int scratch_setup(int device) {
+  if (device < 0 || device >= MAX_DEVICES) return fail(CANS_EUNSUPPORTED, "cans_ctx_create: device id above 63");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return CANS_OK;
+  void* ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(CANS_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[device] = ptr;
+  return CANS_OK;
+}
+
+void* noalloc_scratch(int device) {
+  if (device < 0 || device >= MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
+
 // origin pointer -> (base pointer, element size) of every field allocated through a context
 struct FieldRec {
   void* base;
@@ -55,6 +85,10 @@ extern "C" int cans_ctx_create(const int* n, int ld, int precision, void* stream
   if (e != hipSuccess) {
     delete c;
     return fail(CANS_EHIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  }
+  if (int rc = scratch_setup(c->device)) {  // c->device is the current device
+    delete c;
+    return rc;
   }
   e = hipMalloc((void**)&c->dscal, 64 * sizeof(double));
   if (e == hipSuccess) e = hipMemsetAsync(c->dscal, 0, 64 * sizeof(double), c->stream);

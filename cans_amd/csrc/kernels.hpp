@@ -229,6 +229,9 @@ struct SolveWork {
 };
 // bytes of workspace one solve of the context's grid needs with element size es
 size_t solver_work_bytes(const cans_ctx* c, size_t es, bool periodic_z);
+// device address of the fixed scratch (CANS_NOALLOC_SCRATCH_BYTES) on a device that has a
+// context, or nullptr
+void* noalloc_scratch(int device);
 template <typename S>
 int run_solver(cans_ctx* c, const cans_plan* plan, double normfft, const double* lam, const double* a,
                const double* b, const double* cc, const char* bc, const char* c_or_f, S* p,

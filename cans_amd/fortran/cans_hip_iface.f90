@@ -211,6 +211,15 @@ module cans_hip
       real(c_double), value :: normfft
       character(kind=c_char), intent(in) :: bc(6), c_or_f(3)
     end function
+    ! cans_solver on a fixed device scratch: no allocation in the call (for callers that cannot allocate)
+    integer(c_int) function cans_solver_noalloc(ctx, n, ng, arrplan, normfft, lambdaxy, a, b, c, bc, c_or_f, p) &
+                   bind(C, name='cans_solver_noalloc')
+      import :: c_int, c_ptr, c_double, c_char
+      type(c_ptr), value :: ctx, arrplan, lambdaxy, a, b, c, p
+      integer(c_int), intent(in) :: n(3), ng(3)
+      real(c_double), value :: normfft
+      character(kind=c_char), intent(in) :: bc(6), c_or_f(3)
+    end function
     ! many independent Poisson problems of different sizes, five launches per solve for all of them
     integer(c_int) function cans_poisson_batch_create(ctx, nbox, boxes, normfft, batch) &
                    bind(C, name='cans_poisson_batch_create')

@@ -156,6 +156,20 @@ int cans_fft_tuning(const cans_plan *arrplan, int *shapes);
 int cans_solver(cans_ctx *ctx, const int *n, const int *ng, const cans_plan *arrplan,
                 double normfft, const double *lambdaxy_d, const double *a_d, const double *b_d,
                 const double *c_d, const char *bc, const char *c_or_f, void *p_d);
+/* No counterpart in the reference.  cans_solver with the same arguments and results, for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): the transform work buffer and the z
+ * pass's checkpoints come from a fixed device array of CANS_NOALLOC_SCRATCH_BYTES that
+ * the library holds on every GPU it runs on until the process ends, and the call makes
+ * no allocation (cans_solver grows the context's scratch on its first call).  A grid
+ * whose solve needs more scratch is refused with CANS_EINVAL (64 MiB covers, e.g., a
+ * 128^3 tri-periodic FP64 solve, 38.8 MB).  The distributed z solve (cans_ctx_set_dtdma)
+ * is refused.  All calls on one GPU share its array, so calls that overlap on one GPU
+ * (two streams, or two host threads) race; calls ordered on one stream do not. */
+#define CANS_NOALLOC_SCRATCH_BYTES (64L << 20)
+int cans_solver_noalloc(cans_ctx *ctx, const int *n, const int *ng, const cans_plan *arrplan,
+                        double normfft, const double *lambdaxy_d, const double *a_d, const double *b_d,
+                        const double *c_d, const char *bc, const char *c_or_f, void *p_d);
 /* Many independent Poisson problems of different sizes, solved together: each box
  * carries the arguments of one src/solver.f90:17 call, and cans_poisson_batch_create
  * does each box's fftini (src/fft.f90:25) as well.  A solve is five launches for
