@@ -7,8 +7,12 @@ twice from the same state: in the mixed mode (velocity stored as FP64; p, pp,
 RHS_old, the RK increment and the transform passes stored as FP32; FFT stages
 computing in FP32, every other kernel in FP64) and in plain FP64.  A step is one full RK3 step (3
 substeps) through the fused C-ABI call; inputs are resident in HBM.  `value` is
-the mixed mode's ns/cell/step; the FP64 figures, the speed-up and the relative
-L-inf difference of the final fields (mixed vs FP64) are reported beside it.
+the mixed mode's ns/cell/step, with two FP64 refinement sweeps of the last
+substep's pressure solve on every 10th step (the steps whose divergence the
+reference checks, icheck = 10; the last timed step is one of them, so the
+reported divmax is that check's).  Beside it: the mixed step refined on every
+step, the plain mixed step (no refinement), the FP64 step, the speed-ups and the
+relative L-inf difference of the final fields (mixed vs FP64).
 
 The Poisson solve is timed on its own with HIP events on the context stream:
 B_P = 5 passes x N cells x (s_in + s_out) = 40 N bytes with FP32 storage, 80 N
@@ -37,6 +41,7 @@ sys.path.insert(0, ROOT)
 METRIC = "ns/cell per RK3 step + Poisson GB/s (% HBM peak), mixed vs FP64, 1 GPU"
 HBM_PEAK_GBS = 8000.0
 DT = 2e-4
+ICHECK = 10  # steps between divergence checks (icheck of every reference example input)
 # HBM bytes moved by one standalone Poisson solve, measured with rocprofv3 --pmc
 # (FETCH_SIZE x 2 + WRITE_SIZE x 1, the factors a 1 GiB device copy calibrates);
 # the per-kernel table is profiles/channel_1024x512x384_hbm_traffic.md
@@ -86,31 +91,40 @@ def cpu_baseline(p, init, budget_s=20.0):
             "min_s": float(np.min(times)), "max_s": float(np.max(times)), "avg_s": float(np.mean(times))}
 
 
-def run(p, init, precision, args, sweeps=0, solves=True):
+def run(p, init, precision, args, sweeps=0, icheck=1, solves=True):
     """Time K steps (and, with solves, S Poisson solves) in one precision mode; return figures and
-    final fields.  sweeps: FP64 refinement sweeps of the last substep's pressure solve (mixed mode)."""
+    final fields.  sweeps: FP64 refinement sweeps of the last substep's pressure solve (mixed mode),
+    on every icheck-th step: the steps whose divergence the reference checks (main.f90:501-510,
+    icheck = 10 in every example input), counted so that the last timed step is one of them."""
     import numpy as np
     import torch
     from cans_amd import _lib as LB
     from cans_amd.host import Simulation
     from cans_amd import initflow as IF
     from tests import cases
-    label = ("mixed, %d refinement sweeps" % sweeps) if precision == LB.CANS_MIXED else "fp64"
+    label = ("mixed, %d refinement sweeps every %d steps" % (sweeps, icheck)) if precision == LB.CANS_MIXED \
+        else "fp64"
     print("[bench] %s: set-up" % label, file=sys.stderr, flush=True)
     sim = Simulation(cases.host_case(p), precision=precision)
-    if sweeps:
-        sim.set_refinement(sweeps, 4)  # the last substep of each step, whose projection sets div(u)
+
+    def refine(i):  # timed step i: refine the last substep, whose projection sets div(u), on icheck steps
+        if sweeps:
+            on = (args.steps - 1 - i) % icheck == 0
+            sim.set_refinement(sweeps if on else 0, 4 if on else 0)
+
     sim.set_state(*(IF.halo(p["ng"], q) for q in init))
     sim.init_bounds()
     ncell = int(np.prod(p["ng"]))
-    for _ in range(args.warmup):
+    for i in range(args.warmup):
+        refine(i - args.warmup)
         sim.rk3_step(DT)
     torch.cuda.synchronize()
     # the context runs on torch's current stream, so these events bracket its work
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     e0.record()
-    for _ in range(args.steps):
+    for i in range(args.steps):
+        refine(i)
         sim.rk3_step(DT)
     e1.record()
     torch.cuda.synchronize()
@@ -123,6 +137,8 @@ def run(p, init, precision, args, sweeps=0, solves=True):
            "divmax": divmax}
     if precision == LB.CANS_MIXED:
         res["refinement_sweeps"] = sweeps
+        if sweeps:
+            res["refined_every_steps"] = icheck
     if not solves:
         sim.close()
         del sim
@@ -377,20 +393,23 @@ def main():
     grid = Grid(cases.host_case(p))
     # one seeded state, FP32-representable so that both modes start from the same values
     init = [q.astype(np.float32).astype(np.float64) for q in cases.initial_state("channel", p, grid, seed=1)]
-    # mixed: config 2 as stated, two FP64 refinement sweeps of the last substep's pressure solve;
-    # mixed_plain: the same step with a single FP32-storage solve (a labelled variant)
-    modes = {"both": ("mixed", "mixed_plain", "fp64"), "mixed": ("mixed", "mixed_plain"),
-             "fp64": ("fp64",)}[args.precision]
+    # mixed: config 2 as stated, two FP64 refinement sweeps of the last substep's pressure solve on
+    # the steps whose divergence is checked (every 10th, the last timed step among them);
+    # mixed_refined_every_step: the sweeps on every step; mixed_plain: a single FP32-storage solve
+    modes = {"both": ("mixed", "mixed_refined_every_step", "mixed_plain", "fp64"),
+             "mixed": ("mixed", "mixed_refined_every_step", "mixed_plain"), "fp64": ("fp64",)}[args.precision]
     res, final = {}, {}
     for m in modes:
         res[m], final[m] = run(p, init, LB.CANS_FP64 if m == "fp64" else LB.CANS_MIXED, args,
-                               sweeps=2 if m == "mixed" else 0, solves=m != "mixed_plain")
+                               sweeps=0 if m in ("fp64", "mixed_plain") else 2,
+                               icheck=ICHECK if m == "mixed" else 1, solves=m == "mixed" or m == "fp64")
     head = res[modes[0]]
     head_mode = modes[0]
     wl = ("periodic channel %dx%dx%d, stretched z (gr=1.8), Re_b=5600, dt=2e-4; %s"
           % (tuple(p["ng"]) + ("mixed precision (FP32 pressure-side storage and FFTs, FP64 velocity, stencils, "
                                "z elimination and divergence, 2 FP64 refinement sweeps of the last substep's "
-                               "pressure solve) vs FP64" if head_mode == "mixed" else head_mode,)))
+                               "pressure solve on the divergence-checked steps, every %d) vs FP64" % ICHECK
+                               if head_mode == "mixed" else head_mode,)))
     pois = head["poisson"]
     out = {
         "metric": METRIC,
@@ -422,10 +441,12 @@ def main():
     if "fp64" in modes and "mixed" in modes:
         out["speedup_mixed_vs_fp64"] = res["fp64"]["ms_per_step"] / res["mixed"]["ms_per_step"]
         out["speedup_mixed_plain_vs_fp64"] = res["fp64"]["ms_per_step"] / res["mixed_plain"]["ms_per_step"]
+        out["speedup_mixed_refined_every_step_vs_fp64"] = (res["fp64"]["ms_per_step"] /
+                                                           res["mixed_refined_every_step"]["ms_per_step"])
         b = final["fp64"]
         wgt = np.broadcast_to(1 / grid.dzfi[1:-1], b["p"].shape)
         db = b["p"] - (b["p"] * wgt).sum() / wgt.sum()
-        for m in ("mixed", "mixed_plain"):
+        for m in ("mixed", "mixed_refined_every_step", "mixed_plain"):
             err = {}
             a = final[m]
             for k in "uvw":
