@@ -12,6 +12,9 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "common.hpp"
@@ -351,7 +354,12 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& a, int bx, int sys) {
           smem[ix(mk_pos(dn ? nd - 2 - e : e + 1, N))] = narrow<T>(a2.y, b2.y);
         }
       } else {
-        for (int e = 2 * lt; e < N; e += 2 * Tn) {
+        // compile-time trip count (N = 8 Tn, or N < 8 with Tn = 1): the unrolled loop issues
+        // all of the lane's row loads before its first LDS store
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int e = 2 * lt + 2 * r * Tn;
+          if (e >= N) continue;
           d2 a2, b2;
           if constexpr (DV != 0) {
             a2 = div_pair<T, DV == 2>(a, ra, e);
@@ -445,7 +453,10 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& a, int bx, int sys) {
       if (N == 1) {
         if (lt == 0) smem[ix(0)] = narrow<T>(merge_pp((double)xa[0], 0., xb ? (double)xb[0] : 0., 0.));
       } else if (kind == K_PP) {
-        for (int m = lt; m < H; m += Tn) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int m = lt + r * Tn;
+          if (m >= H) continue;
           const d2 a2 = ld2(xa + 2 * m);  // m = 0: (r0, r_{N/2})
           const d2 b2 = xb ? ld2(xb + 2 * m) : d2(0., 0.);
           if (m == 0) {
@@ -458,7 +469,10 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& a, int bx, int sys) {
         }
       } else {
         const bool dd = kind == K_DD;  // RODFT01 reads the reversed input
-        for (int m = lt; m <= H; m += Tn) {
+#pragma unroll
+        for (int r = 0; r < 5; r++) {
+          const int m = lt + r * Tn;
+          if (m > H) continue;
           if (m == 0) {
             const int i0 = dd ? N - 1 : 0;
             smem[ix(0)] = narrow<T>(merge_dct(a.mk[0], (double)xa[i0], 0., xb ? (double)xb[i0] : 0., 0.));
@@ -521,66 +535,198 @@ struct YArgs {
   const cplx<double>* mk;
 };
 
-// bx: work-group index among this system's; sys: system of a batch.  Lanes past
-// a.lines * a.Tn only take part in the barriers.  FLIP: as in xpass_body.
-template <int DIR, typename T, bool FLIP = false>
-__device__ __forceinline__ void ypass_body(const YArgs<T>& a, int bx, int sys) {
-  extern __shared__ unsigned char smem_raw[];
-  cplx<T>* smem = (cplx<T>*)smem_raw;  // LDS line in the storage type
-  const int LN = a.lines, N = a.N, Tn = a.Tn, kind = FLIP ? K_DD : a.kind;
-  constexpr bool FWD = (DIR > 0) != FLIP;
-  const int c = threadIdx.x % LN, lt = threadIdx.x / LN;
-  const bool act = lt < Tn;
+// One tile of the y pass in three steps: ypass_load reads the elements this lane's
+// pre-processing needs into registers (raw), all loads of a batch issued before any LDS
+// store; ypass_pre forms the complex line in LDS from them; ypass_fft_post transforms it
+// and stores the result.  The forward-type forms take their 8 elements in one batch; the
+// inverse forms (two elements per spectral index m = lt + r Tn) may split r into batches
+// [R0, R1) to bound the registers raw holds (one batch measured fastest: issuing every
+// load before the first LDS store keeps the most bytes in flight).  Compile-time trip
+// counts (N = 8 Tn, or N < 8 with Tn = 1) keep raw in registers.
+constexpr int YRAW = 10;  // raw elements per lane: 8 (forward-type forms), up to 10 (inverse DCT/DST)
+
+template <typename T>
+struct YTile {
+  int c, lt, k, ia;
+  bool act, va, vb;
+};
+
+template <typename T>
+__device__ __forceinline__ YTile<T> ytile(const YArgs<T>& a, long bx) {
+  YTile<T> t;
+  const int LN = a.lines;
+  t.c = threadIdx.x % LN;
+  t.lt = threadIdx.x / LN;
+  t.act = t.lt < a.Tn;
   // column pairs are numbered plane by plane; a work-group's lines may span planes
   const int ncp = (a.n1 + 1) / 2;
-  const long g = (long)bx * LN + c;
-  const int k = (int)(g / ncp);
-  const int ia = 2 * (int)(g % ncp);
-  const bool va = act && k < a.n3, vb = va && ia + 1 < a.n1;
-  const IxStride ix{c, LN};
-  // element e (along y) of column ia in the tiled work buffer; columns ia, ia+1 are adjacent
-  const int nic = wnic(a.n1);
-  T* const W = a.W[sys];
-  auto col = [&](int e) { return W + woff(ia, e, k, a.n2, nic); };
-  // (A, B) pair of element e of this line
-  auto ld = [&](int e) -> d2 { return vb ? ld2(col(e)) : d2((double)*col(e), 0.); };
-  auto st = [&](int e, double x, double y) {
-    if (vb) st2(col(e), x, y);
-    else *col(e) = (T)x;
+  const long g = bx * LN + t.c;
+  t.k = (int)(g / ncp);
+  t.ia = 2 * (int)(g % ncp);
+  t.va = t.act && t.k < a.n3;
+  t.vb = t.va && t.ia + 1 < a.n1;
+  return t;
+}
+
+template <typename T>
+__device__ __forceinline__ T* ycol(const YArgs<T>& a, const YTile<T>& t, int sys, int e) {
+  return a.W[sys] + woff(t.ia, e, t.k, a.n2, wnic(a.n1));
+}
+
+template <int DIR, typename T, bool FLIP, int R0, int R1>
+__device__ __forceinline__ void ypass_load(const YArgs<T>& a, const YTile<T>& t, int sys,
+                                           HIP_vector_type<T, 2>* raw) {
+  typedef HIP_vector_type<T, 2> v2;
+  const int N = a.N, Tn = a.Tn, lt = t.lt, kind = FLIP ? K_DD : a.kind;
+  constexpr bool FWD = (DIR > 0) != FLIP;
+  if (!t.va) return;
+  auto ld = [&](int e) -> v2 {
+    const T* p = ycol(a, t, sys, e);
+    return t.vb ? *(const v2*)p : v2(*p, (T)0);
   };
   if (FWD || k_double(kind)) {
-    if (va && kind == K_DDF) {
+    if (R0 != 0) return;
+    const int lim = kind == K_DDF ? a.nd - 1 : (kind == K_ND || kind == K_DN) ? a.nd : N;
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+      const int e = lt + r * Tn;
+      if (e < lim) raw[r] = ld(e);
+    }
+  } else if (N == 1) {
+    if (R0 == 0 && lt == 0) raw[0] = ld(0);
+  } else if (kind == K_PP) {
+    const int H = N >> 1;
+#pragma unroll
+    for (int r = R0; r < R1 && r < 4; r++) {
+      const int m = lt + r * Tn;
+      if (m < H) {
+        raw[2 * (r - R0)] = ld(2 * m);
+        raw[2 * (r - R0) + 1] = ld(2 * m + 1);
+      }
+    }
+  } else {
+    const int H = N >> 1;
+    const bool dd = kind == K_DD;  // RODFT01 reads the reversed input
+#pragma unroll
+    for (int r = R0; r < R1; r++) {
+      const int m = lt + r * Tn;
+      if (m > H) continue;
+      if (m == 0) {
+        raw[2 * (r - R0)] = ld(dd ? N - 1 : 0);
+        continue;
+      }
+      raw[2 * (r - R0)] = ld(dd ? N - 1 - m : m);
+      raw[2 * (r - R0) + 1] = ld(dd ? m - 1 : N - m);
+    }
+  }
+}
+
+template <int DIR, typename T, bool FLIP, int R0, int R1>
+__device__ __forceinline__ void ypass_pre(const YArgs<T>& a, const YTile<T>& t, const HIP_vector_type<T, 2>* raw,
+                                          cplx<T>* smem) {
+  const int N = a.N, Tn = a.Tn, lt = t.lt, kind = FLIP ? K_DD : a.kind;
+  constexpr bool FWD = (DIR > 0) != FLIP;
+  const IxStride ix{t.c, a.lines};
+  if (!t.va) return;
+  auto w = [&](int r) -> d2 { return d2((double)raw[r].x, (double)raw[r].y); };
+  if (FWD || k_double(kind)) {
+    if (R0 != 0) return;
+    if (kind == K_DDF) {
       // odd extension (as in the x pass); the boundary face nd-1 stays in place
       const int nd = a.nd;
       if (lt == 0) {
         smem[ix(0)] = narrow<T>(0., 0.);
         smem[ix(nd)] = narrow<T>(0., 0.);
       }
-      for (int e = lt; e < nd - 1; e += Tn) {
-        const d2 v = ld(e);
-        smem[ix(e + 1)] = narrow<T>(v.x, v.y);
-        smem[ix(N - 1 - e)] = narrow<T>(-v.x, -v.y);
+#pragma unroll
+      for (int r = 0; r < 8; r++) {
+        const int e = lt + r * Tn;
+        if (e < nd - 1) {
+          const d2 v = w(r);
+          smem[ix(e + 1)] = narrow<T>(v.x, v.y);
+          smem[ix(N - 1 - e)] = narrow<T>(-v.x, -v.y);
+        }
       }
-    } else if (va && (kind == K_ND || kind == K_DN)) {
+    } else if (kind == K_ND || kind == K_DN) {
       const int nd = a.nd;
       for (int p = (nd + 1) / 2 + lt; p < N - nd / 2; p += Tn) smem[ix(p)] = narrow<T>(0., 0.);
-      for (int e = lt; e < nd; e += Tn) {
-        const d2 v = ld(e);
-        smem[ix(mk_pos(kind == K_DN ? nd - 1 - e : e, N))] = narrow<T>(v.x, v.y);
-      }
-    } else if (va) {
-      for (int e = lt; e < N; e += Tn) {
-        d2 v = ld(e);
-        if (kind == K_DD && (e & 1)) {
-          v.x = -v.x;
-          v.y = -v.y;
+#pragma unroll
+      for (int r = 0; r < 8; r++) {
+        const int e = lt + r * Tn;
+        if (e < nd) {
+          const d2 v = w(r);
+          smem[ix(mk_pos(kind == K_DN ? nd - 1 - e : e, N))] = narrow<T>(v.x, v.y);
         }
-        smem[ix(kind == K_PP ? e : mk_pos(e, N))] = narrow<T>(v.x, v.y);
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < 8; r++) {
+        const int e = lt + r * Tn;
+        if (e < N) {
+          d2 v = w(r);
+          if (kind == K_DD && (e & 1)) {
+            v.x = -v.x;
+            v.y = -v.y;
+          }
+          smem[ix(kind == K_PP ? e : mk_pos(e, N))] = narrow<T>(v.x, v.y);
+        }
       }
     }
-    __syncthreads();
-    fft_line<T, -1>(smem, ix, N, a.logN, lt, Tn, line_tw<T>(a.tw, N), a.odd, act);
-    if (!va) return;
+    return;
+  }
+  // spectral indices m and N-m read the same two rows: one thread takes both
+  const int H = N >> 1;
+  if (N == 1) {
+    if (R0 == 0 && lt == 0) {
+      const d2 v = w(0);
+      smem[ix(0)] = narrow<T>(merge_pp(v.x, 0., v.y, 0.));
+    }
+  } else if (kind == K_PP) {
+#pragma unroll
+    for (int r = R0; r < R1 && r < 4; r++) {
+      const int m = lt + r * Tn;
+      if (m >= H) continue;
+      const d2 re = w(2 * (r - R0)), im = w(2 * (r - R0) + 1);  // m = 0: r0 and r_{N/2}
+      if (m == 0) {
+        smem[ix(0)] = narrow<T>(merge_pp(re.x, 0., re.y, 0.));
+        smem[ix(H)] = narrow<T>(merge_pp(im.x, 0., im.y, 0.));
+      } else {
+        smem[ix(m)] = narrow<T>(merge_pp(re.x, im.x, re.y, im.y));
+        smem[ix(N - m)] = narrow<T>(merge_pp(re.x, -im.x, re.y, -im.y));
+      }
+    }
+  } else {
+#pragma unroll
+    for (int r = R0; r < R1; r++) {
+      const int m = lt + r * Tn;
+      if (m > H) continue;
+      if (m == 0) {
+        const d2 x0 = w(2 * (r - R0));
+        smem[ix(0)] = narrow<T>(merge_dct(a.mk[0], x0.x, 0., x0.y, 0.));
+        continue;
+      }
+      const d2 xa = w(2 * (r - R0)), xb = w(2 * (r - R0) + 1);
+      smem[ix(m)] = narrow<T>(merge_dct(a.mk[m], xa.x, xb.x, xa.y, xb.y));
+      if (m != H) smem[ix(N - m)] = narrow<T>(merge_dct(a.mk[N - m], xb.x, xa.x, xb.y, xa.y));
+    }
+  }
+}
+
+// the line in LDS (after a barrier) -> transform -> post-processing -> stores
+template <int DIR, typename T, bool FLIP>
+__device__ __forceinline__ void ypass_fft_post(const YArgs<T>& a, const YTile<T>& t, int sys, cplx<T>* smem) {
+  const int N = a.N, Tn = a.Tn, lt = t.lt, kind = FLIP ? K_DD : a.kind;
+  constexpr bool FWD = (DIR > 0) != FLIP;
+  const IxStride ix{t.c, a.lines};
+  const bool vb = t.vb;
+  auto st = [&](int e, double x, double y) {
+    T* p = ycol(a, t, sys, e);
+    if (vb) st2(p, x, y);
+    else *p = (T)x;
+  };
+  if (FWD || k_double(kind)) {
+    fft_line<T, -1>(smem, ix, N, a.logN, lt, Tn, line_tw<T>(a.tw, N), a.odd, t.act);
+    if (!t.va) return;
     if (kind == K_PP) {
       const int H = N >> 1;
       if (N == 1) {
@@ -624,42 +770,8 @@ __device__ __forceinline__ void ypass_body(const YArgs<T>& a, int bx, int sys) {
       }
     }
   } else {
-    // spectral indices m and N-m read the same two rows: one thread takes both
-    if (va) {
-      const int H = N >> 1;
-      if (N == 1) {
-        if (lt == 0) {
-          const d2 v = ld(0);
-          smem[ix(0)] = narrow<T>(merge_pp(v.x, 0., v.y, 0.));
-        }
-      } else if (kind == K_PP) {
-        for (int m = lt; m < H; m += Tn) {
-          const d2 re = ld(2 * m), im = ld(2 * m + 1);  // m = 0: r0 and r_{N/2}
-          if (m == 0) {
-            smem[ix(0)] = narrow<T>(merge_pp(re.x, 0., re.y, 0.));
-            smem[ix(H)] = narrow<T>(merge_pp(im.x, 0., im.y, 0.));
-          } else {
-            smem[ix(m)] = narrow<T>(merge_pp(re.x, im.x, re.y, im.y));
-            smem[ix(N - m)] = narrow<T>(merge_pp(re.x, -im.x, re.y, -im.y));
-          }
-        }
-      } else {
-        const bool dd = kind == K_DD;  // RODFT01 reads the reversed input
-        for (int m = lt; m <= H; m += Tn) {
-          if (m == 0) {
-            const d2 x0 = ld(dd ? N - 1 : 0);
-            smem[ix(0)] = narrow<T>(merge_dct(a.mk[0], x0.x, 0., x0.y, 0.));
-            continue;
-          }
-          const d2 xa = ld(dd ? N - 1 - m : m), xb = ld(dd ? m - 1 : N - m);
-          smem[ix(m)] = narrow<T>(merge_dct(a.mk[m], xa.x, xb.x, xa.y, xb.y));
-          if (m != H) smem[ix(N - m)] = narrow<T>(merge_dct(a.mk[N - m], xb.x, xa.x, xb.y, xa.y));
-        }
-      }
-    }
-    __syncthreads();
-    fft_line<T, +1>(smem, ix, N, a.logN, lt, Tn, line_tw<T>(a.tw, N), a.odd, act);
-    if (!va) return;
+    fft_line<T, +1>(smem, ix, N, a.logN, lt, Tn, line_tw<T>(a.tw, N), a.odd, t.act);
+    if (!t.va) return;
     for (int e = lt; e < N; e += Tn) {
       cplx<double> z = widen(smem[ix(kind == K_PP ? e : mk_pos(e, N))]);
       if (kind == K_DD && (e & 1)) {
@@ -669,6 +781,20 @@ __device__ __forceinline__ void ypass_body(const YArgs<T>& a, int bx, int sys) {
       st(e, z.re, z.im);
     }
   }
+}
+
+// bx: work-group index among this system's; sys: system of a batch.  Lanes past
+// a.lines * a.Tn only take part in the barriers.  FLIP: as in xpass_body.
+template <int DIR, typename T, bool FLIP = false>
+__device__ __forceinline__ void ypass_body(const YArgs<T>& a, int bx, int sys) {
+  extern __shared__ unsigned char smem_raw[];
+  cplx<T>* smem = (cplx<T>*)smem_raw;  // LDS line in the storage type
+  HIP_vector_type<T, 2> raw[YRAW];
+  const YTile<T> t = ytile(a, bx);
+  ypass_load<DIR, T, FLIP, 0, 5>(a, t, sys, raw);
+  ypass_pre<DIR, T, FLIP, 0, 5>(a, t, raw, smem);
+  __syncthreads();
+  ypass_fft_post<DIR, T, FLIP>(a, t, sys, smem);
 }
 
 // FP32 lines: 16 lines x N/8 threads = 1024 threads in 64 KB of LDS; at most 64 VGPRs
@@ -2091,12 +2217,17 @@ static int tune_type(cans_ctx* c, cans_plan* p) {
     }
     return best;
   };
-  // x pass
+  // x pass: shapes whose LDS exceeds the device's per-work-group limit are skipped
+  int lds_max = 65536;
+  if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) != hipSuccess)
+    lds_max = 65536;
   const int tnx = p->nfft[0] >= 8 ? p->nfft[0] / 8 : 1;
   float bx = 1e30f;
   int xbest = 256;
   for (int xt : {128, 256, 512}) {
     if (xt < tnx && xt != 512) continue;
+    const size_t xshm = (size_t)(tnx >= xt ? 1 : xt / tnx) * padded_len(p->nfft[0]) * sizeof(cplx<T>);
+    if (xshm > (size_t)lds_max) continue;
     p->xthreads[slot] = xt;
     const float ms = timed([&]() -> int {
       int r = xpass<T>(&cs, p, &fld, &W, 1, +1);
@@ -2124,11 +2255,12 @@ static int tune_type(cans_ctx* c, cans_plan* p) {
   hipEventDestroy(e1);
   hipFree(fb);
   hipFree(wb);
-  if (rc) {  // keep the heuristics
+  if (rc) {  // a failed trial keeps the heuristic shapes; the plan itself stays valid
     p->xthreads[slot] = 0;
     p->ylines[slot] = 0;
+    (void)hipGetLastError();
   }
-  return rc;
+  return CANS_OK;
 }
 
 extern "C" int cans_fftini(cans_ctx* c, const int* ng, const char* bcxy, const char* c_or_f, cans_plan** out,
