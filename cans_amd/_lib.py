@@ -117,6 +117,7 @@ SIGNATURES = {
                              C.c_int, _dp]),
     "cans_bulk_mean": (C.c_int, [_vp, _ip, _vp, _vp, _dp]),
     "cans_rk3_substep": (C.c_int, [_vp, C.POINTER(StepParams), C.c_int, C.c_double, C.POINTER(State)]),
+    "cans_rk3_step": (C.c_int, [_vp, C.POINTER(StepParams), C.c_double, C.POINTER(State)]),
     "cans_ctx_forcing": (C.c_int, [_vp, C.POINTER(_vp)]),
     "cans_mom_jt": (C.c_int, [_vp, _ip, _dp, C.c_double] + [_vp] * 12),
     "cans_rk3_substep_adjoint": (C.c_int, [_vp, C.POINTER(StepParams), C.c_int, C.c_double] + [_vp] * 16),

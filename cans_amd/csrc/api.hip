@@ -178,12 +178,13 @@ int do_updt_rhs_b(cans_ctx* c, const char* c_or_f, const char* cbc, const int* i
   return CANS_OK;
 }
 
-// three halo'd scratch fields of element type T (room for double), aligned like cans_field_alloc's
+// three halo'd scratch fields of element type T (room for double), aligned like cans_field_alloc's;
+// set 1 is the second set cans_rk3_step alternates with
 template <typename T>
-int ensure_tmp_fields(cans_ctx* c, T** t) {
+int ensure_tmp_fields(cans_ctx* c, T** t, int set = 0) {
   void* base = nullptr;
   const size_t fe = (size_t)c->L.sz * (c->L.n3 + 2);
-  int rc = ensure_work(c, 0, 3 * fe * sizeof(double) + 256, &base);
+  int rc = ensure_work(c, set ? 8 : 0, 3 * fe * sizeof(double) + 256, &base);
   if (rc) return rc;
   for (int k = 0; k < 3; k++) t[k] = (T*)((char*)base + 256 - sizeof(T)) + k * fe;
   return CANS_OK;
@@ -201,7 +202,7 @@ static int rk_core(cans_ctx* c, const double* rkpar, const double* dli, const do
                    const int* is_forced, const double* velf, const double* bforce, S* ru, S* rv, S* rw,
                    const double* u, const double* v, const double* w, double* uo, double* vo, double* wo,
                    int write_ro = 1, S* const* eo = nullptr, S* divu = nullptr, const double* sb = nullptr,
-                   const double* gacc = nullptr, double beta = 0.) {
+                   const double* gacc = nullptr, double beta = 0., const MomFold<S>* fold = nullptr) {
   if (c->rk_first) {  // rk.f90:72-100: RHS_old starts at zero on the first call
     c->rk_first = false;
     const size_t bytes = (size_t)c->L.cells() * sizeof(S);
@@ -211,6 +212,7 @@ static int rk_core(cans_ctx* c, const double* rkpar, const double* dli, const do
   }
   MomLaunch<S> m;
   m.u = u; m.v = v; m.w = w; m.p = p;
+  m.fold = fold;
   m.uo = uo; m.vo = vo; m.wo = wo;
   if (eo) {
     m.eu = eo[0]; m.ev = eo[1]; m.ew = eo[2];
@@ -515,8 +517,45 @@ static int helmholtz(cans_ctx* c, int n, const cans_helmholtz* const* h, const i
 // With implicit diffusion the mixed mode keeps the FP64 form of the velocity (the forced
 // intermediate velocity is the rhs of its Helmholtz systems, solved in FP64 on the FP64
 // velocity fields); the pressure side (p, pp, RHS_old, the Poisson solve) stays FP32.
+// The boundary rules the folded projection needs (MomFold): which directions are periodic,
+// and, at the faces of the others, set_bc's centred rule for the tangential components
+// (bound.f90:193-206; make_setbc's factor and sign)
 template <typename S>
-static int rk3_substep(cans_ctx* c, const cans_step_params* s, int irk, double dt, cans_state* st) {
+static void fold_rules(const cans_step_params* s, const cans_ctx* c, MomFold<S>& f) {
+  for (int d = 0; d < 3; d++) f.periodic[d] = s->cbcvel[2 * d + 6 * d] == 'P' && s->cbcvel[1 + 2 * d + 6 * d] == 'P';
+  for (int cc = 0; cc < 3; cc++)
+    for (int d = 0; d < 3; d++)
+      for (int ib = 0; ib < 2; ib++) {
+        const char t = s->cbcvel[ib + 2 * d + 6 * cc];
+        const double bv = s->bcvel[ib + 2 * d + 6 * cc];
+        const double dr = d < 2 ? s->dl[d] : (ib == 0 ? s->dzc_h[0] : s->dzc_h[c->L.n3]);
+        if (t == 'D') {
+          f.factor[cc][d][ib] = 2. * bv;
+          f.sgn[cc][d][ib] = -1.;
+        } else {
+          f.factor[cc][d][ib] = ib == 0 ? -dr * bv : dr * bv;
+          f.sgn[cc][d][ib] = 1.;
+        }
+      }
+}
+
+// Folding one substep's projection into the next (cans_rk3_step): a deferring substep
+// stops after its pressure solve (and boundp of pp), leaving its intermediate velocity in
+// scratch set `set`; the next substep's momentum kernel then forms the projected velocity
+// and p + pp while it stages its tiles (MomFold), reading its intermediate velocity from
+// the other set and writing p + pp to the other pressure buffer.
+struct StepFold {
+  bool defer = false;    // this substep leaves its projection to the next one
+  bool fold_in = false;  // this substep folds the previous substep's projection
+  int set = 0;           // scratch set of this substep's intermediate velocity
+  double dtrk_in = 0.;   // the folded substep's dtrk
+  void* p_in = nullptr;  // the pressure buffer holding p^(k-1)
+  void* p_out = nullptr; // fold_in: the buffer that receives p^(k-1) + pp
+};
+
+template <typename S>
+static int rk3_substep(cans_ctx* c, const cans_step_params* s, int irk, double dt, cans_state* st,
+                       const StepFold* fo = nullptr) {
   const bool mixed = sizeof(S) != sizeof(double);
   const bool delta = mixed && !c->imp;
   const double* rkpar = RKCOEFF[irk - 1];
@@ -527,11 +566,40 @@ static int rk3_substep(cans_ctx* c, const cans_step_params* s, int irk, double d
   double* w = (double*)st->w;
   S* p = (S*)st->p;
   S* pp = (S*)st->pp;
+  const bool defer = fo && fo->defer, fold_in = fo && fo->fold_in;
+  const int set = fo ? fo->set : 0;
   double* t[3];
   S* e[3];
-  int rc = ensure_tmp_fields<double>(c, t);
+  int rc = ensure_tmp_fields<double>(c, t, set);
   if (rc) return rc;
-  if ((rc = ensure_tmp_fields<S>(c, e))) return rc;  // same scratch, viewed as S
+  if ((rc = ensure_tmp_fields<S>(c, e, set))) return rc;  // same scratch, viewed as S
+  // the pressure buffers: p^(k-1) is read from pin; with fold_in, p^(k-1) + pp goes to pnew
+  // (the other buffer) and the substep continues with it
+  S* pin = (S*)(fo ? fo->p_in : st->p);
+  S* pnew = pin;
+  MomFold<S> mf;
+  const double* sbase[3] = {u, v, w};  // the velocity the momentum kernel reads
+  if (fold_in) {
+    double* tp[3];
+    S* ep[3];
+    if ((rc = ensure_tmp_fields<double>(c, tp, 1 - set))) return rc;
+    if ((rc = ensure_tmp_fields<S>(c, ep, 1 - set))) return rc;
+    pnew = (S*)fo->p_out;
+    mf.pp = pp;
+    mf.eu = delta ? ep[0] : nullptr;
+    mf.ev = delta ? ep[1] : nullptr;
+    mf.ew = delta ? ep[2] : nullptr;
+    mf.f = c->dscal;
+    mf.dt = fo->dtrk_in;
+    mf.pout = pnew;
+    fold_rules<S>(s, c, mf);
+    if (!delta) {  // FP64: the stencil reads the folded substep's intermediate velocity
+      sbase[0] = tp[0];
+      sbase[1] = tp[1];
+      sbase[2] = tp[2];
+    }
+  }
+  p = pin;
   const double* base[3] = {u, v, w};
   // bulk forcing (rk writes it to dscal[0..2], 0 for unforced components): added where the
   // intermediate velocity is read (divergence, correc) rather than stored, except before the
@@ -585,10 +653,18 @@ static int rk3_substep(cans_ctx* c, const cans_step_params* s, int irk, double d
   }
   const double* sb = (s->is_buoyancy && s->nscal > 0) ? (const double*)s->scal[0].val : nullptr;
   // rk: updated velocity (or its increment) into the scratch; u,v,w keep the old level for the stencil
+  // (a folding substep keeps div(u) of the step's initial velocity, written by substep 1: its
+  // increments are over that velocity)
   rc = rk_core<S>(c, rkpar, s->dli, s->dzci_d, s->dzfi_d, s->grid_vol_ratio_c_d, s->grid_vol_ratio_f_d, dt,
-                  s->visc, p, s->is_forced, s->velf, s->bforce, (S*)st->dudtrko, (S*)st->dvdtrko, (S*)st->dwdtrko, u,
-                  v, w, t[0], t[1], t[2], irk != 3, delta ? e : nullptr, divu, sb, s->gacc, s->beta);
+                  s->visc, p, s->is_forced, s->velf, s->bforce, (S*)st->dudtrko, (S*)st->dvdtrko, (S*)st->dwdtrko,
+                  sbase[0], sbase[1], sbase[2], t[0], t[1], t[2], irk != 3, delta ? e : nullptr,
+                  fold_in ? nullptr : divu, sb, s->gacc, s->beta, fold_in ? &mf : nullptr);
   if (rc) return rc;
+  if (fold_in) {  // p + pp of the folded substep: its upper ghosts, as updatep + boundp leave them
+    p = pnew;
+    if ((rc = plan_boundp(c, s->cbcpre, s->bcpre, s->nb, s->is_bound, s->dl, s->dzc_h, p, CANS_HALO_UPPER)))
+      return rc;
+  }
   for (int k = 0; k < 3 && !fvec; k++)
     if (s->is_forced[k] && (rc = launch_add_forcing<double>(c, t[k], t[k], k, 1))) return rc;
   // implicit diffusion (main.f90:453-462): the three components' Helmholtz systems, alpha = -visc*dtrk/2,
@@ -632,11 +708,14 @@ static int rk3_substep(cans_ctx* c, const cans_step_params* s, int irk, double d
   // residual formed in FP64 from the velocity (b is fillps's rhs) and the boundp'd corrections
   // so far.  correc then projects with pp + x_1 + x_2, so div(u) after this substep is dtrk times
   // the residual left by the last sweep; pp keeps the first solve's part only.
+  // The residual's b is formed from the velocity itself (u + e + f, FP64), not from the stored
+  // FP32 div(u) of the step's initial velocity, whose rounding the sweeps could not remove.
   for (int q = 1; q <= nsweep; q++) {
     S* xq = corr[q - 1];
     if (fuse_div) {
-      DivSrc<S> ds = {su, sv, sw, eu, ev, ew, divu, s->dzfi_d, dtrki, dtrki * s->dli[0], dtrki * s->dli[1],
+      DivSrc<S> ds = {su, sv, sw, eu, ev, ew, nullptr, s->dzfi_d, dtrki, dtrki * s->dli[0], dtrki * s->dli[1],
                       s->dli[0], s->dli[1]};
+      ds.f = fvec;
       ds.corr[0] = pp;
       ds.corr[1] = q > 1 ? corr[0] : nullptr;
       ds.ncorr = q;
@@ -656,6 +735,7 @@ static int rk3_substep(cans_ctx* c, const cans_step_params* s, int irk, double d
     if ((rc = plan_boundp(c, s->cbcpre, s->bcpre, s->nb, s->is_bound, s->dl, s->dzc_h, xq, CANS_HALO_ABSENT)))
       return rc;
   }
+  if (defer) return CANS_OK;  // the next substep's momentum kernel projects (StepFold)
   // correc reads the intermediate velocity and writes the caller's fields (whole halo'd
   // box); updatep (p += pp) is independent of the velocity halos and rides along
   // with implicit diffusion updatep also adds alpha*lap(pp) (updatep.f90:16-33): its own pass
@@ -690,6 +770,64 @@ extern "C" int cans_rk3_substep(cans_ctx* c, const cans_step_params* s, int irk,
       !s->dzf_h || !s->grid_vol_ratio_c_d || !s->grid_vol_ratio_f_d)
     return fail(CANS_EINVAL, "cans_rk3_substep: null parameter array");
   return CANS_DISPATCH(c, rk3_substep<T>(c, s, irk, dt, st));
+}
+
+// the three substeps of one step; substep k's projection (correc, bounduvw, updatep,
+// boundp of p) is folded into substep k+1's momentum kernel for k = 1, 2 where that
+// applies: explicit diffusion, no scalars, homogeneous pressure boundary values, and no
+// refinement of substeps 1 and 2.  The state between the substeps lives in the context's
+// scratch (two sets of intermediate fields and a second pressure field); after the call
+// st holds what three cans_rk3_substep calls leave, bit for bit.
+template <typename S>
+static int rk3_step(cans_ctx* c, const cans_step_params* s, double dt, cans_state* st) {
+  bool fold = !c->imp && s->nscal == 0 && !(s->rhsbx_d || s->rhsby_d || s->rhsbz_d) && !(c->refine_mask & 3);
+  for (int q = 0; q < 6 && fold; q++) fold = s->bcpre[q] == 0. || s->cbcpre[q] == 'P';
+  if (!fold) {
+    for (int irk = 1; irk <= 3; irk++) {
+      int rc = rk3_substep<S>(c, s, irk, dt, st);
+      if (rc) return rc;
+    }
+    return CANS_OK;
+  }
+  // the second pressure buffer: a halo'd field of S, element 1 on a 256-byte boundary
+  void* pb = nullptr;
+  const size_t fe = (size_t)c->L.sz * (c->L.n3 + 2);
+  int rc = ensure_work(c, 9, fe * sizeof(S) + 256, &pb);
+  if (rc) return rc;
+  void* p2 = (char*)pb + 256 - sizeof(S);
+  const double* rkp = RKCOEFF[0];
+  StepFold f1;  // substep 1: intermediate velocity in set 0, projection deferred
+  f1.defer = true;
+  f1.set = 0;
+  f1.p_in = st->p;
+  if ((rc = rk3_substep<S>(c, s, 1, dt, st, &f1))) return rc;
+  StepFold f2;  // substep 2: folds substep 1 (set 0, p -> p2), intermediate in set 1, deferred
+  f2.defer = true;
+  f2.fold_in = true;
+  f2.set = 1;
+  f2.dtrk_in = (rkp[0] + rkp[1]) * dt;
+  f2.p_in = st->p;
+  f2.p_out = p2;
+  if ((rc = rk3_substep<S>(c, s, 2, dt, st, &f2))) return rc;
+  rkp = RKCOEFF[1];
+  StepFold f3;  // substep 3: folds substep 2 (set 1, p2 -> p), intermediate in set 0, projects
+  f3.fold_in = true;
+  f3.set = 0;
+  f3.dtrk_in = (rkp[0] + rkp[1]) * dt;
+  f3.p_in = p2;
+  f3.p_out = st->p;
+  return rk3_substep<S>(c, s, 3, dt, st, &f3);
+}
+
+extern "C" int cans_rk3_step(cans_ctx* c, const cans_step_params* s, double dt, cans_state* st) {
+  CHECK_CTX("cans_rk3_step");
+  if (!s || !st) return fail(CANS_EINVAL, "cans_rk3_step: bad argument");
+  if (!st->u || !st->v || !st->w || !st->p || !st->pp || !st->dudtrko || !st->dvdtrko || !st->dwdtrko)
+    return fail(CANS_EINVAL, "cans_rk3_step: null field");
+  if (!s->plan || !s->lambdaxy_d || !s->a_d || !s->b_d || !s->c_d || !s->dzci_d || !s->dzfi_d || !s->dzc_h ||
+      !s->dzf_h || !s->grid_vol_ratio_c_d || !s->grid_vol_ratio_f_d)
+    return fail(CANS_EINVAL, "cans_rk3_step: null parameter array");
+  return CANS_DISPATCH(c, rk3_step<T>(c, s, dt, st));
 }
 
 // ---------------------------------------------------------------------------

@@ -21,10 +21,27 @@ struct PlaneOps {
   int nops;
 };
 
+// The previous substep's projection, applied while the momentum kernel stages its
+// velocity: u = correc(intermediate velocity, pp) with the ghost values bounduvw gives
+// after correc (bound.f90, keep_norm), and p + pp (updatep) for the pressure term.
+template <typename S>
+struct MomFold {
+  const S* pp;                    // its pressure correction (halo'd, ghosts set by boundp)
+  const S *eu, *ev, *ew;          // mixed: its increments over the kernel's u, v, w; FP64: nullptr
+                                  // (u, v, w then hold its intermediate velocity)
+  const double* f;                // its bulk forcing (device, 3)
+  double dt;                      // its dtrk
+  S* pout;                        // p + pp, written for the interior cells
+  int periodic[3];                // direction periodic
+  // tangential components at non-periodic faces: ghost = factor + sgn * adjacent value
+  double factor[3][3][2], sgn[3][3][2];  // [component][direction][side]
+};
+
 template <typename S>
 struct MomLaunch {
   const double *u, *v, *w;
   const S* p;
+  const MomFold<S>* fold = nullptr;  // fold the previous substep's projection (see MomFold)
   double *uo = nullptr, *vo = nullptr, *wo = nullptr;  // updated velocity, or
   S *eu = nullptr, *ev = nullptr, *ew = nullptr;       // its increment over (u,v,w)
   S* divu = nullptr;                                   // optional: div(u) per cell, (n1,n2,n3)

@@ -50,6 +50,13 @@ struct MomArgs {
   double bq[3];                  // factor12*gacc(d)*beta*0.5
   int imp;                       // 0 explicit, 1 implicit diffusion, 2 implicit in z only
   double hf12;                   // 0.5*factor12
+  // FOLD kernels: the previous substep's correc + bounduvw + updatep (MomFold)
+  const S* fpp;
+  const S *feu, *fev, *few;
+  const double* ff;
+  double fdt, fdtdxi, fdtdyi;
+  S* pout;
+  const double *gu, *gv, *gw;    // the projected velocity at the ghost cells (k_fold_ghosts)
 };
 
 // Tiling of the momentum kernel: a work-group owns a TX x TY column of cells and
@@ -72,8 +79,23 @@ __global__ void k_rdz(int n3, const double* dzci, const double* dzfi, double* ta
 // Boussinesq term of the scalar sb.  Compile-time
 // choices, so that each form keeps only the pointers it uses in scalar registers (the
 // arguments otherwise spill to vector-register lanes inside the plane loop).
-template <typename S, bool DELTA, bool DIVU, bool BUOY>
-__global__ void __launch_bounds__(MOM_TX* MOM_TY) k_mom_rk(MomArgs<S> a) {
+// FOLD: the velocity this kernel advances is the previous substep's intermediate velocity
+// projected (correc.f90:13-61) with that substep's pp, with the ghost values bounduvw
+// (keep_norm, upper normal halo only, no tangential exchange; bound.f90:17-121) gives it
+// after correc; the pressure is p + pp (updatep), which is also stored to pout.  The values
+// are formed with the same operations in the same order as those routines, so the result is
+// the one of the unfused sequence, bit for bit.  Each ghost's value is correc's at a source
+// cell, then the wall rules of the directions that have one, in x, y, z order:
+//  * periodic: the ghosts that bounduvw copies (along x every tangential ghost and the normal
+//    component's upper one; along y and z the normal component's upper one) take their
+//    periodic image; every other ghost keeps correc's own value there;
+//  * at a wall a tangential component's ghost is factor + sgn * (the adjacent value);
+//  * the normal component keeps correc's value at the ghost (keep_norm): the formula at
+//    indices 0..n, the intermediate velocity itself at n+1.
+// With DELTA the increments written are over the kernel's u, v, w (the step's initial
+// velocity), not over the projected one.
+template <typename S, bool DELTA, bool DIVU, bool BUOY, bool FOLD = false>
+__global__ void __launch_bounds__(MOM_TX* MOM_TY, 4) k_mom_rk(MomArgs<S> a) {
   __shared__ double su[3][MOM_PY][MOM_PX], sv[3][MOM_PY][MOM_PX], sw[3][MOM_PY][MOM_PX];
   const Layout& L = a.L;
   const int ntx = (L.n1 + MOM_TX - 1) / MOM_TX, nty = (L.n2 + MOM_TY - 1) / MOM_TY;
@@ -88,6 +110,55 @@ __global__ void __launch_bounds__(MOM_TX* MOM_TY) k_mom_rk(MomArgs<S> a) {
   const int i = i0 + li + 1, j = j0 + lj + 1;
   const bool in = i <= L.n1 && j <= L.n2;
   const long sy = L.sy, sz = L.sz;
+  // FOLD: the projected velocity.  At an interior cell it is correc's value there (k_correc's
+  // expression); ghost cells were formed by k_fold_ghosts (the bounduvw rules) into gu, gv, gw.
+  auto fval = [&](int cm, int gi, int gj, int gk, long o) -> double {
+    if (gi < 1 || gi > L.n1 || gj < 1 || gj > L.n2 || gk < 1 || gk > L.n3)
+      return cm == 0 ? a.gu[o] : cm == 1 ? a.gv[o] : a.gw[o];
+    const double* b = cm == 0 ? a.u : cm == 1 ? a.v : a.w;
+    const S* e = cm == 0 ? a.feu : cm == 1 ? a.fev : a.few;
+    const double fc = a.ff[cm];
+    const double tu = e ? b[o] + (double)e[o] + fc : b[o] + fc;
+    const double pc = (double)a.fpp[o];
+    if (cm == 0) return tu - a.fdtdxi * ((double)a.fpp[o + 1] - pc);
+    if (cm == 1) return tu - a.fdtdyi * ((double)a.fpp[o + sy] - pc);
+    return tu - a.fdt * a.dzci[gk] * ((double)a.fpp[o + sz] - pc);
+  };
+  // FOLD: the three components at one cell (the pressure correction loaded once for all)
+  auto fval3 = [&](int gi, int gj, int gk, long o, double& xu, double& xv, double& xw) {
+    if (gi < 1 || gi > L.n1 || gj < 1 || gj > L.n2 || gk < 1 || gk > L.n3) {
+      xu = a.gu[o];
+      xv = a.gv[o];
+      xw = a.gw[o];
+      return;
+    }
+    const double bu = a.u[o], bv = a.v[o], bw = a.w[o];
+    const double pc = (double)a.fpp[o], px = (double)a.fpp[o + 1], py = (double)a.fpp[o + sy],
+                 pz = (double)a.fpp[o + sz];
+    const double dz = a.dzci[gk];
+    double tu, tv, tw;
+    if (a.feu) {
+      tu = bu + (double)a.feu[o] + a.ff[0];
+      tv = bv + (double)a.fev[o] + a.ff[1];
+      tw = bw + (double)a.few[o] + a.ff[2];
+    } else {
+      tu = bu + a.ff[0];
+      tv = bv + a.ff[1];
+      tw = bw + a.ff[2];
+    }
+    xu = tu - a.fdtdxi * (px - pc);
+    xv = tv - a.fdtdyi * (py - pc);
+    xw = tw - a.fdt * dz * (pz - pc);
+  };
+  // the pressure the kernel reads: p, or (FOLD) p + pp rounded to the storage type
+  auto pval = [&](long o) -> double {
+    if constexpr (FOLD) return (double)(S)((double)a.p[o] + (double)a.fpp[o]);
+    else return (double)a.p[o];
+  };
+  auto uval = [&](int cm, int gi, int gj, int gk, long o) -> double {
+    if constexpr (FOLD) return fval(cm, gi, gj, gk, o);
+    else return cm == 0 ? a.u[o] : cm == 1 ? a.v[o] : a.w[o];
+  };
   // stage plane kk of the tile (+ ring) into buffer slot
   auto stage = [&](int kk, int slot) {
     for (int q = t; q < MOM_PX * MOM_PY; q += MOM_TX * MOM_TY) {
@@ -95,9 +166,17 @@ __global__ void __launch_bounds__(MOM_TX* MOM_TY) k_mom_rk(MomArgs<S> a) {
       const int gi = i0 + pi, gj = j0 + pj;  // halo'd indices (i0+pi is cell i0+pi)
       if (gi <= L.n1 + 1 && gj <= L.n2 + 1) {
         const long o = L.at(gi, gj, kk);
-        su[slot][pj][pi] = a.u[o];
-        sv[slot][pj][pi] = a.v[o];
-        sw[slot][pj][pi] = a.w[o];
+        if constexpr (FOLD) {
+          double xu, xv, xw;
+          fval3(gi, gj, kk, o, xu, xv, xw);
+          su[slot][pj][pi] = xu;
+          sv[slot][pj][pi] = xv;
+          sw[slot][pj][pi] = xw;
+        } else {
+          su[slot][pj][pi] = a.u[o];
+          sv[slot][pj][pi] = a.v[o];
+          sw[slot][pj][pi] = a.w[o];
+        }
       }
     }
   };
@@ -106,12 +185,12 @@ __global__ void __launch_bounds__(MOM_TX* MOM_TY) k_mom_rk(MomArgs<S> a) {
   double u_ccm = 0., v_ccm = 0., w_ccm = 0., w_pcm = 0., w_cpm = 0., p_c = 0.;
   if (in) {
     const long c0 = L.at(i, j, kb);
-    u_ccm = a.u[c0 - sz];
-    v_ccm = a.v[c0 - sz];
-    w_ccm = a.w[c0 - sz];
-    w_pcm = a.w[c0 - sz + 1];
-    w_cpm = a.w[c0 - sz + sy];
-    p_c = (double)a.p[c0];
+    u_ccm = uval(0, i, j, kb - 1, c0 - sz);
+    v_ccm = uval(1, i, j, kb - 1, c0 - sz);
+    w_ccm = uval(2, i, j, kb - 1, c0 - sz);
+    w_pcm = uval(2, i + 1, j, kb - 1, c0 - sz + 1);
+    w_cpm = uval(2, i, j + 1, kb - 1, c0 - sz + sy);
+    p_c = pval(c0);
   }
   // plane k+2 is loaded into registers before plane k's arithmetic and stored to its slot
   // after it (the slot plane k-1 used, free since the previous barrier), so the loads'
@@ -125,9 +204,13 @@ __global__ void __launch_bounds__(MOM_TX* MOM_TY) k_mom_rk(MomArgs<S> a) {
       const int gi = i0 + q % MOM_PX, gj = j0 + q / MOM_PX;
       if (q < MOM_PX * MOM_PY && gi <= L.n1 + 1 && gj <= L.n2 + 1) {
         const long o = L.at(gi, gj, kk);
-        nu[r] = a.u[o];
-        nv[r] = a.v[o];
-        nw[r] = a.w[o];
+        if constexpr (FOLD) {
+          fval3(gi, gj, kk, o, nu[r], nv[r], nw[r]);
+        } else {
+          nu[r] = a.u[o];
+          nv[r] = a.v[o];
+          nw[r] = a.w[o];
+        }
       }
     }
   };
@@ -235,7 +318,7 @@ __global__ void __launch_bounds__(MOM_TX* MOM_TY) k_mom_rk(MomArgs<S> a) {
       // RK update (rk.f90:156-186) and RHS_old <- RHS (rk.f90:266-276).  With rkpar(2) = 0
       // (first substep) the RHS_old term is factor2*RHS_old = 0 and is not read.
       const long o = (long)(i - 1) + (long)L.n1 * ((j - 1) + (long)L.n2 * (k - 1));
-      const double p_ip = (double)a.p[c + 1], p_jp = (double)a.p[c + sy], p_kp = (double)a.p[c + sz];
+      const double p_ip = pval(c + 1), p_jp = pval(c + sy), p_kp = pval(c + sz);
       const double ru = a.read_ro ? (double)a.ru[o] : 0., rv = a.read_ro ? (double)a.rv[o] : 0.,
                    rw = a.read_ro ? (double)a.rw[o] : 0.;
       double un = a.read_ro ? u_ccc + a.factor1 * dudt + a.factor2 * ru +
@@ -264,7 +347,11 @@ __global__ void __launch_bounds__(MOM_TX* MOM_TY) k_mom_rk(MomArgs<S> a) {
         vout = vn - a.hf12 * dvdtd;
         wout = wn - a.hf12 * dwdtd;
       }
-      if constexpr (DELTA) {
+      if constexpr (DELTA && FOLD) {  // increments over the step's initial velocity
+        a.eu[c] = (S)(uout - a.u[c]);
+        a.ev[c] = (S)(vout - a.v[c]);
+        a.ew[c] = (S)(wout - a.w[c]);
+      } else if constexpr (DELTA) {
         a.eu[c] = (S)(uout - u_ccc);
         a.ev[c] = (S)(vout - v_ccc);
         a.ew[c] = (S)(wout - w_ccc);
@@ -278,6 +365,7 @@ __global__ void __launch_bounds__(MOM_TX* MOM_TY) k_mom_rk(MomArgs<S> a) {
         a.rv[o] = (S)dvdt;
         a.rw[o] = (S)dwdt;
       }
+      if constexpr (FOLD) a.pout[c] = (S)p_c;  // updatep: p + pp
       if constexpr (DIVU)  // fillps.f90:38-50 association, without the 1/dt factor
         a.divu[o] = (S)((w_ccc - w_ccm) * dzfi_k + (v_ccc - v_cmc) * dyi + (u_ccc - u_mcc) * dxi);
       if (a.sum_mask) {
@@ -300,6 +388,86 @@ __global__ void __launch_bounds__(MOM_TX* MOM_TY) k_mom_rk(MomArgs<S> a) {
     __syncthreads();
     block_sum3<MOM_TX * MOM_TY>(s0, s1, s2, a.partials + blockIdx.x, a.partials + nb + blockIdx.x,
                                 a.partials + 2 * nb + blockIdx.x);
+  }
+}
+
+// The projected velocity of a folded substep (MomFold) at the ghost cells, the six face planes
+// of the halo'd box, with the rules k_mom_rk<FOLD> states: each ghost's value is correc's at a
+// source cell (k_correc's expression, including its copy branch at index n+1), then the wall
+// rules of the directions that have one, in x, y, z order (bounduvw applies them direction by
+// direction).  Periodic: the ghosts bounduvw copies after correc (along x every tangential
+// ghost and the normal component's upper one, is_bound on one GPU; along y and z the normal
+// component's upper one, the only exchanged halo) take their periodic image, every other
+// ghost keeps correc's own value; walls: a tangential ghost is factor + sgn * (the adjacent
+// value), the normal component keeps correc's value at the ghost (keep_norm).
+template <typename S>
+struct FoldGhostArgs {
+  Layout L;
+  const double *u, *v, *w;
+  const S *eu, *ev, *ew, *pp;
+  const double *f, *dzci;
+  double dt, dtdxi, dtdyi;
+  int per;  // bit d: direction d periodic
+  double factor[3][3][2], sgn[3][3][2];
+  double *gu, *gv, *gw;
+};
+
+template <typename S>
+__global__ void __launch_bounds__(256) k_fold_ghosts(FoldGhostArgs<S> a) {
+  const Layout& L = a.L;
+  const int face = blockIdx.y, dir = face >> 1, side = face & 1;
+  const int e1 = dir == 0 ? L.n2 + 2 : L.n1 + 2;
+  const int e2 = dir == 2 ? L.n2 + 2 : L.n3 + 2;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)e1 * e2) return;
+  const int pa = (int)(idx % e1), pb = (int)(idx / e1);
+  int gi, gj, gk;
+  if (dir == 0) gi = side ? L.n1 + 1 : 0, gj = pa, gk = pb;
+  else if (dir == 1) gi = pa, gj = side ? L.n2 + 1 : 0, gk = pb;
+  else gi = pa, gj = pb, gk = side ? L.n3 + 1 : 0;
+  const long go = L.at(gi, gj, gk);
+  for (int cm = 0; cm < 3; cm++) {
+    int si = gi, sj = gj, sk = gk;
+    bool ox = false, oy = false, oz = false;
+    int xs = 0, ys = 0, zs = 0;
+    if (gi < 1 || gi > L.n1) {
+      xs = gi > L.n1;
+      if (a.per & 1) {
+        if (cm != 0 || xs) si = xs ? 1 : L.n1;
+      } else if (cm != 0) {
+        si = xs ? L.n1 : 1, ox = true;
+      }
+    }
+    if (gj < 1 || gj > L.n2) {
+      ys = gj > L.n2;
+      if (a.per & 2) {
+        if (cm == 1 && ys) sj = 1;
+      } else if (cm != 1) {
+        sj = ys ? L.n2 : 1, oy = true;
+      }
+    }
+    if (gk < 1 || gk > L.n3) {
+      zs = gk > L.n3;
+      if (a.per & 4) {
+        if (cm == 2 && zs) sk = 1;
+      } else if (cm != 2) {
+        sk = zs ? L.n3 : 1, oz = true;
+      }
+    }
+    const long o = L.at(si, sj, sk);
+    const double* b = cm == 0 ? a.u : cm == 1 ? a.v : a.w;
+    const S* e = cm == 0 ? a.eu : cm == 1 ? a.ev : a.ew;
+    const double fc = a.f[cm];
+    const double tu = e ? b[o] + (double)e[o] + fc : b[o] + fc;
+    const double pc = (double)a.pp[o];
+    double x;
+    if (cm == 0) x = si <= L.n1 ? tu - a.dtdxi * ((double)a.pp[o + 1] - pc) : tu;
+    else if (cm == 1) x = sj <= L.n2 ? tu - a.dtdyi * ((double)a.pp[o + L.sy] - pc) : tu;
+    else x = sk <= L.n3 ? tu - a.dt * a.dzci[sk] * ((double)a.pp[o + L.sz] - pc) : tu;
+    if (ox) x = a.factor[cm][0][xs] + a.sgn[cm][0][xs] * x;
+    if (oy) x = a.factor[cm][1][ys] + a.sgn[cm][1][ys] * x;
+    if (oz) x = a.factor[cm][2][zs] + a.sgn[cm][2][zs] * x;
+    (cm == 0 ? a.gu : cm == 1 ? a.gv : a.gw)[go] = x;
   }
 }
 
@@ -760,6 +928,55 @@ int launch_mom_rk(cans_ctx* c, const MomLaunch<S>& m) {
   a.imp = m.imp;
   a.hf12 = 0.5 * a.factor12;
   const dim3 blk(MOM_TX, MOM_TY);
+  if (m.fold) {
+    const MomFold<S>& f = *m.fold;
+    if (a.divu || a.sb || a.imp || !f.pp || !f.f || !f.pout)
+      return fail(CANS_EINVAL, "k_mom_rk: the folded projection excludes div(u), buoyancy and implicit diffusion");
+    a.fpp = f.pp;
+    a.feu = f.eu; a.fev = f.ev; a.few = f.ew;
+    a.ff = f.f;
+    a.fdt = f.dt;
+    a.fdtdxi = f.dt * m.dxi;  // correc's factori = dt*dli(1)
+    a.fdtdyi = f.dt * m.dyi;
+    a.pout = f.pout;
+    // the ghost cells' projected values, three halo'd FP64 fields of which only the faces are used
+    void* gb = nullptr;
+    const size_t fe = (size_t)L.sz * (L.n3 + 2);
+    if ((rc = ensure_work(c, 10, 3 * fe * sizeof(double), &gb))) return rc;
+    FoldGhostArgs<S> g;
+    g.L = L;
+    g.u = m.u; g.v = m.v; g.w = m.w;
+    g.eu = f.eu; g.ev = f.ev; g.ew = f.ew;
+    g.pp = f.pp;
+    g.f = f.f;
+    g.dzci = m.dzci;
+    g.dt = f.dt;
+    g.dtdxi = a.fdtdxi;
+    g.dtdyi = a.fdtdyi;
+    g.per = (f.periodic[0] ? 1 : 0) | (f.periodic[1] ? 2 : 0) | (f.periodic[2] ? 4 : 0);
+    for (int q = 0; q < 3; q++)
+      for (int d = 0; d < 3; d++)
+        for (int sd = 0; sd < 2; sd++) {
+          g.factor[q][d][sd] = f.factor[q][d][sd];
+          g.sgn[q][d][sd] = f.sgn[q][d][sd];
+        }
+    g.gu = (double*)gb;
+    g.gv = g.gu + fe;
+    g.gw = g.gv + fe;
+    a.gu = g.gu; a.gv = g.gv; a.gw = g.gw;
+    const long emax = (long)std::max(L.n1, L.n2) + 2;
+    const long plane = emax * (std::max(L.n2, L.n3) + 2);
+    k_fold_ghosts<S><<<dim3(cdiv(plane, 256), 6), 256, 0, c->stream>>>(g);
+    CANS_LAUNCH_CHECK("k_fold_ghosts");
+    if (a.delta) k_mom_rk<S, true, false, false, true><<<grid, blk, 0, c->stream>>>(a);
+    else k_mom_rk<S, false, false, false, true><<<grid, blk, 0, c->stream>>>(a);
+    CANS_LAUNCH_CHECK("k_mom_rk");
+    if (a.sum_mask)
+      return reduce_partials(c, c->partials, (int)nb, 3, 0, c->dscal, c->partials + 3 * nb, 1, a.sum_mask, m.velf);
+    k_fill<<<1, 64, 0, c->stream>>>(c->dscal, 3, 0.);
+    CANS_LAUNCH_CHECK("k_fill");
+    return CANS_OK;
+  }
   const int form = (a.delta ? 4 : 0) | (a.divu ? 2 : 0) | (a.sb ? 1 : 0);
   switch (form) {
     case 0: k_mom_rk<S, false, false, false><<<grid, blk, 0, c->stream>>>(a); break;

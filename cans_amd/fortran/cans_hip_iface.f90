@@ -358,6 +358,14 @@ module cans_hip
       real(c_double), value :: dt
       type(cans_state), intent(inout) :: st
     end function
+    ! the whole do irk = 1,3 loop body of main.f90 (projections folded into the next momentum kernel)
+    integer(c_int) function cans_rk3_step(ctx, prm, dt, st) bind(C, name='cans_rk3_step')
+      import :: c_int, c_ptr, c_double, cans_step_params, cans_state
+      type(c_ptr), value :: ctx
+      type(cans_step_params), intent(in) :: prm
+      real(c_double), value :: dt
+      type(cans_state), intent(inout) :: st
+    end function
     ! device address of the bulk forcing f(1:3) of the last rk (no counterpart: rk.f90 keeps it on the host)
     integer(c_int) function cans_ctx_forcing(ctx, f_d) bind(C, name='cans_ctx_forcing')
       import :: c_int, c_ptr

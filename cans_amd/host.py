@@ -595,6 +595,11 @@ class Simulation:
                  "cans_rk3_substep")
 
     def rk3_step(self, dt):
+        """cans_rk3_step: one whole step (the substeps' projections folded where that applies)."""
+        LB.check(LB.lib().cans_rk3_step(self.ctx.h, C.byref(self.prm), dt, C.byref(self.st)), "cans_rk3_step")
+
+    def rk3_step_substeps(self, dt):
+        """The same step as three cans_rk3_substep calls."""
         for irk in (1, 2, 3):
             self.rk3_substep(irk, dt)
 

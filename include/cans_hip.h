@@ -340,6 +340,14 @@ typedef struct cans_state {
  * 0) and substep 3 does not write it (the next step's substep 1 overwrites it before
  * any use), so after substep 3 dudtrko still holds substep 2's right-hand side. */
 int cans_rk3_substep(cans_ctx *ctx, const cans_step_params *prm, int irk, double dt, cans_state *st);
+/* One whole RK3 step, the loop body of src/main.f90:437-473 (do irk = 1,3): the three
+ * substeps of cans_rk3_substep with the same results, bit for bit.  Where diffusion is
+ * explicit, there are no scalars, the pressure boundary values are homogeneous and
+ * substeps 1 and 2 are not refined, each substep's projection (correc, bounduvw, updatep,
+ * boundp) is folded into the next substep's momentum kernel, which forms the projected
+ * velocity and p + pp while it stages them, so substeps 1 and 2 write no corrected
+ * velocity; the context keeps that state between the substeps in its own scratch. */
+int cans_rk3_step(cans_ctx *ctx, const cans_step_params *prm, double dt, cans_state *st);
 /* forcing of the last substep, device pointer to 3 doubles (f(1:3) of rk.f90) */
 int cans_ctx_forcing(cans_ctx *ctx, const double **f_d);
 
