@@ -188,6 +188,33 @@ def run(p, init, precision, args, sweeps=0, icheck=1, solves=True):
     return res, final
 
 
+def solve_only(p, precision, args):
+    """args.solves Poisson solves of the channel's pressure system on a seeded rhs, timed with
+    HIP events on the context stream (the same figure as the step runs' "poisson")."""
+    import numpy as np
+    import torch
+    from cans_amd.host import Simulation
+    from tests import cases
+    sim = Simulation(cases.host_case(p), precision=precision)
+    sim.pp.buf.uniform_(-1, 1)
+    sim.solver()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.solves):
+        sim.solver()
+    e1.record()
+    torch.cuda.synchronize()
+    t_p = e0.elapsed_time(e1) / args.solves * 1e-3
+    b_p = 5 * 2 * sim.ctx.p_es * int(np.prod(p["ng"]))
+    sim.close()
+    del sim
+    torch.cuda.empty_cache()
+    gbs = b_p / t_p / 1e9
+    print("[bench] solve-only: %.3f ms (%.0f GB/s)" % (t_p * 1e3, gbs), file=sys.stderr, flush=True)
+    return {"ms": t_p * 1e3, "GBps": gbs, "pct_hbm_peak": 100.0 * gbs / HBM_PEAK_GBS, "algorithmic_bytes": b_p}
+
+
 def secondary_cavity(args, precision="fp64"):
     """A second workload beside the headline: the 256^3 heated cube (cases.params("dhc3d")) with no-slip
     walls on all faces, a scalar with buoyancy and 3-D implicit diffusion (is_impdiff: per substep the
@@ -371,6 +398,9 @@ def main():
     ap.add_argument("--boxes-only", action="store_true", help="time only the 64 Poisson boxes (for profiling)")
     ap.add_argument("--adjoint-only", action="store_true", help="time only the 256^3 adjoint (for profiling)")
     ap.add_argument("--cavity-only", action="store_true", help="time only the 256^3 heated cube (for profiling)")
+    ap.add_argument("--solve-only", action="store_true",
+                    help="time only the channel's Poisson solves, FP32 storage then FP64 (for profiling: after each "
+                         "mode's set-up, the last dispatches of every pass kernel are the timed solves)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs (--gpus %d) are not part of it" % args.gpus,
@@ -389,6 +419,10 @@ def main():
     from tests import cases
 
     p = cases.params("channel", ng=tuple(args.ng))
+    if args.solve_only:
+        print(json.dumps({"poisson": {m: solve_only(p, prec, args) for m, prec in
+                                      (("mixed", LB.CANS_MIXED), ("fp64", LB.CANS_FP64))}}))
+        return
     ncell = int(np.prod(p["ng"]))
     grid = Grid(cases.host_case(p))
     # one seeded state, FP32-representable so that both modes start from the same values
