@@ -250,14 +250,35 @@ def secondary_cavity(args, precision="fp64"):
     ms = e0.elapsed_time(e1) / nst
     _, divmax = sim.chkdiv()
     ncell = int(np.prod(p["ng"]))
+    # the three velocity Helmholtz systems of one substep (alpha = -visc*dtrk/2), their x and y
+    # transforms differing along each component's wall normal: as one batch (one launch per pass
+    # for all three) and one cans_solve_helmholtz call at a time
+    fields = [sim.ctx.field() for _ in range(3)]
+    for f, q in zip(fields, (sim.u, sim.v, sim.w)):
+        f.buf.copy_(q.buf)
+    alpha = -0.5 * sim.grid.visc * (32. / 60.) * dt
+    helm = {}
+    for form, fn in (("batch", lambda: sim.solve_helmholtz_batch(fields, sim.hplans, [alpha] * 3)),
+                     ("one_at_a_time", lambda: [hp.solve_helmholtz(f, alpha) for f, hp in zip(fields, sim.hplans)])):
+        fn()
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(nst):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        helm["ms_" + form] = e0.elapsed_time(e1) / nst
+    del fields
     sim.close()
     del sim
     torch.cuda.empty_cache()
-    print("[bench] heated cube %s: %.3f ms/step" % (precision, ms), file=sys.stderr, flush=True)
+    print("[bench] heated cube %s: %.3f ms/step; velocity Helmholtz batch %.3f ms, one at a time %.3f ms"
+          % (precision, ms, helm["ms_batch"], helm["ms_one_at_a_time"]), file=sys.stderr, flush=True)
     return {"workload": "heated cube %dx%dx%d, no-slip walls, scalar with buoyancy, 3-D implicit diffusion "
                         "(is_impdiff; velocity Helmholtz with face-centred DST-I along the wall normal), "
                         "DCT x DCT pressure, dt=1e-3, %s" % (tuple(p["ng"]) + (precision,)),
-            "ms_per_step": ms, "ns_per_cell_step": ms * 1e-3 / ncell * 1e9, "steps": nst, "divmax": divmax}
+            "ms_per_step": ms, "ns_per_cell_step": ms * 1e-3 / ncell * 1e9, "steps": nst, "divmax": divmax,
+            "velocity_helmholtz": helm}
 
 
 def secondary_boxes(args):
@@ -276,14 +297,14 @@ def secondary_boxes(args):
     big = tuple(max(b["n"][d] for b in boxes) for d in range(3))
     nrep = max(5, args.solves)
 
-    def timed(fn):
+    def timed(fn, st=None):  # events on st (the stream the work runs on), or torch's current one
         fn()
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
+        e0.record(st)
         for _ in range(nrep):
             fn()
-        e1.record()
+        e1.record(st)
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / nrep
 
@@ -292,15 +313,25 @@ def secondary_boxes(args):
     for name, prec, es in (("fp64", LB.CANS_FP64, 8), ("mixed", LB.CANS_MIXED, 4)):
         print("[bench] %d Poisson boxes (%d cells), %s: set-up" % (len(boxes), ncell, name), file=sys.stderr,
               flush=True)
-        ctx = Context(big, precision=prec)
+        stream = torch.cuda.Stream()  # the context's own stream, so that a solve can be captured
+        ctx = Context(big, precision=prec, stream=stream)
         batch = PoissonBatch(ctx, boxes)
         rng = np.random.default_rng(6)
         for i, b in enumerate(boxes):
             batch.set(i, IF.halo(b["n"], cases.box_rhs(b, 1 / batch.dzfi[i][1:-1], rng)))
-        ms = timed(batch.solve)
+        torch.cuda.synchronize()  # the copies ran on torch's stream, the solves run on the context's
+        ms = timed(batch.solve, stream)
         nbytes = 10 * es * ncell  # 5 passes x (s_in + s_out) per cell
         out[name] = {"ms_per_batch_solve": ms, "algorithmic_bytes": nbytes, "GBps": nbytes / (ms * 1e-3) / 1e9,
                      "frac_hbm_peak": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "threads": batch.threads()}
+        # the same solve captured once in a HIP graph (its five launches as one replay)
+        g = torch.cuda.CUDAGraph()
+        torch.cuda.synchronize()
+        with torch.cuda.graph(g, stream=stream):
+            batch.solve()
+        # replay() launches the graph on torch's current stream: time it there
+        out[name]["ms_per_batch_solve_graph_replay"] = timed(g.replay)
+        del g
         batch.close()
         del batch
         if name == "fp64":  # the same boxes, one cans_solver call each

@@ -98,6 +98,7 @@ SIGNATURES = {
     "cans_poisson_batch_destroy": (C.c_int, [_vp]),
     "cans_solver_refined": (C.c_int, [_vp, _ip, _ip, _vp, C.c_double, _vp, _vp, _vp, _vp, _cp, _cp, _dp, _vp, _vp,
                                       C.c_int, _vp]),
+    "cans_solve_helmholtz_batch": (C.c_int, [_vp, C.c_int, C.POINTER(Helmholtz), _dp, _ip, C.POINTER(_vp)]),
     "cans_solve_helmholtz": (C.c_int, [_vp, _ip, _ip, _vp, C.c_double, C.c_double, _vp, _vp, _vp, _vp,
                                        _vp, _vp, _vp, _ip, _cp, _cp, _vp]),
     "cans_rk": (C.c_int, [_vp, _dp, _ip, _dp, _vp, _vp, _vp, _vp, C.c_double, C.c_double, _vp, _ip, _dp,

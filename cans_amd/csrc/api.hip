@@ -442,6 +442,27 @@ extern "C" int cans_solve_helmholtz(cans_ctx* c, const int* n, const int* ng, co
   return run_helmholtz_3d<float>(c, plan, normfft, alpha, lam, a, b, cc, cbc, c_or_f, (float*)p);
 }
 
+static int helmholtz(cans_ctx* c, int n, const cans_helmholtz* const* h, const int* is_bound, const double* alpha,
+                     double* const* f);
+
+extern "C" int cans_solve_helmholtz_batch(cans_ctx* c, int nsys, const cans_helmholtz* h, const double* alpha,
+                                          const int* is_bound, void* const* f) {
+  CHECK_CTX("cans_solve_helmholtz_batch");
+  if (nsys < 1 || nsys > CANS_MAX_SCALARS + 3 || !h || !alpha || !is_bound || !f)
+    return fail(CANS_EINVAL, "cans_solve_helmholtz_batch: bad argument");
+  const cans_helmholtz* hp[CANS_MAX_SCALARS + 3];
+  double* fp[CANS_MAX_SCALARS + 3];
+  for (int q = 0; q < nsys; q++) {
+    if (!f[q] || !h[q].a_d || !h[q].b_d || !h[q].c_d)
+      return fail(CANS_EINVAL, "cans_solve_helmholtz_batch: null field or coefficients");
+    if (field_elem_size(c, f[q]) == (int)sizeof(float))
+      return fail(CANS_EINVAL, "cans_solve_helmholtz_batch: the fields are FP64");
+    hp[q] = &h[q];
+    fp[q] = (double*)f[q];
+  }
+  return helmholtz(c, nsys, hp, is_bound, alpha, fp);
+}
+
 extern "C" int cans_chkdiv(cans_ctx* c, const int* lo, const int* hi, const double* l, const double* dli,
                            const double* dzfi, const void* u, const void* v, const void* w, double* divtot,
                            double* divmax) {

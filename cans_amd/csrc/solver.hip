@@ -821,6 +821,70 @@ void launch_ypass(dim3 g, int t, size_t shm, hipStream_t st, const YArgs<T>& a) 
 }
 
 // ---------------------------------------------------------------------------
+// x and y passes of a batch whose systems use different transforms (the velocity
+// components' Helmholtz systems next to walls: face-centred DST-I along each component's
+// normal, cell-centred DST-II along the others).  blockIdx.y selects the system; each
+// system's work-groups run the single-system pass body with that system's line length and
+// kind, as many teams (x) or lines (y) as fit the common work-group size, the lanes left
+// over joining only the barriers; work-groups past a system's own count return at once.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct HetArgs {
+  Layout L;
+  T* fld[NBATCH];
+  T* W[NBATCH];
+  int N[NBATCH], nd[NBATCH], logN[NBATCH], odd[NBATCH], Tn[NBATCH], kind[NBATCH];
+  int per[NBATCH];     // x: complex lines (teams) per work-group; y: lines per work-group
+  int blocks[NBATCH];  // work-groups of the system
+  const cplx<double>* tw[NBATCH];
+  const cplx<double>* mk[NBATCH];
+};
+
+template <int DIR, typename T>
+__global__ void __launch_bounds__(512) k_het_xpass(HetArgs<T> h) {
+  const int s = blockIdx.y;
+  if ((int)blockIdx.x >= h.blocks[s]) return;
+  XArgs<T> a;
+  a.L = h.L;
+  a.fld[0] = h.fld[s];
+  a.W[0] = h.W[s];
+  a.N = h.N[s];
+  a.nd = h.nd[s];
+  a.logN = h.logN[s];
+  a.odd = h.odd[s];
+  a.Tn = h.Tn[s];
+  a.pairs = h.per[s];
+  a.kind = h.kind[s];
+  a.tw = h.tw[s];
+  a.mk = h.mk[s];
+  a.div = 0;
+  if (a.kind == K_DNF) xpass_body<DIR, T, 0, true>(a, blockIdx.x, 0);
+  else xpass_body<DIR, T, 0, false>(a, blockIdx.x, 0);
+}
+
+template <int DIR, typename T>
+__global__ void __launch_bounds__(1024) k_het_ypass(HetArgs<T> h) {
+  const int s = blockIdx.y;
+  if ((int)blockIdx.x >= h.blocks[s]) return;
+  YArgs<T> a;
+  a.n1 = h.L.n1;
+  a.n2 = h.L.n2;
+  a.n3 = h.L.n3;
+  a.W[0] = h.W[s];
+  a.N = h.N[s];
+  a.nd = h.nd[s];
+  a.logN = h.logN[s];
+  a.odd = h.odd[s];
+  a.Tn = h.Tn[s];
+  a.lines = h.per[s];
+  a.kind = h.kind[s];
+  a.tw = h.tw[s];
+  a.mk = h.mk[s];
+  if (a.kind == K_DNF) ypass_body<DIR, T, true>(a, blockIdx.x, 0);
+  else ypass_body<DIR, T, false>(a, blockIdx.x, 0);
+}
+
+// ---------------------------------------------------------------------------
 // pass Z: Thomas elimination per (kx,ky) line, gaussel (solver.f90:114-307)
 // with identical arithmetic.  The backward sweep needs d(k) and the forward-
 // modified rhs in reverse order; instead of storing them for every k
@@ -1440,6 +1504,72 @@ static int ypass(cans_ctx* c, const cans_plan* p, T* const* W, int nsys, int dir
   return CANS_OK;
 }
 
+// one x or y pass (d = 0, 1) of nsys systems with their own plans (HetArgs)
+template <typename T>
+static int het_pass(cans_ctx* c, const cans_plan* const* ps, T* const* fld, T* const* W, int nsys, int d, int dir) {
+  HetArgs<T> h;
+  h.L = c->L;
+  const long nrows = (long)c->L.n2 * c->L.n3;
+  int threads = 0, bmax = 0;
+  size_t shm = 0;
+  for (int q = 0; q < nsys; q++) {
+    const cans_plan* p = ps[q];
+    h.fld[q] = fld ? fld[q] : nullptr;
+    h.W[q] = W[q];
+    h.N[q] = p->nfft[d];
+    h.nd[q] = p->len[d];
+    h.logN[q] = p->logn[d];
+    h.odd[q] = p->odd[d];
+    h.Tn[q] = h.N[q] >= 8 ? h.N[q] / 8 : 1;
+    h.kind[q] = p->kind[d];
+    h.tw[q] = p->tw[d];
+    h.mk[q] = p->mk[d];
+  }
+  int tmax = 1;
+  for (int q = 0; q < nsys; q++) tmax = std::max(tmax, h.Tn[q]);
+  if (d == 0) {
+    const int xt = std::max(256, tmax);  // lanes per work-group: whole teams of every system
+    for (int q = 0; q < nsys; q++) {
+      h.per[q] = std::max(1, xt / h.Tn[q]);
+      h.blocks[q] = (int)cdiv((nrows + 1) / 2, h.per[q]);
+      threads = std::max(threads, h.per[q] * h.Tn[q]);
+      shm = std::max(shm, (size_t)h.per[q] * padded_len(h.N[q]) * sizeof(cplx<T>));
+      bmax = std::max(bmax, h.blocks[q]);
+    }
+    const dim3 g(bmax, nsys);
+    if (dir > 0) k_het_xpass<1, T><<<g, threads, shm, c->stream>>>(h);
+    else k_het_xpass<-1, T><<<g, threads, shm, c->stream>>>(h);
+    CANS_LAUNCH_CHECK("k_het_xpass");
+    return CANS_OK;
+  }
+  // y: every system's work-group holds yt lanes, yt / Tn lines of its length (the same LDS for
+  // all lengths of 8 or more), and at least 16 lines in FP32 (128-byte rows)
+  int yt = std::max(256, tmax);
+  for (int q = 0; q < nsys; q++)
+    while ((long)(yt / h.Tn[q]) * 2 * (long)sizeof(T) < 128 && yt < 1024) yt *= 2;
+  const int ncp = (c->L.n1 + 1) / 2;
+  for (int q = 0; q < nsys; q++) {
+    h.per[q] = std::max(1, yt / h.Tn[q]);
+    h.blocks[q] = (int)cdiv((long)c->L.n3 * ncp, h.per[q]);
+    threads = std::max(threads, h.per[q] * h.Tn[q]);
+    shm = std::max(shm, (size_t)h.per[q] * h.N[q] * sizeof(cplx<T>));
+    bmax = std::max(bmax, h.blocks[q]);
+  }
+  if (threads > 1024) return fail(CANS_EUNSUPPORTED, "k_het_ypass: more than 1024 lanes per work-group");
+  const dim3 g(bmax, nsys);
+  if (dir > 0) k_het_ypass<1, T><<<g, threads, shm, c->stream>>>(h);
+  else k_het_ypass<-1, T><<<g, threads, shm, c->stream>>>(h);
+  CANS_LAUNCH_CHECK("k_het_ypass");
+  return CANS_OK;
+}
+
+// do the systems' plans share direction d's transform (kind and length)?
+static bool same_transform(const cans_plan* const* ps, int nsys, int d) {
+  for (int q = 1; q < nsys; q++)
+    if (ps[q]->kind[d] != ps[0]->kind[d] || ps[q]->len[d] != ps[0]->len[d]) return false;
+  return true;
+}
+
 // Solve of nsys systems that share the x and y transforms (the plan p): each pass
 // is one launch for all of them (blockIdx.y = system).  The z systems may differ
 // (coefficients, c/f grid, normalisation); their z boundary pairs must agree on
@@ -1461,7 +1591,7 @@ size_t solver_work_bytes(const cans_ctx* c, size_t es, bool periodic_z) {
 
 template <typename T>
 int run_solver_batch(cans_ctx* c, const cans_plan* p, int nsys, const SolveSys<T>* sy, const DivSrc<T>* div,
-                     const SolveWork* ws = nullptr) {
+                     const SolveWork* ws = nullptr, const cans_plan* const* ps = nullptr) {
   const Layout& L = c->L;
   const long nl = (long)L.n1 * L.n2;
   if (nsys < 1 || nsys > NBATCH) return fail(CANS_EINVAL, "run_solver: batch size out of range");
@@ -1497,8 +1627,11 @@ int run_solver_batch(cans_ctx* c, const cans_plan* p, int nsys, const SolveSys<T
   }
   if (div && (p->kind[0] != K_PP || p->len[0] < 2))
     return fail(CANS_EINVAL, "run_solver: the velocity-divergence input needs a periodic x transform");
-  if ((rc = xpass<T>(c, p, fld, Wq, nsys, +1, div))) return rc;
-  if ((rc = ypass<T>(c, p, Wq, nsys, +1))) return rc;
+  // systems with their own plans (ps): a direction whose transforms differ runs the heterogeneous pass
+  const bool hx = ps && !same_transform(ps, nsys, 0), hy = ps && !same_transform(ps, nsys, 1);
+  if (hx && div) return fail(CANS_EINVAL, "run_solver: the divergence input needs one x transform");
+  if ((rc = hx ? het_pass<T>(c, ps, fld, Wq, nsys, 0, +1) : xpass<T>(c, p, fld, Wq, nsys, +1, div))) return rc;
+  if ((rc = hy ? het_pass<T>(c, ps, nullptr, Wq, nsys, 1, +1) : ypass<T>(c, p, Wq, nsys, +1))) return rc;
   ZArgs<T> z;
   z.nl = nl;
   z.n1 = L.n1;
@@ -1568,8 +1701,8 @@ int run_solver_batch(cans_ctx* c, const cans_plan* p, int nsys, const SolveSys<T
     k_thomas<T, sizeof(T) == sizeof(float)><<<dim3(cdiv(nl, 256), nsys), 256, 0, c->stream>>>(z);
     CANS_LAUNCH_CHECK("k_thomas");
   }
-  if ((rc = ypass<T>(c, p, Wq, nsys, -1))) return rc;
-  if ((rc = xpass<T>(c, p, fld, Wq, nsys, -1))) return rc;
+  if ((rc = hy ? het_pass<T>(c, ps, nullptr, Wq, nsys, 1, -1) : ypass<T>(c, p, Wq, nsys, -1))) return rc;
+  if ((rc = hx ? het_pass<T>(c, ps, fld, Wq, nsys, 0, -1) : xpass<T>(c, p, fld, Wq, nsys, -1))) return rc;
   return CANS_OK;
 }
 
@@ -1706,12 +1839,12 @@ int run_helmholtz_batch(cans_ctx* c, int nsys, const HelmSys* hs) {
     CANS_LAUNCH_CHECK("k_z1_solve");
     return CANS_OK;
   }
-  bool same = true;
-  for (int q = 1; q < nsys; q++)
-    same = same && hs[q].plan && hs[0].plan && hs[q].plan->kind[0] == hs[0].plan->kind[0] &&
-           hs[q].plan->kind[1] == hs[0].plan->kind[1] && hs[q].plan->len[0] == hs[0].plan->len[0] &&
-           hs[q].plan->len[1] == hs[0].plan->len[1] && hs[q].cbc[4] != 'P';
-  if (!same || hs[0].cbc[4] == 'P' || c->dtdma > 1) {
+  // one batch (one launch per pass) unless z is periodic (the periodic sweep solves one system)
+  // or the z solve is distributed; systems whose x or y transforms differ take the
+  // heterogeneous x/y passes (HetArgs)
+  bool batch = hs[0].plan && hs[0].cbc[4] != 'P' && c->dtdma <= 1;
+  for (int q = 1; q < nsys && batch; q++) batch = hs[q].plan && hs[q].cbc[4] != 'P';
+  if (!batch) {
     for (int q = 0; q < nsys; q++) {
       int rc = run_helmholtz_3d<double>(c, hs[q].plan, hs[q].normfft, hs[q].alpha, hs[q].lam, hs[q].a, hs[q].b,
                                         hs[q].c, hs[q].cbc, hs[q].c_or_f, hs[q].fld);
@@ -1731,7 +1864,9 @@ int run_helmholtz_batch(cans_ctx* c, int nsys, const HelmSys* hs) {
     sy[q] = SolveSys<double>{hs[q].fld, hs[q].normfft * alphai, hs[q].lam, hs[q].a, bb, hs[q].c, hs[q].cbc,
                              hs[q].c_or_f};
   }
-  return run_solver_batch<double>(c, hs[0].plan, nsys, sy, nullptr);
+  const cans_plan* ps[NBATCH];
+  for (int q = 0; q < nsys; q++) ps[q] = hs[q].plan;
+  return run_solver_batch<double>(c, hs[0].plan, nsys, sy, nullptr, nullptr, ps);
 }
 
 int run_helmholtz_z(cans_ctx* c, const double* a, const double* b, const double* cc, double alpha, const char* bcz,

@@ -247,6 +247,17 @@ module cans_hip
       real(c_double), value :: normfft, alpha
       character(kind=c_char), intent(in) :: cbc(6), c_or_f(3)
     end function
+    ! several solve_helmholtz calls (main.f90:453-462) as one batch: one launch per transform pass
+    integer(c_int) function cans_solve_helmholtz_batch(ctx, nsys, h, alpha, is_bound, f) &
+                   bind(C, name='cans_solve_helmholtz_batch')
+      import :: c_int, c_ptr, c_double, cans_helmholtz
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nsys
+      type(cans_helmholtz), intent(in) :: h(*)
+      real(c_double), intent(in) :: alpha(*)
+      integer(c_int), intent(in) :: is_bound(6)
+      type(c_ptr), intent(in) :: f(*)
+    end function
     ! s: the first scalar's field with Boussinesq buoyancy (rk.f90:69-72), else c_null_ptr
     integer(c_int) function cans_rk(ctx, rkpar, n, dli, dzci, dzfi, grid_vol_ratio_c, grid_vol_ratio_f, dt, visc, &
                                     p, is_forced, velf, bforce, gacc, beta, s, dudtrko, dvdtrko, dwdtrko, &

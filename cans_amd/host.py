@@ -598,6 +598,18 @@ class Simulation:
         """cans_rk3_step: one whole step (the substeps' projections folded where that applies)."""
         LB.check(LB.lib().cans_rk3_step(self.ctx.h, C.byref(self.prm), dt, C.byref(self.st)), "cans_rk3_step")
 
+    def solve_helmholtz_batch(self, fields, plans, alphas):
+        """cans_solve_helmholtz_batch: solve_helmholtz (solve_helmholtz.f90:28-75) of each FP64
+        DeviceField with its Plan and alpha, as one batch."""
+        n = len(fields)
+        hs = (LB.Helmholtz * n)()
+        for q, pl in enumerate(plans):
+            pl.fill(hs[q])
+        al = (C.c_double * n)(*alphas)
+        fp = (C.c_void_p * n)(*[f.ptr for f in fields])
+        LB.check(LB.lib().cans_solve_helmholtz_batch(self.ctx.h, n, hs, al, LB.ivec(self.grid.is_bound), fp),
+                 "cans_solve_helmholtz_batch")
+
     def rk3_step_substeps(self, dt):
         """The same step as three cans_rk3_substep calls."""
         for irk in (1, 2, 3):

@@ -287,6 +287,16 @@ typedef struct cans_helmholtz {
   char c_or_f[3];
 } cans_helmholtz;
 
+/* No counterpart in the reference, whose main.f90:453-462 (and 440-450 for the scalars) calls
+ * solve_helmholtz once per system: the nsys systems h[q], each with its alpha[q], on the FP64
+ * fields f_d[q] (halo'd, rhs in, solution out), each as cans_solve_helmholtz would solve it, as
+ * one batch -- every system's boundary rhs, then one launch per transform pass for up to four
+ * systems together (more run in groups of four), also when their x and y transforms differ (the
+ * velocity components next to walls); with is_impdiff_1d one z launch for all.  Systems with a
+ * periodic z, or under the distributed z solve, are solved one after another. */
+int cans_solve_helmholtz_batch(cans_ctx *ctx, int nsys, const cans_helmholtz *h, const double *alpha,
+                               const int *is_bound, void *const *f_d);
+
 typedef struct cans_step_params {
   char cbcvel[18];
   double bcvel[18];
