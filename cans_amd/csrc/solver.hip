@@ -1241,26 +1241,43 @@ __global__ void __launch_bounds__(256) k_thomas(ZArgs<T> z) {
 }
 
 // ---------------------------------------------------------------------------
-// Distributed tridiagonal solve (gaussel_dtdma, solver.f90:309-517) for a
-// non-periodic z: the rows are split into nslab consecutive slabs, as the ranks
-// of a z decomposition hold them (slab s: n/nslab rows, the first n%nslab slabs
-// one more).  On several GPUs each slab's elimination and back-substitution run
-// on its own GPU and only the 2*nslab boundary rows per line travel; on one GPU
-// the three phases show those kernels at their real shape, and the slabs give
-// nslab times more independent sweeps.  The arithmetic is the reference's; the
-// modified rhs and the aa/cc factors are kept in FP64 scratch ([row][line]).
+// Distributed tridiagonal solve (gaussel_dtdma, solver.f90:309-517): the z rows are
+// split into nslab consecutive slabs, as the ranks of a z decomposition hold them
+// (slab s: n/nslab rows, the first n%nslab slabs one more).  On several GPUs each
+// slab's elimination and back-substitution run on its own GPU and only the 2*nslab
+// boundary rows per line travel; on one GPU the three phases show those kernels at
+// their real shape, and the slabs give nslab times more independent sweeps.
+//
+// The arithmetic is the reference's: in each slab the lower diagonals are eliminated
+// downwards (every row then refers to the slab's first unknown), the upper diagonals
+// upwards (every row refers to the slab's last unknown), the first row is closed
+// against the second, and the interior follows from the two boundary unknowns,
+// x_k = p_k - aa_k x_first - cc_k x_last.  Instead of storing the eliminated
+// (p, aa, cc) of every row (three FP64 arrays written and read twice), the kernels keep
+// the elimination states only at the ends of DCK-row blocks and recompute a block's rows
+// from its checkpoint and the right-hand side, as the plain sweep does: the right-hand
+// side is read three times and the solution written once.
+//
+// Periodic z: a(1) couples row 1 to row n and c(n) row n to row 1, so the reduced
+// system of the boundary rows is cyclic; it is closed as solver.f90:469-498 does.  The
+// reference pins no zero mode here; a line whose final reduced pivot (or cyclic
+// closure) vanishes to rounding (the constant mode of an all-Neumann/periodic
+// operator) gets its last unknown pinned to zero, the gauge the plain sweep chooses.
 // ---------------------------------------------------------------------------
+constexpr int DCK = 16;  // rows per checkpoint block of the slab interiors
+
 template <typename T>
 struct DtArgs {
   T* W;
   long nl;
   int n1, n2, nic;
   long PS;
-  int n, nslab;
-  double norm;
+  int n, nslab, periodic, nbk;  // nbk: checkpoint blocks of the largest slab's interior
+  double norm, pin_eps;
   const double *a, *b, *c, *lam;
-  double *pd, *aa, *cc;  // [n][nl]
-  double *ra, *rc, *rp;  // [2*nslab][nl]
+  double *ckl, *cku;     // [nbk][3][nslab][nl]: lower / upper elimination state per block
+  double *ra, *rc, *rp;  // [2*nslab][nl]: the reduced system
+  double *rc0, *r2;      // periodic: [2*nslab][nl], the unmodified rc and the auxiliary solution
 };
 
 __device__ __forceinline__ void slab_range(int n, int nslab, int s, int& lo, int& cnt) {
@@ -1269,87 +1286,184 @@ __device__ __forceinline__ void slab_range(int n, int nslab, int s, int& lo, int
   lo = s * m + (s < r ? s : r);
 }
 
-// phase 1, one thread per (line, slab): eliminate the slab down to its first and last unknowns
+struct DtRow {
+  double p, a, c;
+};
+
+// one slab of one line: coefficients, the line's rhs rows and the checkpoint addressing
+template <typename T>
+struct DtSlab {
+  long line, nl, PS;
+  int s, nslab, lo, ns, nb, haslam;
+  double l, norm;
+  kcoef a, b, c;
+  T* P;
+  const double* ckl;
+  __device__ DtSlab(const DtArgs<T>& z, long line_, int s_) : line(line_), nl(z.nl), PS(z.PS), s(s_) {
+    nslab = z.nslab;
+    slab_range(z.n, z.nslab, s, lo, ns);
+    nb = (ns - 2 + DCK - 1) / DCK;
+    haslam = z.lam != nullptr;
+    l = haslam ? z.lam[line] : 0.;
+    norm = z.norm;
+    a = (kcoef)(z.a + lo);
+    b = (kcoef)(z.b + lo);
+    c = (kcoef)(z.c + lo);
+    P = z.W + woff((int)(line % z.n1), (int)(line / z.n1), 0, z.n2, z.nic);
+    ckl = z.ckl;
+  }
+  // lambda present: bb = b + lambda; absent: b itself (solver.f90:344-347, 395-398)
+  __device__ __forceinline__ double bb(int k) const { return haslam ? b[k] + l : b[k]; }
+  __device__ __forceinline__ T& row(int k) const { return P[zrow(lo + k, PS)]; }
+  __device__ __forceinline__ long ck(int blk, int v) const { return (((long)blk * 3 + v) * nslab + s) * nl + line; }
+  // rows 0 and 1 start the elimination (solver.f90:350-353)
+  __device__ __forceinline__ DtRow first(int k, double r) const {
+    const double zz = 1. / bb(k);
+    return {r * norm * zz, a[k] * zz, c[k] * zz};
+  }
+  // elimination of the lower diagonal of row k >= 2 (solver.f90:357-362)
+  __device__ __forceinline__ void lower(int k, double r, DtRow& L) const {
+    const double zk = 1. / (bb(k) - a[k] * L.c);
+    L.p = (r * norm - a[k] * L.p) * zk;
+    L.a = -a[k] * L.a * zk;
+    L.c = c[k] * zk;
+  }
+  // the interior block blk: rows r0 .. r0 + cnt - 1 (rows 1 .. ns-2 of the slab)
+  __device__ __forceinline__ void block(int blk, int& r0, int& cnt) const {
+    r0 = 1 + blk * DCK;
+    cnt = min(DCK, ns - 1 - r0);
+  }
+  // the rhs rows of a block; rows past the slab's interior lie inside the work buffer
+  // (at most DCK - 2 rows past row n-1) and are never used
+  __device__ __forceinline__ void load(int blk, T* rr) const {
+    const int r0 = 1 + blk * DCK;
+#pragma unroll
+    for (int t = 0; t < DCK; t++) rr[t] = row(r0 + t);
+  }
+  // the lower-elimination states of a block's rows, from the checkpoint before it
+  __device__ __forceinline__ void recompute(int blk, const T* rr, DtRow* Lv) const {
+    int r0, cnt;
+    block(blk, r0, cnt);
+    DtRow L;
+    if (blk == 0) {
+      L = first(1, (double)rr[0]);
+    } else {
+      L = {ckl[ck(blk - 1, 0)], ckl[ck(blk - 1, 1)], ckl[ck(blk - 1, 2)]};
+      lower(r0, (double)rr[0], L);
+    }
+    Lv[0] = L;
+#pragma unroll
+    for (int t = 1; t < DCK; t++) {
+      if (t < cnt) lower(r0 + t, (double)rr[t], L);
+      Lv[t] = L;
+    }
+  }
+  // elimination of the upper diagonal of row k <= ns-3 (solver.f90:366-370); row ns-2 starts it
+  __device__ __forceinline__ void upper(int k, const DtRow& L, DtRow& U) const {
+    if (k == ns - 2) {
+      U = L;
+    } else {
+      U.p = L.p - L.c * U.p;
+      U.a = L.a - L.c * U.a;
+      U.c = -L.c * U.c;
+    }
+  }
+};
+
+// phase 1, one thread per (line, slab): the slab's first and last rows of the reduced system
 template <typename T>
 __global__ void __launch_bounds__(256) k_dtdma_local(DtArgs<T> z) {
   const long line = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (line >= z.nl) return;
-  const int s = blockIdx.y;
-  int lo, ns;
-  slab_range(z.n, z.nslab, s, lo, ns);
-  const T* P = z.W + woff((int)(line % z.n1), (int)(line / z.n1), 0, z.n2, z.nic);
-  const long nl = z.nl, PS = z.PS;
-  const double l = z.lam ? z.lam[line] : 0., norm = z.norm;
-  const kcoef a = (kcoef)(z.a + lo), b = (kcoef)(z.b + lo), c = (kcoef)(z.c + lo);
-  double* pd = z.pd + (long)lo * nl + line;
-  double* aa = z.aa + (long)lo * nl + line;
-  double* cc = z.cc + (long)lo * nl + line;
-  // lambda present: bb = b + lambda; absent: b itself (solver.f90:344-347, 395-398)
-  auto bbk = [&](int k) { return z.lam ? b[k] + l : b[k]; };
-  for (int k = 0; k < 2; k++) {
-    const double zz = 1. / bbk(k);
-    aa[k * nl] = a[k] * zz;
-    cc[k * nl] = c[k] * zz;
-    pd[k * nl] = (double)P[zrow(lo + k, PS)] * norm * zz;
+  const DtSlab<T> S(z, line, blockIdx.y);
+  const int nb = S.nb, ns = S.ns;
+  const DtRow R0 = S.first(0, (double)S.row(0));
+  // downwards: lower elimination, checkpoints at the end of every interior block but the last
+  T rr[DCK], rn[DCK];
+  DtRow L = {0., 0., 0.};
+  S.load(0, rr);
+  for (int blk = 0; blk < nb; blk++) {
+    if (blk + 1 < nb) S.load(blk + 1, rn);
+    int r0, cnt;
+    S.block(blk, r0, cnt);
+#pragma unroll
+    for (int t = 0; t < DCK; t++) {
+      if (t < cnt) {
+        if (r0 + t == 1)
+          L = S.first(1, (double)rr[t]);
+        else
+          S.lower(r0 + t, (double)rr[t], L);
+      }
+    }
+    if (blk + 1 < nb) {
+      z.ckl[S.ck(blk, 0)] = L.p;
+      z.ckl[S.ck(blk, 1)] = L.a;
+      z.ckl[S.ck(blk, 2)] = L.c;
+    }
+#pragma unroll
+    for (int t = 0; t < DCK; t++) rr[t] = rn[t];
   }
-  double pprev = pd[nl], aprev = aa[nl], cprev = cc[nl];
-  for (int k = 2; k < ns; k++) {  // elimination of lower diagonals
-    const double zk = 1. / (bbk(k) - a[k] * cprev);
-    pprev = ((double)P[zrow(lo + k, PS)] * norm - a[k] * pprev) * zk;
-    aprev = -a[k] * aprev * zk;
-    cprev = c[k] * zk;
-    pd[k * nl] = pprev;
-    aa[k * nl] = aprev;
-    cc[k * nl] = cprev;
+  S.lower(ns - 1, (double)S.row(ns - 1), L);  // the last row keeps its lower-eliminated form
+  const long r1 = (long)(2 * S.s + 1) * z.nl + line;
+  z.ra[r1] = L.a;
+  z.rc[r1] = L.c;
+  z.rp[r1] = L.p;
+  // upwards: each block recomputed from its checkpoint, the upper elimination carried
+  // through it; the state entering block blk (at the first row of block blk+1) is kept
+  DtRow U = {0., 0., 0.};
+  S.load(nb - 1, rr);
+  for (int blk = nb - 1; blk >= 0; blk--) {
+    if (blk > 0) S.load(blk - 1, rn);
+    if (blk + 1 < nb) {
+      z.cku[S.ck(blk, 0)] = U.p;
+      z.cku[S.ck(blk, 1)] = U.a;
+      z.cku[S.ck(blk, 2)] = U.c;
+    }
+    DtRow Lv[DCK];
+    S.recompute(blk, rr, Lv);
+    int r0, cnt;
+    S.block(blk, r0, cnt);
+#pragma unroll
+    for (int t = DCK - 1; t >= 0; t--)
+      if (t < cnt) S.upper(r0 + t, Lv[t], U);
+#pragma unroll
+    for (int t = 0; t < DCK; t++) rr[t] = rn[t];
   }
-  // elimination of upper diagonals, k = ns-3 .. 1: pprev/aprev/cprev hold row k+1
-  pprev = pd[(long)(ns - 2) * nl];
-  aprev = aa[(long)(ns - 2) * nl];
-  cprev = cc[(long)(ns - 2) * nl];
-  for (int k = ns - 3; k >= 1; k--) {
-    const double ck = cc[(long)k * nl];
-    pprev = pd[(long)k * nl] - ck * pprev;
-    aprev = aa[(long)k * nl] - ck * aprev;
-    cprev = -ck * cprev;
-    pd[(long)k * nl] = pprev;
-    aa[(long)k * nl] = aprev;
-    cc[(long)k * nl] = cprev;
-  }
-  // first row; row 1 now holds the upper-eliminated values (pprev, aprev, cprev when ns > 3)
-  const double p1 = pd[nl], a1 = aa[nl], c1 = cc[nl];
-  const double c0 = cc[0];
-  const double zf = 1. / (1. - a1 * c0);
-  const double p0 = (pd[0] - c0 * p1) * zf;
-  const double a0 = aa[0] * zf;
-  const double cn0 = -c0 * c1 * zf;
-  pd[0] = p0;
-  aa[0] = a0;
-  cc[0] = cn0;
-  const long r0 = (long)(2 * s) * nl + line, r1 = r0 + nl;
-  z.ra[r0] = a0;
-  z.rc[r0] = cn0;
-  z.rp[r0] = p0;
-  z.ra[r1] = aa[(long)(ns - 1) * nl];
-  z.rc[r1] = cc[(long)(ns - 1) * nl];
-  z.rp[r1] = pd[(long)(ns - 1) * nl];
+  // the first row against the second (solver.f90:372-375)
+  const double zf = 1. / (1. - U.a * R0.c);
+  const long q0 = (long)(2 * S.s) * z.nl + line;
+  z.rp[q0] = (R0.p - R0.c * U.p) * zf;
+  z.ra[q0] = R0.a * zf;
+  z.rc[q0] = -R0.c * U.c * zf;
 }
 
-// phase 2, one thread per line: the reduced system of the 2*nslab boundary rows (unit diagonal)
+// phase 2, one thread per line: the reduced system of the 2*nslab boundary rows (unit
+// diagonal, solver.f90:446-468), cyclic for a periodic z (solver.f90:469-498)
 template <typename T>
 __global__ void __launch_bounds__(256) k_dtdma_reduced(DtArgs<T> z) {
   const long line = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (line >= z.nl) return;
   const long nl = z.nl;
-  const int nn = 2 * z.nslab;
+  const int nr = 2 * z.nslab;
+  const int nn = z.periodic ? nr - 1 : nr;
   double* ra = z.ra + line;
   double* rc = z.rc + line;
   double* rp = z.rp + line;
+  double* rc0 = z.periodic ? z.rc0 + line : nullptr;
+  if (z.periodic)
+    for (int k = 0; k < nr; k++) rc0[(long)k * nl] = rc[(long)k * nl];
   double pprev = rp[0], cprev = rc[0];
   for (int k = 1; k < nn; k++) {
     const double ak = ra[(long)k * nl];
-    const double zk = 1. / (1. - ak * cprev);
-    pprev = (rp[(long)k * nl] - ak * pprev) * zk;
-    cprev = rc[(long)k * nl] * zk;
+    const double den = 1. - ak * cprev;
+    if (!z.periodic && k == nn - 1 && fabs(den) <= z.pin_eps * fmax(1., fabs(ak * cprev))) {
+      pprev = 0.;  // the constant mode: pin the last unknown
+      cprev = 0.;
+    } else {
+      const double zk = 1. / den;
+      pprev = (rp[(long)k * nl] - ak * pprev) * zk;
+      cprev = rc[(long)k * nl] * zk;
+    }
     rp[(long)k * nl] = pprev;
     rc[(long)k * nl] = cprev;
   }
@@ -1357,25 +1471,68 @@ __global__ void __launch_bounds__(256) k_dtdma_reduced(DtArgs<T> z) {
     pprev = rp[(long)k * nl] - rc[(long)k * nl] * pprev;
     rp[(long)k * nl] = pprev;
   }
+  if (!z.periodic) return;
+  // the auxiliary solution for a unit last unknown, with the unmodified coefficients
+  double* r2 = z.r2 + line;
+  double qprev = -ra[0];
+  cprev = rc0[0];
+  if (nn == 1) qprev -= rc0[0];
+  r2[0] = qprev;
+  for (int k = 1; k < nn; k++) {
+    const double ak = ra[(long)k * nl];
+    const double zk = 1. / (1. - ak * cprev);
+    const double rhs = k == nn - 1 ? -rc0[(long)k * nl] : 0.;
+    qprev = (rhs - ak * qprev) * zk;
+    cprev = rc0[(long)k * nl] * zk;
+    r2[(long)k * nl] = qprev;
+    rc0[(long)k * nl] = cprev;
+  }
+  for (int k = nn - 2; k >= 0; k--) {
+    qprev = r2[(long)k * nl] - rc0[(long)k * nl] * qprev;
+    r2[(long)k * nl] = qprev;
+  }
+  // the last unknown, then the correction of the others
+  const double cl = rc[(long)nn * nl], al = ra[(long)nn * nl];  // row nn: never modified
+  const double q0 = r2[0], qn = r2[(long)(nn - 1) * nl];
+  const double den = 1. + cl * q0 + al * qn;
+  const double xl = fabs(den) <= z.pin_eps * fmax(1., fabs(cl * q0 + al * qn))
+                        ? 0.
+                        : (rp[(long)nn * nl] - cl * rp[0] - al * rp[(long)(nn - 1) * nl]) / den;
+  rp[(long)nn * nl] = xl;
+  for (int k = 0; k < nn; k++) rp[(long)k * nl] += r2[(long)k * nl] * xl;
 }
 
-// phase 3, one thread per (line, slab): boundary unknowns in, interior rows back-substituted
+// phase 3, one thread per (line, slab): the boundary unknowns in, the interior rows
+// x_k = p_k - aa_k x_first - cc_k x_last from each block's recomputed elimination (solver.f90:503-514)
 template <typename T>
 __global__ void __launch_bounds__(256) k_dtdma_back(DtArgs<T> z) {
   const long line = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (line >= z.nl) return;
-  const int s = blockIdx.y;
-  int lo, ns;
-  slab_range(z.n, z.nslab, s, lo, ns);
-  T* P = z.W + woff((int)(line % z.n1), (int)(line / z.n1), 0, z.n2, z.nic);
-  const long nl = z.nl, PS = z.PS;
-  const double p1 = z.rp[(long)(2 * s) * nl + line], pn = z.rp[(long)(2 * s + 1) * nl + line];
-  P[zrow(lo, PS)] = (T)p1;
-  P[zrow(lo + ns - 1, PS)] = (T)pn;
-  for (int k = 1; k < ns - 1; k++) {
-    const long o = (long)(lo + k) * nl + line;
-    P[zrow(lo + k, PS)] = (T)(z.pd[o] - z.aa[o] * p1 - z.cc[o] * pn);
+  const DtSlab<T> S(z, line, blockIdx.y);
+  const int nb = S.nb, ns = S.ns;
+  const double x0 = z.rp[(long)(2 * S.s) * z.nl + line], xl = z.rp[(long)(2 * S.s + 1) * z.nl + line];
+  T rr[DCK], rn[DCK];
+  S.load(0, rr);
+  for (int blk = 0; blk < nb; blk++) {
+    if (blk + 1 < nb) S.load(blk + 1, rn);
+    DtRow Lv[DCK];
+    S.recompute(blk, rr, Lv);
+    DtRow U = {0., 0., 0.};
+    if (blk + 1 < nb) U = {z.cku[S.ck(blk, 0)], z.cku[S.ck(blk, 1)], z.cku[S.ck(blk, 2)]};
+    int r0, cnt;
+    S.block(blk, r0, cnt);
+#pragma unroll
+    for (int t = DCK - 1; t >= 0; t--) {
+      if (t < cnt) {
+        S.upper(r0 + t, Lv[t], U);
+        S.row(r0 + t) = (T)(U.p - U.a * x0 - U.c * xl);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < DCK; t++) rr[t] = rn[t];
   }
+  S.row(0) = (T)x0;
+  S.row(ns - 1) = (T)xl;
 }
 
 // ---------------------------------------------------------------------------
@@ -1661,11 +1818,9 @@ int run_solver_batch(cans_ctx* c, const cans_plan* p, int nsys, const SolveSys<T
     zs.ckq = zs.ckp + nblk * nl;
   }
   z.X2 = periodic ? z.sys[0].ckq + nblk * nl : nullptr;
-  const int nn = periodic ? z.sys[0].n - 1 : z.sys[0].n;
-  if (periodic) {
-    k_gaussel<T, true><<<cdiv(nl, 256), 256, 0, c->stream>>>(z);
-    CANS_LAUNCH_CHECK("k_gaussel");
-  } else if (c->dtdma > 1) {
+  if (c->dtdma > 1) {
+    // all n rows take part (periodic too: the wrap couplings enter the cyclic reduced system)
+    const int nn = z.sys[0].n;
     if (nn / c->dtdma < 3) return fail(CANS_EINVAL, "cans_solver: distributed z solve needs 3 rows per slab");
     DtArgs<T> d;
     d.W = Wq[0];
@@ -1676,20 +1831,26 @@ int run_solver_batch(cans_ctx* c, const cans_plan* p, int nsys, const SolveSys<T
     d.PS = z.PS;
     d.n = nn;
     d.nslab = c->dtdma;
+    d.periodic = periodic;
+    const int nsmax = nn / d.nslab + (nn % d.nslab ? 1 : 0);
+    d.nbk = (nsmax - 2 + DCK - 1) / DCK;
     d.norm = sy[0].normfft;
+    d.pin_eps = 4096. * 2.220446049250313e-16;
     d.a = sy[0].a;
     d.b = sy[0].b;
     d.c = sy[0].c;
     d.lam = sy[0].lam;
     void* dt = nullptr;
-    rc = ensure_work(c, 6, ((size_t)3 * nn + (size_t)6 * d.nslab) * nl * sizeof(double), &dt);
+    const size_t ckn = (size_t)d.nbk * 3 * d.nslab * nl, rn = (size_t)2 * d.nslab * nl;
+    rc = ensure_work(c, 6, (2 * ckn + (periodic ? 5 : 3) * rn) * sizeof(double), &dt);
     if (rc) return rc;
-    d.pd = (double*)dt;
-    d.aa = d.pd + (long)nn * nl;
-    d.cc = d.aa + (long)nn * nl;
-    d.ra = d.cc + (long)nn * nl;
-    d.rc = d.ra + (long)2 * d.nslab * nl;
-    d.rp = d.rc + (long)2 * d.nslab * nl;
+    d.ckl = (double*)dt;
+    d.cku = d.ckl + ckn;
+    d.ra = d.cku + ckn;
+    d.rc = d.ra + rn;
+    d.rp = d.rc + rn;
+    d.rc0 = periodic ? d.rp + rn : nullptr;
+    d.r2 = periodic ? d.rc0 + rn : nullptr;
     const dim3 gs(cdiv(nl, 256), d.nslab);
     k_dtdma_local<T><<<gs, 256, 0, c->stream>>>(d);
     CANS_LAUNCH_CHECK("k_dtdma_local");
@@ -1697,6 +1858,9 @@ int run_solver_batch(cans_ctx* c, const cans_plan* p, int nsys, const SolveSys<T
     CANS_LAUNCH_CHECK("k_dtdma_reduced");
     k_dtdma_back<T><<<gs, 256, 0, c->stream>>>(d);
     CANS_LAUNCH_CHECK("k_dtdma_back");
+  } else if (periodic) {
+    k_gaussel<T, true><<<cdiv(nl, 256), 256, 0, c->stream>>>(z);
+    CANS_LAUNCH_CHECK("k_gaussel");
   } else {
     k_thomas<T, sizeof(T) == sizeof(float)><<<dim3(cdiv(nl, 256), nsys), 256, 0, c->stream>>>(z);
     CANS_LAUNCH_CHECK("k_thomas");

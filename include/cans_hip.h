@@ -84,8 +84,10 @@ int cans_ctx_set_numerics(cans_ctx *ctx, int is_impdiff, int is_impdiff_1d);
 /* is_poisson_dtdma (param.f90, solver.f90:309-517): the z elimination of the pressure
  * solve (and of the Helmholtz solves' 3-D form) as the distributed tridiagonal algorithm
  * with the z rows split into nslab slabs, the ranks of a z decomposition (each slab at
- * least 3 rows).  0 or 1: the plain sweep.  Non-periodic z only; like the reference's
- * gaussel_dtdma it has no zero-mode pin. */
+ * least 3 rows).  0 or 1: the plain sweep.  A periodic z closes the reduced system
+ * cyclically, as the reference does.  Unlike the reference's gaussel_dtdma, a line whose
+ * reduced system is singular (the constant mode of a Neumann/periodic operator) has its
+ * last unknown pinned to zero, the gauge of the plain sweep. */
 int cans_ctx_set_dtdma(cans_ctx *ctx, int nslab);
 /* Iterative refinement of the pressure solve inside cans_rk3_substep (no counterpart in the
  * reference, whose solves run in one precision; CANS_MIXED contexts only).  Each substep irk

@@ -275,11 +275,14 @@ void orc_gaussel(int nx, int ny, int n, const double *a, const double *b, const 
 
 /* ------------------------------------------------------------------------ */
 /* distributed tridiagonal solve: src/solver.f90:309-517 (gaussel_dtdma),     */
-/* non-periodic z, with the z rows split into nslab consecutive slabs (the    */
-/* ranks of the z decomposition; slab s has n/nslab rows, the first n%nslab   */
-/* slabs one more).  Every slab eliminates its rows down to its first and     */
-/* last unknown, the 2*nslab boundary rows form the reduced system (unit      */
-/* diagonal) solved per line, and the slab interiors are back-substituted.    */
+/* with the z rows split into nslab consecutive slabs (the ranks of the z     */
+/* decomposition; slab s has n/nslab rows, the first n%nslab slabs one        */
+/* more).  Every slab eliminates its rows down to its first and last unknown, */
+/* the 2*nslab boundary rows form the reduced system (unit diagonal) solved   */
+/* per line, and the slab interiors are back-substituted.  Periodic z: a(1)   */
+/* couples row 1 to row n and c(n) row n to row 1, so the reduced system is   */
+/* cyclic; it is closed as solver.f90:469-498 does (first 2*nslab-1 rows,     */
+/* the auxiliary solution for a unit last unknown, then the last unknown).    */
 /* ------------------------------------------------------------------------ */
 void orc_slab_range(int n, int nslab, int s, int *lo, int *cnt) {
   const int m = n / nslab, r = n % nslab;
@@ -288,9 +291,10 @@ void orc_slab_range(int n, int nslab, int s, int *lo, int *cnt) {
 }
 
 int orc_gaussel_dtdma(int nx, int ny, int n, int nslab, const double *a, const double *b,
-                      const double *c, double norm, double *p, const double *lambdaxy) {
+                      const double *c, double norm, double *p, const double *lambdaxy, int is_periodic) {
   const size_t pl = (size_t)nx * ny;
   if (nslab < 1 || n / nslab < 3) return -1;  /* the boundary-row step needs three rows per slab */
+  if (is_periodic && nslab < 2) return -1;    /* one slab's first and last rows would be one pair */
   int ok = 0;
   #pragma omp parallel
   {
@@ -300,6 +304,8 @@ int orc_gaussel_dtdma(int nx, int ny, int n, int nslab, const double *a, const d
     double *ra = (double *)malloc(sizeof(double) * 2 * (size_t)nslab);
     double *rc = (double *)malloc(sizeof(double) * 2 * (size_t)nslab);
     double *rp = (double *)malloc(sizeof(double) * 2 * (size_t)nslab);
+    double *rc0 = (double *)malloc(sizeof(double) * 2 * (size_t)nslab);
+    double *r2 = (double *)malloc(sizeof(double) * 2 * (size_t)nslab);
     #pragma omp for collapse(2) schedule(static)
     for (int j = 0; j < ny; j++)
       for (int i = 0; i < nx; i++) {
@@ -341,14 +347,29 @@ int orc_gaussel_dtdma(int nx, int ny, int n, int nslab, const double *a, const d
           #undef A
           #undef C
         }
-        /* reduced system (solver.f90:456-468) */
-        const int nn = 2 * nslab;
+        /* reduced system (solver.f90:446-468); periodic: rows 0..nn-1 of 0..nn */
+        const int nn = is_periodic ? 2 * nslab - 1 : 2 * nslab;
+        for (int k = 0; k < 2 * nslab; k++) rc0[k] = rc[k];
         for (int k = 1; k < nn; k++) {
           const double z = 1. / (1. - ra[k] * rc[k - 1]);
           rp[k] = (rp[k] - ra[k] * rp[k - 1]) * z;
           rc[k] = rc[k] * z;
         }
         for (int k = nn - 2; k >= 0; k--) rp[k] = rp[k] - rc[k] * rp[k + 1];
+        if (is_periodic) { /* solver.f90:469-498, with the unmodified coefficients rc0 */
+          for (int k = 0; k < nn; k++) r2[k] = 0.;
+          r2[0] = -ra[0];
+          r2[nn - 1] = r2[nn - 1] - rc0[nn - 1];
+          for (int k = 1; k < nn; k++) {
+            const double z = 1. / (1. - ra[k] * rc0[k - 1]);
+            r2[k] = (r2[k] - ra[k] * r2[k - 1]) * z;
+            rc0[k] = rc0[k] * z;
+          }
+          for (int k = nn - 2; k >= 0; k--) r2[k] = r2[k] - rc0[k] * r2[k + 1];
+          rp[nn] = (rp[nn] - rc0[nn] * rp[0] - ra[nn] * rp[nn - 1]) /
+                   (1. + rc0[nn] * r2[0] + ra[nn] * r2[nn - 1]);
+          for (int k = 0; k < nn; k++) rp[k] = rp[k] + r2[k] * rp[nn];
+        }
         /* back to the slabs (solver.f90:503-514) */
         for (int s = 0; s < nslab; s++) {
           int lo, ns;
@@ -360,7 +381,7 @@ int orc_gaussel_dtdma(int nx, int ny, int n, int nslab, const double *a, const d
         }
         #undef PK
       }
-    free(aa); free(cc); free(bb); free(ra); free(rc); free(rp);
+    free(aa); free(cc); free(bb); free(ra); free(rc); free(rp); free(rc0); free(r2);
   }
   return ok;
 }

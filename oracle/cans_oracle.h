@@ -66,10 +66,11 @@ int orc_solve_helmholtz(const int *n, const int *kinds, const int *lens, double 
                         const double *rhsby, const double *rhsbz, const int *is_bound,
                         const char *cbc, const char *c_or_f, double *p);
 
-/* src/solver.f90:309-517 (gaussel_dtdma), non-periodic: z rows split into nslab slabs */
+/* src/solver.f90:309-517 (gaussel_dtdma): z rows split into nslab slabs; periodic z closes
+   the reduced system cyclically (nslab >= 2) */
 void orc_slab_range(int n, int nslab, int s, int *lo, int *cnt);
 int orc_gaussel_dtdma(int nx, int ny, int n, int nslab, const double *a, const double *b,
-                      const double *c, double norm, double *p, const double *lambdaxy);
+                      const double *c, double norm, double *p, const double *lambdaxy, int is_periodic);
 /* src/solver.f90:547-616 (one rank): z-only elimination of the halo'd field p */
 int orc_solver_gaussel_z(const int *n, const double *a, const double *b, const double *c,
                          const char *bcz, char c_or_f_z, double norm, double *p);

@@ -81,7 +81,7 @@ def _declare(L):
                               C.c_double, C.c_double, _dp, _dp, _dp, _dp, C.c_int, C.c_int]
     L.orc_bulk_forcing_s.argtypes = [_ip, C.c_int, C.c_double, _dp]
     L.orc_gaussel_dtdma.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, C.c_double, _dp,
-                                    C.c_void_p]
+                                    C.c_void_p, C.c_int]
     L.orc_solver_gaussel_z.argtypes = [_ip, _dp, _dp, _dp, C.c_char_p, C.c_char, C.c_double, _dp]
     L.orc_solve_helmholtz_1d.argtypes = [_ip, C.c_double, _dp, _dp, _dp, C.c_void_p, _ip, C.c_char_p,
                                          C.c_char_p, _dp]
