@@ -632,7 +632,7 @@ static int rk3_substep(cans_ctx* c, const cans_step_params* s, int irk, double d
   // With homogeneous pressure BCs and a periodic x transform the divergence is formed
   // inside the first transform pass (fillps -> updt_rhs_b -> solver otherwise)
   const bool fuse_div = !(s->rhsbx_d || s->rhsby_d || s->rhsbz_d) && s->cbcpre[0] == 'P' && s->cbcpre[1] == 'P' &&
-                        c->L.n1 >= 2 && c->L.n1 % 2 == 0;
+                        c->L.n1 >= 2 && c->L.n1 % 2 == 0 && plan_fft_x(s->plan);
   void* divu_v = nullptr;
   if (delta && fuse_div && (rc = ensure_work(c, 4, (size_t)c->L.cells() * sizeof(S), &divu_v))) return rc;
   S* divu = (S*)divu_v;

@@ -77,9 +77,9 @@ struct cans_ctx {
   // 0 RK intermediate fields, 1 transform work buffer, 2 Thomas checkpoints,
   // 3 small z tables, 4 divergence of the old velocity (mixed mode), 5 Helmholtz z tables,
   // 6 distributed z solve (factors and reduced systems), 7 refinement corrections (mixed mode),
-  // 8 the second set of RK intermediate fields, 9 the second pressure field and 10 the folded
-  // projection's ghost values (cans_rk3_step)
-  static constexpr int NWORK = 11;
+  // 8 the second set of RK intermediate fields, 9 the second pressure field, 10 the folded
+  // projection's ghost values (cans_rk3_step) and 11 the input copy of a dense y transform
+  static constexpr int NWORK = 12;
   void* work[NWORK] = {};
   size_t work_bytes[NWORK] = {};
   double* dscal = nullptr;     // device scalars (64): [0..2] forcing f, [8..11] reductions,

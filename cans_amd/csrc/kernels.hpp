@@ -249,6 +249,8 @@ size_t solver_work_bytes(const cans_ctx* c, size_t es, bool periodic_z);
 // device address of the fixed scratch (CANS_NOALLOC_SCRATCH_BYTES) on a device that has a
 // context, or nullptr
 void* noalloc_scratch(int device);
+// whether the plan's x direction runs the FFT passes (which can form the divergence), not a dense matrix
+bool plan_fft_x(const cans_plan* p);
 template <typename S>
 int run_solver(cans_ctx* c, const cans_plan* plan, double normfft, const double* lam, const double* a,
                const double* b, const double* cc, const char* bc, const char* c_or_f, S* p,

@@ -45,6 +45,10 @@ struct cans_plan {
   // launch shapes chosen by the tuner at cans_fftini, [0] FP32 storage, [1] FP64 (0: heuristic)
   int xthreads[2] = {0, 0};   // x pass: threads per work-group (lines = threads / (N/8))
   int ylines[2] = {0, 0};     // y pass: complex lines (2 columns each) per work-group
+  // directions whose length the FFT passes do not take: the transform as a dense n x n
+  // matrix (forward dm[d][0], backward dm[d][1], row-major, FP64) applied by k_dense
+  int dense[2] = {0, 0};
+  double* dm[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
 };
 
 namespace cans {
@@ -1565,6 +1569,213 @@ static void tw_pack(const std::vector<cplx<double>>& tw, unsigned char* out) {
   for (size_t t = 0; t < tw.size(); t++) f[t] = {(float)tw[t].re, (float)tw[t].im};
 }
 
+// ---------------------------------------------------------------------------
+// Transforms of any length: a direction whose length the FFT passes do not take
+// (any factor other than 2, 3 and 5, more than three odd factors, an odd factor on
+// a line shorter than 8, more than 4096 points, or an odd n1, whose rows the passes
+// cannot move as element pairs) applies its transform as a dense n x n matrix.  The
+// matrix holds FFTW's unnormalised definition of the direction's kind (fft.f90:260-313)
+// with the output in the order the FFT passes produce (the packed real-FFT order of the
+// reference's GPU build for PP, fft.f90:402-451 and initsolver.f90:98-117; natural
+// order for the DCT/DST kinds), so lambdaxy, the z solve and the backward pass are
+// those of any other length.  Each pass is one FP64 product of the matrix with a block
+// of lines: 64 x 64 output tiles over 16-deep LDS slices of the matrix and the lines,
+// a 4 x 4 register block per lane, computed in FP64 whatever the storage type.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct DenseArgs {
+  const double* M;  // n x n, row-major: M[m * n + e] multiplies input e into output m
+  int n;
+  long nlines;
+  Layout L;
+  int n2, nic;
+  const T* in;   // the field (x forward), or a work buffer
+  T* out;        // a work buffer, or the field (x backward)
+};
+
+constexpr int DN_TM = 64, DN_TL = 64, DN_TK = 16;
+constexpr int DENSE_MAX = 8192;  // longest dense direction (two 512 MB matrices)
+
+// MODE 0: x forward (field rows -> work buffer); 1: x backward (work buffer -> field rows);
+// 2: y (work buffer columns -> work buffer columns, out of place)
+template <typename T, int MODE>
+__device__ __forceinline__ long dense_off(const DenseArgs<T>& a, bool field, long l, int e) {
+  if (MODE == 2) return woff((int)(l % a.L.n1), e, (int)(l / a.L.n1), a.n2, a.nic);
+  const int j = (int)(l % a.L.n2), k = (int)(l / a.L.n2);
+  return field ? a.L.at(1 + e, 1 + j, 1 + k) : woff(e, j, k, a.n2, a.nic);
+}
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256) k_dense(DenseArgs<T> a) {
+  __shared__ double As[DN_TK][DN_TM + 1], Bs[DN_TK][DN_TL + 1];
+  const int t = threadIdx.x, tx = t % 16, ty = t / 16;
+  const long l0 = (long)blockIdx.x * DN_TL;
+  const int m0 = blockIdx.y * DN_TM;
+  const int n = a.n;
+  double acc[4][4] = {};
+  for (int e0 = 0; e0 < n; e0 += DN_TK) {
+    for (int q = t; q < DN_TM * DN_TK; q += 256) {  // matrix slice, e contiguous
+      const int r = q / DN_TK, cc = q % DN_TK;
+      const int m = m0 + r, e = e0 + cc;
+      As[cc][r] = (m < n && e < n) ? a.M[(long)m * n + e] : 0.;
+    }
+    for (int q = t; q < DN_TK * DN_TL; q += 256) {  // lines: along e for x, along the lines for y
+      const int r = MODE == 2 ? q / DN_TL : q % DN_TK, cc = MODE == 2 ? q % DN_TL : q / DN_TK;
+      const int e = e0 + r;
+      const long l = l0 + cc;
+      Bs[r][cc] = (e < n && l < a.nlines) ? (double)a.in[dense_off<T, MODE>(a, MODE == 0, l, e)] : 0.;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < DN_TK; kk++) {
+      double x[4], y[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) x[i] = As[kk][ty * 4 + i];
+#pragma unroll
+      for (int i = 0; i < 4; i++) y[i] = Bs[kk][tx + 16 * i];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++) acc[i][jj] = fma(x[i], y[jj], acc[i][jj]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int m = m0 + ty * 4 + i;
+    if (m >= n) continue;
+#pragma unroll
+    for (int jj = 0; jj < 4; jj++) {
+      const long l = l0 + tx + 16 * jj;
+      if (l < a.nlines) a.out[dense_off<T, MODE>(a, MODE == 1, l, m)] = (T)acc[i][jj];
+    }
+  }
+}
+
+template <typename T>
+__global__ void k_copy(const T* __restrict__ s, T* __restrict__ d, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) d[i] = s[i];
+}
+
+// cos(pi p / q) and sin(pi p / q) from the integer angle p/q reduced to [0, 2)
+static long double cospi_q(long p, long q) {
+  p %= 2 * q;
+  if (p < 0) p += 2 * q;
+  return cosl(3.141592653589793238462643383279502884L * (long double)p / (long double)q);
+}
+static long double sinpi_q(long p, long q) {
+  p %= 2 * q;
+  if (p < 0) p += 2 * q;
+  return sinl(3.141592653589793238462643383279502884L * (long double)p / (long double)q);
+}
+
+// the n x n matrix of a kind's forward (dir > 0) or backward transform, with FFTW's
+// unnormalised definitions (fft.f90:260-313) and the FFT passes' output order
+static void dense_matrix(int kind, int n, int dir, std::vector<double>& M) {
+  M.assign((size_t)n * n, 0.);
+  auto at = [&](int m, int e) -> double& { return M[(size_t)m * n + e]; };
+  const long N = n;
+  switch (kind) {
+    case K_PP: {
+      // R2HC r_k = sum x_j cos(2 pi jk/n), i_k = -sum x_j sin(2 pi jk/n), stored
+      // (r0, r_{n/2} | i_{(n-1)/2}, r1, i1, ...); HC2R its unnormalised inverse
+      const int h = n / 2;
+      auto slot_r = [&](int k) { return (n % 2 == 0 && k == h) ? 1 : 2 * k; };  // r_k (k = 0 .. h)
+      auto slot_i = [&](int k) { return (n % 2 == 1 && k == h) ? 1 : 2 * k + 1; };  // i_k (0 < k < n/2)
+      for (int k = 0; k <= h; k++) {
+        const bool has_i = k > 0 && 2 * k != n;
+        for (int j = 0; j < n; j++) {
+          const long pj = 2L * j * k;
+          if (dir > 0) {
+            at(slot_r(k), j) = (double)cospi_q(pj, N);
+            if (has_i) at(slot_i(k), j) = -(double)sinpi_q(pj, N);
+          } else {
+            const double w = (k == 0 || 2 * k == n) ? 1. : 2.;
+            at(j, slot_r(k)) = w * (double)cospi_q(pj, N);
+            if (has_i) at(j, slot_i(k)) = -2. * (double)sinpi_q(pj, N);
+          }
+        }
+      }
+      break;
+    }
+    case K_NN:  // REDFT10 forward, REDFT01 backward
+      for (int k = 0; k < n; k++)
+        for (int j = 0; j < n; j++) {
+          if (dir > 0) at(k, j) = 2. * (double)cospi_q((2L * j + 1) * k, 2 * N);
+          else at(k, j) = j == 0 ? 1. : 2. * (double)cospi_q((long)j * (2L * k + 1), 2 * N);
+        }
+      break;
+    case K_DD:    // RODFT10 forward, RODFT01 backward
+    case K_DNF: { // RODFT01 forward, RODFT10 backward
+      const bool ro10 = (kind == K_DD) == (dir > 0);
+      for (int k = 0; k < n; k++)
+        for (int j = 0; j < n; j++) {
+          if (ro10) at(k, j) = 2. * (double)sinpi_q((2L * j + 1) * (k + 1), 2 * N);
+          else at(k, j) = j == n - 1 ? ((k % 2) ? -1. : 1.) : 2. * (double)sinpi_q((j + 1L) * (2L * k + 1), 2 * N);
+        }
+      break;
+    }
+    case K_ND:  // REDFT11, its own inverse up to 2n
+      for (int k = 0; k < n; k++)
+        for (int j = 0; j < n; j++) at(k, j) = 2. * (double)cospi_q((2L * j + 1) * (2L * k + 1), 4 * N);
+      break;
+    case K_DN:  // RODFT11
+      for (int k = 0; k < n; k++)
+        for (int j = 0; j < n; j++) at(k, j) = 2. * (double)sinpi_q((2L * j + 1) * (2L * k + 1), 4 * N);
+      break;
+    default: {  // K_DDF: RODFT00 on the n-1 interior faces (fft.f90:80-82); face n passes through
+      const long m1 = N;  // (n - 1) + 1
+      for (int k = 0; k < n - 1; k++)
+        for (int j = 0; j < n - 1; j++) at(k, j) = 2. * (double)sinpi_q((j + 1L) * (k + 1), m1);
+      at(n - 1, n - 1) = 1.;
+      break;
+    }
+  }
+}
+
+static int upload_dense(cans_plan* p, int d) {
+  const int n = p->len[d];
+  for (int dir = 0; dir < 2; dir++) {
+    std::vector<double> M;
+    dense_matrix(p->kind[d], n, dir == 0 ? +1 : -1, M);
+    CANS_HIP_TRY(hipMalloc((void**)&p->dm[d][dir], M.size() * sizeof(double)));
+    CANS_HIP_TRY(hipMemcpy(p->dm[d][dir], M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  return CANS_OK;
+}
+
+// one dense pass of direction d for one system: x (field <-> work buffer) or y (in the work buffer)
+template <typename T>
+static int dense_pass(cans_ctx* c, const cans_plan* p, T* fld, T* W, int d, int dir) {
+  const Layout& L = c->L;
+  DenseArgs<T> a;
+  a.M = p->dm[d][dir > 0 ? 0 : 1];
+  a.n = p->len[d];
+  a.L = L;
+  a.n2 = L.n2;
+  a.nic = wnic(L.n1);
+  a.nlines = d == 0 ? (long)L.n2 * L.n3 : (long)L.n1 * L.n3;
+  const dim3 grid((unsigned)cdiv(a.nlines, DN_TL), (unsigned)cdiv(a.n, DN_TM));
+  if (d == 0) {
+    a.in = dir > 0 ? fld : W;
+    a.out = dir > 0 ? W : fld;
+    if (dir > 0) k_dense<T, 0><<<grid, 256, 0, c->stream>>>(a);
+    else k_dense<T, 1><<<grid, 256, 0, c->stream>>>(a);
+  } else {
+    const long wel = wbuf_elems(L.n1, L.n2, L.n3);
+    void* cp = nullptr;
+    int rc = ensure_work(c, 11, (size_t)wel * sizeof(T), &cp);
+    if (rc) return rc;
+    k_copy<T><<<(unsigned)std::min<long>(cdiv(wel, 256), 65536), 256, 0, c->stream>>>(W, (T*)cp, wel);
+    CANS_LAUNCH_CHECK("k_copy");
+    a.in = (const T*)cp;
+    a.out = W;
+    k_dense<T, 2><<<grid, 256, 0, c->stream>>>(a);
+  }
+  CANS_LAUNCH_CHECK("k_dense");
+  return CANS_OK;
+}
+
 static int upload_tables(cans_plan* p, int d) {
   std::vector<cplx<double>> tw, mk;
   make_tables(p->nfft[d], tw, mk);
@@ -1721,6 +1932,8 @@ static int het_pass(cans_ctx* c, const cans_plan* const* ps, T* const* fld, T* c
 }
 
 // do the systems' plans share direction d's transform (kind and length)?
+bool plan_fft_x(const cans_plan* p) { return p && !p->dense[0]; }
+
 static bool same_transform(const cans_plan* const* ps, int nsys, int d) {
   for (int q = 1; q < nsys; q++)
     if (ps[q]->kind[d] != ps[0]->kind[d] || ps[q]->len[d] != ps[0]->len[d]) return false;
@@ -1774,21 +1987,41 @@ int run_solver_batch(cans_ctx* c, const cans_plan* p, int nsys, const SolveSys<T
   }
   T* fld[NBATCH];
   T* Wq[NBATCH];
+  auto plan_of = [&](int q) { return ps ? ps[q] : p; };
+  bool dense[2] = {false, false};  // a system with a dense direction: that direction system by system
+  for (int q = 0; q < nsys; q++)
+    for (int d = 0; d < 2; d++) dense[d] = dense[d] || plan_of(q)->dense[d];
   for (int q = 0; q < nsys; q++) {
     fld[q] = sy[q].fld;
     Wq[q] = (T*)W + q * wel;
-    // the transform passes move element pairs: interior rows must start on pair boundaries
-    if (L.n1 % 2 || L.sy % 2 || ((uintptr_t)(fld[q] + 1) & (2 * sizeof(T) - 1)))
+    // the FFT passes move element pairs: interior rows must start on pair boundaries
+    if (!plan_of(q)->dense[0] && (L.n1 % 2 || L.sy % 2 || ((uintptr_t)(fld[q] + 1) & (2 * sizeof(T) - 1))))
       return fail(CANS_EINVAL, "cans_solver: rows of p must start aligned to an element pair (n1 and ld even, "
                                "field from cans_field_alloc or with element (1,0,0) on a pair boundary)");
   }
-  if (div && (p->kind[0] != K_PP || p->len[0] < 2))
-    return fail(CANS_EINVAL, "run_solver: the velocity-divergence input needs a periodic x transform");
+  if (div && (p->kind[0] != K_PP || p->len[0] < 2 || p->dense[0]))
+    return fail(CANS_EINVAL, "run_solver: the velocity-divergence input needs a periodic x transform of an FFT length");
+  if (ws && dense[1]) return fail(CANS_EUNSUPPORTED, "run_solver: a caller workspace does not cover dense y transforms");
   // systems with their own plans (ps): a direction whose transforms differ runs the heterogeneous pass
   const bool hx = ps && !same_transform(ps, nsys, 0), hy = ps && !same_transform(ps, nsys, 1);
   if (hx && div) return fail(CANS_EINVAL, "run_solver: the divergence input needs one x transform");
-  if ((rc = hx ? het_pass<T>(c, ps, fld, Wq, nsys, 0, +1) : xpass<T>(c, p, fld, Wq, nsys, +1, div))) return rc;
-  if ((rc = hy ? het_pass<T>(c, ps, nullptr, Wq, nsys, 1, +1) : ypass<T>(c, p, Wq, nsys, +1))) return rc;
+  // one direction's pass for every system: batched, or system by system where a plan is dense
+  auto pass = [&](int d, int dir) -> int {
+    if (dense[d]) {
+      for (int q = 0; q < nsys; q++) {
+        const cans_plan* pq = plan_of(q);
+        int r = pq->dense[d] ? dense_pass<T>(c, pq, fld[q], Wq[q], d, dir)
+                : d == 0     ? xpass<T>(c, pq, &fld[q], &Wq[q], 1, dir)
+                             : ypass<T>(c, pq, &Wq[q], 1, dir);
+        if (r) return r;
+      }
+      return CANS_OK;
+    }
+    if (d == 0) return hx ? het_pass<T>(c, ps, fld, Wq, nsys, 0, dir) : xpass<T>(c, p, fld, Wq, nsys, dir, div);
+    return hy ? het_pass<T>(c, ps, nullptr, Wq, nsys, 1, dir) : ypass<T>(c, p, Wq, nsys, dir);
+  };
+  if ((rc = pass(0, +1))) return rc;
+  if ((rc = pass(1, +1))) return rc;
   ZArgs<T> z;
   z.nl = nl;
   z.n1 = L.n1;
@@ -1865,8 +2098,8 @@ int run_solver_batch(cans_ctx* c, const cans_plan* p, int nsys, const SolveSys<T
     k_thomas<T, sizeof(T) == sizeof(float)><<<dim3(cdiv(nl, 256), nsys), 256, 0, c->stream>>>(z);
     CANS_LAUNCH_CHECK("k_thomas");
   }
-  if ((rc = hy ? het_pass<T>(c, ps, nullptr, Wq, nsys, 1, -1) : ypass<T>(c, p, Wq, nsys, -1))) return rc;
-  if ((rc = hx ? het_pass<T>(c, ps, fld, Wq, nsys, 0, -1) : xpass<T>(c, p, fld, Wq, nsys, -1))) return rc;
+  if ((rc = pass(1, -1))) return rc;
+  if ((rc = pass(0, -1))) return rc;
   return CANS_OK;
 }
 
@@ -2524,6 +2757,7 @@ static int tune_type(cans_ctx* c, cans_plan* p) {
   float bx = 1e30f;
   int xbest = 256;
   for (int xt : {128, 256, 512}) {
+    if (p->dense[0]) break;  // a dense direction has no launch shape to choose
     if (xt < tnx && xt != 512) continue;
     const size_t xshm = (size_t)(tnx >= xt ? 1 : xt / tnx) * padded_len(p->nfft[0]) * sizeof(cplx<T>);
     if (xshm > (size_t)lds_max) continue;
@@ -2540,6 +2774,7 @@ static int tune_type(cans_ctx* c, cans_plan* p) {
   float by = 1e30f;
   int ybest = 0;
   for (int yl : {4, 8, 16, 32}) {
+    if (p->dense[1]) break;
     if ((long)yl * tny > 1024 || (size_t)yl * p->nfft[1] * sizeof(cplx<T>) > 65536) continue;
     p->ylines[slot] = yl;
     const float ms = timed([&]() -> int {
@@ -2562,6 +2797,36 @@ static int tune_type(cans_ctx* c, cans_plan* p) {
   return CANS_OK;
 }
 
+// the kind of a direction's transforms (fft.f90:260-313) and the normalisation factor
+// norm(1)*(n+norm(2)-ix) of fft.f90:75,140
+static int bc_kind(char b0, char b1, bool face, int n, int* kind, double* nfac, const char* who) {
+  if (b0 == 'P' && b1 == 'P') *kind = K_PP, *nfac = n;
+  else if (b0 == 'N' && b1 == 'N' && !face) *kind = K_NN, *nfac = 2. * n;
+  else if (b0 == 'D' && b1 == 'D') *kind = face ? K_DDF : K_DD, *nfac = 2. * n;
+  else if (b0 == 'N' && b1 == 'D') *kind = face ? K_NN : K_ND, *nfac = 2. * n;
+  else if (b0 == 'D' && b1 == 'N') *kind = face ? K_DNF : K_DN, *nfac = 2. * n;
+  else if (b0 == 'N' && b1 == 'N')  // REDFT00: the reference forbids face-centred NN velocity walls
+    // with implicit diffusion (sanity.f90:151-162), the only user of face-centred transforms
+    return fail(CANS_EUNSUPPORTED, std::string(who) + ": face-centred NN (REDFT00) is not built on the device");
+  else
+    return fail(CANS_EINVAL, std::string(who) + ": boundary pairs must be PP, NN, DD, ND or DN");
+  return CANS_OK;
+}
+
+extern "C" int cans_transform_matrix(int n, const char* bc, char c_or_f, int dir, double* m) {
+  if (n < 1 || n > DENSE_MAX || !bc || !m || (c_or_f != 'c' && c_or_f != 'f') || (dir != 1 && dir != -1))
+    return fail(CANS_EINVAL, "cans_transform_matrix: bad argument");
+  int kind;
+  double nfac;
+  const int rc = bc_kind(bc[0], bc[1], c_or_f == 'f', n, &kind, &nfac, "cans_transform_matrix");
+  if (rc) return rc;
+  if (kind == K_DDF && n < 2) return fail(CANS_EINVAL, "cans_transform_matrix: face-centred DD needs 2 points");
+  std::vector<double> M;
+  dense_matrix(kind, n, dir, M);
+  std::memcpy(m, M.data(), M.size() * sizeof(double));
+  return CANS_OK;
+}
+
 extern "C" int cans_fftini(cans_ctx* c, const int* ng, const char* bcxy, const char* c_or_f, cans_plan** out,
                            double* normfft) {
   if (!c || !ng || !bcxy || !c_or_f || !out) return fail(CANS_EINVAL, "cans_fftini: null argument");
@@ -2578,21 +2843,11 @@ extern "C" int cans_fftini(cans_ctx* c, const int* ng, const char* bcxy, const c
       delete p;
       return fail(CANS_EINVAL, "cans_fftini: c_or_f must be 'c' or 'f'");
     }
-    // kinds of fft.f90:260-313; the normalisation factor norm(1)*(n+norm(2)-ix) of fft.f90:75,140
     double nfac;
-    if (b0 == 'P' && b1 == 'P') p->kind[d] = K_PP, nfac = ng[d];
-    else if (b0 == 'N' && b1 == 'N' && !face) p->kind[d] = K_NN, nfac = 2. * ng[d];
-    else if (b0 == 'D' && b1 == 'D') p->kind[d] = face ? K_DDF : K_DD, nfac = 2. * ng[d];
-    else if (b0 == 'N' && b1 == 'D') p->kind[d] = face ? K_NN : K_ND, nfac = 2. * ng[d];
-    else if (b0 == 'D' && b1 == 'N') p->kind[d] = face ? K_DNF : K_DN, nfac = 2. * ng[d];
-    else if (b0 == 'N' && b1 == 'N') {
-      // REDFT00: the reference forbids face-centred NN velocity walls with implicit diffusion
-      // (sanity.f90:151-162), the only user of face-centred transforms
+    const int krc = bc_kind(b0, b1, face, ng[d], &p->kind[d], &nfac, "cans_fftini");
+    if (krc) {
       delete p;
-      return fail(CANS_EUNSUPPORTED, "cans_fftini: face-centred NN (REDFT00) is not built on the device");
-    } else {
-      delete p;
-      return fail(CANS_EINVAL, "cans_fftini: boundary pairs must be PP, NN, DD, ND or DN");
+      return krc;
     }
     if (p->kind[d] == K_DDF && ng[d] < 2) {
       delete p;
@@ -2600,14 +2855,16 @@ extern "C" int cans_fftini(cans_ctx* c, const int* ng, const char* bcxy, const c
     }
     p->len[d] = ng[d];
     p->nfft[d] = k_double(p->kind[d]) ? 2 * ng[d] : ng[d];
-    if (!fft_factor(p->nfft[d], &p->logn[d], &p->odd[d])) {
+    // the FFT passes move x rows as element pairs and pair the columns of the y pass
+    p->dense[d] = !fft_factor(p->nfft[d], &p->logn[d], &p->odd[d]) || ng[0] % 2 != 0;
+    if (p->dense[d] && ng[d] > DENSE_MAX) {
       delete p;
-      return fail(CANS_EUNSUPPORTED,
-                  "cans_fftini: transform lengths must be 2^m, or 2^m (m >= 3) times at most three factors 3 "
-                  "and 5, at most 4096 (ND and DN pairs: the doubled length)");
+      return fail(CANS_EUNSUPPORTED, "cans_fftini: a transform length outside 2^m 3^a 5^b (a + b <= 3, at most "
+                                     "4096) must not exceed 8192 (its dense matrix)");
     }
+    if (p->dense[d]) p->nfft[d] = ng[d];
     nf = nf * nfac;
-    int rc = upload_tables(p, d);
+    int rc = p->dense[d] ? upload_dense(p, d) : upload_tables(p, d);
     if (rc) {
       cans_fftend(p);
       return rc;
@@ -2641,6 +2898,8 @@ extern "C" int cans_fftend(cans_plan* p) {
   for (int d = 0; d < 2; d++) {
     hipFree(p->tw[d]);
     hipFree(p->mk[d]);
+    hipFree(p->dm[d][0]);
+    hipFree(p->dm[d][1]);
   }
   delete p;
   return CANS_OK;

@@ -97,6 +97,8 @@ __global__ void k_rdz(int n3, const double* dzci, const double* dzfi, double* ta
 template <typename S, bool DELTA, bool DIVU, bool BUOY, bool FOLD = false>
 __global__ void __launch_bounds__(MOM_TX* MOM_TY, 4) k_mom_rk(MomArgs<S> a) {
   __shared__ double su[3][MOM_PY][MOM_PX], sv[3][MOM_PY][MOM_PX], sw[3][MOM_PY][MOM_PX];
+  // FOLD: p + pp of the staged planes, so that the plane loop reads no pressure from memory
+  __shared__ double sp[FOLD ? 3 : 1][FOLD ? MOM_PY : 1][FOLD ? MOM_PX : 1];
   const Layout& L = a.L;
   const int ntx = (L.n1 + MOM_TX - 1) / MOM_TX, nty = (L.n2 + MOM_TY - 1) / MOM_TY;
   const int nkc = (L.n3 + MOM_KC - 1) / MOM_KC;
@@ -172,6 +174,7 @@ __global__ void __launch_bounds__(MOM_TX* MOM_TY, 4) k_mom_rk(MomArgs<S> a) {
           su[slot][pj][pi] = xu;
           sv[slot][pj][pi] = xv;
           sw[slot][pj][pi] = xw;
+          sp[slot][pj][pi] = pval(o);
         } else {
           su[slot][pj][pi] = a.u[o];
           sv[slot][pj][pi] = a.v[o];
@@ -190,13 +193,13 @@ __global__ void __launch_bounds__(MOM_TX* MOM_TY, 4) k_mom_rk(MomArgs<S> a) {
     w_ccm = uval(2, i, j, kb - 1, c0 - sz);
     w_pcm = uval(2, i + 1, j, kb - 1, c0 - sz + 1);
     w_cpm = uval(2, i, j + 1, kb - 1, c0 - sz + sy);
-    p_c = pval(c0);
+    if constexpr (!FOLD) p_c = pval(c0);
   }
   // plane k+2 is loaded into registers before plane k's arithmetic and stored to its slot
   // after it (the slot plane k-1 used, free since the previous barrier), so the loads'
   // latency overlaps that work; one barrier per plane
   constexpr int NQ = (MOM_PX * MOM_PY + MOM_TX * MOM_TY - 1) / (MOM_TX * MOM_TY);  // ring cells per lane
-  double nu[NQ], nv[NQ], nw[NQ];
+  double nu[NQ], nv[NQ], nw[NQ], np[FOLD ? NQ : 1];
   auto load = [&](int kk) {
 #pragma unroll
     for (int r = 0; r < NQ; r++) {
@@ -206,6 +209,7 @@ __global__ void __launch_bounds__(MOM_TX* MOM_TY, 4) k_mom_rk(MomArgs<S> a) {
         const long o = L.at(gi, gj, kk);
         if constexpr (FOLD) {
           fval3(gi, gj, kk, o, nu[r], nv[r], nw[r]);
+          np[r] = pval(o);
         } else {
           nu[r] = a.u[o];
           nv[r] = a.v[o];
@@ -223,6 +227,7 @@ __global__ void __launch_bounds__(MOM_TX* MOM_TY, 4) k_mom_rk(MomArgs<S> a) {
         su[slot][pj][pi] = nu[r];
         sv[slot][pj][pi] = nv[r];
         sw[slot][pj][pi] = nw[r];
+        if constexpr (FOLD) sp[slot][pj][pi] = np[r];
       }
     }
   };
@@ -318,7 +323,17 @@ __global__ void __launch_bounds__(MOM_TX* MOM_TY, 4) k_mom_rk(MomArgs<S> a) {
       // RK update (rk.f90:156-186) and RHS_old <- RHS (rk.f90:266-276).  With rkpar(2) = 0
       // (first substep) the RHS_old term is factor2*RHS_old = 0 and is not read.
       const long o = (long)(i - 1) + (long)L.n1 * ((j - 1) + (long)L.n2 * (k - 1));
-      const double p_ip = pval(c + 1), p_jp = pval(c + sy), p_kp = pval(c + sz);
+      double p_ip, p_jp, p_kp;
+      if constexpr (FOLD) {
+        p_c = sp[b0][pj][pi];
+        p_ip = sp[b0][pj][pi + 1];
+        p_jp = sp[b0][pj + 1][pi];
+        p_kp = sp[b1][pj][pi];
+      } else {
+        p_ip = pval(c + 1);
+        p_jp = pval(c + sy);
+        p_kp = pval(c + sz);
+      }
       const double ru = a.read_ro ? (double)a.ru[o] : 0., rv = a.read_ro ? (double)a.rv[o] : 0.,
                    rw = a.read_ro ? (double)a.rw[o] : 0.;
       double un = a.read_ro ? u_ccc + a.factor1 * dudt + a.factor2 * ru +

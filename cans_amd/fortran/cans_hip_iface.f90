@@ -174,6 +174,13 @@ module cans_hip
       character(kind=c_char), value :: c_or_f
       real(c_double), intent(out) :: lambda(*)
     end function
+    integer(c_int) function cans_transform_matrix(n, bc, c_or_f, dir, m) bind(C, name='cans_transform_matrix')
+      import :: c_int, c_double, c_char
+      integer(c_int), value :: n, dir
+      character(kind=c_char), intent(in) :: bc(2)
+      character(kind=c_char), value :: c_or_f
+      real(c_double), intent(out) :: m(*)
+    end function
     integer(c_int) function cans_bc_rhs(cbc, bc, dlc, dlf, c_or_f, rhs) bind(C, name='cans_bc_rhs')
       import :: c_int, c_double, c_char
       character(kind=c_char), intent(in) :: cbc(2)

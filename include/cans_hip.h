@@ -122,6 +122,12 @@ int cans_initgrid(int gtype, int n, double gr, double lz, int is_periodic,
 /* src/initsolver.f90:85-144 in the device ordering of this library (the
  * reference's GPU ordering, initsolver.f90:98-117, for periodic directions) */
 int cans_eigenvalues(int n, const char *bc, char c_or_f, double *lambda_h);
+/* The n x n matrix (row-major, m[k*n + j] multiplies input j into output k) of one
+ * direction's forward (dir = 1) or backward (dir = -1) transform: the kind of
+ * fft.f90:260-313 for the boundary pair bc and grid c_or_f, with FFTW's unnormalised
+ * definition, its output in the order cans_eigenvalues uses.  A plan whose length the
+ * FFT passes do not take (cans_fftini) applies exactly this matrix; n <= 8192. */
+int cans_transform_matrix(int n, const char *bc, char c_or_f, int dir, double *m);
 /* src/initsolver.f90:189-232: the two plane values of rhs(:,:,0:1) (every entry of
  * plane ib equals rhs_h[ib]); dlc, dlf: spacings at the two boundaries */
 int cans_bc_rhs(const char *cbc, const double *bc, const double *dlc, const double *dlf, char c_or_f,
@@ -133,8 +139,10 @@ int cans_tridmatrix(const char *bc, int n, const double *dzci_h, const double *d
 /* ---------------------------------------------------------------------------
  * Transforms: twins of fftini/fftend (src/fft.f90:25-245).  arrplan holds the
  * four batched transforms ((fwd,bwd) x (x,y)); normfft is returned as the
- * reference computes it (fft.f90:75,140,202,208).  Device transform lengths: 2^m, or
- * 2^m (m >= 3) times at most three factors 3 and 5 (e.g. 24, 40, 72, 120), at most 4096.
+ * reference computes it (fft.f90:75,140,202,208).  Any length up to 8192 along x and y:
+ * 2^m, or 2^m (m >= 3) times at most three factors 3 and 5 (e.g. 24, 40, 72, 120), at most
+ * 4096, run the FFT passes; other lengths (and both directions when n1 is odd) apply the
+ * dense matrix of cans_transform_matrix, O(n) work per element.
  * Boundary pairs along x and y (fft.f90:260-313): PP; cell-centred NN, DD, ND and DN (ND/DN
  * run on a line of twice the length, which must be such a length); face-centred ND and DN
  * (REDFT10/01, RODFT01/10) and DD (RODFT00 on the n-1 interior faces, fft.f90:80-82, through a

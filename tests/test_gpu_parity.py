@@ -276,8 +276,9 @@ def test_fused_substep_bitwise_per_routine(torch, name, ng, dt):
 
 
 def test_transform_lengths_and_pairs_refused(torch):
-    """cans_fftini accepts x/y lengths 2^m, or 2^m (m >= 3) times at most three factors 3 and 5
-    (ND/DN pairs: twice the length), and refuses others with CANS_EUNSUPPORTED; boundary pairs
+    """cans_fftini takes any x/y length up to 8192: 2^m, or 2^m (m >= 3) times at most three
+    factors 3 and 5 (ND/DN pairs: twice the length), through the FFT passes, the others as a
+    dense matrix; longer non-FFT lengths are refused with CANS_EUNSUPPORTED, boundary pairs
     other than PP, NN, DD, ND, DN with CANS_EINVAL.  The checks run before any device work."""
     import ctypes as C
     from cans_amd import _lib as LB
@@ -295,9 +296,10 @@ def test_transform_lengths_and_pairs_refused(torch):
         L.cans_ctx_destroy(h)
         return rc, msg
 
-    for ng, bc, want in [((14, 16, 8), b"PPPPNN", LB.CANS_EUNSUPPORTED),   # 7 * 2
-                         ((12, 16, 8), b"PPPPNN", LB.CANS_EUNSUPPORTED),   # 3 * 4: m < 3
-                         ((16, 270, 8), b"PPPPNN", LB.CANS_EUNSUPPORTED),  # 27 * 10: four odd factors
+    for ng, bc, want in [((14, 16, 8), b"PPPPNN", LB.CANS_OK),             # 7 * 2: dense x
+                         ((12, 16, 8), b"PPPPNN", LB.CANS_OK),             # 3 * 4: m < 3
+                         ((16, 270, 8), b"PPPPNN", LB.CANS_OK),            # 27 * 10: four odd factors
+                         ((8200, 2, 8), b"PPPPNN", LB.CANS_EUNSUPPORTED),  # 8 * 1025: dense and too long
                          ((16, 8, 8), b"PNPPNN", LB.CANS_EINVAL),
                          ((12, 16, 8), b"NDPPNN", LB.CANS_OK),             # REDFT11 on a line of 24
                          ((16, 120, 8), b"PPDNNN", LB.CANS_OK),            # RODFT11 on a line of 240
