@@ -84,6 +84,7 @@ SIGNATURES = {
     "cans_memcpy_h2d": (C.c_int, [_vp, _vp, _vp, C.c_long]),
     "cans_memcpy_d2h": (C.c_int, [_vp, _vp, _vp, C.c_long]),
     "cans_initgrid": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp, _dp, _dp]),
+    "cans_initgrid_natural": (C.c_int, [C.c_int, C.c_double, C.c_int, _dp, _dp, _dp, _dp]),
     "cans_eigenvalues": (C.c_int, [C.c_int, _cp, C.c_char, _dp]),
     "cans_transform_matrix": (C.c_int, [C.c_int, _cp, C.c_char, C.c_int, _dp]),
     "cans_bc_rhs": (C.c_int, [_cp, _dp, _dp, _dp, C.c_char, _dp]),

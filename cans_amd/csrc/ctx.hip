@@ -350,15 +350,36 @@ static double gp_stretch(int gtype, double alpha, double z0) {
   }
 }
 
-extern "C" int cans_initgrid(int gtype, int n, double gr, double lz, int is_periodic, double* dzc,
-                             double* dzf, double* zc, double* zf) {
+// gridpoint_natural (initgrid.f90:162-203): the wall-resolved channel spacing of Pirozzoli &
+// Orlandi (JCP 439, 110408, 2021), clustered at both walls; kb, alpha, c_eta and dyp at the
+// reference's defaults.  All FP64, as there.
+static double gp_natural(int kg, int nzg) {
+  const double pi = std::acos(-1.);
+  const double kb = 32., alpha = pi / 1.5, c_eta = 0.8, dyp = 0.05;
+  const double n = nzg / 2.;
+  const double rn = n / kb;
+  const double retau = 1. / (1. + rn * rn) * (dyp * n + std::pow(3. / 4. * alpha * c_eta * n, 4. / 3.) * (rn * rn));
+  const double k = 1. * std::min(kg, nzg - kg);
+  const double rk = k / kb;
+  double z = 1. / (1. + rk * rk) * (dyp * k + std::pow(3. / 4. * alpha * c_eta * k, 4. / 3.) * (rk * rk)) /
+             (2. * retau);
+  if (kg > nzg - kg) z = 1. - z;
+  return z;
+}
+
+static int initgrid(int gtype, int n, double gr, double lz, int is_periodic, bool natural, double* dzc,
+                    double* dzf, double* zc, double* zf) {
   if (n < 1 || !dzc || !dzf || !zc || !zf) return fail(CANS_EINVAL, "cans_initgrid: bad argument");
   zf[0] = 0.;
   for (int k = 1; k <= n; k++) {
-    float z0 = ((float)k - 0.f) / (1.f * (float)n);
-    zf[k] = gp_stretch(gtype, gr, (double)z0) * lz;
+    if (natural) {
+      zf[k] = gp_natural(k, n) * lz;
+    } else {
+      float z0 = ((float)k - 0.f) / (1.f * (float)n);
+      zf[k] = gp_stretch(gtype, gr, (double)z0) * lz;
+    }
   }
-  if (std::fabs(gr) < 2.220446049250313e-16) {
+  if (std::fabs(gr) < 2.220446049250313e-16 && !natural) {
     for (int k = 0; k <= n + 1; k++) dzf[k] = dzc[k] = lz / (double)(1.f * (float)n);
   } else {
     for (int k = 1; k <= n; k++) dzf[k] = zf[k] - zf[k - 1];
@@ -374,6 +395,16 @@ extern "C" int cans_initgrid(int gtype, int n, double gr, double lz, int is_peri
     zf[k] = zf[k - 1] + dzf[k];
   }
   return CANS_OK;
+}
+
+extern "C" int cans_initgrid(int gtype, int n, double gr, double lz, int is_periodic, double* dzc,
+                             double* dzf, double* zc, double* zf) {
+  return initgrid(gtype, n, gr, lz, is_periodic, false, dzc, dzf, zc, zf);
+}
+
+extern "C" int cans_initgrid_natural(int n, double lz, int is_periodic, double* dzc, double* dzf, double* zc,
+                                     double* zf) {
+  return initgrid(1, n, 0., lz, is_periodic, true, dzc, dzf, zc, zf);
 }
 
 extern "C" int cans_eigenvalues(int n, const char* bc, char c_or_f, double* lam) {

@@ -167,6 +167,13 @@ module cans_hip
       real(c_double), value :: gr, lz
       real(c_double), intent(out) :: dzc(*), dzf(*), zc(*), zf(*)
     end function
+    integer(c_int) function cans_initgrid_natural(n, lz, is_periodic, dzc, dzf, zc, zf) &
+                   bind(C, name='cans_initgrid_natural')
+      import :: c_int, c_double
+      integer(c_int), value :: n, is_periodic
+      real(c_double), value :: lz
+      real(c_double), intent(out) :: dzc(*), dzf(*), zc(*), zf(*)
+    end function
     integer(c_int) function cans_eigenvalues(n, bc, c_or_f, lambda) bind(C, name='cans_eigenvalues')
       import :: c_int, c_double, c_char
       integer(c_int), value :: n

@@ -30,6 +30,8 @@ class Case:
     l: tuple
     gtype: int = 1
     gr: float = 0.0
+    # &numerics is_gridpoint_natural_channel (param.f90:99): gridpoint_natural faces, gtype/gr unused
+    is_gridpoint_natural_channel: bool = False
     cbcvel: str = "PP" * 9          # cbcvel(0:1,3,3) flattened, component slowest
     cbcpre: str = "PP" * 3          # cbcpre(0:1,3)
     bcvel: tuple = (0.0,) * 18
@@ -70,8 +72,12 @@ class Grid:
         n3 = ng[2]
         self.dzc, self.dzf, self.zc, self.zf = (np.zeros(n3 + 2) for _ in range(4))
         periodic_z = case.cbcpre[4:6] == "PP"
-        LB.check(L.cans_initgrid(case.gtype, n3, case.gr, self.l[2], int(periodic_z), _dp(self.dzc),
-                                 _dp(self.dzf), _dp(self.zc), _dp(self.zf)), "cans_initgrid")
+        if case.is_gridpoint_natural_channel:
+            LB.check(L.cans_initgrid_natural(n3, self.l[2], int(periodic_z), _dp(self.dzc), _dp(self.dzf),
+                                             _dp(self.zc), _dp(self.zf)), "cans_initgrid_natural")
+        else:
+            LB.check(L.cans_initgrid(case.gtype, n3, case.gr, self.l[2], int(periodic_z), _dp(self.dzc),
+                                     _dp(self.dzf), _dp(self.zc), _dp(self.zf)), "cans_initgrid")
         self.dzci = self.dzc ** (-1)
         self.dzfi = self.dzf ** (-1)
         vol = self.l[0] * self.l[1] * self.l[2]

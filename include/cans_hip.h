@@ -119,6 +119,11 @@ int cans_memcpy_d2h(cans_ctx *ctx, void *dst_h, const void *src_d, long bytes);
 /* src/initgrid.f90:14-118 (gtype 1-4); arrays (0:n+1) */
 int cans_initgrid(int gtype, int n, double gr, double lz, int is_periodic,
                   double *dzc_h, double *dzf_h, double *zc_h, double *zf_h);
+/* src/initgrid.f90:14-99 with is_gridpoint_natural_channel = .true. (param.f90:99): the
+ * faces of gridpoint_natural (initgrid.f90:162-203, Pirozzoli & Orlandi 2021), which takes
+ * neither gtype nor gr; arrays (0:n+1) */
+int cans_initgrid_natural(int n, double lz, int is_periodic, double *dzc_h, double *dzf_h, double *zc_h,
+                          double *zf_h);
 /* src/initsolver.f90:85-144 in the device ordering of this library (the
  * reference's GPU ordering, initsolver.f90:98-117, for periodic directions) */
 int cans_eigenvalues(int n, const char *bc, char c_or_f, double *lambda_h);

@@ -69,21 +69,37 @@ static double gp_middle(double alpha, double z0) {    /* initgrid.f90:145-161 */
   return z0;
 }
 
-void orc_initgrid(int gtype, int n, double gr, double lz, int is_periodic,
-                  double *dzc, double *dzf, double *zc, double *zf) {
-  /* initgrid.f90:14-99 (the 'natural' channel grid is not restated) */
+/* initgrid.f90:162-203 (gridpoint_natural, Pirozzoli & Orlandi, JCP 439, 2021): kb = 32,
+   alpha = pi/1.5, c_eta = 0.8, dyp = 0.05; everything in double precision (_rp literals) */
+static double gp_natural(int kg, int nzg) {
+  const double pi = acos(-1.);
+  const double kb = 32., alpha = pi / 1.5, c_eta = 0.8, dyp = 0.05;
+  const double n = nzg / 2.;
+  const double retau = 1. / (1. + (n / kb) * (n / kb)) *
+                       (dyp * n + pow(3. / 4. * alpha * c_eta * n, 4. / 3.) * ((n / kb) * (n / kb)));
+  const double k = 1. * (kg < nzg - kg ? kg : nzg - kg);
+  double z = 1. / (1. + (k / kb) * (k / kb)) * (dyp * k + pow(3. / 4. * alpha * c_eta * k, 4. / 3.) * ((k / kb) * (k / kb))) /
+             (2. * retau);
+  if (kg > nzg - kg) z = 1. - z;
+  return z;
+}
+
+static void initgrid(int gtype, int n, double gr, double lz, int is_periodic, int natural,
+                     double *dzc, double *dzf, double *zc, double *zf) {
+  /* initgrid.f90:14-99 */
   double (*gp)(double, double) = gp_two_end;
   if (gtype == 2) gp = gp_one_end;
   else if (gtype == 3) gp = gp_one_end_r;
   else if (gtype == 4) gp = gp_middle;
   zf[0] = 0.;
   for (int k = 1; k <= n; k++) {
+    if (natural) { zf[k] = gp_natural(k, n); continue; }
     /* z0 = (k-0.)/(1.*n) is a default-real (single precision) expression */
     float z0f = ((float)k - 0.f) / (1.f * (float)n);
     zf[k] = gp(gr, (double)z0f);
   }
   for (int k = 1; k <= n; k++) zf[k] = zf[k] * lz;
-  if (fabs(gr) < EPS) {
+  if (fabs(gr) < EPS && !natural) {
     for (int k = 0; k <= n + 1; k++) { dzf[k] = lz / (double)(1.f * (float)n); dzc[k] = dzf[k]; }
   } else {
     for (int k = 1; k <= n; k++) dzf[k] = zf[k] - zf[k - 1];
@@ -95,6 +111,16 @@ void orc_initgrid(int gtype, int n, double gr, double lz, int is_periodic,
   zc[0] = -dzc[0] / 2.;
   zf[0] = 0.;
   for (int k = 1; k <= n + 1; k++) { zc[k] = zc[k - 1] + dzc[k - 1]; zf[k] = zf[k - 1] + dzf[k]; }
+}
+
+void orc_initgrid(int gtype, int n, double gr, double lz, int is_periodic,
+                  double *dzc, double *dzf, double *zc, double *zf) {
+  initgrid(gtype, n, gr, lz, is_periodic, 0, dzc, dzf, zc, zf);
+}
+
+/* is_gridpoint_natural_channel = .true. (param.f90:99, initgrid.f90:46-58) */
+void orc_initgrid_natural(int n, double lz, int is_periodic, double *dzc, double *dzf, double *zc, double *zf) {
+  initgrid(1, n, 0., lz, is_periodic, 1, dzc, dzf, zc, zf);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -31,6 +31,7 @@ enum {
 /* grid: src/initgrid.f90:14-99 (+ stretching functions 105-200) */
 void orc_initgrid(int gtype, int n, double gr, double lz, int is_periodic,
                   double *dzc, double *dzf, double *zc, double *zf);
+void orc_initgrid_natural(int n, double lz, int is_periodic, double *dzc, double *dzf, double *zc, double *zf);
 /* src/initsolver.f90:85-144 (CPU ordering, i.e. FFTW halfcomplex for PP) */
 void orc_eigenvalues(int n, const char *bc, char c_or_f, double *lambda);
 /* src/initsolver.f90:146-187 */

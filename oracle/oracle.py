@@ -62,6 +62,7 @@ _ip = C.POINTER(C.c_int)
 def _declare(L):
     L.orc_r2r.argtypes = [C.c_int, C.c_int, C.c_long, C.c_long, C.c_long, _dp]
     L.orc_initgrid.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp, _dp, _dp]
+    L.orc_initgrid_natural.argtypes = [C.c_int, C.c_double, C.c_int, _dp, _dp, _dp, _dp]
     L.orc_eigenvalues.argtypes = [C.c_int, C.c_char_p, C.c_char, _dp]
     L.orc_tridmatrix.argtypes = [C.c_char_p, C.c_int, _dp, _dp, C.c_char, _dp, _dp, _dp]
     L.orc_bc_rhs.argtypes = [C.c_char_p, _dp, _dp, _dp, C.c_char, _dp]
@@ -155,6 +156,12 @@ def initgrid(gtype, n, gr, lz, is_periodic):
     return dzc, dzf, zc, zf
 
 
+def initgrid_natural(n, lz, is_periodic):
+    dzc, dzf, zc, zf = (np.zeros(n + 2) for _ in range(4))
+    lib().orc_initgrid_natural(n, lz, int(is_periodic), P(dzc), P(dzf), P(zc), P(zf))
+    return dzc, dzf, zc, zf
+
+
 def eigenvalues(n, bc, c_or_f):
     lam = np.zeros(n)
     lib().orc_eigenvalues(n, chars(bc), chars(c_or_f), P(lam))
@@ -204,7 +211,7 @@ class OracleCase:
     def __init__(self, ng, l, gtype=1, gr=0.0, cbcvel="PP" * 9, cbcpre="PP" * 3,
                  bcvel=None, bcpre=None, visci=1.0, bforce=(0, 0, 0), is_forced=(0, 0, 0),
                  velf=(0, 0, 0), is_impdiff=False, is_impdiff_1d=False, scalars=(), gacc=(0, 0, 0),
-                 beta=0.0, is_boussinesq_buoyancy=False):
+                 beta=0.0, is_boussinesq_buoyancy=False, is_gridpoint_natural_channel=False):
         L = lib()
         self.ng = [int(x) for x in ng]
         self.n = list(self.ng)
@@ -227,8 +234,10 @@ class OracleCase:
         self.is_bound = [1, 1] + [0 if self.is_periodic[d] else 1 for d in (1, 1, 2, 2)]
         n3 = self.n[2]
         # main.f90:241-285
-        self.dzc, self.dzf, self.zc, self.zf = initgrid(gtype, n3, gr, self.l[2],
-                                                         cbcpre[4:6] == "PP")
+        if is_gridpoint_natural_channel:
+            self.dzc, self.dzf, self.zc, self.zf = initgrid_natural(n3, self.l[2], cbcpre[4:6] == "PP")
+        else:
+            self.dzc, self.dzf, self.zc, self.zf = initgrid(gtype, n3, gr, self.l[2], cbcpre[4:6] == "PP")
         self.dzci = self.dzc ** (-1)
         self.dzfi = self.dzf ** (-1)
         vol = self.l[0] * self.l[1] * self.l[2]

@@ -70,7 +70,8 @@ def oracle_case(p):
                         scalars=p.get("scalars", ()), gacc=p.get("gacc", (0, 0, 0)), beta=p.get("beta", 0.0),
                         is_boussinesq_buoyancy=p.get("is_boussinesq_buoyancy", False),
                         is_impdiff=p.get("is_impdiff", False) or p.get("is_impdiff_1d", False),
-                        is_impdiff_1d=p.get("is_impdiff_1d", False))
+                        is_impdiff_1d=p.get("is_impdiff_1d", False),
+                        is_gridpoint_natural_channel=bool(p.get("natural_grid", False)))
 
 
 def host_case(p):
@@ -82,7 +83,8 @@ def host_case(p):
                 gacc=tuple(p.get("gacc", (0.0, 0.0, 0.0))), beta=float(p.get("beta", 0.0)),
                 is_boussinesq_buoyancy=bool(p.get("is_boussinesq_buoyancy", False)),
                 is_impdiff=bool(p.get("is_impdiff", False) or p.get("is_impdiff_1d", False)),
-                is_impdiff_1d=bool(p.get("is_impdiff_1d", False)), dtdma_slabs=int(p.get("dtdma_slabs", 0)))
+                is_impdiff_1d=bool(p.get("is_impdiff_1d", False)), dtdma_slabs=int(p.get("dtdma_slabs", 0)),
+                is_gridpoint_natural_channel=bool(p.get("natural_grid", False)))
 
 
 def initial_state(name, p, grid, seed=2024):

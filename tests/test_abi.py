@@ -45,6 +45,11 @@ def test_host_setup_matches_oracle():
         b = O.initgrid(gtype, 32, gr, 1.3, per)
         for x, y in zip(a, b):
             np.testing.assert_array_equal(x, y)
+    for n, per in ((64, 0), (33, 0), (48, 1)):  # the natural channel grid (is_gridpoint_natural_channel)
+        a = [np.zeros(n + 2) for _ in range(4)]
+        L.cans_initgrid_natural(n, 1.7, per, *[H._dp(x) for x in a])
+        for x, y in zip(a, O.initgrid_natural(n, 1.7, per)):
+            np.testing.assert_allclose(x, y, rtol=1e-15, atol=0)
     dzc, dzf, zc, zf = O.initgrid(1, 24, 1.5, 1.0, False)
     for bc in ("NN", "DD", "PP", "ND"):
         for cf in "cf":

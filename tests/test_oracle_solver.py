@@ -54,6 +54,30 @@ def test_initgrid_two_end_closed_form():
     np.testing.assert_allclose(zc[1:17], (np.arange(16) + 0.5) / 8, atol=1e-15)
 
 
+def test_initgrid_natural_closed_form():
+    """gridpoint_natural (initgrid.f90:162-203): symmetric about the mid-plane, the mid face at
+    lz/2 (there k = n/2 makes z the Re_tau formula over twice itself), the first face at the
+    closed form of k = 1, and increasing.  Swapping the k and n-k branches or dropping the
+    (k/kb)^2 blending fails these."""
+    n, lz = 96, 2.0
+    dzc, dzf, zc, zf = O.initgrid_natural(n, lz, False)
+    kb, alpha, c_eta, dyp = 32.0, np.pi / 1.5, 0.8, 0.05
+
+    def f(k):
+        r = (k / kb) ** 2
+        return (dyp * k + (0.75 * alpha * c_eta * k) ** (4.0 / 3.0) * r) / (1 + r)
+
+    retau = f(n / 2)
+    faces = np.cumsum(dzf[1:n + 1])
+    np.testing.assert_allclose(faces[0], lz * f(1) / (2 * retau), rtol=1e-14)
+    np.testing.assert_allclose(faces[n // 2 - 1], lz / 2, rtol=1e-14)
+    np.testing.assert_allclose(faces[:n - 1] + faces[:n - 1][::-1], lz, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(faces[-1], lz, rtol=1e-14)
+    assert np.all(dzf[1:n + 1] > 0) and dzf[1] < dzf[n // 2] / 10
+    np.testing.assert_allclose(dzc[:n + 1], 0.5 * (dzf[:-1] + dzf[1:]), atol=0)
+    assert dzf[0] == dzf[1] and dzf[n + 1] == dzf[n] and dzc[n + 1] == dzc[n]
+
+
 def _grid(n, gr, periodic):
     dzc, dzf, zc, zf = O.initgrid(1, n, gr, 1.0, periodic)
     return 1 / dzc, 1 / dzf
