@@ -92,6 +92,8 @@ SIGNATURES = {
     "cans_fftini": (C.c_int, [_vp, _ip, _cp, _cp, C.POINTER(_vp), _dp]),
     "cans_fftend": (C.c_int, [_vp]),
     "cans_fft_tuning": (C.c_int, [_vp, _ip]),
+    "cans_fft_radix": (C.c_int, [_vp, _ip]),
+    "cans_fft_set_radix": (C.c_int, [_vp, _ip]),
     "cans_solver": (C.c_int, [_vp, _ip, _ip, _vp, C.c_double, _vp, _vp, _vp, _vp, _cp, _cp, _vp]),
     "cans_solver_noalloc": (C.c_int, [_vp, _ip, _ip, _vp, C.c_double, _vp, _vp, _vp, _vp, _cp, _cp, _vp]),
     "cans_poisson_batch_create": (C.c_int, [_vp, C.c_int, C.POINTER(PoissonBox), _dp, C.POINTER(_vp)]),

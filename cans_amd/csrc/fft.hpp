@@ -259,9 +259,12 @@ __device__ __forceinline__ void fft_stage_odd(cplx<L>* buf, IX ix, int N, int Ns
 // N = 2^logN * odd, where odd (the odd part of N) is 1 or a product of at most three
 // factors 3 and 5, and logN >= 3 when odd > 1.  The power-of-two stages run first with
 // spans 1, 8, ..., then one radix-3 stage per factor 3 and one radix-5 stage per factor 5.
+// r4: radix-4 power-of-two stages (two butterflies a thread, spans 1, 4, 16, ...) instead
+// of radix 8: twice the LDS round trips, half the values a butterfly holds (the launch-shape
+// tuner chooses).
 template <typename L, int s, class IX>
 __device__ __forceinline__ void fft_line(cplx<L>* buf, IX ix, int N, int logN, int lt, int Tn,
-                         const cplx<L>* __restrict__ tw, int odd = 1, bool act = true) {
+                         const cplx<L>* __restrict__ tw, int odd = 1, bool act = true, bool r4 = false) {
   if (logN == 0 && odd == 1) return;
   if (logN == 1 && odd == 1) {
     fft_stage<L, s, 2, 1>(buf, ix, N, 0, lt, Tn, tw, act);
@@ -273,13 +276,13 @@ __device__ __forceinline__ void fft_line(cplx<L>* buf, IX ix, int N, int logN, i
   }
   int lNs = 0;
   int rem = logN;
-  while (rem >= 3) {
-    fft_stage<L, s, 8, 1>(buf, ix, N, lNs, lt, Tn, tw, act);
-    lNs += 3;
-    rem -= 3;
+  if (r4) {
+    for (; rem >= 2; lNs += 2, rem -= 2) fft_stage<L, s, 4, 2>(buf, ix, N, lNs, lt, Tn, tw, act);
+  } else {
+    for (; rem >= 3; lNs += 3, rem -= 3) fft_stage<L, s, 8, 1>(buf, ix, N, lNs, lt, Tn, tw, act);
+    if (rem == 2) fft_stage<L, s, 4, 2>(buf, ix, N, lNs, lt, Tn, tw, act);
   }
-  if (rem == 2) fft_stage<L, s, 4, 2>(buf, ix, N, lNs, lt, Tn, tw, act);
-  else if (rem == 1) fft_stage<L, s, 2, 4>(buf, ix, N, lNs, lt, Tn, tw, act);
+  if (rem == 1) fft_stage<L, s, 2, 4>(buf, ix, N, lNs, lt, Tn, tw, act);
   int Ns = 1 << logN;
   for (; odd % 3 == 0; odd /= 3, Ns *= 3) fft_stage_odd<L, s, 3>(buf, ix, N, Ns, lt, Tn, tw, act);
   for (; odd % 5 == 0; odd /= 5, Ns *= 5) fft_stage_odd<L, s, 5>(buf, ix, N, Ns, lt, Tn, tw, act);

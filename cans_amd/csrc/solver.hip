@@ -45,6 +45,7 @@ struct cans_plan {
   // launch shapes chosen by the tuner at cans_fftini, [0] FP32 storage, [1] FP64 (0: heuristic)
   int xthreads[2] = {0, 0};   // x pass: threads per work-group (lines = threads / (N/8))
   int ylines[2] = {0, 0};     // y pass: complex lines (2 columns each) per work-group
+  int radix4[2][2] = {{0, 0}, {0, 0}};  // [direction][storage]: radix-4 instead of radix-8 stages
   // directions whose length the FFT passes do not take: the transform as a dense n x n
   // matrix (forward dm[d][0], backward dm[d][1], row-major, FP64) applied by k_dense
   int dense[2] = {0, 0};
@@ -194,6 +195,7 @@ struct XArgs {
   const cplx<double>* mk;
   int div;         // forward only: build the rhs from the velocity instead of reading fld
   DivSrc<T> ds;
+  int r4 = 0;      // radix-4 power-of-two stages (fft_line)
 };
 
 // L(corr[0] + ... ) at the cells c, c+1 of row (j,k): the 7-point operator of the solver
@@ -287,7 +289,7 @@ __device__ __forceinline__ d2 div_pair(const XArgs<T>& a, long r, int e) {
 // last whole team (a.pairs teams of a.Tn) only take part in the barriers.
 // FLIP (face-centred DN, K_DNF): the K_DD pair with swapped directions, so the forward pass
 // runs the RODFT01 code and the backward pass the RODFT10 code.
-template <int DIR, typename T, int DV = 0, bool FLIP = false>
+template <int DIR, typename T, int DV = 0, bool FLIP = false, int R4 = 0>
 __device__ __forceinline__ void xpass_body(const XArgs<T>& a, int bx, int sys) {
   T* const fld = a.fld[sys];
   T* const W = a.W[sys];
@@ -383,7 +385,7 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& a, int bx, int sys) {
       }
     }
     __syncthreads();
-    fft_line<T, -1>(smem, ix, N, a.logN, lt, Tn, line_tw<T>(a.tw, N), a.odd, act);
+    fft_line<T, -1>(smem, ix, N, a.logN, lt, Tn, line_tw<T>(a.tw, N), a.odd, act, R4 != 0);
     if (!va) return;
     if (kind == K_PP) {
       const int H = N >> 1;
@@ -491,7 +493,7 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& a, int bx, int sys) {
       }
     }
     __syncthreads();
-    fft_line<T, +1>(smem, ix, N, a.logN, lt, Tn, line_tw<T>(a.tw, N), a.odd, act);
+    fft_line<T, +1>(smem, ix, N, a.logN, lt, Tn, line_tw<T>(a.tw, N), a.odd, act, R4 != 0);
     if (!va) return;
     if (N == 1) {
       if (lt == 0) {
@@ -513,14 +515,15 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& a, int bx, int sys) {
   }
 }
 
-// one kernel per storage type (the FP32 and FP64 forms differ in LDS and register use)
-template <int DIR, int DV, bool FLIP = false>
+// one kernel per storage type (the FP32 and FP64 forms differ in LDS and register use); R4:
+// the radix-4 stage split, a kernel of its own, so that neither form carries the other's registers
+template <int DIR, int DV, bool FLIP = false, int R4 = 0>
 __global__ void __launch_bounds__(512) k_xpass_f(XArgs<float> a) {
-  xpass_body<DIR, float, DV, FLIP>(a, blockIdx.x, blockIdx.y);
+  xpass_body<DIR, float, DV, FLIP, R4>(a, blockIdx.x, blockIdx.y);
 }
-template <int DIR, int DV, bool FLIP = false>
+template <int DIR, int DV, bool FLIP = false, int R4 = 0>
 __global__ void __launch_bounds__(512) k_xpass_d(XArgs<double> a) {
-  xpass_body<DIR, double, DV, FLIP>(a, blockIdx.x, blockIdx.y);
+  xpass_body<DIR, double, DV, FLIP, R4>(a, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -537,6 +540,7 @@ struct YArgs {
   int nd;          // data length along y: N, or N/2 for K_ND / K_DN / K_DDF
   const cplx<double>* tw;
   const cplx<double>* mk;
+  int r4 = 0;      // radix-4 power-of-two stages (fft_line)
 };
 
 // One tile of the y pass in three steps: ypass_load reads the elements this lane's
@@ -717,7 +721,7 @@ __device__ __forceinline__ void ypass_pre(const YArgs<T>& a, const YTile<T>& t, 
 }
 
 // the line in LDS (after a barrier) -> transform -> post-processing -> stores
-template <int DIR, typename T, bool FLIP>
+template <int DIR, typename T, bool FLIP, int R4 = 0>
 __device__ __forceinline__ void ypass_fft_post(const YArgs<T>& a, const YTile<T>& t, int sys, cplx<T>* smem) {
   const int N = a.N, Tn = a.Tn, lt = t.lt, kind = FLIP ? K_DD : a.kind;
   constexpr bool FWD = (DIR > 0) != FLIP;
@@ -729,7 +733,7 @@ __device__ __forceinline__ void ypass_fft_post(const YArgs<T>& a, const YTile<T>
     else *p = (T)x;
   };
   if (FWD || k_double(kind)) {
-    fft_line<T, -1>(smem, ix, N, a.logN, lt, Tn, line_tw<T>(a.tw, N), a.odd, t.act);
+    fft_line<T, -1>(smem, ix, N, a.logN, lt, Tn, line_tw<T>(a.tw, N), a.odd, t.act, R4 != 0);
     if (!t.va) return;
     if (kind == K_PP) {
       const int H = N >> 1;
@@ -774,7 +778,7 @@ __device__ __forceinline__ void ypass_fft_post(const YArgs<T>& a, const YTile<T>
       }
     }
   } else {
-    fft_line<T, +1>(smem, ix, N, a.logN, lt, Tn, line_tw<T>(a.tw, N), a.odd, t.act);
+    fft_line<T, +1>(smem, ix, N, a.logN, lt, Tn, line_tw<T>(a.tw, N), a.odd, t.act, R4 != 0);
     if (!t.va) return;
     for (int e = lt; e < N; e += Tn) {
       cplx<double> z = widen(smem[ix(kind == K_PP ? e : mk_pos(e, N))]);
@@ -789,7 +793,7 @@ __device__ __forceinline__ void ypass_fft_post(const YArgs<T>& a, const YTile<T>
 
 // bx: work-group index among this system's; sys: system of a batch.  Lanes past
 // a.lines * a.Tn only take part in the barriers.  FLIP: as in xpass_body.
-template <int DIR, typename T, bool FLIP = false>
+template <int DIR, typename T, bool FLIP = false, int R4 = 0>
 __device__ __forceinline__ void ypass_body(const YArgs<T>& a, int bx, int sys) {
   extern __shared__ unsigned char smem_raw[];
   cplx<T>* smem = (cplx<T>*)smem_raw;  // LDS line in the storage type
@@ -798,28 +802,30 @@ __device__ __forceinline__ void ypass_body(const YArgs<T>& a, int bx, int sys) {
   ypass_load<DIR, T, FLIP, 0, 5>(a, t, sys, raw);
   ypass_pre<DIR, T, FLIP, 0, 5>(a, t, raw, smem);
   __syncthreads();
-  ypass_fft_post<DIR, T, FLIP>(a, t, sys, smem);
+  ypass_fft_post<DIR, T, FLIP, R4>(a, t, sys, smem);
 }
 
 // FP32 lines: 16 lines x N/8 threads = 1024 threads in 64 KB of LDS; at most 64 VGPRs
 // lets two such work-groups share a CU.  FP64 lines: LDS already limits a CU to two
 // 512-thread work-groups, which needs no register cap.
-template <int DIR, bool FLIP = false>
+template <int DIR, bool FLIP = false, int R4 = 0>
 __global__ void __launch_bounds__(1024, 8) k_ypass_f(YArgs<float> a) {
-  ypass_body<DIR, float, FLIP>(a, blockIdx.x, blockIdx.y);
+  ypass_body<DIR, float, FLIP, R4>(a, blockIdx.x, blockIdx.y);
 }
-template <int DIR, bool FLIP = false>
+template <int DIR, bool FLIP = false, int R4 = 0>
 __global__ void __launch_bounds__(1024) k_ypass_d(YArgs<double> a) {
-  ypass_body<DIR, double, FLIP>(a, blockIdx.x, blockIdx.y);
+  ypass_body<DIR, double, FLIP, R4>(a, blockIdx.x, blockIdx.y);
 }
 template <int DIR, typename T>
 void launch_ypass(dim3 g, int t, size_t shm, hipStream_t st, const YArgs<T>& a) {
   const bool flip = a.kind == K_DNF;
   if constexpr (sizeof(T) == sizeof(float)) {
     if (flip) k_ypass_f<DIR, true><<<g, t, shm, st>>>(a);
+    else if (a.r4) k_ypass_f<DIR, false, 1><<<g, t, shm, st>>>(a);
     else k_ypass_f<DIR, false><<<g, t, shm, st>>>(a);
   } else {
     if (flip) k_ypass_d<DIR, true><<<g, t, shm, st>>>(a);
+    else if (a.r4) k_ypass_d<DIR, false, 1><<<g, t, shm, st>>>(a);
     else k_ypass_d<DIR, false><<<g, t, shm, st>>>(a);
   }
 }
@@ -1809,6 +1815,7 @@ static int xpass(cans_ctx* c, const cans_plan* p, T* const* fld, T* const* W, in
   a.kind = p->kind[0];
   a.tw = p->tw[0];
   a.mk = p->mk[0];
+  a.r4 = p->radix4[0][sizeof(T) == sizeof(double)];
   const long nrows = (long)c->L.n2 * c->L.n3;
   const long npairs = (nrows + 1) / 2;
   dim3 grid(cdiv(npairs, a.pairs), nsys);
@@ -1827,16 +1834,34 @@ static int xpass(cans_ctx* c, const cans_plan* p, T* const* fld, T* const* W, in
     CANS_LAUNCH_CHECK("k_xpass");
     return CANS_OK;
   }
+  // radix-4 stages where the tuner chose them: the plain and divergence-fused passes (the
+  // residual pass of the refinement keeps radix 8)
+  const bool r4 = a.r4 && !res;
   if constexpr (sizeof(T) == sizeof(float)) {
-    if (dir < 0) k_xpass_f<-1, 0><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
-    else if (res) k_xpass_f<1, 2><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
-    else if (a.div) k_xpass_f<1, 1><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
-    else k_xpass_f<1, 0><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
+    if (dir < 0) {
+      if (r4) k_xpass_f<-1, 0, false, 1><<<grid, nt, shm, c->stream>>>(a);
+      else k_xpass_f<-1, 0><<<grid, nt, shm, c->stream>>>(a);
+    } else if (res) {
+      k_xpass_f<1, 2><<<grid, nt, shm, c->stream>>>(a);
+    } else if (a.div) {
+      if (r4) k_xpass_f<1, 1, false, 1><<<grid, nt, shm, c->stream>>>(a);
+      else k_xpass_f<1, 1><<<grid, nt, shm, c->stream>>>(a);
+    } else {
+      if (r4) k_xpass_f<1, 0, false, 1><<<grid, nt, shm, c->stream>>>(a);
+      else k_xpass_f<1, 0><<<grid, nt, shm, c->stream>>>(a);
+    }
   } else {
     if (res) return fail(CANS_EUNSUPPORTED, "k_xpass: refinement residuals are formed for FP32 storage only");
-    if (dir < 0) k_xpass_d<-1, 0><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
-    else if (a.div) k_xpass_d<1, 1><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
-    else k_xpass_d<1, 0><<<grid, a.pairs * a.Tn, shm, c->stream>>>(a);
+    if (dir < 0) {
+      if (r4) k_xpass_d<-1, 0, false, 1><<<grid, nt, shm, c->stream>>>(a);
+      else k_xpass_d<-1, 0><<<grid, nt, shm, c->stream>>>(a);
+    } else if (a.div) {
+      if (r4) k_xpass_d<1, 1, false, 1><<<grid, nt, shm, c->stream>>>(a);
+      else k_xpass_d<1, 1><<<grid, nt, shm, c->stream>>>(a);
+    } else {
+      if (r4) k_xpass_d<1, 0, false, 1><<<grid, nt, shm, c->stream>>>(a);
+      else k_xpass_d<1, 0><<<grid, nt, shm, c->stream>>>(a);
+    }
   }
   CANS_LAUNCH_CHECK("k_xpass");
   return CANS_OK;
@@ -1864,6 +1889,7 @@ static int ypass(cans_ctx* c, const cans_plan* p, T* const* W, int nsys, int dir
   a.kind = p->kind[1];
   a.tw = p->tw[1];
   a.mk = p->mk[1];
+  a.r4 = p->radix4[1][sizeof(T) == sizeof(double)];
   dim3 grid(cdiv((long)a.n3 * ((a.n1 + 1) / 2), lines), nsys);
   const size_t shm = (size_t)lines * a.N * sizeof(cplx<T>);
   if (dir > 0) launch_ypass<1, T>(grid, lines * a.Tn, shm, c->stream, a);
@@ -2714,6 +2740,10 @@ extern "C" int cans_poisson_batch_destroy(cans_poisson_batch* B) {
 // most 128 z-planes (the passes treat planes independently) and keeps the fastest
 // forward+backward pair per storage type.
 // ---------------------------------------------------------------------------
+// the tuner's radix choices: (direction, line length, kind, storage slot, device) -> radix 4
+static std::map<std::tuple<int, int, int, int, int>, int> g_radix;
+static std::mutex g_radix_mu;
+
 template <typename T>
 static int tune_type(cans_ctx* c, cans_plan* p) {
   const int slot = sizeof(T) == sizeof(double);
@@ -2784,6 +2814,34 @@ static int tune_type(cans_ctx* c, cans_plan* p) {
     if (ms < by) by = ms, ybest = yl;
   }
   p->ylines[slot] = ybest;
+  // radix-4 instead of radix-8 power-of-two stages, at the chosen shapes (lines of 2^m, m >= 3).
+  // The split into stages changes the rounding, so the choice is made once per direction,
+  // length, kind and storage type in a process and reused by every later plan: solves through
+  // two plans of the same lengths agree bit for bit.
+  for (int d = 0; d < 2 && !rc; d++) {
+    const float base = d == 0 ? bx : by;
+    if (p->dense[d] || p->logn[d] < 3 || !(base < 1e29f)) continue;
+    const auto key = std::make_tuple(d, p->nfft[d], p->kind[d], slot, c->device);
+    {
+      std::lock_guard<std::mutex> lk(g_radix_mu);
+      const auto it = g_radix.find(key);
+      if (it != g_radix.end()) {
+        p->radix4[d][slot] = it->second;
+        continue;
+      }
+    }
+    p->radix4[d][slot] = 1;
+    const float ms = timed([&]() -> int {
+      int r = d == 0 ? xpass<T>(&cs, p, &fld, &W, 1, +1) : ypass<T>(&cs, p, &W, 1, +1);
+      if (r) return r;
+      return d == 0 ? xpass<T>(&cs, p, &fld, &W, 1, -1) : ypass<T>(&cs, p, &W, 1, -1);
+    });
+    if (!(ms < base)) p->radix4[d][slot] = 0;
+    if (!rc) {
+      std::lock_guard<std::mutex> lk(g_radix_mu);
+      g_radix.emplace(key, p->radix4[d][slot]);
+    }
+  }
   hipStreamSynchronize(cs.stream);
   hipEventDestroy(e0);
   hipEventDestroy(e1);
@@ -2792,6 +2850,7 @@ static int tune_type(cans_ctx* c, cans_plan* p) {
   if (rc) {  // a failed trial keeps the heuristic shapes; the plan itself stays valid
     p->xthreads[slot] = 0;
     p->ylines[slot] = 0;
+    p->radix4[0][slot] = p->radix4[1][slot] = 0;
     (void)hipGetLastError();
   }
   return CANS_OK;
@@ -2890,6 +2949,22 @@ extern "C" int cans_fft_tuning(const cans_plan* p, int* shapes) {
   shapes[1] = p->xthreads[1];
   shapes[2] = p->ylines[0];
   shapes[3] = p->ylines[1];
+  return CANS_OK;
+}
+
+extern "C" int cans_fft_radix(const cans_plan* p, int* radix) {
+  if (!p || !radix) return fail(CANS_EINVAL, "cans_fft_radix: null argument");
+  for (int d = 0; d < 2; d++)
+    for (int slot = 0; slot < 2; slot++) radix[2 * d + slot] = p->dense[d] ? 0 : p->radix4[d][slot] ? 4 : 8;
+  return CANS_OK;
+}
+
+extern "C" int cans_fft_set_radix(cans_plan* p, const int* radix) {
+  if (!p || !radix) return fail(CANS_EINVAL, "cans_fft_set_radix: null argument");
+  for (int q = 0; q < 4; q++)
+    if (radix[q] != 4 && radix[q] != 8) return fail(CANS_EINVAL, "cans_fft_set_radix: radix 4 or 8");
+  for (int d = 0; d < 2; d++)
+    for (int slot = 0; slot < 2; slot++) p->radix4[d][slot] = radix[2 * d + slot] == 4;
   return CANS_OK;
 }
 

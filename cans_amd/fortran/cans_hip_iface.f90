@@ -408,6 +408,17 @@ module cans_hip
       type(c_ptr), value :: arrplan
       integer(c_int), intent(out) :: shapes(4)
     end function
+    ! radix of the power-of-two FFT stages: x (FP32, FP64), y (FP32, FP64); 8, 4, or 0 (dense)
+    integer(c_int) function cans_fft_radix(arrplan, radix) bind(C, name='cans_fft_radix')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: arrplan
+      integer(c_int), intent(out) :: radix(4)
+    end function
+    integer(c_int) function cans_fft_set_radix(arrplan, radix) bind(C, name='cans_fft_set_radix')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: arrplan
+      integer(c_int), intent(in) :: radix(4)
+    end function
     integer(c_int) function cans_poisson_batch_tuning(batch, threads) bind(C, name='cans_poisson_batch_tuning')
       import :: c_int, c_ptr
       type(c_ptr), value :: batch

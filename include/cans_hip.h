@@ -162,6 +162,12 @@ int cans_fftend(cans_plan *arrplan);
  * work-group for FP32/FP64 storage, shapes[2..3] y-pass lines per work-group (0: not
  * tuned for that type; a heuristic applies). */
 int cans_fft_tuning(const cans_plan *arrplan, int *shapes);
+/* The radix of the power-of-two FFT stages the tuner chose (no counterpart in the reference):
+ * radix[0..1] x pass FP32/FP64, radix[2..3] y pass FP32/FP64; 8 (radix-8 stages), 4 (radix-4
+ * stages: twice the LDS round trips, half the values per butterfly) or 0 (a dense direction). */
+int cans_fft_radix(const cans_plan *arrplan, int *radix);
+/* Replaces the tuner's radix choice (4 or 8 each, same order; ignored for dense directions). */
+int cans_fft_set_radix(cans_plan *arrplan, const int *radix);
 
 /* ---------------------------------------------------------------------------
  * Operators.  Each replaces the reference routine cited; argument meaning and
