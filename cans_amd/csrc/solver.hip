@@ -848,6 +848,7 @@ struct HetArgs {
   int blocks[NBATCH];  // work-groups of the system
   const cplx<double>* tw[NBATCH];
   const cplx<double>* mk[NBATCH];
+  int r4[NBATCH];      // the system's plan runs radix-4 stages (its tuner's choice)
 };
 
 template <int DIR, typename T>
@@ -869,6 +870,7 @@ __global__ void __launch_bounds__(512) k_het_xpass(HetArgs<T> h) {
   a.mk = h.mk[s];
   a.div = 0;
   if (a.kind == K_DNF) xpass_body<DIR, T, 0, true>(a, blockIdx.x, 0);
+  else if (h.r4[s]) xpass_body<DIR, T, 0, false, 1>(a, blockIdx.x, 0);
   else xpass_body<DIR, T, 0, false>(a, blockIdx.x, 0);
 }
 
@@ -891,6 +893,7 @@ __global__ void __launch_bounds__(1024) k_het_ypass(HetArgs<T> h) {
   a.tw = h.tw[s];
   a.mk = h.mk[s];
   if (a.kind == K_DNF) ypass_body<DIR, T, true>(a, blockIdx.x, 0);
+  else if (h.r4[s]) ypass_body<DIR, T, false, 1>(a, blockIdx.x, 0);
   else ypass_body<DIR, T, false>(a, blockIdx.x, 0);
 }
 
@@ -1918,6 +1921,7 @@ static int het_pass(cans_ctx* c, const cans_plan* const* ps, T* const* fld, T* c
     h.kind[q] = p->kind[d];
     h.tw[q] = p->tw[d];
     h.mk[q] = p->mk[d];
+    h.r4[q] = p->radix4[d][sizeof(T) == sizeof(double)];
   }
   int tmax = 1;
   for (int q = 0; q < nsys; q++) tmax = std::max(tmax, h.Tn[q]);
