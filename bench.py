@@ -432,6 +432,9 @@ def main():
     ap.add_argument("--solve-only", action="store_true",
                     help="time only the channel's Poisson solves, FP32 storage then FP64 (for profiling: after each "
                          "mode's set-up, the last dispatches of every pass kernel are the timed solves)")
+    ap.add_argument("--dtdma-slabs", type=int, default=0,
+                    help="with --solve-only: the z elimination as the distributed solve over this many z slabs "
+                         "(is_poisson_dtdma; 0: the plain sweep)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs (--gpus %d) are not part of it" % args.gpus,
@@ -451,6 +454,7 @@ def main():
 
     p = cases.params("channel", ng=tuple(args.ng))
     if args.solve_only:
+        p["dtdma_slabs"] = args.dtdma_slabs
         print(json.dumps({"poisson": {m: solve_only(p, prec, args) for m, prec in
                                       (("mixed", LB.CANS_MIXED), ("fp64", LB.CANS_FP64))}}))
         return
