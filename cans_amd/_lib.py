@@ -77,6 +77,7 @@ SIGNATURES = {
     "cans_ctx_set_numerics": (C.c_int, [_vp, C.c_int, C.c_int]),
     "cans_ctx_set_dtdma": (C.c_int, [_vp, C.c_int]),
     "cans_ctx_set_refinement": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "cans_ctx_set_mixed_increments": (C.c_int, [_vp, C.c_int]),
     "cans_field_upload": (C.c_int, [_vp, _vp, _dp]),
     "cans_field_download": (C.c_int, [_vp, _vp, _dp]),
     "cans_device_alloc": (C.c_int, [_vp, C.c_long, C.POINTER(_vp)]),

@@ -578,7 +578,7 @@ template <typename S>
 static int rk3_substep(cans_ctx* c, const cans_step_params* s, int irk, double dt, cans_state* st,
                        const StepFold* fo = nullptr) {
   const bool mixed = sizeof(S) != sizeof(double);
-  const bool delta = mixed && !c->imp;
+  const bool delta = mixed && !c->imp && c->mixed_incr;
   const double* rkpar = RKCOEFF[irk - 1];
   const double dtrk = (rkpar[0] + rkpar[1]) * dt;  // main.f90:438-439
   const double dtrki = 1. / dtrk;
@@ -779,6 +779,14 @@ extern "C" int cans_ctx_set_refinement(cans_ctx* c, int sweeps, int substep_mask
     return fail(CANS_EINVAL, "cans_ctx_set_refinement: refinement applies to CANS_MIXED contexts");
   c->refine_sweeps = sweeps;
   c->refine_mask = sweeps ? substep_mask : 0;
+  return CANS_OK;
+}
+
+extern "C" int cans_ctx_set_mixed_increments(cans_ctx* c, int on) {
+  CHECK_CTX("cans_ctx_set_mixed_increments");
+  if (on && c->precision != CANS_MIXED)
+    return fail(CANS_EINVAL, "cans_ctx_set_mixed_increments: increments apply to CANS_MIXED contexts");
+  c->mixed_incr = on != 0;
   return CANS_OK;
 }
 

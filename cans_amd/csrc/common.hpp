@@ -90,5 +90,6 @@ struct cans_ctx {
   int imp = 0;                 // &numerics: 0 explicit diffusion, 1 is_impdiff, 2 is_impdiff_1d
   int dtdma = 0;               // &numerics is_poisson_dtdma: z slabs of the distributed solve (<= 1: off)
   int refine_sweeps = 0;       // refinement sweeps of the fused mixed substep's pressure solve
+  int mixed_incr = 0;          // mixed mode: the RK intermediate velocity as FP32 increments (else FP64)
   int refine_mask = 0;         // substeps (bit irk-1) that refine
 };

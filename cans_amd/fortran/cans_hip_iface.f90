@@ -124,6 +124,12 @@ module cans_hip
       type(c_ptr), value :: ctx
       integer(c_int), value :: sweeps, substep_mask
     end function
+    ! mixed mode: the RK intermediate velocity as FP32 increments (1) or an FP64 field (0, default)
+    integer(c_int) function cans_ctx_set_mixed_increments(ctx, on) bind(C, name='cans_ctx_set_mixed_increments')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: on
+    end function
     integer(c_int) function cans_field_register(ctx, field, kind) bind(C, name='cans_field_register')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, field

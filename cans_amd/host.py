@@ -475,6 +475,11 @@ class Simulation:
         LB.check(LB.lib().cans_ctx_set_refinement(self.ctx.h, int(sweeps), int(substep_mask)),
                  "cans_ctx_set_refinement")
 
+    def set_mixed_increments(self, on=True):
+        """cans_ctx_set_mixed_increments: keep the mixed step's intermediate velocity as FP32
+        increments over the step's velocity instead of an FP64 field."""
+        LB.check(LB.lib().cans_ctx_set_mixed_increments(self.ctx.h, int(bool(on))), "cans_ctx_set_mixed_increments")
+
     # ---- state ----
     def set_state(self, u=None, v=None, w=None, p=None, s=()):
         for f, a in ((self.u, u), (self.v, v), (self.w, w), (self.p, p)) + tuple(zip(self.s, s)):

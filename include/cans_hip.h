@@ -100,6 +100,14 @@ int cans_ctx_set_dtdma(cans_ctx *ctx, int nslab);
  * refined last substep (mask 4) leaves every step's velocity divergence-free to FP64 level.
  * Homogeneous pressure boundary values only.  Default: 0 sweeps (the unrefined mixed step). */
 int cans_ctx_set_refinement(cans_ctx *ctx, int sweeps, int substep_mask);
+/* How the fused mixed substep (explicit diffusion) keeps the RK intermediate velocity u*
+ * (no counterpart in the reference, which stores it in place of u): on = 0 (default) as an
+ * FP64 field, as in CANS_FP64; on = 1 as FP32 increments e = u* - u over the step's velocity,
+ * half the scratch memory, with div(u) kept in FP32 and every consumer rebuilding u + e + f.
+ * Both meet the mixed mode's 1e-6 bar; the FP64 field moves fewer bytes per step (the folded
+ * momentum kernel and correc read u* instead of u and e, and re-read no u for the increment)
+ * and is faster on MI355X.  CANS_MIXED contexts only. */
+int cans_ctx_set_mixed_increments(cans_ctx *ctx, int on);
 /* host arrays are the reference's contiguous (0:n1+1,0:n2+1,0:n3+1) Fortran arrays
  * of double; float fields round on upload.  In a CANS_MIXED context the field
  * must come from cans_field_alloc_kind (its element type is looked up) */

@@ -86,10 +86,15 @@ STEP_CASES = [("channel", (64, 32, 48), 2e-4, 10), ("tgv", (32, 32, 32), 1e-3, 1
               ("box", (16, 16, 12), 1e-3, 5), ("open_nd", (16, 24, 12), 1e-3, 5), ("channel", (80, 40, 24), 2e-4, 5)]
 
 
+# the intermediate velocity as an FP64 field (the default) and as FP32 increments
+# (cans_ctx_set_mixed_increments) for the fused step; the per-routine sequence has no such choice
 @pytest.mark.parametrize("name,ng,dt,nsteps", STEP_CASES, ids=["%s-%dx%dx%d" % ((c[0],) + c[1]) for c in STEP_CASES])
-@pytest.mark.parametrize("fused", [True, False], ids=["fused", "per-routine"])
+@pytest.mark.parametrize("fused", ["fused", "fused-increments", "per-routine"])
 def test_mixed_steps(torch, name, ng, dt, nsteps, fused):
     p, sim, orc, st = mixed_pair(name, ng)
+    if fused == "fused-increments":
+        sim.set_mixed_increments(True)
+    fused = fused != "per-routine"
     sim.init_bounds()
     orc.init_bounds(st)
     for it in range(nsteps):
@@ -124,11 +129,14 @@ REFINED = [("channel", (64, 32, 48), 2e-4, 5, 2, 4), ("channel", (64, 32, 48), 2
            ("box", (16, 16, 12), 1e-3, 4, 2, 4), ("open_nd", (16, 24, 12), 1e-3, 4, 2, 7)]
 
 
+@pytest.mark.parametrize("increments", [False, True], ids=["fp64-intermediate", "increments"])
 @pytest.mark.parametrize("name,ng,dt,nsteps,sweeps,mask", REFINED,
                          ids=["%s-%dx%dx%d-s%d-m%d" % ((c[0],) + c[1] + c[4:]) for c in REFINED])
-def test_mixed_refined_steps(torch, name, ng, dt, nsteps, sweeps, mask):
+def test_mixed_refined_steps(torch, name, ng, dt, nsteps, sweeps, mask, increments):
     p, sim, orc, st = mixed_pair(name, ng)
     sim.set_refinement(sweeps, mask)
+    if increments:
+        sim.set_mixed_increments(True)
     sim.init_bounds()
     orc.init_bounds(st)
     g = sim.grid
@@ -150,6 +158,15 @@ def test_mixed_refined_steps(torch, name, ng, dt, nsteps, sweeps, mask):
     err = relerr(gauge_fixed(sim.p.interior(), dzf, singular), gauge_fixed(st["p"][1:-1, 1:-1, 1:-1], dzf, singular))
     print("p rel err (gauge-fixed) %.3e" % err, flush=True)
     assert err < MIXED_TOL, ("p", err)
+
+
+def test_increments_setting_refused_outside_mixed(torch):
+    from cans_amd import _lib as LB
+    from cans_amd.host import Simulation
+    sim = Simulation(cases.host_case(cases.params("tgv", ng=(16, 16, 16))))
+    with pytest.raises(LB.CansError):
+        sim.set_mixed_increments(True)
+    sim.set_mixed_increments(False)
 
 
 def test_refinement_setting_refused_outside_mixed(torch):

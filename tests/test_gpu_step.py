@@ -1,12 +1,12 @@
 """cans_rk3_step (one whole RK3 step, each substep's projection folded into the next
 substep's momentum kernel) against three cans_rk3_substep calls, after several steps.
 
-FP64: the same fields, bit for bit.  Mixed mode: the folded step stores the RK increments
-of substeps 2 and 3 over the step's initial velocity (the corrected velocity of substeps 1
-and 2 is never stored), so they round in FP32 differently from the substep calls'
-increments over the corrected velocity; the bar there is 1e-6 of each field's largest
-value (the mixed mode's bar against the FP64 oracle), and the oracle comparison of the
-folded mixed step is in the mixed-mode tests.
+FP64: the same fields, bit for bit.  Mixed mode: compared at 1e-6 of each field's largest value
+(the mixed mode's bar against the FP64 oracle; the oracle comparison of the folded mixed step is
+in the mixed-mode tests).  With the intermediate velocity as FP32 increments
+(cans_ctx_set_mixed_increments) the folded step stores them over the step's initial velocity
+(the corrected velocity of substeps 1 and 2 is never stored), the substep calls over the
+corrected one, so they round differently.
 
 The cases cover every boundary rule the fold reproduces: periodic directions (the ghost
 copies bounduvw makes after correc and the ghosts that keep correc's own value), walls with
@@ -26,7 +26,7 @@ CASES = [("channel", (64, 32, 48)), ("tgv", (32, 32, 32)), ("duct", (32, 16, 24)
          ("open_nd", (16, 24, 12)), ("cavity_dd", (32, 32, 24)), ("ldc", (2, 64, 64)), ("open_x", (64, 16, 12))]
 
 
-def run_pair(name, ng, precision, nsteps=3, dt=None, refine=None):
+def run_pair(name, ng, precision, nsteps=3, dt=None, refine=None, increments=False):
     import torch
     from cans_amd import _lib as LB
     from cans_amd.host import Simulation
@@ -43,6 +43,8 @@ def run_pair(name, ng, precision, nsteps=3, dt=None, refine=None):
         sim.init_bounds()
         if refine:
             sim.set_refinement(*refine)
+        if increments:
+            sim.set_mixed_increments(True)
     dt = dt or (2e-4 if name == "channel" else 1e-3)
     for _ in range(nsteps):
         sims[0].rk3_step(dt)
@@ -93,6 +95,13 @@ def test_rk3_step_equals_substeps(torch, name, ng, precision):
 
 def test_rk3_step_refined_last_substep(torch):
     run_pair("channel", (64, 32, 48), "mixed", refine=(2, 4))
+
+
+@pytest.mark.parametrize("name,ng", [("channel", (64, 32, 48)), ("cavity_dd", (32, 32, 24)), ("ldc", (2, 64, 64))])
+def test_rk3_step_mixed_increments(torch, name, ng):
+    """The intermediate velocity as FP32 increments over the step's velocity: the folded substeps
+    store theirs over the step's initial velocity, the unfolded ones over the corrected one."""
+    run_pair(name, ng, "mixed", increments=True)
 
 
 def test_rk3_step_with_scalars_unfolded(torch):
