@@ -1,10 +1,13 @@
-"""Gradients through RK3 steps on the GPU (tri-periodic uniform grids, FP64).
+"""Gradients through RK3 steps on the GPU (FP64; boxes periodic in x and y, with a
+periodic z on a uniform grid or walls in z on any grid; bulk forcing of u and v).
 
 ``kinetic_energy_after_steps(sim, u0, v0, w0, dt, nsteps, every)`` runs nsteps fused
 RK3 steps (cans_rk3_substep) from the interior velocity (u0, v0, w0) with zero
 pressure and returns the volume-averaged kinetic energy of the final velocity as a
 torch scalar whose backward pass runs the adjoint substeps
-(cans_rk3_substep_adjoint).  The forward pass keeps the velocity and pressure every
+(cans_rk3_substep_adjoint).  The average weighs u and v by their cells' dzf(k) and
+w by dzc(k); w's wall face k = n3, which the boundary conditions fix, is left out,
+and its gradient is zero.  The forward pass keeps the velocity and pressure every
 `every` steps; the backward pass restarts from each checkpoint in turn (last first),
 recomputes its steps while keeping the velocity at the start of every substep, and
 runs their adjoints in reverse.  Recomputed steps repeat the forward ones bit for bit
@@ -20,6 +23,18 @@ from cans_amd import _lib as LB
 
 def _interior(f):
     return f.view[1:-1, 1:-1, 1:-1]
+
+
+def _weights(sim, like):
+    """Per-k volume weights of u, v, w divided by the box volume (torch (n3, 1, 1) columns)."""
+    g, n = sim.grid, sim.case.n
+    n3 = n[2]
+    vol = float(g.dzf[1:n3 + 1].sum()) * n[0] * n[1]
+    wf = torch.tensor(g.dzf[1:n3 + 1] / vol, dtype=like.dtype, device=like.device).view(n3, 1, 1)
+    wc = torch.tensor(g.dzc[1:n3 + 1] / vol, dtype=like.dtype, device=like.device).view(n3, 1, 1)
+    if sim.case.cbcpre[4:6] != "PP":
+        wc[-1] = 0.
+    return wf, wf, wc
 
 
 def _addr(f, buf):
@@ -44,19 +59,20 @@ class _KineticEnergyAfterSteps(torch.autograd.Function):
                 ckpt[s] = [f.buf.clone() for f in (sim.u, sim.v, sim.w, sim.p)]
             sim.rk3_step(dt)
         fin = [_interior(f).clone() for f in (sim.u, sim.v, sim.w)]
-        E = 0.5 * sum((x * x).mean() for x in fin)
+        wts = _weights(sim, fin[0])
+        E = 0.5 * sum((m * x * x).sum() for m, x in zip(wts, fin))
         ctx.sim, ctx.dt, ctx.nsteps, ctx.every, ctx.ckpt, ctx.fin = sim, dt, nsteps, every, ckpt, fin
         return E
 
     @staticmethod
     def backward(ctx, gE):
         sim, dt, nsteps, every = ctx.sim, ctx.dt, ctx.nsteps, ctx.every
-        ncell = ctx.fin[0].numel()
+        wts = _weights(sim, ctx.fin[0])
         # adjoint velocity and right-hand side, each with a second set the substeps write into
         ub, ubo, fb, fbo = ([sim.ctx.field() for _ in range(3)] for _ in range(4))
         pp = sim.ctx.field(kind=LB.CANS_FIELD_PRESSURE)
-        for f, x in zip(ub, ctx.fin):  # dE/du = u / N
-            _interior(f).copy_(x * (gE / ncell))
+        for f, m, x in zip(ub, wts, ctx.fin):  # dE/du = m u
+            _interior(f).copy_(m * x * gE)
         fields = (sim.u, sim.v, sim.w, sim.p)
         for s0 in sorted(ctx.ckpt, reverse=True):
             for f, b in zip(fields, ctx.ckpt[s0]):

@@ -1,14 +1,24 @@
-// Adjoint kernels of the RK3 substep on tri-periodic uniform grids.
+// Adjoint kernels of the RK3 substep on boxes periodic in x and y, either periodic
+// in z on a uniform grid or bounded by walls in z on any grid.
 //
 // The momentum right-hand side of mom_xyz_ad (src/mom.f90:771-1019) is, for
-// component c and with K^c_d(x) = (q_d(x) + q_d(x + e_c))/4,
-//   F_c(x) = -sum_d h_d [K^c_d(x) q_c(x + e_d) - K^c_d(x - e_d) q_c(x - e_d)] + nu L q_c,
-// quadratic in the velocity q.  With M^c_d(x) = g_c(x) q_c(x + e_d) - g_c(x + e_d) q_c(x),
-// the transpose of its Jacobian applied to g is
-//   (J^T g)_c(y) = -sum_d h_d [K^c_d(y - e_d) g_c(y - e_d) - K^c_d(y) g_c(y + e_d)]
-//                  - h_c/4 sum_c' [M^c'_c(y) + M^c'_c(y - e_c')] + nu L g_c(y).
-// Every value it reads lies within one cell of y (diagonal neighbours included), so
-// halo'd fields with periodic ghost cells serve as they are.
+// component c, with K^c_d(x) = alpha^c_d(k) q_d(x) + beta^c_d(k) q_d(x + e_c) (1/4 and
+// 1/4, except rdzm(k)/4 and rdzp(k)/4 for w advected along x and y) and H^c_d = 1/dx,
+// 1/dy along x, y and dzfi(k) (u, v) or dzci(k) (w) along z,
+//   F_c(x) = -sum_d H^c_d(k) [K^c_d(x) q_c(x + e_d) - K^c_d(x - e_d) q_c(x - e_d)] + nu L_c q_c,
+// quadratic in the velocity q.  With G^c_d(x) = H^c_d(k_x) g_c(x) and
+// M^c_d(x) = G^c_d(x) q_c(x + e_d) - G^c_d(x + e_d) q_c(x), the transpose of its
+// Jacobian applied to g is
+//   (J^T g)_c(y) = -sum_d [K^c_d(y - e_d) G^c_d(y - e_d) - K^c_d(y) G^c_d(y + e_d)]
+//                  - sum_c' [alpha^c'_c(k_y) M^c'_c(y) + beta^c'_c(k_y - [c' = z]) M^c'_c(y - e_c')]
+//                  + nu L_c^T g_c(y).
+// Every value it reads lies within one cell of y (diagonal neighbours included).  On
+// a uniform periodic grid the coefficients are constants and halo'd fields with
+// periodic ghost cells serve as they are.  With walls in z the sum also runs over the
+// ghost planes k = 0 and n3 + 1 (g is zero wherever F is not a degree of freedom:
+// those planes and w's wall face k = n3), and k_jt_fold folds the ghost planes into
+// the first and last interior planes by the derivative of bound.f90's ghost value
+// (u(0) = 2 ub - u(1): -1; a Neumann ghost: +1; w's wall faces are constants).
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -22,8 +32,10 @@ struct JtArgs {
   const double* base[3];  // added to the result (own cell only: may be out)
   double* out[3];
   double* fbo[3];         // COMB: receives f2 wb
-  double h[3];            // 1/dx, 1/dy, 1/dz
+  double h[3];            // 1/dx, 1/dy, 1/dz (periodic z)
   double visc, f1, f2;
+  const double* dzci;     // walls in z: dzci(0:n3+1), dzfi(0:n3+1)
+  const double* dzfi;
 };
 
 // A work-group owns a JT_X x JT_Y tile of (i, j) and marches over JT_K planes of k.  The
@@ -36,24 +48,31 @@ struct JtArgs {
 constexpr int JT_X = 32, JT_Y = 8, JT_K = 16;
 constexpr int JT_PX = JT_X + 2, JT_PLANE = (JT_X + 2) * (JT_Y + 2);
 
-template <bool COMB>
+// WALL: walls in z.  The output planes are k = 0 .. n3 + 1 (a work-group's first plane is
+// blockIdx.z * JT_K), the staged planes -1 .. n3 + 2, of which -1 and n3 + 2 read as zero.
+template <bool COMB, bool WALL>
 __global__ void __launch_bounds__(JT_X * JT_Y) k_mom_jt(JtArgs a) {
   __shared__ double sq[3][3][JT_PLANE];  // [component][slot][cell of the padded plane]
   __shared__ double sg[3][3][JT_PLANE];
   const Layout& L = a.L;
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * JT_X + tx;
   const int i0 = blockIdx.x * JT_X, j0 = blockIdx.y * JT_Y;  // padded cell (0, 0) is (i0, j0)
-  const int kb = blockIdx.z * JT_K, ke = min(kb + JT_K, L.n3);
+  const int kb = WALL ? (int)blockIdx.z * JT_K - 1 : (int)blockIdx.z * JT_K;
+  const int ke = WALL ? min(kb + JT_K, L.n3 + 1) : min(kb + JT_K, L.n3);
+  // plane kk exists in the halo'd fields; g_c is a degree of freedom on it
+  auto qplane = [&](int kk) { return !WALL || (kk >= 0 && kk <= L.n3 + 1); };
+  auto gplane = [&](int c, int kk) { return !WALL || (kk >= 1 && kk <= L.n3 - (c == 2)); };
   auto stage = [&](int kk, int slot) {
     for (int e = tid; e < JT_PLANE; e += JT_X * JT_Y) {
       const int i = i0 + e % JT_PX, j = j0 + e / JT_PX;
-      const bool in = i <= L.n1 + 1 && j <= L.n2 + 1;
-      const long o = L.at(i, j, kk);
+      const bool in = i <= L.n1 + 1 && j <= L.n2 + 1 && qplane(kk);
+      const long o = in ? L.at(i, j, kk) : 0;
 #pragma unroll
       for (int c = 0; c < 3; c++) {
+        const bool ing = in && gplane(c, kk);
         sq[c][slot][e] = in ? a.q[c][o] : 0.;
-        if constexpr (COMB) sg[c][slot][e] = in ? a.f1 * a.g[c][o] + a.fb[c][o] : 0.;
-        else sg[c][slot][e] = in ? a.g[c][o] : 0.;
+        if constexpr (COMB) sg[c][slot][e] = ing ? a.f1 * a.g[c][o] + a.fb[c][o] : 0.;
+        else sg[c][slot][e] = ing ? a.g[c][o] : 0.;
       }
     }
   };
@@ -66,13 +85,14 @@ __global__ void __launch_bounds__(JT_X * JT_Y) k_mom_jt(JtArgs a) {
     for (int r = 0; r < NE; r++) {
       const int e = tid + r * JT_X * JT_Y;
       const int i = i0 + e % JT_PX, j = j0 + e / JT_PX;
-      const bool in = e < JT_PLANE && i <= L.n1 + 1 && j <= L.n2 + 1;
+      const bool in = e < JT_PLANE && i <= L.n1 + 1 && j <= L.n2 + 1 && qplane(kk);
       const long o = in ? L.at(i, j, kk) : 0;
 #pragma unroll
       for (int c = 0; c < 3; c++) {
+        const bool ing = in && gplane(c, kk);
         rq[r][c] = in ? a.q[c][o] : 0.;
-        if constexpr (COMB) rg[r][c] = in ? a.f1 * a.g[c][o] + a.fb[c][o] : 0.;
-        else rg[r][c] = in ? a.g[c][o] : 0.;
+        if constexpr (COMB) rg[r][c] = ing ? a.f1 * a.g[c][o] + a.fb[c][o] : 0.;
+        else rg[r][c] = ing ? a.g[c][o] : 0.;
       }
     }
   };
@@ -91,7 +111,7 @@ __global__ void __launch_bounds__(JT_X * JT_Y) k_mom_jt(JtArgs a) {
   };
   stage(kb, 0);
   stage(kb + 1, 1);
-  stage(kb + 2, 2);  // kb < n3, so kb + 2 <= n3 + 1
+  stage(kb + 2, 2);  // periodic z: kb < n3, so kb + 2 <= n3 + 1; walls: planes past n3 + 1 read as zero
   __syncthreads();
   const int i = i0 + tx + 1, j = j0 + ty + 1;
   const bool own = i <= L.n1 && j <= L.n2;
@@ -105,40 +125,101 @@ __global__ void __launch_bounds__(JT_X * JT_Y) k_mom_jt(JtArgs a) {
       auto Q = [&](int c, const int* d) { return sq[c][sl[1 + d[2]]][at(d[0], d[1])]; };
       auto G = [&](int c, const int* d) { return sg[c][sl[1 + d[2]]][at(d[0], d[1])]; };
       const long o = L.at(i, j, k);
+      if constexpr (!WALL) {
 #pragma unroll
-      for (int m = 0; m < 3; m++) {
-        const int* em = ec[m];
-        const int z0[3] = {0, 0, 0};
-        const double g0 = G(m, z0);
-        double lap = 0.;
+        for (int m = 0; m < 3; m++) {
+          const int* em = ec[m];
+          const int z0[3] = {0, 0, 0};
+          const double g0 = G(m, z0);
+          double lap = 0.;
 #pragma unroll
-        for (int d = 0; d < 3; d++) {
-          const int p[3] = {ec[d][0], ec[d][1], ec[d][2]}, n[3] = {-ec[d][0], -ec[d][1], -ec[d][2]};
-          lap += a.h[d] * a.h[d] * (G(m, p) - 2. * g0 + G(m, n));
+          for (int d = 0; d < 3; d++) {
+            const int p[3] = {ec[d][0], ec[d][1], ec[d][2]}, n[3] = {-ec[d][0], -ec[d][1], -ec[d][2]};
+            lap += a.h[d] * a.h[d] * (G(m, p) - 2. * g0 + G(m, n));
+          }
+          double acc = a.visc * lap;
+          // g_m as the advected quantity: the coefficients K^m_d at y - e_d and y
+#pragma unroll
+          for (int d = 0; d < 3; d++) {
+            const int p[3] = {ec[d][0], ec[d][1], ec[d][2]}, n[3] = {-ec[d][0], -ec[d][1], -ec[d][2]};
+            const int nm[3] = {n[0] + em[0], n[1] + em[1], n[2] + em[2]};
+            const double kmm = 0.25 * (Q(d, n) + Q(d, nm));
+            const double km0 = 0.25 * (Q(d, z0) + Q(d, em));
+            acc -= a.h[d] * (kmm * G(m, n) - km0 * G(m, p));
+          }
+          // q_m as the advecting velocity of every component c (direction m)
+          double s = 0.;
+#pragma unroll
+          for (int c = 0; c < 3; c++) {
+            const int n[3] = {-ec[c][0], -ec[c][1], -ec[c][2]};
+            const int nm[3] = {n[0] + em[0], n[1] + em[1], n[2] + em[2]};
+            s += G(c, z0) * Q(c, em) - G(c, em) * Q(c, z0);
+            s += G(c, n) * Q(c, nm) - G(c, nm) * Q(c, n);
+          }
+          acc -= 0.25 * a.h[m] * s;
+          const double b = a.base[m][o];
+          a.out[m][o] = b + acc;
+          if constexpr (COMB) a.fbo[m][o] = a.f2 * b;  // base is wb
         }
-        double acc = a.visc * lap;
-        // g_m as the advected quantity: the coefficients K^m_d at y - e_d and y
+      } else {
+        const int n3 = L.n3;
+        auto zc = [&](int kk) { return kk >= 0 && kk <= n3 + 1 ? a.dzci[kk] : 0.; };
+        auto zf = [&](int kk) { return kk >= 0 && kk <= n3 + 1 ? a.dzfi[kk] : 0.; };
+        const double zcm = zc(k - 1), zc0 = zc(k), zcp = zc(k + 1);
+        const double zfm = zf(k - 1), zf0 = zf(k), zfp = zf(k + 1);
+        const bool in0 = k >= 1 && k <= n3, inm = k - 1 >= 1 && k - 1 <= n3;
+        // rdzm(k) = dzci(k)/dzfi(k), rdzp(k) = dzci(k)/dzfi(k+1) (mom.f90's weights of u, v at w's
+        // height); zero off the interior, where every term that would use them has g = 0
+        const double rdzm0 = in0 ? zc0 / zf0 : 0., rdzp0 = in0 ? zc0 / zfp : 0., rdzpm = inm ? zcm / zf0 : 0.;
+        // H^c_z on plane k + dk
+        auto hz = [&](int c, int dk) { return c == 2 ? (dk < 0 ? zcm : dk > 0 ? zcp : zc0)
+                                                     : (dk < 0 ? zfm : dk > 0 ? zfp : zf0); };
+        auto H = [&](int c, int d, int dk) { return d < 2 ? a.h[d] : hz(c, dk); };
 #pragma unroll
-        for (int d = 0; d < 3; d++) {
-          const int p[3] = {ec[d][0], ec[d][1], ec[d][2]}, n[3] = {-ec[d][0], -ec[d][1], -ec[d][2]};
-          const int nm[3] = {n[0] + em[0], n[1] + em[1], n[2] + em[2]};
-          const double kmm = 0.25 * (Q(d, n) + Q(d, nm));
-          const double km0 = 0.25 * (Q(d, z0) + Q(d, em));
-          acc -= a.h[d] * (kmm * G(m, n) - km0 * G(m, p));
-        }
-        // q_m as the advecting velocity of every component c (direction m)
-        double s = 0.;
+        for (int m = 0; m < 3; m++) {
+          const int* em = ec[m];
+          const int z0[3] = {0, 0, 0}, zp[3] = {0, 0, 1}, zm[3] = {0, 0, -1};
+          const double g0 = G(m, z0);
+          double lap = 0.;
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-          const int n[3] = {-ec[c][0], -ec[c][1], -ec[c][2]};
-          const int nm[3] = {n[0] + em[0], n[1] + em[1], n[2] + em[2]};
-          s += G(c, z0) * Q(c, em) - G(c, em) * Q(c, z0);
-          s += G(c, n) * Q(c, nm) - G(c, nm) * Q(c, n);
+          for (int d = 0; d < 2; d++) {
+            const int p[3] = {ec[d][0], ec[d][1], ec[d][2]}, n[3] = {-ec[d][0], -ec[d][1], -ec[d][2]};
+            lap += a.h[d] * a.h[d] * (G(m, p) - 2. * g0 + G(m, n));
+          }
+          if (m < 2) {  // L = dzfi(k) [dzci(k) d+ - dzci(k-1) d-]: L^T g = dzci(k) d+(dzfi g) - dzci(k-1) d-(dzfi g)
+            const double gp = zfp * G(m, zp), gc = zf0 * g0, gm = zfm * G(m, zm);
+            lap += zc0 * (gp - gc) - zcm * (gc - gm);
+          } else {      // L = dzci(k) [dzfi(k+1) d+ - dzfi(k) d-]
+            const double gp = zcp * G(m, zp), gc = zc0 * g0, gm = zcm * G(m, zm);
+            lap += zfp * (gp - gc) - zf0 * (gc - gm);
+          }
+          double acc = a.visc * lap;
+          // g_m as the advected quantity: K^m_d at y - e_d and y
+#pragma unroll
+          for (int d = 0; d < 3; d++) {
+            const int p[3] = {ec[d][0], ec[d][1], ec[d][2]}, n[3] = {-ec[d][0], -ec[d][1], -ec[d][2]};
+            const int nm[3] = {n[0] + em[0], n[1] + em[1], n[2] + em[2]};
+            const bool wt = m == 2 && d < 2;  // same k at y - e_d and y
+            const double al = wt ? 0.25 * rdzm0 : 0.25, be = wt ? 0.25 * rdzp0 : 0.25;
+            const double kmm = al * Q(d, n) + be * Q(d, nm);
+            const double km0 = al * Q(d, z0) + be * Q(d, em);
+            acc -= kmm * H(m, d, -ec[d][2]) * G(m, n) - km0 * H(m, d, ec[d][2]) * G(m, p);
+          }
+          // q_m as the advecting velocity of every component c (direction m)
+#pragma unroll
+          for (int c = 0; c < 3; c++) {
+            const int n[3] = {-ec[c][0], -ec[c][1], -ec[c][2]};
+            const int nm[3] = {n[0] + em[0], n[1] + em[1], n[2] + em[2]};
+            const int dkc = -ec[c][2], dkm = ec[m][2];
+            const double m0 = H(c, m, 0) * G(c, z0) * Q(c, em) - H(c, m, dkm) * G(c, em) * Q(c, z0);
+            const double m1 = H(c, m, dkc) * G(c, n) * Q(c, nm) - H(c, m, dkc + dkm) * G(c, nm) * Q(c, n);
+            const bool wt = c == 2 && m < 2;
+            acc -= (wt ? 0.25 * rdzm0 : 0.25) * m0 + (wt ? 0.25 * rdzpm : 0.25) * m1;
+          }
+          const double b = in0 ? a.base[m][o] : 0.;  // the ghost planes carry J^T g only
+          a.out[m][o] = b + acc;
+          if constexpr (COMB) a.fbo[m][o] = a.f2 * b;
         }
-        acc -= 0.25 * a.h[m] * s;
-        const double b = a.base[m][o];
-        a.out[m][o] = b + acc;
-        if constexpr (COMB) a.fbo[m][o] = a.f2 * b;  // base is wb
       }
     }
     __syncthreads();  // every lane is done with plane k-1
@@ -149,46 +230,107 @@ __global__ void __launch_bounds__(JT_X * JT_Y) k_mom_jt(JtArgs a) {
   }
 }
 
-static dim3 jt_grid(const Layout& L) { return dim3(cdiv(L.n1, JT_X), cdiv(L.n2, JT_Y), cdiv(L.n3, JT_K)); }
+static dim3 jt_grid(const Layout& L, bool wall) {
+  return dim3(cdiv(L.n1, JT_X), cdiv(L.n2, JT_Y), cdiv(wall ? L.n3 + 2 : L.n3, JT_K));
+}
 
-int launch_mom_jt(cans_ctx* c, const double* const* q, const double* const* g, const double* const* base,
-                  double* const* out, const double* h, double visc) {
+// Folds the ghost planes that k_mom_jt<.., true> wrote into the interior: u(1) += s0 u(0),
+// u(n3) += s1 u(n3 + 1) (likewise v; s = -1 for a Dirichlet wall, +1 for Neumann), and w's
+// wall face k = n3 is cleared.  One thread per (i, j).
+__global__ void k_jt_fold(Layout L, double* u, double* v, double* w, double su0, double su1, double sv0,
+                          double sv1) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1;
+  if (i > L.n1) return;
+  const long b = L.at(i, j, 0), t = L.at(i, j, L.n3), dz = L.sz;
+  u[b + dz] += su0 * u[b];
+  u[t] += su1 * u[t + dz];
+  v[b + dz] += sv0 * v[b];
+  v[t] += sv1 * v[t + dz];
+  w[t] = 0.;
+}
+
+// The volume weights of the projection's transpose P^T = M P M^-1 (M: dzf(k) for u, v; dzc(k)
+// for w) and the transposed bulk forcing, on the interior:
+//   inv:  u *= dzfi(k), v *= dzfi(k), w *= dzci(k);
+//   else: u = (u - mu) / dzfi(k), v = (v - mv) / dzfi(k), w = w / dzci(k)
+// (dzfi, dzci null: no weights, only the means subtracted).  wall: w's wall face k = n3 is
+// cleared.
+__global__ void k_adj_weigh(Layout L, double* u, double* v, double* w, const double* dzfi, const double* dzci,
+                            int inv, double mu, double mv, int wall) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
+  if (i > L.n1) return;
+  const long o = L.at(i, j, k);
+  if (inv) {
+    if (dzfi) {
+      u[o] *= dzfi[k];
+      v[o] *= dzfi[k];
+      w[o] *= dzci[k];
+    }
+  } else if (dzfi) {
+    u[o] = (u[o] - mu) / dzfi[k];
+    v[o] = (v[o] - mv) / dzfi[k];
+    w[o] = w[o] / dzci[k];
+  } else {
+    u[o] -= mu;
+    v[o] -= mv;
+  }
+  if (wall && k == L.n3) w[o] = 0.;
+}
+
+static JtArgs jt_args(cans_ctx* c, const double* const* q, const double* const* g, const double* const* fb,
+                      const double* const* base, double* const* out, double* const* fbo, const double* h,
+                      double visc, double f1, double f2, const double* dzci, const double* dzfi) {
   JtArgs a;
   a.L = c->L;
   for (int m = 0; m < 3; m++) {
     a.q[m] = q[m];
     a.g[m] = g[m];
-    a.fb[m] = nullptr;
+    a.fb[m] = fb ? fb[m] : nullptr;
     a.base[m] = base[m];
     a.out[m] = out[m];
-    a.fbo[m] = nullptr;
-    a.h[m] = h[m];
-  }
-  a.visc = visc;
-  a.f1 = a.f2 = 0.;
-  k_mom_jt<false><<<jt_grid(c->L), dim3(JT_X, JT_Y), 0, c->stream>>>(a);
-  CANS_LAUNCH_CHECK("k_mom_jt");
-  return CANS_OK;
-}
-
-int launch_mom_jt_adj(cans_ctx* c, const double* const* q, const double* const* wb, const double* const* fb,
-                      double* const* out, double* const* fbo, const double* h, double visc, double f1, double f2) {
-  JtArgs a;
-  a.L = c->L;
-  for (int m = 0; m < 3; m++) {
-    a.q[m] = q[m];
-    a.g[m] = wb[m];
-    a.fb[m] = fb[m];
-    a.base[m] = wb[m];
-    a.out[m] = out[m];
-    a.fbo[m] = fbo[m];
+    a.fbo[m] = fbo ? fbo[m] : nullptr;
     a.h[m] = h[m];
   }
   a.visc = visc;
   a.f1 = f1;
   a.f2 = f2;
-  k_mom_jt<true><<<jt_grid(c->L), dim3(JT_X, JT_Y), 0, c->stream>>>(a);
+  a.dzci = dzci;
+  a.dzfi = dzfi;
+  return a;
+}
+
+int launch_mom_jt(cans_ctx* c, const double* const* q, const double* const* g, const double* const* base,
+                  double* const* out, const double* h, double visc) {
+  const JtArgs a = jt_args(c, q, g, nullptr, base, out, nullptr, h, visc, 0., 0., nullptr, nullptr);
+  k_mom_jt<false, false><<<jt_grid(c->L, false), dim3(JT_X, JT_Y), 0, c->stream>>>(a);
   CANS_LAUNCH_CHECK("k_mom_jt");
+  return CANS_OK;
+}
+
+int launch_mom_jt_adj(cans_ctx* c, const double* const* q, const double* const* wb, const double* const* fb,
+                      double* const* out, double* const* fbo, const double* h, double visc, double f1, double f2,
+                      const AdjWalls* wall) {
+  const JtArgs a = jt_args(c, q, wb, fb, wb, out, fbo, h, visc, f1, f2, wall ? wall->dzci : nullptr,
+                           wall ? wall->dzfi : nullptr);
+  if (!wall) {
+    k_mom_jt<true, false><<<jt_grid(c->L, false), dim3(JT_X, JT_Y), 0, c->stream>>>(a);
+    CANS_LAUNCH_CHECK("k_mom_jt");
+    return CANS_OK;
+  }
+  k_mom_jt<true, true><<<jt_grid(c->L, true), dim3(JT_X, JT_Y), 0, c->stream>>>(a);
+  CANS_LAUNCH_CHECK("k_mom_jt");
+  k_jt_fold<<<dim3(cdiv(c->L.n1, 64), c->L.n2), 64, 0, c->stream>>>(c->L, out[0], out[1], out[2], wall->sign[0],
+                                                                      wall->sign[1], wall->sign[2], wall->sign[3]);
+  CANS_LAUNCH_CHECK("k_jt_fold");
+  return CANS_OK;
+}
+
+int launch_adj_weigh(cans_ctx* c, double* const* f, const double* dzfi, const double* dzci, bool inv, double mu,
+                     double mv, bool wall) {
+  const Layout& L = c->L;
+  k_adj_weigh<<<dim3(cdiv(L.n1, 64), L.n2, L.n3), 64, 0, c->stream>>>(L, f[0], f[1], f[2], dzfi, dzci, inv, mu, mv,
+                                                                      wall);
+  CANS_LAUNCH_CHECK("k_adj_weigh");
   return CANS_OK;
 }
 

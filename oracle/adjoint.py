@@ -3,38 +3,59 @@
 Like the rest of ``oracle/``, this module is the checker for the HIP path: only
 ``tests/`` and ``bench.py``'s cpu_baseline leg may use it.
 
-Scope: tri-periodic boxes on uniform grids, explicit diffusion, no forcing,
-scalars or buoyancy (a Taylor-Green flow).  There the velocity map of RK3
-substep irk (src/main.f90:437-473 with rk.f90:24-333, fillps, solver, correc) is
+Scope: boxes periodic in x and y, either periodic in z on a uniform grid or
+bounded by walls in z on any (stretched) grid (Neumann pressure, Dirichlet w,
+Dirichlet or Neumann u and v); explicit diffusion, bulk forcing of u and v, no
+scalars.  The velocity map of RK3 substep irk (src/main.f90:437-473 with
+rk.f90:24-333, fillps, solver, correc) on the degrees of freedom (every interior
+u, v; w without the wall face k = n3, which bound.f90 fixes) is
 
-    u' = P(u + f1 F(u) + f2 Fo),      Fo' = F(u),
+    u' = P Fr(u + f1 F(B u) + f2 Fo),      Fo' = F(B u),
 
-with F the momentum right-hand side of mom_xyz_ad (mom.f90:771-1019), f1 =
-rkcoeff(1,irk) dt, f2 = rkcoeff(2,irk) dt (rk.f90:78-80) and P = I - G L^-1 D the
-projection that fillps/solver/correc apply.  The rk's -dtrk G p term drops out:
-the solver's eigenvalues are those of the same 7-point Laplacian L = D G, so
-P G p = 0 and the pressure only follows the velocity.  On a uniform periodic
-grid G = -D^T, so P is symmetric and the adjoint of a substep, given the
-adjoints ub of u' and fb of Fo', is
+with F the momentum right-hand side of mom_xyz_ad (mom.f90:771-1019) at the
+velocity whose ghost cells B sets (bound.f90:17-122: affine in the interior),
+f1 = rkcoeff(1,irk) dt, f2 = rkcoeff(2,irk) dt (rk.f90:78-80), Fr the bulk forcing
+u <- u + (velf - sum_k gvr_f(k) u) (rk.f90:493-518, mom.f90:725-769) and
+P = I - G L^-1 D the projection that fillps/solver/correc apply.  The rk's
+-dtrk G p term drops out: the solver's eigenvalues and tridiagonal coefficients
+are those of the same 7-point L = D G (homogeneous Neumann rows at the walls),
+so P G p = 0 and the pressure only follows the velocity.
 
-    wb = P ub,   ub <- wb + J(u)^T (f1 wb + fb),   fb <- f2 wb,
+P is self-adjoint in the inner product weighted by the cell volumes M (dzf(k)
+for u and v, dzc(k) for w; D and G are negative adjoints there:
+sum_k dzf(k) p(k) (w(k) - w(k-1))/dzf(k) = -sum_k dzc(k) w(k) (p(k+1) - p(k))/dzc(k)
+with w = 0 on the walls), so its transpose is P^T = M P M^-1; on a uniform
+periodic grid M is a multiple of I and P^T = P.  The forcing's Jacobian
+I - 1 gvr_f^T has the transpose I - gvr_f 1^T.  Given the adjoints ub of u' and
+fb of Fo', the adjoint of a substep is
 
-where J(u) is the Jacobian of F at the substep's starting velocity.
+    wb = Fr'^T P^T ub,   ub <- wb + J(u)^T (f1 wb + fb),   fb <- f2 wb,
 
-J^T: every advection term of mom_xyz_ad has the form K(x) q_c(x +- e_d) with
-K^c_d(x) = (q_d(x) + q_d(x + e_c))/4 (c the component, d the direction; e.g.
-uuip = 0.25 (u + u(i+1)) u(i+1), vujm = 0.25 (v(j-1) + v(i+1,j-1)) u(j-1)), so
+with J(u) the Jacobian of F(B u) at the substep's starting velocity.
 
-    F_c(x) = -sum_d h_d [K^c_d(x) q_c(x+e_d) - K^c_d(x-e_d) q_c(x-e_d)] + nu L q_c,
+J^T: every advection term of mom_xyz_ad has the form K(x) q_c(x +- e_d), with
+K^c_d(x) = alpha^c_d(k) q_d(x) + beta^c_d(k) q_d(x + e_c): 1/4 and 1/4, except
+for w advected along x and y on a stretched grid, where the reference weighs u
+and v by rdzm(k)/4 = dzci(k)/dzfi(k)/4 and rdzp(k)/4 = dzci(k)/dzfi(k+1)/4
+(e.g. uuip = 0.25 (u + u(i+1)) u(i+1), uwip = 0.25 (rdzm u + rdzp u(k+1)) w(i+1)),
+so that, with H^c_d = dxi, dyi along x, y and dzfi(k) (u, v) or dzci(k) (w) along z,
 
-and with M^c_d(x) = g_c(x) q_c(x+e_d) - g_c(x+e_d) q_c(x),
+    F_c(x) = -sum_d H^c_d(k) [K^c_d(x) q_c(x+e_d) - K^c_d(x-e_d) q_c(x-e_d)] + visc L_c q_c.
 
-    (J^T g)_c(y) = -sum_d h_d [K^c_d(y-e_d) g_c(y-e_d) - K^c_d(y) g_c(y+e_d)]
-                   - h_c/4 sum_c' [M^c'_c(y) + M^c'_c(y-e_c')] + nu L g_c(y).
+With G^c_d(x) = H^c_d(k_x) g_c(x) and M^c_d(x) = G^c_d(x) q_c(x+e_d) - G^c_d(x+e_d) q_c(x),
+
+    (J^T g)_c(y) = -sum_d [K^c_d(y-e_d) G^c_d(y-e_d) - K^c_d(y) G^c_d(y+e_d)]
+                   -sum_c' [alpha^c'_c(k_y) M^c'_c(y) + beta^c'_c(k_y - [c' = z]) M^c'_c(y-e_c')]
+                   + visc L_c^T g_c(y),
+
+evaluated at every cell and ghost cell y, g being zero wherever F is not a degree
+of freedom; B^T then folds the ghost values into the interior (bound.f90:182-463:
+u(0) = 2 ub - u(1) gives -1, a Neumann ghost +1; w's wall faces are constants).
 
 The tests pin mom_jt by dot products against central differences of the
-oracle's own F (exact: F is quadratic), P by symmetry and idempotence, and the
-whole gradient by central differences of the oracle's RK3 steps.
+oracle's own F (exact: F is quadratic) through the oracle's own ghost map, P^T by
+<P^T a, b> = <a, P b>, and the whole gradient by central differences of the
+oracle's RK3 steps.
 """
 import numpy as np
 
@@ -50,13 +71,25 @@ def halo(case, a):
     return np.asfortranarray(np.pad(a, 1, mode="wrap"))
 
 
+def walls(case):
+    """True when z is bounded by walls (the pressure is then Neumann there)."""
+    return case.cbcpre[4:6] != "PP"
+
+
 def check_case(case):
-    if case.cbcpre != "PP" * 3 or case.cbcvel != "PP" * 9:
-        raise ValueError("the adjoint covers tri-periodic boxes")
-    if np.ptp(case.dzci) > 1e-12 * np.abs(case.dzci).max():
-        raise ValueError("the adjoint covers uniform grids")
-    if any(case.is_forced) or case.scalars or case.is_impdiff:
-        raise ValueError("the adjoint covers unforced flows with explicit diffusion and no scalars")
+    per_xy = case.cbcpre[:4] == "PPPP" and all(case.cbcvel[6 * c + q] == "P" for c in range(3) for q in range(4))
+    if not per_xy:
+        raise ValueError("the adjoint covers boxes periodic in x and y")
+    if walls(case):
+        if (case.cbcpre[4:6] != "NN" or case.cbcvel[16:18] != "DD"
+                or any(case.cbcvel[6 * c + 4 + q] not in "DN" for c in (0, 1) for q in (0, 1))):
+            raise ValueError("z walls: Neumann pressure, Dirichlet w, Dirichlet or Neumann u and v")
+        if case.n[2] < 2:
+            raise ValueError("z walls need n3 >= 2")
+    elif np.ptp(case.dzci) > 1e-12 * np.abs(case.dzci).max():
+        raise ValueError("a periodic z needs a uniform grid")
+    if case.is_forced[2] or case.scalars or case.is_impdiff:
+        raise ValueError("the adjoint covers explicit diffusion, forcing of u and v only and no scalars")
 
 
 def mom(case, u, v, w):
@@ -102,26 +135,162 @@ def project(case, a):
     return [interior(x) for x in (u, v, w)]
 
 
-def kinetic_energy(u, v, w):
-    """Volume-averaged kinetic energy of interior arrays on a uniform grid."""
-    return 0.5 * float(np.mean(u * u) + np.mean(v * v) + np.mean(w * w))
+class _Homogeneous:
+    """The case with zero boundary values: the linear part of bound.f90's ghost map."""
+
+    def __init__(self, case):
+        self.case = case
+
+    def __enter__(self):
+        self.saved = (self.case.bcvel, self.case.bcpre)
+        self.case.bcvel = np.zeros_like(self.case.bcvel)
+        self.case.bcpre = np.zeros_like(self.case.bcpre)
+        return self.case
+
+    def __exit__(self, *exc):
+        self.case.bcvel, self.case.bcpre = self.saved
+
+
+def dof_fields(case, a):
+    """Three interior arrays -> halo'd fields with the homogeneous ghost values of bounduvw."""
+    f = [O.field(case.n) for _ in range(3)]
+    for x, y in zip(f, a):
+        x[1:-1, 1:-1, 1:-1] = y
+    if walls(case):
+        f[2][:, :, -2] = 0.  # w(n3) is a wall face
+    with _Homogeneous(case) as hc:
+        hc.bounduvw(*f)
+    return f
+
+
+def project_dofs(case, a):
+    """P a on the degrees of freedom (walls: homogeneous ghosts, w(n3) = 0)."""
+    u, v, w = dof_fields(case, a)
+    pp = O.field(case.n)
+    with _Homogeneous(case) as hc:
+        hc.fillps(u, v, w, pp, 1.0)
+        hc.solver(pp)
+        hc.boundp(pp)
+        hc.correc(pp, u, v, w, 1.0)
+    return [interior(x) for x in (u, v, w)]
+
+
+def volume_weights(case):
+    """Per-k volume weights of u, v and w (broadcast over i, j): dzf, dzf, dzc; 0 on w's wall face."""
+    n3 = case.n[2]
+    wf = np.asarray(case.dzf[1:n3 + 1], dtype=np.float64).copy()
+    wc = np.asarray(case.dzc[1:n3 + 1], dtype=np.float64).copy()
+    if walls(case):
+        wc[-1] = 0.
+    return [x[None, None, :] for x in (wf, wf, wc)]
+
+
+def project_t(case, a):
+    """P^T a = M P M^-1 a (P^T = P on a uniform periodic grid)."""
+    if not walls(case):
+        return project(case, a)
+    m = volume_weights(case)
+    safe = [np.where(x > 0, x, 1.) for x in m]
+    pa = project_dofs(case, [y / s for y, s in zip(a, safe)])
+    return [x * y for x, y in zip(pa, m)]
+
+
+def forcing_t(case, b):
+    """Transpose of the bulk forcing's Jacobian, I - gvr_f 1^T, on each forced component."""
+    n3 = case.n[2]
+    gvr = np.asarray(case.grid_vol_ratio_f[1:n3 + 1], dtype=np.float64)[None, None, :]
+    out = [x.copy() for x in b]
+    for c in (0, 1):
+        if case.is_forced[c]:
+            out[c] = out[c] - gvr * float(np.sum(b[c]))
+    return out
+
+
+def mom_jt_walls(case, qh, g, fold=True, swap_rdz=False):
+    """(d F(B q)/dq)^T g on a box with walls in z: qh three halo'd fields (ghosts as the forward set
+    them), g three interior arrays (g_w at k = n3 is ignored).  Returns three interior arrays, w's wall
+    face zero.  fold=False and swap_rdz=True plant mistakes for the tests' negative controls."""
+    n1, n2, n3 = case.n
+    # planes kk = k + 1 for k = -1 .. n3 + 2; the planes -1 and n3 + 2 are zero
+    Q = [np.pad(x[1:-1, 1:-1, :], ((0, 0), (0, 0), (1, 1))) for x in qh]
+    G = [np.pad(x, ((0, 0), (0, 0), (2, 2))) for x in g]
+    G[2][:, :, n3 + 1] = 0.
+    zc, zf = np.zeros(n3 + 4), np.zeros(n3 + 4)
+    zc[1:n3 + 3], zf[1:n3 + 3] = case.dzci, case.dzfi
+    rdzm, rdzp = np.zeros(n3 + 4), np.zeros(n3 + 4)
+    rdzm[2:n3 + 2] = zc[2:n3 + 2] / zf[2:n3 + 2]
+    rdzp[2:n3 + 2] = zc[2:n3 + 2] / zf[3:n3 + 3]
+    if swap_rdz:
+        rdzm, rdzp = rdzp, rdzm
+    zc_, zf_, rdzm_, rdzp_ = (x[None, None, :] for x in (zc, zf, rdzm, rdzp))
+    h = case.dli[:2]
+
+    def H(c, d):
+        return h[d] if d < 2 else (zc_ if c == 2 else zf_)
+
+    def al(c, d):
+        return 0.25 * rdzm_ if (c == 2 and d < 2) else 0.25
+
+    def be(c, d):
+        return 0.25 * rdzp_ if (c == 2 and d < 2) else 0.25
+
+    out = [np.zeros_like(x) for x in G]
+    for m in range(3):  # visc L_m^T g_m
+        gm = G[m]
+        xy = sum(h[d] ** 2 * (_sh(gm, d, 1) - 2 * gm + _sh(gm, d, -1)) for d in range(2))
+        if m < 2:
+            gh = zf_ * gm
+            z = zc_ * (_sh(gh, 2, 1) - gh) - _sh(zc_, 2, -1) * (gh - _sh(gh, 2, -1))
+        else:
+            gh = zc_ * gm
+            z = _sh(zf_, 2, 1) * (_sh(gh, 2, 1) - gh) - zf_ * (gh - _sh(gh, 2, -1))
+        out[m] += case.visc * (xy + z)
+    for c in range(3):
+        for d in range(3):
+            K = al(c, d) * Q[d] + be(c, d) * _sh(Q[d], c, 1)
+            Gd = H(c, d) * G[c]
+            out[c] -= _sh(K * Gd, d, -1) - K * _sh(Gd, d, 1)
+            M = Gd * _sh(Q[c], d, 1) - _sh(Gd, d, 1) * Q[c]
+            out[d] -= al(c, d) * M + _sh(be(c, d) * M, c, -1)
+    res = [np.asfortranarray(x[:, :, 2:n3 + 2]) for x in out]
+    for m in (0, 1) if fold else ():  # B^T: tangential ghosts u(0) = 2 ub - u(1) (D) or u(0) = u(1) + dzc(0) ub (N)
+        s0 = -1. if case.cbcvel[6 * m + 4] == "D" else 1.
+        s1 = -1. if case.cbcvel[6 * m + 5] == "D" else 1.
+        res[m][:, :, 0] += s0 * out[m][:, :, 1]
+        res[m][:, :, n3 - 1] += s1 * out[m][:, :, n3 + 2]
+    res[2][:, :, n3 - 1] = 0.  # w's wall faces are constants
+    return res
+
+
+def kinetic_energy(case, u, v, w):
+    """Volume-averaged kinetic energy of the interior degrees of freedom (dzf- and dzc-weighted)."""
+    wts = volume_weights(case)
+    vol = float(np.sum(case.dzf[1:case.n[2] + 1])) * case.n[0] * case.n[1]
+    return 0.5 * sum(float(np.sum(m * q * q)) for m, q in zip(wts, (u, v, w))) / vol
+
+
+def ke_grad(case, u, v, w):
+    """d kinetic_energy / d(u, v, w)."""
+    wts = volume_weights(case)
+    vol = float(np.sum(case.dzf[1:case.n[2] + 1])) * case.n[0] * case.n[1]
+    return [np.asfortranarray(m * q / vol) for m, q in zip(wts, (u, v, w))]
 
 
 def _state(case, u0, v0, w0):
     st = case.new_state()
     for k, q in zip("uvw", (u0, v0, w0)):
-        st[k][...] = halo(case, q)
+        st[k][1:-1, 1:-1, 1:-1] = q
     case.init_bounds(st)
     return st
 
 
 def _step_recording(case, st, dt):
-    """One RK3 step (the oracle's rk3_step sequence) returning the interior velocity at the start of each
+    """One RK3 step (the oracle's rk3_step sequence) returning the halo'd velocity at the start of each
     substep."""
     u, v, w, p, pp = st["u"], st["v"], st["w"], st["p"], st["pp"]
     starts = []
     for irk in range(3):
-        starts.append([interior(x) for x in (u, v, w)])
+        starts.append([x.copy(order="F") for x in (u, v, w)])
         rkpar = O.RKCOEFF[irk]
         dtrk = (rkpar[0] + rkpar[1]) * dt
         case.rk(rkpar, dt, p, st["dudtrko"], st["dvdtrko"], st["dwdtrko"], u, v, w)
@@ -137,16 +306,23 @@ def _step_recording(case, st, dt):
     return starts
 
 
+def substep_adjoint(case, start, irk, dt, ub, fb):
+    """Adjoint of substep irk (0-based) at the halo'd starting velocity `start`: (ub, fb) -> (ub, fb)."""
+    f1, f2 = O.RKCOEFF[irk][0] * dt, O.RKCOEFF[irk][1] * dt
+    wb = forcing_t(case, project_t(case, ub))
+    g = [f1 * a + b for a, b in zip(wb, fb)]
+    if walls(case):
+        jt = mom_jt_walls(case, start, g)
+    else:
+        jt = mom_jt(case, [interior(x) for x in start], g)
+    return [a + b for a, b in zip(wb, jt)], [f2 * a for a in wb]
+
+
 def step_adjoint(case, starts, ub, dt):
     """Adjoint of one step: ub (adjoint of the step's final velocity) -> adjoint of its initial velocity."""
     fb = [np.zeros_like(x) for x in ub]
     for irk in (2, 1, 0):
-        f1, f2 = O.RKCOEFF[irk][0] * dt, O.RKCOEFF[irk][1] * dt
-        wb = project(case, ub)
-        g = [f1 * a + b for a, b in zip(wb, fb)]
-        fb = [f2 * a for a in wb]
-        jt = mom_jt(case, starts[irk], g)
-        ub = [a + b for a, b in zip(wb, jt)]
+        ub, fb = substep_adjoint(case, starts[irk], irk, dt, ub, fb)
     return ub
 
 
@@ -155,7 +331,7 @@ def ke_after_steps(case, u0, v0, w0, dt, nsteps):
     st = _state(case, u0, v0, w0)
     for _ in range(nsteps):
         case.rk3_step(st, dt)
-    return kinetic_energy(*(interior(st[k]) for k in "uvw"))
+    return kinetic_energy(case, *(interior(st[k]) for k in "uvw"))
 
 
 def grad_ke_after_steps(case, u0, v0, w0, dt, nsteps, every=4):
@@ -169,9 +345,8 @@ def grad_ke_after_steps(case, u0, v0, w0, dt, nsteps, every=4):
             ckpt[s] = {k: st[k].copy(order="F") for k in st}
         case.rk3_step(st, dt)
     fin = [interior(st[k]) for k in "uvw"]
-    E = kinetic_energy(*fin)
-    ncell = float(np.prod(case.n))
-    ub = [x / ncell for x in fin]
+    E = kinetic_energy(case, *fin)
+    ub = ke_grad(case, *fin)
     for s0 in sorted(ckpt, reverse=True):
         seg = {k: v.copy(order="F") for k, v in ckpt[s0].items()}
         rec = [_step_recording(case, seg, dt) for _ in range(s0, min(s0 + every, nsteps))]

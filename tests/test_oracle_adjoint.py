@@ -86,3 +86,108 @@ def test_gradient_matches_central_differences():
         fd = (ep - em) / (2 * eps)
         ad = dot(grad, d)
         assert abs(fd - ad) <= 1e-6 * abs(ad), (fd, ad)
+
+
+# ---- walls in z on a stretched grid (the channel), bulk forcing ----
+
+def channel_case(ng=(8, 6, 10), gr=1.8, cbcvel="PPPPDD" * 3, bcvel=None, is_forced=(1, 0, 0), visci=90.0):
+    return O.OracleCase(ng=ng, l=(2.0, 1.2, 1.0), gtype=1, gr=gr, cbcvel=cbcvel, cbcpre="PPPPNN",
+                        bcvel=bcvel, visci=visci, is_forced=is_forced, velf=(1.0, 0.0, 0.0))
+
+
+def dofs(case, rng):
+    """Random values on the degrees of freedom (w's wall face k = n3 zero)."""
+    a = [rng.uniform(-1, 1, tuple(case.n)) for _ in range(3)]
+    a[2][:, :, -1] = 0.
+    return a
+
+
+def channel_state(case, rng, noise=0.1):
+    """Laminar profile u = 6 z (1 - z) on the cell centres plus uniform noise (initflow.f90:44-63)."""
+    zc = np.asarray(case.zc[1:case.n[2] + 1])
+    u = np.broadcast_to(6 * zc * (1 - zc), tuple(case.n))
+    q = [u, np.zeros(tuple(case.n)), np.zeros(tuple(case.n))]
+    out = [np.asfortranarray(x + noise * rng.uniform(-1, 1, x.shape)) for x in q]
+    out[2][:, :, -1] = 0.
+    return out
+
+
+def ghosts(case, a):
+    """Interior arrays -> halo'd fields with the case's own ghost values (bounduvw, non-zero wall velocity)."""
+    f = [O.field(case.n) for _ in range(3)]
+    for x, y in zip(f, a):
+        x[1:-1, 1:-1, 1:-1] = y
+    case.bounduvw(*f)
+    return f
+
+
+def test_mom_jt_walls_dot_product():
+    # stretched z, a moving top wall for u (Dirichlet 1) and a Neumann v at the bottom: both folds
+    bc = np.zeros(18)
+    bc[5] = 1.0
+    bc[10] = 0.3
+    case = channel_case(cbcvel="PPPPDD" + "PPPPND" + "PPPPDD", bcvel=bc, visci=37.0)
+    rng = np.random.default_rng(11)
+    q, d, g = dofs(case, rng), dofs(case, rng), dofs(case, rng)
+    fp = A.mom(case, *ghosts(case, [a + b for a, b in zip(q, d)]))
+    fm = A.mom(case, *ghosts(case, [a - b for a, b in zip(q, d)]))
+    jd = [(a - b) / 2 for a, b in zip(fp, fm)]
+    jd[2][:, :, -1] = 0.  # F_w on the wall face is overwritten by bound
+    lhs = dot(g, jd)
+    jt = A.mom_jt_walls(case, ghosts(case, q), g)
+    assert np.all(jt[2][:, :, -1] == 0.)
+    rhs = dot(jt, d)
+    scale = sum(float(np.abs(x).sum()) for x in jd)
+    assert abs(lhs - rhs) <= 1e-13 * scale, (lhs, rhs)
+    # planted mistakes: the ghost fold dropped, rdzm and rdzp swapped, the uniform-grid formula
+    for bad in (A.mom_jt_walls(case, ghosts(case, q), g, fold=False),
+                A.mom_jt_walls(case, ghosts(case, q), g, swap_rdz=True)):
+        assert abs(dot(bad, d) - lhs) > 1e-6 * scale
+    assert abs(dot(A.mom_jt(case, q, g), d) - lhs) > 1e-6 * scale
+
+
+def test_projection_transpose_walls():
+    case = channel_case(ng=(8, 8, 12))
+    rng = np.random.default_rng(12)
+    a, b = dofs(case, rng), dofs(case, rng)
+    pa = A.project_dofs(case, a)
+    assert np.all(pa[2][:, :, -1] == 0.)
+    lhs, rhs = dot(A.project_t(case, a), b), dot(a, A.project_dofs(case, b))
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs), (lhs, rhs)
+    # the volume weights matter on the stretched grid: P itself is not symmetric
+    assert abs(dot(pa, b) - rhs) > 1e-6 * abs(rhs)
+    ppa = A.project_dofs(case, pa)
+    assert max(float(np.abs(x - y).max()) for x, y in zip(ppa, pa)) < 1e-12
+    _, divmax = case.chkdiv(*A.dof_fields(case, pa))
+    assert divmax < 1e-11
+
+
+def test_forcing_transpose():
+    case = channel_case(is_forced=(1, 1, 0))
+    rng = np.random.default_rng(13)
+    a, b = dofs(case, rng), dofs(case, rng)
+    gvr = np.asarray(case.grid_vol_ratio_f[1:case.n[2] + 1])[None, None, :]
+    fa = [x - float(np.sum(gvr * x)) if c < 2 else x for c, x in enumerate(a)]  # Jacobian of u + velf - <u>
+    assert abs(dot(A.forcing_t(case, b), a) - dot(b, fa)) <= 1e-13 * sum(float(np.abs(x).sum()) for x in b)
+    u = np.full(tuple(case.n), 0.7)
+    assert np.abs(A.forcing_t(case, [u, u, u])[0] - (0.7 - 0.7 * gvr * u.size)).max() < 1e-15 * u.size
+
+
+@pytest.mark.timeout(300)
+def test_gradient_channel_matches_central_differences():
+    case = channel_case(ng=(8, 6, 10), visci=60.0)
+    rng = np.random.default_rng(14)
+    u0 = channel_state(case, rng)
+    dt, nsteps = 2e-3, 4
+    E, grad = A.grad_ke_after_steps(case, *u0, dt, nsteps, every=2)
+    # the oracle's bulk mean is an OpenMP reduction, so repeated forward runs agree to round-off only
+    assert E == pytest.approx(A.ke_after_steps(case, *u0, dt, nsteps), rel=1e-13)
+    assert np.all(grad[2][:, :, -1] == 0.)
+    eps = 1e-6
+    for _ in range(3):
+        d = dofs(case, rng)
+        ep = A.ke_after_steps(case, *(a + eps * b for a, b in zip(u0, d)), dt, nsteps)
+        em = A.ke_after_steps(case, *(a - eps * b for a, b in zip(u0, d)), dt, nsteps)
+        fd = (ep - em) / (2 * eps)
+        ad = dot(grad, d)
+        assert abs(fd - ad) <= 1e-6 * abs(ad), (fd, ad)
