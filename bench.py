@@ -23,11 +23,12 @@ Secondary workloads, timed after the headline one and reported under "secondary"
 the 256^3 heated cube with a scalar, buoyancy and 3-D implicit diffusion (FP64 and mixed),
 64 independent Poisson boxes of different sizes solved as one batch (FP64 and mixed
 storage, beside the same boxes solved one at a time), and the gradient of the kinetic
-energy after 32 steps of the 256^3 Taylor-Green flow (forward and adjoint, FP64).
+energy after 32 steps of the 256^3 Taylor-Green flow and after 16 steps of a 512x256x192
+channel with walls in z (forward and adjoint, FP64).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--ng N1 N2 N3]
                        [--precision both|mixed|fp64] [--solves S] [--no-cpu] [--no-secondary]
-                       [--cavity-only] [--boxes-only] [--adjoint-only]
+                       [--cavity-only] [--boxes-only] [--adjoint-only [tgv|channel]]
 """
 import argparse
 import json
@@ -355,20 +356,34 @@ def secondary_boxes(args):
     return out
 
 
-def secondary_adjoint(args):
-    """Gradient of the kinetic energy after 32 RK3 steps of the 256^3 Taylor-Green flow (nu = 1/1600,
-    dt = 5e-4, FP64) w.r.t. the initial velocity: forward with checkpoints every 4 steps, backward with
-    recompute and the adjoint substeps (cans_amd.adjoint)."""
+ADJOINT_WORKLOADS = {
+    "tgv": dict(ng=(256, 256, 256), seed=1, dt=5e-4, nsteps=32, every=4,
+                text="gradient of the kinetic energy after 32 RK3 steps, Taylor-Green 256^3, nu=1/1600, "
+                     "dt=5e-4, FP64, checkpoints every 4 steps"),
+    "channel": dict(ng=(512, 256, 192), seed=3, dt=2e-4, nsteps=16, every=4,
+                    text="gradient of the kinetic energy after 16 RK3 steps, channel 512x256x192 (walls in z, "
+                         "tanh 1.8, constant flow rate), Re_b=5600, dt=2e-4, FP64, checkpoints every 4 steps"),
+}
+
+
+def secondary_adjoint(args, name="tgv"):
+    """Gradient of the kinetic energy after nsteps RK3 steps w.r.t. the initial velocity (FP64): forward with
+    checkpoints every 4 steps, backward with recompute and the adjoint substeps (cans_amd.adjoint).  tgv: the
+    256^3 Taylor-Green flow (nu = 1/1600, dt = 5e-4, 32 steps); channel: 512x256x192 with walls in z on the
+    tanh-stretched grid and bulk forcing (dt = 2e-4, 16 steps)."""
     import numpy as np
     import torch
     from cans_amd.adjoint import kinetic_energy_after_steps
     from cans_amd.host import Simulation
     from tests import cases
-    p = cases.params("tgv", ng=(256, 256, 256))
-    print("[bench] adjoint %s: set-up" % (p["ng"],), file=sys.stderr, flush=True)
+    wl = ADJOINT_WORKLOADS[name]
+    p = cases.params(name, ng=wl["ng"])
+    print("[bench] adjoint %s %s: set-up" % (name, p["ng"]), file=sys.stderr, flush=True)
     sim = Simulation(cases.host_case(p))
-    u0 = cases.initial_state("tgv", p, sim.grid, seed=1)[:3]
-    dt, nsteps, every = 5e-4, 32, 4
+    u0 = list(cases.initial_state(name, p, sim.grid, seed=wl["seed"])[:3])
+    if p["cbcpre"][4:6] != "PP":
+        u0[2][:, :, -1] = 0.  # w's wall face
+    dt, nsteps, every = wl["dt"], wl["nsteps"], wl["every"]
     res = {}
     for rep in range(2):  # the first run warms up (allocations, code loading)
         t0 = [torch.tensor(np.ascontiguousarray(a.transpose(2, 1, 0)), device="cuda", requires_grad=True)
@@ -386,12 +401,11 @@ def secondary_adjoint(args):
     sim.close()
     del sim, t0, E
     torch.cuda.empty_cache()
-    out = {"workload": "gradient of the kinetic energy after 32 RK3 steps, Taylor-Green 256^3, nu=1/1600, "
-                       "dt=5e-4, FP64, checkpoints every 4 steps", "steps": nsteps,
+    out = {"workload": wl["text"], "steps": nsteps,
            "forward_ms": res["forward_ms"], "backward_ms": res["backward_ms"],
            "backward_over_forward": res["backward_ms"] / res["forward_ms"],
            "ns_per_cell_step_forward": res["forward_ms"] * 1e-3 / nsteps / ncell * 1e9}
-    print("[bench] adjoint: forward %.1f ms, backward %.1f ms" % (res["forward_ms"], res["backward_ms"]),
+    print("[bench] adjoint %s: forward %.1f ms, backward %.1f ms" % (name, res["forward_ms"], res["backward_ms"]),
           file=sys.stderr, flush=True)
     return out
 
@@ -427,7 +441,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--boxes-only", action="store_true", help="time only the 64 Poisson boxes (for profiling)")
-    ap.add_argument("--adjoint-only", action="store_true", help="time only the 256^3 adjoint (for profiling)")
+    ap.add_argument("--adjoint-only", nargs="?", const="both", choices=("both", "tgv", "channel"),
+                    help="time only the adjoint workloads, both or one of them (for profiling)")
     ap.add_argument("--cavity-only", action="store_true", help="time only the 256^3 heated cube (for profiling)")
     ap.add_argument("--solve-only", action="store_true",
                     help="time only the channel's Poisson solves, FP32 storage then FP64 (for profiling: after each "
@@ -442,7 +457,9 @@ def main():
         sys.exit(2)
 
     only = [f for flag, f in ((args.cavity_only, secondary_cavity), (args.boxes_only, secondary_boxes),
-                              (args.adjoint_only, secondary_adjoint)) if flag]
+                              (args.adjoint_only in ("both", "tgv"), secondary_adjoint),
+                              (args.adjoint_only in ("both", "channel"), lambda a: secondary_adjoint(a, "channel")))
+            if flag]
     if only:
         print(json.dumps({"secondary": [f(args) for f in only]}))
         return
@@ -528,7 +545,7 @@ def main():
     out["copy_GBps"] = copy_bandwidth()
     if not args.no_secondary:
         out["secondary"] = [secondary_cavity(args), secondary_cavity(args, "mixed"), secondary_boxes(args),
-                            secondary_adjoint(args)]
+                            secondary_adjoint(args), secondary_adjoint(args, "channel")]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, init)
     print(json.dumps(out))
