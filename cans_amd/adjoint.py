@@ -1,13 +1,14 @@
-"""Gradients through RK3 steps on the GPU (FP64; boxes periodic in x and y, with a
-periodic z on a uniform grid or walls in z on any grid; bulk forcing of u and v).
+"""Gradients through RK3 steps on the GPU (FP64; each direction periodic or bounded by
+walls, a periodic z on a uniform grid, walls in z on any grid; bulk forcing of u and v
+along periodic directions).
 
 ``kinetic_energy_after_steps(sim, u0, v0, w0, dt, nsteps, every)`` runs nsteps fused
 RK3 steps (cans_rk3_substep) from the interior velocity (u0, v0, w0) with zero
 pressure and returns the volume-averaged kinetic energy of the final velocity as a
 torch scalar whose backward pass runs the adjoint substeps
 (cans_rk3_substep_adjoint).  The average weighs u and v by their cells' dzf(k) and
-w by dzc(k); w's wall face k = n3, which the boundary conditions fix, is left out,
-and its gradient is zero.  The forward pass keeps the velocity and pressure every
+w by dzc(k); the normal velocities' wall faces (u at i = n1, v at j = n2, w at k = n3),
+which the boundary conditions fix, are left out, and their gradient is zero.  The forward pass keeps the velocity and pressure every
 `every` steps; the backward pass restarts from each checkpoint in turn (last first),
 recomputes its steps while keeping the velocity at the start of every substep, and
 runs their adjoints in reverse.  Recomputed steps repeat the forward ones bit for bit
@@ -32,9 +33,18 @@ def _weights(sim, like):
     vol = float(g.dzf[1:n3 + 1].sum()) * n[0] * n[1]
     wf = torch.tensor(g.dzf[1:n3 + 1] / vol, dtype=like.dtype, device=like.device).view(n3, 1, 1)
     wc = torch.tensor(g.dzc[1:n3 + 1] / vol, dtype=like.dtype, device=like.device).view(n3, 1, 1)
-    if sim.case.cbcpre[4:6] != "PP":
-        wc[-1] = 0.
     return wf, wf, wc
+
+
+def _clear_wall_faces(sim, fields):
+    """Zero the normal velocities' wall faces of (u, v, w) interiors in (k, j, i) order."""
+    cb = sim.case.cbcpre
+    if cb[0:2] != "PP":
+        fields[0][:, :, -1] = 0.
+    if cb[2:4] != "PP":
+        fields[1][:, -1, :] = 0.
+    if cb[4:6] != "PP":
+        fields[2][-1, :, :] = 0.
 
 
 def _addr(f, buf):
@@ -59,6 +69,7 @@ class _KineticEnergyAfterSteps(torch.autograd.Function):
                 ckpt[s] = [f.buf.clone() for f in (sim.u, sim.v, sim.w, sim.p)]
             sim.rk3_step(dt)
         fin = [_interior(f).clone() for f in (sim.u, sim.v, sim.w)]
+        _clear_wall_faces(sim, fin)
         wts = _weights(sim, fin[0])
         E = 0.5 * sum((m * x * x).sum() for m, x in zip(wts, fin))
         ctx.sim, ctx.dt, ctx.nsteps, ctx.every, ctx.ckpt, ctx.fin = sim, dt, nsteps, every, ckpt, fin
@@ -71,7 +82,7 @@ class _KineticEnergyAfterSteps(torch.autograd.Function):
         # adjoint velocity and right-hand side, each with a second set the substeps write into
         ub, ubo, fb, fbo = ([sim.ctx.field() for _ in range(3)] for _ in range(4))
         pp = sim.ctx.field(kind=LB.CANS_FIELD_PRESSURE)
-        for f, m, x in zip(ub, wts, ctx.fin):  # dE/du = m u
+        for f, m, x in zip(ub, wts, ctx.fin):  # dE/du = m u (zero on the wall faces: fin is)
             _interior(f).copy_(m * x * gE)
         fields = (sim.u, sim.v, sim.w, sim.p)
         for s0 in sorted(ctx.ckpt, reverse=True):
@@ -91,6 +102,7 @@ class _KineticEnergyAfterSteps(torch.autograd.Function):
                     ub, ubo, fb, fbo = ubo, ub, fbo, fb
             del steps
         grads = [_interior(f).clone() for f in ub]
+        _clear_wall_faces(sim, grads)
         return grads[0], grads[1], grads[2], None, None, None, None
 
 

@@ -1,5 +1,5 @@
-// Adjoint kernels of the RK3 substep on boxes periodic in x and y, either periodic
-// in z on a uniform grid or bounded by walls in z on any grid.
+// Adjoint kernels of the RK3 substep on boxes whose directions are each periodic or
+// bounded by walls (a periodic z on a uniform grid, walls in z on any grid).
 //
 // The momentum right-hand side of mom_xyz_ad (src/mom.f90:771-1019) is, for
 // component c, with K^c_d(x) = alpha^c_d(k) q_d(x) + beta^c_d(k) q_d(x + e_c) (1/4 and
@@ -14,11 +14,13 @@
 //                  + nu L_c^T g_c(y).
 // Every value it reads lies within one cell of y (diagonal neighbours included).  On
 // a uniform periodic grid the coefficients are constants and halo'd fields with
-// periodic ghost cells serve as they are.  With walls in z the sum also runs over the
-// ghost planes k = 0 and n3 + 1 (g is zero wherever F is not a degree of freedom:
-// those planes and w's wall face k = n3), and k_jt_fold folds the ghost planes into
-// the first and last interior planes by the derivative of bound.f90's ghost value
-// (u(0) = 2 ub - u(1): -1; a Neumann ghost: +1; w's wall faces are constants).
+// periodic ghost cells serve as they are.  Along a wall direction the sum also runs
+// over the ghost layers (g is zero wherever F is not a degree of freedom: those
+// layers and a normal velocity's wall face), and k_jt_fold folds each ghost layer
+// into the adjacent interior one by the derivative of bound.f90's ghost value
+// (u(0) = 2 ub - u(1): -1; a Neumann ghost: +1; the normal velocity's wall faces
+// are constants).  bound.f90 sets the ghosts x, then y, then z, each over the
+// others' ghost layers, so the folds run z, then y, then x.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -48,28 +50,39 @@ struct JtArgs {
 constexpr int JT_X = 32, JT_Y = 8, JT_K = 16;
 constexpr int JT_PX = JT_X + 2, JT_PLANE = (JT_X + 2) * (JT_Y + 2);
 
-// WALL: walls in z.  The output planes are k = 0 .. n3 + 1 (a work-group's first plane is
-// blockIdx.z * JT_K), the staged planes -1 .. n3 + 2, of which -1 and n3 + 2 read as zero.
-template <bool COMB, bool WALL>
+// WM: the directions bounded by walls (bit 0 x, bit 1 y, bit 2 z; 0: tri-periodic uniform).
+// Along a wall direction the outputs include the ghost layers (i = 0 .. n1 + 1; a work-group's
+// first output is blockIdx.x * JT_X) and the staged ring reaches -1 and n + 2, which read as
+// zero.  g is read as zero off the degrees of freedom: the ghost layers of a wall direction and
+// a normal velocity's wall face (u at i = n1, v at j = n2, w at k = n3).
+template <bool COMB, int WM>
 __global__ void __launch_bounds__(JT_X * JT_Y) k_mom_jt(JtArgs a) {
+  constexpr bool WALL = WM != 0;
+  constexpr int OX = WM & 1, OY = (WM >> 1) & 1, OZ = (WM >> 2) & 1;
   __shared__ double sq[3][3][JT_PLANE];  // [component][slot][cell of the padded plane]
   __shared__ double sg[3][3][JT_PLANE];
   const Layout& L = a.L;
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * JT_X + tx;
-  const int i0 = blockIdx.x * JT_X, j0 = blockIdx.y * JT_Y;  // padded cell (0, 0) is (i0, j0)
-  const int kb = WALL ? (int)blockIdx.z * JT_K - 1 : (int)blockIdx.z * JT_K;
-  const int ke = WALL ? min(kb + JT_K, L.n3 + 1) : min(kb + JT_K, L.n3);
-  // plane kk exists in the halo'd fields; g_c is a degree of freedom on it
-  auto qplane = [&](int kk) { return !WALL || (kk >= 0 && kk <= L.n3 + 1); };
-  auto gplane = [&](int c, int kk) { return !WALL || (kk >= 1 && kk <= L.n3 - (c == 2)); };
+  const int i0 = blockIdx.x * JT_X - OX, j0 = blockIdx.y * JT_Y - OY;  // padded cell (0, 0) is (i0, j0)
+  const int kb = (int)blockIdx.z * JT_K - OZ;
+  const int ke = min(kb + JT_K, L.n3 + OZ);
+  // (i, j, kk) lies in the halo'd fields; g_c is a degree of freedom there
+  auto qin = [&](int i, int j, int kk) {
+    if (!WALL) return i <= L.n1 + 1 && j <= L.n2 + 1;
+    return i >= 0 && i <= L.n1 + 1 && j >= 0 && j <= L.n2 + 1 && kk >= 0 && kk <= L.n3 + 1;
+  };
+  auto gin = [&](int c, int i, int j, int kk) {
+    return (!OX || (i >= 1 && i <= L.n1 - (c == 0))) && (!OY || (j >= 1 && j <= L.n2 - (c == 1))) &&
+           (!OZ || (kk >= 1 && kk <= L.n3 - (c == 2)));
+  };
   auto stage = [&](int kk, int slot) {
     for (int e = tid; e < JT_PLANE; e += JT_X * JT_Y) {
       const int i = i0 + e % JT_PX, j = j0 + e / JT_PX;
-      const bool in = i <= L.n1 + 1 && j <= L.n2 + 1 && qplane(kk);
+      const bool in = qin(i, j, kk);
       const long o = in ? L.at(i, j, kk) : 0;
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        const bool ing = in && gplane(c, kk);
+        const bool ing = in && gin(c, i, j, kk);
         sq[c][slot][e] = in ? a.q[c][o] : 0.;
         if constexpr (COMB) sg[c][slot][e] = ing ? a.f1 * a.g[c][o] + a.fb[c][o] : 0.;
         else sg[c][slot][e] = ing ? a.g[c][o] : 0.;
@@ -85,11 +98,11 @@ __global__ void __launch_bounds__(JT_X * JT_Y) k_mom_jt(JtArgs a) {
     for (int r = 0; r < NE; r++) {
       const int e = tid + r * JT_X * JT_Y;
       const int i = i0 + e % JT_PX, j = j0 + e / JT_PX;
-      const bool in = e < JT_PLANE && i <= L.n1 + 1 && j <= L.n2 + 1 && qplane(kk);
+      const bool in = e < JT_PLANE && qin(i, j, kk);
       const long o = in ? L.at(i, j, kk) : 0;
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        const bool ing = in && gplane(c, kk);
+        const bool ing = in && gin(c, i, j, kk);
         rq[r][c] = in ? a.q[c][o] : 0.;
         if constexpr (COMB) rg[r][c] = ing ? a.f1 * a.g[c][o] + a.fb[c][o] : 0.;
         else rg[r][c] = ing ? a.g[c][o] : 0.;
@@ -114,7 +127,8 @@ __global__ void __launch_bounds__(JT_X * JT_Y) k_mom_jt(JtArgs a) {
   stage(kb + 2, 2);  // periodic z: kb < n3, so kb + 2 <= n3 + 1; walls: planes past n3 + 1 read as zero
   __syncthreads();
   const int i = i0 + tx + 1, j = j0 + ty + 1;
-  const bool own = i <= L.n1 && j <= L.n2;
+  const bool own = i <= L.n1 + OX && j <= L.n2 + OY;
+  const bool inxy = i >= 1 && i <= L.n1 && j >= 1 && j <= L.n2;
   const int ec[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
   for (int k = kb + 1; k <= ke; k++) {
     const bool next = k + 2 <= ke + 1;
@@ -125,7 +139,7 @@ __global__ void __launch_bounds__(JT_X * JT_Y) k_mom_jt(JtArgs a) {
       auto Q = [&](int c, const int* d) { return sq[c][sl[1 + d[2]]][at(d[0], d[1])]; };
       auto G = [&](int c, const int* d) { return sg[c][sl[1 + d[2]]][at(d[0], d[1])]; };
       const long o = L.at(i, j, k);
-      if constexpr (!WALL) {
+      if constexpr (WM == 0) {
 #pragma unroll
         for (int m = 0; m < 3; m++) {
           const int* em = ec[m];
@@ -167,9 +181,10 @@ __global__ void __launch_bounds__(JT_X * JT_Y) k_mom_jt(JtArgs a) {
         auto zf = [&](int kk) { return kk >= 0 && kk <= n3 + 1 ? a.dzfi[kk] : 0.; };
         const double zcm = zc(k - 1), zc0 = zc(k), zcp = zc(k + 1);
         const double zfm = zf(k - 1), zf0 = zf(k), zfp = zf(k + 1);
-        const bool in0 = k >= 1 && k <= n3, inm = k - 1 >= 1 && k - 1 <= n3;
+        const bool in0 = k >= 1 && k <= n3, inm = !OZ || (k - 1 >= 1 && k - 1 <= n3);
         // rdzm(k) = dzci(k)/dzfi(k), rdzp(k) = dzci(k)/dzfi(k+1) (mom.f90's weights of u, v at w's
-        // height); zero off the interior, where every term that would use them has g = 0
+        // height; a periodic z reads them off the ghost entries); zero off the interior of a walled
+        // z, where every term that would use them has g = 0
         const double rdzm0 = in0 ? zc0 / zf0 : 0., rdzp0 = in0 ? zc0 / zfp : 0., rdzpm = inm ? zcm / zf0 : 0.;
         // H^c_z on plane k + dk
         auto hz = [&](int c, int dk) { return c == 2 ? (dk < 0 ? zcm : dk > 0 ? zcp : zc0)
@@ -216,7 +231,7 @@ __global__ void __launch_bounds__(JT_X * JT_Y) k_mom_jt(JtArgs a) {
             const bool wt = c == 2 && m < 2;
             acc -= (wt ? 0.25 * rdzm0 : 0.25) * m0 + (wt ? 0.25 * rdzpm : 0.25) * m1;
           }
-          const double b = in0 ? a.base[m][o] : 0.;  // the ghost planes carry J^T g only
+          const double b = in0 && inxy ? a.base[m][o] : 0.;  // the ghost layers carry J^T g only
           a.out[m][o] = b + acc;
           if constexpr (COMB) a.fbo[m][o] = a.f2 * b;
         }
@@ -230,33 +245,56 @@ __global__ void __launch_bounds__(JT_X * JT_Y) k_mom_jt(JtArgs a) {
   }
 }
 
-static dim3 jt_grid(const Layout& L, bool wall) {
-  return dim3(cdiv(L.n1, JT_X), cdiv(L.n2, JT_Y), cdiv(wall ? L.n3 + 2 : L.n3, JT_K));
+static dim3 jt_grid(const Layout& L, int wm) {
+  return dim3(cdiv(L.n1 + 2 * (wm & 1), JT_X), cdiv(L.n2 + 2 * (wm >> 1 & 1), JT_Y), cdiv(L.n3 + 2 * (wm >> 2 & 1), JT_K));
 }
 
-// Folds the ghost planes that k_mom_jt<.., true> wrote into the interior: u(1) += s0 u(0),
-// u(n3) += s1 u(n3 + 1) (likewise v; s = -1 for a Dirichlet wall, +1 for Neumann), and w's
-// wall face k = n3 is cleared.  One thread per (i, j).
-__global__ void k_jt_fold(Layout L, double* u, double* v, double* w, double su0, double su1, double sv0,
-                          double sv1) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1;
-  if (i > L.n1) return;
-  const long b = L.at(i, j, 0), t = L.at(i, j, L.n3), dz = L.sz;
-  u[b + dz] += su0 * u[b];
-  u[t] += su1 * u[t + dz];
-  v[b + dz] += sv0 * v[b];
-  v[t] += sv1 * v[t + dz];
-  w[t] = 0.;
+// Folds the ghost layers of a wall direction d that k_mom_jt<.., WM> wrote into the interior:
+// for each tangential component m, q_m(1) += s0 q_m(0) and q_m(n) += s1 q_m(n + 1) along d (s = -1 for
+// a Dirichlet wall, +1 for Neumann), the ghost layers then cleared; the normal component's wall
+// face and ghost layers are cleared (constants).  One thread per point of the plane of the other
+// two directions, over lo..hi of each (their ghost layers included while they are still to fold).
+struct JtFold {
+  int d;
+  int lo[3], hi[3];
+  double s[3][2];
+};
+
+__global__ void k_jt_fold(Layout L, double* u, double* v, double* w, JtFold f) {
+  const int a = f.d == 0 ? 1 : 0, b = f.d == 2 ? 1 : 2;  // the plane's two directions
+  int idx[3];
+  idx[a] = f.lo[a] + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  idx[b] = f.lo[b] + (int)blockIdx.y;
+  if (idx[a] > f.hi[a]) return;
+  const int nd = f.d == 0 ? L.n1 : f.d == 1 ? L.n2 : L.n3;
+  auto at = [&](int t) {
+    int x[3] = {idx[0], idx[1], idx[2]};
+    x[f.d] = t;
+    return L.at(x[0], x[1], x[2]);
+  };
+  double* q[3] = {u, v, w};
+  for (int m = 0; m < 3; m++) {
+    if (m == f.d) {
+      q[m][at(0)] = 0.;
+      q[m][at(nd)] = 0.;
+      q[m][at(nd + 1)] = 0.;
+    } else {
+      q[m][at(1)] += f.s[m][0] * q[m][at(0)];
+      q[m][at(nd)] += f.s[m][1] * q[m][at(nd + 1)];
+      q[m][at(0)] = 0.;
+      q[m][at(nd + 1)] = 0.;
+    }
+  }
 }
 
 // The volume weights of the projection's transpose P^T = M P M^-1 (M: dzf(k) for u, v; dzc(k)
 // for w) and the transposed bulk forcing, on the interior:
 //   inv:  u *= dzfi(k), v *= dzfi(k), w *= dzci(k);
 //   else: u = (u - mu) / dzfi(k), v = (v - mv) / dzfi(k), w = w / dzci(k)
-// (dzfi, dzci null: no weights, only the means subtracted).  wall: w's wall face k = n3 is
-// cleared.
+// (dzfi, dzci null: no weights, only the means subtracted).  The normal velocities' wall faces
+// of the directions in wm (u at i = n1, v at j = n2, w at k = n3) are cleared.
 __global__ void k_adj_weigh(Layout L, double* u, double* v, double* w, const double* dzfi, const double* dzci,
-                            int inv, double mu, double mv, int wall) {
+                            int inv, double mu, double mv, int wm) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
   if (i > L.n1) return;
   const long o = L.at(i, j, k);
@@ -274,7 +312,9 @@ __global__ void k_adj_weigh(Layout L, double* u, double* v, double* w, const dou
     u[o] -= mu;
     v[o] -= mv;
   }
-  if (wall && k == L.n3) w[o] = 0.;
+  if ((wm & 1) && i == L.n1) u[o] = 0.;
+  if ((wm & 2) && j == L.n2) v[o] = 0.;
+  if ((wm & 4) && k == L.n3) w[o] = 0.;
 }
 
 static JtArgs jt_args(cans_ctx* c, const double* const* q, const double* const* g, const double* const* fb,
@@ -302,34 +342,61 @@ static JtArgs jt_args(cans_ctx* c, const double* const* q, const double* const* 
 int launch_mom_jt(cans_ctx* c, const double* const* q, const double* const* g, const double* const* base,
                   double* const* out, const double* h, double visc) {
   const JtArgs a = jt_args(c, q, g, nullptr, base, out, nullptr, h, visc, 0., 0., nullptr, nullptr);
-  k_mom_jt<false, false><<<jt_grid(c->L, false), dim3(JT_X, JT_Y), 0, c->stream>>>(a);
+  k_mom_jt<false, 0><<<jt_grid(c->L, 0), dim3(JT_X, JT_Y), 0, c->stream>>>(a);
   CANS_LAUNCH_CHECK("k_mom_jt");
   return CANS_OK;
+}
+
+template <int WM>
+static void jt_adj_launch(cans_ctx* c, const JtArgs& a) {
+  k_mom_jt<true, WM><<<jt_grid(c->L, WM), dim3(JT_X, JT_Y), 0, c->stream>>>(a);
 }
 
 int launch_mom_jt_adj(cans_ctx* c, const double* const* q, const double* const* wb, const double* const* fb,
                       double* const* out, double* const* fbo, const double* h, double visc, double f1, double f2,
                       const AdjWalls* wall) {
-  const JtArgs a = jt_args(c, q, wb, fb, wb, out, fbo, h, visc, f1, f2, wall ? wall->dzci : nullptr,
-                           wall ? wall->dzfi : nullptr);
-  if (!wall) {
-    k_mom_jt<true, false><<<jt_grid(c->L, false), dim3(JT_X, JT_Y), 0, c->stream>>>(a);
-    CANS_LAUNCH_CHECK("k_mom_jt");
-    return CANS_OK;
+  const int wm = wall ? wall->mask & 7 : 0;
+  const JtArgs a = jt_args(c, q, wb, fb, wb, out, fbo, h, visc, f1, f2, wm ? wall->dzci : nullptr,
+                           wm ? wall->dzfi : nullptr);
+  switch (wm) {
+    case 0: jt_adj_launch<0>(c, a); break;
+    case 1: jt_adj_launch<1>(c, a); break;
+    case 2: jt_adj_launch<2>(c, a); break;
+    case 3: jt_adj_launch<3>(c, a); break;
+    case 4: jt_adj_launch<4>(c, a); break;
+    case 5: jt_adj_launch<5>(c, a); break;
+    case 6: jt_adj_launch<6>(c, a); break;
+    default: jt_adj_launch<7>(c, a); break;
   }
-  k_mom_jt<true, true><<<jt_grid(c->L, true), dim3(JT_X, JT_Y), 0, c->stream>>>(a);
   CANS_LAUNCH_CHECK("k_mom_jt");
-  k_jt_fold<<<dim3(cdiv(c->L.n1, 64), c->L.n2), 64, 0, c->stream>>>(c->L, out[0], out[1], out[2], wall->sign[0],
-                                                                      wall->sign[1], wall->sign[2], wall->sign[3]);
-  CANS_LAUNCH_CHECK("k_jt_fold");
+  // B^T in the reverse of bound's x, y, z order: z first; a direction still to fold keeps its
+  // ghost layers in the plane of the later folds
+  const Layout& L = c->L;
+  const int n[3] = {L.n1, L.n2, L.n3};
+  for (int d = 2; d >= 0; d--) {
+    if (!(wm >> d & 1)) continue;
+    JtFold f;
+    f.d = d;
+    for (int e = 0; e < 3; e++) {
+      const bool ghosts = e < d && (wm >> e & 1);
+      f.lo[e] = ghosts ? 0 : 1;
+      f.hi[e] = ghosts ? n[e] + 1 : n[e];
+      f.s[e][0] = wall->sign[e][d][0];
+      f.s[e][1] = wall->sign[e][d][1];
+    }
+    const int pa = d == 0 ? 1 : 0, pb = d == 2 ? 1 : 2;
+    const int na = f.hi[pa] - f.lo[pa] + 1, nb = f.hi[pb] - f.lo[pb] + 1;
+    k_jt_fold<<<dim3(cdiv(na, 64), nb), 64, 0, c->stream>>>(L, out[0], out[1], out[2], f);
+    CANS_LAUNCH_CHECK("k_jt_fold");
+  }
   return CANS_OK;
 }
 
 int launch_adj_weigh(cans_ctx* c, double* const* f, const double* dzfi, const double* dzci, bool inv, double mu,
-                     double mv, bool wall) {
+                     double mv, int wm) {
   const Layout& L = c->L;
   k_adj_weigh<<<dim3(cdiv(L.n1, 64), L.n2, L.n3), 64, 0, c->stream>>>(L, f[0], f[1], f[2], dzfi, dzci, inv, mu, mv,
-                                                                      wall);
+                                                                      wm);
   CANS_LAUNCH_CHECK("k_adj_weigh");
   return CANS_OK;
 }

@@ -869,37 +869,52 @@ extern "C" int cans_rk3_step(cans_ctx* c, const cans_step_params* s, double dt, 
 // ---------------------------------------------------------------------------
 namespace {
 
+// the directions bounded by walls, bit 0 x, 1 y, 2 z
+int adjoint_walls(const cans_step_params* s) {
+  int wm = 0;
+  for (int d = 0; d < 3; d++)
+    if (s->cbcpre[2 * d] != 'P' || s->cbcpre[2 * d + 1] != 'P') wm |= 1 << d;
+  return wm;
+}
+
 int adjoint_scope(cans_ctx* c, const cans_step_params* s, const char* who) {
   const std::string w(who);
   if (c->precision != CANS_FP64) return fail(CANS_EUNSUPPORTED, w + ": the adjoint needs a CANS_FP64 context");
   if (c->imp) return fail(CANS_EUNSUPPORTED, w + ": the adjoint covers explicit diffusion");
   if (!s) return CANS_OK;
-  for (int q = 0; q < 4; q++)
-    if (s->cbcpre[q] != 'P') return fail(CANS_EUNSUPPORTED, w + ": the adjoint covers boxes periodic in x and y");
-  for (int m = 0; m < 3; m++)
-    for (int q = 0; q < 4; q++)
-      if (s->cbcvel[6 * m + q] != 'P')
-        return fail(CANS_EUNSUPPORTED, w + ": the adjoint covers boxes periodic in x and y");
-  if (s->is_forced[2]) return fail(CANS_EUNSUPPORTED, w + ": the adjoint covers forcing of u and v");
   if (s->nscal) return fail(CANS_EUNSUPPORTED, w + ": the adjoint covers flows without scalars");
-  if (s->cbcpre[4] == 'P' && s->cbcpre[5] == 'P') {
+  const int n[3] = {c->L.n1, c->L.n2, c->L.n3};
+  const int wm = adjoint_walls(s);
+  for (int d = 0; d < 3; d++) {
+    const char* cp = s->cbcpre + 2 * d;
+    if (!(wm >> d & 1)) {
+      for (int m = 0; m < 3; m++)
+        if (s->cbcvel[6 * m + 2 * d] != 'P' || s->cbcvel[6 * m + 2 * d + 1] != 'P')
+          return fail(CANS_EUNSUPPORTED, w + ": velocity and pressure disagree on a periodic direction");
+      continue;
+    }
+    // walls: Neumann pressure, Dirichlet normal velocity (its wall faces constants), Dirichlet or
+    // Neumann tangential velocity
+    if (cp[0] != 'N' || cp[1] != 'N' || s->cbcvel[6 * d + 2 * d] != 'D' || s->cbcvel[6 * d + 2 * d + 1] != 'D')
+      return fail(CANS_EUNSUPPORTED, w + ": walls need Neumann pressure and a Dirichlet normal velocity");
     for (int m = 0; m < 3; m++)
-      if (s->cbcvel[6 * m + 4] != 'P' || s->cbcvel[6 * m + 5] != 'P')
-        return fail(CANS_EUNSUPPORTED, w + ": velocity and pressure disagree on a periodic z");
+      for (int e = 0; e < 2; e++) {
+        const char b = s->cbcvel[6 * m + 2 * d + e];
+        if (m != d && b != 'D' && b != 'N')
+          return fail(CANS_EUNSUPPORTED, w + ": walls need a Dirichlet or Neumann tangential velocity");
+      }
+    if (n[d] < 2) return fail(CANS_EUNSUPPORTED, w + ": walls need at least two cells across");
+  }
+  if (s->is_forced[2]) return fail(CANS_EUNSUPPORTED, w + ": the adjoint covers forcing of u and v");
+  for (int m = 0; m < 2; m++)
+    if (s->is_forced[m] && (wm >> m & 1))
+      return fail(CANS_EUNSUPPORTED, w + ": forcing needs its direction periodic");
+  if (!(wm & 4)) {
     const double dz = s->dzc_h[1];
     for (int k = 0; k <= c->L.n3 + 1; k++)
       if (std::fabs(s->dzc_h[k] - dz) > 1e-12 * dz || std::fabs(s->dzf_h[k] - dz) > 1e-12 * dz)
         return fail(CANS_EUNSUPPORTED, w + ": with a periodic z the adjoint covers uniform grids");
-    return CANS_OK;
   }
-  // walls in z: Neumann pressure, Dirichlet w (its wall faces constants), Dirichlet or Neumann u and v
-  if (s->cbcpre[4] != 'N' || s->cbcpre[5] != 'N' || s->cbcvel[16] != 'D' || s->cbcvel[17] != 'D')
-    return fail(CANS_EUNSUPPORTED, w + ": z walls need Neumann pressure and Dirichlet w");
-  for (int m = 0; m < 2; m++)
-    for (int q = 4; q < 6; q++)
-      if (s->cbcvel[6 * m + q] != 'D' && s->cbcvel[6 * m + q] != 'N')
-        return fail(CANS_EUNSUPPORTED, w + ": z walls need Dirichlet or Neumann u and v");
-  if (c->L.n3 < 2) return fail(CANS_EUNSUPPORTED, w + ": z walls need n3 >= 2");
   if (!s->grid_vol_ratio_f_d && (s->is_forced[0] || s->is_forced[1]))
     return fail(CANS_EINVAL, w + ": null grid_vol_ratio_f");
   return CANS_OK;
@@ -950,19 +965,21 @@ extern "C" int cans_rk3_substep_adjoint(cans_ctx* c, const cans_step_params* s, 
   double* a[3] = {(double*)ub, (double*)vb, (double*)wb};
   double* fb[3] = {(double*)fu, (double*)fv, (double*)fw};
   double* P = (double*)pp;
-  const bool wall = s->cbcpre[4] != 'P';
+  const int wm = adjoint_walls(s);
+  const bool zwall = wm & 4;
   // the linear part of bound.f90's ghost map (boundary values zero): periodic images, and at walls
-  // u(0) = -u(1) (Dirichlet) or u(0) = u(1) (Neumann), w = 0 on the wall faces
+  // u(0) = -u(1) (Dirichlet) or u(0) = u(1) (Neumann) tangentially, 0 on the normal wall faces
   const double zero[18] = {};
   auto ghosts = [&](double* const* f) {
     return plan_bounduvw(c, s->cbcvel, zero, s->nb, s->is_bound, s->dl, s->dzc_h, s->dzf_h, f[0], f[1], f[2], -1,
                          CANS_HALO_ABSENT, CANS_HALO_ABSENT);
   };
   // wb = Fr'^T P^T ub in place, P^T = M P M^-1 (M the cells' dz: dzf for u and v, dzc for w; a
-  // multiple of I on a uniform periodic grid) and Fr'^T b = b - gvr_f sum(b) the transposed bulk
-  // forcing, applied as dzf (P a - mean_f(P a)).  P: fillps with dti = 1, solver, boundp, correc
-  // with dt = 1 (correc reads each cell's input before it writes that cell)
-  if (wall && (rc = launch_adj_weigh(c, a, s->dzfi_d, s->dzci_d, true, 0., 0., true))) return rc;
+  // multiple of I with a uniform z) and Fr'^T b = b - gvr_f sum(b) the transposed bulk forcing,
+  // applied as dzf (P a - mean_f(P a)).  P: fillps with dti = 1, solver, boundp, correc with dt = 1
+  // (correc reads each cell's input before it writes that cell)
+  if (wm && (rc = launch_adj_weigh(c, a, zwall ? s->dzfi_d : nullptr, zwall ? s->dzci_d : nullptr, true, 0., 0., wm)))
+    return rc;
   if ((rc = ghosts(a))) return rc;
   if ((rc = launch_fillps<double>(c, a[0], a[1], a[2], nullptr, nullptr, nullptr, nullptr, P, s->dzfi_d, 1., s->dli)))
     return rc;
@@ -975,9 +992,9 @@ extern "C" int cans_rk3_substep_adjoint(cans_ctx* c, const cans_step_params* s, 
   double mean[2] = {0., 0.};
   for (int m = 0; m < 2; m++)
     if (s->is_forced[m] && (rc = run_bulk_mean<double>(c, a[m], s->grid_vol_ratio_f_d, &mean[m]))) return rc;
-  if (wall || s->is_forced[0] || s->is_forced[1]) {
-    if ((rc = launch_adj_weigh(c, a, wall ? s->dzfi_d : nullptr, wall ? s->dzci_d : nullptr, false, mean[0], mean[1],
-                               wall)))
+  if (wm || s->is_forced[0] || s->is_forced[1]) {
+    if ((rc = launch_adj_weigh(c, a, zwall ? s->dzfi_d : nullptr, zwall ? s->dzci_d : nullptr, false, mean[0],
+                               mean[1], wm)))
       return rc;
   }
   // (ubo, fbo) = (wb + J(u)^T (f1 wb + fb), f2 wb): one pass over wb and fb with their ghosts
@@ -986,12 +1003,14 @@ extern "C" int cans_rk3_substep_adjoint(cans_ctx* c, const cans_step_params* s, 
   const double* q[3] = {(const double*)u, (const double*)v, (const double*)w};
   double* o[3] = {(double*)ubo, (double*)vbo, (double*)wbo};
   double* fo[3] = {(double*)fuo, (double*)fvo, (double*)fwo};
-  if (!wall) return launch_mom_jt_adj(c, q, a, fb, o, fo, h, s->visc, f1, f2, nullptr);
+  if (!wm) return launch_mom_jt_adj(c, q, a, fb, o, fo, h, s->visc, f1, f2, nullptr);
   AdjWalls aw;
+  aw.mask = wm;
   aw.dzci = s->dzci_d;
   aw.dzfi = s->dzfi_d;
-  for (int m = 0; m < 2; m++)
-    for (int q2 = 0; q2 < 2; q2++) aw.sign[2 * m + q2] = s->cbcvel[6 * m + 4 + q2] == 'D' ? -1. : 1.;
+  for (int m = 0; m < 3; m++)
+    for (int d = 0; d < 3; d++)
+      for (int e = 0; e < 2; e++) aw.sign[m][d][e] = s->cbcvel[6 * m + 2 * d + e] == 'N' ? 1. : -1.;
   return launch_mom_jt_adj(c, q, a, fb, o, fo, h, s->visc, f1, f2, &aw);
 }
 

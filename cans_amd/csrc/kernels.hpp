@@ -206,27 +206,29 @@ int run_chkdt(cans_ctx* c, const T* u, const T* v, const T* w, const double* dzc
 template <typename T>
 int run_bulk_mean(cans_ctx* c, const T* q, const double* gvr, double* mean);
 
-// adjoint.hip (FP64; x, y periodic; z periodic on a uniform grid or walls on any grid)
+// adjoint.hip (FP64; each direction periodic or walled; z periodic on a uniform grid or walled on any grid)
 // out = base + J(q)^T g, J the Jacobian of mom_xyz_ad's right-hand side at q; h = (1/dx, 1/dy, 1/dz)
 // (tri-periodic uniform grids)
 int launch_mom_jt(cans_ctx* c, const double* const* q, const double* const* g, const double* const* base,
                   double* const* out, const double* h, double visc);
-// walls in z: the device arrays dzci, dzfi (0:n3+1) and the derivative of the tangential
-// velocities' ghost values with respect to the adjacent interior value, {u bottom, u top,
-// v bottom, v top} (-1 Dirichlet, +1 Neumann)
+// walls: mask = the directions bounded by walls (bit 0 x, 1 y, 2 z), the device arrays dzci,
+// dzfi (0:n3+1), and sign[m][d][end] the derivative of the tangential velocity m's ghost value
+// at end 0/1 of wall direction d with respect to the adjacent interior value (-1 Dirichlet, +1
+// Neumann)
 struct AdjWalls {
+  int mask;
   const double *dzci, *dzfi;
-  double sign[4];
+  double sign[3][3][2];
 };
 // the adjoint substep's last part: out = wb + J(q)^T (f1 wb + fb), fbo = f2 wb (wb, fb halo'd with
-// periodic ghosts in x and y, and in z unless wall; out and fbo distinct from every input; with
-// walls, out's ghost planes are folded into the interior and w's wall face cleared)
+// periodic ghosts along the periodic directions; out and fbo distinct from every input; with walls,
+// out's ghost layers are folded into the interior and the normal velocities' wall faces cleared)
 int launch_mom_jt_adj(cans_ctx* c, const double* const* q, const double* const* wb, const double* const* fb,
                       double* const* out, double* const* fbo, const double* h, double visc, double f1, double f2,
                       const AdjWalls* wall);
 // the volume weights of P^T = M P M^-1 and the transposed bulk forcing (k_adj_weigh)
 int launch_adj_weigh(cans_ctx* c, double* const* f, const double* dzfi, const double* dzci, bool inv, double mu,
-                     double mv, bool wall);
+                     double mv, int wm);
 
 // solver.hip
 // right-hand side of the pressure equation computed from the velocity (plus its

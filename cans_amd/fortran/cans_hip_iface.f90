@@ -451,7 +451,7 @@ module cans_hip
       real(c_double), value :: visc
     end function
     ! one RK3 substep run backwards: adjoints (ub, fb) of its outputs -> (ubo, fbo) of its inputs
-    ! (FP64; x, y periodic; z periodic on a uniform grid or walls on any grid; forcing of u and v)
+    ! (FP64; each direction periodic or walled, a periodic z uniform; forcing of u and v)
     integer(c_int) function cans_rk3_substep_adjoint(ctx, prm, irk, dt, u, v, w, ub, vb, wb, fu, fv, fw, &
                                                      ubo, vbo, wbo, fuo, fvo, fwo, pp) &
                    bind(C, name='cans_rk3_substep_adjoint')

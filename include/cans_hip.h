@@ -393,16 +393,19 @@ int cans_ctx_forcing(cans_ctx *ctx, const double **f_d);
 /* ---------------------------------------------------------------------------
  * Adjoint of the RK3 substep (no counterpart in the reference; for gradients of a
  * loss on the velocity).  Scope: CANS_FP64 contexts, explicit diffusion, no scalars
- * or buoyancy, boxes periodic in x and y, and in z either periodic on a uniform grid
- * or bounded by walls on any grid (Neumann pressure, Dirichlet w, Dirichlet or
- * Neumann u and v, any wall values); bulk forcing of u and v.  There substep irk maps
- * the degrees of freedom (interior u, v; w without its wall face k = n3) as
+ * or buoyancy; each direction periodic or bounded by walls (Neumann pressure,
+ * Dirichlet normal velocity, Dirichlet or Neumann tangential velocity, any wall
+ * values), a periodic z on a uniform grid and walls in z on any grid; bulk forcing of
+ * u and v along periodic x and y.  There substep irk maps the degrees of freedom
+ * (every interior value except a normal velocity's wall face: u at i = n1, v at
+ * j = n2, w at k = n3) as
  * (u, Fo) -> (P Fr(u + f1 F(B u) + f2 Fo), F(B u)), F the right-hand side of
  * mom_xyz_ad (src/mom.f90:771-1019) at the velocity whose ghost cells bound.f90 sets
  * (B), f1, f2 = rkcoeff(1:2,irk)*dt, Fr the bulk forcing (rk.f90:493-518) and P the
  * projection of fillps/solver/correc; the pressure drops out (P G p = 0).  P is
  * self-adjoint in the inner product weighted by the cells' dz (dzf for u, v; dzc for
- * w), so its transpose is P itself conjugated by those weights.
+ * w), so its transpose is P itself conjugated by those weights, whatever transforms
+ * (FFT, DCT) the solve uses along x and y.
  * ------------------------------------------------------------------------- */
 /* out = base + J(q)^T g: J the Jacobian of mom_xyz_ad's right-hand side at the
  * velocity q = (u,v,w) on a tri-periodic uniform grid; dli = (1/dx, 1/dy, 1/dz).  q
@@ -416,8 +419,8 @@ int cans_mom_jt(cans_ctx *ctx, const int *n, const double *dli, double visc, con
  * velocity, (fu, fv, fw) that of the right-hand side this substep stored for the next
  * one (zero after substep 3).  (ubo, vbo, wbo) and (fuo, fvo, fwo) receive the adjoints of
  * the substep's input velocity and of the right-hand side it read; ub..wb are
- * overwritten (they receive Fr'^T P^T ub).  Values at w's wall face are ignored on
- * input and zero on output.  pp_d is a halo'd scratch field; all sixteen fields are
+ * overwritten (they receive Fr'^T P^T ub).  Values on the normal wall faces are
+ * ignored on input and zero on output.  pp_d is a halo'd scratch field; all sixteen fields are
  * distinct.  prm as for cans_rk3_substep (plan, lambdaxy, grid arrays, is_forced,
  * grid_vol_ratio_f). */
 int cans_rk3_substep_adjoint(cans_ctx *ctx, const cans_step_params *prm, int irk, double dt,

@@ -3,12 +3,13 @@
 Like the rest of ``oracle/``, this module is the checker for the HIP path: only
 ``tests/`` and ``bench.py``'s cpu_baseline leg may use it.
 
-Scope: boxes periodic in x and y, either periodic in z on a uniform grid or
-bounded by walls in z on any (stretched) grid (Neumann pressure, Dirichlet w,
-Dirichlet or Neumann u and v); explicit diffusion, bulk forcing of u and v, no
-scalars.  The velocity map of RK3 substep irk (src/main.f90:437-473 with
-rk.f90:24-333, fillps, solver, correc) on the degrees of freedom (every interior
-u, v; w without the wall face k = n3, which bound.f90 fixes) is
+Scope: each direction either periodic or bounded by walls (Neumann pressure,
+Dirichlet normal velocity, Dirichlet or Neumann tangential velocity); a periodic
+z on a uniform grid, walls in z on any (stretched) grid; explicit diffusion, bulk
+forcing of u and v along periodic x and y, no scalars.  The velocity map of RK3
+substep irk (src/main.f90:437-473 with rk.f90:24-333, fillps, solver, correc) on
+the degrees of freedom (every interior value except a normal velocity's wall face,
+u(n1), v(n2) or w(n3), which bound.f90 fixes) is
 
     u' = P Fr(u + f1 F(B u) + f2 Fo),      Fo' = F(B u),
 
@@ -24,8 +25,11 @@ so P G p = 0 and the pressure only follows the velocity.
 P is self-adjoint in the inner product weighted by the cell volumes M (dzf(k)
 for u and v, dzc(k) for w; D and G are negative adjoints there:
 sum_k dzf(k) p(k) (w(k) - w(k-1))/dzf(k) = -sum_k dzc(k) w(k) (p(k+1) - p(k))/dzc(k)
-with w = 0 on the walls), so its transpose is P^T = M P M^-1; on a uniform
-periodic grid M is a multiple of I and P^T = P.  The forcing's Jacobian
+with w = 0 on the walls, and likewise along x and y with uniform weights), so its
+transpose is P^T = M P M^-1; with a uniform z, M is a multiple of I and P^T = P.
+This holds whatever transforms the solver uses along x and y: with walls there the
+DCT pair REDFT10/REDFT01 (each the other's transpose up to scale) diagonalises the
+same symmetric L, so the forward solve is its own weighted transpose.  The forcing's Jacobian
 I - 1 gvr_f^T has the transpose I - gvr_f 1^T.  Given the adjoints ub of u' and
 fb of Fo', the adjoint of a substep is
 
@@ -50,7 +54,10 @@ With G^c_d(x) = H^c_d(k_x) g_c(x) and M^c_d(x) = G^c_d(x) q_c(x+e_d) - G^c_d(x+e
 
 evaluated at every cell and ghost cell y, g being zero wherever F is not a degree
 of freedom; B^T then folds the ghost values into the interior (bound.f90:182-463:
-u(0) = 2 ub - u(1) gives -1, a Neumann ghost +1; w's wall faces are constants).
+u(0) = 2 ub - u(1) gives -1, a Neumann ghost +1; a normal velocity's wall faces
+are constants).  bound.f90 sets the ghosts direction by direction, x, then y,
+then z, each over the others' ghost layers too, so B^T folds z first, then y,
+then x.
 
 The tests pin mom_jt by dot products against central differences of the
 oracle's own F (exact: F is quadratic) through the oracle's own ghost map, P^T by
@@ -71,25 +78,41 @@ def halo(case, a):
     return np.asfortranarray(np.pad(a, 1, mode="wrap"))
 
 
+def wall_dirs(case):
+    """The directions bounded by walls (the pressure is Neumann there)."""
+    return [d for d in range(3) if case.cbcpre[2 * d:2 * d + 2] != "PP"]
+
+
 def walls(case):
-    """True when z is bounded by walls (the pressure is then Neumann there)."""
-    return case.cbcpre[4:6] != "PP"
+    """True when some direction is bounded by walls."""
+    return bool(wall_dirs(case))
 
 
 def check_case(case):
-    per_xy = case.cbcpre[:4] == "PPPP" and all(case.cbcvel[6 * c + q] == "P" for c in range(3) for q in range(4))
-    if not per_xy:
-        raise ValueError("the adjoint covers boxes periodic in x and y")
-    if walls(case):
-        if (case.cbcpre[4:6] != "NN" or case.cbcvel[16:18] != "DD"
-                or any(case.cbcvel[6 * c + 4 + q] not in "DN" for c in (0, 1) for q in (0, 1))):
-            raise ValueError("z walls: Neumann pressure, Dirichlet w, Dirichlet or Neumann u and v")
-        if case.n[2] < 2:
-            raise ValueError("z walls need n3 >= 2")
-    elif np.ptp(case.dzci) > 1e-12 * np.abs(case.dzci).max():
+    for d in range(3):
+        cp = case.cbcpre[2 * d:2 * d + 2]
+        cv = [case.cbcvel[6 * c + 2 * d:6 * c + 2 * d + 2] for c in range(3)]
+        if cp == "PP":
+            if any(x != "PP" for x in cv):
+                raise ValueError("velocity and pressure disagree on a periodic direction")
+        elif cp != "NN" or cv[d] != "DD" or any(ch not in "DN" for c in range(3) if c != d for ch in cv[c]):
+            raise ValueError("walls: Neumann pressure, Dirichlet normal and Dirichlet or Neumann tangential velocity")
+        elif case.n[d] < 2:
+            raise ValueError("walls need at least two cells across")
+    if 2 not in wall_dirs(case) and np.ptp(case.dzci) > 1e-12 * np.abs(case.dzci).max():
         raise ValueError("a periodic z needs a uniform grid")
-    if case.is_forced[2] or case.scalars or case.is_impdiff:
-        raise ValueError("the adjoint covers explicit diffusion, forcing of u and v only and no scalars")
+    if case.is_forced[2] or any(case.is_forced[c] and c in wall_dirs(case) for c in (0, 1)):
+        raise ValueError("forcing covers u and v along periodic directions")
+    if case.scalars or case.is_impdiff:
+        raise ValueError("the adjoint covers explicit diffusion and no scalars")
+
+
+def _clear_wall_faces(case, a):
+    """Zero each normal velocity's wall face (u(n1), v(n2), w(n3) on an interior array)."""
+    for d in wall_dirs(case):
+        idx = [slice(None)] * 3
+        idx[d] = -1
+        a[d][tuple(idx)] = 0.
 
 
 def mom(case, u, v, w):
@@ -156,15 +179,17 @@ def dof_fields(case, a):
     f = [O.field(case.n) for _ in range(3)]
     for x, y in zip(f, a):
         x[1:-1, 1:-1, 1:-1] = y
-    if walls(case):
-        f[2][:, :, -2] = 0.  # w(n3) is a wall face
+    for d in wall_dirs(case):  # the normal velocity's wall face
+        idx = [slice(None)] * 3
+        idx[d] = -2
+        f[d][tuple(idx)] = 0.
     with _Homogeneous(case) as hc:
         hc.bounduvw(*f)
     return f
 
 
 def project_dofs(case, a):
-    """P a on the degrees of freedom (walls: homogeneous ghosts, w(n3) = 0)."""
+    """P a on the degrees of freedom (walls: homogeneous ghosts, the normal wall faces zero)."""
     u, v, w = dof_fields(case, a)
     pp = O.field(case.n)
     with _Homogeneous(case) as hc:
@@ -176,17 +201,17 @@ def project_dofs(case, a):
 
 
 def volume_weights(case):
-    """Per-k volume weights of u, v and w (broadcast over i, j): dzf, dzf, dzc; 0 on w's wall face."""
+    """Volume weights of u, v and w as interior arrays: dzf, dzf, dzc along z; 0 on the normal wall faces."""
     n3 = case.n[2]
-    wf = np.asarray(case.dzf[1:n3 + 1], dtype=np.float64).copy()
-    wc = np.asarray(case.dzc[1:n3 + 1], dtype=np.float64).copy()
-    if walls(case):
-        wc[-1] = 0.
-    return [x[None, None, :] for x in (wf, wf, wc)]
+    wf = np.asarray(case.dzf[1:n3 + 1], dtype=np.float64)[None, None, :]
+    wc = np.asarray(case.dzc[1:n3 + 1], dtype=np.float64)[None, None, :]
+    m = [np.broadcast_to(x, tuple(case.n)).copy() for x in (wf, wf, wc)]
+    _clear_wall_faces(case, m)
+    return m
 
 
 def project_t(case, a):
-    """P^T a = M P M^-1 a (P^T = P on a uniform periodic grid)."""
+    """P^T a = M P M^-1 a (P^T = P with a uniform z, up to the wall faces, which it zeroes)."""
     if not walls(case):
         return project(case, a)
     m = volume_weights(case)
@@ -207,19 +232,32 @@ def forcing_t(case, b):
 
 
 def mom_jt_walls(case, qh, g, fold=True, swap_rdz=False):
-    """(d F(B q)/dq)^T g on a box with walls in z: qh three halo'd fields (ghosts as the forward set
-    them), g three interior arrays (g_w at k = n3 is ignored).  Returns three interior arrays, w's wall
-    face zero.  fold=False and swap_rdz=True plant mistakes for the tests' negative controls."""
-    n1, n2, n3 = case.n
-    # planes kk = k + 1 for k = -1 .. n3 + 2; the planes -1 and n3 + 2 are zero
-    Q = [np.pad(x[1:-1, 1:-1, :], ((0, 0), (0, 0), (1, 1))) for x in qh]
-    G = [np.pad(x, ((0, 0), (0, 0), (2, 2))) for x in g]
-    G[2][:, :, n3 + 1] = 0.
-    zc, zf = np.zeros(n3 + 4), np.zeros(n3 + 4)
-    zc[1:n3 + 3], zf[1:n3 + 3] = case.dzci, case.dzfi
-    rdzm, rdzp = np.zeros(n3 + 4), np.zeros(n3 + 4)
-    rdzm[2:n3 + 2] = zc[2:n3 + 2] / zf[2:n3 + 2]
-    rdzp[2:n3 + 2] = zc[2:n3 + 2] / zf[3:n3 + 3]
+    """(d F(B q)/dq)^T g on a box with walls: qh three halo'd fields (ghosts as the forward set them), g
+    three interior arrays (ignored on the normal wall faces).  Returns three interior arrays, the normal
+    wall faces zero.  fold=False and swap_rdz=True plant mistakes for the tests' negative controls."""
+    n = list(case.n)
+    wd = wall_dirs(case)
+    # along a wall direction the arrays hold the ghost layers plus one zero layer on each side (index
+    # i + 1 for i = -1 .. n + 2); along a periodic one the interior only, and np.roll wraps
+    qsl = tuple(slice(None) if d in wd else slice(1, -1) for d in range(3))
+    qpad = [(1, 1) if d in wd else (0, 0) for d in range(3)]
+    gpad = [(2, 2) if d in wd else (0, 0) for d in range(3)]
+    Q = [np.pad(x[qsl], qpad) for x in qh]
+    G = [np.pad(x, gpad) for x in g]
+    for d in wd:  # the normal velocity's wall face i = n_d is not a degree of freedom
+        idx = [slice(None)] * 3
+        idx[d] = n[d] + 1
+        G[d][tuple(idx)] = 0.
+    n3 = n[2]
+    if 2 in wd:
+        zc, zf = np.zeros(n3 + 4), np.zeros(n3 + 4)
+        zc[1:n3 + 3], zf[1:n3 + 3] = case.dzci, case.dzfi
+        rdzm, rdzp = np.zeros(n3 + 4), np.zeros(n3 + 4)
+        rdzm[2:n3 + 2] = zc[2:n3 + 2] / zf[2:n3 + 2]
+        rdzp[2:n3 + 2] = zc[2:n3 + 2] / zf[3:n3 + 3]
+    else:
+        zc, zf = np.asarray(case.dzci[1:n3 + 1], float), np.asarray(case.dzfi[1:n3 + 1], float)
+        rdzm, rdzp = zc / zf, zc / np.roll(zf, -1)
     if swap_rdz:
         rdzm, rdzp = rdzp, rdzm
     zc_, zf_, rdzm_, rdzp_ = (x[None, None, :] for x in (zc, zf, rdzm, rdzp))
@@ -252,13 +290,24 @@ def mom_jt_walls(case, qh, g, fold=True, swap_rdz=False):
             out[c] -= _sh(K * Gd, d, -1) - K * _sh(Gd, d, 1)
             M = Gd * _sh(Q[c], d, 1) - _sh(Gd, d, 1) * Q[c]
             out[d] -= al(c, d) * M + _sh(be(c, d) * M, c, -1)
-    res = [np.asfortranarray(x[:, :, 2:n3 + 2]) for x in out]
-    for m in (0, 1) if fold else ():  # B^T: tangential ghosts u(0) = 2 ub - u(1) (D) or u(0) = u(1) + dzc(0) ub (N)
-        s0 = -1. if case.cbcvel[6 * m + 4] == "D" else 1.
-        s1 = -1. if case.cbcvel[6 * m + 5] == "D" else 1.
-        res[m][:, :, 0] += s0 * out[m][:, :, 1]
-        res[m][:, :, n3 - 1] += s1 * out[m][:, :, n3 + 2]
-    res[2][:, :, n3 - 1] = 0.  # w's wall faces are constants
+    # B^T, the reverse of bound's x, y, z order: z first
+    for d in sorted(wd, reverse=True) if fold else ():
+        lo, hi, f_lo, f_hi, face = ([slice(None)] * 3 for _ in range(5))
+        lo[d], hi[d], f_lo[d], f_hi[d], face[d] = 1, n[d] + 2, 2, n[d] + 1, n[d] + 1
+        for m in range(3):
+            if m == d:  # constants: the wall faces and whatever lies beyond them
+                for sl in (lo, hi, face):
+                    out[m][tuple(sl)] = 0.
+                continue
+            s0 = -1. if case.cbcvel[6 * m + 2 * d] == "D" else 1.
+            s1 = -1. if case.cbcvel[6 * m + 2 * d + 1] == "D" else 1.
+            out[m][tuple(f_lo)] += s0 * out[m][tuple(lo)]
+            out[m][tuple(f_hi)] += s1 * out[m][tuple(hi)]
+            out[m][tuple(lo)] = 0.
+            out[m][tuple(hi)] = 0.
+    isl = tuple(slice(2, n[d] + 2) if d in wd else slice(None) for d in range(3))
+    res = [np.asfortranarray(x[isl]) for x in out]
+    _clear_wall_faces(case, res)
     return res
 
 

@@ -13,6 +13,8 @@ differences.
 * walls in z on a tanh-stretched grid with bulk forcing (the channel): one adjoint
   substep at 1e-12 against the oracle, with a moving top wall and a Neumann v wall so
   that both ghost folds act; the gradient after 4 steps at 1e-10 against the oracle's;
+  the same for walls in y and z (a forced duct) and on all six faces (a closed box with
+  a moving lid, DCT pressure solves along x and y);
   and a 256x128x96 channel (Re_b = 5600, 16 steps, checkpoints every 4) whose
   directional derivatives along 4 seeded directions agree with central differences to
   1e-5.
@@ -158,19 +160,16 @@ def channel_pair(p):
     return Simulation(cases.host_case(p)), cases.oracle_case(p)
 
 
-def test_substep_adjoint_channel_matches_oracle(torch):
+def substep_vs_oracle(p, seed):
+    """One adjoint substep on the GPU against the oracle's, random degrees of freedom."""
     from oracle import adjoint as A
     from oracle import oracle as O
-    print("\n[case] cans_rk3_substep_adjoint, channel 16x12x20 (stretched, forced, moving wall)", flush=True)
-    bc = np.zeros(18)
-    bc[5], bc[10] = 1.0, 0.3
-    p = channel_params((16, 12, 20), visci=37.0, cbcvel="PPPPDD" + "PPPPND" + "PPPPDD", bcvel=bc)
     sim, orc = channel_pair(p)
-    rng = np.random.default_rng(21)
+    rng = np.random.default_rng(seed)
 
     def dofs():
         a = [rng.uniform(-1, 1, tuple(orc.n)) for _ in range(3)]
-        a[2][:, :, -1] = 0.
+        A._clear_wall_faces(orc, a)
         return a
 
     q, ub, fb = dofs(), dofs(), dofs()
@@ -183,10 +182,7 @@ def test_substep_adjoint_channel_matches_oracle(torch):
         out = []
         for a in arrays:
             f = sim.ctx.field()
-            if halo:
-                f.set(a)
-            else:
-                f.set(np.pad(a, 1))
+            f.set(a if halo else np.pad(a, 1))
             out.append(f)
         return out
 
@@ -198,21 +194,48 @@ def test_substep_adjoint_channel_matches_oracle(torch):
     sim.ctx.sync()
     ref_ub, ref_fb = A.substep_adjoint(orc, qh, irk - 1, dt, ub, fb)
     got_ub, got_fb = [f.interior() for f in fubo], [f.interior() for f in ffbo]
-    assert got_ub[2][:, :, -1].max() == 0. and got_fb[2][:, :, -1].max() == 0.
+    for d in A.wall_dirs(orc):  # the normal wall faces are zero
+        idx = [slice(None)] * 3
+        idx[d] = -1
+        assert np.all(got_ub[d][tuple(idx)] == 0.) and np.all(got_fb[d][tuple(idx)] == 0.)
     e1, e2 = relerr(got_ub, ref_ub), relerr(got_fb, ref_fb)
     print("adjoint substep rel err: velocity %.3e, rhs %.3e" % (e1, e2), flush=True)
     assert e1 < 1e-12 and e2 < 1e-12
 
 
-def test_gradient_channel_matches_oracle(torch):
+def test_substep_adjoint_channel_matches_oracle(torch):
+    print("\n[case] cans_rk3_substep_adjoint, channel 16x12x20 (stretched, forced, moving wall)", flush=True)
+    bc = np.zeros(18)
+    bc[5], bc[10] = 1.0, 0.3
+    substep_vs_oracle(channel_params((16, 12, 20), visci=37.0, cbcvel="PPPPDD" + "PPPPND" + "PPPPDD", bcvel=bc), 21)
+
+
+def walled_params(kind, ng, visci=60.0, is_forced=(0, 0, 0)):
+    """duct: periodic x, walls in y and z; box: walls on all six faces; a moving lid (u = 1 at z = lz),
+    a Neumann u on the y = 0 wall, tanh-stretched z."""
+    bc = np.zeros(18)
+    bc[5] = 1.0
+    if kind == "duct":
+        cbcvel, cbcpre = "PPNDDD" + "PPDDDD" + "PPDDDD", "PPNNNN"
+    else:
+        cbcvel, cbcpre = "DDNDDD" + "DDDDDD" + "DDDDDD", "NNNNNN"
+    return dict(ng=ng, l=(1.3, 1.0, 1.0), gtype=1, gr=1.5, cbcvel=cbcvel, cbcpre=cbcpre, bcvel=tuple(bc),
+                visci=visci, is_forced=is_forced, velf=(1.0, 0.0, 0.0))
+
+
+@pytest.mark.parametrize("kind", ["duct", "box"])
+def test_substep_adjoint_walls_xyz_matches_oracle(torch, kind):
+    print("\n[case] cans_rk3_substep_adjoint, %s 16x12x20 (stretched, moving lid)" % kind, flush=True)
+    substep_vs_oracle(walled_params(kind, (16, 12, 20), visci=37.0,
+                                    is_forced=(1, 0, 0) if kind == "duct" else (0, 0, 0)), 22)
+
+
+def gradient_vs_oracle(p, u0, dt=2e-3, nsteps=4):
     from cans_amd.adjoint import kinetic_energy_after_steps
     from oracle import adjoint as A
-    print("\n[case] gradient, channel 16x12x20, 4 steps", flush=True)
-    p = channel_params((16, 12, 20))
+    import torch
     sim, orc = channel_pair(p)
-    u0 = list(cases.initial_state("channel", p, sim.grid, seed=6)[:3])
-    u0[2][:, :, -1] = 0.
-    dt, nsteps = 2e-3, 4
+    A._clear_wall_faces(orc, u0)
     E_orc, g_orc = A.grad_ke_after_steps(orc, *u0, dt, nsteps, every=2)
     t0 = [torch.tensor(np.ascontiguousarray(a.transpose(2, 1, 0)), device="cuda", requires_grad=True) for a in u0]
     E = kinetic_energy_after_steps(sim, *t0, dt, nsteps, every=2)
@@ -222,7 +245,22 @@ def test_gradient_channel_matches_oracle(torch):
     eg = relerr(g_gpu, g_orc)
     print("E %.15e (oracle %.15e, rel %.2e); gradient rel err %.3e" % (E.item(), E_orc, eE, eg), flush=True)
     assert eE < 1e-13 and eg < 1e-10
-    assert float(np.abs(g_gpu[2][:, :, -1]).max()) == 0.
+
+
+def test_gradient_channel_matches_oracle(torch):
+    print("\n[case] gradient, channel 16x12x20, 4 steps", flush=True)
+    p = channel_params((16, 12, 20))
+    u0 = list(cases.initial_state("channel", p, channel_pair(p)[0].grid, seed=6)[:3])
+    gradient_vs_oracle(p, u0)
+
+
+@pytest.mark.parametrize("kind", ["duct", "box"])
+def test_gradient_walls_xyz_matches_oracle(torch, kind):
+    print("\n[case] gradient, %s 16x12x20, 4 steps" % kind, flush=True)
+    p = walled_params(kind, (16, 12, 20), is_forced=(1, 0, 0) if kind == "duct" else (0, 0, 0))
+    rng = np.random.default_rng(7)
+    u0 = [0.3 * rng.uniform(-1, 1, tuple(p["ng"])) for _ in range(3)]
+    gradient_vs_oracle(p, u0)
 
 
 @pytest.mark.timeout(600)

@@ -96,9 +96,9 @@ def channel_case(ng=(8, 6, 10), gr=1.8, cbcvel="PPPPDD" * 3, bcvel=None, is_forc
 
 
 def dofs(case, rng):
-    """Random values on the degrees of freedom (w's wall face k = n3 zero)."""
+    """Random values on the degrees of freedom (the normal velocities' wall faces zero)."""
     a = [rng.uniform(-1, 1, tuple(case.n)) for _ in range(3)]
-    a[2][:, :, -1] = 0.
+    A._clear_wall_faces(case, a)
     return a
 
 
@@ -132,7 +132,7 @@ def test_mom_jt_walls_dot_product():
     fp = A.mom(case, *ghosts(case, [a + b for a, b in zip(q, d)]))
     fm = A.mom(case, *ghosts(case, [a - b for a, b in zip(q, d)]))
     jd = [(a - b) / 2 for a, b in zip(fp, fm)]
-    jd[2][:, :, -1] = 0.  # F_w on the wall face is overwritten by bound
+    A._clear_wall_faces(case, jd)  # F on a normal wall face is overwritten by bound
     lhs = dot(g, jd)
     jt = A.mom_jt_walls(case, ghosts(case, q), g)
     assert np.all(jt[2][:, :, -1] == 0.)
@@ -183,6 +183,78 @@ def test_gradient_channel_matches_central_differences():
     # the oracle's bulk mean is an OpenMP reduction, so repeated forward runs agree to round-off only
     assert E == pytest.approx(A.ke_after_steps(case, *u0, dt, nsteps), rel=1e-13)
     assert np.all(grad[2][:, :, -1] == 0.)
+    eps = 1e-6
+    for _ in range(3):
+        d = dofs(case, rng)
+        ep = A.ke_after_steps(case, *(a + eps * b for a, b in zip(u0, d)), dt, nsteps)
+        em = A.ke_after_steps(case, *(a - eps * b for a, b in zip(u0, d)), dt, nsteps)
+        fd = (ep - em) / (2 * eps)
+        ad = dot(grad, d)
+        assert abs(fd - ad) <= 1e-6 * abs(ad), (fd, ad)
+
+
+# ---- walls in x and y too (duct, closed box): DCT-based pressure solves ----
+
+def walled_case(kind, ng=(8, 6, 10), gr=1.5, visci=37.0, is_forced=(0, 0, 0)):
+    """duct: periodic x, walls in y and z; box: walls on all six faces (a moving lid: u = 1 at z = lz,
+    a Neumann u on the y = 0 wall); both on a tanh-stretched z."""
+    bc = np.zeros(18)
+    bc[5] = 1.0
+    if kind == "duct":
+        cbcvel = "PPNDDD" + "PPDDDD" + "PPDDDD"
+        cbcpre = "PPNNNN"
+    else:
+        cbcvel = "DDNDDD" + "DDDDDD" + "DDDDDD"
+        cbcpre = "NNNNNN"
+    return O.OracleCase(ng=ng, l=(1.3, 1.0, 1.0), gtype=1, gr=gr, cbcvel=cbcvel, cbcpre=cbcpre, bcvel=bc,
+                        visci=visci, is_forced=is_forced, velf=(1.0, 0.0, 0.0))
+
+
+@pytest.mark.parametrize("kind", ["duct", "box"])
+def test_mom_jt_walls_xyz_dot_product(kind):
+    case = walled_case(kind)
+    A.check_case(case)
+    rng = np.random.default_rng(31)
+    q, d, g = dofs(case, rng), dofs(case, rng), dofs(case, rng)
+    fp = A.mom(case, *ghosts(case, [a + b for a, b in zip(q, d)]))
+    fm = A.mom(case, *ghosts(case, [a - b for a, b in zip(q, d)]))
+    jd = [(a - b) / 2 for a, b in zip(fp, fm)]
+    A._clear_wall_faces(case, jd)
+    lhs = dot(g, jd)
+    jt = A.mom_jt_walls(case, ghosts(case, q), g)
+    rhs = dot(jt, d)
+    scale = sum(float(np.abs(x).sum()) for x in jd)
+    assert abs(lhs - rhs) <= 1e-13 * scale, (lhs, rhs)
+    assert abs(dot(A.mom_jt_walls(case, ghosts(case, q), g, fold=False), d) - lhs) > 1e-6 * scale
+
+
+@pytest.mark.parametrize("kind", ["duct", "box"])
+def test_projection_transpose_walls_xyz(kind):
+    case = walled_case(kind, ng=(8, 8, 12))
+    rng = np.random.default_rng(32)
+    a, b = dofs(case, rng), dofs(case, rng)
+    pa = A.project_dofs(case, a)
+    lhs, rhs = dot(A.project_t(case, a), b), dot(a, A.project_dofs(case, b))
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs), (lhs, rhs)
+    assert abs(dot(pa, b) - rhs) > 1e-6 * abs(rhs)
+    _, divmax = case.chkdiv(*A.dof_fields(case, pa))
+    assert divmax < 1e-11
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("kind", ["duct", "box"])
+def test_gradient_walls_xyz_matches_central_differences(kind):
+    case = walled_case(kind, ng=(8, 6, 10), visci=60.0, is_forced=(1, 0, 0) if kind == "duct" else (0, 0, 0))
+    rng = np.random.default_rng(33)
+    u0 = dofs(case, rng)
+    u0 = [0.3 * x for x in u0]
+    dt, nsteps = 2e-3, 4
+    E, grad = A.grad_ke_after_steps(case, *u0, dt, nsteps, every=2)
+    for m in range(3):  # the normal wall faces carry no gradient
+        idx = [slice(None)] * 3
+        idx[m] = -1
+        if case.cbcpre[2 * m] != "P":
+            assert np.all(grad[m][tuple(idx)] == 0.)
     eps = 1e-6
     for _ in range(3):
         d = dofs(case, rng)
