@@ -19,6 +19,9 @@ B_P = 5 passes x N cells x (s_in + s_out) = 40 N bytes with FP32 storage, 80 N
 with FP64 storage.  The oracle (C/OpenMP restatement) is timed on the host on a
 bounded number of full steps of the same case as the CPU baseline.
 
+"library_fft_reference" times rocFFT (through torch.fft) on the Poisson solve's x and y
+transforms of the same field, for scale beside the solve's own transform passes.
+
 Secondary workloads, timed after the headline one and reported under "secondary":
 the 256^3 heated cube with a scalar, buoyancy and 3-D implicit diffusion (FP64 and mixed),
 64 independent Poisson boxes of different sizes solved as one batch (FP64 and mixed
@@ -214,6 +217,39 @@ def solve_only(p, precision, args):
     gbs = b_p / t_p / 1e9
     print("[bench] solve-only: %.3f ms (%.0f GB/s)" % (t_p * 1e3, gbs), file=sys.stderr, flush=True)
     return {"ms": t_p * 1e3, "GBps": gbs, "pct_hbm_peak": 100.0 * gbs / HBM_PEAK_GBS, "algorithmic_bytes": b_p}
+
+
+def library_fft_reference(p, reps=10):
+    """The vendor library on the solve's x and y transforms, for comparison with the four transform passes
+    of the Poisson solve: rocFFT through torch.fft (hipFFT) on the channel's (n3, n2, n1) field, a real
+    transform along x (rfft, n1/2+1 complex), a complex one along y, then their inverses; FP32 and FP64,
+    out of place, mean of `reps` after a warm-up."""
+    import torch
+    n1, n2, n3 = p["ng"]
+    out = {"workload": "rfft along x + fft along y, then ifft along y + irfft along x, on (n3, n2, n1) = "
+                       "(%d, %d, %d), torch.fft (rocFFT)" % (n3, n2, n1)}
+    for name, dt in (("fp32_ms", torch.float32), ("fp64_ms", torch.float64)):
+        x = torch.empty((n3, n2, n1), dtype=dt, device="cuda").uniform_(-1, 1)
+
+        def run():
+            c = torch.fft.fft(torch.fft.rfft(x, dim=2), dim=1)
+            return torch.fft.irfft(torch.fft.ifft(c, dim=1), n=n1, dim=2)
+
+        y = run()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            y = run()
+        e1.record()
+        torch.cuda.synchronize()
+        out[name] = e0.elapsed_time(e1) / reps
+        out[name.replace("_ms", "_roundtrip_rel_err")] = float((y - x).abs().max() / x.abs().max())
+        del x, y
+        torch.cuda.empty_cache()
+    print("[bench] rocFFT x/y transforms: %.3f ms FP32, %.3f ms FP64" % (out["fp32_ms"], out["fp64_ms"]),
+          file=sys.stderr, flush=True)
+    return out
 
 
 def secondary_cavity(args, precision="fp64"):
@@ -543,6 +579,7 @@ def main():
             out[m + "_vs_fp64_rel_linf"] = err
     del final
     out["copy_GBps"] = copy_bandwidth()
+    out["library_fft_reference"] = library_fft_reference(p)
     if not args.no_secondary:
         out["secondary"] = [secondary_cavity(args), secondary_cavity(args, "mixed"), secondary_boxes(args),
                             secondary_adjoint(args), secondary_adjoint(args, "channel")]
