@@ -317,6 +317,34 @@ __global__ void k_adj_weigh(Layout L, double* u, double* v, double* w, const dou
   if ((wm & 4) && k == L.n3) w[o] = 0.;
 }
 
+// out += c L a on the interior, L the 7-point Laplacian of a tri-periodic uniform grid (h2: 1/dx^2,
+// 1/dy^2, 1/dz^2), a halo'd with its periodic ghost cells set: the implicit-diffusion adjoint's
+// (f1 + f2)/2 visc L hb
+__global__ void k_adj_lap(Layout L, const double* a0, const double* a1, const double* a2, double* o0, double* o1,
+                          double* o2, double c, double hx2, double hy2, double hz2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
+  if (i > L.n1) return;
+  const long o = L.at(i, j, k), sy = L.sy, sz = L.sz;
+  const double* a[3] = {a0, a1, a2};
+  double* out[3] = {o0, o1, o2};
+#pragma unroll
+  for (int m = 0; m < 3; m++) {
+    const double* x = a[m];
+    const double x0 = x[o];
+    const double lap = hx2 * (x[o + 1] - 2. * x0 + x[o - 1]) + hy2 * (x[o + sy] - 2. * x0 + x[o - sy]) +
+                       hz2 * (x[o + sz] - 2. * x0 + x[o - sz]);
+    out[m][o] += c * lap;
+  }
+}
+
+int launch_adj_lap(cans_ctx* c, const double* const* a, double* const* out, double coef, const double* h) {
+  const Layout& L = c->L;
+  k_adj_lap<<<dim3(cdiv(L.n1, 64), L.n2, L.n3), 64, 0, c->stream>>>(L, a[0], a[1], a[2], out[0], out[1], out[2],
+                                                                    coef, h[0] * h[0], h[1] * h[1], h[2] * h[2]);
+  CANS_LAUNCH_CHECK("k_adj_lap");
+  return CANS_OK;
+}
+
 static JtArgs jt_args(cans_ctx* c, const double* const* q, const double* const* g, const double* const* fb,
                       const double* const* base, double* const* out, double* const* fbo, const double* h,
                       double visc, double f1, double f2, const double* dzci, const double* dzfi) {

@@ -880,8 +880,15 @@ int adjoint_walls(const cans_step_params* s) {
 int adjoint_scope(cans_ctx* c, const cans_step_params* s, const char* who) {
   const std::string w(who);
   if (c->precision != CANS_FP64) return fail(CANS_EUNSUPPORTED, w + ": the adjoint needs a CANS_FP64 context");
-  if (c->imp) return fail(CANS_EUNSUPPORTED, w + ": the adjoint covers explicit diffusion");
-  if (!s) return CANS_OK;
+  if (!s) {
+    if (c->imp) return fail(CANS_EUNSUPPORTED, w + ": the transposed Jacobian covers explicit diffusion");
+    return CANS_OK;
+  }
+  // implicit diffusion: the 3-D form on tri-periodic uniform grids, where the Helmholtz operator
+  // commutes with the gradient, so that the pressure still drops out (P H^-1 G p = 0)
+  if (c->imp == 2) return fail(CANS_EUNSUPPORTED, w + ": the adjoint covers 3-D implicit diffusion, not z-only");
+  if (c->imp && adjoint_walls(s))
+    return fail(CANS_EUNSUPPORTED, w + ": with implicit diffusion the adjoint covers tri-periodic grids");
   if (s->nscal) return fail(CANS_EUNSUPPORTED, w + ": the adjoint covers flows without scalars");
   const int n[3] = {c->L.n1, c->L.n2, c->L.n3};
   const int wm = adjoint_walls(s);
@@ -989,6 +996,20 @@ extern "C" int cans_rk3_substep_adjoint(cans_ctx* c, const cans_step_params* s, 
   if ((rc = launch_correc<double>(c, P, a[0], a[1], a[2], nullptr, nullptr, nullptr, nullptr, a[0], a[1], a[2],
                                   s->dzci_d, 1., s->dli, nullptr)))
     return rc;
+  // implicit diffusion (main.f90:453-462): hb = H^-1 P ub, H = I - visc dtrk/2 L symmetric here, one
+  // batched solve of the three components' systems with homogeneous boundary values
+  const double dtrk = f1 + f2;
+  if (c->imp) {
+    HelmSys hs[3];
+    for (int q = 0; q < 3; q++) {
+      const cans_helmholtz& hq = s->hvel[q];
+      if (!hq.plan || !hq.lambdaxy_d || !hq.a_d || !hq.b_d || !hq.c_d)
+        return fail(CANS_EINVAL, "cans_rk3_substep_adjoint: Helmholtz system without a plan");
+      hs[q] = HelmSys{hq.plan, hq.normfft, hq.lambdaxy_d, hq.a_d, hq.b_d, hq.c_d, hq.cbc, hq.c_or_f, a[q],
+                      -.5 * s->visc * dtrk};
+    }
+    if ((rc = run_helmholtz_batch(c, 3, hs))) return rc;
+  }
   double mean[2] = {0., 0.};
   for (int m = 0; m < 2; m++)
     if (s->is_forced[m] && (rc = run_bulk_mean<double>(c, a[m], s->grid_vol_ratio_f_d, &mean[m]))) return rc;
@@ -1003,7 +1024,13 @@ extern "C" int cans_rk3_substep_adjoint(cans_ctx* c, const cans_step_params* s, 
   const double* q[3] = {(const double*)u, (const double*)v, (const double*)w};
   double* o[3] = {(double*)ubo, (double*)vbo, (double*)wbo};
   double* fo[3] = {(double*)fuo, (double*)fvo, (double*)fwo};
-  if (!wm) return launch_mom_jt_adj(c, q, a, fb, o, fo, h, s->visc, f1, f2, nullptr);
+  if (!wm) {
+    // implicit diffusion: J is the advection's alone, and the diffusion's explicit half adds
+    // (f1 + f2)/2 visc L hb
+    if ((rc = launch_mom_jt_adj(c, q, a, fb, o, fo, h, c->imp ? 0. : s->visc, f1, f2, nullptr))) return rc;
+    if (c->imp) return launch_adj_lap(c, a, o, .5 * dtrk * s->visc, h);
+    return CANS_OK;
+  }
   AdjWalls aw;
   aw.mask = wm;
   aw.dzci = s->dzci_d;

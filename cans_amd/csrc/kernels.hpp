@@ -226,6 +226,8 @@ struct AdjWalls {
 int launch_mom_jt_adj(cans_ctx* c, const double* const* q, const double* const* wb, const double* const* fb,
                       double* const* out, double* const* fbo, const double* h, double visc, double f1, double f2,
                       const AdjWalls* wall);
+// out += coef L a (tri-periodic uniform grid; h = 1/dx, 1/dy, 1/dz; a with periodic ghosts)
+int launch_adj_lap(cans_ctx* c, const double* const* a, double* const* out, double coef, const double* h);
 // the volume weights of P^T = M P M^-1 and the transposed bulk forcing (k_adj_weigh)
 int launch_adj_weigh(cans_ctx* c, double* const* f, const double* dzfi, const double* dzci, bool inv, double mu,
                      double mv, int wm);

@@ -392,8 +392,11 @@ int cans_ctx_forcing(cans_ctx *ctx, const double **f_d);
 
 /* ---------------------------------------------------------------------------
  * Adjoint of the RK3 substep (no counterpart in the reference; for gradients of a
- * loss on the velocity).  Scope: CANS_FP64 contexts, explicit diffusion, no scalars
- * or buoyancy; each direction periodic or bounded by walls (Neumann pressure,
+ * loss on the velocity).  Scope: CANS_FP64 contexts, no scalars or buoyancy; explicit
+ * diffusion, or 3-D implicit diffusion on tri-periodic uniform grids (where the
+ * substep's Helmholtz operator H = I - visc dtrk/2 L commutes with the gradient and is
+ * symmetric: the adjoint applies H^-1 after P^T, the advection's transposed Jacobian,
+ * and the diffusion's explicit half dtrk/2 visc L); each direction periodic or bounded by walls (Neumann pressure,
  * Dirichlet normal velocity, Dirichlet or Neumann tangential velocity, any wall
  * values), a periodic z on a uniform grid and walls in z on any grid; bulk forcing of
  * u and v along periodic x and y.  There substep irk maps the degrees of freedom

@@ -5,8 +5,9 @@ Like the rest of ``oracle/``, this module is the checker for the HIP path: only
 
 Scope: each direction either periodic or bounded by walls (Neumann pressure,
 Dirichlet normal velocity, Dirichlet or Neumann tangential velocity); a periodic
-z on a uniform grid, walls in z on any (stretched) grid; explicit diffusion, bulk
-forcing of u and v along periodic x and y, no scalars.  The velocity map of RK3
+z on a uniform grid, walls in z on any (stretched) grid; explicit diffusion, or
+3-D implicit diffusion on a tri-periodic uniform grid (see substep_adjoint); bulk
+forcing of u and v along periodic x and y; no scalars.  The velocity map of RK3
 substep irk (src/main.f90:437-473 with rk.f90:24-333, fillps, solver, correc) on
 the degrees of freedom (every interior value except a normal velocity's wall face,
 u(n1), v(n2) or w(n3), which bound.f90 fixes) is
@@ -103,8 +104,10 @@ def check_case(case):
         raise ValueError("a periodic z needs a uniform grid")
     if case.is_forced[2] or any(case.is_forced[c] and c in wall_dirs(case) for c in (0, 1)):
         raise ValueError("forcing covers u and v along periodic directions")
-    if case.scalars or case.is_impdiff:
-        raise ValueError("the adjoint covers explicit diffusion and no scalars")
+    if case.scalars:
+        raise ValueError("the adjoint covers flows without scalars")
+    if case.is_impdiff and (walls(case) or case.is_impdiff_1d):
+        raise ValueError("implicit diffusion: the adjoint covers the 3-D form on tri-periodic uniform grids")
 
 
 def _clear_wall_faces(case, a):
@@ -134,10 +137,12 @@ def _lap(case, a):
     return sum(h2[d] * (_sh(a, d, 1) - 2 * a + _sh(a, d, -1)) for d in range(3))
 
 
-def mom_jt(case, q, g):
-    """(dF/dq)^T g at velocity q; q and g are lists of three interior arrays."""
+def mom_jt(case, q, g, visc=None):
+    """(dF/dq)^T g at velocity q; q and g are lists of three interior arrays (visc: the diffusion
+    coefficient, case.visc unless given; 0 for the advection alone)."""
     h = [case.dli[0], case.dli[1], case.dzci[1]]
-    out = [case.visc * _lap(case, gc) for gc in g]
+    visc = case.visc if visc is None else visc
+    out = [visc * _lap(case, gc) for gc in g]
     for c in range(3):
         for d in range(3):
             K = 0.25 * (q[d] + _sh(q[d], c, 1))
@@ -343,6 +348,11 @@ def _step_recording(case, st, dt):
         rkpar = O.RKCOEFF[irk]
         dtrk = (rkpar[0] + rkpar[1]) * dt
         case.rk(rkpar, dt, p, st["dudtrko"], st["dvdtrko"], st["dwdtrko"], u, v, w)
+        alpha = 0.0
+        if case.is_impdiff:  # main.f90:453-462
+            alpha = -.5 * case.visc * dtrk
+            for q, h in zip((u, v, w), case.h_vel):
+                case.solve_helmholtz(q, alpha, h)
         case.bounduvw(u, v, w, 0, -1)
         case.fillps(u, v, w, pp, dtrk ** (-1))
         case.updt_rhs_b(pp)
@@ -350,14 +360,36 @@ def _step_recording(case, st, dt):
         case.boundp(pp)
         case.correc(pp, u, v, w, dtrk)
         case.bounduvw(u, v, w, 1, 1, 0)
-        case.updatep(pp, p, 0.0)
+        case.updatep(pp, p, alpha)
         case.boundp(p, 1)
     return starts
 
 
+def helmholtz_inv(case, a, dtrk):
+    """H^-1 a on each component, H = I - visc dtrk/2 L the implicit-diffusion operator of the substep
+    (main.f90:453-462; symmetric on a tri-periodic uniform grid, so H^-T = H^-1)."""
+    out = []
+    for c, x in enumerate(a):
+        f = halo(case, x)
+        case.solve_helmholtz(f, -0.5 * case.visc * dtrk, case.h_vel[c])
+        out.append(interior(f))
+    return out
+
+
 def substep_adjoint(case, start, irk, dt, ub, fb):
-    """Adjoint of substep irk (0-based) at the halo'd starting velocity `start`: (ub, fb) -> (ub, fb)."""
+    """Adjoint of substep irk (0-based) at the halo'd starting velocity `start`: (ub, fb) -> (ub, fb).
+
+    With implicit diffusion (3-D, tri-periodic uniform grid) the substep is
+    u' = P H^-1 Fr(u + f1 A(u) + f2 Ao + (f1 + f2)/2 visc L u), Ao' = A(u), A the advection alone
+    (rk.f90:156-186, 312-326), so its adjoint is hb = Fr'^T H^-1 P ub,
+    ub <- hb + J_A^T (f1 hb + fb) + (f1 + f2)/2 visc L hb, fb <- f2 hb."""
     f1, f2 = O.RKCOEFF[irk][0] * dt, O.RKCOEFF[irk][1] * dt
+    if case.is_impdiff:
+        hb = forcing_t(case, helmholtz_inv(case, project_t(case, ub), f1 + f2))
+        g = [f1 * a + b for a, b in zip(hb, fb)]
+        jt = mom_jt(case, [interior(x) for x in start], g, visc=0.)
+        half = 0.5 * (f1 + f2) * case.visc
+        return [a + b + half * _lap(case, a) for a, b in zip(hb, jt)], [f2 * a for a in hb]
     wb = forcing_t(case, project_t(case, ub))
     g = [f1 * a + b for a, b in zip(wb, fb)]
     if walls(case):

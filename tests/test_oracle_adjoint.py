@@ -263,3 +263,30 @@ def test_gradient_walls_xyz_matches_central_differences(kind):
         fd = (ep - em) / (2 * eps)
         ad = dot(grad, d)
         assert abs(fd - ad) <= 1e-6 * abs(ad), (fd, ad)
+
+
+@pytest.mark.timeout(300)
+def test_gradient_implicit_diffusion_matches_central_differences():
+    """3-D implicit diffusion (Helmholtz solves in every substep) on a tri-periodic box."""
+    case = O.OracleCase(ng=(12, 10, 8), l=(2 * np.pi,) * 3, gtype=1, gr=0.0, cbcvel="PP" * 9, cbcpre="PP" * 3,
+                        visci=20.0, is_impdiff=True, is_forced=(1, 0, 0), velf=(0.3, 0.0, 0.0))
+    rng = np.random.default_rng(41)
+    u0 = tgv_state(case, rng, noise=0.1)
+    dt, nsteps = 2e-2, 4
+    E, grad = A.grad_ke_after_steps(case, *u0, dt, nsteps, every=2)
+    eps = 1e-6
+    for _ in range(3):
+        d = [rng.standard_normal(tuple(case.n)) for _ in range(3)]
+        ep = A.ke_after_steps(case, *(a + eps * b for a, b in zip(u0, d)), dt, nsteps)
+        em = A.ke_after_steps(case, *(a - eps * b for a, b in zip(u0, d)), dt, nsteps)
+        fd = (ep - em) / (2 * eps)
+        ad = dot(grad, d)
+        assert abs(fd - ad) <= 1e-6 * abs(ad), (fd, ad)
+    # planted mistake: the adjoint without its Helmholtz solves misses the central difference
+    helm = A.helmholtz_inv
+    try:
+        A.helmholtz_inv = lambda c, a, dtrk: a
+        _, bad = A.grad_ke_after_steps(case, *u0, dt, nsteps, every=2)
+    finally:
+        A.helmholtz_inv = helm
+    assert abs(dot(bad, d) - fd) > 1e-4 * abs(fd)
