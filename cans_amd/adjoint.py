@@ -1,6 +1,7 @@
 """Gradients through RK3 steps on the GPU (FP64; each direction periodic or bounded by
 walls, a periodic z on a uniform grid, walls in z on any grid; bulk forcing of u and v
-along periodic directions).
+along periodic directions; explicit diffusion, or 3-D implicit diffusion on tri-periodic
+grids).
 
 ``kinetic_energy_after_steps(sim, u0, v0, w0, dt, nsteps, every)`` runs nsteps fused
 RK3 steps (cans_rk3_substep) from the interior velocity (u0, v0, w0) with zero
