@@ -1,7 +1,8 @@
 """Gradients through RK3 steps on the GPU (FP64; each direction periodic or bounded by
 walls, a periodic z on a uniform grid, walls in z on any grid; bulk forcing of u and v
 along periodic directions; explicit diffusion, or 3-D implicit diffusion on tri-periodic
-grids).
+grids).  Under CANS_MIXED the projections (forward and adjoint) run with FP32 pressure
+storage and transforms, the velocity and everything else in FP64.
 
 ``kinetic_energy_after_steps(sim, u0, v0, w0, dt, nsteps, every)`` runs nsteps fused
 RK3 steps (cans_rk3_substep) from the interior velocity (u0, v0, w0) with zero
@@ -56,8 +57,8 @@ def _addr(f, buf):
 class _KineticEnergyAfterSteps(torch.autograd.Function):
     @staticmethod
     def forward(ctx, u0, v0, w0, sim, dt, nsteps, every):
-        if sim.ctx.precision != LB.CANS_FP64:
-            raise LB.CansError("the adjoint needs a CANS_FP64 simulation")
+        if sim.ctx.precision not in (LB.CANS_FP64, LB.CANS_MIXED):
+            raise LB.CansError("the adjoint needs a CANS_FP64 or CANS_MIXED simulation")
         for f, a in zip((sim.u, sim.v, sim.w), (u0, v0, w0)):
             f.buf.zero_()
             _interior(f).copy_(a)

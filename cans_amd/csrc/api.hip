@@ -879,7 +879,8 @@ int adjoint_walls(const cans_step_params* s) {
 
 int adjoint_scope(cans_ctx* c, const cans_step_params* s, const char* who) {
   const std::string w(who);
-  if (c->precision != CANS_FP64) return fail(CANS_EUNSUPPORTED, w + ": the adjoint needs a CANS_FP64 context");
+  if (c->precision != CANS_FP64 && c->precision != CANS_MIXED)
+    return fail(CANS_EUNSUPPORTED, w + ": the adjoint needs a CANS_FP64 or CANS_MIXED context");
   if (!s) {
     if (c->imp) return fail(CANS_EUNSUPPORTED, w + ": the transposed Jacobian covers explicit diffusion");
     return CANS_OK;
@@ -971,7 +972,6 @@ extern "C" int cans_rk3_substep_adjoint(cans_ctx* c, const cans_step_params* s, 
   const double f1 = rkpar[0] * dt, f2 = rkpar[1] * dt;
   double* a[3] = {(double*)ub, (double*)vb, (double*)wb};
   double* fb[3] = {(double*)fu, (double*)fv, (double*)fw};
-  double* P = (double*)pp;
   const int wm = adjoint_walls(s);
   const bool zwall = wm & 4;
   // the linear part of bound.f90's ghost map (boundary values zero): periodic images, and at walls
@@ -988,14 +988,21 @@ extern "C" int cans_rk3_substep_adjoint(cans_ctx* c, const cans_step_params* s, 
   if (wm && (rc = launch_adj_weigh(c, a, zwall ? s->dzfi_d : nullptr, zwall ? s->dzci_d : nullptr, true, 0., 0., wm)))
     return rc;
   if ((rc = ghosts(a))) return rc;
-  if ((rc = launch_fillps<double>(c, a[0], a[1], a[2], nullptr, nullptr, nullptr, nullptr, P, s->dzfi_d, 1., s->dli)))
-    return rc;
-  if ((rc = run_solver<double>(c, s->plan, s->normfft, s->lambdaxy_d, s->a_d, s->b_d, s->c_d, s->cbcpre, "ccc", P)))
-    return rc;
-  if ((rc = plan_boundp(c, s->cbcpre, zero, s->nb, s->is_bound, s->dl, s->dzc_h, P, CANS_HALO_ABSENT))) return rc;
-  if ((rc = launch_correc<double>(c, P, a[0], a[1], a[2], nullptr, nullptr, nullptr, nullptr, a[0], a[1], a[2],
-                                  s->dzci_d, 1., s->dli, nullptr)))
-    return rc;
+  // the projection in the context's pressure storage (CANS_MIXED: FP32 pp and transforms, FP64
+  // elimination and velocity, as the forward substep projects)
+  auto project = [&](auto tag) -> int {
+    using S = decltype(tag);
+    S* P = (S*)pp;
+    int r;
+    if ((r = launch_fillps<S>(c, a[0], a[1], a[2], nullptr, nullptr, nullptr, nullptr, P, s->dzfi_d, 1., s->dli)))
+      return r;
+    if ((r = run_solver<S>(c, s->plan, s->normfft, s->lambdaxy_d, s->a_d, s->b_d, s->c_d, s->cbcpre, "ccc", P)))
+      return r;
+    if ((r = plan_boundp(c, s->cbcpre, zero, s->nb, s->is_bound, s->dl, s->dzc_h, P, CANS_HALO_ABSENT))) return r;
+    return launch_correc<S>(c, P, a[0], a[1], a[2], nullptr, nullptr, nullptr, nullptr, a[0], a[1], a[2], s->dzci_d,
+                            1., s->dli, nullptr);
+  };
+  if ((rc = c->precision == CANS_MIXED ? project(float{}) : project(double{}))) return rc;
   // implicit diffusion (main.f90:453-462): hb = H^-1 P ub, H = I - visc dtrk/2 L symmetric here, one
   // batched solve of the three components' systems with homogeneous boundary values
   const double dtrk = f1 + f2;

@@ -392,7 +392,9 @@ int cans_ctx_forcing(cans_ctx *ctx, const double **f_d);
 
 /* ---------------------------------------------------------------------------
  * Adjoint of the RK3 substep (no counterpart in the reference; for gradients of a
- * loss on the velocity).  Scope: CANS_FP64 contexts, no scalars or buoyancy; explicit
+ * loss on the velocity).  Scope: CANS_FP64 and CANS_MIXED contexts (under CANS_MIXED
+ * the adjoint's projection runs in FP32 pressure storage, as the forward's does; pp_d
+ * is then an FP32 field), no scalars or buoyancy; explicit
  * diffusion, or 3-D implicit diffusion on tri-periodic uniform grids (where the
  * substep's Helmholtz operator H = I - visc dtrk/2 L commutes with the gradient and is
  * symmetric: the adjoint applies H^-1 after P^T, the advection's transposed Jacobian,
