@@ -2809,7 +2809,7 @@ static int tune_type(cans_ctx* c, cans_plan* p) {
   int ybest = 0;
   for (int yl : {4, 8, 16, 32}) {
     if (p->dense[1]) break;
-    if ((long)yl * tny > 1024 || (size_t)yl * p->nfft[1] * sizeof(cplx<T>) > 65536) continue;
+    if ((long)yl * tny > 1024 || (size_t)yl * p->nfft[1] * sizeof(cplx<T>) > (size_t)lds_max) continue;
     p->ylines[slot] = yl;
     const float ms = timed([&]() -> int {
       int r = ypass<T>(&cs, p, &W, 1, +1);
