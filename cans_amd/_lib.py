@@ -127,6 +127,8 @@ SIGNATURES = {
     "cans_ctx_forcing": (C.c_int, [_vp, C.POINTER(_vp)]),
     "cans_mom_jt": (C.c_int, [_vp, _ip, _dp, C.c_double] + [_vp] * 12),
     "cans_rk3_substep_adjoint": (C.c_int, [_vp, C.POINTER(StepParams), C.c_int, C.c_double] + [_vp] * 16),
+    "cans_rk3_substep_adjoint_scal": (C.c_int, [_vp, C.POINTER(StepParams), C.c_int, C.c_double] + [_vp] * 16
+                                      + [C.POINTER(_vp)] * 5),
 }
 
 _lib = None

@@ -2,7 +2,10 @@
 walls, a periodic z on a uniform grid, walls in z on any grid; bulk forcing of u and v
 along periodic directions; explicit diffusion, or 3-D implicit diffusion on tri-periodic
 grids).  Under CANS_MIXED the projections (forward and adjoint) run with FP32 pressure
-storage and transforms, the velocity and everything else in FP64.
+storage and transforms, the velocity and everything else in FP64.  Scalars (explicit
+diffusion, x and y periodic, unforced; the first one buoyant when the case says so)
+are advanced by the forward steps, and their adjoints carried by the backward ones
+(cans_rk3_substep_adjoint_scal); the gradient is taken with respect to the velocity.
 
 ``kinetic_energy_after_steps(sim, u0, v0, w0, dt, nsteps, every)`` runs nsteps fused
 RK3 steps (cans_rk3_substep) from the interior velocity (u0, v0, w0) with zero
@@ -56,7 +59,7 @@ def _addr(f, buf):
 
 class _KineticEnergyAfterSteps(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, u0, v0, w0, sim, dt, nsteps, every):
+    def forward(ctx, u0, v0, w0, sim, dt, nsteps, every, s0):
         if sim.ctx.precision not in (LB.CANS_FP64, LB.CANS_MIXED):
             raise LB.CansError("the adjoint needs a CANS_FP64 or CANS_MIXED simulation")
         for f, a in zip((sim.u, sim.v, sim.w), (u0, v0, w0)):
@@ -64,11 +67,16 @@ class _KineticEnergyAfterSteps(torch.autograd.Function):
             _interior(f).copy_(a)
         for f in (sim.p, sim.pp):
             f.buf.zero_()
+        for i, f in enumerate(sim.s):
+            f.buf.zero_()
+            if s0 is not None:
+                _interior(f).copy_(s0[i])
         sim.init_bounds()
+        fields = (sim.u, sim.v, sim.w, sim.p) + tuple(sim.s)
         ckpt = {}
         for s in range(nsteps):
             if s % every == 0:
-                ckpt[s] = [f.buf.clone() for f in (sim.u, sim.v, sim.w, sim.p)]
+                ckpt[s] = [f.buf.clone() for f in fields]
             sim.rk3_step(dt)
         fin = [_interior(f).clone() for f in (sim.u, sim.v, sim.w)]
         _clear_wall_faces(sim, fin)
@@ -86,7 +94,12 @@ class _KineticEnergyAfterSteps(torch.autograd.Function):
         pp = sim.ctx.field(kind=LB.CANS_FIELD_PRESSURE)
         for f, m, x in zip(ub, wts, ctx.fin):  # dE/du = m u (zero on the wall faces: fin is)
             _interior(f).copy_(m * x * gE)
-        fields = (sim.u, sim.v, sim.w, sim.p)
+        ns = len(sim.s)
+        # the scalars' adjoints (zero at the end: the loss reads the velocity only), FP64 fields
+        sb, sbo, fsb, fsbo = ([sim.ctx.field() for _ in range(ns)] for _ in range(4))
+        for f in sb + fsb:
+            f.buf.zero_()
+        fields = (sim.u, sim.v, sim.w, sim.p) + tuple(sim.s)
         for s0 in sorted(ctx.ckpt, reverse=True):
             for f, b in zip(fields, ctx.ckpt[s0]):
                 f.buf.copy_(b)
@@ -94,20 +107,26 @@ class _KineticEnergyAfterSteps(torch.autograd.Function):
             for _ in range(s0, min(s0 + every, nsteps)):
                 starts = []
                 for irk in (1, 2, 3):
-                    starts.append([f.buf.clone() for f in (sim.u, sim.v, sim.w)])
+                    starts.append([f.buf.clone() for f in (sim.u, sim.v, sim.w) + tuple(sim.s)])
                     sim.rk3_substep(irk, dt)
                 steps.append(starts)
             for starts in reversed(steps):
                 for irk in (3, 2, 1):
-                    q = [_addr(f, b) for f, b in zip((sim.u, sim.v, sim.w), starts[irk - 1])]
-                    sim.rk3_substep_adjoint(irk, dt, q, ub, fb, ubo, fbo, pp)
+                    st = starts[irk - 1]
+                    q = [_addr(f, b) for f, b in zip((sim.u, sim.v, sim.w), st[:3])]
+                    scal = None
+                    if ns:
+                        scal = ([_addr(f, b) for f, b in zip(sim.s, st[3:])], sb, fsb, sbo, fsbo)
+                    sim.rk3_substep_adjoint(irk, dt, q, ub, fb, ubo, fbo, pp, scal)
                     ub, ubo, fb, fbo = ubo, ub, fbo, fb
+                    sb, sbo, fsb, fsbo = sbo, sb, fsbo, fsb
             del steps
         grads = [_interior(f).clone() for f in ub]
         _clear_wall_faces(sim, grads)
-        return grads[0], grads[1], grads[2], None, None, None, None
+        return grads[0], grads[1], grads[2], None, None, None, None, None
 
 
-def kinetic_energy_after_steps(sim, u0, v0, w0, dt, nsteps, every=4):
-    """Volume-averaged kinetic energy after nsteps RK3 steps from (u0, v0, w0); differentiable in them."""
-    return _KineticEnergyAfterSteps.apply(u0, v0, w0, sim, dt, nsteps, every)
+def kinetic_energy_after_steps(sim, u0, v0, w0, dt, nsteps, every=4, s0=None):
+    """Volume-averaged kinetic energy after nsteps RK3 steps from (u0, v0, w0) (and the scalars s0, a
+    sequence of tensors, zero if None); differentiable in the velocity."""
+    return _KineticEnergyAfterSteps.apply(u0, v0, w0, sim, dt, nsteps, every, s0)

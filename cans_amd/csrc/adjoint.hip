@@ -345,6 +345,143 @@ int launch_adj_lap(cans_ctx* c, const double* const* a, double* const* out, doub
   return CANS_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Scalars (explicit diffusion; x and y periodic, z periodic or walled).  The
+// scalar right-hand side of scal.f90:41-98 is
+//   S(x) = -sum_d H_d(k) [F_d(x) - F_d(x - e_d)] + alpha L s,  F_d(x) = (s(x + e_d) + s(x))/2 q_d(x),
+// H = dxi, dyi, dzfi(k); with R_d(x) = H_d(k_x) sigma(x) - H_d(k_x+e_d) sigma(x + e_d) its
+// transposed Jacobians are
+//   (J_s^T sigma)(y)   = -1/2 sum_d [q_d(y - e_d) R_d(y - e_d) + q_d(y) R_d(y)] + alpha L^T sigma(y),
+//   (J_q^T sigma)_d(y) = -1/2 (s(y + e_d) + s(y)) R_d(y).
+// k_scal_jt forms sigma = f1 sh + fsb as it reads it (zero off the scalar's degrees of
+// freedom), writes sbo = sh + J_s^T sigma (with walls also on the ghost planes, which
+// k_scal_fold folds), fsbo = f2 sh, and adds J_q^T sigma to the velocity adjoint at the
+// velocity's degrees of freedom.  One thread per cell.
+struct ScalJtArgs {
+  Layout L;
+  const double* q[3];  // the velocity the forward substep started from (halo'd, ghosts set)
+  const double* s;     // the scalar it started from (halo'd, ghosts set)
+  const double* sh;    // the updated scalar's adjoint (halo'd, periodic ghosts set)
+  const double* fsb;   // the stored scalar right-hand side's adjoint (halo'd, periodic ghosts set)
+  double *sbo, *fsbo;
+  double* ubo[3];
+  const double *dzci, *dzfi;  // (0:n3+1)
+  double dxi, dyi, alpha, f1, f2;
+  int zwall;
+};
+
+__global__ void k_scal_jt(ScalJtArgs a) {
+  const Layout& L = a.L;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1;
+  const int k = (int)blockIdx.z + (a.zwall ? 0 : 1);
+  if (i > L.n1) return;
+  const int n3 = L.n3;
+  auto kin = [&](int kk) { return kk >= 0 && kk <= n3 + 1; };
+  auto dof = [&](int kk) { return !a.zwall || (kk >= 1 && kk <= n3); };
+  auto sig = [&](int di, int dj, int kk) -> double {
+    if (!kin(kk) || !dof(kk)) return 0.;
+    const long o = L.at(i + di, j + dj, kk);
+    return a.f1 * a.sh[o] + a.fsb[o];
+  };
+  auto Q = [&](int d, int di, int dj, int kk) -> double { return kin(kk) ? a.q[d][L.at(i + di, j + dj, kk)] : 0.; };
+  auto zf = [&](int kk) { return kin(kk) ? a.dzfi[kk] : 0.; };
+  auto zc = [&](int kk) { return kin(kk) ? a.dzci[kk] : 0.; };
+  auto H = [&](int d, int kk) { return d == 0 ? a.dxi : d == 1 ? a.dyi : zf(kk); };
+  // R_d at the cell (i + di, j + dj, kk)
+  auto R = [&](int d, int di, int dj, int kk) {
+    const int ex = d == 0, ey = d == 1, ez = d == 2;
+    return H(d, kk) * sig(di, dj, kk) - H(d, kk + ez) * sig(di + ex, dj + ey, kk + ez);
+  };
+  double js = 0.;
+  double rd[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const int ex = d == 0, ey = d == 1, ez = d == 2;
+    rd[d] = R(d, 0, 0, k);
+    js -= 0.5 * (Q(d, -ex, -ey, k - ez) * R(d, -ex, -ey, k - ez) + Q(d, 0, 0, k) * rd[d]);
+  }
+  const double s0 = sig(0, 0, k);
+  const double lxy = a.dxi * a.dxi * (sig(1, 0, k) - 2. * s0 + sig(-1, 0, k)) +
+                     a.dyi * a.dyi * (sig(0, 1, k) - 2. * s0 + sig(0, -1, k));
+  // L = dzfi(k) [dzci(k) d+ - dzci(k-1) d-]: L^T sigma = dzci(k) d+(dzfi sigma) - dzci(k-1) d-(dzfi sigma)
+  const double gp = zf(k + 1) * sig(0, 0, k + 1), gc = zf(k) * s0, gm = zf(k - 1) * sig(0, 0, k - 1);
+  js += a.alpha * (lxy + zc(k) * (gp - gc) - zc(k - 1) * (gc - gm));
+  const long o = L.at(i, j, k);
+  const bool in = k >= 1 && k <= n3;
+  const double base = in ? a.sh[o] : 0.;
+  a.sbo[o] = base + js;
+  if (!in) return;
+  a.fsbo[o] = a.f2 * base;
+  const double sc = a.s[o];
+  const double sn[3] = {a.s[o + 1], a.s[o + L.sy], a.s[o + L.sz]};
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+    if (!(d == 2 && a.zwall && k == n3)) a.ubo[d][o] += -0.5 * (sn[d] + sc) * rd[d];  // w's wall face: none
+}
+
+// s(1) += s0 s(0), s(n3) += s1 s(n3 + 1) (the scalar's z ghosts at walls: -1 Dirichlet, +1 Neumann)
+__global__ void k_scal_fold(Layout L, double* sb, double s0, double s1) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1;
+  if (i > L.n1) return;
+  const long b = L.at(i, j, 0), t = L.at(i, j, L.n3);
+  sb[b + L.sz] += s0 * sb[b];
+  sb[t] += s1 * sb[t + L.sz];
+}
+
+// sh += the Boussinesq term's transpose (rk.f90:163-177: u_c -= coef_c (s(x + e_c) + s(x))):
+//   sh(y) -= sum_c coef_c (wb_c(y - e_c) + wb_c(y)),
+// wb read as zero off the velocity's degrees of freedom (w's wall faces when z is walled).
+__global__ void k_scal_buoy_t(Layout L, double* sh, const double* wu, const double* wv, const double* ww,
+                              double c0, double c1, double c2, int zwall) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
+  if (i > L.n1) return;
+  const long o = L.at(i, j, k);
+  const double wm = (zwall && k == 1) ? 0. : ww[o - L.sz];
+  const double w0 = (zwall && k == L.n3) ? 0. : ww[o];
+  sh[o] -= c0 * (wu[o - 1] + wu[o]) + c1 * (wv[o - L.sy] + wv[o]) + c2 * (wm + w0);
+}
+
+int launch_scal_adjoint(cans_ctx* c, const ScalAdj& sa) {
+  const Layout& L = c->L;
+  if (sa.buoy) {
+    k_scal_buoy_t<<<dim3(cdiv(L.n1, 64), L.n2, L.n3), 64, 0, c->stream>>>(L, sa.sh, sa.wb[0], sa.wb[1], sa.wb[2],
+                                                                          sa.coef[0], sa.coef[1], sa.coef[2],
+                                                                          sa.zwall);
+    CANS_LAUNCH_CHECK("k_scal_buoy_t");
+  }
+  return CANS_OK;
+}
+
+int launch_scal_jt(cans_ctx* c, const ScalAdj& sa) {
+  const Layout& L = c->L;
+  ScalJtArgs a;
+  a.L = L;
+  for (int d = 0; d < 3; d++) {
+    a.q[d] = sa.q[d];
+    a.ubo[d] = sa.ubo[d];
+  }
+  a.s = sa.s;
+  a.sh = sa.sh;
+  a.fsb = sa.fsb;
+  a.sbo = sa.sbo;
+  a.fsbo = sa.fsbo;
+  a.dzci = sa.dzci;
+  a.dzfi = sa.dzfi;
+  a.dxi = sa.dxi;
+  a.dyi = sa.dyi;
+  a.alpha = sa.alpha;
+  a.f1 = sa.f1;
+  a.f2 = sa.f2;
+  a.zwall = sa.zwall;
+  k_scal_jt<<<dim3(cdiv(L.n1, 64), L.n2, L.n3 + (sa.zwall ? 2 : 0)), 64, 0, c->stream>>>(a);
+  CANS_LAUNCH_CHECK("k_scal_jt");
+  if (sa.zwall) {
+    k_scal_fold<<<dim3(cdiv(L.n1, 64), L.n2), 64, 0, c->stream>>>(L, sa.sbo, sa.sign[0], sa.sign[1]);
+    CANS_LAUNCH_CHECK("k_scal_fold");
+  }
+  return CANS_OK;
+}
+
 static JtArgs jt_args(cans_ctx* c, const double* const* q, const double* const* g, const double* const* fb,
                       const double* const* base, double* const* out, double* const* fbo, const double* h,
                       double visc, double f1, double f2, const double* dzci, const double* dzfi) {

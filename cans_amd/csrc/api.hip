@@ -877,7 +877,7 @@ int adjoint_walls(const cans_step_params* s) {
   return wm;
 }
 
-int adjoint_scope(cans_ctx* c, const cans_step_params* s, const char* who) {
+int adjoint_scope(cans_ctx* c, const cans_step_params* s, const char* who, bool scalars = false) {
   const std::string w(who);
   if (c->precision != CANS_FP64 && c->precision != CANS_MIXED)
     return fail(CANS_EUNSUPPORTED, w + ": the adjoint needs a CANS_FP64 or CANS_MIXED context");
@@ -885,12 +885,29 @@ int adjoint_scope(cans_ctx* c, const cans_step_params* s, const char* who) {
     if (c->imp) return fail(CANS_EUNSUPPORTED, w + ": the transposed Jacobian covers explicit diffusion");
     return CANS_OK;
   }
+  if (s->nscal && !scalars)
+    return fail(CANS_EINVAL, w + ": a step with scalars runs backwards through cans_rk3_substep_adjoint_scal");
+  if (s->nscal) {
+    // scalars: explicit diffusion, x and y periodic, unforced; z periodic or walled (Dirichlet or Neumann)
+    if (c->imp) return fail(CANS_EUNSUPPORTED, w + ": with scalars the adjoint covers explicit diffusion");
+    if (adjoint_walls(s) & 3) return fail(CANS_EUNSUPPORTED, w + ": with scalars the adjoint covers periodic x and y");
+    if (!s->scal || s->nscal > CANS_MAX_SCALARS) return fail(CANS_EINVAL, w + ": bad scalar list");
+    const bool zw = adjoint_walls(s) & 4;
+    for (int i = 0; i < s->nscal; i++) {
+      const cans_scalar& q = s->scal[i];
+      if (q.is_forced) return fail(CANS_EUNSUPPORTED, w + ": the adjoint covers unforced scalars");
+      for (int e = 0; e < 4; e++)
+        if (q.cbc[e] != 'P') return fail(CANS_EUNSUPPORTED, w + ": a scalar must be periodic in x and y");
+      for (int e = 4; e < 6; e++)
+        if (zw ? (q.cbc[e] != 'D' && q.cbc[e] != 'N') : q.cbc[e] != 'P')
+          return fail(CANS_EUNSUPPORTED, w + ": a scalar's z boundaries must match the walls");
+    }
+  }
   // implicit diffusion: the 3-D form on tri-periodic uniform grids, where the Helmholtz operator
   // commutes with the gradient, so that the pressure still drops out (P H^-1 G p = 0)
   if (c->imp == 2) return fail(CANS_EUNSUPPORTED, w + ": the adjoint covers 3-D implicit diffusion, not z-only");
   if (c->imp && adjoint_walls(s))
     return fail(CANS_EUNSUPPORTED, w + ": with implicit diffusion the adjoint covers tri-periodic grids");
-  if (s->nscal) return fail(CANS_EUNSUPPORTED, w + ": the adjoint covers flows without scalars");
   const int n[3] = {c->L.n1, c->L.n2, c->L.n3};
   const int wm = adjoint_walls(s);
   for (int d = 0; d < 3; d++) {
@@ -951,11 +968,10 @@ extern "C" int cans_mom_jt(cans_ctx* c, const int* n, const double* dli, double 
   return launch_mom_jt(c, q, g, b, o, dli, visc);
 }
 
-extern "C" int cans_rk3_substep_adjoint(cans_ctx* c, const cans_step_params* s, int irk, double dt, const void* u,
-                                        const void* v, const void* w, void* ub, void* vb, void* wb, void* fu,
-                                        void* fv, void* fw, void* ubo, void* vbo, void* wbo, void* fuo, void* fvo,
-                                        void* fwo, void* pp) {
-  CHECK_CTX("cans_rk3_substep_adjoint");
+static int substep_adjoint_vel(cans_ctx* c, const cans_step_params* s, int irk, double dt, const void* u,
+                               const void* v, const void* w, void* ub, void* vb, void* wb, void* fu, void* fv,
+                               void* fw, void* ubo, void* vbo, void* wbo, void* fuo, void* fvo, void* fwo, void* pp,
+                               bool scalars) {
   if (!s || irk < 1 || irk > 3) return fail(CANS_EINVAL, "cans_rk3_substep_adjoint: bad argument");
   const void* all[16] = {u, v, w, ub, vb, wb, fu, fv, fw, ubo, vbo, wbo, fuo, fvo, fwo, pp};
   for (int q = 0; q < 16; q++) {
@@ -966,7 +982,7 @@ extern "C" int cans_rk3_substep_adjoint(cans_ctx* c, const cans_step_params* s, 
   if (!s->plan || !s->lambdaxy_d || !s->a_d || !s->b_d || !s->c_d || !s->dzci_d || !s->dzfi_d || !s->dzc_h ||
       !s->dzf_h)
     return fail(CANS_EINVAL, "cans_rk3_substep_adjoint: null parameter array");
-  int rc = adjoint_scope(c, s, "cans_rk3_substep_adjoint");
+  int rc = adjoint_scope(c, s, "cans_rk3_substep_adjoint", scalars);
   if (rc) return rc;
   const double* rkpar = RKCOEFF[irk - 1];
   const double f1 = rkpar[0] * dt, f2 = rkpar[1] * dt;
@@ -1046,6 +1062,92 @@ extern "C" int cans_rk3_substep_adjoint(cans_ctx* c, const cans_step_params* s, 
     for (int d = 0; d < 3; d++)
       for (int e = 0; e < 2; e++) aw.sign[m][d][e] = s->cbcvel[6 * m + 2 * d + e] == 'N' ? 1. : -1.;
   return launch_mom_jt_adj(c, q, a, fb, o, fo, h, s->visc, f1, f2, &aw);
+}
+
+extern "C" int cans_rk3_substep_adjoint(cans_ctx* c, const cans_step_params* s, int irk, double dt, const void* u,
+                                        const void* v, const void* w, void* ub, void* vb, void* wb, void* fu,
+                                        void* fv, void* fw, void* ubo, void* vbo, void* wbo, void* fuo, void* fvo,
+                                        void* fwo, void* pp) {
+  CHECK_CTX("cans_rk3_substep_adjoint");
+  return substep_adjoint_vel(c, s, irk, dt, u, v, w, ub, vb, wb, fu, fv, fw, ubo, vbo, wbo, fuo, fvo, fwo, pp, false);
+}
+
+// The adjoint substep with scalars (explicit diffusion; x and y periodic; unforced scalars): the
+// velocity part as cans_rk3_substep_adjoint, then per scalar i (main.f90:440-450: the scalars
+// advance first, and the velocity's rk reads the first one's new value in the Boussinesq term,
+// rk.f90:163-177) sh_i = sb_i (+ that term's transpose of wb for i = 0), sbo_i = sh_i + J_s^T
+// (f1 sh_i + fsb_i), fsbo_i = f2 sh_i, ubo += J_q^T (f1 sh_i + fsb_i).
+extern "C" int cans_rk3_substep_adjoint_scal(cans_ctx* c, const cans_step_params* s, int irk, double dt,
+                                             const void* u, const void* v, const void* w, void* ub, void* vb,
+                                             void* wb, void* fu, void* fv, void* fw, void* ubo, void* vbo, void* wbo,
+                                             void* fuo, void* fvo, void* fwo, void* pp, const void* const* sc,
+                                             void* const* sb, void* const* fsb, void* const* sbo,
+                                             void* const* fsbo) {
+  CHECK_CTX("cans_rk3_substep_adjoint_scal");
+  if (!s || !s->nscal || !sc || !sb || !fsb || !sbo || !fsbo)
+    return fail(CANS_EINVAL, "cans_rk3_substep_adjoint_scal: null argument or no scalars");
+  if (s->nscal > CANS_MAX_SCALARS) return fail(CANS_EINVAL, "cans_rk3_substep_adjoint_scal: too many scalars");
+  {
+    std::vector<const void*> all = {u, v, w, ub, vb, wb, fu, fv, fw, ubo, vbo, wbo, fuo, fvo, fwo, pp};
+    for (int i = 0; i < s->nscal; i++) {
+      all.push_back(sc[i]);
+      all.push_back(sb[i]);
+      all.push_back(fsb[i]);
+      all.push_back(sbo[i]);
+      all.push_back(fsbo[i]);
+    }
+    for (size_t q = 16; q < all.size(); q++) {
+      if (!all[q]) return fail(CANS_EINVAL, "cans_rk3_substep_adjoint_scal: null field");
+      for (size_t r = 0; r < q; r++)
+        if (all[q] == all[r]) return fail(CANS_EINVAL, "cans_rk3_substep_adjoint_scal: every field must be distinct");
+    }
+  }
+  int rc = substep_adjoint_vel(c, s, irk, dt, u, v, w, ub, vb, wb, fu, fv, fw, ubo, vbo, wbo, fuo, fvo, fwo, pp, true);
+  if (rc) return rc;
+  const double* rkpar = RKCOEFF[irk - 1];
+  const double f1 = rkpar[0] * dt, f2 = rkpar[1] * dt;
+  const bool zwall = adjoint_walls(s) & 4;
+  const double zero6[6] = {};
+  ScalAdj sa;
+  sa.q[0] = (const double*)u;
+  sa.q[1] = (const double*)v;
+  sa.q[2] = (const double*)w;
+  sa.ubo[0] = (double*)ubo;
+  sa.ubo[1] = (double*)vbo;
+  sa.ubo[2] = (double*)wbo;
+  sa.wb[0] = (const double*)ub;  // ub..wb hold wb = Fr'^T P^T ub with its ghosts (above)
+  sa.wb[1] = (const double*)vb;
+  sa.wb[2] = (const double*)wb;
+  sa.dzci = s->dzci_d;
+  sa.dzfi = s->dzfi_d;
+  sa.dxi = s->dli[0];
+  sa.dyi = s->dli[1];
+  sa.f1 = f1;
+  sa.f2 = f2;
+  sa.zwall = zwall;
+  for (int cc = 0; cc < 3; cc++) sa.coef[cc] = (f1 + f2) * s->gacc[cc] * s->beta * 0.5;
+  for (int i = 0; i < s->nscal; i++) {
+    const cans_scalar& qs = s->scal[i];
+    sa.s = (const double*)sc[i];
+    sa.sh = (double*)sb[i];
+    sa.fsb = (const double*)fsb[i];
+    sa.sbo = (double*)sbo[i];
+    sa.fsbo = (double*)fsbo[i];
+    sa.alpha = qs.alpha;
+    sa.sign[0] = qs.cbc[4] == 'N' ? 1. : -1.;
+    sa.sign[1] = qs.cbc[5] == 'N' ? 1. : -1.;
+    sa.buoy = i == 0 && s->is_buoyancy;
+    if ((rc = launch_scal_adjoint(c, sa))) return rc;
+    // periodic images of sh and fsb (the wall ghosts are read as zero)
+    if ((rc = plan_boundp(c, qs.cbc, zero6, s->nb, s->is_bound, s->dl, s->dzc_h, sb[i], CANS_HALO_ABSENT,
+                          (int)sizeof(double))))
+      return rc;
+    if ((rc = plan_boundp(c, qs.cbc, zero6, s->nb, s->is_bound, s->dl, s->dzc_h, fsb[i], CANS_HALO_ABSENT,
+                          (int)sizeof(double))))
+      return rc;
+    if ((rc = launch_scal_jt(c, sa))) return rc;
+  }
+  return CANS_OK;
 }
 
 // ---------------------------------------------------------------------------

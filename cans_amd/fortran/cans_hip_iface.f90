@@ -461,6 +461,18 @@ module cans_hip
       integer(c_int), value :: irk
       real(c_double), value :: dt
     end function
+    ! the same with scalars: s(i) the scalars the forward substep started from, (sb, fsb) -> (sbo, fsbo)
+    ! their adjoints and those of their stored right-hand sides (arrays of nscal device addresses)
+    integer(c_int) function cans_rk3_substep_adjoint_scal(ctx, prm, irk, dt, u, v, w, ub, vb, wb, fu, fv, fw, &
+                                                          ubo, vbo, wbo, fuo, fvo, fwo, pp, s, sb, fsb, sbo, fsbo) &
+                   bind(C, name='cans_rk3_substep_adjoint_scal')
+      import :: c_int, c_ptr, c_double, cans_step_params
+      type(c_ptr), value :: ctx, u, v, w, ub, vb, wb, fu, fv, fw, ubo, vbo, wbo, fuo, fvo, fwo, pp
+      type(cans_step_params), intent(in) :: prm
+      integer(c_int), value :: irk
+      real(c_double), value :: dt
+      type(c_ptr), intent(in) :: s(*), sb(*), fsb(*), sbo(*), fsbo(*)
+    end function
   end interface
 contains
   subroutine cans_check(istat, what)

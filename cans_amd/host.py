@@ -626,16 +626,27 @@ class Simulation:
         for irk in (1, 2, 3):
             self.rk3_substep(irk, dt)
 
-    def rk3_substep_adjoint(self, irk, dt, q, ub, fb, ubo, fbo, pp):
+    def rk3_substep_adjoint(self, irk, dt, q, ub, fb, ubo, fbo, pp, scal=None):
         """cans_rk3_substep_adjoint: q = three device addresses of the velocity the forward
         substep irk started from; ub, fb: DeviceFields holding the adjoints of the substep's
         output velocity and stored right-hand side (ub is overwritten); ubo, fbo receive those of
-        its input velocity and the right-hand side it read; pp: scratch DeviceField."""
-        LB.check(LB.lib().cans_rk3_substep_adjoint(self.ctx.h, C.byref(self.prm), irk, dt,
-                                                   *[C.c_void_p(x) for x in q],
-                                                   *[C.c_void_p(f.ptr) for f in ub + fb + ubo + fbo],
-                                                   C.c_void_p(pp.ptr)),
-                 "cans_rk3_substep_adjoint")
+        its input velocity and the right-hand side it read; pp: scratch DeviceField.  With scalars
+        (cans_rk3_substep_adjoint_scal) scal = (s, sb, fsb, sbo, fsbo): the device addresses of the
+        scalars the substep started from, and per scalar the DeviceFields of the adjoints of its
+        output and stored right-hand side (sb overwritten) and of those it read."""
+        args = [C.c_void_p(x) for x in q] + [C.c_void_p(f.ptr) for f in ub + fb + ubo + fbo] + [C.c_void_p(pp.ptr)]
+        if not scal:
+            LB.check(LB.lib().cans_rk3_substep_adjoint(self.ctx.h, C.byref(self.prm), irk, dt, *args),
+                     "cans_rk3_substep_adjoint")
+            return
+        n = len(self.s)
+        s_addr, sb, fsb, sbo, fsbo = scal
+        arrs = [(C.c_void_p * n)(*[C.c_void_p(x) for x in s_addr])]
+        for fl in (sb, fsb, sbo, fsbo):
+            arrs.append((C.c_void_p * n)(*[C.c_void_p(f.ptr) for f in fl]))
+        LB.check(LB.lib().cans_rk3_substep_adjoint_scal(self.ctx.h, C.byref(self.prm), irk, dt, *args,
+                                                        *[C.cast(a, C.POINTER(C.c_void_p)) for a in arrs]),
+                 "cans_rk3_substep_adjoint_scal")
 
     def capture_step(self, dt, nsteps=1):
         """Record nsteps fused RK3 steps of size dt into a HIP graph (torch.cuda.CUDAGraph)

@@ -433,6 +433,20 @@ int cans_rk3_substep_adjoint(cans_ctx *ctx, const cans_step_params *prm, int irk
                              void *vb_d, void *wb_d, void *fu_d, void *fv_d, void *fw_d,
                              void *ubo_d, void *vbo_d, void *wbo_d, void *fuo_d, void *fvo_d,
                              void *fwo_d, void *pp_d);
+/* The same with scalars (prm->nscal > 0; cans_rk3_substep_adjoint refuses them): explicit
+ * diffusion, x and y periodic, unforced scalars with z periodic or walled (Dirichlet or
+ * Neumann).  s_d[i] is scalar i as the forward substep started from (halo'd, ghosts
+ * set); sb_d[i] and fsb_d[i] hold the adjoints of its output and of the right-hand side
+ * it stored (sb_d[i] is overwritten); sbo_d[i] and fsbo_d[i] receive those of its input
+ * and of the right-hand side it read.  The first scalar's adjoint also takes the
+ * Boussinesq term's transpose (rk.f90:163-177) when prm->is_buoyancy.  Every field is
+ * distinct. */
+int cans_rk3_substep_adjoint_scal(cans_ctx *ctx, const cans_step_params *prm, int irk, double dt,
+                                  const void *u_d, const void *v_d, const void *w_d, void *ub_d,
+                                  void *vb_d, void *wb_d, void *fu_d, void *fv_d, void *fw_d,
+                                  void *ubo_d, void *vbo_d, void *wbo_d, void *fuo_d, void *fvo_d,
+                                  void *fwo_d, void *pp_d, const void *const *s_d, void *const *sb_d,
+                                  void *const *fsb_d, void *const *sbo_d, void *const *fsbo_d);
 
 #ifdef __cplusplus
 }

@@ -105,7 +105,14 @@ def check_case(case):
     if case.is_forced[2] or any(case.is_forced[c] and c in wall_dirs(case) for c in (0, 1)):
         raise ValueError("forcing covers u and v along periodic directions")
     if case.scalars:
-        raise ValueError("the adjoint covers flows without scalars")
+        if case.is_impdiff or walls(case) and wall_dirs(case) != [2]:
+            raise ValueError("scalars: explicit diffusion, periodic x and y")
+        for sc in case.scalars:
+            if sc["is_forced"] or sc["cbc"][:4] != "PPPP" or sc["cbc"][4:6] != ("PP" if 2 not in wall_dirs(case)
+                                                                                else sc["cbc"][4:6]):
+                raise ValueError("scalars: unforced, periodic along the periodic directions")
+            if 2 in wall_dirs(case) and any(ch not in "DN" for ch in sc["cbc"][4:6]):
+                raise ValueError("scalars: Dirichlet or Neumann at the z walls")
     if case.is_impdiff and (walls(case) or case.is_impdiff_1d):
         raise ValueError("implicit diffusion: the adjoint covers the 3-D form on tri-periodic uniform grids")
 
@@ -316,6 +323,82 @@ def mom_jt_walls(case, qh, g, fold=True, swap_rdz=False):
     return res
 
 
+def _zext(case, a, pad_ghosts):
+    """Along z: a halo'd array's planes 0..n3+1 plus a zero plane each side (walls), or its interior
+    planes (periodic z); x and y interior (periodic)."""
+    if 2 in wall_dirs(case):
+        b = a[1:-1, 1:-1, :] if pad_ghosts else np.pad(a, ((0, 0), (0, 0), (1, 1)))
+        return np.pad(b, ((0, 0), (0, 0), (1, 1)))
+    return a[1:-1, 1:-1, 1:-1] if pad_ghosts else a
+
+
+def _z_tables(case):
+    n3 = case.n[2]
+    if 2 in wall_dirs(case):
+        zc, zf = np.zeros(n3 + 4), np.zeros(n3 + 4)
+        zc[1:n3 + 3], zf[1:n3 + 3] = case.dzci, case.dzfi
+    else:
+        zc, zf = np.asarray(case.dzci[1:n3 + 1], float), np.asarray(case.dzfi[1:n3 + 1], float)
+    return zc[None, None, :], zf[None, None, :]
+
+
+def _fold_scalar_z(case, out, cbc):
+    """B^T of a cell-centred scalar's z ghosts (walls): s(0) = 2 sb - s(1) (D, -1) or s(1) - dzc sb (N, +1)."""
+    n3 = case.n[2]
+    if 2 not in wall_dirs(case):
+        return np.asfortranarray(out)
+    res = np.asfortranarray(out[:, :, 2:n3 + 2])
+    res[:, :, 0] += (-1. if cbc[4] == "D" else 1.) * out[:, :, 1]
+    res[:, :, n3 - 1] += (-1. if cbc[5] == "D" else 1.) * out[:, :, n3 + 2]
+    return res
+
+
+def scal_jt(case, qh, sh, sigma, alpha, cbc):
+    """Transposed Jacobians of the scalar right-hand side of scal.f90:41-98 (explicit diffusion),
+        S(x) = -sum_d H_d(k) [F_d(x) - F_d(x - e_d)] + alpha L s,  F_d(x) = (s(x + e_d) + s(x))/2 q_d(x),
+    H = dxi, dyi, dzfi(k): with R_d(x) = H_d(k_x) sigma(x) - H_d(k_x+e_d) sigma(x + e_d),
+        (J_s^T sigma)(y) = -1/2 sum_d [q_d(y - e_d) R_d(y - e_d) + q_d(y) R_d(y)] + alpha L^T sigma(y),
+        (J_q^T sigma)_d(y) = -1/2 (s(y + e_d) + s(y)) R_d(y).
+    qh, sh: the halo'd velocity and scalar (ghosts as the forward set them); sigma: interior array.
+    Returns (J_s^T sigma folded by the scalar's ghost map, [J_q^T sigma] on the velocity's degrees of
+    freedom)."""
+    Q = [_zext(case, x, True) for x in qh]
+    S = _zext(case, sh, True)
+    G = _zext(case, sigma, False)
+    zc_, zf_ = _z_tables(case)
+    H = [case.dli[0], case.dli[1], zf_]
+    out = np.zeros_like(G)
+    qt = []
+    for d in range(3):
+        hs = H[d] * G
+        R = hs - _sh(hs, d, 1)
+        out -= 0.5 * (_sh(Q[d] * R, d, -1) + Q[d] * R)
+        qt.append(-0.5 * (_sh(S, d, 1) + S) * R)
+    xy = sum(case.dli[d] ** 2 * (_sh(G, d, 1) - 2 * G + _sh(G, d, -1)) for d in range(2))
+    gh = zf_ * G
+    z = zc_ * (_sh(gh, 2, 1) - gh) - _sh(zc_, 2, -1) * (gh - _sh(gh, 2, -1))
+    out += alpha * (xy + z)
+    n3 = case.n[2]
+    isl = (slice(None), slice(None), slice(2, n3 + 2)) if 2 in wall_dirs(case) else (slice(None),) * 3
+    qres = [np.asfortranarray(x[isl]) for x in qt]
+    _clear_wall_faces(case, qres)
+    return _fold_scalar_z(case, out, cbc), qres
+
+
+def buoyancy_t(case, wb, cbc, coef):
+    """Transpose of the Boussinesq term u_c -= coef_c (s(x + e_c) + s(x)) (rk.f90:163-177, coef_c =
+    f12 gacc_c beta / 2) applied to wb, folded by the scalar's ghost map: sum_c -coef_c (wb_c(y - e_c) + wb_c(y)),
+    wb zero off the velocity's degrees of freedom."""
+    W = [_zext(case, x, False) for x in wb]
+    for x in W[2:3]:
+        if 2 in wall_dirs(case):
+            x[:, :, case.n[2] + 1] = 0.
+    out = np.zeros_like(W[0])
+    for c in range(3):
+        out -= coef[c] * (_sh(W[c], c, -1) + W[c])
+    return _fold_scalar_z(case, out, cbc)
+
+
 def kinetic_energy(case, u, v, w):
     """Volume-averaged kinetic energy of the interior degrees of freedom (dzf- and dzc-weighted)."""
     wts = volume_weights(case)
@@ -330,24 +413,32 @@ def ke_grad(case, u, v, w):
     return [np.asfortranarray(m * q / vol) for m, q in zip(wts, (u, v, w))]
 
 
-def _state(case, u0, v0, w0):
+def _state(case, u0, v0, w0, scal0=()):
     st = case.new_state()
     for k, q in zip("uvw", (u0, v0, w0)):
         st[k][1:-1, 1:-1, 1:-1] = q
+    for i, q in enumerate(scal0):
+        st["s%d" % i][1:-1, 1:-1, 1:-1] = q
     case.init_bounds(st)
     return st
 
 
 def _step_recording(case, st, dt):
-    """One RK3 step (the oracle's rk3_step sequence) returning the halo'd velocity at the start of each
-    substep."""
+    """One RK3 step (the oracle's rk3_step sequence) returning, for each substep, the halo'd velocity and
+    scalars at its start."""
     u, v, w, p, pp = st["u"], st["v"], st["w"], st["p"], st["pp"]
     starts = []
     for irk in range(3):
-        starts.append([x.copy(order="F") for x in (u, v, w)])
+        starts.append(([x.copy(order="F") for x in (u, v, w)],
+                       [st["s%d" % i].copy(order="F") for i in range(len(case.scalars))]))
         rkpar = O.RKCOEFF[irk]
         dtrk = (rkpar[0] + rkpar[1]) * dt
-        case.rk(rkpar, dt, p, st["dudtrko"], st["dvdtrko"], st["dwdtrko"], u, v, w)
+        for i, sc in enumerate(case.scalars):  # main.f90:440-450
+            sf = st["s%d" % i]
+            fs = case.rk_scal(rkpar, dt, sc, u, v, w, sf, st["dsdtrko%d" % i])
+            case.bulk_forcing_s(sc["is_forced"], fs, sf)
+            case.boundp(sf, cbc=sc["cbc"], bc=sc["bc"])
+        case.rk(rkpar, dt, p, st["dudtrko"], st["dvdtrko"], st["dwdtrko"], u, v, w, st.get("s0"))
         alpha = 0.0
         if case.is_impdiff:  # main.f90:453-462
             alpha = -.5 * case.visc * dtrk
@@ -376,61 +467,101 @@ def helmholtz_inv(case, a, dtrk):
     return out
 
 
-def substep_adjoint(case, start, irk, dt, ub, fb):
-    """Adjoint of substep irk (0-based) at the halo'd starting velocity `start`: (ub, fb) -> (ub, fb).
+def substep_adjoint(case, start, irk, dt, ub, fb, sb=(), fsb=()):
+    """Adjoint of substep irk (0-based): start = the halo'd velocity (and, as (velocity, scalars), the
+    scalars) it started from; (ub, fb[, sb, fsb]) -> (ub, fb[, sb, fsb]), the adjoints of the velocity, its
+    stored right-hand side and, per scalar, of the scalar and its stored right-hand side.
 
     With implicit diffusion (3-D, tri-periodic uniform grid) the substep is
     u' = P H^-1 Fr(u + f1 A(u) + f2 Ao + (f1 + f2)/2 visc L u), Ao' = A(u), A the advection alone
     (rk.f90:156-186, 312-326), so its adjoint is hb = Fr'^T H^-1 P ub,
-    ub <- hb + J_A^T (f1 hb + fb) + (f1 + f2)/2 visc L hb, fb <- f2 hb."""
+    ub <- hb + J_A^T (f1 hb + fb) + (f1 + f2)/2 visc L hb, fb <- f2 hb.
+
+    Scalars (explicit diffusion, unforced; main.f90:440-450): each is advanced first,
+    s' = s + f1 S(u, s) + f2 So, So' = S(u, s), and the velocity's rk then reads the first one's new
+    value in the Boussinesq term -f12 gacc beta (s(x + e_c) + s(x))/2 (rk.f90:163-177).  So the first
+    scalar's adjoint gains that term's transpose of wb before the scalars' own adjoints
+    sh = sb (+ buoyancy^T wb), sb <- sh + J_s^T (f1 sh + fsb), fsb <- f2 sh, ub += J_u^T (f1 sh + fsb)."""
+    vel, scal = start if isinstance(start, tuple) else (start, [])
     f1, f2 = O.RKCOEFF[irk][0] * dt, O.RKCOEFF[irk][1] * dt
     if case.is_impdiff:
         hb = forcing_t(case, helmholtz_inv(case, project_t(case, ub), f1 + f2))
         g = [f1 * a + b for a, b in zip(hb, fb)]
-        jt = mom_jt(case, [interior(x) for x in start], g, visc=0.)
+        jt = mom_jt(case, [interior(x) for x in vel], g, visc=0.)
         half = 0.5 * (f1 + f2) * case.visc
         return [a + b + half * _lap(case, a) for a, b in zip(hb, jt)], [f2 * a for a in hb]
     wb = forcing_t(case, project_t(case, ub))
     g = [f1 * a + b for a, b in zip(wb, fb)]
     if walls(case):
-        jt = mom_jt_walls(case, start, g)
+        jt = mom_jt_walls(case, vel, g)
     else:
-        jt = mom_jt(case, [interior(x) for x in start], g)
-    return [a + b for a, b in zip(wb, jt)], [f2 * a for a in wb]
+        jt = mom_jt(case, [interior(x) for x in vel], g)
+    ub_new, fb_new = [a + b for a, b in zip(wb, jt)], [f2 * a for a in wb]
+    if not case.scalars:
+        return ub_new, fb_new
+    sh = [np.asarray(x, dtype=np.float64).copy() for x in sb]
+    if case.is_buoyancy:
+        coef = [(f1 + f2) * case.gacc[c] * case.beta * 0.5 for c in range(3)]
+        sh[0] = sh[0] + buoyancy_t(case, wb, case.scalars[0]["cbc"], coef)
+    sb_new, fsb_new = [], []
+    for i, sc in enumerate(case.scalars):
+        sigma = f1 * sh[i] + fsb[i]
+        js, jq = scal_jt(case, vel, scal[i], sigma, sc["alpha"], sc["cbc"])
+        sb_new.append(sh[i] + js)
+        fsb_new.append(f2 * sh[i])
+        ub_new = [a + b for a, b in zip(ub_new, jq)]
+    return ub_new, fb_new, sb_new, fsb_new
 
 
 def step_adjoint(case, starts, ub, dt):
-    """Adjoint of one step: ub (adjoint of the step's final velocity) -> adjoint of its initial velocity."""
+    """Adjoint of one step: ub (adjoint of the step's final velocity) -> adjoint of its initial velocity
+    (the scalars' adjoints start at zero: the loss reads the velocity only)."""
     fb = [np.zeros_like(x) for x in ub]
+    ns = len(case.scalars)
+    sb = [np.zeros_like(ub[0]) for _ in range(ns)]
+    fsb = [np.zeros_like(ub[0]) for _ in range(ns)]
     for irk in (2, 1, 0):
-        ub, fb = substep_adjoint(case, starts[irk], irk, dt, ub, fb)
-    return ub
+        r = substep_adjoint(case, starts[irk], irk, dt, ub, fb, sb, fsb)
+        ub, fb = r[0], r[1]
+        if ns:
+            sb, fsb = r[2], r[3]
+    return ub, sb
 
 
-def ke_after_steps(case, u0, v0, w0, dt, nsteps):
-    """Kinetic energy after nsteps RK3 steps from the interior velocity (u0, v0, w0), zero pressure."""
-    st = _state(case, u0, v0, w0)
+def ke_after_steps(case, u0, v0, w0, dt, nsteps, scal0=()):
+    """Kinetic energy after nsteps RK3 steps from the interior velocity (u0, v0, w0) (and scalars scal0),
+    zero pressure."""
+    st = _state(case, u0, v0, w0, scal0)
     for _ in range(nsteps):
         case.rk3_step(st, dt)
     return kinetic_energy(case, *(interior(st[k]) for k in "uvw"))
 
 
-def grad_ke_after_steps(case, u0, v0, w0, dt, nsteps, every=4):
-    """(E, dE/du0): E the kinetic energy after nsteps steps; checkpoints of the velocity every `every`
-    steps, the steps in between recomputed during the backward sweep."""
+def grad_ke_after_steps(case, u0, v0, w0, dt, nsteps, every=4, scal0=()):
+    """(E, dE/du0): E the kinetic energy after nsteps steps; checkpoints of the state every `every`
+    steps, the steps in between recomputed during the backward sweep.  With scalars the adjoint carries
+    their adjoints across steps too."""
     check_case(case)
-    st = _state(case, u0, v0, w0)
+    st = _state(case, u0, v0, w0, scal0)
     ckpt = {}
     for s in range(nsteps):
         if s % every == 0:
-            ckpt[s] = {k: st[k].copy(order="F") for k in st}
+            ckpt[s] = {k: v.copy(order="F") for k, v in st.items()}
         case.rk3_step(st, dt)
     fin = [interior(st[k]) for k in "uvw"]
     E = kinetic_energy(case, *fin)
     ub = ke_grad(case, *fin)
+    ns = len(case.scalars)
+    sb = [np.zeros_like(ub[0]) for _ in range(ns)]
     for s0 in sorted(ckpt, reverse=True):
         seg = {k: v.copy(order="F") for k, v in ckpt[s0].items()}
         rec = [_step_recording(case, seg, dt) for _ in range(s0, min(s0 + every, nsteps))]
         for starts in reversed(rec):
-            ub = step_adjoint(case, starts, ub, dt)
+            fb = [np.zeros_like(x) for x in ub]
+            fsb = [np.zeros_like(ub[0]) for _ in range(ns)]
+            for irk in (2, 1, 0):
+                r = substep_adjoint(case, starts[irk], irk, dt, ub, fb, sb, fsb)
+                ub, fb = r[0], r[1]
+                if ns:
+                    sb, fsb = r[2], r[3]
     return E, ub

@@ -21,6 +21,11 @@ differences.
 * central differences of the GPU's own steps at larger sizes: a closed 96^3 box with a
   moving lid (8 steps) and a 128^3 tri-periodic box with 3-D implicit diffusion (8 steps),
   3 directions each, to 1e-5;
+* scalars (cans_rk3_substep_adjoint_scal): a heated channel with a buoyant scalar
+  (Dirichlet walls, a source, gravity along x and z) and a passive one (Neumann walls):
+  one substep against the oracle at 1e-12 (velocity, scalars and both right-hand sides),
+  the gradient after 4 steps at 1e-10, and at 128x64x96 x 8 steps central differences
+  along 3 directions to 1e-5.
   and a 256x128x96 channel (Re_b = 5600, 16 steps, checkpoints every 4) whose
   directional derivatives along 4 seeded directions agree with central differences to
   1e-5.
@@ -400,3 +405,121 @@ def test_implicit_diffusion_gradient_central_differences(torch):
     sim, _ = channel_pair(p)
     u0 = list(cases.initial_state("tgv", p, sim.grid, seed=11)[:3])
     fd_check(sim, u0, 1e-2, 8, 4, 2027)
+
+
+def scalar_params(ng):
+    """The heated channel (walls in z, stretched; the first scalar buoyant with gravity along x and z,
+    Dirichlet walls and a source; the second passive with Neumann walls), both scalars unforced."""
+    p = cases.params("rb_channel", ng=ng)
+    p["scalars"] = (p["scalars"][0], dict(p["scalars"][1], is_forced=False))
+    return p
+
+
+def test_substep_adjoint_scalars_matches_oracle(torch):
+    from oracle import adjoint as A
+    from oracle import oracle as O
+    print("\n[case] cans_rk3_substep_adjoint_scal, heated channel 16x12x20, two scalars, buoyancy", flush=True)
+    p = scalar_params((16, 12, 20))
+    sim, orc = channel_pair(p)
+    rng = np.random.default_rng(24)
+
+    def dofs():
+        a = [rng.uniform(-1, 1, tuple(orc.n)) for _ in range(3)]
+        A._clear_wall_faces(orc, a)
+        return a
+
+    q, ub, fb = dofs(), dofs(), dofs()
+    qh = [O.field(orc.n) for _ in range(3)]
+    for x, y in zip(qh, q):
+        x[1:-1, 1:-1, 1:-1] = y
+    orc.bounduvw(*qh)
+    ns = len(orc.scalars)
+    s_in = [rng.uniform(-1, 1, tuple(orc.n)) for _ in range(ns)]
+    sh = []
+    for i, a in enumerate(s_in):
+        f = O.field(orc.n)
+        f[1:-1, 1:-1, 1:-1] = a
+        orc.boundp(f, cbc=orc.scalars[i]["cbc"], bc=orc.scalars[i]["bc"])
+        sh.append(f)
+    sb = [rng.uniform(-1, 1, tuple(orc.n)) for _ in range(ns)]
+    fsb = [rng.uniform(-1, 1, tuple(orc.n)) for _ in range(ns)]
+
+    def put(arrays, halo):
+        out = []
+        for a in arrays:
+            f = sim.ctx.field()
+            f.set(a if halo else np.pad(a, 1))
+            out.append(f)
+        return out
+
+    fq, fub, ffb = put(qh, True), put(ub, False), put(fb, False)
+    fs, fsbf, ffsb = put(sh, True), put(sb, False), put(fsb, False)
+    fubo, ffbo = [sim.ctx.field() for _ in range(3)], [sim.ctx.field() for _ in range(3)]
+    fsbo, ffsbo = [sim.ctx.field() for _ in range(ns)], [sim.ctx.field() for _ in range(ns)]
+    pp = sim.ctx.field()
+    dt, irk = 2e-3, 2
+    sim.rk3_substep_adjoint(irk, dt, [f.ptr for f in fq], fub, ffb, fubo, ffbo, pp,
+                            ([f.ptr for f in fs], fsbf, ffsb, fsbo, ffsbo))
+    sim.ctx.sync()
+    r_ub, r_fb, r_sb, r_fsb = A.substep_adjoint(orc, (qh, sh), irk - 1, dt, ub, fb, sb, fsb)
+    errs = [relerr([f.interior() for f in fubo], r_ub), relerr([f.interior() for f in ffbo], r_fb),
+            relerr([f.interior() for f in fsbo], r_sb), relerr([f.interior() for f in ffsbo], r_fsb)]
+    print("adjoint substep rel err: velocity %.3e, rhs %.3e, scalars %.3e, scalar rhs %.3e" % tuple(errs), flush=True)
+    assert max(errs) < 1e-12
+
+
+def test_gradient_scalars_matches_oracle(torch):
+    from cans_amd.adjoint import kinetic_energy_after_steps
+    from oracle import adjoint as A
+    print("\n[case] gradient, heated channel 16x12x20 with two scalars and buoyancy, 4 steps", flush=True)
+    p = scalar_params((16, 12, 20))
+    sim, orc = channel_pair(p)
+    u0 = list(cases.initial_state("rb_channel", p, sim.grid, seed=12)[:3])
+    A._clear_wall_faces(orc, u0)
+    s0 = cases.initial_scalars("rb_channel", p, sim.grid, seed=13)
+    dt, nsteps = 2e-3, 4
+    E_orc, g_orc = A.grad_ke_after_steps(orc, *u0, dt, nsteps, every=2, scal0=s0)
+    t0 = [torch.tensor(np.ascontiguousarray(a.transpose(2, 1, 0)), device="cuda", requires_grad=True) for a in u0]
+    ts = [torch.tensor(np.ascontiguousarray(a.transpose(2, 1, 0)), device="cuda") for a in s0]
+    E = kinetic_energy_after_steps(sim, *t0, dt, nsteps, every=2, s0=ts)
+    E.backward()
+    g_gpu = [t.grad.cpu().numpy().transpose(2, 1, 0) for t in t0]
+    eE = abs(E.item() - E_orc) / E_orc
+    eg = relerr(g_gpu, g_orc)
+    print("E %.15e (oracle %.15e, rel %.2e); gradient rel err %.3e" % (E.item(), E_orc, eE, eg), flush=True)
+    assert eE < 1e-13 and eg < 1e-10
+
+
+@pytest.mark.timeout(600)
+def test_scalars_gradient_central_differences(torch):
+    """The heated channel with two scalars and buoyancy at 128x64x96, 8 steps: central differences."""
+    from cans_amd.adjoint import kinetic_energy_after_steps
+    from oracle import adjoint as A
+    print("\n[case] heated channel 128x64x96 with two scalars and buoyancy, 8 steps, checkpoints every 4", flush=True)
+    p = scalar_params((128, 64, 96))
+    sim, orc = channel_pair(p)
+    u0 = list(cases.initial_state("rb_channel", p, sim.grid, seed=14)[:3])
+    A._clear_wall_faces(orc, u0)
+    s0 = [torch.tensor(np.ascontiguousarray(a.transpose(2, 1, 0)), device="cuda")
+          for a in cases.initial_scalars("rb_channel", p, sim.grid, seed=15)]
+    dt, nsteps, every = 1e-3, 8, 4
+    t0 = [torch.tensor(np.ascontiguousarray(a.transpose(2, 1, 0)), device="cuda", requires_grad=True) for a in u0]
+    E = kinetic_energy_after_steps(sim, *t0, dt, nsteps, every, s0=s0)
+    E.backward()
+    grad = [t.grad.detach() for t in t0]
+    base = [t.detach() for t in t0]
+    gen = torch.Generator(device="cuda").manual_seed(2028)
+    eps = 1e-6
+    with torch.no_grad():
+        for r in range(3):
+            d = [torch.randn(x.shape, generator=gen, device="cuda", dtype=torch.float64) for x in base]
+            d[2][-1] = 0.
+            ep = kinetic_energy_after_steps(sim, *(x + eps * y for x, y in zip(base, d)), dt, nsteps, every,
+                                            s0=s0).item()
+            em = kinetic_energy_after_steps(sim, *(x - eps * y for x, y in zip(base, d)), dt, nsteps, every,
+                                            s0=s0).item()
+            fd = (ep - em) / (2 * eps)
+            ad = sum(float((g * y).sum()) for g, y in zip(grad, d))
+            rel = abs(fd - ad) / abs(ad)
+            print("direction %d: central difference %.12e, adjoint %.12e, rel %.2e" % (r, fd, ad, rel), flush=True)
+            assert rel < 1e-5, (r, fd, ad)

@@ -37,6 +37,14 @@ def dot(a, b):
     return sum(float(np.sum(x * y)) for x, y in zip(a, b))
 
 
+def fd_close(fd, ad, grad, d, rtol=1e-6):
+    """|fd - ad| within rtol of max(|ad|, 1e-3 |grad| |d|): the oracle's bulk means are OpenMP reductions whose
+    summation order varies between runs, so a direction nearly orthogonal to the gradient would otherwise
+    test the difference quotient's rounding rather than the adjoint."""
+    norm = np.sqrt(sum(float(np.sum(g * g)) for g in grad) * sum(float(np.sum(x * x)) for x in d))
+    return abs(fd - ad) <= rtol * max(abs(ad), 1e-3 * norm)
+
+
 def test_mom_jt_dot_product():
     case = tgv_case((8, 6, 5), l=(1.0, 1.5, 0.7), visci=37.0)
     rng = np.random.default_rng(1)
@@ -190,7 +198,7 @@ def test_gradient_channel_matches_central_differences():
         em = A.ke_after_steps(case, *(a - eps * b for a, b in zip(u0, d)), dt, nsteps)
         fd = (ep - em) / (2 * eps)
         ad = dot(grad, d)
-        assert abs(fd - ad) <= 1e-6 * abs(ad), (fd, ad)
+        assert fd_close(fd, ad, grad, d), (fd, ad)
 
 
 # ---- walls in x and y too (duct, closed box): DCT-based pressure solves ----
@@ -262,7 +270,7 @@ def test_gradient_walls_xyz_matches_central_differences(kind):
         em = A.ke_after_steps(case, *(a - eps * b for a, b in zip(u0, d)), dt, nsteps)
         fd = (ep - em) / (2 * eps)
         ad = dot(grad, d)
-        assert abs(fd - ad) <= 1e-6 * abs(ad), (fd, ad)
+        assert fd_close(fd, ad, grad, d), (fd, ad)
 
 
 @pytest.mark.timeout(300)
@@ -281,7 +289,7 @@ def test_gradient_implicit_diffusion_matches_central_differences():
         em = A.ke_after_steps(case, *(a - eps * b for a, b in zip(u0, d)), dt, nsteps)
         fd = (ep - em) / (2 * eps)
         ad = dot(grad, d)
-        assert abs(fd - ad) <= 1e-6 * abs(ad), (fd, ad)
+        assert fd_close(fd, ad, grad, d), (fd, ad)
     # planted mistake: the adjoint without its Helmholtz solves misses the central difference
     helm = A.helmholtz_inv
     try:
@@ -290,3 +298,95 @@ def test_gradient_implicit_diffusion_matches_central_differences():
     finally:
         A.helmholtz_inv = helm
     assert abs(dot(bad, d) - fd) > 1e-4 * abs(fd)
+
+
+# ---- scalars: transport and Boussinesq buoyancy ----
+
+def scalar_channel(ng=(8, 6, 10), gr=1.5, visci=60.0):
+    """Channel (walls in z, stretched) with a buoyant scalar (Dirichlet walls, hot bottom) and a passive
+    one (Neumann walls with a flux); gravity along z."""
+    scal = (dict(alphai=80.0, cbc="PPPPDD", bc=(0.0, 0.0, 0.0, 0.0, 1.0, -0.5), is_forced=False),
+            dict(alphai=120.0, cbc="PPPPNN", bc=(0.0, 0.0, 0.0, 0.0, 0.2, 0.3), is_forced=False))
+    return O.OracleCase(ng=ng, l=(2.0, 1.2, 1.0), gtype=1, gr=gr, cbcvel="PPPPDD" * 3, cbcpre="PPPPNN", visci=visci,
+                        is_forced=(1, 0, 0), velf=(1.0, 0.0, 0.0), scalars=scal, gacc=(0.0, 0.0, -1.0), beta=2.0,
+                        is_boussinesq_buoyancy=True)
+
+
+def scalar_ghosts(case, a, i):
+    f = O.field(case.n)
+    f[1:-1, 1:-1, 1:-1] = a
+    sc = case.scalars[i]
+    case.boundp(f, cbc=sc["cbc"], bc=sc["bc"])
+    return f
+
+
+def test_scal_jt_dot_product():
+    case = scalar_channel()
+    A.check_case(case)
+    rng = np.random.default_rng(51)
+    for i, sc in enumerate(case.scalars):
+        q, dq = dofs(case, rng), dofs(case, rng)
+        s0, ds = rng.uniform(-1, 1, tuple(case.n)), rng.uniform(-1, 1, tuple(case.n))
+        sig = rng.uniform(-1, 1, tuple(case.n))
+        fp = case.scal(*ghosts(case, [a + b for a, b in zip(q, dq)]), scalar_ghosts(case, s0 + ds, i), sc["alpha"])
+        fm = case.scal(*ghosts(case, [a - b for a, b in zip(q, dq)]), scalar_ghosts(case, s0 - ds, i), sc["alpha"])
+        lhs = float(np.sum(sig * (fp - fm) / 2))
+        js, jq = A.scal_jt(case, ghosts(case, q), scalar_ghosts(case, s0, i), sig, sc["alpha"], sc["cbc"])
+        rhs = float(np.sum(js * ds)) + dot(jq, dq)
+        scale = float(np.abs(fp - fm).sum())
+        assert abs(lhs - rhs) <= 1e-13 * scale, (i, lhs, rhs)
+        # planted mistake: the ghost fold of the walls dropped
+        bad = A._fold_scalar_z
+        try:
+            A._fold_scalar_z = lambda c, out, cbc: np.asfortranarray(out[:, :, 2:c.n[2] + 2])
+            js2, _ = A.scal_jt(case, ghosts(case, q), scalar_ghosts(case, s0, i), sig, sc["alpha"], sc["cbc"])
+        finally:
+            A._fold_scalar_z = bad
+        assert abs(float(np.sum(js2 * ds)) + dot(jq, dq) - lhs) > 1e-6 * scale
+
+
+def test_buoyancy_transpose():
+    case = scalar_channel()
+    rng = np.random.default_rng(52)
+    wb = dofs(case, rng)
+    ds = rng.uniform(-1, 1, tuple(case.n))
+    coef = [0.3, -0.7, 1.1]
+    sc = case.scalars[0]
+    # the linear part of the term: ghosts with zero boundary values
+    saved = sc["bc"]
+    try:
+        sc["bc"] = np.zeros(6)
+        sg = scalar_ghosts(case, ds, 0)
+    finally:
+        sc["bc"] = saved
+    fwd = [-coef[c] * (np.roll(sg, -1, axis=c) + sg)[1:-1, 1:-1, 1:-1] for c in range(3)]
+    A._clear_wall_faces(case, fwd)
+    lhs = dot(wb, fwd)
+    rhs = float(np.sum(A.buoyancy_t(case, wb, sc["cbc"], coef) * ds))
+    assert abs(lhs - rhs) <= 1e-13 * sum(float(np.abs(x).sum()) for x in fwd), (lhs, rhs)
+
+
+@pytest.mark.timeout(300)
+def test_gradient_scalars_matches_central_differences():
+    case = scalar_channel()
+    rng = np.random.default_rng(53)
+    u0 = channel_state(case, rng)
+    s0 = [rng.uniform(-0.5, 0.5, tuple(case.n)) for _ in case.scalars]
+    dt, nsteps = 2e-3, 4
+    E, grad = A.grad_ke_after_steps(case, *u0, dt, nsteps, every=2, scal0=s0)
+    eps = 1e-6
+    for _ in range(3):
+        d = dofs(case, rng)
+        ep = A.ke_after_steps(case, *(a + eps * b for a, b in zip(u0, d)), dt, nsteps, scal0=s0)
+        em = A.ke_after_steps(case, *(a - eps * b for a, b in zip(u0, d)), dt, nsteps, scal0=s0)
+        fd = (ep - em) / (2 * eps)
+        ad = dot(grad, d)
+        assert fd_close(fd, ad, grad, d), (fd, ad)
+    # planted mistake: the scalars' adjoint left out (their effect on the velocity through buoyancy)
+    sj = A.scal_jt
+    try:
+        A.scal_jt = lambda c, qh, sh, sig, al, cbc: (np.zeros(tuple(c.n)), [np.zeros(tuple(c.n))] * 3)
+        _, bad = A.grad_ke_after_steps(case, *u0, dt, nsteps, every=2, scal0=s0)
+    finally:
+        A.scal_jt = sj
+    assert abs(dot(bad, d) - fd) > 1e-6 * abs(fd)
