@@ -452,6 +452,21 @@ int launch_scal_adjoint(cans_ctx* c, const ScalAdj& sa) {
   return CANS_OK;
 }
 
+// the bulk forcing s <- s + (scalf - sum_k gvr_f(k) s) has the Jacobian I - 1 gvr_f^T: its transpose
+// subtracts gvr_f(k) times the plain sum of the adjoint
+__global__ void k_scal_force_t(Layout L, double* sh, const double* gvr_f, double sum) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
+  if (i > L.n1) return;
+  sh[L.at(i, j, k)] -= gvr_f[k] * sum;
+}
+
+int launch_scal_force_t(cans_ctx* c, double* sh, const double* gvr_f, double sum) {
+  const Layout& L = c->L;
+  k_scal_force_t<<<dim3(cdiv(L.n1, 64), L.n2, L.n3), 64, 0, c->stream>>>(L, sh, gvr_f, sum);
+  CANS_LAUNCH_CHECK("k_scal_force_t");
+  return CANS_OK;
+}
+
 int launch_scal_jt(cans_ctx* c, const ScalAdj& sa) {
   const Layout& L = c->L;
   ScalJtArgs a;

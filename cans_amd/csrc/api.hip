@@ -888,14 +888,14 @@ int adjoint_scope(cans_ctx* c, const cans_step_params* s, const char* who, bool 
   if (s->nscal && !scalars)
     return fail(CANS_EINVAL, w + ": a step with scalars runs backwards through cans_rk3_substep_adjoint_scal");
   if (s->nscal) {
-    // scalars: explicit diffusion, x and y periodic, unforced; z periodic or walled (Dirichlet or Neumann)
+    // scalars: explicit diffusion, x and y periodic; z periodic or walled (Dirichlet or Neumann)
     if (c->imp) return fail(CANS_EUNSUPPORTED, w + ": with scalars the adjoint covers explicit diffusion");
     if (adjoint_walls(s) & 3) return fail(CANS_EUNSUPPORTED, w + ": with scalars the adjoint covers periodic x and y");
     if (!s->scal || s->nscal > CANS_MAX_SCALARS) return fail(CANS_EINVAL, w + ": bad scalar list");
     const bool zw = adjoint_walls(s) & 4;
     for (int i = 0; i < s->nscal; i++) {
       const cans_scalar& q = s->scal[i];
-      if (q.is_forced) return fail(CANS_EUNSUPPORTED, w + ": the adjoint covers unforced scalars");
+      if (q.is_forced && !s->grid_vol_ratio_f_d) return fail(CANS_EINVAL, w + ": null grid_vol_ratio_f");
       for (int e = 0; e < 4; e++)
         if (q.cbc[e] != 'P') return fail(CANS_EUNSUPPORTED, w + ": a scalar must be periodic in x and y");
       for (int e = 4; e < 6; e++)
@@ -1072,11 +1072,11 @@ extern "C" int cans_rk3_substep_adjoint(cans_ctx* c, const cans_step_params* s, 
   return substep_adjoint_vel(c, s, irk, dt, u, v, w, ub, vb, wb, fu, fv, fw, ubo, vbo, wbo, fuo, fvo, fwo, pp, false);
 }
 
-// The adjoint substep with scalars (explicit diffusion; x and y periodic; unforced scalars): the
-// velocity part as cans_rk3_substep_adjoint, then per scalar i (main.f90:440-450: the scalars
-// advance first, and the velocity's rk reads the first one's new value in the Boussinesq term,
-// rk.f90:163-177) sh_i = sb_i (+ that term's transpose of wb for i = 0), sbo_i = sh_i + J_s^T
-// (f1 sh_i + fsb_i), fsbo_i = f2 sh_i, ubo += J_q^T (f1 sh_i + fsb_i).
+// The adjoint substep with scalars (explicit diffusion; x and y periodic): the velocity part as
+// cans_rk3_substep_adjoint, then per scalar i (main.f90:440-450: the scalars advance first, their
+// bulk forcing is added, and the velocity's rk reads the first one's new value in the Boussinesq
+// term, rk.f90:163-177) sh_i = Fs'^T (sb_i (+ that term's transpose of wb for i = 0)), sbo_i = sh_i +
+// J_s^T (f1 sh_i + fsb_i), fsbo_i = f2 sh_i, ubo += J_q^T (f1 sh_i + fsb_i).
 extern "C" int cans_rk3_substep_adjoint_scal(cans_ctx* c, const cans_step_params* s, int irk, double dt,
                                              const void* u, const void* v, const void* w, void* ub, void* vb,
                                              void* wb, void* fu, void* fv, void* fw, void* ubo, void* vbo, void* wbo,
@@ -1138,6 +1138,11 @@ extern "C" int cans_rk3_substep_adjoint_scal(cans_ctx* c, const cans_step_params
     sa.sign[1] = qs.cbc[5] == 'N' ? 1. : -1.;
     sa.buoy = i == 0 && s->is_buoyancy;
     if ((rc = launch_scal_adjoint(c, sa))) return rc;
+    if (qs.is_forced) {  // the forcing acts before the buoyancy reads the scalar: its transpose after
+      double sum = 0.;
+      if ((rc = run_bulk_mean<double>(c, sa.sh, nullptr, &sum))) return rc;
+      if ((rc = launch_scal_force_t(c, sa.sh, s->grid_vol_ratio_f_d, sum))) return rc;
+    }
     // periodic images of sh and fsb (the wall ghosts are read as zero)
     if ((rc = plan_boundp(c, qs.cbc, zero6, s->nb, s->is_bound, s->dl, s->dzc_h, sb[i], CANS_HALO_ABSENT,
                           (int)sizeof(double))))

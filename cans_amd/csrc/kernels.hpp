@@ -203,6 +203,7 @@ int run_chkdiv(cans_ctx* c, const T* u, const T* v, const T* w, const double* dz
 template <typename T>
 int run_chkdt(cans_ctx* c, const T* u, const T* v, const T* w, const double* dzci, const double* dzfi,
               const double* dl, double* dti_out);
+// sum over the interior of q times gvr(k) (utils.f90:15-42); gvr null: the plain sum
 template <typename T>
 int run_bulk_mean(cans_ctx* c, const T* q, const double* gvr, double* mean);
 
@@ -246,6 +247,8 @@ struct ScalAdj {
   int zwall, buoy;
 };
 int launch_scal_adjoint(cans_ctx* c, const ScalAdj& sa);  // the buoyancy transpose into sh
+// the transposed scalar forcing (scal.f90:313-329, rk.f90:474-475): sh -= gvr_f(k) * sum
+int launch_scal_force_t(cans_ctx* c, double* sh, const double* gvr_f, double sum);
 int launch_scal_jt(cans_ctx* c, const ScalAdj& sa);
 // the volume weights of P^T = M P M^-1 and the transposed bulk forcing (k_adj_weigh)
 int launch_adj_weigh(cans_ctx* c, double* const* f, const double* dzfi, const double* dzci, bool inv, double mu,

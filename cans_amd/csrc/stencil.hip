@@ -893,7 +893,7 @@ __global__ void k_bulk_mean(Layout L, const T* q, const double* gvr, double* par
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int j = blockIdx.y + 1, k = blockIdx.z + 1;
   double s = 0.;
-  if (i <= L.n1) s = (double)q[L.at(i, j, k)] * gvr[k];
+  if (i <= L.n1) s = gvr ? (double)q[L.at(i, j, k)] * gvr[k] : (double)q[L.at(i, j, k)];  // gvr null: a plain sum
   const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
   double d1, d2;
   block_sum3(s, 0., 0., partials + bid, &d1, &d2, true);

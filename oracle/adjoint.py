@@ -108,9 +108,8 @@ def check_case(case):
         if case.is_impdiff or walls(case) and wall_dirs(case) != [2]:
             raise ValueError("scalars: explicit diffusion, periodic x and y")
         for sc in case.scalars:
-            if sc["is_forced"] or sc["cbc"][:4] != "PPPP" or sc["cbc"][4:6] != ("PP" if 2 not in wall_dirs(case)
-                                                                                else sc["cbc"][4:6]):
-                raise ValueError("scalars: unforced, periodic along the periodic directions")
+            if sc["cbc"][:4] != "PPPP" or sc["cbc"][4:6] != ("PP" if 2 not in wall_dirs(case) else sc["cbc"][4:6]):
+                raise ValueError("scalars: periodic along the periodic directions")
             if 2 in wall_dirs(case) and any(ch not in "DN" for ch in sc["cbc"][4:6]):
                 raise ValueError("scalars: Dirichlet or Neumann at the z walls")
     if case.is_impdiff and (walls(case) or case.is_impdiff_1d):
@@ -477,11 +476,12 @@ def substep_adjoint(case, start, irk, dt, ub, fb, sb=(), fsb=()):
     (rk.f90:156-186, 312-326), so its adjoint is hb = Fr'^T H^-1 P ub,
     ub <- hb + J_A^T (f1 hb + fb) + (f1 + f2)/2 visc L hb, fb <- f2 hb.
 
-    Scalars (explicit diffusion, unforced; main.f90:440-450): each is advanced first,
-    s' = s + f1 S(u, s) + f2 So, So' = S(u, s), and the velocity's rk then reads the first one's new
-    value in the Boussinesq term -f12 gacc beta (s(x + e_c) + s(x))/2 (rk.f90:163-177).  So the first
-    scalar's adjoint gains that term's transpose of wb before the scalars' own adjoints
-    sh = sb (+ buoyancy^T wb), sb <- sh + J_s^T (f1 sh + fsb), fsb <- f2 sh, ub += J_u^T (f1 sh + fsb)."""
+    Scalars (explicit diffusion; main.f90:440-450): each is advanced first,
+    s' = Fs(s + f1 S(u, s) + f2 So), So' = S(u, s), Fs its bulk forcing when forced, and the velocity's rk
+    then reads the first one's new value in the Boussinesq term -f12 gacc beta (s(x + e_c) + s(x))/2
+    (rk.f90:163-177).  So the first scalar's adjoint gains that term's transpose of wb before the
+    scalars' own adjoints sh = Fs'^T (sb (+ buoyancy^T wb)), sb <- sh + J_s^T (f1 sh + fsb),
+    fsb <- f2 sh, ub += J_u^T (f1 sh + fsb)."""
     vel, scal = start if isinstance(start, tuple) else (start, [])
     f1, f2 = O.RKCOEFF[irk][0] * dt, O.RKCOEFF[irk][1] * dt
     if case.is_impdiff:
@@ -503,6 +503,10 @@ def substep_adjoint(case, start, irk, dt, ub, fb, sb=(), fsb=()):
     if case.is_buoyancy:
         coef = [(f1 + f2) * case.gacc[c] * case.beta * 0.5 for c in range(3)]
         sh[0] = sh[0] + buoyancy_t(case, wb, case.scalars[0]["cbc"], coef)
+    gvr = np.asarray(case.grid_vol_ratio_f[1:case.n[2] + 1], dtype=np.float64)[None, None, :]
+    for i, sc in enumerate(case.scalars):
+        if sc["is_forced"]:  # s <- s + (scalf - sum gvr_f s) (rk.f90:474-475, scal.f90:313-329): I - 1 gvr_f^T
+            sh[i] = sh[i] - gvr * float(np.sum(sh[i]))
     sb_new, fsb_new = [], []
     for i, sc in enumerate(case.scalars):
         sigma = f1 * sh[i] + fsb[i]

@@ -22,7 +22,8 @@ differences.
   moving lid (8 steps) and a 128^3 tri-periodic box with 3-D implicit diffusion (8 steps),
   3 directions each, to 1e-5;
 * scalars (cans_rk3_substep_adjoint_scal): a heated channel with a buoyant scalar
-  (Dirichlet walls, a source, gravity along x and z) and a passive one (Neumann walls):
+  (Dirichlet walls, a source, gravity along x and z) and a passive one (Neumann walls),
+  both bulk-forced:
   one substep against the oracle at 1e-12 (velocity, scalars and both right-hand sides),
   the gradient after 4 steps at 1e-10, and at 128x64x96 x 8 steps central differences
   along 3 directions to 1e-5.
@@ -409,9 +410,9 @@ def test_implicit_diffusion_gradient_central_differences(torch):
 
 def scalar_params(ng):
     """The heated channel (walls in z, stretched; the first scalar buoyant with gravity along x and z,
-    Dirichlet walls and a source; the second passive with Neumann walls), both scalars unforced."""
+    Dirichlet walls and a source; the second passive with Neumann walls), both scalars bulk-forced."""
     p = cases.params("rb_channel", ng=ng)
-    p["scalars"] = (p["scalars"][0], dict(p["scalars"][1], is_forced=False))
+    p["scalars"] = (dict(p["scalars"][0], is_forced=True, scalf=0.4), p["scalars"][1])
     return p
 
 

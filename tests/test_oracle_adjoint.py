@@ -303,9 +303,9 @@ def test_gradient_implicit_diffusion_matches_central_differences():
 # ---- scalars: transport and Boussinesq buoyancy ----
 
 def scalar_channel(ng=(8, 6, 10), gr=1.5, visci=60.0):
-    """Channel (walls in z, stretched) with a buoyant scalar (Dirichlet walls, hot bottom) and a passive
-    one (Neumann walls with a flux); gravity along z."""
-    scal = (dict(alphai=80.0, cbc="PPPPDD", bc=(0.0, 0.0, 0.0, 0.0, 1.0, -0.5), is_forced=False),
+    """Channel (walls in z, stretched) with a buoyant scalar (Dirichlet walls, hot bottom, bulk-forced) and a
+    passive one (Neumann walls with a flux); gravity along z."""
+    scal = (dict(alphai=80.0, cbc="PPPPDD", bc=(0.0, 0.0, 0.0, 0.0, 1.0, -0.5), is_forced=True, scalf=0.2),
             dict(alphai=120.0, cbc="PPPPNN", bc=(0.0, 0.0, 0.0, 0.0, 0.2, 0.3), is_forced=False))
     return O.OracleCase(ng=ng, l=(2.0, 1.2, 1.0), gtype=1, gr=gr, cbcvel="PPPPDD" * 3, cbcpre="PPPPNN", visci=visci,
                         is_forced=(1, 0, 0), velf=(1.0, 0.0, 0.0), scalars=scal, gacc=(0.0, 0.0, -1.0), beta=2.0,
