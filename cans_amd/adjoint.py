@@ -3,7 +3,7 @@ walls, a periodic z on a uniform grid, walls in z on any grid; bulk forcing of u
 along periodic directions; explicit diffusion, or 3-D implicit diffusion on tri-periodic
 grids).  Under CANS_MIXED the projections (forward and adjoint) run with FP32 pressure
 storage and transforms, the velocity and everything else in FP64.  Scalars (explicit
-diffusion, x and y periodic, bulk-forced or not; the first one buoyant when the case says so)
+diffusion, bulk-forced or not, walled where the box is; the first one buoyant when the case says so)
 are advanced by the forward steps, and their adjoints carried by the backward ones
 (cans_rk3_substep_adjoint_scal); the gradient is taken with respect to the velocity.
 

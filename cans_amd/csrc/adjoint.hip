@@ -346,17 +346,18 @@ int launch_adj_lap(cans_ctx* c, const double* const* a, double* const* out, doub
 }
 
 // ---------------------------------------------------------------------------
-// Scalars (explicit diffusion; x and y periodic, z periodic or walled).  The
-// scalar right-hand side of scal.f90:41-98 is
+// Scalars (explicit diffusion; each direction periodic or walled).  The scalar
+// right-hand side of scal.f90:41-98 is
 //   S(x) = -sum_d H_d(k) [F_d(x) - F_d(x - e_d)] + alpha L s,  F_d(x) = (s(x + e_d) + s(x))/2 q_d(x),
 // H = dxi, dyi, dzfi(k); with R_d(x) = H_d(k_x) sigma(x) - H_d(k_x+e_d) sigma(x + e_d) its
 // transposed Jacobians are
 //   (J_s^T sigma)(y)   = -1/2 sum_d [q_d(y - e_d) R_d(y - e_d) + q_d(y) R_d(y)] + alpha L^T sigma(y),
 //   (J_q^T sigma)_d(y) = -1/2 (s(y + e_d) + s(y)) R_d(y).
 // k_scal_jt forms sigma = f1 sh + fsb as it reads it (zero off the scalar's degrees of
-// freedom), writes sbo = sh + J_s^T sigma (with walls also on the ghost planes, which
-// k_scal_fold folds), fsbo = f2 sh, and adds J_q^T sigma to the velocity adjoint at the
-// velocity's degrees of freedom.  One thread per cell.
+// freedom: the ghost layers of the wall directions), writes sbo = sh + J_s^T sigma (on those
+// ghost layers too, which k_scal_fold folds, z first, then y, then x: boundp sets the ghosts x,
+// y, z), fsbo = f2 sh, and adds J_q^T sigma to the velocity adjoint at the velocity's degrees of
+// freedom (not on a normal velocity's wall face).  One thread per cell.
 struct ScalJtArgs {
   Layout L;
   const double* q[3];  // the velocity the forward substep started from (halo'd, ghosts set)
@@ -367,78 +368,109 @@ struct ScalJtArgs {
   double* ubo[3];
   const double *dzci, *dzfi;  // (0:n3+1)
   double dxi, dyi, alpha, f1, f2;
-  int zwall;
+  int wm;  // the wall directions, bit 0 x, 1 y, 2 z
 };
 
 __global__ void k_scal_jt(ScalJtArgs a) {
   const Layout& L = a.L;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1;
-  const int k = (int)blockIdx.z + (a.zwall ? 0 : 1);
-  if (i > L.n1) return;
-  const int n3 = L.n3;
-  auto kin = [&](int kk) { return kk >= 0 && kk <= n3 + 1; };
-  auto dof = [&](int kk) { return !a.zwall || (kk >= 1 && kk <= n3); };
-  auto sig = [&](int di, int dj, int kk) -> double {
-    if (!kin(kk) || !dof(kk)) return 0.;
-    const long o = L.at(i + di, j + dj, kk);
+  const int ox = a.wm & 1, oy = a.wm >> 1 & 1, oz = a.wm >> 2 & 1;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1 - ox, j = (int)blockIdx.y + 1 - oy;
+  const int k = (int)blockIdx.z + 1 - oz;
+  if (i > L.n1 + ox) return;
+  const int n[3] = {L.n1, L.n2, L.n3};
+  // (ii, jj, kk) lies in the halo'd fields; sigma is a degree of freedom there
+  auto inh = [&](int ii, int jj, int kk) {
+    return ii >= 0 && ii <= n[0] + 1 && jj >= 0 && jj <= n[1] + 1 && kk >= 0 && kk <= n[2] + 1;
+  };
+  auto dof = [&](int ii, int jj, int kk) {
+    return (!ox || (ii >= 1 && ii <= n[0])) && (!oy || (jj >= 1 && jj <= n[1])) && (!oz || (kk >= 1 && kk <= n[2]));
+  };
+  auto sig = [&](int di, int dj, int dk) -> double {
+    const int ii = i + di, jj = j + dj, kk = k + dk;
+    if (!inh(ii, jj, kk) || !dof(ii, jj, kk)) return 0.;
+    const long o = L.at(ii, jj, kk);
     return a.f1 * a.sh[o] + a.fsb[o];
   };
-  auto Q = [&](int d, int di, int dj, int kk) -> double { return kin(kk) ? a.q[d][L.at(i + di, j + dj, kk)] : 0.; };
-  auto zf = [&](int kk) { return kin(kk) ? a.dzfi[kk] : 0.; };
-  auto zc = [&](int kk) { return kin(kk) ? a.dzci[kk] : 0.; };
-  auto H = [&](int d, int kk) { return d == 0 ? a.dxi : d == 1 ? a.dyi : zf(kk); };
-  // R_d at the cell (i + di, j + dj, kk)
-  auto R = [&](int d, int di, int dj, int kk) {
+  auto Q = [&](int d, int di, int dj, int dk) -> double {
+    const int ii = i + di, jj = j + dj, kk = k + dk;
+    return inh(ii, jj, kk) ? a.q[d][L.at(ii, jj, kk)] : 0.;
+  };
+  auto zf = [&](int kk) { return kk >= 0 && kk <= n[2] + 1 ? a.dzfi[kk] : 0.; };
+  auto zc = [&](int kk) { return kk >= 0 && kk <= n[2] + 1 ? a.dzci[kk] : 0.; };
+  auto H = [&](int d, int dk) { return d == 0 ? a.dxi : d == 1 ? a.dyi : zf(k + dk); };
+  // R_d at the cell (i + di, j + dj, k + dk)
+  auto R = [&](int d, int di, int dj, int dk) {
     const int ex = d == 0, ey = d == 1, ez = d == 2;
-    return H(d, kk) * sig(di, dj, kk) - H(d, kk + ez) * sig(di + ex, dj + ey, kk + ez);
+    return H(d, dk) * sig(di, dj, dk) - H(d, dk + ez) * sig(di + ex, dj + ey, dk + ez);
   };
   double js = 0.;
   double rd[3];
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     const int ex = d == 0, ey = d == 1, ez = d == 2;
-    rd[d] = R(d, 0, 0, k);
-    js -= 0.5 * (Q(d, -ex, -ey, k - ez) * R(d, -ex, -ey, k - ez) + Q(d, 0, 0, k) * rd[d]);
+    rd[d] = R(d, 0, 0, 0);
+    js -= 0.5 * (Q(d, -ex, -ey, -ez) * R(d, -ex, -ey, -ez) + Q(d, 0, 0, 0) * rd[d]);
   }
-  const double s0 = sig(0, 0, k);
-  const double lxy = a.dxi * a.dxi * (sig(1, 0, k) - 2. * s0 + sig(-1, 0, k)) +
-                     a.dyi * a.dyi * (sig(0, 1, k) - 2. * s0 + sig(0, -1, k));
+  const double s0 = sig(0, 0, 0);
+  const double lxy = a.dxi * a.dxi * (sig(1, 0, 0) - 2. * s0 + sig(-1, 0, 0)) +
+                     a.dyi * a.dyi * (sig(0, 1, 0) - 2. * s0 + sig(0, -1, 0));
   // L = dzfi(k) [dzci(k) d+ - dzci(k-1) d-]: L^T sigma = dzci(k) d+(dzfi sigma) - dzci(k-1) d-(dzfi sigma)
-  const double gp = zf(k + 1) * sig(0, 0, k + 1), gc = zf(k) * s0, gm = zf(k - 1) * sig(0, 0, k - 1);
+  const double gp = zf(k + 1) * sig(0, 0, 1), gc = zf(k) * s0, gm = zf(k - 1) * sig(0, 0, -1);
   js += a.alpha * (lxy + zc(k) * (gp - gc) - zc(k - 1) * (gc - gm));
   const long o = L.at(i, j, k);
-  const bool in = k >= 1 && k <= n3;
+  const bool in = i >= 1 && i <= n[0] && j >= 1 && j <= n[1] && k >= 1 && k <= n[2];
   const double base = in ? a.sh[o] : 0.;
   a.sbo[o] = base + js;
   if (!in) return;
   a.fsbo[o] = a.f2 * base;
   const double sc = a.s[o];
   const double sn[3] = {a.s[o + 1], a.s[o + L.sy], a.s[o + L.sz]};
+  const int idx[3] = {i, j, k};
 #pragma unroll
   for (int d = 0; d < 3; d++)
-    if (!(d == 2 && a.zwall && k == n3)) a.ubo[d][o] += -0.5 * (sn[d] + sc) * rd[d];  // w's wall face: none
+    if (!((a.wm >> d & 1) && idx[d] == n[d])) a.ubo[d][o] += -0.5 * (sn[d] + sc) * rd[d];  // not a wall face
 }
 
-// s(1) += s0 s(0), s(n3) += s1 s(n3 + 1) (the scalar's z ghosts at walls: -1 Dirichlet, +1 Neumann)
-__global__ void k_scal_fold(Layout L, double* sb, double s0, double s1) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1;
-  if (i > L.n1) return;
-  const long b = L.at(i, j, 0), t = L.at(i, j, L.n3);
-  sb[b + L.sz] += s0 * sb[b];
-  sb[t] += s1 * sb[t + L.sz];
+// The scalar's ghost layers of wall direction d folded into the interior: s(1) += s0 s(0), s(n) +=
+// s1 s(n + 1) (-1 Dirichlet, +1 Neumann), then cleared; one thread per point of the plane of the
+// other two directions, over lo..hi of each (their ghost layers included while still to fold).
+struct ScalFold {
+  int d;
+  int lo[3], hi[3];
+  double s0, s1;
+};
+
+__global__ void k_scal_fold(Layout L, double* sb, ScalFold f) {
+  const int a = f.d == 0 ? 1 : 0, b = f.d == 2 ? 1 : 2;
+  int idx[3];
+  idx[a] = f.lo[a] + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  idx[b] = f.lo[b] + (int)blockIdx.y;
+  if (idx[a] > f.hi[a]) return;
+  const int nd = f.d == 0 ? L.n1 : f.d == 1 ? L.n2 : L.n3;
+  auto at = [&](int t) {
+    int x[3] = {idx[0], idx[1], idx[2]};
+    x[f.d] = t;
+    return L.at(x[0], x[1], x[2]);
+  };
+  sb[at(1)] += f.s0 * sb[at(0)];
+  sb[at(nd)] += f.s1 * sb[at(nd + 1)];
+  sb[at(0)] = 0.;
+  sb[at(nd + 1)] = 0.;
 }
 
 // sh += the Boussinesq term's transpose (rk.f90:163-177: u_c -= coef_c (s(x + e_c) + s(x))):
 //   sh(y) -= sum_c coef_c (wb_c(y - e_c) + wb_c(y)),
-// wb read as zero off the velocity's degrees of freedom (w's wall faces when z is walled).
+// wb read as zero off the velocity's degrees of freedom (a normal velocity's wall faces; its
+// terms at the scalar's ghost cells all vanish, so the interior is all it changes).
 __global__ void k_scal_buoy_t(Layout L, double* sh, const double* wu, const double* wv, const double* ww,
-                              double c0, double c1, double c2, int zwall) {
+                              double c0, double c1, double c2, int wm) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
   if (i > L.n1) return;
   const long o = L.at(i, j, k);
-  const double wm = (zwall && k == 1) ? 0. : ww[o - L.sz];
-  const double w0 = (zwall && k == L.n3) ? 0. : ww[o];
-  sh[o] -= c0 * (wu[o - 1] + wu[o]) + c1 * (wv[o - L.sy] + wv[o]) + c2 * (wm + w0);
+  const double um = ((wm & 1) && i == 1) ? 0. : wu[o - 1], u0 = ((wm & 1) && i == L.n1) ? 0. : wu[o];
+  const double vm = ((wm & 2) && j == 1) ? 0. : wv[o - L.sy], v0 = ((wm & 2) && j == L.n2) ? 0. : wv[o];
+  const double wmv = ((wm & 4) && k == 1) ? 0. : ww[o - L.sz], w0 = ((wm & 4) && k == L.n3) ? 0. : ww[o];
+  sh[o] -= c0 * (um + u0) + c1 * (vm + v0) + c2 * (wmv + w0);
 }
 
 int launch_scal_adjoint(cans_ctx* c, const ScalAdj& sa) {
@@ -446,7 +478,7 @@ int launch_scal_adjoint(cans_ctx* c, const ScalAdj& sa) {
   if (sa.buoy) {
     k_scal_buoy_t<<<dim3(cdiv(L.n1, 64), L.n2, L.n3), 64, 0, c->stream>>>(L, sa.sh, sa.wb[0], sa.wb[1], sa.wb[2],
                                                                           sa.coef[0], sa.coef[1], sa.coef[2],
-                                                                          sa.zwall);
+                                                                          sa.wm);
     CANS_LAUNCH_CHECK("k_scal_buoy_t");
   }
   return CANS_OK;
@@ -487,11 +519,25 @@ int launch_scal_jt(cans_ctx* c, const ScalAdj& sa) {
   a.alpha = sa.alpha;
   a.f1 = sa.f1;
   a.f2 = sa.f2;
-  a.zwall = sa.zwall;
-  k_scal_jt<<<dim3(cdiv(L.n1, 64), L.n2, L.n3 + (sa.zwall ? 2 : 0)), 64, 0, c->stream>>>(a);
+  a.wm = sa.wm;
+  const int ox = sa.wm & 1, oy = sa.wm >> 1 & 1, oz = sa.wm >> 2 & 1;
+  k_scal_jt<<<dim3(cdiv(L.n1 + 2 * ox, 64), L.n2 + 2 * oy, L.n3 + 2 * oz), 64, 0, c->stream>>>(a);
   CANS_LAUNCH_CHECK("k_scal_jt");
-  if (sa.zwall) {
-    k_scal_fold<<<dim3(cdiv(L.n1, 64), L.n2), 64, 0, c->stream>>>(L, sa.sbo, sa.sign[0], sa.sign[1]);
+  const int n[3] = {L.n1, L.n2, L.n3};
+  for (int d = 2; d >= 0; d--) {  // B^T in the reverse of boundp's x, y, z order
+    if (!(sa.wm >> d & 1)) continue;
+    ScalFold f;
+    f.d = d;
+    for (int e = 0; e < 3; e++) {
+      const bool ghosts = e < d && (sa.wm >> e & 1);
+      f.lo[e] = ghosts ? 0 : 1;
+      f.hi[e] = ghosts ? n[e] + 1 : n[e];
+    }
+    f.s0 = sa.sign[d][0];
+    f.s1 = sa.sign[d][1];
+    const int pa = d == 0 ? 1 : 0, pb = d == 2 ? 1 : 2;
+    k_scal_fold<<<dim3(cdiv(f.hi[pa] - f.lo[pa] + 1, 64), f.hi[pb] - f.lo[pb] + 1), 64, 0, c->stream>>>(L, sa.sbo,
+                                                                                                          f);
     CANS_LAUNCH_CHECK("k_scal_fold");
   }
   return CANS_OK;

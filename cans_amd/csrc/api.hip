@@ -888,19 +888,18 @@ int adjoint_scope(cans_ctx* c, const cans_step_params* s, const char* who, bool 
   if (s->nscal && !scalars)
     return fail(CANS_EINVAL, w + ": a step with scalars runs backwards through cans_rk3_substep_adjoint_scal");
   if (s->nscal) {
-    // scalars: explicit diffusion, x and y periodic; z periodic or walled (Dirichlet or Neumann)
+    // scalars: explicit diffusion; periodic where the box is, Dirichlet or Neumann at its walls
     if (c->imp) return fail(CANS_EUNSUPPORTED, w + ": with scalars the adjoint covers explicit diffusion");
-    if (adjoint_walls(s) & 3) return fail(CANS_EUNSUPPORTED, w + ": with scalars the adjoint covers periodic x and y");
     if (!s->scal || s->nscal > CANS_MAX_SCALARS) return fail(CANS_EINVAL, w + ": bad scalar list");
-    const bool zw = adjoint_walls(s) & 4;
+    const int swm = adjoint_walls(s);
     for (int i = 0; i < s->nscal; i++) {
       const cans_scalar& q = s->scal[i];
       if (q.is_forced && !s->grid_vol_ratio_f_d) return fail(CANS_EINVAL, w + ": null grid_vol_ratio_f");
-      for (int e = 0; e < 4; e++)
-        if (q.cbc[e] != 'P') return fail(CANS_EUNSUPPORTED, w + ": a scalar must be periodic in x and y");
-      for (int e = 4; e < 6; e++)
-        if (zw ? (q.cbc[e] != 'D' && q.cbc[e] != 'N') : q.cbc[e] != 'P')
-          return fail(CANS_EUNSUPPORTED, w + ": a scalar's z boundaries must match the walls");
+      for (int e = 0; e < 6; e++) {
+        const bool wall = swm >> (e / 2) & 1;
+        if (wall ? (q.cbc[e] != 'D' && q.cbc[e] != 'N') : q.cbc[e] != 'P')
+          return fail(CANS_EUNSUPPORTED, w + ": a scalar must be periodic where the box is, D or N at its walls");
+      }
     }
   }
   // implicit diffusion: the 3-D form on tri-periodic uniform grids, where the Helmholtz operator
@@ -1072,7 +1071,7 @@ extern "C" int cans_rk3_substep_adjoint(cans_ctx* c, const cans_step_params* s, 
   return substep_adjoint_vel(c, s, irk, dt, u, v, w, ub, vb, wb, fu, fv, fw, ubo, vbo, wbo, fuo, fvo, fwo, pp, false);
 }
 
-// The adjoint substep with scalars (explicit diffusion; x and y periodic): the velocity part as
+// The adjoint substep with scalars (explicit diffusion): the velocity part as
 // cans_rk3_substep_adjoint, then per scalar i (main.f90:440-450: the scalars advance first, their
 // bulk forcing is added, and the velocity's rk reads the first one's new value in the Boussinesq
 // term, rk.f90:163-177) sh_i = Fs'^T (sb_i (+ that term's transpose of wb for i = 0)), sbo_i = sh_i +
@@ -1106,7 +1105,7 @@ extern "C" int cans_rk3_substep_adjoint_scal(cans_ctx* c, const cans_step_params
   if (rc) return rc;
   const double* rkpar = RKCOEFF[irk - 1];
   const double f1 = rkpar[0] * dt, f2 = rkpar[1] * dt;
-  const bool zwall = adjoint_walls(s) & 4;
+  const int swm = adjoint_walls(s);
   const double zero6[6] = {};
   ScalAdj sa;
   sa.q[0] = (const double*)u;
@@ -1124,7 +1123,7 @@ extern "C" int cans_rk3_substep_adjoint_scal(cans_ctx* c, const cans_step_params
   sa.dyi = s->dli[1];
   sa.f1 = f1;
   sa.f2 = f2;
-  sa.zwall = zwall;
+  sa.wm = swm;
   for (int cc = 0; cc < 3; cc++) sa.coef[cc] = (f1 + f2) * s->gacc[cc] * s->beta * 0.5;
   for (int i = 0; i < s->nscal; i++) {
     const cans_scalar& qs = s->scal[i];
@@ -1134,8 +1133,8 @@ extern "C" int cans_rk3_substep_adjoint_scal(cans_ctx* c, const cans_step_params
     sa.sbo = (double*)sbo[i];
     sa.fsbo = (double*)fsbo[i];
     sa.alpha = qs.alpha;
-    sa.sign[0] = qs.cbc[4] == 'N' ? 1. : -1.;
-    sa.sign[1] = qs.cbc[5] == 'N' ? 1. : -1.;
+    for (int d = 0; d < 3; d++)
+      for (int e = 0; e < 2; e++) sa.sign[d][e] = qs.cbc[2 * d + e] == 'N' ? 1. : -1.;
     sa.buoy = i == 0 && s->is_buoyancy;
     if ((rc = launch_scal_adjoint(c, sa))) return rc;
     if (qs.is_forced) {  // the forcing acts before the buoyancy reads the scalar: its transpose after

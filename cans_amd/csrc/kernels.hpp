@@ -229,9 +229,9 @@ int launch_mom_jt_adj(cans_ctx* c, const double* const* q, const double* const* 
                       const AdjWalls* wall);
 // out += coef L a (tri-periodic uniform grid; h = 1/dx, 1/dy, 1/dz; a with periodic ghosts)
 int launch_adj_lap(cans_ctx* c, const double* const* a, double* const* out, double coef, const double* h);
-// one scalar's part of the adjoint substep (explicit diffusion; x, y periodic): sh is the updated
+// one scalar's part of the adjoint substep (explicit diffusion): sh is the updated
 // scalar's adjoint (in place: + the Boussinesq transpose of wb when buoy), then sbo = sh + J_s^T
-// (f1 sh + fsb) (folded at z walls), fsbo = f2 sh, ubo += J_q^T (f1 sh + fsb)
+// (f1 sh + fsb) (folded at the walls), fsbo = f2 sh, ubo += J_q^T (f1 sh + fsb)
 struct ScalAdj {
   const double* q[3];
   const double* s;
@@ -242,9 +242,9 @@ struct ScalAdj {
   const double *wb[3];
   const double *dzci, *dzfi;
   double dxi, dyi, alpha, f1, f2;
-  double coef[3];  // f12 gacc_c beta / 2
-  double sign[2];  // z walls: -1 Dirichlet, +1 Neumann
-  int zwall, buoy;
+  double coef[3];     // f12 gacc_c beta / 2
+  double sign[3][2];  // per wall direction and end: -1 Dirichlet, +1 Neumann
+  int wm, buoy;       // wm: the wall directions, bit 0 x, 1 y, 2 z
 };
 int launch_scal_adjoint(cans_ctx* c, const ScalAdj& sa);  // the buoyancy transpose into sh
 // the transposed scalar forcing (scal.f90:313-329, rk.f90:474-475): sh -= gvr_f(k) * sum

@@ -434,8 +434,8 @@ int cans_rk3_substep_adjoint(cans_ctx *ctx, const cans_step_params *prm, int irk
                              void *ubo_d, void *vbo_d, void *wbo_d, void *fuo_d, void *fvo_d,
                              void *fwo_d, void *pp_d);
 /* The same with scalars (prm->nscal > 0; cans_rk3_substep_adjoint refuses them): explicit
- * diffusion, x and y periodic, scalars (bulk-forced or not) with z periodic or walled (Dirichlet or
- * Neumann).  s_d[i] is scalar i as the forward substep started from (halo'd, ghosts
+ * diffusion; scalars bulk-forced or not, periodic where the box is and Dirichlet or
+ * Neumann at its walls.  s_d[i] is scalar i as the forward substep started from (halo'd, ghosts
  * set); sb_d[i] and fsb_d[i] hold the adjoints of its output and of the right-hand side
  * it stored (sb_d[i] is overwritten); sbo_d[i] and fsbo_d[i] receive those of its input
  * and of the right-hand side it read.  The first scalar's adjoint also takes the

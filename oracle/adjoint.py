@@ -105,13 +105,13 @@ def check_case(case):
     if case.is_forced[2] or any(case.is_forced[c] and c in wall_dirs(case) for c in (0, 1)):
         raise ValueError("forcing covers u and v along periodic directions")
     if case.scalars:
-        if case.is_impdiff or walls(case) and wall_dirs(case) != [2]:
-            raise ValueError("scalars: explicit diffusion, periodic x and y")
+        if case.is_impdiff:
+            raise ValueError("scalars: explicit diffusion")
         for sc in case.scalars:
-            if sc["cbc"][:4] != "PPPP" or sc["cbc"][4:6] != ("PP" if 2 not in wall_dirs(case) else sc["cbc"][4:6]):
-                raise ValueError("scalars: periodic along the periodic directions")
-            if 2 in wall_dirs(case) and any(ch not in "DN" for ch in sc["cbc"][4:6]):
-                raise ValueError("scalars: Dirichlet or Neumann at the z walls")
+            for d in range(3):
+                cb = sc["cbc"][2 * d:2 * d + 2]
+                if (d in wall_dirs(case)) != (cb != "PP") or any(ch not in "DNP" for ch in cb):
+                    raise ValueError("scalars: periodic where the box is, Dirichlet or Neumann at its walls")
     if case.is_impdiff and (walls(case) or case.is_impdiff_1d):
         raise ValueError("implicit diffusion: the adjoint covers the 3-D form on tri-periodic uniform grids")
 
@@ -322,13 +322,15 @@ def mom_jt_walls(case, qh, g, fold=True, swap_rdz=False):
     return res
 
 
-def _zext(case, a, pad_ghosts):
-    """Along z: a halo'd array's planes 0..n3+1 plus a zero plane each side (walls), or its interior
-    planes (periodic z); x and y interior (periodic)."""
-    if 2 in wall_dirs(case):
-        b = a[1:-1, 1:-1, :] if pad_ghosts else np.pad(a, ((0, 0), (0, 0), (1, 1)))
-        return np.pad(b, ((0, 0), (0, 0), (1, 1)))
-    return a[1:-1, 1:-1, 1:-1] if pad_ghosts else a
+def _ext(case, a, halo):
+    """Along a wall direction: the ghost layers plus one zero layer each side (index i + 1 for
+    i = -1 .. n + 2); along a periodic one the interior only (np.roll wraps).  halo: a is halo'd (its ghosts
+    kept) or interior (its ghost layers zero)."""
+    wd = wall_dirs(case)
+    if halo:
+        sl = tuple(slice(None) if d in wd else slice(1, -1) for d in range(3))
+        return np.pad(a[sl], [(1, 1) if d in wd else (0, 0) for d in range(3)])
+    return np.pad(a, [(2, 2) if d in wd else (0, 0) for d in range(3)])
 
 
 def _z_tables(case):
@@ -341,15 +343,21 @@ def _z_tables(case):
     return zc[None, None, :], zf[None, None, :]
 
 
-def _fold_scalar_z(case, out, cbc):
-    """B^T of a cell-centred scalar's z ghosts (walls): s(0) = 2 sb - s(1) (D, -1) or s(1) - dzc sb (N, +1)."""
-    n3 = case.n[2]
-    if 2 not in wall_dirs(case):
-        return np.asfortranarray(out)
-    res = np.asfortranarray(out[:, :, 2:n3 + 2])
-    res[:, :, 0] += (-1. if cbc[4] == "D" else 1.) * out[:, :, 1]
-    res[:, :, n3 - 1] += (-1. if cbc[5] == "D" else 1.) * out[:, :, n3 + 2]
-    return res
+def _fold_scalar(case, out, cbc):
+    """B^T of a cell-centred scalar's ghosts at the walls (s(0) = 2 sb - s(1): -1 Dirichlet; s(0) = s(1) -
+    dz sb: +1 Neumann), z first, then y, then x (boundp sets them x, y, z); returns the interior."""
+    out = out.copy()
+    n = list(case.n)
+    wd = wall_dirs(case)
+    for d in sorted(wd, reverse=True):
+        lo, hi, f_lo, f_hi = ([slice(None)] * 3 for _ in range(4))
+        lo[d], hi[d], f_lo[d], f_hi[d] = 1, n[d] + 2, 2, n[d] + 1
+        out[tuple(f_lo)] += (-1. if cbc[2 * d] == "D" else 1.) * out[tuple(lo)]
+        out[tuple(f_hi)] += (-1. if cbc[2 * d + 1] == "D" else 1.) * out[tuple(hi)]
+        out[tuple(lo)] = 0.
+        out[tuple(hi)] = 0.
+    isl = tuple(slice(2, n[d] + 2) if d in wd else slice(None) for d in range(3))
+    return np.asfortranarray(out[isl])
 
 
 def scal_jt(case, qh, sh, sigma, alpha, cbc):
@@ -361,9 +369,9 @@ def scal_jt(case, qh, sh, sigma, alpha, cbc):
     qh, sh: the halo'd velocity and scalar (ghosts as the forward set them); sigma: interior array.
     Returns (J_s^T sigma folded by the scalar's ghost map, [J_q^T sigma] on the velocity's degrees of
     freedom)."""
-    Q = [_zext(case, x, True) for x in qh]
-    S = _zext(case, sh, True)
-    G = _zext(case, sigma, False)
+    Q = [_ext(case, x, True) for x in qh]
+    S = _ext(case, sh, True)
+    G = _ext(case, sigma, False)
     zc_, zf_ = _z_tables(case)
     H = [case.dli[0], case.dli[1], zf_]
     out = np.zeros_like(G)
@@ -377,25 +385,24 @@ def scal_jt(case, qh, sh, sigma, alpha, cbc):
     gh = zf_ * G
     z = zc_ * (_sh(gh, 2, 1) - gh) - _sh(zc_, 2, -1) * (gh - _sh(gh, 2, -1))
     out += alpha * (xy + z)
-    n3 = case.n[2]
-    isl = (slice(None), slice(None), slice(2, n3 + 2)) if 2 in wall_dirs(case) else (slice(None),) * 3
+    n, wd = case.n, wall_dirs(case)
+    isl = tuple(slice(2, n[d] + 2) if d in wd else slice(None) for d in range(3))
     qres = [np.asfortranarray(x[isl]) for x in qt]
     _clear_wall_faces(case, qres)
-    return _fold_scalar_z(case, out, cbc), qres
+    return _fold_scalar(case, out, cbc), qres
 
 
 def buoyancy_t(case, wb, cbc, coef):
     """Transpose of the Boussinesq term u_c -= coef_c (s(x + e_c) + s(x)) (rk.f90:163-177, coef_c =
     f12 gacc_c beta / 2) applied to wb, folded by the scalar's ghost map: sum_c -coef_c (wb_c(y - e_c) + wb_c(y)),
     wb zero off the velocity's degrees of freedom."""
-    W = [_zext(case, x, False) for x in wb]
-    for x in W[2:3]:
-        if 2 in wall_dirs(case):
-            x[:, :, case.n[2] + 1] = 0.
+    wz = [x.copy() for x in wb]
+    _clear_wall_faces(case, wz)
+    W = [_ext(case, x, False) for x in wz]
     out = np.zeros_like(W[0])
     for c in range(3):
         out -= coef[c] * (_sh(W[c], c, -1) + W[c])
-    return _fold_scalar_z(case, out, cbc)
+    return _fold_scalar(case, out, cbc)
 
 
 def kinetic_energy(case, u, v, w):

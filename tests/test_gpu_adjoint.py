@@ -524,3 +524,89 @@ def test_scalars_gradient_central_differences(torch):
             rel = abs(fd - ad) / abs(ad)
             print("direction %d: central difference %.12e, adjoint %.12e, rel %.2e" % (r, fd, ad, rel), flush=True)
             assert rel < 1e-5, (r, fd, ad)
+
+
+def heated_cavity_params(ng):
+    """Walls in x and z (y periodic), stretched z: the buoyant scalar Dirichlet on the x walls (hot, cold),
+    Neumann in z; a passive one the other way round."""
+    return dict(ng=ng, l=(1.0, 0.5, 1.0), gtype=1, gr=1.2, cbcvel="DDPPDD" * 3, cbcpre="NNPPNN", visci=60.0,
+                scalars=(dict(alphai=80.0, cbc="DDPPNN", bc=(-0.5, 0.5, 0.0, 0.0, 0.1, 0.2)),
+                         dict(alphai=120.0, cbc="NNPPDD", bc=(0.3, -0.2, 0.0, 0.0, 1.0, 0.0))),
+                gacc=(0.0, 0.0, -1.0), beta=3.0, is_boussinesq_buoyancy=True)
+
+
+def test_substep_adjoint_scalars_walls_matches_oracle(torch):
+    from oracle import adjoint as A
+    from oracle import oracle as O
+    print("\n[case] cans_rk3_substep_adjoint_scal, heated cavity 16x8x20 (walls in x and z)", flush=True)
+    p = heated_cavity_params((16, 8, 20))
+    sim, orc = channel_pair(p)
+    rng = np.random.default_rng(25)
+
+    def dofs():
+        a = [rng.uniform(-1, 1, tuple(orc.n)) for _ in range(3)]
+        A._clear_wall_faces(orc, a)
+        return a
+
+    q, ub, fb = dofs(), dofs(), dofs()
+    qh = [O.field(orc.n) for _ in range(3)]
+    for x, y in zip(qh, q):
+        x[1:-1, 1:-1, 1:-1] = y
+    orc.bounduvw(*qh)
+    ns = len(orc.scalars)
+    sh = []
+    for i in range(ns):
+        f = O.field(orc.n)
+        f[1:-1, 1:-1, 1:-1] = rng.uniform(-1, 1, tuple(orc.n))
+        orc.boundp(f, cbc=orc.scalars[i]["cbc"], bc=orc.scalars[i]["bc"])
+        sh.append(f)
+    sb = [rng.uniform(-1, 1, tuple(orc.n)) for _ in range(ns)]
+    fsb = [rng.uniform(-1, 1, tuple(orc.n)) for _ in range(ns)]
+
+    def put(arrays, halo):
+        out = []
+        for a in arrays:
+            f = sim.ctx.field()
+            f.set(a if halo else np.pad(a, 1))
+            out.append(f)
+        return out
+
+    fq, fub, ffb = put(qh, True), put(ub, False), put(fb, False)
+    fs, fsbf, ffsb = put(sh, True), put(sb, False), put(fsb, False)
+    fubo, ffbo = [sim.ctx.field() for _ in range(3)], [sim.ctx.field() for _ in range(3)]
+    fsbo, ffsbo = [sim.ctx.field() for _ in range(ns)], [sim.ctx.field() for _ in range(ns)]
+    pp = sim.ctx.field()
+    dt, irk = 2e-3, 2
+    sim.rk3_substep_adjoint(irk, dt, [f.ptr for f in fq], fub, ffb, fubo, ffbo, pp,
+                            ([f.ptr for f in fs], fsbf, ffsb, fsbo, ffsbo))
+    sim.ctx.sync()
+    r_ub, r_fb, r_sb, r_fsb = A.substep_adjoint(orc, (qh, sh), irk - 1, dt, ub, fb, sb, fsb)
+    errs = [relerr([f.interior() for f in fubo], r_ub), relerr([f.interior() for f in ffbo], r_fb),
+            relerr([f.interior() for f in fsbo], r_sb), relerr([f.interior() for f in ffsbo], r_fsb)]
+    print("adjoint substep rel err: velocity %.3e, rhs %.3e, scalars %.3e, scalar rhs %.3e" % tuple(errs), flush=True)
+    assert max(errs) < 1e-12
+
+
+def test_gradient_differentially_heated_cavity_matches_oracle(torch):
+    """The reference's differentially heated cavity (tests/differentially_heated_cavity: walls in x and z, hot
+    and cold x walls, buoyancy), at 64x2x64 from its linear temperature profile: the gradient after 4 steps."""
+    from cans_amd.adjoint import kinetic_energy_after_steps
+    from oracle import adjoint as A
+    print("\n[case] gradient, differentially heated cavity 64x2x64, 4 steps", flush=True)
+    p = cases.params("dhc", ng=(64, 2, 64))
+    sim, orc = channel_pair(p)
+    rng = np.random.default_rng(16)
+    u0 = [0.05 * rng.uniform(-1, 1, tuple(p["ng"])) for _ in range(3)]
+    A._clear_wall_faces(orc, u0)
+    s0 = cases.initial_scalars("dhc", p, sim.grid)
+    dt, nsteps = 2e-3, 4
+    E_orc, g_orc = A.grad_ke_after_steps(orc, *u0, dt, nsteps, every=2, scal0=s0)
+    t0 = [torch.tensor(np.ascontiguousarray(a.transpose(2, 1, 0)), device="cuda", requires_grad=True) for a in u0]
+    ts = [torch.tensor(np.ascontiguousarray(a.transpose(2, 1, 0)), device="cuda") for a in s0]
+    E = kinetic_energy_after_steps(sim, *t0, dt, nsteps, every=2, s0=ts)
+    E.backward()
+    g_gpu = [t.grad.cpu().numpy().transpose(2, 1, 0) for t in t0]
+    eE = abs(E.item() - E_orc) / E_orc
+    eg = relerr(g_gpu, g_orc)
+    print("E %.15e (oracle %.15e, rel %.2e); gradient rel err %.3e" % (E.item(), E_orc, eE, eg), flush=True)
+    assert eE < 1e-13 and eg < 1e-10

@@ -336,12 +336,12 @@ def test_scal_jt_dot_product():
         scale = float(np.abs(fp - fm).sum())
         assert abs(lhs - rhs) <= 1e-13 * scale, (i, lhs, rhs)
         # planted mistake: the ghost fold of the walls dropped
-        bad = A._fold_scalar_z
+        bad = A._fold_scalar
         try:
-            A._fold_scalar_z = lambda c, out, cbc: np.asfortranarray(out[:, :, 2:c.n[2] + 2])
+            A._fold_scalar = lambda c, out, cbc: np.asfortranarray(out[:, :, 2:c.n[2] + 2])
             js2, _ = A.scal_jt(case, ghosts(case, q), scalar_ghosts(case, s0, i), sig, sc["alpha"], sc["cbc"])
         finally:
-            A._fold_scalar_z = bad
+            A._fold_scalar = bad
         assert abs(float(np.sum(js2 * ds)) + dot(jq, dq) - lhs) > 1e-6 * scale
 
 
@@ -390,3 +390,47 @@ def test_gradient_scalars_matches_central_differences():
     finally:
         A.scal_jt = sj
     assert abs(dot(bad, d) - fd) > 1e-6 * abs(fd)
+
+
+def heated_cavity(ng=(8, 4, 10), visci=60.0):
+    """A differentially heated cavity like the reference's (walls in x and z, y periodic): the buoyant scalar
+    Dirichlet on the x walls (hot, cold) and Neumann in z; a passive one the other way round; stretched z."""
+    scal = (dict(alphai=80.0, cbc="DDPPNN", bc=(-0.5, 0.5, 0.0, 0.0, 0.1, 0.2), is_forced=False),
+            dict(alphai=120.0, cbc="NNPPDD", bc=(0.3, -0.2, 0.0, 0.0, 1.0, 0.0), is_forced=False))
+    return O.OracleCase(ng=ng, l=(1.0, 0.5, 1.0), gtype=1, gr=1.2, cbcvel="DDPPDD" * 3, cbcpre="NNPPNN",
+                        visci=visci, scalars=scal, gacc=(0.0, 0.0, -1.0), beta=3.0, is_boussinesq_buoyancy=True)
+
+
+def test_scal_jt_walls_xz_dot_product():
+    case = heated_cavity()
+    A.check_case(case)
+    rng = np.random.default_rng(61)
+    for i, sc in enumerate(case.scalars):
+        q, dq = dofs(case, rng), dofs(case, rng)
+        s0, ds = rng.uniform(-1, 1, tuple(case.n)), rng.uniform(-1, 1, tuple(case.n))
+        sig = rng.uniform(-1, 1, tuple(case.n))
+        fp = case.scal(*ghosts(case, [a + b for a, b in zip(q, dq)]), scalar_ghosts(case, s0 + ds, i), sc["alpha"])
+        fm = case.scal(*ghosts(case, [a - b for a, b in zip(q, dq)]), scalar_ghosts(case, s0 - ds, i), sc["alpha"])
+        lhs = float(np.sum(sig * (fp - fm) / 2))
+        js, jq = A.scal_jt(case, ghosts(case, q), scalar_ghosts(case, s0, i), sig, sc["alpha"], sc["cbc"])
+        rhs = float(np.sum(js * ds)) + dot(jq, dq)
+        scale = float(np.abs(fp - fm).sum())
+        assert abs(lhs - rhs) <= 1e-13 * scale, (i, lhs, rhs)
+
+
+@pytest.mark.timeout(300)
+def test_gradient_heated_cavity_matches_central_differences():
+    case = heated_cavity()
+    rng = np.random.default_rng(62)
+    u0 = [0.3 * x for x in dofs(case, rng)]
+    s0 = [rng.uniform(-0.5, 0.5, tuple(case.n)) for _ in case.scalars]
+    dt, nsteps = 2e-3, 4
+    E, grad = A.grad_ke_after_steps(case, *u0, dt, nsteps, every=2, scal0=s0)
+    eps = 1e-6
+    for _ in range(3):
+        d = dofs(case, rng)
+        ep = A.ke_after_steps(case, *(a + eps * b for a, b in zip(u0, d)), dt, nsteps, scal0=s0)
+        em = A.ke_after_steps(case, *(a - eps * b for a, b in zip(u0, d)), dt, nsteps, scal0=s0)
+        fd = (ep - em) / (2 * eps)
+        ad = dot(grad, d)
+        assert fd_close(fd, ad, grad, d), (fd, ad)
