@@ -26,7 +26,11 @@ differences.
   both bulk-forced:
   one substep against the oracle at 1e-12 (velocity, scalars and both right-hand sides),
   the gradient after 4 steps at 1e-10, and at 128x64x96 x 8 steps central differences
-  along 3 directions to 1e-5.
+  along 3 directions to 1e-5; a heated cavity with walls in x and z (substep), and the
+  reference's differentially heated cavity at 64x2x64 (gradient);
+* refusals: a step with scalars through the velocity-only entry point, implicit
+  diffusion next to walls and in its z-only form, each an error that leaves the outputs
+  untouched.
   and a 256x128x96 channel (Re_b = 5600, 16 steps, checkpoints every 4) whose
   directional derivatives along 4 seeded directions agree with central differences to
   1e-5.
@@ -610,3 +614,31 @@ def test_gradient_differentially_heated_cavity_matches_oracle(torch):
     eg = relerr(g_gpu, g_orc)
     print("E %.15e (oracle %.15e, rel %.2e); gradient rel err %.3e" % (E.item(), E_orc, eE, eg), flush=True)
     assert eE < 1e-13 and eg < 1e-10
+
+
+def test_adjoint_refusals(torch):
+    """Out-of-scope substeps are refused with an error and leave the fields untouched: a step with scalars
+    through the velocity-only entry point, implicit diffusion next to walls, z-only implicit diffusion."""
+    from cans_amd import _lib as LB
+    print("\n[case] adjoint refusals", flush=True)
+
+    def attempt(p):
+        sim, _ = channel_pair(p)
+        f = [sim.ctx.field() for _ in range(12)]
+        for x in f:
+            x.buf.fill_(0.25)
+        pp = sim.ctx.field(kind=LB.CANS_FIELD_PRESSURE)
+        q = [sim.u.ptr, sim.v.ptr, sim.w.ptr]
+        with pytest.raises(LB.CansError) as e:
+            sim.rk3_substep_adjoint(2, 1e-3, q, f[0:3], f[3:6], f[6:9], f[9:12], pp)
+        assert all(float(x.buf.min()) == 0.25 == float(x.buf.max()) for x in f[6:12])
+        return str(e.value)
+
+    msg = attempt(scalar_params((16, 12, 20)))
+    assert "cans_rk3_substep_adjoint_scal" in msg
+    p = channel_params((16, 12, 20))
+    p["is_impdiff"] = True
+    assert "tri-periodic" in attempt(p)
+    p = channel_params((16, 12, 20))
+    p.update(is_impdiff=True, is_impdiff_1d=True)
+    assert "z-only" in attempt(p)
